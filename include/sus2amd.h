@@ -1,0 +1,110 @@
+/*
+ * sus2amd.h -- C ABI of the MI355X-native SUS2-SH energy/force/virial (EFS)
+ * and loss -> parameter-gradient path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns SUS2_OK (0) or a
+ * status code; sus2_last_error() gives the message of the calling thread's
+ * last failure.  "dev" pointers are HIP device memory, "host" pointers are
+ * host memory; `stream` is a hipStream_t (NULL = the default stream).
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * superlinear repository):
+ *   model handle    MLMTPR(path) / Load / Save / CoeffCount / Coeff
+ *                   dev_src/mtpr.h:52, dev_src/mtpr.cpp:4490, :6893;
+ *                   interfaces/lammps/ML-SUS2/sus2_sl_model.cpp:174 (Model::load)
+ *   EFS             MLMTPR::CalcEFS(Configuration&)  dev_src/mtpr.cpp:8779
+ *                   (per-frame energy, forces, PlusStress virial, site energies)
+ *   EFS gradient    MLMTPR::AccumulateEFSCombinationGrad  dev_src/mtpr.cpp:9188,
+ *                   src/basic_mlip.cpp:421 (ene_weight, frc_weights, str_weights)
+ *   loss + gradient NonLinearRegression::CalcObjectiveFunctionGrad
+ *                   dev_src/non_linear_regression.cpp:1112 (AddLossGrad :892)
+ *   pair compute()  interfaces/lammps/ML-SUS2/sus2_sl_kernel.h:12-43
+ *                   (Kernel::begin / forward comm / outer / reverse comm / finish)
+ */
+#ifndef SUS2AMD_H
+#define SUS2AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SUS2_ABI_VERSION 1
+
+enum sus2_status {
+  SUS2_OK = 0,
+  SUS2_E_INVALID = 1,     /* malformed model or arguments */
+  SUS2_E_IO = 2,          /* file cannot be opened / written */
+  SUS2_E_UNSUPPORTED = 3, /* a model family or shape this build does not run */
+  SUS2_E_DEVICE = 4,      /* HIP runtime failure */
+  SUS2_E_RANGE = 5        /* species / index / size out of range */
+};
+
+enum sus2_precision {
+  SUS2_FP64 = 0, /* all arithmetic in FP64 */
+  SUS2_FP32 = 1  /* FP32 bulk arithmetic, FP64 per-frame energy/virial reductions */
+};
+
+typedef struct sus2_model sus2_model;
+
+int sus2_abi_version(void);
+const char* sus2_last_error(void);
+
+/* --- model handle (MLMTPR ctor / Load, Save, CoeffCount, Coeff) ------------ */
+int sus2_model_load(const char* path, sus2_model** out);
+int sus2_model_save(const sus2_model* m, const char* path);
+void sus2_model_free(sus2_model* m);
+/* dims[16]: C L K B F M Q R O Qg Kw nbasic nproducts gated species_readout H */
+int sus2_model_dims(const sus2_model* m, int* dims);
+double sus2_model_cutoff(const sus2_model* m);
+int sus2_coeff_count(const sus2_model* m);
+/* theta in the reference optimiser layout:
+ * shift[C] | scal[2C^2Kw] | radial[F(B+C)] | gate radial[FB] | gate scal |
+ * alpha[CF] | gate type[CF] | gate W[F Qg] | gate rank W | outer rank W |
+ * species[C] | readout[Q or CQ]   (dev_src/mtpr.cpp:925-975,1586-1740,2108-2210) */
+int sus2_coeff_get(const sus2_model* m, double* theta /* host [ncoeff] */);
+int sus2_coeff_set(sus2_model* m, const double* theta /* host [ncoeff] */);
+/* offsets[13]: shift scal rad grad gscal alpha gtype W gw ow species read ncoeff (-1 = absent) */
+int sus2_coeff_offsets(const sus2_model* m, int* offsets);
+
+/* --- frame-batch EFS (CalcEFS over a batch; dev pointers) -------------------
+ * frame_off [nframes+1] atom offsets (int64), types [natoms] model species,
+ * pos [natoms*3] Cartesian, cell [nframes*9] lattice rows, periodic [nframes]
+ * (0 = molecule, no images).  Outputs: energy [nframes], force [natoms*3],
+ * virial [nframes*6] PlusStress xx yy zz xy xz yz (= -sum d (x) dE/dd),
+ * site [natoms] site energies (may be NULL). */
+int sus2_efs(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+             const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+             double* energy, double* force, double* virial, double* site, void* stream);
+
+/* Same with host buffers (copies in and out; synchronous). */
+int sus2_efs_host(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                  const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+                  double* energy, double* force, double* virial, double* site);
+
+/* --- parameter gradient (AccumulateEFSCombinationGrad over a batch) ---------
+ * grad[ncoeff] += sum_i ene_weight_i dE_i/dtheta
+ *               + sum_e w_e . d^2 E / dtheta dd_e,
+ * w_e = frc_weight_i - frc_weight_j + str_weight(frame) d_e  for e = (i -> j).
+ * ene_weight [natoms], frc_weight [natoms*3], str_weight [nframes*9] row-major,
+ * grad [ncoeff] (dev, FP64, accumulated). */
+int sus2_efs_grad(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                  const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+                  const double* ene_weight, const double* frc_weight, const double* str_weight, double* grad,
+                  void* stream);
+
+/* --- training objective + gradient (CalcObjectiveFunctionGrad) -------------
+ * Labels: e0 [nframes], f0 [natoms*3], s0 [nframes*6] (xx yy zz xy xz yz).
+ * weighting: 0 vibrations, 1 structures, 2 molecules; weights wE wF wS.
+ * loss_out (host double) = (1/nframes) sum_f L_f; grad [ncoeff] (dev FP64)
+ * is overwritten with the gradient of loss_out. */
+int sus2_loss_grad(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                   const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+                   const double* e0, const double* f0, const double* s0, int weighting, double wE, double wF,
+                   double wS, double* loss_out, double* grad, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SUS2AMD_H */

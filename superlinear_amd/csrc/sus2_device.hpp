@@ -1,0 +1,303 @@
+// Device-side model view and per-edge math shared by the SUS2-SH kernels:
+// real spherical harmonics (values, or values + Cartesian gradient streamed in
+// (m, l) order), the Chebyshev-tanh radial basis with the (r - rc)^2 envelope,
+// and the one-wavefront-per-workgroup helpers (wave reductions, layered
+// product-graph walks over LDS).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace sus2 {
+
+constexpr int WAVE = 64;
+constexpr int BMAX = 16;  // largest radial basis size the kernels are built for
+
+// ------------------------------------------------------------------ model view
+template <class T>
+struct DPass {  // layered product-graph schedule of one pass (see PassPlan)
+  int nfl = 0, nrl = 0, rank = 0, begin = 0;
+  const int *fl_off = nullptr, *f_out = nullptr, *f_roff = nullptr, *f_a = nullptr, *f_b = nullptr;
+  const T* f_c = nullptr;
+  const int *rl_off = nullptr, *r_node = nullptr, *r_roff = nullptr, *r_out = nullptr, *r_partner = nullptr;
+  const T* r_c = nullptr;
+};
+
+template <class T>
+struct DModel {
+  int C, L, H, K, F, B, M, Q, Kw, smode, R, O, Qg;
+  T rc, rca, scaling;
+  int gated, species_readout;
+  const T* theta;
+  int o_shift, o_scal, o_rad, o_grad, o_gscal, o_alpha, o_gtype, o_W, o_gw, o_ow, o_species, o_read;
+  const int* basic_node;  // [H*K], canonical moment (h, k) -> node or -1
+  const int* scalar_map;  // [Q]
+  const int* gate_nodes;  // [Qg]
+  DPass<T> gp, op;
+
+  __device__ __forceinline__ int block(int k, int l) const {
+    return smode == 2 ? k : smode == 1 ? l : smode == 0 ? 0 : k * (L + 1) + l;
+  }
+  __device__ __forceinline__ T tau(int t, int mu) const { return theta[o_rad + mu * (B + C) + B + t]; }
+  __device__ __forceinline__ T alpha(int t, int mu) const { return theta[o_alpha + t * F + mu]; }
+  __device__ __forceinline__ T gtype(int t, int mu) const { return theta[o_gtype + t * F + mu]; }
+  __device__ __forceinline__ T W(int mu, int q) const { return theta[o_W + mu * Qg + q]; }
+  __device__ __forceinline__ T readout(int t, int q) const {
+    return species_readout ? theta[o_read + t * Q + q] : theta[o_species + t] * theta[o_read + q];
+  }
+  __device__ __forceinline__ T site0(int t) const { return theta[o_species + t] + theta[o_shift + t]; }
+  __device__ __forceinline__ T rankw(bool gate, int l, int a, int k) const {
+    return gate ? theta[o_gw + (l * R + a) * K + k] : theta[o_ow + (l * O + a) * K + k];
+  }
+};
+
+// ------------------------------------------------------------------ constants
+constexpr double csqrt_iter(double x, double g, int n) { return n == 0 ? g : csqrt_iter(x, 0.5 * (g + x / g), n - 1); }
+constexpr double csqrt(double x) { return x <= 0 ? 0.0 : csqrt_iter(x, x > 1 ? x : 1.0, 80); }
+constexpr double cfact_ratio(int l, int m) {  // (l-m)! / (l+m)!
+  double r = 1;
+  for (int a = l - m + 1; a <= l + m; ++a) r /= a;
+  return r;
+}
+constexpr double kPi = 3.14159265358979323846;
+// normalisation of the real harmonic (sqrt 2 folded in for m > 0)
+constexpr double ynorm(int l, int m) {
+  return csqrt((2 * l + 1) * cfact_ratio(l, m) / (4 * kPi)) * (m > 0 ? 1.4142135623730950488 : 1.0);
+}
+constexpr double dfact(int m) {  // (2m-1)!!
+  double r = 1;
+  for (int a = 1; a <= 2 * m - 1; a += 2) r *= a;
+  return r;
+}
+__host__ __device__ constexpr int hidx(int l, int m) { return l * l + l + m; }
+
+// ------------------------------------------------------------------ harmonics
+// Real Y_lm(u) for a unit vector u, l <= LL, flat index l^2+l+m; Condon-
+// Shortley phase cancelled, Y_1{-1,0,1} ~ (y, z, x).  Associated Legendre
+// part by the three-term recurrence in l; azimuthal part (ux + i uy)^m.
+template <class T, int LL>
+__device__ __forceinline__ void sh_values(T ux, T uy, T uz, T* Y) {
+  T cm = T(1), sm = T(0);
+#pragma unroll
+  for (int m = 0; m <= LL; ++m) {
+    if (m > 0) {
+      const T c2 = ux * cm - uy * sm;
+      sm = ux * sm + uy * cm;
+      cm = c2;
+    }
+    T p2 = T(0), p1 = T(0);
+#pragma unroll
+    for (int l = m; l <= LL; ++l) {
+      T p;
+      if (l == m) p = T(dfact(m));
+      else p = (T(2 * l - 1) * uz * p1 - T(l + m - 1) * p2) * T(1.0 / (l - m));
+      const T n = T(ynorm(l, m));
+      Y[hidx(l, m)] = n * p * cm;
+      if (m > 0) Y[hidx(l, -m)] = n * p * sm;
+      p2 = p1;
+      p1 = p;
+    }
+  }
+}
+
+// Y_lm(d/|d|) and grad_d Y_lm streamed to fn(l, m, Y, gx, gy, gz) in (m, l)
+// order.  The Legendre factor is carried as a polynomial in (u_z, rho = |u|^2)
+// with both partial derivatives so that the solid harmonic S = r^l Y and its
+// gradient are exact; grad_d Y = (grad_u S - l S u) / r.
+template <class T, int LL, class Fn>
+__device__ __forceinline__ void sh_stream_grad(T ux, T uy, T uz, T inv_r, Fn&& fn) {
+  T cm = T(1), sm = T(0);
+  T cx = T(0), cy = T(0), sx = T(0), sy = T(0);  // d(cm)/dux, d(cm)/duy, d(sm)/dux, d(sm)/duy
+#pragma unroll
+  for (int m = 0; m <= LL; ++m) {
+    if (m > 0) {
+      cx = T(m) * cm; cy = -T(m) * sm;
+      sx = T(m) * sm; sy = T(m) * cm;
+      const T c2 = ux * cm - uy * sm;
+      sm = ux * sm + uy * cm;
+      cm = c2;
+    }
+    T p2 = T(0), p1 = T(0), z2 = T(0), z1 = T(0), r2 = T(0), r1 = T(0);
+#pragma unroll
+    for (int l = m; l <= LL; ++l) {
+      T p, pz, pr;
+      if (l == m) { p = T(dfact(m)); pz = T(0); pr = T(0); }
+      else {
+        const T A = T(2 * l - 1), Bc = T(l + m - 1), inv = T(1.0 / (l - m));
+        p = (A * uz * p1 - Bc * p2) * inv;
+        pz = (A * (p1 + uz * z1) - Bc * z2) * inv;
+        pr = (A * uz * r1 - Bc * (p2 + r2)) * inv;
+      }
+      const T n = T(ynorm(l, m));
+      // grad of the Legendre polynomial in (x, y, z) at u
+      const T gpx = T(2) * ux * pr, gpy = T(2) * uy * pr, gpz = pz + T(2) * uz * pr;
+      {
+        const T S = n * p * cm;
+        const T gx = n * (gpx * cm + p * cx), gy = n * (gpy * cm + p * cy), gz = n * gpz * cm;
+        const T lS = T(l) * S;
+        fn(l, m, S, (gx - lS * ux) * inv_r, (gy - lS * uy) * inv_r, (gz - lS * uz) * inv_r);
+      }
+      if (m > 0) {
+        const T S = n * p * sm;
+        const T gx = n * (gpx * sm + p * sx), gy = n * (gpy * sm + p * sy), gz = n * gpz * sm;
+        const T lS = T(l) * S;
+        fn(l, -m, S, (gx - lS * ux) * inv_r, (gy - lS * uy) * inv_r, (gz - lS * uz) * inv_r);
+      }
+      p2 = p1; p1 = p; z2 = z1; z1 = pz; r2 = r1; r1 = pr;
+    }
+  }
+}
+
+__device__ __forceinline__ float dtanh(float x) { return tanhf(x); }
+__device__ __forceinline__ double dtanh(double x) { return tanh(x); }
+__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+
+// Radial functions R_mu(r) (and dR/dr when DER) of one directed edge with
+// ordered species pair `pair` = t_i*C + t_j, for the gate or outer
+// coefficients.  phi_b = scaling T_b(tanh(s (r - t)/2)) (r - rc_l)^2, rc_0 =
+// rc, rc_{l>0} = rca; channels with l > 0 vanish for r >= rca.
+template <class T, int LL, int KK, bool DER>
+__device__ __forceinline__ void radial(const DModel<T>& m, bool gate, int pair, T r, T* R, T* dR) {
+  constexpr int L1 = LL + 1;
+  const bool own_rad = gate && m.o_grad >= 0;
+  const T* coef = m.theta + (own_rad ? m.o_grad : m.o_rad);
+  const int stride = own_rad ? m.B : m.B + m.C;
+  const T* sc = m.theta + ((gate && m.o_gscal >= 0) ? m.o_gscal : m.o_scal);
+  const int CC = m.C * m.C;
+  T Tb[BMAX], Db[BMAX];
+  int prev = -1;
+#pragma unroll
+  for (int k = 0; k < KK; ++k) {
+#pragma unroll
+    for (int l = 0; l < L1; ++l) {
+      const int mu = k * L1 + l;
+      if (l > 0 && r >= m.rca) {
+        R[mu] = T(0);
+        if (DER) dR[mu] = T(0);
+        continue;
+      }
+      const int blk = m.block(k, l);
+      if (blk != prev) {
+        prev = blk;
+        const T slope = sc[2 * blk * CC + pair], shift = sc[(2 * blk + 1) * CC + pair];
+        const T xi = dtanh(slope * (r - shift) * T(0.5));
+        const T dxi = (T(1) - xi * xi) * slope * T(0.5);
+        Tb[0] = m.scaling;
+        Tb[1] = m.scaling * xi;
+        Db[0] = T(0);
+        Db[1] = m.scaling * dxi;
+#pragma unroll
+        for (int b = 2; b < BMAX; ++b) {
+          if (b < m.B) {
+            Tb[b] = T(2) * xi * Tb[b - 1] - Tb[b - 2];
+            if (DER) Db[b] = T(2) * (dxi * Tb[b - 1] + xi * Db[b - 1]) - Db[b - 2];
+          }
+        }
+      }
+      const T dr = r - (l == 0 ? m.rc : m.rca);
+      const T f = dr * dr;
+      const T* c = coef + mu * stride;
+      T s = T(0), sd = T(0);
+#pragma unroll
+      for (int b = 0; b < BMAX; ++b) {
+        if (b < m.B) {
+          s += c[b] * Tb[b];
+          if (DER) sd += c[b] * Db[b];
+        }
+      }
+      R[mu] = f * s;
+      if (DER) dR[mu] = T(2) * dr * s + f * sd;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ wave utils
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x; }
+
+// Forward walk of one pass's product layers: out = sum_rows c v[a] v[b].
+template <class T>
+__device__ void graph_forward(const DPass<T>& p, T* v) {
+  const int lane = lane_id();
+  for (int l = 0; l < p.nfl; ++l) {
+    for (int o = p.fl_off[l] + lane; o < p.fl_off[l + 1]; o += WAVE) {
+      T s = T(0);
+      for (int r = p.f_roff[o]; r < p.f_roff[o + 1]; ++r) s += p.f_c[r] * v[p.f_a[r]] * v[p.f_b[r]];
+      v[p.f_out[o]] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// Reverse walk: a[n] += sum over consumer rows c a[out] v[partner].
+template <class T>
+__device__ void graph_reverse(const DPass<T>& p, const T* v, T* a) {
+  const int lane = lane_id();
+  for (int l = 0; l < p.nrl; ++l) {
+    for (int i = p.rl_off[l] + lane; i < p.rl_off[l + 1]; i += WAVE) {
+      const int n = p.r_node[i];
+      T s = a[n];
+      for (int r = p.r_roff[i]; r < p.r_roff[i + 1]; ++r) s += p.r_c[r] * a[p.r_out[r]] * v[p.r_partner[r]];
+      a[n] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// rank node of channel a at (l, m) for a pass
+__device__ __forceinline__ int rank_node(int begin, int rank, int H, int l, int m, int a) {
+  return begin + rank * (H - (l + 1) * (l + 1)) + (rank - 1 - a) * (2 * l + 1) + m + l;
+}
+
+// Contraction of canonical moments (in LDS `mom`, h-major k-minor) into the
+// rank nodes of a pass; basic nodes also receive their moments.
+template <class T>
+__device__ void load_and_contract(const DModel<T>& m, bool gate, const T* mom, T* v) {
+  const int lane = lane_id();
+  const int KH = m.K * m.H;
+  for (int n = lane; n < m.M; n += WAVE) v[n] = T(0);
+  __syncthreads();
+  for (int c = lane; c < KH; c += WAVE) {
+    const int node = m.basic_node[c];
+    if (node >= 0) v[node] = mom[c];
+  }
+  const int rank = gate ? m.R : m.O, begin = gate ? m.gp.begin : m.op.begin;
+  const int nr = rank * m.H;
+  for (int idx = lane; idx < nr; idx += WAVE) {
+    const int a = idx / m.H, h = idx % m.H;
+    int l = 0;
+    while ((l + 1) * (l + 1) <= h) ++l;
+    const int mm = h - l * l - l;
+    T s = T(0);
+    for (int k = 0; k < m.K; ++k)
+      if (m.basic_node[h * m.K + k] >= 0) s += m.rankw(gate, l, a, k) * mom[h * m.K + k];
+    v[rank_node(begin, rank, m.H, l, mm, a)] = s;
+  }
+  __syncthreads();
+}
+
+// Transposed contraction: canonical basic adjoints from node adjoints.
+template <class T>
+__device__ void adjoint_to_basic(const DModel<T>& m, bool gate, const T* a, T* out /* global or LDS [H*K] */) {
+  const int lane = lane_id();
+  const int KH = m.K * m.H;
+  const int rank = gate ? m.R : m.O, begin = gate ? m.gp.begin : m.op.begin;
+  for (int c = lane; c < KH; c += WAVE) {
+    const int h = c / m.K, k = c % m.K;
+    int l = 0;
+    while ((l + 1) * (l + 1) <= h) ++l;
+    const int mm = h - l * l - l;
+    const int node = m.basic_node[c];
+    T s = node >= 0 ? a[node] : T(0);
+    if (node >= 0)
+      for (int aa = 0; aa < rank; ++aa) s += m.rankw(gate, l, aa, k) * a[rank_node(begin, rank, m.H, l, mm, aa)];
+    out[c] = s;
+  }
+}
+
+}  // namespace sus2

@@ -1,0 +1,173 @@
+// Model upload, workspace management and the neighbour-count scan.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+
+#include "../../include/sus2amd.h"
+#include "sus2_runtime.hpp"
+
+namespace sus2 {
+
+#define RT_CHECK(x)                                                                                \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess) throw Error(SUS2_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+DevBuf::~DevBuf() {
+  if (p) (void)hipFree(p);
+}
+
+void DevBuf::ensure(size_t n) {
+  if (n <= bytes) return;
+  if (p) { RT_CHECK(hipFree(p)); p = nullptr; bytes = 0; }
+  const size_t want = std::max<size_t>(n + n / 4, 256);
+  RT_CHECK(hipMalloc(&p, want));
+  bytes = want;
+}
+
+Workspace::~Workspace() {
+  if (host_total) (void)hipHostFree(host_total);
+}
+
+void Workspace::ensure_atoms(int n) {
+  nbr_cnt.ensure(sizeof(int) * (size_t)std::max(n, 1));
+  nbr_off.ensure(sizeof(int) * ((size_t)n + 1));
+  if (!host_total) RT_CHECK(hipHostMalloc((void**)&host_total, sizeof(int64_t), hipHostMallocDefault));
+}
+
+// Single-workgroup exclusive scan: 1024 threads each own a contiguous chunk,
+// a workgroup scan of the chunk sums, then each chunk is rewritten.  Totals
+// above 2^31 are reported through the int64 total.
+__global__ __launch_bounds__(1024) void k_scan(int n, const int* __restrict__ cnt, int* __restrict__ off,
+                                               int64_t* __restrict__ total) {
+  __shared__ long part[1024];
+  const int t = threadIdx.x;
+  const int chunk = (n + 1023) / 1024;
+  const int b = t * chunk, e = min(n, b + chunk);
+  long s = 0;
+  for (int i = b; i < e; ++i) s += cnt[i];
+  part[t] = s;
+  __syncthreads();
+  for (int w = 1; w < 1024; w <<= 1) {
+    const long x = t >= w ? part[t - w] : 0;
+    __syncthreads();
+    part[t] += x;
+    __syncthreads();
+  }
+  long run = t ? part[t - 1] : 0;
+  for (int i = b; i < e; ++i) {
+    off[i] = (int)run;
+    run += cnt[i];
+  }
+  if (t == 1023) {
+    off[n] = (int)part[1023];
+    *total = part[1023];
+  }
+}
+
+long exclusive_scan(Workspace& ws, int n, hipStream_t st) {
+  ws.scan_tmp.ensure(sizeof(int64_t));
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, n, ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(),
+                     ws.scan_tmp.get<int64_t>());
+  RT_CHECK(hipGetLastError());
+  RT_CHECK(hipMemcpyAsync(ws.host_total, ws.scan_tmp.get<int64_t>(), sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  RT_CHECK(hipStreamSynchronize(st));
+  return (long)*ws.host_total;
+}
+
+namespace {
+struct IntPack {
+  std::vector<int> data;
+  size_t add(const std::vector<int>& v) {
+    const size_t o = data.size();
+    data.insert(data.end(), v.begin(), v.end());
+    if (v.empty()) data.push_back(0);
+    return o;
+  }
+};
+
+template <class T>
+void fill_view(DevModelT<T>& dv, const Model& m, const int* ibase, const std::vector<size_t>& io) {
+  // coefficient arrays in T
+  std::vector<T> th(m.theta.begin(), m.theta.end());
+  dv.theta.ensure(sizeof(T) * th.size());
+  RT_CHECK(hipMemcpy(dv.theta.p, th.data(), sizeof(T) * th.size(), hipMemcpyHostToDevice));
+  std::vector<T> fc, rc;
+  const size_t gfc = 0;
+  fc.insert(fc.end(), m.gate_plan.fw_c.begin(), m.gate_plan.fw_c.end());
+  const size_t ofc = fc.size();
+  fc.insert(fc.end(), m.outer_plan.fw_c.begin(), m.outer_plan.fw_c.end());
+  const size_t grc = 0;
+  rc.insert(rc.end(), m.gate_plan.rv_c.begin(), m.gate_plan.rv_c.end());
+  const size_t orc = rc.size();
+  rc.insert(rc.end(), m.outer_plan.rv_c.begin(), m.outer_plan.rv_c.end());
+  fc.push_back(T(0));
+  rc.push_back(T(0));
+  dv.fc.ensure(sizeof(T) * fc.size());
+  dv.rc.ensure(sizeof(T) * rc.size());
+  RT_CHECK(hipMemcpy(dv.fc.p, fc.data(), sizeof(T) * fc.size(), hipMemcpyHostToDevice));
+  RT_CHECK(hipMemcpy(dv.rc.p, rc.data(), sizeof(T) * rc.size(), hipMemcpyHostToDevice));
+  DModel<T>& v = dv.view;
+  v.C = m.C; v.L = m.L; v.H = m.H; v.K = m.K; v.F = m.F; v.B = m.B; v.M = m.M; v.Q = m.Q; v.Kw = m.Kw;
+  v.smode = m.smode; v.R = m.R; v.O = m.O; v.Qg = m.Qg;
+  v.rc = T(m.rc); v.rca = T(m.rca); v.scaling = T(m.scaling);
+  v.gated = m.gated; v.species_readout = m.species_readout;
+  v.theta = dv.theta.template get<T>();
+  v.o_shift = m.o_shift; v.o_scal = m.o_scal; v.o_rad = m.o_rad; v.o_grad = m.o_grad; v.o_gscal = m.o_gscal;
+  v.o_alpha = m.o_alpha; v.o_gtype = m.o_gtype; v.o_W = m.o_W; v.o_gw = m.o_gw; v.o_ow = m.o_ow;
+  v.o_species = m.o_species; v.o_read = m.o_read;
+  size_t k = 0;
+  v.basic_node = ibase + io[k++];
+  v.scalar_map = ibase + io[k++];
+  v.gate_nodes = ibase + io[k++];
+  for (int pass = 0; pass < 2; ++pass) {
+    const PassPlan& pp = pass == 0 ? m.gate_plan : m.outer_plan;
+    DPass<T>& dp = pass == 0 ? v.gp : v.op;
+    dp.nfl = (int)pp.fw_layer_off.size() - 1;
+    dp.nrl = (int)pp.rv_layer_off.size() - 1;
+    if (dp.nfl < 0) dp.nfl = 0;
+    if (dp.nrl < 0) dp.nrl = 0;
+    dp.rank = pp.rank;
+    dp.begin = pp.begin;
+    dp.fl_off = ibase + io[k++]; dp.f_out = ibase + io[k++]; dp.f_roff = ibase + io[k++];
+    dp.f_a = ibase + io[k++]; dp.f_b = ibase + io[k++];
+    dp.rl_off = ibase + io[k++]; dp.r_node = ibase + io[k++]; dp.r_roff = ibase + io[k++];
+    dp.r_out = ibase + io[k++]; dp.r_partner = ibase + io[k++];
+    dp.f_c = dv.fc.template get<T>() + (pass == 0 ? gfc : ofc);
+    dp.r_c = dv.rc.template get<T>() + (pass == 0 ? grc : orc);
+  }
+}
+}  // namespace
+
+void DevModelSet::upload(const Model& m) {
+  host = &m;
+  for (int n : m.gate_nodes)
+    if (std::count(m.gate_nodes.begin(), m.gate_nodes.end(), n) != 1)
+      throw Error(SUS2_E_UNSUPPORTED, "SUS2 model: repeated gate scalar node");
+  IntPack pk;
+  std::vector<size_t> io;
+  io.push_back(pk.add(m.basic_node));
+  io.push_back(pk.add(m.scalar_map));
+  io.push_back(pk.add(m.gate_nodes));
+  for (int pass = 0; pass < 2; ++pass) {
+    const PassPlan& pp = pass == 0 ? m.gate_plan : m.outer_plan;
+    for (const auto* vec : {&pp.fw_layer_off, &pp.fw_out, &pp.fw_row_off, &pp.fw_a, &pp.fw_b, &pp.rv_layer_off,
+                            &pp.rv_node, &pp.rv_row_off, &pp.rv_out, &pp.rv_partner})
+      io.push_back(pk.add(*vec));
+  }
+  ints.ensure(sizeof(int) * pk.data.size());
+  RT_CHECK(hipMemcpy(ints.p, pk.data.data(), sizeof(int) * pk.data.size(), hipMemcpyHostToDevice));
+  fill_view(d, m, ints.get<int>(), io);
+  fill_view(f, m, ints.get<int>(), io);
+}
+
+void DevModelSet::upload_theta(const Model& m) {
+  std::vector<double> td(m.theta.begin(), m.theta.end());
+  std::vector<float> tf(m.theta.begin(), m.theta.end());
+  RT_CHECK(hipMemcpy(d.theta.p, td.data(), sizeof(double) * td.size(), hipMemcpyHostToDevice));
+  RT_CHECK(hipMemcpy(f.theta.p, tf.data(), sizeof(float) * tf.size(), hipMemcpyHostToDevice));
+}
+
+}  // namespace sus2

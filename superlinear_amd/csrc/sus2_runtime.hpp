@@ -1,0 +1,80 @@
+// Device-resident model copies, per-handle workspace and batch descriptors.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "sus2_device.hpp"
+#include "sus2_model.hpp"
+
+// (L, K) shapes with compiled kernel instances
+#define SUS2_FOR_EACH_SHAPE(X) X(1, 2) X(2, 2) X(3, 4) X(4, 6) X(5, 6)
+
+namespace sus2 {
+
+// Growable device allocation (never shrinks; freed with its owner).
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  void ensure(size_t n);
+  template <class U> U* get() const { return static_cast<U*>(p); }
+};
+
+struct Batch {
+  int nframes = 0, natoms = 0;
+  const int64_t* frame_off = nullptr;
+  const int32_t* types = nullptr;
+  const double* pos = nullptr;
+  const double* cell = nullptr;
+  const int32_t* periodic = nullptr;
+};
+
+struct EfsOut {
+  double *energy = nullptr, *force = nullptr, *virial = nullptr, *site = nullptr;
+};
+
+struct Workspace {
+  DevBuf nbr_cnt, nbr_off, scan_tmp, rec, h, gbasic, abar, gbar, site, fw;
+  // parameter-gradient scratch
+  DevBuf gtan, hdot, ebar, wedge, tgt, gpart, gacc;
+  long nedges = 0;
+  int64_t* host_total = nullptr;  // pinned
+  ~Workspace();
+  void ensure_atoms(int n);
+};
+
+// exclusive scan of ws.nbr_cnt[0..n) into ws.nbr_off[0..n]; returns the total
+long exclusive_scan(Workspace& ws, int n, hipStream_t st);
+
+// One precision's device view of a model.
+template <class T>
+struct DevModelT {
+  DevBuf theta, fc, rc;  // coefficient vector, forward/reverse product coefficients (gate then outer)
+  DModel<T> view{};
+};
+
+struct DevModelSet {
+  const Model* host = nullptr;
+  DevBuf ints;  // all integer schedule arrays
+  DevModelT<double> d;
+  DevModelT<float> f;
+  template <class T> const DModel<T>& get() const;
+  void upload(const Model& m);
+  void upload_theta(const Model& m);
+};
+template <> inline const DModel<double>& DevModelSet::get<double>() const { return d.view; }
+template <> inline const DModel<float>& DevModelSet::get<float>() const { return f.view; }
+
+void efs_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const EfsOut& out,
+                  hipStream_t st);
+void grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* ene_w,
+                   const double* frc_w, const double* str_w, double* grad, hipStream_t st);
+
+}  // namespace sus2

@@ -1,0 +1,93 @@
+"""GPU parity of the HIP EFS path against the FP64 oracle and the reference's
+golden fixtures (the HIP path is called through the C ABI)."""
+import numpy as np
+import pytest
+
+from conftest import close, fixture_cases, voigt_to_matrix
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lib(gpu_available):
+    from superlinear_amd import _lib
+    _lib.lib()
+    return _lib
+
+
+@pytest.mark.parametrize("case", fixture_cases(), ids=lambda c: c[0])
+def test_fixture_fp64(lib, case):
+    """FP64 HIP EFS matches the reference's FP64 fixtures to 1e-9(1+|x|)
+    (tolerance of embedded_two_cutoff_check.py:146) and the oracle to 1e-10."""
+    from superlinear_amd.model import Sus2Model
+    name, path, frame, rec = case
+    print(f"[case] fixture {name}", flush=True)
+    r = Sus2Model(path).calc_efs([frame], "fp64")
+    close(r["energy"][0], rec["energy"], 1e-9)
+    close(r["forces"], rec["forces"], 1e-9)
+    close(voigt_to_matrix(r["virial"][0]), rec["plus_stress"], 1e-9)
+    close(r["site"].sum(), r["energy"][0], 1e-12)
+    close(r["forces"].sum(0), np.zeros(3), 1e-10)
+
+
+def _oracle_efs(oracle_lib, path, frames):
+    return oracle_lib.OracleModel(path).efs(frames, nthreads=8)
+
+
+CONFIG_CASES = [
+    ("static_l3k4o2", "diamond2", dict(ncell=2, species=1, seed=7)),
+    ("effect_l3k4r2o2", "diamond2", dict(ncell=2, species=2, seed=8)),
+    ("effect_l3k4r2o2", "diamond3", dict(ncell=3, species=2, seed=9)),
+    ("effect_l4k6r2o3", "diamond2", dict(ncell=2, species=3, seed=10)),
+]
+
+
+@pytest.mark.parametrize("mname,label,kw", CONFIG_CASES, ids=[f"{c[0]}-{c[1]}" for c in CONFIG_CASES])
+def test_config_models_fp64_fp32(lib, oracle_lib, mname, label, kw):
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    print(f"[case] {mname} {label}", flush=True)
+    path = W.model_path(mname)
+    frames = [W.diamond(**kw)]
+    ref = _oracle_efs(oracle_lib, path, frames)
+    m = Sus2Model(path)
+    r64 = m.calc_efs(frames, "fp64")
+    scale = 1 + np.abs(ref["forces"]).max()
+    assert abs(r64["energy"][0] - ref["energy"][0]) <= 1e-10 * (1 + abs(ref["energy"][0]))
+    assert np.abs(r64["forces"] - ref["forces"]).max() <= 1e-10 * scale
+    assert np.abs(r64["virial"] - ref["virial"]).max() <= 1e-10 * (1 + np.abs(ref["virial"]).max())
+    r32 = m.calc_efs(frames, "fp32")
+    n = len(frames[0]["types"])
+    # FP32 bulk arithmetic: force max-abs error <= 1e-5 eV/A, E/atom <= 1e-6 eV
+    assert np.abs(r32["forces"] - ref["forces"]).max() <= 1e-5
+    assert abs(r32["energy"][0] - ref["energy"][0]) / n <= 1e-6
+    assert np.abs(r32["virial"] - ref["virial"]).max() <= 1e-4
+
+
+def test_batch_mixed_frames(lib, oracle_lib):
+    """Uneven frames (periodic triclinic and molecules) in one batch."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    print("[case] mixed batch", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frames = W.periodic_frames(3, 8, 40, 2, 0.15, seed=5) + W.molecule_frames(2, 16, 30, 2, seed=6)
+    ref = _oracle_efs(oracle_lib, path, frames)
+    r = Sus2Model(path).calc_efs(frames, "fp64")
+    assert np.abs(r["energy"] - ref["energy"]).max() <= 1e-10 * (1 + np.abs(ref["energy"]).max())
+    assert np.abs(r["forces"] - ref["forces"]).max() <= 1e-10 * (1 + np.abs(ref["forces"]).max())
+    assert np.abs(r["virial"] - ref["virial"]).max() <= 1e-10 * (1 + np.abs(ref["virial"]).max())
+
+
+def test_device_entry_point_matches_host(lib):
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    print("[case] device entry", flush=True)
+    m = Sus2Model(W.model_path("effect_l3k4r2o2"))
+    frames = [W.diamond(2, species=2, seed=3), W.diamond(2, species=2, seed=4)]
+    host = m.calc_efs(frames, "fp64")
+    db = DeviceBatch(frames)
+    out = m.efs_device(db, "fp64")
+    torch.cuda.synchronize()
+    assert np.array_equal(out["energy"].cpu().numpy(), host["energy"])
+    assert np.array_equal(out["forces"].cpu().numpy(), host["forces"])
