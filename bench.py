@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Benchmark of the SUS2-SH EFS hot path on one MI355X.
+
+Workload ("effect4096"): Effect L3/K4/R2/O2 model (gate coupling cross, outer
+coupling diagonal, body order 4, radial basis 10, warp sharing K, per-element
+readout), 2 species drawn 50/50, diamond 8x8x8 conventional cells = 4096
+atoms, a = 3.567 A, Gaussian displacements sigma = 0.10 A, cutoff 5.0 A.
+Random-initialised coefficients from a seed.
+
+A step = one EFS evaluation of the frame (device neighbour list, gate pass,
+outer pass, gather-based force/virial backward, FP64 per-frame reductions)
+with inputs resident in HBM.  `value` is FP32 atom-steps/s; the FP64 rate, the
+FP32/FP64 speed ratio and the FP32 errors against the FP64 oracle ride along.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "EFS atom-steps/s FP64 vs FP32 @4096 atoms; train grad frames/s; % roofline"
+
+
+def _timed(model, batch, precision, steps, warmup, profile):
+    import torch
+    for _ in range(warmup):
+        model.efs_device(batch, precision)
+    torch.cuda.synchronize()
+    model.profile_reset()
+    model.profile(profile)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out = model.efs_device(batch, precision)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    model.profile(False)
+    prof = model.profile_read() if profile else {}
+    return t1 - t0, out, prof
+
+
+def cpu_baseline(model_path, frame, threads, budget_s=20.0):
+    """FP64 oracle EFS on the host cores: `threads` copies of the frame, one
+    per OpenMP thread (frame-parallel, as the reference shards frames over
+    MPI ranks).  Returns atom-steps/s."""
+    from oracle import oracle
+    try:
+        oracle.lib()
+    except RuntimeError:
+        oracle.build()
+    m = oracle.OracleModel(model_path)
+    t0 = time.perf_counter()
+    m.efs([frame], nthreads=1)
+    single = time.perf_counter() - t0
+    copies = max(1, int(round(budget_s / max(single, 1e-3))))
+    copies = threads * max(1, -(-copies // threads))  # whole rounds of `threads` frames
+    t0 = time.perf_counter()
+    m.efs([frame] * copies, nthreads=threads)
+    dt = time.perf_counter() - t0
+    n = len(frame["types"])
+    return dict(value=n * copies / dt, unit="atom-steps/s", cores=threads, kind="port",
+                sample=f"{copies} copies of the {n}-atom frame, FP64 oracle, OpenMP over frames, {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (the path runs on one MI355X); --gpus {args.gpus} is not run",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd.flops import PEAK, kernel_flops, survey_efs_per_edge
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+
+    path = W.model_path("effect_l3k4r2o2")
+    frame = W.config_frames("effect4096")[0]
+    n = len(frame["types"])
+    model = Sus2Model(path)
+    dims = model.dims()
+    batch = DeviceBatch([frame])
+    print(f"[bench] {n} atoms, model {os.path.basename(path)}, steps={args.steps} warmup={args.warmup}",
+          file=sys.stderr, flush=True)
+
+    t32, out32, prof32 = _timed(model, batch, "fp32", args.steps, args.warmup, not args.no_profile)
+    nedges = model.last_edge_count()
+    t64, out64, prof64 = _timed(model, batch, "fp64", args.steps, args.warmup, False)
+    rate32 = n * args.steps / t32
+    rate64 = n * args.steps / t64
+    print(f"[bench] fp32 {rate32:.4g} atom-steps/s  fp64 {rate64:.4g} atom-steps/s", file=sys.stderr, flush=True)
+
+    # FP32 / FP64-GPU errors against the FP64 oracle
+    e32 = {k: v.cpu().numpy() for k, v in out32.items()}
+    e64 = {k: v.cpu().numpy() for k, v in out64.items()}
+    errors = {}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle
+        try:
+            oracle.lib()
+        except RuntimeError:
+            oracle.build()
+        ref = oracle.OracleModel(path).efs([frame], nthreads=1)
+        errors = dict(
+            fp64_gpu_energy_rel=float(abs(e64["energy"][0] - ref["energy"][0]) / abs(ref["energy"][0])),
+            fp64_gpu_force_maxabs=float(np.abs(e64["forces"] - ref["forces"]).max()),
+            fp32_energy_per_atom_abs=float(abs(e32["energy"][0] - ref["energy"][0]) / n),
+            fp32_force_maxabs=float(np.abs(e32["forces"] - ref["forces"]).max()),
+            fp32_virial_maxabs=float(np.abs(e32["virial"] - ref["virial"]).max()),
+            force_scale_maxabs=float(np.abs(ref["forces"]).max()))
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(path, frame, threads)
+
+    roof = None
+    kern = {}
+    if prof32:
+        for name, (ms, cnt) in prof32.items():
+            kern[name] = dict(avg_us=1e3 * ms / max(cnt, 1), launches=cnt)
+        dom = max(prof32.items(), key=lambda kv: kv[1][0])[0]
+        avg_s = prof32[dom][0] / prof32[dom][1] / 1e3
+        fl = kernel_flops(dom, dims, nedges)
+        if fl is not None:
+            ach = fl / avg_s / 1e12
+            roof = dict(bound="mfma", pipe="valu", kernel=dom, achieved=round(ach, 4), peak=PEAK["fp32_tflops"],
+                        unit="TFLOP/s", frac=round(ach / PEAK["fp32_tflops"], 5), traffic=None,
+                        flops_per_launch=fl, avg_launch_us=round(avg_s * 1e6, 2))
+    survey_rate = survey_efs_per_edge(dims) * nedges * args.steps / t32 / 1e12
+
+    line = {
+        "metric": METRIC,
+        "value": round(rate32, 1),
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t32 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "effect4096", "atoms": n, "species": 2, "sus2": "Effect L3/K4/R2/O2 body 4, B=10",
+                   "cutoff_A": 5.0, "directed_edges": nedges, "neighbors_per_atom": round(nedges / n, 2),
+                   "step": "neighbour list + gate pass + outer pass + gather force/virial backward"},
+        "fp64_atom_steps_per_s": round(rate64, 1),
+        "fp64_ms_per_step": round(1e3 * t64 / args.steps, 4),
+        "fp32_over_fp64": round(rate32 / rate64, 3),
+        "train_grad_frames_per_s": None,
+        "step_survey_tflops": round(survey_rate, 4),
+        "errors_vs_fp64_oracle": errors,
+        "kernels_fp32": kern,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "gpu": torch.cuda.get_device_name(0),
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

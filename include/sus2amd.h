@@ -104,6 +104,21 @@ int sus2_loss_grad(sus2_model* m, int precision, int nframes, int natoms, const 
                    const double* e0, const double* f0, const double* s0, int weighting, double wE, double wF,
                    double wS, double* loss_out, double* grad, void* stream);
 
+/* --- per-kernel timing ------------------------------------------------------
+ * Replaces the reference's env-gated wall-clock stage timers
+ * (SUS2_SH_ACCUM_PROFILE, dev_src/mtpr_sh.cpp:1855-1957; pair_style
+ * profile_every, interfaces/lammps/ML-SUS2/pair_sus2_embedded.cpp:320-361)
+ * with HIP events recorded around every kernel launch on its stream.
+ * sus2_profile_read waits for pending events and fills up to `cap` entries:
+ * names (cap * 64 chars, NUL-terminated), total milliseconds and launch
+ * counts; *n receives the number of distinct kernels. */
+int sus2_profile_enable(sus2_model* m, int on);
+int sus2_profile_reset(sus2_model* m);
+int sus2_profile_read(sus2_model* m, int cap, char* names, double* total_ms, int64_t* launches, int* n);
+/* Directed-edge count of the last batch's neighbour list (the sum of the
+ * reference's Neighborhood::count, src/neighborhoods.h:15, over all atoms). */
+int sus2_last_edge_count(const sus2_model* m, int64_t* n);
+
 #ifdef __cplusplus
 }
 #endif

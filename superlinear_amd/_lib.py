@@ -42,6 +42,10 @@ SIGNATURES = {
     "sus2_efs_grad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sus2_loss_grad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _dp, _vp,
                             _vp]),
+    "sus2_profile_enable": (_i, [_vp, _i]),
+    "sus2_profile_reset": (_i, [_vp]),
+    "sus2_profile_read": (_i, [_vp, _i, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_int64), _ip]),
+    "sus2_last_edge_count": (_i, [_vp, ctypes.POINTER(ctypes.c_int64)]),
 }
 
 

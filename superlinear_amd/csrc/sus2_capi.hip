@@ -233,4 +233,41 @@ int sus2_efs_grad(sus2_model* m, int precision, int nframes, int natoms, const i
   });
 }
 
+int sus2_profile_enable(sus2_model* m, int on) {
+  return guard([&] {
+    if (!m) throw sus2::Error(SUS2_E_INVALID, "null model");
+    m->ws.prof.on = on != 0;
+  });
+}
+
+int sus2_profile_reset(sus2_model* m) {
+  return guard([&] {
+    if (!m) throw sus2::Error(SUS2_E_INVALID, "null model");
+    m->ws.prof.reset();
+  });
+}
+
+int sus2_profile_read(sus2_model* m, int cap, char* names, double* total_ms, int64_t* launches, int* n) {
+  return guard([&] {
+    if (!m || !n || cap < 0 || (cap > 0 && (!names || !total_ms || !launches)))
+      throw sus2::Error(SUS2_E_INVALID, "null argument");
+    auto& p = m->ws.prof;
+    p.collect_all();
+    *n = (int)p.names.size();
+    for (int k = 0; k < cap && k < *n; ++k) {
+      std::strncpy(names + 64 * k, p.names[k].c_str(), 63);
+      names[64 * k + 63] = 0;
+      total_ms[k] = p.total_ms[k];
+      launches[k] = p.launches[k];
+    }
+  });
+}
+
+int sus2_last_edge_count(const sus2_model* m, int64_t* n) {
+  return guard([&] {
+    if (!m || !n) throw sus2::Error(SUS2_E_INVALID, "null argument");
+    *n = m->ws.nedges;
+  });
+}
+
 }  // extern "C"

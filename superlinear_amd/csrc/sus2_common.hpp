@@ -1,0 +1,52 @@
+// Shared pieces of the SUS2-SH HIP sources: neighbour records, launch
+// macro with the optional per-kernel HIP-event profiler.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/sus2amd.h"
+#include "sus2_device.hpp"
+#include "sus2_model.hpp"
+#include "sus2_runtime.hpp"
+
+namespace sus2 {
+
+// One directed edge i -> j: d = x_j - x_i (+ image) and the index j stored in
+// the bits of the fourth component (one 16 / 32-byte load per edge).
+template <class T>
+struct alignas(4 * sizeof(T)) Rec {
+  T x, y, z, jb;
+};
+__device__ __forceinline__ int rec_j(const Rec<float>& r) { return __float_as_int(r.jb); }
+__device__ __forceinline__ int rec_j(const Rec<double>& r) { return (int)__double_as_longlong(r.jb); }
+__device__ __forceinline__ float rec_jbits(int j, float) { return __int_as_float(j); }
+__device__ __forceinline__ double rec_jbits(int j, double) { return __longlong_as_double((long long)j); }
+
+#define SUS2_CHECK(x)                                                                                 \
+  do {                                                                                                \
+    hipError_t e_ = (x);                                                                              \
+    if (e_ != hipSuccess) throw Error(SUS2_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Launch with per-kernel event timing when the workspace profiler is on.
+#define SUS2_LAUNCH(ws, st, name, kern, grid, block, lds, ...)   \
+  do {                                                           \
+    const int ev_ = (ws).prof.begin((name), (st));               \
+    hipLaunchKernelGGL(kern, grid, block, lds, st, __VA_ARGS__); \
+    SUS2_CHECK(hipGetLastError());                               \
+    (ws).prof.end(ev_, (st));                                    \
+  } while (0)
+
+template <class T> constexpr const char* tname();
+template <> constexpr const char* tname<float>() { return "f32"; }
+template <> constexpr const char* tname<double>() { return "f64"; }
+
+template <class T>
+void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st);
+template <class T>
+void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st);
+template <class T, int LL, int KK>
+void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st);
+
+}  // namespace sus2

@@ -140,7 +140,7 @@ PassPlan make_plan(const Model& m, bool gate) {
 
 std::string fmt_list(const std::vector<double>& v) {
   std::ostringstream o;
-  o << std::scientific << std::setprecision(15) << "{";
+  o << std::scientific << std::setprecision(16) << "{";  // 17 significant digits: exact round trip
   for (size_t i = 0; i < v.size(); ++i) o << (i ? ", " : "") << v[i];
   o << "}";
   return o.str();

@@ -27,6 +27,62 @@ void DevBuf::ensure(size_t n) {
   bytes = want;
 }
 
+Profiler::~Profiler() {
+  for (auto& p : pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  for (auto e : pool) (void)hipEventDestroy(e);
+}
+
+int Profiler::begin(const std::string& name, hipStream_t st) {
+  if (!on) return -1;
+  int id = -1;
+  for (size_t k = 0; k < names.size(); ++k)
+    if (names[k] == name) { id = (int)k; break; }
+  if (id < 0) {
+    id = (int)names.size();
+    names.push_back(name);
+    total_ms.push_back(0);
+    launches.push_back(0);
+  }
+  hipEvent_t ev[2];
+  for (auto& e : ev) {
+    if (!pool.empty()) { e = pool.back(); pool.pop_back(); }
+    else RT_CHECK(hipEventCreate(&e));
+  }
+  RT_CHECK(hipEventRecord(ev[0], st));
+  pending.push_back({id, ev[0], ev[1]});
+  return (int)pending.size() - 1;
+}
+
+void Profiler::end(int idx, hipStream_t st) {
+  if (idx < 0) return;
+  RT_CHECK(hipEventRecord(pending[idx].b, st));
+}
+
+void Profiler::collect_ready() {
+  if (pending.size() < 4096) return;  // bound the number of outstanding events
+  collect_all();
+}
+
+void Profiler::collect_all() {
+  for (auto& p : pending) {
+    RT_CHECK(hipEventSynchronize(p.b));
+    float ms = 0;
+    RT_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+    total_ms[p.id] += ms;
+    launches[p.id] += 1;
+    pool.push_back(p.a);
+    pool.push_back(p.b);
+  }
+  pending.clear();
+}
+
+void Profiler::reset() {
+  collect_all();
+  names.clear();
+  total_ms.clear();
+  launches.clear();
+}
+
 Workspace::~Workspace() {
   if (host_total) (void)hipHostFree(host_total);
 }

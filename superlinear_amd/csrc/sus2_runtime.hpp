@@ -40,7 +40,26 @@ struct EfsOut {
   double *energy = nullptr, *force = nullptr, *virial = nullptr, *site = nullptr;
 };
 
+// Optional per-kernel timing: an event pair around every launch on the
+// launch stream, accumulated per kernel name when the events complete.
+struct Profiler {
+  bool on = false;
+  struct Pending { int id; hipEvent_t a, b; };
+  std::vector<std::string> names;
+  std::vector<double> total_ms;
+  std::vector<long> launches;
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+  ~Profiler();
+  int begin(const std::string& name, hipStream_t st);
+  void end(int idx, hipStream_t st);
+  void collect_ready();  // accumulate completed pairs (non-blocking)
+  void collect_all();    // wait for and accumulate every pending pair
+  void reset();
+};
+
 struct Workspace {
+  Profiler prof;
   DevBuf nbr_cnt, nbr_off, scan_tmp, rec, h, gbasic, abar, gbar, site, fw;
   // parameter-gradient scratch
   DevBuf gtan, hdot, ebar, wedge, tgt, gpart, gacc;

@@ -1,0 +1,66 @@
+"""Algorithmic FLOP counts (FMA = 2 flops) of the SUS2-SH EFS path.
+
+Counts are per *directed edge* of one pass (each directed edge once, as the
+reference's scatter evaluates it), from the loaded model's dimensions:
+H = (L+1)^2, F = K(L+1), B radial basis size, Kw warp blocks, KH = K H.
+
+* ``moments``  (one pass, forward):
+  geometry 10 + radial basis Kw (13 + 3B) + contraction/envelope F (2B + 3)
+  + type weights 3F + harmonics 5H + moment accumulation 2 KH
+* ``force``    (one pass, coordinate reverse):
+  geometry 10 + radial basis and r-derivative Kw (13 + 7B)
+  + contraction/envelope of R and R' F (4B + 6) + weights 4F
+  + harmonics and gradient 20H + adjoint contraction 11 KH
+  (+ 4 KH for the outer pass of an Effect model: the gate adjoint h_bar)
+  + force / virial accumulation 21
+* ``survey_efs`` is the whole-step figure F_e^E (Effect) / F_e^S (Static)
+  of the path survey: F_e^S = 11 + 8H + Kw (10B + 12) + 4FB + 11 KH + F + 3,
+  F_e^E = F_e^S + 4FB + 13 KH + 2F.
+
+The gather-based backward evaluates every directed edge from both ends, so
+its executed work is about twice the algorithmic count; the roofline fraction
+reports achieved algorithmic work, so that overhead shows up as a lower
+fraction, not as inflated throughput.
+"""
+
+PEAK = {  # MI355X published peaks
+    "fp32_tflops": 157.3,
+    "fp64_tflops": 78.6,
+    "hbm_gbs": 8000.0,
+}
+
+
+def dims_of(d):
+    L, K, B, Kw = d["L"], d["K"], d["B"], d["Kw"]
+    H = (L + 1) ** 2
+    F = K * (L + 1)
+    return L, K, B, Kw, H, F, K * H
+
+
+def moments_per_edge(d):
+    L, K, B, Kw, H, F, KH = dims_of(d)
+    return 10 + Kw * (13 + 3 * B) + F * (2 * B + 3) + 3 * F + 5 * H + 2 * KH
+
+
+def force_per_edge(d, outer_effect=False):
+    L, K, B, Kw, H, F, KH = dims_of(d)
+    return 10 + Kw * (13 + 7 * B) + F * (4 * B + 6) + 4 * F + 20 * H + 11 * KH + (4 * KH if outer_effect else 0) + 21
+
+
+def survey_efs_per_edge(d):
+    L, K, B, Kw, H, F, KH = dims_of(d)
+    s = 11 + 8 * H + Kw * (10 * B + 12) + 4 * F * B + 11 * KH + F + 3
+    if d.get("gated"):
+        s += 4 * F * B + 13 * KH + 2 * F
+    return s
+
+
+def kernel_flops(name, d, nedges):
+    """Algorithmic flops of one launch of the named kernel over nedges."""
+    if name.startswith("moments_"):
+        return moments_per_edge(d) * nedges
+    if name.startswith("force_outer"):
+        return force_per_edge(d, outer_effect=bool(d.get("gated"))) * nedges
+    if name.startswith("force_gate"):
+        return force_per_edge(d) * nedges
+    return None

@@ -93,6 +93,35 @@ class Sus2Model:
     def save(self, path):
         _lib.check(_lib.lib().sus2_model_save(self._h, str(path).encode()))
 
+    # ------------------------------------------------------------ diagnostics
+    def last_edge_count(self):
+        n = ctypes.c_int64()
+        _lib.check(_lib.lib().sus2_last_edge_count(self._h, ctypes.byref(n)))
+        return n.value
+
+    def profile(self, on=True):
+        _lib.check(_lib.lib().sus2_profile_enable(self._h, 1 if on else 0))
+
+    def profile_reset(self):
+        _lib.check(_lib.lib().sus2_profile_reset(self._h))
+
+    def profile_read(self):
+        """{kernel name: (total ms, launches)} from the HIP-event timers."""
+        cap = 64
+        names = ctypes.create_string_buffer(64 * cap)
+        ms = np.zeros(cap)
+        cnt = np.zeros(cap, np.int64)
+        n = ctypes.c_int()
+        _lib.check(_lib.lib().sus2_profile_read(self._h, cap, names, ms.ctypes.data_as(_lib._dp),
+                                                cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                                ctypes.byref(n)))
+        raw = names.raw
+        out = {}
+        for k in range(min(n.value, cap)):
+            nm = raw[64 * k:64 * (k + 1)].split(b"\0", 1)[0].decode()
+            out[nm] = (float(ms[k]), int(cnt[k]))
+        return out
+
     # ------------------------------------------------------------ evaluation
     def calc_efs(self, frames, precision="fp64"):
         """EFS of a list of frames (host arrays in, host arrays out)."""

@@ -1,0 +1,156 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every declared symbol, the model parser / coefficient layout agree with the
+oracle's independent parser, save/load round trips, the topology generator
+reproduces the reference fixture graphs, CFG I/O."""
+import io
+import re
+
+import numpy as np
+import pytest
+
+from conftest import FIXTURES, ROOT
+
+
+def _declared_symbols():
+    text = (ROOT / "include" / "sus2amd.h").read_text()
+    return sorted(set(re.findall(r"\b(sus2_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_declared_symbols():
+    from superlinear_amd import _lib
+    L = _lib.lib()
+    declared = _declared_symbols()
+    assert len(declared) >= 12
+    for name in declared:
+        assert hasattr(L, name), name
+    assert set(declared) == set(_lib.SIGNATURES)
+    assert L.sus2_abi_version() == 1
+
+
+MODELS = [FIXTURES / "fixtures_v2" / "static.mtp", FIXTURES / "fixtures_v2" / "effect.mtp",
+          FIXTURES / "fixtures_hi_l" / "effect_l3k4r3o3.mtp"]
+
+
+@pytest.mark.parametrize("path", MODELS + ["static_l3k4o2", "effect_l3k4r2o2", "effect_l4k6r2o3", "effect_l5k6r2o3"],
+                         ids=str)
+def test_parser_layout_matches_oracle(oracle_lib, path):
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    if isinstance(path, str):
+        path = W.model_path(path)
+    m = Sus2Model(path)
+    o = oracle_lib.OracleModel(path)
+    assert np.array_equal(m.coeffs(), o.coeffs())
+    off_m, off_o = m.offsets(), o.offsets()
+    for k in off_o:
+        assert off_m[k] == off_o[k], k
+    dm, do = m.dims(), o.dims()
+    for k in do:
+        assert dm[k] == do[k], k
+
+
+def test_save_load_round_trip(tmp_path, oracle_lib):
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    m = Sus2Model(W.model_path("effect_l3k4r2o2"))
+    theta = m.coeffs()
+    rng = np.random.default_rng(0)
+    t2 = theta + 1e-3 * rng.normal(size=theta.size)
+    m.set_coeffs(t2)
+    p = tmp_path / "x.sus2"
+    m.save(p)
+    m2 = Sus2Model(p)
+    assert np.array_equal(m2.coeffs(), t2)
+    # the saved file is a valid model for the independent oracle parser too
+    assert np.array_equal(oracle_lib.OracleModel(p).coeffs(), t2)
+
+
+def test_invalid_models_rejected(tmp_path):
+    from superlinear_amd import _lib
+    from superlinear_amd.model import Sus2Model
+    with pytest.raises(_lib.Sus2Error):
+        Sus2Model(tmp_path / "missing.sus2")
+    bad = tmp_path / "bad.sus2"
+    bad.write_text((FIXTURES / "fixtures_v2" / "static.mtp").read_text().replace("RBChebyshev_sss", "RBShapeev"))
+    with pytest.raises(_lib.Sus2Error) as e:
+        Sus2Model(bad)
+    assert e.value.code == 3
+    cyc = tmp_path / "cyc.sus2"
+    cyc.write_text((FIXTURES / "fixtures_v2" / "static.mtp").read_text().replace("{0, 0, 8,", "{9, 0, 8,"))
+    with pytest.raises(_lib.Sus2Error):
+        Sus2Model(cyc)
+
+
+def _fields(text):
+    out = {}
+    for line in text.splitlines():
+        if "=" in line:
+            k, v = line.split("=", 1)
+            out[k.strip()] = v.strip()
+    return out
+
+
+def _nums(s):
+    return np.array([float(x) for x in re.findall(r"[-+]?\d+\.?\d*(?:[eE][-+]?\d+)?", s)])
+
+
+GRAPH_KEYS = ["alpha_moments_count", "alpha_index_basic", "sh_products", "alpha_moment_mapping", "sh_scalar_info"]
+GATE_KEYS = GRAPH_KEYS + ["two_layer_gate_k_rank_basic_begin", "outer_k_rank1_basic_begin", "outer_k_rank1_product_begin",
+                          "two_layer_gate_k_rank_product_end", "two_layer_gate_scalar_moment_indices",
+                          "two_layer_gate_scalar_body_orders"]
+
+
+@pytest.mark.parametrize("fixture,kw,keys", [
+    ("fixtures_v2/static.mtp", dict(lmax=1, kmax=2, species=2, body_order=3, outer_rank=0, rb_size=3), GRAPH_KEYS),
+    ("fixtures_v2/effect.mtp", dict(lmax=1, kmax=2, species=2, body_order=3, outer_rank=2, gate=True, gate_rank=2,
+                                    rb_size=3), GATE_KEYS),
+    ("fixtures_hi_l/effect_l3k4r3o3.mtp", dict(lmax=3, kmax=4, species=2, body_order=3, outer_rank=3, gate=True,
+                                                gate_rank=3, rb_size=3), GATE_KEYS),
+])
+def test_topology_reproduces_reference_graphs(fixture, kw, keys):
+    """The generator re-derives the product graphs (nodes, real CG
+    coefficients, scalar and segment layout) serialised in the reference's
+    fixtures (dev_src/sh_model_init.cpp conventions)."""
+    from superlinear_amd.topology import build_model
+    a = _fields((FIXTURES / fixture).read_text())
+    b = _fields(build_model(**kw))
+    for k in keys:
+        x, y = _nums(a[k]), _nums(b[k])
+        assert x.shape == y.shape, k
+        assert np.abs(x - y).max(initial=0) <= 1e-12, k
+
+
+def test_real_cg_orthogonality():
+    """Real coupling coefficients form an orthogonal transform of the
+    (l1 x l2) product space (sum over L, M of C C = delta)."""
+    from superlinear_amd.topology import real_cg
+    for l1, l2 in ((1, 1), (2, 1), (2, 2), (3, 2)):
+        n = (2 * l1 + 1) * (2 * l2 + 1)
+        U = np.zeros((n, n))
+        row = 0
+        for L in range(abs(l1 - l2), l1 + l2 + 1):
+            for M in range(-L, L + 1):
+                U[row] = [real_cg(l1, a, l2, b, L, M) for a in range(-l1, l1 + 1) for b in range(-l2, l2 + 1)]
+                row += 1
+        assert np.allclose(U @ U.T, np.eye(n), atol=1e-12)
+
+
+def test_cfg_round_trip():
+    from superlinear_amd.cfg import read_cfg, write_cfg
+    frames = list(read_cfg(FIXTURES / "fixtures_v2" / "geometries.cfg"))
+    assert len(frames) == 7
+    f = dict(frames[0], forces=np.arange(9.0).reshape(3, 3) * 0.1, energy=-1.5, virial=np.arange(6.0))
+    buf = io.StringIO()
+    write_cfg(buf, f, lossless=True)
+    path = ROOT / "_work" / "rt.cfg"
+    path.parent.mkdir(exist_ok=True)
+    path.write_text(buf.getvalue())
+    g = next(read_cfg(path))
+    assert np.array_equal(g["positions"], f["positions"])
+    assert np.array_equal(g["forces"], f["forces"])
+    assert g["energy"] == f["energy"]
+    assert np.array_equal(g["virial"], f["virial"])
+    mol = dict(types=np.array([0, 1]), positions=np.array([[0, 0, 0], [1.0, 0, 0]]), periodic=False)
+    buf = io.StringIO()
+    write_cfg(buf, mol)
+    assert "Supercell" not in buf.getvalue()

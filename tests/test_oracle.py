@@ -1,0 +1,222 @@
+"""Pins the FP64 oracle: reference golden fixtures, the compiled reference
+engine (oracle/_ref), closed forms, finite differences and invariants."""
+import json
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import FIXTURES, close, fixture_cases, voigt_to_matrix
+
+
+@pytest.mark.parametrize("case", fixture_cases(), ids=lambda c: c[0])
+def test_oracle_matches_reference_fixtures(oracle_lib, case):
+    """interfaces/lammps/tests/embedded_two_cutoff_check.py:136-150 semantics:
+    E/F/virial within 1e-9 (1 + |x|), site sum = energy, sum F = 0."""
+    name, path, frame, rec = case
+    r = oracle_lib.OracleModel(path).efs([frame])
+    close(r["energy"][0], rec["energy"], 1e-9)
+    close(r["forces"], rec["forces"], 1e-9)
+    close(voigt_to_matrix(r["virial"][0]), rec["plus_stress"], 1e-9)
+    close(r["site"].sum(), r["energy"][0], 1e-12)
+    close(r["forces"].sum(0), np.zeros(3), 1e-10)
+
+
+def _brute_neighbors(x, cell, periodic, rc):
+    n = len(x)
+    out = set()
+    rng = range(-4, 5) if periodic else range(0, 1)
+    for i in range(n):
+        for j in range(n):
+            for a in rng:
+                for b in rng:
+                    for c in rng:
+                        if i == j and a == b == c == 0:
+                            continue
+                        d = x[j] - x[i] + (a * cell[0] + b * cell[1] + c * cell[2] if periodic else 0)
+                        if d @ d < rc * rc:
+                            out.add((i, j, round(d[0], 9), round(d[1], 9), round(d[2], 9)))
+    return out
+
+
+@pytest.mark.parametrize("seed", [5, 6, 7])
+def test_neighbors_match_brute_force(oracle_lib, seed):
+    from superlinear_amd import workloads as W
+    for f in W.periodic_frames(3, 6, 14, 2, 0.15, seed=seed) + W.molecule_frames(1, 10, 14, 2, seed=seed):
+        ij, d = oracle_lib.neighbors(f["positions"], f["cell"], f["periodic"], 5.0)
+        got = {(int(a), int(b), round(v[0], 9), round(v[1], 9), round(v[2], 9)) for (a, b), v in zip(ij, d)}
+        want = _brute_neighbors(f["positions"], f["cell"], f["periodic"], 5.0)
+        assert len(ij) == len(got) == len(want)
+        assert got == want
+
+
+def _real_sh_scipy(L, d):
+    from scipy.special import sph_harm_y
+    r = np.linalg.norm(d)
+    theta, phi = np.arccos(d[2] / r), np.arctan2(d[1], d[0])
+    out = np.zeros((L + 1) ** 2)
+    for l in range(L + 1):
+        for m in range(-l, l + 1):
+            y = sph_harm_y(l, abs(m), theta, phi)  # includes the Condon-Shortley phase
+            cs = (-1) ** abs(m)
+            if m == 0:
+                v = y.real
+            elif m > 0:
+                v = np.sqrt(2) * cs * y.real
+            else:
+                v = np.sqrt(2) * cs * y.imag
+            out[l * l + l + m] = v
+    return out
+
+
+def test_real_sh_against_scipy_and_fd(oracle_lib):
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        d = rng.normal(size=3) * 2
+        Y, dY = oracle_lib.sh(6, d)
+        assert np.allclose(Y, _real_sh_scipy(6, d), atol=1e-13)
+        h = 1e-6
+        for q in range(3):
+            e = np.zeros(3)
+            e[q] = h
+            fd = (oracle_lib.sh(6, d + e)[0] - oracle_lib.sh(6, d - e)[0]) / (2 * h)
+            assert np.allclose(dY[:, q], fd, atol=1e-8)
+    # (y, z, x) convention of l = 1 and the polar axis
+    Y, _ = oracle_lib.sh(1, np.array([0.0, 0.0, 2.0]))
+    assert np.allclose(Y[1:], [0, np.sqrt(3 / (4 * np.pi)), 0])
+
+
+def _ref_efs(path, frame):
+    from oracle import oracle
+    from superlinear_amd.cfg import write_cfg
+    import tempfile
+    with tempfile.NamedTemporaryFile("w", suffix=".cfg", delete=False) as fh:
+        write_cfg(fh, frame, lossless=True)
+        name = fh.name
+    out = subprocess.run([str(oracle.ref_binary()), str(path), name, "1", "direct"], capture_output=True, text=True,
+                         check=True)
+    return json.loads(out.stdout.split("NATIVE_EFS_JSON ")[1].splitlines()[0])
+
+
+REF_CASES = [("static_l3k4o2", 1, 2), ("effect_l3k4r2o2", 2, 3), ("effect_l4k6r2o3", 3, 4)]
+
+
+@pytest.mark.parametrize("mname,species,seed", REF_CASES)
+def test_oracle_matches_compiled_reference(oracle_lib, mname, species, seed):
+    """Oracle vs the reference embedded engine built from its own sources
+    (oracle/_ref, L <= 4), on generated config models and a periodic
+    triclinic frame with several images per pair."""
+    from superlinear_amd import workloads as W
+    if oracle_lib.ref_binary() is None:
+        pytest.skip("reference engine not built (reference tree absent)")
+    path = W.model_path(mname)
+    for frame in [W.diamond(2, species=species, seed=seed)] + W.periodic_frames(1, 10, 14, species, 0.15, seed=seed):
+        ref = _ref_efs(path, frame)
+        r = oracle_lib.OracleModel(path).efs([frame])
+        close(r["energy"][0], ref["energy"], 1e-10)
+        close(r["forces"], ref["forces"], 1e-10)
+        close(voigt_to_matrix(r["virial"][0]), ref["virial"], 1e-10)
+        close(r["site"], ref["site_energies"], 1e-10)
+
+
+def _energy(m, frame):
+    return m.efs([frame])["energy"][0]
+
+
+@pytest.mark.parametrize("mname,species", [("effect_l3k4r2o2", 2), ("static_l3k4o2", 1)])
+def test_forces_and_virial_by_finite_differences(oracle_lib, mname, species):
+    from superlinear_amd import workloads as W
+    m = oracle_lib.OracleModel(W.model_path(mname))
+    frame = W.periodic_frames(1, 8, 10, species, 0.15, seed=11)[0]
+    r = m.efs([frame])
+    h = 1e-5
+    x = frame["positions"]
+    for i in range(0, len(x), 3):
+        for q in range(3):
+            fp = dict(frame, positions=x.copy())
+            fm = dict(frame, positions=x.copy())
+            fp["positions"][i, q] += h
+            fm["positions"][i, q] -= h
+            fd = -(_energy(m, fp) - _energy(m, fm)) / (2 * h)
+            assert abs(fd - r["forces"][i, q]) <= 1e-7 * (1 + abs(fd))
+    W6 = voigt_to_matrix(r["virial"][0])
+    for a, b in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)):
+        def strained(s):
+            f = dict(frame, positions=x.copy(), cell=frame["cell"].copy())
+            f["positions"][:, a] += s * h * x[:, b]
+            f["cell"][:, a] += s * h * frame["cell"][:, b]
+            return _energy(m, f)
+        fd = -(strained(1) - strained(-1)) / (2 * h)
+        assert abs(fd - W6[a, b]) <= 1e-7 * (1 + abs(fd))
+
+
+def test_invariances(oracle_lib):
+    from superlinear_amd import workloads as W
+    m = oracle_lib.OracleModel(W.model_path("effect_l3k4r2o2"))
+    frame = W.periodic_frames(1, 12, 16, 2, 0.15, seed=3)[0]
+    base = m.efs([frame])
+    rng = np.random.default_rng(1)
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    rot = dict(frame, positions=frame["positions"] @ q.T, cell=frame["cell"] @ q.T)
+    r = m.efs([rot])
+    close(r["energy"], base["energy"], 1e-12)
+    close(r["forces"], base["forces"] @ q.T, 1e-12)
+    sh = dict(frame, positions=frame["positions"] + np.array([0.3, -1.1, 2.0]))
+    close(m.efs([sh])["energy"], base["energy"], 1e-12)
+    perm = rng.permutation(len(frame["types"]))
+    pf = dict(frame, positions=frame["positions"][perm], types=frame["types"][perm])
+    rp = m.efs([pf])
+    close(rp["energy"], base["energy"], 1e-12)
+    close(rp["forces"], base["forces"][perm], 1e-12)
+    Wm = voigt_to_matrix(base["virial"][0])
+    assert np.allclose(Wm, Wm.T)
+
+
+def test_cutoff_endpoint_and_far_shell(oracle_lib):
+    """embedded_two_cutoff_check.py:155-182: a pair exactly at rc gives zero
+    force; between rca and rc only the l = 0 channel interacts."""
+    m = oracle_lib.OracleModel(FIXTURES / "fixtures_v2" / "effect.mtp")
+    rc, rca = 6.5, 5.0
+
+    def pair(d):
+        return dict(types=np.array([0, 1], np.int32), positions=np.array([[0, 0, 0], [d, 0, 0]], float),
+                    cell=np.eye(3) * 4 * rc, periodic=True)
+    end = m.efs([pair(rc)])
+    out = m.efs([pair(rc + 1e-5)])
+    close(end["energy"], out["energy"], 1e-12)
+    close(end["forces"], np.zeros((2, 3)), 1e-12)
+    far = m.efs([pair((rc + rca) / 2)])
+    assert np.abs(far["forces"]).max() > 1e-8
+
+
+def test_loss_gradient_by_finite_differences(oracle_lib):
+    """The hyper-dual parameter gradient of the training objective matches
+    central finite differences (h = 1e-6) coefficient by coefficient."""
+    from superlinear_amd import workloads as W
+    m = oracle_lib.OracleModel(W.model_path("effect_l3k4r2o2"))
+    frames = W.periodic_frames(2, 5, 7, 2, 0.15, seed=21)
+    n = sum(len(f["types"]) for f in frames)
+    rng = np.random.default_rng(2)
+    labels = dict(energy=rng.normal(size=2), forces=rng.normal(size=(n, 3)) * 0.1, virial=rng.normal(size=(2, 6)))
+    out = m.loss_grad(frames, labels, mode=1, wE=1.0, wF=0.01, wS=0.001, nthreads=4)
+    theta = m.coeffs()
+    g = out["grad"]
+    # the 30 largest gradient entries plus 30 random ones
+    idx = np.unique(np.concatenate([np.argsort(-np.abs(g))[:30], rng.choice(len(theta), 30, replace=False)]))
+    h = 1e-6
+    fds = []
+    for p in idx:
+        t = theta.copy()
+        t[p] += h
+        m.set_coeffs(t)
+        lp = m.loss(frames, labels, mode=1, wE=1.0, wF=0.01, wS=0.001)
+        t[p] -= 2 * h
+        m.set_coeffs(t)
+        lm = m.loss(frames, labels, mode=1, wE=1.0, wF=0.01, wS=0.001)
+        m.set_coeffs(theta)
+        fd = (lp - lm) / (2 * h)
+        fds.append(fd)
+        # central-difference noise floor ~ 1e-16 |L| / h
+        assert abs(fd - g[p]) <= 1e-5 * abs(g[p]) + 1e-9 * (1 + abs(out["loss"])), p
+    fds = np.array(fds)
+    assert np.linalg.norm(fds - g[idx]) <= 1e-6 * np.linalg.norm(g[idx])
