@@ -128,7 +128,8 @@ def main():
     if prof32:
         for name, (ms, cnt) in prof32.items():
             kern[name] = dict(avg_us=1e3 * ms / max(cnt, 1), launches=cnt)
-        dom = max(prof32.items(), key=lambda kv: kv[1][0])[0]
+        timed = {k: v for k, v in prof32.items() if kernel_flops(k, dims, 1) is not None}
+        dom = max(timed.items(), key=lambda kv: kv[1][0])[0]
         avg_s = prof32[dom][0] / prof32[dom][1] / 1e3
         fl = kernel_flops(dom, dims, nedges)
         if fl is not None:

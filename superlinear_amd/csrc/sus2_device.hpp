@@ -14,7 +14,8 @@ constexpr int BMAX = 16;  // largest radial basis size the kernels are built for
 // ------------------------------------------------------------------ model view
 template <class T>
 struct DPass {  // layered product-graph schedule of one pass (see PassPlan)
-  int nfl = 0, nrl = 0, rank = 0, begin = 0;
+  int nfl = 0, nrl = 0, rank = 0, begin = 0, node_lo = 0, node_cnt = 0;
+  int nfout = 0, nrnode = 0;  // total forward outs / reverse nodes
   const int *fl_off = nullptr, *f_out = nullptr, *f_roff = nullptr, *f_a = nullptr, *f_b = nullptr;
   const T* f_c = nullptr;
   const int *rl_off = nullptr, *r_node = nullptr, *r_roff = nullptr, *r_out = nullptr, *r_partner = nullptr;
@@ -220,84 +221,9 @@ __device__ __forceinline__ T wave_sum(T v) {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x; }
 
-// Forward walk of one pass's product layers: out = sum_rows c v[a] v[b].
-template <class T>
-__device__ void graph_forward(const DPass<T>& p, T* v) {
-  const int lane = lane_id();
-  for (int l = 0; l < p.nfl; ++l) {
-    for (int o = p.fl_off[l] + lane; o < p.fl_off[l + 1]; o += WAVE) {
-      T s = T(0);
-      for (int r = p.f_roff[o]; r < p.f_roff[o + 1]; ++r) s += p.f_c[r] * v[p.f_a[r]] * v[p.f_b[r]];
-      v[p.f_out[o]] = s;
-    }
-    __syncthreads();
-  }
-}
-
-// Reverse walk: a[n] += sum over consumer rows c a[out] v[partner].
-template <class T>
-__device__ void graph_reverse(const DPass<T>& p, const T* v, T* a) {
-  const int lane = lane_id();
-  for (int l = 0; l < p.nrl; ++l) {
-    for (int i = p.rl_off[l] + lane; i < p.rl_off[l + 1]; i += WAVE) {
-      const int n = p.r_node[i];
-      T s = a[n];
-      for (int r = p.r_roff[i]; r < p.r_roff[i + 1]; ++r) s += p.r_c[r] * a[p.r_out[r]] * v[p.r_partner[r]];
-      a[n] = s;
-    }
-    __syncthreads();
-  }
-}
-
 // rank node of channel a at (l, m) for a pass
-__device__ __forceinline__ int rank_node(int begin, int rank, int H, int l, int m, int a) {
+__host__ __device__ __forceinline__ int rank_node(int begin, int rank, int H, int l, int m, int a) {
   return begin + rank * (H - (l + 1) * (l + 1)) + (rank - 1 - a) * (2 * l + 1) + m + l;
-}
-
-// Contraction of canonical moments (in LDS `mom`, h-major k-minor) into the
-// rank nodes of a pass; basic nodes also receive their moments.
-template <class T>
-__device__ void load_and_contract(const DModel<T>& m, bool gate, const T* mom, T* v) {
-  const int lane = lane_id();
-  const int KH = m.K * m.H;
-  for (int n = lane; n < m.M; n += WAVE) v[n] = T(0);
-  __syncthreads();
-  for (int c = lane; c < KH; c += WAVE) {
-    const int node = m.basic_node[c];
-    if (node >= 0) v[node] = mom[c];
-  }
-  const int rank = gate ? m.R : m.O, begin = gate ? m.gp.begin : m.op.begin;
-  const int nr = rank * m.H;
-  for (int idx = lane; idx < nr; idx += WAVE) {
-    const int a = idx / m.H, h = idx % m.H;
-    int l = 0;
-    while ((l + 1) * (l + 1) <= h) ++l;
-    const int mm = h - l * l - l;
-    T s = T(0);
-    for (int k = 0; k < m.K; ++k)
-      if (m.basic_node[h * m.K + k] >= 0) s += m.rankw(gate, l, a, k) * mom[h * m.K + k];
-    v[rank_node(begin, rank, m.H, l, mm, a)] = s;
-  }
-  __syncthreads();
-}
-
-// Transposed contraction: canonical basic adjoints from node adjoints.
-template <class T>
-__device__ void adjoint_to_basic(const DModel<T>& m, bool gate, const T* a, T* out /* global or LDS [H*K] */) {
-  const int lane = lane_id();
-  const int KH = m.K * m.H;
-  const int rank = gate ? m.R : m.O, begin = gate ? m.gp.begin : m.op.begin;
-  for (int c = lane; c < KH; c += WAVE) {
-    const int h = c / m.K, k = c % m.K;
-    int l = 0;
-    while ((l + 1) * (l + 1) <= h) ++l;
-    const int mm = h - l * l - l;
-    const int node = m.basic_node[c];
-    T s = node >= 0 ? a[node] : T(0);
-    if (node >= 0)
-      for (int aa = 0; aa < rank; ++aa) s += m.rankw(gate, l, aa, k) * a[rank_node(begin, rank, m.H, l, mm, aa)];
-    out[c] = s;
-  }
 }
 
 }  // namespace sus2

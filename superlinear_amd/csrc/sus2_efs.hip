@@ -37,6 +37,25 @@
 namespace sus2 {
 
 // ------------------------------------------------------------------ neighbours
+// Cell-list build of the full directed neighbour list, every periodic image
+// with |d| < rc (the semantics of src/neighborhoods.cpp:276-348 and
+// interfaces/lammps/tests/embedded_efs.cpp:44-66: strict r < rc, self images
+// count, a pair may appear through several images in small cells).
+//
+// Per frame a grid of n_q bins along each lattice direction in fractional
+// coordinates (bin width >= the cutoff's fractional reach, at most one bin per
+// atom so the bins of frame f can use ids frame_off[f] ...); molecules use
+// their bounding box.  An atom scans bin offsets [-S_q, S_q] with
+// S_q = floor(reach_q n_q) + 1, which covers every image in small cells
+// (each (bin, shift) pair is a distinct image).  Bin contents are sorted by
+// atom index, so the edge order -- and every sum over it -- is deterministic.
+struct Grid {
+  double inv[9];  // fractional = cartesian * inv (row vector convention)
+  double org[3];  // molecule bounding-box origin
+  int n[3], S[3];
+  int per;
+};
+
 __device__ __forceinline__ int frame_of(const int64_t* off, int nframes, int i) {
   int lo = 0, hi = nframes;
   while (hi - lo > 1) {
@@ -47,61 +66,167 @@ __device__ __forceinline__ int frame_of(const int64_t* off, int nframes, int i) 
   return lo;
 }
 
-// Full neighbour list of atom i: every atom j of its frame and every periodic
-// image with |d| < rc, skipping the atom itself (same order and image ranges
-// as interfaces/lammps/tests/embedded_efs.cpp:44-66).  Count pass when rec is
-// null, fill pass otherwise.
-template <class T>
-__global__ void k_neighbors(int natoms, int nframes, const int64_t* __restrict__ frame_off,
-                            const double* __restrict__ pos, const double* __restrict__ cell,
-                            const int32_t* __restrict__ periodic, double rc, const int* __restrict__ off,
-                            int* __restrict__ cnt, Rec<T>* __restrict__ rec) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= natoms) return;
-  const int f = frame_of(frame_off, nframes, i);
-  const long base = frame_off[f], n = frame_off[f + 1] - base;
-  const double* a = cell + 9 * f;
-  double inv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, reach[3] = {0, 0, 0};
-  const bool per = periodic[f] != 0;
-  if (per) {
+__global__ void k_grid_setup(int nframes, const int64_t* __restrict__ frame_off, const double* __restrict__ pos,
+                             const double* __restrict__ cell, const int32_t* __restrict__ periodic, double rc,
+                             Grid* __restrict__ grid) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nframes) return;
+  Grid g;
+  const long b0 = frame_off[f], na = frame_off[f + 1] - b0;
+  g.per = periodic[f] != 0;
+  double reach[3];
+  if (g.per) {
+    const double* a = cell + 9 * f;
     double b[3][3];
     for (int p = 0; p < 3; ++p)
       for (int q = 0; q < 3; ++q)
         b[q][p] = a[((p + 1) % 3) * 3 + (q + 1) % 3] * a[((p + 2) % 3) * 3 + (q + 2) % 3] -
                   a[((p + 1) % 3) * 3 + (q + 2) % 3] * a[((p + 2) % 3) * 3 + (q + 1) % 3];
-    double det = a[0] * b[0][0] + a[1] * b[1][0] + a[2] * b[2][0];
+    const double det = a[0] * b[0][0] + a[1] * b[1][0] + a[2] * b[2][0];
     for (int p = 0; p < 3; ++p)
-      for (int q = 0; q < 3; ++q) inv[p][q] = b[p][q] / det;
-    for (int q = 0; q < 3; ++q) reach[q] = rc * sqrt(inv[0][q] * inv[0][q] + inv[1][q] * inv[1][q] + inv[2][q] * inv[2][q]);
+      for (int q = 0; q < 3; ++q) g.inv[3 * p + q] = b[p][q] / det;
+    for (int q = 0; q < 3; ++q) {
+      reach[q] = rc * sqrt(g.inv[q] * g.inv[q] + g.inv[3 + q] * g.inv[3 + q] + g.inv[6 + q] * g.inv[6 + q]);
+      g.n[q] = (int)fmin(1024.0, fmax(1.0, floor(1.0 / reach[q])));
+      g.org[q] = 0;
+    }
+  } else {
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (long k = 0; k < na; ++k)
+      for (int q = 0; q < 3; ++q) {
+        const double x = pos[3 * (b0 + k) + q];
+        if (k == 0 || x < lo[q]) lo[q] = x;
+        if (k == 0 || x > hi[q]) hi[q] = x;
+      }
+    for (int p = 0; p < 9; ++p) g.inv[p] = 0;
+    for (int q = 0; q < 3; ++q) {
+      const double ext = fmax(hi[q] - lo[q], 1e-6) * (1 + 1e-9) + 1e-9;
+      g.org[q] = lo[q];
+      g.inv[4 * q] = 1.0 / ext;
+      reach[q] = rc / ext;
+      g.n[q] = (int)fmin(1024.0, fmax(1.0, floor(ext / rc)));
+    }
   }
+  // at most one bin per atom
+  while ((long)g.n[0] * g.n[1] * g.n[2] > (na > 0 ? na : 1)) {
+    int q = 0;
+    if (g.n[1] > g.n[q]) q = 1;
+    if (g.n[2] > g.n[q]) q = 2;
+    g.n[q] = max(1, g.n[q] - 1);
+  }
+  for (int q = 0; q < 3; ++q) g.S[q] = g.per ? (int)floor(reach[q] * g.n[q]) + 1 : 1;
+  grid[f] = g;
+}
+
+__device__ __forceinline__ void frac_of(const Grid& g, const double* x, double* f) {
+  for (int q = 0; q < 3; ++q)
+    f[q] = (x[0] - g.org[0]) * g.inv[q] + (x[1] - g.org[1]) * g.inv[3 + q] + (x[2] - g.org[2]) * g.inv[6 + q];
+}
+
+__global__ void k_bin(int natoms, int nframes, const int64_t* __restrict__ frame_off, const double* __restrict__ pos,
+                      const Grid* __restrict__ grid, int* __restrict__ cell_of, int* __restrict__ wrap,
+                      int* __restrict__ cell_cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natoms) return;
+  const int f = frame_of(frame_off, nframes, i);
+  const Grid g = grid[f];
+  double fr[3];
+  frac_of(g, pos + 3 * i, fr);
+  int c[3];
+  for (int q = 0; q < 3; ++q) {
+    const double w = g.per ? floor(fr[q]) : 0.0;
+    wrap[3 * i + q] = (int)w;
+    c[q] = min(g.n[q] - 1, max(0, (int)((fr[q] - w) * g.n[q])));
+  }
+  const int id = (int)frame_off[f] + c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
+  cell_of[i] = id;
+  atomicAdd(cell_cnt + id, 1);
+}
+
+__global__ void k_fill_bins(int natoms, const int* __restrict__ cell_of, const int* __restrict__ cell_start,
+                            int* __restrict__ cell_cur, int* __restrict__ cell_atoms) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natoms) return;
+  const int c = cell_of[i];
+  cell_atoms[cell_start[c] + atomicAdd(cell_cur + c, 1)] = i;
+}
+
+__global__ void k_sort_bins(int ncells, const int* __restrict__ cell_start, int* __restrict__ cell_atoms) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int b = cell_start[c], e = cell_start[c + 1];
+  for (int k = b + 1; k < e; ++k) {
+    const int v = cell_atoms[k];
+    int p = k - 1;
+    while (p >= b && cell_atoms[p] > v) {
+      cell_atoms[p + 1] = cell_atoms[p];
+      --p;
+    }
+    cell_atoms[p + 1] = v;
+  }
+}
+
+__global__ void k_zero_int(int n, int* __restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+// Count pass when rec is null, fill pass otherwise.
+template <class T>
+__global__ void k_neighbors(int natoms, int nframes, const int64_t* __restrict__ frame_off,
+                            const double* __restrict__ pos, const double* __restrict__ cell,
+                            const Grid* __restrict__ grid, const int* __restrict__ cell_of,
+                            const int* __restrict__ wrap, const int* __restrict__ cell_start,
+                            const int* __restrict__ cell_atoms, double rc, const int* __restrict__ off,
+                            int* __restrict__ cnt, Rec<T>* __restrict__ rec) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= natoms) return;
+  const int f = frame_of(frame_off, nframes, i);
+  const Grid g = grid[f];
+  const double* a = cell + 9 * f;
+  const int cbase = (int)frame_off[f];
+  int ci = cell_of[i] - cbase;
+  const int c0 = ci % g.n[0];
+  ci /= g.n[0];
+  const int c1 = ci % g.n[1], c2 = ci / g.n[1];
   const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+  const int wi0 = wrap[3 * i], wi1 = wrap[3 * i + 1], wi2 = wrap[3 * i + 2];
   const double rc2 = rc * rc;
   int count = 0;
   const int w0 = rec ? off[i] : 0;
-  for (long jj = 0; jj < n; ++jj) {
-    const long j = base + jj;
-    const double d0 = pos[3 * j] - xi, d1 = pos[3 * j + 1] - yi, d2 = pos[3 * j + 2] - zi;
-    int lo0 = 0, lo1 = 0, lo2 = 0, hi0 = 0, hi1 = 0, hi2 = 0;
-    if (per) {
-      const double f0 = d0 * inv[0][0] + d1 * inv[1][0] + d2 * inv[2][0];
-      const double f1 = d0 * inv[0][1] + d1 * inv[1][1] + d2 * inv[2][1];
-      const double f2 = d0 * inv[0][2] + d1 * inv[1][2] + d2 * inv[2][2];
-      lo0 = (int)ceil(-f0 - reach[0]); hi0 = (int)floor(-f0 + reach[0]);
-      lo1 = (int)ceil(-f1 - reach[1]); hi1 = (int)floor(-f1 + reach[1]);
-      lo2 = (int)ceil(-f2 - reach[2]); hi2 = (int)floor(-f2 + reach[2]);
-    }
-    for (int p = lo0; p <= hi0; ++p)
-      for (int q = lo1; q <= hi1; ++q)
-        for (int s = lo2; s <= hi2; ++s) {
-          if (j == i && p == 0 && q == 0 && s == 0) continue;
-          const double e0 = d0 + p * a[0] + q * a[3] + s * a[6];
-          const double e1 = d1 + p * a[1] + q * a[4] + s * a[7];
-          const double e2 = d2 + p * a[2] + q * a[5] + s * a[8];
-          if (e0 * e0 + e1 * e1 + e2 * e2 < rc2) {
-            if (rec) rec[w0 + count] = Rec<T>{T(e0), T(e1), T(e2), rec_jbits((int)j, T(0))};
+  for (int o2 = -g.S[2]; o2 <= g.S[2]; ++o2) {
+    int z = c2 + o2, s2 = 0;
+    if (g.per) { s2 = (int)floor((double)z / g.n[2]); z -= s2 * g.n[2]; }
+    else if (z < 0 || z >= g.n[2]) continue;
+    for (int o1 = -g.S[1]; o1 <= g.S[1]; ++o1) {
+      int y = c1 + o1, s1 = 0;
+      if (g.per) { s1 = (int)floor((double)y / g.n[1]); y -= s1 * g.n[1]; }
+      else if (y < 0 || y >= g.n[1]) continue;
+      for (int o0 = -g.S[0]; o0 <= g.S[0]; ++o0) {
+        int x = c0 + o0, s0 = 0;
+        if (g.per) { s0 = (int)floor((double)x / g.n[0]); x -= s0 * g.n[0]; }
+        else if (x < 0 || x >= g.n[0]) continue;
+        const int c = cbase + x + g.n[0] * (y + g.n[1] * z);
+        const int b = cell_start[c], e = cell_start[c + 1];
+        for (int k = b; k < e; ++k) {
+          const int j = cell_atoms[k];
+          double d0 = pos[3 * j] - xi, d1 = pos[3 * j + 1] - yi, d2 = pos[3 * j + 2] - zi;
+          if (g.per) {
+            const int k0 = s0 + wi0 - wrap[3 * j], k1 = s1 + wi1 - wrap[3 * j + 1], k2 = s2 + wi2 - wrap[3 * j + 2];
+            if (j == i && k0 == 0 && k1 == 0 && k2 == 0) continue;
+            d0 = d0 + k0 * a[0] + k1 * a[3] + k2 * a[6];
+            d1 = d1 + k0 * a[1] + k1 * a[4] + k2 * a[7];
+            d2 = d2 + k0 * a[2] + k1 * a[5] + k2 * a[8];
+          } else if (j == i) {
+            continue;
+          }
+          if (d0 * d0 + d1 * d1 + d2 * d2 < rc2) {
+            if (rec) rec[w0 + count] = Rec<T>{T(d0), T(d1), T(d2), rec_jbits(j, T(0))};
             ++count;
           }
         }
+      }
+    }
   }
   if (!rec) cnt[i] = count;
 }
@@ -148,18 +273,44 @@ __global__ void k_atom_out(int natoms, const T* __restrict__ site, const T* __re
 // ------------------------------------------------------------------ driver
 template <class T>
 void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
-  ws.ensure_atoms(b.natoms);
-  const int nb = (b.natoms + 255) / 256;
-  SUS2_LAUNCH(ws, st, std::string("neighbors_count<") + tname<T>() + ">", k_neighbors<T>,
-              dim3(nb), dim3(256), 0, b.natoms, b.nframes, b.frame_off, b.pos, b.cell, b.periodic, rc, nullptr,
-              ws.nbr_cnt.get<int>(), nullptr);
-  const long total = exclusive_scan(ws, b.natoms, st);
+  const int N = b.natoms;
+  ws.ensure_atoms(N);
+  const size_t n1 = std::max(1, N);
+  ws.grid.ensure(sizeof(Grid) * std::max(1, b.nframes));
+  ws.cell_of.ensure(sizeof(int) * n1);
+  ws.wrap.ensure(sizeof(int) * 3 * n1);
+  ws.cell_cnt.ensure(sizeof(int) * n1);
+  ws.cell_start.ensure(sizeof(int) * (n1 + 1));
+  ws.cell_cur.ensure(sizeof(int) * n1);
+  ws.cell_atoms.ensure(sizeof(int) * n1);
+  ws.scan_tmp.ensure(2 * sizeof(int64_t));
+  const int nb = (N + 255) / 256;
+  const std::string tn = std::string("<") + tname<T>() + ">";
+  SUS2_LAUNCH(ws, st, "grid_setup", k_grid_setup, dim3((b.nframes + 63) / 64), dim3(64), 0, b.nframes, b.frame_off,
+              b.pos, b.cell, b.periodic, rc, ws.grid.get<Grid>());
+  if (N > 0) {
+    SUS2_LAUNCH(ws, st, "zero_bins", k_zero_int, dim3(nb), dim3(256), 0, N, ws.cell_cnt.get<int>());
+    SUS2_LAUNCH(ws, st, "zero_bins", k_zero_int, dim3(nb), dim3(256), 0, N, ws.cell_cur.get<int>());
+    SUS2_LAUNCH(ws, st, "bin_atoms", k_bin, dim3(nb), dim3(256), 0, N, b.nframes, b.frame_off, b.pos,
+                ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_cnt.get<int>());
+    device_scan(ws.cell_cnt.get<int>(), ws.cell_start.get<int>(), N, ws.scan_tmp.get<int64_t>() + 1, st);
+    SUS2_LAUNCH(ws, st, "fill_bins", k_fill_bins, dim3(nb), dim3(256), 0, N, ws.cell_of.get<int>(),
+                ws.cell_start.get<int>(), ws.cell_cur.get<int>(), ws.cell_atoms.get<int>());
+    SUS2_LAUNCH(ws, st, "sort_bins", k_sort_bins, dim3(nb), dim3(256), 0, N, ws.cell_start.get<int>(),
+                ws.cell_atoms.get<int>());
+    SUS2_LAUNCH(ws, st, "neighbors_count" + tn, k_neighbors<T>, dim3(nb), dim3(256), 0, N, b.nframes, b.frame_off,
+                b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
+                ws.cell_start.get<int>(), ws.cell_atoms.get<int>(), rc, nullptr, ws.nbr_cnt.get<int>(), nullptr);
+  }
+  const long total = exclusive_scan(ws, N, st);
   if (total > INT32_MAX - 1) throw Error(SUS2_E_RANGE, "neighbour list exceeds 2^31 edges");
   ws.nedges = total;
   ws.rec.ensure(sizeof(Rec<T>) * std::max<long>(total, 1));
-  SUS2_LAUNCH(ws, st, std::string("neighbors_fill<") + tname<T>() + ">", k_neighbors<T>,
-              dim3(nb), dim3(256), 0, b.natoms, b.nframes, b.frame_off, b.pos, b.cell, b.periodic, rc,
-              ws.nbr_off.get<int>(), nullptr, ws.rec.get<Rec<T>>());
+  if (N > 0)
+    SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(nb), dim3(256), 0, N, b.nframes, b.frame_off,
+                b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
+                ws.cell_start.get<int>(), ws.cell_atoms.get<int>(), rc, ws.nbr_off.get<int>(), nullptr,
+                ws.rec.get<Rec<T>>());
 }
 
 template <class T>

@@ -1,6 +1,13 @@
-// Shape-templated EFS kernels (moments, forces) and their per-shape driver;
-// instantiated per (L, K) in sus2_efs_l*k*.hip so the shapes compile in
-// parallel.
+// Shape-templated EFS kernels and their per-shape driver; instantiated per
+// (L, K) in sus2_efs_l*k*.hip so the shapes compile in parallel.
+//
+// Two kinds of kernels:
+//   edge kernels   one 64-lane workgroup (one wavefront) per centre atom,
+//                  lanes over that atom's neighbour edges;
+//   graph kernels  one lane per atom: the K->rank contraction and the product
+//                  graph (same schedule for every atom, so schedule reads are
+//                  wave-uniform scalar loads) over node-major arrays
+//                  V[node][atom] that are coalesced across the wavefront.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -12,33 +19,32 @@
 
 namespace sus2 {
 
-// ------------------------------------------------------------------ moments
 template <class T, int LL, int KK>
 struct Shape {
   static constexpr int L1 = LL + 1, H = L1 * L1, F = KK * L1, KH = KK * H, NM = (KH + WAVE - 1) / WAVE;
-  static constexpr int LDSR = WAVE + 1;  // padded staging row
+  static constexpr int LR = WAVE + 1;  // padded LDS staging row
 };
 
-template <class T, int LL, int KK>
-size_t moments_lds(int M) {
-  using S = Shape<T, LL, KK>;
-  return sizeof(T) * ((S::F + S::H) * S::LDSR + S::KH + 2 * (size_t)M);
+__device__ __forceinline__ int l_of(int h) {
+  int l = 0;
+  while ((l + 1) * (l + 1) <= h) ++l;
+  return l;
 }
 
+// ------------------------------------------------------------------ edge: moments
+// Basic moments of one pass for centre atom i (workgroup i):
+//   mom[(h,k)][i] = sum_j w_mu(t_i, t_j, h_j) R_mu(r_ij) Y_h(d_ij),  mu = (k, l(h)).
+// Edge phase: lane = edge, stages w R and Y in LDS; moment phase: lane =
+// moment, sums the staged edges in edge order.
 template <class T, int LL, int KK, bool GATE>
-__global__ __launch_bounds__(WAVE) void k_moments(DModel<T> m, const int32_t* __restrict__ types,
-                                                  const int* __restrict__ nbr_off, const Rec<T>* __restrict__ rec,
-                                                  const T* __restrict__ h, T* __restrict__ h_out,
-                                                  T* __restrict__ gbasic_out, T* __restrict__ site_out,
-                                                  T* __restrict__ abar_out) {
+__global__ __launch_bounds__(WAVE) void k_edge_moments(DModel<T> m, int N, const int32_t* __restrict__ types,
+                                                       const int* __restrict__ nbr_off,
+                                                       const Rec<T>* __restrict__ rec, const T* __restrict__ h,
+                                                       T* __restrict__ mom) {
   using S = Shape<T, LL, KK>;
-  constexpr int L1 = S::L1, H = S::H, F = S::F, KH = S::KH, NM = S::NM, LR = S::LDSR;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sR = reinterpret_cast<T*>(smem_raw);  // [F][LR]
-  T* sY = sR + F * LR;                     // [H][LR]
-  T* mom = sY + H * LR;                    // [KH]
-  T* v = mom + KH;                         // [M]
-  T* a = v + m.M;                          // [M]
+  constexpr int L1 = S::L1, H = S::H, F = S::F, KH = S::KH, NM = S::NM, LR = S::LR;
+  __shared__ T sR[F * LR];
+  __shared__ T sY[H * LR];
   const int i = blockIdx.x, lane = threadIdx.x;
   const int ti = types[i];
   const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
@@ -77,9 +83,7 @@ __global__ __launch_bounds__(WAVE) void k_moments(DModel<T> m, const int32_t* __
       const int c = lane + WAVE * t;
       if (c < KH) {
         const int hh = c / KK, k = c % KK;
-        int l = 0;
-        while ((l + 1) * (l + 1) <= hh) ++l;
-        const T* pr = sR + (k * L1 + l) * LR;
+        const T* pr = sR + (k * L1 + l_of(hh)) * LR;
         const T* py = sY + hh * LR;
         T s = acc[t];
         for (int ee = 0; ee < cnt; ++ee) s += pr[ee] * py[ee];
@@ -91,66 +95,190 @@ __global__ __launch_bounds__(WAVE) void k_moments(DModel<T> m, const int32_t* __
 #pragma unroll
   for (int t = 0; t < NM; ++t) {
     const int c = lane + WAVE * t;
-    if (c < KH) mom[c] = acc[t];
+    if (c < KH) mom[(size_t)c * N + i] = acc[t];
   }
-  __syncthreads();
-  load_and_contract(m, GATE, mom, v);
-  graph_forward(GATE ? m.gp : m.op, v);
-  if (GATE) {
-    for (int mu = lane; mu < F; mu += WAVE) {
-      T s = T(0);
-      for (int qq = 0; qq < m.Qg; ++qq) s += m.W(mu, qq) * v[m.gate_nodes[qq]];
-      h_out[(size_t)i * F + mu] = s;
-    }
-    for (int c = lane; c < KH; c += WAVE) gbasic_out[(size_t)i * KH + c] = mom[c];
-    return;
-  }
-  T part = T(0);
-  for (int qq = lane; qq < m.Q; qq += WAVE) part += m.readout(ti, qq) * v[m.scalar_map[qq]];
-  const T E = wave_sum(part) + m.site0(ti);
-  if (lane == 0) site_out[i] = E;
-  for (int n = lane; n < m.M; n += WAVE) a[n] = T(0);
-  __syncthreads();
-  if (lane == 0)
-    for (int qq = 0; qq < m.Q; ++qq) a[m.scalar_map[qq]] += m.readout(ti, qq);
-  __syncthreads();
-  graph_reverse(m.op, v, a);
-  adjoint_to_basic(m, false, a, abar_out + (size_t)i * KH);
 }
 
-// ------------------------------------------------------------------ forces
+// ------------------------------------------------------------------ graph: forward
+// lane = atom.  Node values of the pass live in V[(node - lo)][N].
 template <class T, int LL, int KK>
-size_t force_lds(int M) {
+__device__ __forceinline__ void graph_forward_lane(const DModel<T>& m, const DPass<T>& p, bool gate, int N, int i,
+                                                   const T* __restrict__ mom, T* __restrict__ V) {
   using S = Shape<T, LL, KK>;
-  return sizeof(T) * (2 * S::KH + S::F + 2 * (size_t)M);
+  constexpr int H = S::H;
+  const int lo = p.node_lo, cnt = p.node_cnt;
+  // basic nodes that the pass reads directly (unranked graphs)
+  for (int c = 0; c < KK * H; ++c) {
+    const int node = m.basic_node[c];
+    if (node >= lo && node < lo + cnt) V[(size_t)(node - lo) * N + i] = mom[(size_t)c * N + i];
+  }
+  // K -> rank contraction
+  for (int a = 0; a < p.rank; ++a)
+    for (int hh = 0; hh < H; ++hh) {
+      const int l = l_of(hh), mm = hh - l * l - l;
+      T s = T(0);
+#pragma unroll
+      for (int k = 0; k < KK; ++k)
+        if (m.basic_node[hh * KK + k] >= 0) s += m.rankw(gate, l, a, k) * mom[(size_t)(hh * KK + k) * N + i];
+      V[(size_t)(rank_node(p.begin, p.rank, H, l, mm, a) - lo) * N + i] = s;
+    }
+  // products in layer order: out = sum_rows c v[a] v[b]
+  for (int o = 0; o < p.nfout; ++o) {
+    T s = T(0);
+    const int r1 = p.f_roff[o + 1];
+    for (int r = p.f_roff[o]; r < r1; ++r)
+      s += p.f_c[r] * V[(size_t)(p.f_a[r] - lo) * N + i] * V[(size_t)(p.f_b[r] - lo) * N + i];
+    V[(size_t)(p.f_out[o] - lo) * N + i] = s;
+  }
 }
 
-// adj: basic adjoints of this pass for every atom ([N][H][K]); fw: per-atom
-// force (3) and virial (6) partials.  The outer pass writes them and, for a
-// gated model, gathers h_bar_i and leaves the gate basic adjoints in gbar_out;
-// the gate pass adds its part.
-template <class T, int LL, int KK, bool GATE>
-__global__ __launch_bounds__(WAVE) void k_force(DModel<T> m, const int32_t* __restrict__ types,
-                                                const int* __restrict__ nbr_off, const Rec<T>* __restrict__ rec,
-                                                const T* __restrict__ h, const T* __restrict__ adj,
-                                                const T* __restrict__ gbasic, T* __restrict__ gbar_out,
-                                                T* __restrict__ fw) {
+// Reverse: A[n] = seed[n] + sum over consumer rows c A[out] V[partner], nodes in
+// reverse-layer order, then the transposed contraction to canonical basic
+// adjoints abar[i][(h,k)] (atom-major, read per neighbour by the edge kernels).
+template <class T, int LL, int KK>
+__device__ __forceinline__ void graph_reverse_lane(const DModel<T>& m, const DPass<T>& p, bool gate, int N, int i,
+                                                   const T* __restrict__ V, T* __restrict__ A,
+                                                   T* __restrict__ abar) {
   using S = Shape<T, LL, KK>;
-  constexpr int L1 = S::L1, H = S::H, F = S::F, KH = S::KH;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sA = reinterpret_cast<T*>(smem_raw);  // [KH]
-  T* mom = sA + KH;                        // [KH]
-  T* x3i = mom + KH;                       // [F]
-  T* v = x3i + F;                          // [M]
-  T* a = v + m.M;                          // [M]
+  constexpr int H = S::H, KH = S::KH;
+  const int lo = p.node_lo, cnt = p.node_cnt;
+  for (int k = 0; k < p.nrnode; ++k) {
+    const int n = p.r_node[k];
+    T s = A[(size_t)(n - lo) * N + i];
+    const int r1 = p.r_roff[k + 1];
+    for (int r = p.r_roff[k]; r < r1; ++r)
+      s += p.r_c[r] * A[(size_t)(p.r_out[r] - lo) * N + i] * V[(size_t)(p.r_partner[r] - lo) * N + i];
+    A[(size_t)(n - lo) * N + i] = s;
+  }
+  T* out = abar + (size_t)i * KH;
+  for (int hh = 0; hh < H; ++hh) {
+    const int l = l_of(hh), mm = hh - l * l - l;
+#pragma unroll
+    for (int k = 0; k < KK; ++k) {
+      const int node = m.basic_node[hh * KK + k];
+      T s = T(0);
+      if (node >= 0) {
+        if (node >= lo && node < lo + cnt) s = A[(size_t)(node - lo) * N + i];
+        for (int a = 0; a < p.rank; ++a)
+          s += m.rankw(gate, l, a, k) * A[(size_t)(rank_node(p.begin, p.rank, H, l, mm, a) - lo) * N + i];
+      }
+      out[hh * KK + k] = s;
+    }
+  }
+}
+
+// gate forward: V_g, then h_i = W S_i
+template <class T, int LL, int KK>
+__global__ __launch_bounds__(WAVE) void k_graph_gate_fwd(DModel<T> m, int N, const T* __restrict__ mom,
+                                                         T* __restrict__ V, T* __restrict__ h) {
+  const int i = blockIdx.x * WAVE + threadIdx.x;
+  if (i >= N) return;
+  graph_forward_lane<T, LL, KK>(m, m.gp, true, N, i, mom, V);
+  constexpr int F = Shape<T, LL, KK>::F;
+  const int lo = m.gp.node_lo;
+  T acc[F];
+#pragma unroll
+  for (int mu = 0; mu < F; ++mu) acc[mu] = T(0);
+  for (int q = 0; q < m.Qg; ++q) {
+    const T sq = V[(size_t)(m.gate_nodes[q] - lo) * N + i];
+#pragma unroll
+    for (int mu = 0; mu < F; ++mu) acc[mu] += m.W(mu, q) * sq;
+  }
+#pragma unroll
+  for (int mu = 0; mu < F; ++mu) h[(size_t)i * F + mu] = acc[mu];
+}
+
+// outer forward + readout + reverse
+template <class T, int LL, int KK>
+__global__ __launch_bounds__(WAVE) void k_graph_outer(DModel<T> m, int N, const int32_t* __restrict__ types,
+                                                      const T* __restrict__ mom, T* __restrict__ V,
+                                                      T* __restrict__ A, T* __restrict__ site,
+                                                      T* __restrict__ abar) {
+  const int i = blockIdx.x * WAVE + threadIdx.x;
+  if (i >= N) return;
+  const DPass<T>& p = m.op;
+  graph_forward_lane<T, LL, KK>(m, p, false, N, i, mom, V);
+  const int ti = types[i];
+  const int lo = p.node_lo;
+  T E = m.site0(ti);
+  for (int n = 0; n < p.node_cnt; ++n) A[(size_t)n * N + i] = T(0);
+  for (int q = 0; q < m.Q; ++q) {
+    const int node = m.scalar_map[q] - lo;
+    const T w = m.readout(ti, q);
+    E += w * V[(size_t)node * N + i];
+    A[(size_t)node * N + i] += w;
+  }
+  site[i] = E;
+  graph_reverse_lane<T, LL, KK>(m, p, false, N, i, V, A, abar);
+}
+
+// gate reverse: seeds S_bar = W^T h_bar_i, graph reverse over the stored V_g
+template <class T, int LL, int KK>
+__global__ __launch_bounds__(WAVE) void k_graph_gate_rev(DModel<T> m, int N, const T* __restrict__ hbar,
+                                                         const T* __restrict__ V, T* __restrict__ A,
+                                                         T* __restrict__ gbar) {
+  const int i = blockIdx.x * WAVE + threadIdx.x;
+  if (i >= N) return;
+  constexpr int F = Shape<T, LL, KK>::F;
+  const DPass<T>& p = m.gp;
+  const int lo = p.node_lo;
+  T hb[F];
+#pragma unroll
+  for (int mu = 0; mu < F; ++mu) hb[mu] = hbar[(size_t)i * F + mu];
+  for (int n = 0; n < p.node_cnt; ++n) A[(size_t)n * N + i] = T(0);
+  for (int q = 0; q < m.Qg; ++q) {
+    T s = T(0);
+#pragma unroll
+    for (int mu = 0; mu < F; ++mu) s += hb[mu] * m.W(mu, q);
+    A[(size_t)(m.gate_nodes[q] - lo) * N + i] += s;
+  }
+  graph_reverse_lane<T, LL, KK>(m, p, true, N, i, V, A, gbar);
+}
+
+// ------------------------------------------------------------------ edge: forces
+// One directed edge d (r, u = d/r) against basic adjoints `a` (KH, (h,k)
+// order, scaled by `par` = (-1)^l when the edge is evaluated reversed):
+//   sY = sum a dR Y,  g = sum a R grad Y,  and optionally hb_mu += c_mu sum_m a Y.
+template <class T, int LL, int KK, bool REV, bool HB, class Adj>
+__device__ __forceinline__ void edge_reverse(T ux, T uy, T uz, T inv, const T* Rw, const T* dRw, const T* cr,
+                                             Adj&& adj, T& sY, T& gx, T& gy, T& gz, T* hb) {
+  constexpr int L1 = LL + 1;
+  sh_stream_grad<T, LL>(ux, uy, uz, inv, [&](int l, int mm, T Y, T yx, T yy, T yz) {
+    const int hh = hidx(l, mm);
+    const T par = (REV && (l & 1)) ? T(-1) : T(1);
+#pragma unroll
+    for (int k = 0; k < KK; ++k) {
+      const int mu = k * L1 + l;
+      const T a = adj(hh * KK + k) * par;
+      sY += a * dRw[mu] * Y;
+      const T g = a * Rw[mu];
+      gx += g * yx; gy += g * yy; gz += g * yz;
+      if (HB) hb[mu] += cr[mu] * a * Y;
+    }
+  });
+}
+
+// Gather-based backward of one pass for centre i: for every edge i -> j,
+//   g_own = dE_i/dd_ij (centre adjoints), g_rev = dE_j/dd_ji (neighbour's),
+//   F_i = sum (g_own - g_rev),  W_i = -sum d (x) g_own,
+// and for the outer pass of an Effect model the gate adjoint
+//   h_bar_i,mu = alpha_{t_i,mu} sum_j tau_{t_j,mu} R_mu(r; t_j,t_i) sum_m a_j Y(d_ji).
+template <class T, int LL, int KK, bool GATE>
+__global__ __launch_bounds__(WAVE) void k_edge_force(DModel<T> m, const int32_t* __restrict__ types,
+                                                     const int* __restrict__ nbr_off,
+                                                     const Rec<T>* __restrict__ rec, const T* __restrict__ h,
+                                                     const T* __restrict__ adj, T* __restrict__ fw,
+                                                     T* __restrict__ hbar) {
+  using S = Shape<T, LL, KK>;
+  constexpr int F = S::F, KH = S::KH;
+  __shared__ T sA[KH];
+  __shared__ T x3i[F];
   const int i = blockIdx.x, lane = threadIdx.x;
   const int ti = types[i];
   const bool hgate = !GATE && m.gated;
   for (int c = lane; c < KH; c += WAVE) sA[c] = adj[(size_t)i * KH + c];
-  for (int mu = lane; mu < F; mu += WAVE) {
-    if (GATE) x3i[mu] = m.gtype(ti, mu);
-    else x3i[mu] = m.tau(ti, mu) + (m.gated ? m.alpha(ti, mu) * h[(size_t)i * F + mu] : T(0));
-  }
+  for (int mu = lane; mu < F; mu += WAVE)
+    x3i[mu] = GATE ? m.gtype(ti, mu) : m.tau(ti, mu) + (m.gated ? m.alpha(ti, mu) * h[(size_t)i * F + mu] : T(0));
   __syncthreads();
   T fx = T(0), fy = T(0), fz = T(0);
   T w0 = T(0), w1 = T(0), w2 = T(0), w3 = T(0), w4 = T(0), w5 = T(0);
@@ -166,51 +294,47 @@ __global__ __launch_bounds__(WAVE) void k_force(DModel<T> m, const int32_t* __re
     const T inv = T(1) / r;
     const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
     const int tj = types[j];
-    T Ro[F], dRo[F], Rr[F], dRr[F], cr[F];
-    radial<T, LL, KK, true>(m, GATE, ti * m.C + tj, r, Ro, dRo);
-    if (ti == tj) {
-#pragma unroll
-      for (int mu = 0; mu < F; ++mu) { Rr[mu] = Ro[mu]; dRr[mu] = dRo[mu]; }
-    } else {
-      radial<T, LL, KK, true>(m, GATE, tj * m.C + ti, r, Rr, dRr);
-    }
+    T R[F], dR[F], cr[F];
+    // ---- own direction: E_i through d_ij, pair (t_i, t_j)
+    radial<T, LL, KK, true>(m, GATE, ti * m.C + tj, r, R, dR);
 #pragma unroll
     for (int mu = 0; mu < F; ++mu) {
-      T wo, wr;
-      if (GATE) { wo = x3i[mu] * m.gtype(tj, mu); wr = wo; }
+      T w;
+      if (GATE) w = x3i[mu] * m.gtype(tj, mu);
+      else {
+        T x3j = m.tau(tj, mu);
+        if (m.gated) x3j += m.alpha(tj, mu) * h[(size_t)j * F + mu];
+        w = m.tau(ti, mu) * x3j;
+      }
+      R[mu] *= w; dR[mu] *= w;
+    }
+    T sY = T(0), gx = T(0), gy = T(0), gz = T(0);
+    edge_reverse<T, LL, KK, false, false>(ux, uy, uz, inv, R, dR, cr, [&](int c) { return sA[c]; }, sY, gx, gy, gz,
+                                          hb);
+    const T gx0 = ux * sY + gx, gy0 = uy * sY + gy, gz0 = uz * sY + gz;
+    // ---- reverse direction: E_j through d_ji = -d_ij, pair (t_j, t_i)
+    radial<T, LL, KK, true>(m, GATE, tj * m.C + ti, r, R, dR);
+#pragma unroll
+    for (int mu = 0; mu < F; ++mu) {
+      T w;
+      if (GATE) w = x3i[mu] * m.gtype(tj, mu);
       else {
         const T tauj = m.tau(tj, mu);
-        T x3j = tauj;
-        if (m.gated) x3j += m.alpha(tj, mu) * h[(size_t)j * F + mu];
-        wo = m.tau(ti, mu) * x3j;
-        wr = tauj * x3i[mu];
-        cr[mu] = hgate ? tauj * m.alpha(ti, mu) * Rr[mu] : T(0);
+        w = tauj * x3i[mu];
+        if (hgate) cr[mu] = tauj * m.alpha(ti, mu) * R[mu];
       }
-      Ro[mu] *= wo; dRo[mu] *= wo; Rr[mu] *= wr; dRr[mu] *= wr;
+      R[mu] *= w; dR[mu] *= w;
     }
     const T* aj = adj + (size_t)j * KH;
-    T sYo = T(0), gox = T(0), goy = T(0), goz = T(0);
-    T sYr = T(0), grx = T(0), gry = T(0), grz = T(0);
-    sh_stream_grad<T, LL>(ux, uy, uz, inv, [&](int l, int mm, T Y, T gx, T gy, T gz) {
-      const int hh = hidx(l, mm);
-      const T par = (l & 1) ? T(-1) : T(1);
-#pragma unroll
-      for (int k = 0; k < KK; ++k) {
-        const int mu = k * L1 + l;
-        const T ao = sA[hh * KK + k];
-        const T ar = aj[hh * KK + k] * par;
-        sYo += ao * dRo[mu] * Y;
-        const T go = ao * Ro[mu];
-        gox += go * gx; goy += go * gy; goz += go * gz;
-        sYr += ar * dRr[mu] * Y;
-        const T gr = ar * Rr[mu];
-        grx -= gr * gx; gry -= gr * gy; grz -= gr * gz;
-        if (hgate) hb[mu] += cr[mu] * ar * Y;
-      }
-    });
-    // g_own = dE_i/dd_ij ; g_rev = dE_j/dd_ji with d_ji = -d_ij
-    const T gx0 = ux * sYo + gox, gy0 = uy * sYo + goy, gz0 = uz * sYo + goz;
-    const T gx1 = -ux * sYr + grx, gy1 = -uy * sYr + gry, gz1 = -uz * sYr + grz;
+    sY = gx = gy = gz = T(0);
+    if (hgate)
+      edge_reverse<T, LL, KK, true, true>(ux, uy, uz, inv, R, dR, cr, [&](int c) { return aj[c]; }, sY, gx, gy, gz,
+                                          hb);
+    else
+      edge_reverse<T, LL, KK, true, false>(ux, uy, uz, inv, R, dR, cr, [&](int c) { return aj[c]; }, sY, gx, gy,
+                                           gz, hb);
+    // g_rev = dE_j/dd_ji: d(r)/d(d_ji) = -u, grad Y(-d) = -(-1)^l grad Y(d)
+    const T gx1 = -ux * sY - gx, gy1 = -uy * sY - gy, gz1 = -uz * sY - gz;
     fx += gx0 - gx1; fy += gy0 - gy1; fz += gz0 - gz1;
     w0 -= dx * gx0; w1 -= dy * gy0; w2 -= dz * gz0;
     w3 -= dx * gy0; w4 -= dx * gz0; w5 -= dy * gz0;
@@ -230,25 +354,13 @@ __global__ __launch_bounds__(WAVE) void k_force(DModel<T> m, const int32_t* __re
   }
   if (!hgate) return;
 #pragma unroll
-  for (int mu = 0; mu < F; ++mu) hb[mu] = wave_sum(hb[mu]);
-  // gate reverse of atom i: node values from its stored gate moments
-  for (int c = lane; c < KH; c += WAVE) mom[c] = gbasic[(size_t)i * KH + c];
-  __syncthreads();
-  load_and_contract(m, true, mom, v);
-  graph_forward(m.gp, v);
-  for (int n = lane; n < m.M; n += WAVE) a[n] = T(0);
-  __syncthreads();
-  for (int qq = lane; qq < m.Qg; qq += WAVE) {
-    T s = T(0);
-#pragma unroll
-    for (int mu = 0; mu < F; ++mu) s += hb[mu] * m.W(mu, qq);
-    a[m.gate_nodes[qq]] = s;
+  for (int mu = 0; mu < F; ++mu) {
+    const T s = wave_sum(hb[mu]);
+    if (lane == 0) hbar[(size_t)i * F + mu] = s;
   }
-  __syncthreads();
-  graph_reverse(m.gp, v, a);
-  adjoint_to_basic(m, true, a, gbar_out + (size_t)i * KH);
 }
 
+// ------------------------------------------------------------------ driver
 template <class T, int LL, int KK>
 std::string kname(const char* base) {
   return std::string(base) + "<" + tname<T>() + ",L" + std::to_string(LL) + "K" + std::to_string(KK) + ">";
@@ -260,27 +372,40 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
   const DModel<T>& m = dm.get<T>();
   const int N = b.natoms;
   build_neighbors<T>(ws, b, dm.host->rc, st);
-  ws.h.ensure(sizeof(T) * (size_t)std::max(1, N) * S::F);
-  ws.gbasic.ensure(sizeof(T) * (size_t)std::max(1, N) * S::KH);
-  ws.abar.ensure(sizeof(T) * (size_t)std::max(1, N) * S::KH);
-  ws.gbar.ensure(sizeof(T) * (size_t)std::max(1, N) * S::KH);
-  ws.site.ensure(sizeof(T) * (size_t)std::max(1, N));
-  ws.fw.ensure(sizeof(T) * (size_t)std::max(1, N) * 9);
+  const size_t n1 = std::max(1, N);
+  ws.h.ensure(sizeof(T) * n1 * S::F);
+  ws.hbar.ensure(sizeof(T) * n1 * S::F);
+  ws.mom.ensure(sizeof(T) * n1 * S::KH);
+  ws.abar.ensure(sizeof(T) * n1 * S::KH);
+  ws.gbar.ensure(sizeof(T) * n1 * S::KH);
+  ws.vg.ensure(sizeof(T) * n1 * std::max(1, m.gp.node_cnt));
+  ws.vo.ensure(sizeof(T) * n1 * std::max(1, m.op.node_cnt));
+  ws.ag.ensure(sizeof(T) * n1 * std::max(1, std::max(m.gp.node_cnt, m.op.node_cnt)));
+  ws.site.ensure(sizeof(T) * n1);
+  ws.fw.ensure(sizeof(T) * n1 * 9);
   const int* off = ws.nbr_off.get<int>();
   const Rec<T>* rec = ws.rec.get<Rec<T>>();
   T* h = ws.h.get<T>();
+  const int gl = (N + WAVE - 1) / WAVE;
   if (N > 0) {
-    const size_t lm = moments_lds<T, LL, KK>(m.M), lf = force_lds<T, LL, KK>(m.M);
-    if (m.gated)
-      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("moments_gate")), (k_moments<T, LL, KK, true>), dim3(N),
-                  dim3(WAVE), lm, m, b.types, off, rec, h, h, ws.gbasic.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
-    SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("moments_outer")), (k_moments<T, LL, KK, false>), dim3(N),
-                dim3(WAVE), lm, m, b.types, off, rec, h, h, ws.gbasic.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
-    SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("force_outer")), (k_force<T, LL, KK, false>), dim3(N), dim3(WAVE),
-                lf, m, b.types, off, rec, h, ws.abar.get<T>(), ws.gbasic.get<T>(), ws.gbar.get<T>(), ws.fw.get<T>());
-    if (m.gated)
-      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("force_gate")), (k_force<T, LL, KK, true>), dim3(N), dim3(WAVE),
-                  lf, m, b.types, off, rec, h, ws.gbar.get<T>(), ws.gbasic.get<T>(), ws.gbar.get<T>(), ws.fw.get<T>());
+    if (m.gated) {
+      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("edge_moments_gate")), (k_edge_moments<T, LL, KK, true>), dim3(N),
+                  dim3(WAVE), 0, m, N, b.types, off, rec, h, ws.mom.get<T>());
+      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("graph_gate_fwd")), (k_graph_gate_fwd<T, LL, KK>), dim3(gl), dim3(WAVE),
+                  0, m, N, ws.mom.get<T>(), ws.vg.get<T>(), h);
+    }
+    SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("edge_moments_outer")), (k_edge_moments<T, LL, KK, false>), dim3(N),
+                dim3(WAVE), 0, m, N, b.types, off, rec, h, ws.mom.get<T>());
+    SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("graph_outer")), (k_graph_outer<T, LL, KK>), dim3(gl), dim3(WAVE), 0, m, N,
+                b.types, ws.mom.get<T>(), ws.vo.get<T>(), ws.ag.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
+    SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("edge_force_outer")), (k_edge_force<T, LL, KK, false>), dim3(N),
+                dim3(WAVE), 0, m, b.types, off, rec, h, ws.abar.get<T>(), ws.fw.get<T>(), ws.hbar.get<T>());
+    if (m.gated) {
+      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("graph_gate_rev")), (k_graph_gate_rev<T, LL, KK>), dim3(gl), dim3(WAVE),
+                  0, m, N, ws.hbar.get<T>(), ws.vg.get<T>(), ws.ag.get<T>(), ws.gbar.get<T>());
+      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("edge_force_gate")), (k_edge_force<T, LL, KK, true>), dim3(N),
+                  dim3(WAVE), 0, m, b.types, off, rec, h, ws.gbar.get<T>(), ws.fw.get<T>(), ws.hbar.get<T>());
+    }
   }
   finish_efs<T>(ws, b, out, st);
 }

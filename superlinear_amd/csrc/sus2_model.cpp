@@ -86,6 +86,8 @@ struct Fields {
 // every accumulation is deterministic.
 PassPlan make_plan(const Model& m, bool gate) {
   PassPlan p;
+  int lo = m.M, hi = -1;
+  auto touch = [&](int n) { lo = std::min(lo, n); hi = std::max(hi, n); };
   p.prod_lo = gate ? m.gate_pb : m.outer_pb;
   p.prod_hi = gate ? m.gate_pe : (int)m.prod.size();
   p.rank = gate ? m.R : m.O;
@@ -96,7 +98,13 @@ PassPlan make_plan(const Model& m, bool gate) {
     const auto& r = m.prod[q];
     layer[r.out] = std::max(layer[r.out], 1 + std::max(layer[r.a], layer[r.b]));
     is_out[r.out] = 1;
+    touch(r.a); touch(r.b); touch(r.out);
   }
+  if (p.rank > 0) { touch(p.begin); touch(p.begin + p.rank * m.H - 1); }
+  for (int n : gate ? m.gate_nodes : m.scalar_map) touch(n);
+  if (hi < lo) { lo = 0; hi = 0; }
+  p.node_lo = lo;
+  p.node_cnt = hi - lo + 1;
   int nl = 0;
   for (int n = 0; n < m.M; ++n) if (is_out[n]) nl = std::max(nl, layer[n]);
   std::vector<std::vector<int>> rows_of(m.M);

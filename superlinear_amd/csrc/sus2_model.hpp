@@ -28,6 +28,9 @@ struct PassPlan {
   std::vector<double> rv_c;
   // contraction: rank, first rank node
   int rank = 0, begin = 0;
+  // contiguous node range [node_lo, node_lo + node_cnt) touched by the pass
+  // (rank nodes, product inputs/outputs and readout seeds)
+  int node_lo = 0, node_cnt = 0;
 };
 
 struct Model {

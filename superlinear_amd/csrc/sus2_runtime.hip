@@ -123,11 +123,14 @@ __global__ __launch_bounds__(1024) void k_scan(int n, const int* __restrict__ cn
   }
 }
 
-long exclusive_scan(Workspace& ws, int n, hipStream_t st) {
-  ws.scan_tmp.ensure(sizeof(int64_t));
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, n, ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(),
-                     ws.scan_tmp.get<int64_t>());
+void device_scan(const int* cnt, int* off, int n, int64_t* total, hipStream_t st) {
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, n, cnt, off, total);
   RT_CHECK(hipGetLastError());
+}
+
+long exclusive_scan(Workspace& ws, int n, hipStream_t st) {
+  ws.scan_tmp.ensure(2 * sizeof(int64_t));
+  device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), n, ws.scan_tmp.get<int64_t>(), st);
   RT_CHECK(hipMemcpyAsync(ws.host_total, ws.scan_tmp.get<int64_t>(), sizeof(int64_t), hipMemcpyDeviceToHost, st));
   RT_CHECK(hipStreamSynchronize(st));
   return (long)*ws.host_total;
@@ -187,6 +190,10 @@ void fill_view(DevModelT<T>& dv, const Model& m, const int* ibase, const std::ve
     if (dp.nrl < 0) dp.nrl = 0;
     dp.rank = pp.rank;
     dp.begin = pp.begin;
+    dp.node_lo = pp.node_lo;
+    dp.node_cnt = pp.node_cnt;
+    dp.nfout = pp.fw_out.empty() ? 0 : (int)pp.fw_out.size();
+    dp.nrnode = pp.rv_node.empty() ? 0 : (int)pp.rv_node.size();
     dp.fl_off = ibase + io[k++]; dp.f_out = ibase + io[k++]; dp.f_roff = ibase + io[k++];
     dp.f_a = ibase + io[k++]; dp.f_b = ibase + io[k++];
     dp.rl_off = ibase + io[k++]; dp.r_node = ibase + io[k++]; dp.r_roff = ibase + io[k++];

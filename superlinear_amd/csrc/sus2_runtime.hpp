@@ -60,7 +60,9 @@ struct Profiler {
 
 struct Workspace {
   Profiler prof;
-  DevBuf nbr_cnt, nbr_off, scan_tmp, rec, h, gbasic, abar, gbar, site, fw;
+  DevBuf nbr_cnt, nbr_off, scan_tmp, rec, h, hbar, mom, abar, gbar, vg, vo, ag, site, fw;
+  // cell-list neighbour build
+  DevBuf grid, cell_of, wrap, cell_cnt, cell_start, cell_cur, cell_atoms;
   // parameter-gradient scratch
   DevBuf gtan, hdot, ebar, wedge, tgt, gpart, gacc;
   long nedges = 0;
@@ -69,6 +71,8 @@ struct Workspace {
   void ensure_atoms(int n);
 };
 
+// exclusive scan cnt[0..n) -> off[0..n] on the device (total also in *total)
+void device_scan(const int* cnt, int* off, int n, int64_t* total, hipStream_t st);
 // exclusive scan of ws.nbr_cnt[0..n) into ws.nbr_off[0..n]; returns the total
 long exclusive_scan(Workspace& ws, int n, hipStream_t st);
 

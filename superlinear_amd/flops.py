@@ -57,10 +57,10 @@ def survey_efs_per_edge(d):
 
 def kernel_flops(name, d, nedges):
     """Algorithmic flops of one launch of the named kernel over nedges."""
-    if name.startswith("moments_"):
+    if name.startswith("edge_moments_"):
         return moments_per_edge(d) * nedges
-    if name.startswith("force_outer"):
+    if name.startswith("edge_force_outer"):
         return force_per_edge(d, outer_effect=bool(d.get("gated"))) * nedges
-    if name.startswith("force_gate"):
+    if name.startswith("edge_force_gate"):
         return force_per_edge(d) * nedges
     return None
