@@ -16,6 +16,7 @@ template <class T>
 struct DPass {  // layered product-graph schedule of one pass (see PassPlan)
   int nfl = 0, nrl = 0, rank = 0, begin = 0, node_lo = 0, node_cnt = 0;
   int nfout = 0, nrnode = 0;  // total forward outs / reverse nodes
+  int max_fw_rows = 0, max_rv_rows = 0;
   const int *fl_off = nullptr, *f_out = nullptr, *f_roff = nullptr, *f_a = nullptr, *f_b = nullptr;
   const T* f_c = nullptr;
   const int *rl_off = nullptr, *r_node = nullptr, *r_roff = nullptr, *r_out = nullptr, *r_partner = nullptr;

@@ -151,19 +151,41 @@ __global__ void k_fill_bins(int natoms, const int* __restrict__ cell_of, const i
   cell_atoms[cell_start[c] + atomicAdd(cell_cur + c, 1)] = i;
 }
 
-__global__ void k_sort_bins(int ncells, const int* __restrict__ cell_start, int* __restrict__ cell_atoms) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
-  const int b = cell_start[c], e = cell_start[c + 1];
-  for (int k = b + 1; k < e; ++k) {
-    const int v = cell_atoms[k];
-    int p = k - 1;
-    while (p >= b && cell_atoms[p] > v) {
-      cell_atoms[p + 1] = cell_atoms[p];
-      --p;
-    }
-    cell_atoms[p + 1] = v;
+// Sort the ids of one list [b, e) ascending (one wavefront per list; each
+// element's rank = number of smaller ids; ids are distinct).
+__device__ void wave_rank_sort(int* __restrict__ list, int b, int e, int* __restrict__ tmp) {
+  const int lane = threadIdx.x, n = e - b;
+  for (int k = lane; k < n; k += WAVE) tmp[k] = list[b + k];
+  __syncthreads();
+  for (int k = lane; k < n; k += WAVE) {
+    const int v = tmp[k];
+    int rank = 0;
+    for (int q = 0; q < n; ++q) rank += tmp[q] < v;
+    list[b + rank] = v;
   }
+  __syncthreads();
+}
+
+constexpr int SORT_MAX = 2048;
+
+__global__ __launch_bounds__(WAVE) void k_sort_lists(int nlists, const int* __restrict__ start,
+                                                     int* __restrict__ list) {
+  __shared__ int tmp[SORT_MAX];
+  const int c = blockIdx.x;
+  if (c >= nlists) return;
+  const int b = start[c], e = start[c + 1];
+  if (e - b <= 1) return;
+  if (e - b <= SORT_MAX) {
+    wave_rank_sort(list, b, e, tmp);
+    return;
+  }
+  if (threadIdx.x == 0)  // oversized list: serial insertion sort
+    for (int k = b + 1; k < e; ++k) {
+      const int v = list[k];
+      int q = k - 1;
+      while (q >= b && list[q] > v) { list[q + 1] = list[q]; --q; }
+      list[q + 1] = v;
+    }
 }
 
 __global__ void k_zero_int(int n, int* __restrict__ a) {
@@ -171,15 +193,18 @@ __global__ void k_zero_int(int n, int* __restrict__ a) {
   if (i < n) a[i] = 0;
 }
 
-// Count pass when rec is null, fill pass otherwise.
+// One wavefront per atom: lanes test 64 candidates of a bin at a time; hits
+// are compacted in candidate order with a ballot.  Count pass when rec is
+// null, fill pass otherwise.
 template <class T>
-__global__ void k_neighbors(int natoms, int nframes, const int64_t* __restrict__ frame_off,
-                            const double* __restrict__ pos, const double* __restrict__ cell,
-                            const Grid* __restrict__ grid, const int* __restrict__ cell_of,
-                            const int* __restrict__ wrap, const int* __restrict__ cell_start,
-                            const int* __restrict__ cell_atoms, double rc, const int* __restrict__ off,
-                            int* __restrict__ cnt, Rec<T>* __restrict__ rec) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, int nframes, const int64_t* __restrict__ frame_off,
+                                                    const double* __restrict__ pos, const double* __restrict__ cell,
+                                                    const Grid* __restrict__ grid, const int* __restrict__ cell_of,
+                                                    const int* __restrict__ wrap, const int* __restrict__ cell_start,
+                                                    const int* __restrict__ cell_atoms, double rc,
+                                                    const int* __restrict__ off, int* __restrict__ cnt,
+                                                    Rec<T>* __restrict__ rec) {
+  const int i = blockIdx.x, lane = threadIdx.x;
   if (i >= natoms) return;
   const int f = frame_of(frame_off, nframes, i);
   const Grid g = grid[f];
@@ -192,6 +217,7 @@ __global__ void k_neighbors(int natoms, int nframes, const int64_t* __restrict__
   const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
   const int wi0 = wrap[3 * i], wi1 = wrap[3 * i + 1], wi2 = wrap[3 * i + 2];
   const double rc2 = rc * rc;
+  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
   int count = 0;
   const int w0 = rec ? off[i] : 0;
   for (int o2 = -g.S[2]; o2 <= g.S[2]; ++o2) {
@@ -208,27 +234,46 @@ __global__ void k_neighbors(int natoms, int nframes, const int64_t* __restrict__
         else if (x < 0 || x >= g.n[0]) continue;
         const int c = cbase + x + g.n[0] * (y + g.n[1] * z);
         const int b = cell_start[c], e = cell_start[c + 1];
-        for (int k = b; k < e; ++k) {
-          const int j = cell_atoms[k];
-          double d0 = pos[3 * j] - xi, d1 = pos[3 * j + 1] - yi, d2 = pos[3 * j + 2] - zi;
-          if (g.per) {
-            const int k0 = s0 + wi0 - wrap[3 * j], k1 = s1 + wi1 - wrap[3 * j + 1], k2 = s2 + wi2 - wrap[3 * j + 2];
-            if (j == i && k0 == 0 && k1 == 0 && k2 == 0) continue;
-            d0 = d0 + k0 * a[0] + k1 * a[3] + k2 * a[6];
-            d1 = d1 + k0 * a[1] + k1 * a[4] + k2 * a[7];
-            d2 = d2 + k0 * a[2] + k1 * a[5] + k2 * a[8];
-          } else if (j == i) {
-            continue;
+        for (int k0 = b; k0 < e; k0 += WAVE) {
+          const int k = k0 + lane;
+          bool hit = false;
+          double d0 = 0, d1 = 0, d2 = 0;
+          int j = 0;
+          if (k < e) {
+            j = cell_atoms[k];
+            d0 = pos[3 * j] - xi; d1 = pos[3 * j + 1] - yi; d2 = pos[3 * j + 2] - zi;
+            bool self = false;
+            if (g.per) {
+              const int k0i = s0 + wi0 - wrap[3 * j], k1i = s1 + wi1 - wrap[3 * j + 1], k2i = s2 + wi2 - wrap[3 * j + 2];
+              self = j == i && k0i == 0 && k1i == 0 && k2i == 0;
+              d0 = d0 + k0i * a[0] + k1i * a[3] + k2i * a[6];
+              d1 = d1 + k0i * a[1] + k1i * a[4] + k2i * a[7];
+              d2 = d2 + k0i * a[2] + k1i * a[5] + k2i * a[8];
+            } else {
+              self = j == i;
+            }
+            hit = !self && d0 * d0 + d1 * d1 + d2 * d2 < rc2;
           }
-          if (d0 * d0 + d1 * d1 + d2 * d2 < rc2) {
-            if (rec) rec[w0 + count] = Rec<T>{T(d0), T(d1), T(d2), rec_jbits(j, T(0))};
-            ++count;
-          }
+          const unsigned long long mask = __ballot(hit);
+          if (rec && hit) rec[w0 + count + __popcll(mask & below)] = Rec<T>{T(d0), T(d1), T(d2), rec_jbits(j, T(0))};
+          count += __popcll(mask);
         }
       }
     }
   }
-  if (!rec) cnt[i] = count;
+  if (!rec && lane == 0) cnt[i] = count;
+}
+
+// Incoming edges of every atom (edges e' = j -> i grouped by i).  The full
+// list is symmetric, so atom i has as many incoming as outgoing edges and the
+// outgoing offsets serve both; lists are sorted afterwards for a fixed order.
+template <class T>
+__global__ void k_incoming(long nedges, const Rec<T>* __restrict__ rec, const int* __restrict__ off,
+                           int* __restrict__ cur, int* __restrict__ in_edges) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nedges) return;
+  const int dst = rec_j(rec[e]);
+  in_edges[off[dst] + atomicAdd(cur + dst, 1)] = (int)e;
 }
 
 // ------------------------------------------------------------------ finalize
@@ -283,22 +328,24 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   ws.cell_start.ensure(sizeof(int) * (n1 + 1));
   ws.cell_cur.ensure(sizeof(int) * n1);
   ws.cell_atoms.ensure(sizeof(int) * n1);
+  ws.in_cur.ensure(sizeof(int) * n1);
   ws.scan_tmp.ensure(2 * sizeof(int64_t));
   const int nb = (N + 255) / 256;
   const std::string tn = std::string("<") + tname<T>() + ">";
   SUS2_LAUNCH(ws, st, "grid_setup", k_grid_setup, dim3((b.nframes + 63) / 64), dim3(64), 0, b.nframes, b.frame_off,
               b.pos, b.cell, b.periodic, rc, ws.grid.get<Grid>());
   if (N > 0) {
-    SUS2_LAUNCH(ws, st, "zero_bins", k_zero_int, dim3(nb), dim3(256), 0, N, ws.cell_cnt.get<int>());
-    SUS2_LAUNCH(ws, st, "zero_bins", k_zero_int, dim3(nb), dim3(256), 0, N, ws.cell_cur.get<int>());
+    SUS2_LAUNCH(ws, st, "zero", k_zero_int, dim3(nb), dim3(256), 0, N, ws.cell_cnt.get<int>());
+    SUS2_LAUNCH(ws, st, "zero", k_zero_int, dim3(nb), dim3(256), 0, N, ws.cell_cur.get<int>());
+    SUS2_LAUNCH(ws, st, "zero", k_zero_int, dim3(nb), dim3(256), 0, N, ws.in_cur.get<int>());
     SUS2_LAUNCH(ws, st, "bin_atoms", k_bin, dim3(nb), dim3(256), 0, N, b.nframes, b.frame_off, b.pos,
                 ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_cnt.get<int>());
     device_scan(ws.cell_cnt.get<int>(), ws.cell_start.get<int>(), N, ws.scan_tmp.get<int64_t>() + 1, st);
     SUS2_LAUNCH(ws, st, "fill_bins", k_fill_bins, dim3(nb), dim3(256), 0, N, ws.cell_of.get<int>(),
                 ws.cell_start.get<int>(), ws.cell_cur.get<int>(), ws.cell_atoms.get<int>());
-    SUS2_LAUNCH(ws, st, "sort_bins", k_sort_bins, dim3(nb), dim3(256), 0, N, ws.cell_start.get<int>(),
+    SUS2_LAUNCH(ws, st, "sort_bins", k_sort_lists, dim3(N), dim3(WAVE), 0, N, ws.cell_start.get<int>(),
                 ws.cell_atoms.get<int>());
-    SUS2_LAUNCH(ws, st, "neighbors_count" + tn, k_neighbors<T>, dim3(nb), dim3(256), 0, N, b.nframes, b.frame_off,
+    SUS2_LAUNCH(ws, st, "neighbors_count" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, b.nframes, b.frame_off,
                 b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
                 ws.cell_start.get<int>(), ws.cell_atoms.get<int>(), rc, nullptr, ws.nbr_cnt.get<int>(), nullptr);
   }
@@ -306,11 +353,19 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   if (total > INT32_MAX - 1) throw Error(SUS2_E_RANGE, "neighbour list exceeds 2^31 edges");
   ws.nedges = total;
   ws.rec.ensure(sizeof(Rec<T>) * std::max<long>(total, 1));
-  if (N > 0)
-    SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(nb), dim3(256), 0, N, b.nframes, b.frame_off,
+  ws.in_edges.ensure(sizeof(int) * std::max<long>(total, 1));
+  if (N > 0) {
+    SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, b.nframes, b.frame_off,
                 b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
                 ws.cell_start.get<int>(), ws.cell_atoms.get<int>(), rc, ws.nbr_off.get<int>(), nullptr,
                 ws.rec.get<Rec<T>>());
+    if (total > 0) {
+      SUS2_LAUNCH(ws, st, "incoming" + tn, k_incoming<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, total,
+                  ws.rec.get<Rec<T>>(), ws.nbr_off.get<int>(), ws.in_cur.get<int>(), ws.in_edges.get<int>());
+      SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(WAVE), 0, N, ws.nbr_off.get<int>(),
+                  ws.in_edges.get<int>());
+    }
+  }
 }
 
 template <class T>

@@ -1,13 +1,23 @@
 // Shape-templated EFS kernels and their per-shape driver; instantiated per
 // (L, K) in sus2_efs_l*k*.hip so the shapes compile in parallel.
 //
-// Two kinds of kernels:
-//   edge kernels   one 64-lane workgroup (one wavefront) per centre atom,
-//                  lanes over that atom's neighbour edges;
-//   graph kernels  one lane per atom: the K->rank contraction and the product
-//                  graph (same schedule for every atom, so schedule reads are
-//                  wave-uniform scalar loads) over node-major arrays
-//                  V[node][atom] that are coalesced across the wavefront.
+// Every kernel here runs one 64-lane workgroup (one wavefront) per centre
+// atom:
+//   k_pass_forward<gate>  edge phase (lane = edge: radial functions and
+//                         harmonics staged in LDS), moment phase (lane = basic
+//                         moment, sum over the staged edges in edge order),
+//                         then the K->rank contraction and the product graph
+//                         with product rows spread over the lanes.  gate:
+//                         h_i = W S_i and the gate node values for the reverse;
+//                         outer: site energy and the basic-moment adjoints.
+//   k_edge_backward<gate> lane = edge: g_e = dE_i/dd_e of the centre's own
+//                         energy for each directed edge, stored per edge, and
+//                         (outer pass of an Effect model) the gate-adjoint
+//                         message u_e of the neighbour's gate signal.
+//   k_gather<gate>        F_i = sum_{out edges} g_e - sum_{in edges} g_e' over
+//                         the sorted incoming-edge list (no atomics), and for
+//                         the outer pass of an Effect model h_bar_i followed by
+//                         the gate graph reverse to the gate basic adjoints.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -25,26 +35,121 @@ struct Shape {
   static constexpr int LR = WAVE + 1;  // padded LDS staging row
 };
 
-__device__ __forceinline__ int l_of(int h) {
+__host__ __device__ __forceinline__ int l_of(int h) {
   int l = 0;
   while ((l + 1) * (l + 1) <= h) ++l;
   return l;
 }
 
-// ------------------------------------------------------------------ edge: moments
-// Basic moments of one pass for centre atom i (workgroup i):
-//   mom[(h,k)][i] = sum_j w_mu(t_i, t_j, h_j) R_mu(r_ij) Y_h(d_ij),  mu = (k, l(h)).
-// Edge phase: lane = edge, stages w R and Y in LDS; moment phase: lane =
-// moment, sums the staged edges in edge order.
+// ------------------------------------------------------------------ wave graph walks
+// Basic nodes take the moments; rank nodes the K -> rank contraction
+// (sus2_sl_kernel.cpp:104-105, 217-221).  v: LDS [M].
+template <class T, int KK>
+__device__ void wave_contract(const DModel<T>& m, const DPass<T>& p, bool gate, const T* mom, T* v) {
+  const int lane = threadIdx.x, H = m.H, KH = KK * H;
+  for (int c = lane; c < KH; c += WAVE) {
+    const int node = m.basic_node[c];
+    if (node >= 0) v[node] = mom[c];
+  }
+  const int nr = p.rank * H;
+  for (int idx = lane; idx < nr; idx += WAVE) {
+    const int a = idx / H, hh = idx % H;
+    const int l = l_of(hh), mm = hh - l * l - l;
+    T s = T(0);
+#pragma unroll
+    for (int k = 0; k < KK; ++k)
+      if (m.basic_node[hh * KK + k] >= 0) s += m.rankw(gate, l, a, k) * mom[hh * KK + k];
+    v[rank_node(p.begin, p.rank, H, l, mm, a)] = s;
+  }
+  __syncthreads();
+}
+
+// Product layers with rows spread over the lanes, then per-output sums of the
+// staged row products in serialised row order (sus2_sl_kernel.cpp:308-313).
+template <class T>
+__device__ void wave_graph_forward(const DPass<T>& p, T* v, T* scratch) {
+  const int lane = threadIdx.x;
+  for (int l = 0; l < p.nfl; ++l) {
+    const int o0 = p.fl_off[l], o1 = p.fl_off[l + 1];
+    const int r0 = p.f_roff[o0], r1 = p.f_roff[o1];
+    for (int r = r0 + lane; r < r1; r += WAVE) scratch[r - r0] = p.f_c[r] * v[p.f_a[r]] * v[p.f_b[r]];
+    __syncthreads();
+    for (int o = o0 + lane; o < o1; o += WAVE) {
+      T s = T(0);
+      const int e = p.f_roff[o + 1] - r0;
+      for (int r = p.f_roff[o] - r0; r < e; ++r) s += scratch[r];
+      v[p.f_out[o]] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// Reverse: a[n] += sum over consumer rows c a[out] v[partner], nodes in
+// reverse-layer order; a holds the seeds on entry.
+template <class T>
+__device__ void wave_graph_reverse(const DPass<T>& p, const T* v, T* a, T* scratch) {
+  const int lane = threadIdx.x;
+  for (int l = 0; l < p.nrl; ++l) {
+    const int n0 = p.rl_off[l], n1 = p.rl_off[l + 1];
+    const int q0 = p.r_roff[n0], q1 = p.r_roff[n1];
+    for (int q = q0 + lane; q < q1; q += WAVE) scratch[q - q0] = p.r_c[q] * a[p.r_out[q]] * v[p.r_partner[q]];
+    __syncthreads();
+    for (int k = n0 + lane; k < n1; k += WAVE) {
+      const int n = p.r_node[k];
+      T s = a[n];
+      const int e = p.r_roff[k + 1] - q0;
+      for (int q = p.r_roff[k] - q0; q < e; ++q) s += scratch[q];
+      a[n] = s;
+    }
+    __syncthreads();
+  }
+}
+
+// Transposed contraction to canonical basic adjoints out[(h,k)].
+template <class T, int KK>
+__device__ void wave_adjoint_to_basic(const DModel<T>& m, const DPass<T>& p, bool gate, const T* a, T* out) {
+  const int lane = threadIdx.x, H = m.H, KH = KK * H;
+  for (int c = lane; c < KH; c += WAVE) {
+    const int hh = c / KK, k = c % KK;
+    const int l = l_of(hh), mm = hh - l * l - l;
+    const int node = m.basic_node[c];
+    T s = T(0);
+    if (node >= 0) {
+      s = a[node];
+      for (int aa = 0; aa < p.rank; ++aa) s += m.rankw(gate, l, aa, k) * a[rank_node(p.begin, p.rank, H, l, mm, aa)];
+    }
+    out[c] = s;
+  }
+}
+
+template <class T>
+__host__ __device__ inline int graph_scratch(const DPass<T>& p) {
+  return p.max_fw_rows > p.max_rv_rows ? p.max_fw_rows : p.max_rv_rows;
+}
+
+// ------------------------------------------------------------------ pass forward
+template <class T, int LL, int KK>
+size_t forward_lds(const DModel<T>& m) {
+  using S = Shape<T, LL, KK>;
+  const int sc = std::max(graph_scratch(m.gp), graph_scratch(m.op));
+  return sizeof(T) * ((size_t)(S::F + S::H) * S::LR + S::KH + 2 * (size_t)m.M + std::max(sc, 1));
+}
+
 template <class T, int LL, int KK, bool GATE>
-__global__ __launch_bounds__(WAVE) void k_edge_moments(DModel<T> m, int N, const int32_t* __restrict__ types,
+__global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_t* __restrict__ types,
                                                        const int* __restrict__ nbr_off,
-                                                       const Rec<T>* __restrict__ rec, const T* __restrict__ h,
-                                                       T* __restrict__ mom) {
+                                                       const Rec<T>* __restrict__ rec, T* __restrict__ h,
+                                                       T* __restrict__ vg, T* __restrict__ site,
+                                                       T* __restrict__ abar) {
   using S = Shape<T, LL, KK>;
   constexpr int L1 = S::L1, H = S::H, F = S::F, KH = S::KH, NM = S::NM, LR = S::LR;
-  __shared__ T sR[F * LR];
-  __shared__ T sY[H * LR];
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sR = reinterpret_cast<T*>(smem_raw);  // [F][LR]
+  T* sY = sR + F * LR;                     // [H][LR]
+  T* mom = sY + H * LR;                    // [KH]
+  T* v = mom + KH;                         // [M]
+  T* a = v + m.M;                          // [M]
+  T* scratch = a + m.M;
   const int i = blockIdx.x, lane = threadIdx.x;
   const int ti = types[i];
   const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
@@ -92,199 +197,66 @@ __global__ __launch_bounds__(WAVE) void k_edge_moments(DModel<T> m, int N, const
     }
     __syncthreads();
   }
+  for (int n = lane; n < m.M; n += WAVE) v[n] = T(0);
 #pragma unroll
   for (int t = 0; t < NM; ++t) {
     const int c = lane + WAVE * t;
-    if (c < KH) mom[(size_t)c * N + i] = acc[t];
+    if (c < KH) mom[c] = acc[t];
   }
-}
-
-// ------------------------------------------------------------------ graph: forward
-// lane = atom.  Node values of the pass live in V[(node - lo)][N].
-template <class T, int LL, int KK>
-__device__ __forceinline__ void graph_forward_lane(const DModel<T>& m, const DPass<T>& p, bool gate, int N, int i,
-                                                   const T* __restrict__ mom, T* __restrict__ V) {
-  using S = Shape<T, LL, KK>;
-  constexpr int H = S::H;
-  const int lo = p.node_lo, cnt = p.node_cnt;
-  // basic nodes that the pass reads directly (unranked graphs)
-  for (int c = 0; c < KK * H; ++c) {
-    const int node = m.basic_node[c];
-    if (node >= lo && node < lo + cnt) V[(size_t)(node - lo) * N + i] = mom[(size_t)c * N + i];
-  }
-  // K -> rank contraction
-  for (int a = 0; a < p.rank; ++a)
-    for (int hh = 0; hh < H; ++hh) {
-      const int l = l_of(hh), mm = hh - l * l - l;
+  __syncthreads();
+  const DPass<T>& p = GATE ? m.gp : m.op;
+  wave_contract<T, KK>(m, p, GATE, mom, v);
+  wave_graph_forward(p, v, scratch);
+  if (GATE) {
+    // h_i = W S_i (lanes over mu), node values of the gate range for the reverse
+    for (int mu = lane; mu < F; mu += WAVE) {
       T s = T(0);
-#pragma unroll
-      for (int k = 0; k < KK; ++k)
-        if (m.basic_node[hh * KK + k] >= 0) s += m.rankw(gate, l, a, k) * mom[(size_t)(hh * KK + k) * N + i];
-      V[(size_t)(rank_node(p.begin, p.rank, H, l, mm, a) - lo) * N + i] = s;
+      for (int q = 0; q < m.Qg; ++q) s += m.W(mu, q) * v[m.gate_nodes[q]];
+      h[(size_t)i * F + mu] = s;
     }
-  // products in layer order: out = sum_rows c v[a] v[b]
-  for (int o = 0; o < p.nfout; ++o) {
-    T s = T(0);
-    const int r1 = p.f_roff[o + 1];
-    for (int r = p.f_roff[o]; r < r1; ++r)
-      s += p.f_c[r] * V[(size_t)(p.f_a[r] - lo) * N + i] * V[(size_t)(p.f_b[r] - lo) * N + i];
-    V[(size_t)(p.f_out[o] - lo) * N + i] = s;
+    T* out = vg + (size_t)i * p.node_cnt;
+    for (int n = lane; n < p.node_cnt; n += WAVE) out[n] = v[p.node_lo + n];
+    for (int c = lane; c < KH; c += WAVE) out[p.node_cnt + c] = mom[c];
+    return;
   }
+  T part = T(0);
+  for (int q = lane; q < m.Q; q += WAVE) part += m.readout(ti, q) * v[m.scalar_map[q]];
+  const T E = wave_sum(part) + m.site0(ti);
+  if (lane == 0) site[i] = E;
+  for (int n = lane; n < m.M; n += WAVE) a[n] = T(0);
+  __syncthreads();
+  if (lane == 0)
+    for (int q = 0; q < m.Q; ++q) a[m.scalar_map[q]] += m.readout(ti, q);
+  __syncthreads();
+  wave_graph_reverse(p, v, a, scratch);
+  wave_adjoint_to_basic<T, KK>(m, p, false, a, abar + (size_t)i * KH);
 }
 
-// Reverse: A[n] = seed[n] + sum over consumer rows c A[out] V[partner], nodes in
-// reverse-layer order, then the transposed contraction to canonical basic
-// adjoints abar[i][(h,k)] (atom-major, read per neighbour by the edge kernels).
-template <class T, int LL, int KK>
-__device__ __forceinline__ void graph_reverse_lane(const DModel<T>& m, const DPass<T>& p, bool gate, int N, int i,
-                                                   const T* __restrict__ V, T* __restrict__ A,
-                                                   T* __restrict__ abar) {
-  using S = Shape<T, LL, KK>;
-  constexpr int H = S::H, KH = S::KH;
-  const int lo = p.node_lo, cnt = p.node_cnt;
-  for (int k = 0; k < p.nrnode; ++k) {
-    const int n = p.r_node[k];
-    T s = A[(size_t)(n - lo) * N + i];
-    const int r1 = p.r_roff[k + 1];
-    for (int r = p.r_roff[k]; r < r1; ++r)
-      s += p.r_c[r] * A[(size_t)(p.r_out[r] - lo) * N + i] * V[(size_t)(p.r_partner[r] - lo) * N + i];
-    A[(size_t)(n - lo) * N + i] = s;
-  }
-  T* out = abar + (size_t)i * KH;
-  for (int hh = 0; hh < H; ++hh) {
-    const int l = l_of(hh), mm = hh - l * l - l;
-#pragma unroll
-    for (int k = 0; k < KK; ++k) {
-      const int node = m.basic_node[hh * KK + k];
-      T s = T(0);
-      if (node >= 0) {
-        if (node >= lo && node < lo + cnt) s = A[(size_t)(node - lo) * N + i];
-        for (int a = 0; a < p.rank; ++a)
-          s += m.rankw(gate, l, a, k) * A[(size_t)(rank_node(p.begin, p.rank, H, l, mm, a) - lo) * N + i];
-      }
-      out[hh * KK + k] = s;
-    }
-  }
-}
-
-// gate forward: V_g, then h_i = W S_i
-template <class T, int LL, int KK>
-__global__ __launch_bounds__(WAVE) void k_graph_gate_fwd(DModel<T> m, int N, const T* __restrict__ mom,
-                                                         T* __restrict__ V, T* __restrict__ h) {
-  const int i = blockIdx.x * WAVE + threadIdx.x;
-  if (i >= N) return;
-  graph_forward_lane<T, LL, KK>(m, m.gp, true, N, i, mom, V);
-  constexpr int F = Shape<T, LL, KK>::F;
-  const int lo = m.gp.node_lo;
-  T acc[F];
-#pragma unroll
-  for (int mu = 0; mu < F; ++mu) acc[mu] = T(0);
-  for (int q = 0; q < m.Qg; ++q) {
-    const T sq = V[(size_t)(m.gate_nodes[q] - lo) * N + i];
-#pragma unroll
-    for (int mu = 0; mu < F; ++mu) acc[mu] += m.W(mu, q) * sq;
-  }
-#pragma unroll
-  for (int mu = 0; mu < F; ++mu) h[(size_t)i * F + mu] = acc[mu];
-}
-
-// outer forward + readout + reverse
-template <class T, int LL, int KK>
-__global__ __launch_bounds__(WAVE) void k_graph_outer(DModel<T> m, int N, const int32_t* __restrict__ types,
-                                                      const T* __restrict__ mom, T* __restrict__ V,
-                                                      T* __restrict__ A, T* __restrict__ site,
-                                                      T* __restrict__ abar) {
-  const int i = blockIdx.x * WAVE + threadIdx.x;
-  if (i >= N) return;
-  const DPass<T>& p = m.op;
-  graph_forward_lane<T, LL, KK>(m, p, false, N, i, mom, V);
-  const int ti = types[i];
-  const int lo = p.node_lo;
-  T E = m.site0(ti);
-  for (int n = 0; n < p.node_cnt; ++n) A[(size_t)n * N + i] = T(0);
-  for (int q = 0; q < m.Q; ++q) {
-    const int node = m.scalar_map[q] - lo;
-    const T w = m.readout(ti, q);
-    E += w * V[(size_t)node * N + i];
-    A[(size_t)node * N + i] += w;
-  }
-  site[i] = E;
-  graph_reverse_lane<T, LL, KK>(m, p, false, N, i, V, A, abar);
-}
-
-// gate reverse: seeds S_bar = W^T h_bar_i, graph reverse over the stored V_g
-template <class T, int LL, int KK>
-__global__ __launch_bounds__(WAVE) void k_graph_gate_rev(DModel<T> m, int N, const T* __restrict__ hbar,
-                                                         const T* __restrict__ V, T* __restrict__ A,
-                                                         T* __restrict__ gbar) {
-  const int i = blockIdx.x * WAVE + threadIdx.x;
-  if (i >= N) return;
-  constexpr int F = Shape<T, LL, KK>::F;
-  const DPass<T>& p = m.gp;
-  const int lo = p.node_lo;
-  T hb[F];
-#pragma unroll
-  for (int mu = 0; mu < F; ++mu) hb[mu] = hbar[(size_t)i * F + mu];
-  for (int n = 0; n < p.node_cnt; ++n) A[(size_t)n * N + i] = T(0);
-  for (int q = 0; q < m.Qg; ++q) {
-    T s = T(0);
-#pragma unroll
-    for (int mu = 0; mu < F; ++mu) s += hb[mu] * m.W(mu, q);
-    A[(size_t)(m.gate_nodes[q] - lo) * N + i] += s;
-  }
-  graph_reverse_lane<T, LL, KK>(m, p, true, N, i, V, A, gbar);
-}
-
-// ------------------------------------------------------------------ edge: forces
-// One directed edge d (r, u = d/r) against basic adjoints `a` (KH, (h,k)
-// order, scaled by `par` = (-1)^l when the edge is evaluated reversed):
-//   sY = sum a dR Y,  g = sum a R grad Y,  and optionally hb_mu += c_mu sum_m a Y.
-template <class T, int LL, int KK, bool REV, bool HB, class Adj>
-__device__ __forceinline__ void edge_reverse(T ux, T uy, T uz, T inv, const T* Rw, const T* dRw, const T* cr,
-                                             Adj&& adj, T& sY, T& gx, T& gy, T& gz, T* hb) {
-  constexpr int L1 = LL + 1;
-  sh_stream_grad<T, LL>(ux, uy, uz, inv, [&](int l, int mm, T Y, T yx, T yy, T yz) {
-    const int hh = hidx(l, mm);
-    const T par = (REV && (l & 1)) ? T(-1) : T(1);
-#pragma unroll
-    for (int k = 0; k < KK; ++k) {
-      const int mu = k * L1 + l;
-      const T a = adj(hh * KK + k) * par;
-      sY += a * dRw[mu] * Y;
-      const T g = a * Rw[mu];
-      gx += g * yx; gy += g * yy; gz += g * yz;
-      if (HB) hb[mu] += cr[mu] * a * Y;
-    }
-  });
-}
-
-// Gather-based backward of one pass for centre i: for every edge i -> j,
-//   g_own = dE_i/dd_ij (centre adjoints), g_rev = dE_j/dd_ji (neighbour's),
-//   F_i = sum (g_own - g_rev),  W_i = -sum d (x) g_own,
-// and for the outer pass of an Effect model the gate adjoint
-//   h_bar_i,mu = alpha_{t_i,mu} sum_j tau_{t_j,mu} R_mu(r; t_j,t_i) sum_m a_j Y(d_ji).
+// ------------------------------------------------------------------ edge backward
+// One directed edge e = i -> j of the centre's own energy:
+//   g_e = sum_b abar_i[b] w_mu (R'_mu u Y_b + R_mu grad Y_b),
+// written to gedge[e] (each pass overwrites it after the previous pass's
+// gather consumed it), and for the
+// outer pass of an Effect model the gate-adjoint message
+//   u_e,mu = tau_{t_i,mu} R_mu(r; t_i, t_j) sum_m abar_i[k,l,m] Y_lm(d_e),
+// which the neighbour j gathers into h_bar_j (times alpha_{t_j,mu}).
 template <class T, int LL, int KK, bool GATE>
-__global__ __launch_bounds__(WAVE) void k_edge_force(DModel<T> m, const int32_t* __restrict__ types,
-                                                     const int* __restrict__ nbr_off,
-                                                     const Rec<T>* __restrict__ rec, const T* __restrict__ h,
-                                                     const T* __restrict__ adj, T* __restrict__ fw,
-                                                     T* __restrict__ hbar) {
+__global__ __launch_bounds__(WAVE) void k_edge_backward(DModel<T> m, const int32_t* __restrict__ types,
+                                                        const int* __restrict__ nbr_off,
+                                                        const Rec<T>* __restrict__ rec, const T* __restrict__ h,
+                                                        const T* __restrict__ adj, T* __restrict__ gedge,
+                                                        T* __restrict__ uedge, T* __restrict__ fw) {
   using S = Shape<T, LL, KK>;
-  constexpr int F = S::F, KH = S::KH;
+  constexpr int L1 = S::L1, F = S::F, KH = S::KH;
   __shared__ T sA[KH];
-  __shared__ T x3i[F];
+  __shared__ T x0[F];
   const int i = blockIdx.x, lane = threadIdx.x;
   const int ti = types[i];
   const bool hgate = !GATE && m.gated;
   for (int c = lane; c < KH; c += WAVE) sA[c] = adj[(size_t)i * KH + c];
-  for (int mu = lane; mu < F; mu += WAVE)
-    x3i[mu] = GATE ? m.gtype(ti, mu) : m.tau(ti, mu) + (m.gated ? m.alpha(ti, mu) * h[(size_t)i * F + mu] : T(0));
+  for (int mu = lane; mu < F; mu += WAVE) x0[mu] = GATE ? m.gtype(ti, mu) : m.tau(ti, mu);
   __syncthreads();
-  T fx = T(0), fy = T(0), fz = T(0);
   T w0 = T(0), w1 = T(0), w2 = T(0), w3 = T(0), w4 = T(0), w5 = T(0);
-  T hb[F];
-#pragma unroll
-  for (int mu = 0; mu < F; ++mu) hb[mu] = T(0);
   const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
   for (int e = e0 + lane; e < e1; e += WAVE) {
     const Rec<T> q = rec[e];
@@ -294,70 +266,127 @@ __global__ __launch_bounds__(WAVE) void k_edge_force(DModel<T> m, const int32_t*
     const T inv = T(1) / r;
     const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
     const int tj = types[j];
-    T R[F], dR[F], cr[F];
-    // ---- own direction: E_i through d_ij, pair (t_i, t_j)
+    T R[F], dR[F], wt[F], us[F];
     radial<T, LL, KK, true>(m, GATE, ti * m.C + tj, r, R, dR);
 #pragma unroll
     for (int mu = 0; mu < F; ++mu) {
-      T w;
-      if (GATE) w = x3i[mu] * m.gtype(tj, mu);
+      T x1;
+      if (GATE) x1 = m.gtype(tj, mu);
       else {
-        T x3j = m.tau(tj, mu);
-        if (m.gated) x3j += m.alpha(tj, mu) * h[(size_t)j * F + mu];
-        w = m.tau(ti, mu) * x3j;
+        x1 = m.tau(tj, mu);
+        if (m.gated) x1 += m.alpha(tj, mu) * h[(size_t)j * F + mu];
       }
-      R[mu] *= w; dR[mu] *= w;
+      wt[mu] = x0[mu] * x1;
+      dR[mu] *= wt[mu];
+      us[mu] = T(0);
     }
     T sY = T(0), gx = T(0), gy = T(0), gz = T(0);
-    edge_reverse<T, LL, KK, false, false>(ux, uy, uz, inv, R, dR, cr, [&](int c) { return sA[c]; }, sY, gx, gy, gz,
-                                          hb);
-    const T gx0 = ux * sY + gx, gy0 = uy * sY + gy, gz0 = uz * sY + gz;
-    // ---- reverse direction: E_j through d_ji = -d_ij, pair (t_j, t_i)
-    radial<T, LL, KK, true>(m, GATE, tj * m.C + ti, r, R, dR);
+    sh_stream_grad<T, LL>(ux, uy, uz, inv, [&](int l, int mm, T Y, T yx, T yy, T yz) {
+      const int hh = hidx(l, mm);
 #pragma unroll
-    for (int mu = 0; mu < F; ++mu) {
-      T w;
-      if (GATE) w = x3i[mu] * m.gtype(tj, mu);
-      else {
-        const T tauj = m.tau(tj, mu);
-        w = tauj * x3i[mu];
-        if (hgate) cr[mu] = tauj * m.alpha(ti, mu) * R[mu];
+      for (int k = 0; k < KK; ++k) {
+        const int mu = k * L1 + l;
+        const T av = sA[hh * KK + k];
+        sY += av * dR[mu] * Y;
+        const T g = av * wt[mu] * R[mu];
+        gx += g * yx; gy += g * yy; gz += g * yz;
+        if (hgate) us[mu] += av * Y;
       }
-      R[mu] *= w; dR[mu] *= w;
+    });
+    const T g0 = ux * sY + gx, g1 = uy * sY + gy, g2 = uz * sY + gz;
+    T* ge = gedge + 3 * (size_t)e;
+    ge[0] = g0; ge[1] = g1; ge[2] = g2;
+    w0 -= dx * g0; w1 -= dy * g1; w2 -= dz * g2;
+    w3 -= dx * g1; w4 -= dx * g2; w5 -= dy * g2;
+    if (hgate) {
+      T* ue = uedge + (size_t)e * F;
+#pragma unroll
+      for (int mu = 0; mu < F; ++mu) ue[mu] = x0[mu] * R[mu] * us[mu];
     }
-    const T* aj = adj + (size_t)j * KH;
-    sY = gx = gy = gz = T(0);
-    if (hgate)
-      edge_reverse<T, LL, KK, true, true>(ux, uy, uz, inv, R, dR, cr, [&](int c) { return aj[c]; }, sY, gx, gy, gz,
-                                          hb);
-    else
-      edge_reverse<T, LL, KK, true, false>(ux, uy, uz, inv, R, dR, cr, [&](int c) { return aj[c]; }, sY, gx, gy,
-                                           gz, hb);
-    // g_rev = dE_j/dd_ji: d(r)/d(d_ji) = -u, grad Y(-d) = -(-1)^l grad Y(d)
-    const T gx1 = -ux * sY - gx, gy1 = -uy * sY - gy, gz1 = -uz * sY - gz;
-    fx += gx0 - gx1; fy += gy0 - gy1; fz += gz0 - gz1;
-    w0 -= dx * gx0; w1 -= dy * gy0; w2 -= dz * gz0;
-    w3 -= dx * gy0; w4 -= dx * gz0; w5 -= dy * gz0;
   }
-  fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
   w0 = wave_sum(w0); w1 = wave_sum(w1); w2 = wave_sum(w2);
   w3 = wave_sum(w3); w4 = wave_sum(w4); w5 = wave_sum(w5);
   if (lane == 0) {
-    T* o = fw + (size_t)i * 9;
-    if (GATE) {
-      o[0] += fx; o[1] += fy; o[2] += fz;
-      o[3] += w0; o[4] += w1; o[5] += w2; o[6] += w3; o[7] += w4; o[8] += w5;
-    } else {
-      o[0] = fx; o[1] = fy; o[2] = fz;
-      o[3] = w0; o[4] = w1; o[5] = w2; o[6] = w3; o[7] = w4; o[8] = w5;
+    T* o = fw + (size_t)i * 9 + 3;
+    if (GATE) { o[0] += w0; o[1] += w1; o[2] += w2; o[3] += w3; o[4] += w4; o[5] += w5; }
+    else { o[0] = w0; o[1] = w1; o[2] = w2; o[3] = w3; o[4] = w4; o[5] = w5; }
+  }
+}
+
+// ------------------------------------------------------------------ gather
+template <class T, int LL, int KK>
+size_t gather_lds(const DModel<T>& m) {
+  using S = Shape<T, LL, KK>;
+  return sizeof(T) * ((size_t)2 * m.M + S::KH + S::F + std::max(graph_scratch(m.gp), 1));
+}
+
+// F_i = sum_{e in out(i)} g_e - sum_{e' in in(i)} g_e' ; the outer pass of an
+// Effect model also forms h_bar_i = alpha_{t_i} sum_{e' in in(i)} u_e' and runs
+// the gate graph reverse from the stored gate node values.
+template <class T, int LL, int KK, bool GATE>
+__global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __restrict__ types,
+                                                 const int* __restrict__ nbr_off, const int* __restrict__ in_edges,
+                                                 const T* __restrict__ gedge, const T* __restrict__ uedge,
+                                                 const T* __restrict__ vg, T* __restrict__ gbar,
+                                                 T* __restrict__ fw) {
+  using S = Shape<T, LL, KK>;
+  constexpr int F = S::F, KH = S::KH;
+  const int i = blockIdx.x, lane = threadIdx.x;
+  const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
+  const bool hgate = !GATE && m.gated;
+  T fx = T(0), fy = T(0), fz = T(0);
+  T hb[F];
+#pragma unroll
+  for (int mu = 0; mu < F; ++mu) hb[mu] = T(0);
+  for (int k = e0 + lane; k < e1; k += WAVE) {
+    const T* go = gedge + 3 * (size_t)k;
+    const int ein = in_edges[k];
+    const T* gi = gedge + 3 * (size_t)ein;
+    fx += go[0] - gi[0];
+    fy += go[1] - gi[1];
+    fz += go[2] - gi[2];
+    if (hgate) {
+      const T* ue = uedge + (size_t)ein * F;
+#pragma unroll
+      for (int mu = 0; mu < F; ++mu) hb[mu] += ue[mu];
     }
   }
+  fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+  if (lane == 0) {
+    T* o = fw + (size_t)i * 9;
+    if (GATE) { o[0] += fx; o[1] += fy; o[2] += fz; }
+    else { o[0] = fx; o[1] = fy; o[2] = fz; }
+  }
   if (!hgate) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* v = reinterpret_cast<T*>(smem_raw);  // [M]
+  T* a = v + m.M;                          // [M]
+  T* mom = a + m.M;                        // [KH]
+  T* hbs = mom + KH;                       // [F]
+  T* scratch = hbs + F;
+  const int ti = types[i];
 #pragma unroll
   for (int mu = 0; mu < F; ++mu) {
     const T s = wave_sum(hb[mu]);
-    if (lane == 0) hbar[(size_t)i * F + mu] = s;
+    if (lane == 0) hbs[mu] = s * m.alpha(ti, mu);
   }
+  const DPass<T>& p = m.gp;
+  const T* vi = vg + (size_t)i * p.node_cnt;
+  for (int n = lane; n < m.M; n += WAVE) {
+    v[n] = (n >= p.node_lo && n < p.node_lo + p.node_cnt) ? vi[n - p.node_lo] : T(0);
+    a[n] = T(0);
+  }
+  __syncthreads();
+  // seeds S_bar_q = sum_mu h_bar_mu W_mu,q
+  for (int q = lane; q < m.Qg; q += WAVE) {
+    T s = T(0);
+#pragma unroll
+    for (int mu = 0; mu < F; ++mu) s += hbs[mu] * m.W(mu, q);
+    a[m.gate_nodes[q]] = s;
+  }
+  __syncthreads();
+  wave_graph_reverse(p, v, a, scratch);
+  wave_adjoint_to_basic<T, KK>(m, p, true, a, gbar + (size_t)i * KH);
 }
 
 // ------------------------------------------------------------------ driver
@@ -366,45 +395,59 @@ std::string kname(const char* base) {
   return std::string(base) + "<" + tname<T>() + ",L" + std::to_string(LL) + "K" + std::to_string(KK) + ">";
 }
 
+template <class K>
+void allow_lds(K kern, size_t bytes) {
+  if (bytes > 64 * 1024)
+    SUS2_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)bytes));
+}
+
 template <class T, int LL, int KK>
 void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st) {
   using S = Shape<T, LL, KK>;
   const DModel<T>& m = dm.get<T>();
   const int N = b.natoms;
   build_neighbors<T>(ws, b, dm.host->rc, st);
-  const size_t n1 = std::max(1, N);
+  const size_t n1 = std::max(1, N), ne = std::max<long>(1, ws.nedges);
   ws.h.ensure(sizeof(T) * n1 * S::F);
-  ws.hbar.ensure(sizeof(T) * n1 * S::F);
-  ws.mom.ensure(sizeof(T) * n1 * S::KH);
   ws.abar.ensure(sizeof(T) * n1 * S::KH);
   ws.gbar.ensure(sizeof(T) * n1 * S::KH);
-  ws.vg.ensure(sizeof(T) * n1 * std::max(1, m.gp.node_cnt));
-  ws.vo.ensure(sizeof(T) * n1 * std::max(1, m.op.node_cnt));
-  ws.ag.ensure(sizeof(T) * n1 * std::max(1, std::max(m.gp.node_cnt, m.op.node_cnt)));
+  ws.vg.ensure(sizeof(T) * n1 * (std::max(1, m.gp.node_cnt) + S::KH));
   ws.site.ensure(sizeof(T) * n1);
   ws.fw.ensure(sizeof(T) * n1 * 9);
+  ws.gedge.ensure(sizeof(T) * ne * 3);
+  if (m.gated) ws.uedge.ensure(sizeof(T) * ne * S::F);
   const int* off = ws.nbr_off.get<int>();
   const Rec<T>* rec = ws.rec.get<Rec<T>>();
   T* h = ws.h.get<T>();
-  const int gl = (N + WAVE - 1) / WAVE;
   if (N > 0) {
+    const size_t lf = forward_lds<T, LL, KK>(m), lg = gather_lds<T, LL, KK>(m);
     if (m.gated) {
-      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("edge_moments_gate")), (k_edge_moments<T, LL, KK, true>), dim3(N),
-                  dim3(WAVE), 0, m, N, b.types, off, rec, h, ws.mom.get<T>());
-      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("graph_gate_fwd")), (k_graph_gate_fwd<T, LL, KK>), dim3(gl), dim3(WAVE),
-                  0, m, N, ws.mom.get<T>(), ws.vg.get<T>(), h);
+      allow_lds(k_pass_forward<T, LL, KK, true>, lf);
+      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("pass_forward_gate")), (k_pass_forward<T, LL, KK, true>), dim3(N),
+                  dim3(WAVE), lf, m, b.types, off, rec, h, ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
     }
-    SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("edge_moments_outer")), (k_edge_moments<T, LL, KK, false>), dim3(N),
-                dim3(WAVE), 0, m, N, b.types, off, rec, h, ws.mom.get<T>());
-    SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("graph_outer")), (k_graph_outer<T, LL, KK>), dim3(gl), dim3(WAVE), 0, m, N,
-                b.types, ws.mom.get<T>(), ws.vo.get<T>(), ws.ag.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
-    SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("edge_force_outer")), (k_edge_force<T, LL, KK, false>), dim3(N),
-                dim3(WAVE), 0, m, b.types, off, rec, h, ws.abar.get<T>(), ws.fw.get<T>(), ws.hbar.get<T>());
+    allow_lds(k_pass_forward<T, LL, KK, false>, lf);
+    SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("pass_forward_outer")), (k_pass_forward<T, LL, KK, false>), dim3(N),
+                dim3(WAVE), lf, m, b.types, off, rec, h, ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
+    SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("edge_backward_outer")), (k_edge_backward<T, LL, KK, false>), dim3(N),
+                dim3(WAVE), 0, m, b.types, off, rec, h, ws.abar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(),
+                ws.fw.get<T>());
     if (m.gated) {
-      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("graph_gate_rev")), (k_graph_gate_rev<T, LL, KK>), dim3(gl), dim3(WAVE),
-                  0, m, N, ws.hbar.get<T>(), ws.vg.get<T>(), ws.ag.get<T>(), ws.gbar.get<T>());
-      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("edge_force_gate")), (k_edge_force<T, LL, KK, true>), dim3(N),
-                  dim3(WAVE), 0, m, b.types, off, rec, h, ws.gbar.get<T>(), ws.fw.get<T>(), ws.hbar.get<T>());
+      allow_lds(k_gather<T, LL, KK, false>, lg);
+      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("gather_outer")), (k_gather<T, LL, KK, false>), dim3(N), dim3(WAVE), lg,
+                  m, b.types, off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
+                  ws.gbar.get<T>(), ws.fw.get<T>());
+      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("edge_backward_gate")), (k_edge_backward<T, LL, KK, true>), dim3(N),
+                  dim3(WAVE), 0, m, b.types, off, rec, h, ws.gbar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(),
+                  ws.fw.get<T>());
+      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("gather_gate")), (k_gather<T, LL, KK, true>), dim3(N), dim3(WAVE), 0, m,
+                  b.types, off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
+                  ws.gbar.get<T>(), ws.fw.get<T>());
+    } else {
+      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("gather_outer")), (k_gather<T, LL, KK, false>), dim3(N), dim3(WAVE), 0,
+                  m, b.types, off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
+                  ws.gbar.get<T>(), ws.fw.get<T>());
     }
   }
   finish_efs<T>(ws, b, out, st);

@@ -143,6 +143,10 @@ PassPlan make_plan(const Model& m, bool gate) {
     }
     p.rv_layer_off.push_back((int)p.rv_node.size());
   }
+  for (size_t l = 0; l + 1 < p.fw_layer_off.size(); ++l)
+    p.max_fw_rows = std::max(p.max_fw_rows, p.fw_row_off[p.fw_layer_off[l + 1]] - p.fw_row_off[p.fw_layer_off[l]]);
+  for (size_t l = 0; l + 1 < p.rv_layer_off.size(); ++l)
+    p.max_rv_rows = std::max(p.max_rv_rows, p.rv_row_off[p.rv_layer_off[l + 1]] - p.rv_row_off[p.rv_layer_off[l]]);
   return p;
 }
 

@@ -31,6 +31,9 @@ struct PassPlan {
   // contiguous node range [node_lo, node_lo + node_cnt) touched by the pass
   // (rank nodes, product inputs/outputs and readout seeds)
   int node_lo = 0, node_cnt = 0;
+  // largest number of product rows in one forward layer / consumer entries in
+  // one reverse layer (LDS scratch of the row-parallel walks)
+  int max_fw_rows = 0, max_rv_rows = 0;
 };
 
 struct Model {

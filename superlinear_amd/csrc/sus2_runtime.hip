@@ -191,6 +191,8 @@ void fill_view(DevModelT<T>& dv, const Model& m, const int* ibase, const std::ve
     dp.rank = pp.rank;
     dp.begin = pp.begin;
     dp.node_lo = pp.node_lo;
+    dp.max_fw_rows = pp.max_fw_rows;
+    dp.max_rv_rows = pp.max_rv_rows;
     dp.node_cnt = pp.node_cnt;
     dp.nfout = pp.fw_out.empty() ? 0 : (int)pp.fw_out.size();
     dp.nrnode = pp.rv_node.empty() ? 0 : (int)pp.rv_node.size();

@@ -60,7 +60,10 @@ struct Profiler {
 
 struct Workspace {
   Profiler prof;
-  DevBuf nbr_cnt, nbr_off, scan_tmp, rec, h, hbar, mom, abar, gbar, vg, vo, ag, site, fw;
+  DevBuf nbr_cnt, nbr_off, scan_tmp, rec, h, abar, gbar, vg, site, fw;
+  // per-edge backward buffers: dE/dd of each directed edge, gate-adjoint
+  // messages, and the sorted incoming-edge lists
+  DevBuf gedge, uedge, in_edges, in_cur;
   // cell-list neighbour build
   DevBuf grid, cell_of, wrap, cell_cnt, cell_start, cell_cur, cell_atoms;
   // parameter-gradient scratch
