@@ -216,7 +216,6 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
     }
     T* out = vg + (size_t)i * p.node_cnt;
     for (int n = lane; n < p.node_cnt; n += WAVE) out[n] = v[p.node_lo + n];
-    for (int c = lane; c < KH; c += WAVE) out[p.node_cnt + c] = mom[c];
     return;
   }
   T part = T(0);
@@ -412,7 +411,7 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
   ws.h.ensure(sizeof(T) * n1 * S::F);
   ws.abar.ensure(sizeof(T) * n1 * S::KH);
   ws.gbar.ensure(sizeof(T) * n1 * S::KH);
-  ws.vg.ensure(sizeof(T) * n1 * (std::max(1, m.gp.node_cnt) + S::KH));
+  ws.vg.ensure(sizeof(T) * n1 * std::max(1, m.gp.node_cnt));
   ws.site.ensure(sizeof(T) * n1);
   ws.fw.ensure(sizeof(T) * n1 * 9);
   ws.gedge.ensure(sizeof(T) * ne * 3);

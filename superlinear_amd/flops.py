@@ -17,10 +17,8 @@ H = (L+1)^2, F = K(L+1), B radial basis size, Kw warp blocks, KH = K H.
   of the path survey: F_e^S = 11 + 8H + Kw (10B + 12) + 4FB + 11 KH + F + 3,
   F_e^E = F_e^S + 4FB + 13 KH + 2F.
 
-The gather-based backward evaluates every directed edge from both ends, so
-its executed work is about twice the algorithmic count; the roofline fraction
-reports achieved algorithmic work, so that overhead shows up as a lower
-fraction, not as inflated throughput.
+``pass_forward_*`` kernels also run the per-atom product graph, whose cost
+(a few flops per product row per atom) is not counted.
 """
 
 PEAK = {  # MI355X published peaks
@@ -57,10 +55,10 @@ def survey_efs_per_edge(d):
 
 def kernel_flops(name, d, nedges):
     """Algorithmic flops of one launch of the named kernel over nedges."""
-    if name.startswith("edge_moments_"):
+    if name.startswith("pass_forward_"):
         return moments_per_edge(d) * nedges
-    if name.startswith("edge_force_outer"):
+    if name.startswith("edge_backward_outer"):
         return force_per_edge(d, outer_effect=bool(d.get("gated"))) * nedges
-    if name.startswith("edge_force_gate"):
+    if name.startswith("edge_backward_gate"):
         return force_per_edge(d) * nedges
     return None
