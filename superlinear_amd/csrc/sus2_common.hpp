@@ -46,7 +46,7 @@ template <class T>
 void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st);
 template <class T>
 void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st);
-template <class T, int LL, int KK>
+template <class Cf>
 void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st);
 
 }  // namespace sus2

@@ -71,6 +71,31 @@ constexpr double dfact(int m) {  // (2m-1)!!
 }
 __host__ __device__ constexpr int hidx(int l, int m) { return l * l + l + m; }
 
+// Normalisations and (2m-1)!! for l <= 6, evaluated at compile time.
+struct YTable {
+  double n[49];
+  double df[7];
+};
+constexpr YTable make_ytable() {
+  YTable t{};
+  for (int l = 0; l <= 6; ++l)
+    for (int m = 0; m <= l; ++m) t.n[l * l + l + m] = ynorm(l, m);
+  for (int m = 0; m <= 6; ++m) t.df[m] = dfact(m);
+  return t;
+}
+inline constexpr YTable kYT = make_ytable();
+
+// Compile-time shape of a model: max l, channels K, radial basis size B and
+// warp-sharing mode SM (0 pair, 1 L, 2 K, 3 LK).
+template <class T_, int L_, int K_, int B_, int SM_>
+struct Cfg {
+  using T = T_;
+  static constexpr int LL = L_, KK = K_, BB = B_, SM = SM_;
+  static constexpr int L1 = LL + 1, H = L1 * L1, F = KK * L1, KH = KK * H, NM = (KH + 63) / 64;
+  static constexpr int NBLK = SM == 0 ? 1 : SM == 1 ? L1 : SM == 2 ? KK : F;
+  static constexpr int blk(int k, int l) { return SM == 2 ? k : SM == 1 ? l : SM == 0 ? 0 : k * L1 + l; }
+};
+
 // ------------------------------------------------------------------ harmonics
 // Real Y_lm(u) for a unit vector u, l <= LL, flat index l^2+l+m; Condon-
 // Shortley phase cancelled, Y_1{-1,0,1} ~ (y, z, x).  Associated Legendre
@@ -89,9 +114,9 @@ __device__ __forceinline__ void sh_values(T ux, T uy, T uz, T* Y) {
 #pragma unroll
     for (int l = m; l <= LL; ++l) {
       T p;
-      if (l == m) p = T(dfact(m));
+      if (l == m) p = T(kYT.df[m]);
       else p = (T(2 * l - 1) * uz * p1 - T(l + m - 1) * p2) * T(1.0 / (l - m));
-      const T n = T(ynorm(l, m));
+      const T n = T(kYT.n[hidx(l, m)]);
       Y[hidx(l, m)] = n * p * cm;
       if (m > 0) Y[hidx(l, -m)] = n * p * sm;
       p2 = p1;
@@ -121,15 +146,14 @@ __device__ __forceinline__ void sh_stream_grad(T ux, T uy, T uz, T inv_r, Fn&& f
 #pragma unroll
     for (int l = m; l <= LL; ++l) {
       T p, pz, pr;
-      if (l == m) { p = T(dfact(m)); pz = T(0); pr = T(0); }
+      if (l == m) { p = T(kYT.df[m]); pz = T(0); pr = T(0); }
       else {
         const T A = T(2 * l - 1), Bc = T(l + m - 1), inv = T(1.0 / (l - m));
         p = (A * uz * p1 - Bc * p2) * inv;
         pz = (A * (p1 + uz * z1) - Bc * z2) * inv;
         pr = (A * uz * r1 - Bc * (p2 + r2)) * inv;
       }
-      const T n = T(ynorm(l, m));
-      // grad of the Legendre polynomial in (x, y, z) at u
+      const T n = T(kYT.n[hidx(l, m)]);
       const T gpx = T(2) * ux * pr, gpy = T(2) * uy * pr, gpz = pz + T(2) * uz * pr;
       {
         const T S = n * p * cm;
@@ -153,61 +177,96 @@ __device__ __forceinline__ double dtanh(double x) { return tanh(x); }
 __device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
 
+// Per-workgroup copies (LDS) of the small per-species coefficient tables of
+// one pass, so per-edge lookups with the neighbour's species are LDS reads.
+template <class T>
+struct SpeciesTables {
+  T* slope;  // [NBLK][C*C]
+  T* shift;  // [NBLK][C*C]
+  T* x0;     // [C][F] centre factor: tau (outer) or gate type g (gate)
+  T* alpha;  // [C][F] (outer pass of an Effect model)
+};
+
+template <class Cf>
+__host__ __device__ constexpr int species_tables_size(int C) {
+  return 2 * Cf::NBLK * C * C + 2 * C * Cf::F;
+}
+
+// Fill the tables from the coefficient vector (all lanes of the workgroup).
+template <class Cf>
+__device__ void load_species_tables(const DModel<typename Cf::T>& m, bool gate, typename Cf::T* base,
+                                    SpeciesTables<typename Cf::T>& t) {
+  using T = typename Cf::T;
+  const int C = m.C, CC = C * C, F = Cf::F;
+  t.slope = base;
+  t.shift = base + Cf::NBLK * CC;
+  t.x0 = t.shift + Cf::NBLK * CC;
+  t.alpha = t.x0 + C * F;
+  const int so = (gate && m.o_gscal >= 0) ? m.o_gscal : m.o_scal;
+  for (int k = threadIdx.x; k < Cf::NBLK * CC; k += blockDim.x) {
+    const int b = k / CC, pq = k % CC;
+    t.slope[k] = m.theta[so + 2 * b * CC + pq];
+    t.shift[k] = m.theta[so + (2 * b + 1) * CC + pq];
+  }
+  for (int k = threadIdx.x; k < C * F; k += blockDim.x) {
+    const int s = k / F, mu = k % F;
+    t.x0[k] = gate ? m.gtype(s, mu) : m.tau(s, mu);
+    t.alpha[k] = (!gate && m.gated) ? m.alpha(s, mu) : T(0);
+  }
+}
+
 // Radial functions R_mu(r) (and dR/dr when DER) of one directed edge with
 // ordered species pair `pair` = t_i*C + t_j, for the gate or outer
 // coefficients.  phi_b = scaling T_b(tanh(s (r - t)/2)) (r - rc_l)^2, rc_0 =
-// rc, rc_{l>0} = rca; channels with l > 0 vanish for r >= rca.
-template <class T, int LL, int KK, bool DER>
-__device__ __forceinline__ void radial(const DModel<T>& m, bool gate, int pair, T r, T* R, T* dR) {
-  constexpr int L1 = LL + 1;
+// rc, rc_{l>0} = rca; channels with l > 0 vanish for r >= rca.  Warp blocks
+// and basis size are compile-time, so each block's tanh and Chebyshev
+// recurrence is evaluated once and only for the channels that share it.
+template <class Cf, bool DER>
+__device__ __forceinline__ void radial(const DModel<typename Cf::T>& m, bool gate, const SpeciesTables<typename Cf::T>& t,
+                                       int pair, typename Cf::T r, typename Cf::T* R, typename Cf::T* dR) {
+  using T = typename Cf::T;
+  constexpr int L1 = Cf::L1, BB = Cf::BB;
   const bool own_rad = gate && m.o_grad >= 0;
-  const T* coef = m.theta + (own_rad ? m.o_grad : m.o_rad);
-  const int stride = own_rad ? m.B : m.B + m.C;
-  const T* sc = m.theta + ((gate && m.o_gscal >= 0) ? m.o_gscal : m.o_scal);
+  const T* __restrict__ coef = m.theta + (own_rad ? m.o_grad : m.o_rad);
+  const int stride = own_rad ? BB : BB + m.C;
   const int CC = m.C * m.C;
-  T Tb[BMAX], Db[BMAX];
-  int prev = -1;
+  const T d0 = r - m.rc, f0 = d0 * d0, g0 = T(2) * d0;
+  const bool act = r < m.rca;
+  const T d1 = r - m.rca, f1 = act ? d1 * d1 : T(0), g1 = act ? T(2) * d1 : T(0);
 #pragma unroll
-  for (int k = 0; k < KK; ++k) {
+  for (int b = 0; b < Cf::NBLK; ++b) {
+    const T slope = t.slope[b * CC + pair], shift = t.shift[b * CC + pair];
+    const T xi = dtanh(slope * (r - shift) * T(0.5));
+    const T dxi = (T(1) - xi * xi) * slope * T(0.5);
+    T Tb[BB], Db[BB];
+    Tb[0] = m.scaling;
+    Db[0] = T(0);
+    if (BB > 1) {
+      Tb[1] = m.scaling * xi;
+      Db[1] = m.scaling * dxi;
+    }
 #pragma unroll
-    for (int l = 0; l < L1; ++l) {
-      const int mu = k * L1 + l;
-      if (l > 0 && r >= m.rca) {
-        R[mu] = T(0);
-        if (DER) dR[mu] = T(0);
-        continue;
-      }
-      const int blk = m.block(k, l);
-      if (blk != prev) {
-        prev = blk;
-        const T slope = sc[2 * blk * CC + pair], shift = sc[(2 * blk + 1) * CC + pair];
-        const T xi = dtanh(slope * (r - shift) * T(0.5));
-        const T dxi = (T(1) - xi * xi) * slope * T(0.5);
-        Tb[0] = m.scaling;
-        Tb[1] = m.scaling * xi;
-        Db[0] = T(0);
-        Db[1] = m.scaling * dxi;
+    for (int q = 2; q < BB; ++q) {
+      Tb[q] = T(2) * xi * Tb[q - 1] - Tb[q - 2];
+      if (DER) Db[q] = T(2) * (dxi * Tb[q - 1] + xi * Db[q - 1]) - Db[q - 2];
+    }
 #pragma unroll
-        for (int b = 2; b < BMAX; ++b) {
-          if (b < m.B) {
-            Tb[b] = T(2) * xi * Tb[b - 1] - Tb[b - 2];
-            if (DER) Db[b] = T(2) * (dxi * Tb[b - 1] + xi * Db[b - 1]) - Db[b - 2];
-          }
+    for (int k = 0; k < Cf::KK; ++k) {
+#pragma unroll
+      for (int l = 0; l < L1; ++l) {
+        if (Cf::blk(k, l) != b) continue;
+        const int mu = k * L1 + l;
+        const T* c = coef + mu * stride;
+        T s = T(0), sd = T(0);
+#pragma unroll
+        for (int q = 0; q < BB; ++q) {
+          s += c[q] * Tb[q];
+          if (DER) sd += c[q] * Db[q];
         }
+        const T f = l == 0 ? f0 : f1, g = l == 0 ? g0 : g1;
+        R[mu] = f * s;
+        if (DER) dR[mu] = g * s + f * sd;
       }
-      const T dr = r - (l == 0 ? m.rc : m.rca);
-      const T f = dr * dr;
-      const T* c = coef + mu * stride;
-      T s = T(0), sd = T(0);
-#pragma unroll
-      for (int b = 0; b < BMAX; ++b) {
-        if (b < m.B) {
-          s += c[b] * Tb[b];
-          if (DER) sd += c[b] * Db[b];
-        }
-      }
-      R[mu] = f * s;
-      if (DER) dR[mu] = T(2) * dr * s + f * sd;
     }
   }
 }

@@ -383,23 +383,24 @@ template void build_neighbors<double>(Workspace&, const Batch&, double, hipStrea
 template void finish_efs<float>(Workspace&, const Batch&, const EfsOut&, hipStream_t);
 template void finish_efs<double>(Workspace&, const Batch&, const EfsOut&, hipStream_t);
 
-#define SUS2_DECLARE(LL, KK)                                                                                \
-  extern template void run_efs_shape<float, LL, KK>(const DevModelSet&, Workspace&, const Batch&,          \
-                                                    const EfsOut&, hipStream_t);                           \
-  extern template void run_efs_shape<double, LL, KK>(const DevModelSet&, Workspace&, const Batch&,         \
-                                                     const EfsOut&, hipStream_t);
+#define SUS2_DECLARE(LL, KK, BB, SM)                                                                  \
+  extern template void run_efs_shape<Cfg<float, LL, KK, BB, SM>>(const DevModelSet&, Workspace&, const Batch&, \
+                                                                 const EfsOut&, hipStream_t);               \
+  extern template void run_efs_shape<Cfg<double, LL, KK, BB, SM>>(const DevModelSet&, Workspace&, const Batch&, \
+                                                                  const EfsOut&, hipStream_t);
 SUS2_FOR_EACH_SHAPE(SUS2_DECLARE)
 #undef SUS2_DECLARE
 
 template <class T>
 void run_efs(const DevModelSet& dm, Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st) {
-  const int L = dm.host->L, K = dm.host->K;
-  if (dm.host->B > BMAX) throw Error(SUS2_E_UNSUPPORTED, "radial basis size above 16");
-#define SUS2_SHAPE(LL, KK) \
-  if (L == LL && K == KK) return run_efs_shape<T, LL, KK>(dm, ws, b, out, st);
+  const int L = dm.host->L, K = dm.host->K, B = dm.host->B, SM = dm.host->smode;
+#define SUS2_SHAPE(LL, KK, BB, SMM) \
+  if (L == LL && K == KK && B == BB && SM == SMM) return run_efs_shape<Cfg<T, LL, KK, BB, SMM>>(dm, ws, b, out, st);
   SUS2_FOR_EACH_SHAPE(SUS2_SHAPE)
 #undef SUS2_SHAPE
-  throw Error(SUS2_E_UNSUPPORTED, "no kernel instance for L=" + std::to_string(L) + " K=" + std::to_string(K));
+  throw Error(SUS2_E_UNSUPPORTED, "no kernel instance for L=" + std::to_string(L) + " K=" + std::to_string(K) +
+                                      " B=" + std::to_string(B) + " scaling_map mode " + std::to_string(SM) +
+                                      " (instances are listed in SUS2_FOR_EACH_SHAPE)");
 }
 
 void efs_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const EfsOut& out,

@@ -29,11 +29,7 @@
 
 namespace sus2 {
 
-template <class T, int LL, int KK>
-struct Shape {
-  static constexpr int L1 = LL + 1, H = L1 * L1, F = KK * L1, KH = KK * H, NM = (KH + WAVE - 1) / WAVE;
-  static constexpr int LR = WAVE + 1;  // padded LDS staging row
-};
+constexpr int LR = WAVE + 1;  // padded LDS staging row
 
 __host__ __device__ __forceinline__ int l_of(int h) {
   int l = 0;
@@ -128,30 +124,33 @@ __host__ __device__ inline int graph_scratch(const DPass<T>& p) {
 }
 
 // ------------------------------------------------------------------ pass forward
-template <class T, int LL, int KK>
-size_t forward_lds(const DModel<T>& m) {
-  using S = Shape<T, LL, KK>;
+template <class Cf>
+size_t forward_lds(const DModel<typename Cf::T>& m) {
   const int sc = std::max(graph_scratch(m.gp), graph_scratch(m.op));
-  return sizeof(T) * ((size_t)(S::F + S::H) * S::LR + S::KH + 2 * (size_t)m.M + std::max(sc, 1));
+  return sizeof(typename Cf::T) * ((size_t)(Cf::F + Cf::H) * LR + Cf::KH + 2 * (size_t)m.M + std::max(sc, 1) +
+                                   species_tables_size<Cf>(m.C));
 }
 
-template <class T, int LL, int KK, bool GATE>
+template <class Cf, bool GATE, class T = typename Cf::T>
 __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_t* __restrict__ types,
                                                        const int* __restrict__ nbr_off,
                                                        const Rec<T>* __restrict__ rec, T* __restrict__ h,
                                                        T* __restrict__ vg, T* __restrict__ site,
                                                        T* __restrict__ abar) {
-  using S = Shape<T, LL, KK>;
-  constexpr int L1 = S::L1, H = S::H, F = S::F, KH = S::KH, NM = S::NM, LR = S::LR;
+  constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, H = Cf::H, F = Cf::F, KH = Cf::KH, NM = Cf::NM;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sR = reinterpret_cast<T*>(smem_raw);  // [F][LR]
   T* sY = sR + F * LR;                     // [H][LR]
   T* mom = sY + H * LR;                    // [KH]
   T* v = mom + KH;                         // [M]
   T* a = v + m.M;                          // [M]
-  T* scratch = a + m.M;
+  T* tabs_base = a + m.M;
+  T* scratch = tabs_base + species_tables_size<Cf>(m.C);
   const int i = blockIdx.x, lane = threadIdx.x;
   const int ti = types[i];
+  SpeciesTables<T> tab;
+  load_species_tables<Cf>(m, GATE, tabs_base, tab);
+  __syncthreads();
   const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
   T acc[NM];
 #pragma unroll
@@ -165,18 +164,16 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
       const T inv = T(1) / r;
       const int tj = types[j];
       T Rv[F], Y[H];
-      radial<T, LL, KK, false>(m, GATE, ti * m.C + tj, r, Rv, nullptr);
+      radial<Cf, false>(m, GATE, tab, ti * m.C + tj, r, Rv, nullptr);
       sh_values<T, LL>(q.x * inv, q.y * inv, q.z * inv, Y);
+      const T* x0i = tab.x0 + ti * F;
+      const T* x0j = tab.x0 + tj * F;
+      const T* alj = tab.alpha + tj * F;
 #pragma unroll
       for (int mu = 0; mu < F; ++mu) {
-        T w;
-        if (GATE) w = m.gtype(ti, mu) * m.gtype(tj, mu);
-        else {
-          T x1 = m.tau(tj, mu);
-          if (m.gated) x1 += m.alpha(tj, mu) * h[(size_t)j * F + mu];
-          w = m.tau(ti, mu) * x1;
-        }
-        sR[mu * LR + lane] = w * Rv[mu];
+        T x1 = x0j[mu];
+        if (!GATE && m.gated) x1 += alj[mu] * h[(size_t)j * F + mu];
+        sR[mu * LR + lane] = x0i[mu] * x1 * Rv[mu];
       }
 #pragma unroll
       for (int hh = 0; hh < H; ++hh) sY[hh * LR + lane] = Y[hh];
@@ -239,22 +236,23 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
 // outer pass of an Effect model the gate-adjoint message
 //   u_e,mu = tau_{t_i,mu} R_mu(r; t_i, t_j) sum_m abar_i[k,l,m] Y_lm(d_e),
 // which the neighbour j gathers into h_bar_j (times alpha_{t_j,mu}).
-template <class T, int LL, int KK, bool GATE>
+template <class Cf, bool GATE, class T = typename Cf::T>
 __global__ __launch_bounds__(WAVE) void k_edge_backward(DModel<T> m, const int32_t* __restrict__ types,
                                                         const int* __restrict__ nbr_off,
                                                         const Rec<T>* __restrict__ rec, const T* __restrict__ h,
                                                         const T* __restrict__ adj, T* __restrict__ gedge,
                                                         T* __restrict__ uedge, T* __restrict__ fw) {
-  using S = Shape<T, LL, KK>;
-  constexpr int L1 = S::L1, F = S::F, KH = S::KH;
-  __shared__ T sA[KH];
-  __shared__ T x0[F];
+  constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, F = Cf::F, KH = Cf::KH;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sA = reinterpret_cast<T*>(smem_raw);  // [KH]
+  SpeciesTables<T> tab;
   const int i = blockIdx.x, lane = threadIdx.x;
   const int ti = types[i];
   const bool hgate = !GATE && m.gated;
+  load_species_tables<Cf>(m, GATE, sA + KH, tab);
   for (int c = lane; c < KH; c += WAVE) sA[c] = adj[(size_t)i * KH + c];
-  for (int mu = lane; mu < F; mu += WAVE) x0[mu] = GATE ? m.gtype(ti, mu) : m.tau(ti, mu);
   __syncthreads();
+  const T* x0 = tab.x0 + ti * F;
   T w0 = T(0), w1 = T(0), w2 = T(0), w3 = T(0), w4 = T(0), w5 = T(0);
   const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
   for (int e = e0 + lane; e < e1; e += WAVE) {
@@ -266,15 +264,13 @@ __global__ __launch_bounds__(WAVE) void k_edge_backward(DModel<T> m, const int32
     const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
     const int tj = types[j];
     T R[F], dR[F], wt[F], us[F];
-    radial<T, LL, KK, true>(m, GATE, ti * m.C + tj, r, R, dR);
+    radial<Cf, true>(m, GATE, tab, ti * m.C + tj, r, R, dR);
+    const T* x0j = tab.x0 + tj * F;
+    const T* alj = tab.alpha + tj * F;
 #pragma unroll
     for (int mu = 0; mu < F; ++mu) {
-      T x1;
-      if (GATE) x1 = m.gtype(tj, mu);
-      else {
-        x1 = m.tau(tj, mu);
-        if (m.gated) x1 += m.alpha(tj, mu) * h[(size_t)j * F + mu];
-      }
+      T x1 = x0j[mu];
+      if (!GATE && m.gated) x1 += alj[mu] * h[(size_t)j * F + mu];
       wt[mu] = x0[mu] * x1;
       dR[mu] *= wt[mu];
       us[mu] = T(0);
@@ -313,23 +309,26 @@ __global__ __launch_bounds__(WAVE) void k_edge_backward(DModel<T> m, const int32
 }
 
 // ------------------------------------------------------------------ gather
-template <class T, int LL, int KK>
-size_t gather_lds(const DModel<T>& m) {
-  using S = Shape<T, LL, KK>;
-  return sizeof(T) * ((size_t)2 * m.M + S::KH + S::F + std::max(graph_scratch(m.gp), 1));
+template <class Cf>
+size_t gather_lds(const DModel<typename Cf::T>& m) {
+  return sizeof(typename Cf::T) * ((size_t)2 * m.M + Cf::KH + Cf::F + std::max(graph_scratch(m.gp), 1));
+}
+
+template <class Cf>
+size_t backward_lds(const DModel<typename Cf::T>& m) {
+  return sizeof(typename Cf::T) * ((size_t)Cf::KH + species_tables_size<Cf>(m.C));
 }
 
 // F_i = sum_{e in out(i)} g_e - sum_{e' in in(i)} g_e' ; the outer pass of an
 // Effect model also forms h_bar_i = alpha_{t_i} sum_{e' in in(i)} u_e' and runs
 // the gate graph reverse from the stored gate node values.
-template <class T, int LL, int KK, bool GATE>
+template <class Cf, bool GATE, class T = typename Cf::T>
 __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __restrict__ types,
                                                  const int* __restrict__ nbr_off, const int* __restrict__ in_edges,
                                                  const T* __restrict__ gedge, const T* __restrict__ uedge,
                                                  const T* __restrict__ vg, T* __restrict__ gbar,
                                                  T* __restrict__ fw) {
-  using S = Shape<T, LL, KK>;
-  constexpr int F = S::F, KH = S::KH;
+  constexpr int KK = Cf::KK, F = Cf::F, KH = Cf::KH;
   const int i = blockIdx.x, lane = threadIdx.x;
   const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
   const bool hgate = !GATE && m.gated;
@@ -389,9 +388,10 @@ __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __r
 }
 
 // ------------------------------------------------------------------ driver
-template <class T, int LL, int KK>
+template <class Cf>
 std::string kname(const char* base) {
-  return std::string(base) + "<" + tname<T>() + ",L" + std::to_string(LL) + "K" + std::to_string(KK) + ">";
+  return std::string(base) + "<" + tname<typename Cf::T>() + ",L" + std::to_string(Cf::LL) + "K" +
+         std::to_string(Cf::KK) + "B" + std::to_string(Cf::BB) + ">";
 }
 
 template <class K>
@@ -401,52 +401,46 @@ void allow_lds(K kern, size_t bytes) {
                                    (int)bytes));
 }
 
-template <class T, int LL, int KK>
+template <class Cf>
 void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st) {
-  using S = Shape<T, LL, KK>;
+  using T = typename Cf::T;
   const DModel<T>& m = dm.get<T>();
   const int N = b.natoms;
   build_neighbors<T>(ws, b, dm.host->rc, st);
   const size_t n1 = std::max(1, N), ne = std::max<long>(1, ws.nedges);
-  ws.h.ensure(sizeof(T) * n1 * S::F);
-  ws.abar.ensure(sizeof(T) * n1 * S::KH);
-  ws.gbar.ensure(sizeof(T) * n1 * S::KH);
+  ws.h.ensure(sizeof(T) * n1 * Cf::F);
+  ws.abar.ensure(sizeof(T) * n1 * Cf::KH);
+  ws.gbar.ensure(sizeof(T) * n1 * Cf::KH);
   ws.vg.ensure(sizeof(T) * n1 * std::max(1, m.gp.node_cnt));
   ws.site.ensure(sizeof(T) * n1);
   ws.fw.ensure(sizeof(T) * n1 * 9);
   ws.gedge.ensure(sizeof(T) * ne * 3);
-  if (m.gated) ws.uedge.ensure(sizeof(T) * ne * S::F);
+  if (m.gated) ws.uedge.ensure(sizeof(T) * ne * Cf::F);
   const int* off = ws.nbr_off.get<int>();
   const Rec<T>* rec = ws.rec.get<Rec<T>>();
   T* h = ws.h.get<T>();
   if (N > 0) {
-    const size_t lf = forward_lds<T, LL, KK>(m), lg = gather_lds<T, LL, KK>(m);
+    const size_t lf = forward_lds<Cf>(m), lg = gather_lds<Cf>(m), lb = backward_lds<Cf>(m);
     if (m.gated) {
-      allow_lds(k_pass_forward<T, LL, KK, true>, lf);
-      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("pass_forward_gate")), (k_pass_forward<T, LL, KK, true>), dim3(N),
-                  dim3(WAVE), lf, m, b.types, off, rec, h, ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
+      allow_lds(k_pass_forward<Cf, true>, lf);
+      SUS2_LAUNCH(ws, st, kname<Cf>("pass_forward_gate"), (k_pass_forward<Cf, true>), dim3(N), dim3(WAVE), lf, m,
+                  b.types, off, rec, h, ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
     }
-    allow_lds(k_pass_forward<T, LL, KK, false>, lf);
-    SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("pass_forward_outer")), (k_pass_forward<T, LL, KK, false>), dim3(N),
-                dim3(WAVE), lf, m, b.types, off, rec, h, ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
-    SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("edge_backward_outer")), (k_edge_backward<T, LL, KK, false>), dim3(N),
-                dim3(WAVE), 0, m, b.types, off, rec, h, ws.abar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(),
-                ws.fw.get<T>());
+    allow_lds(k_pass_forward<Cf, false>, lf);
+    SUS2_LAUNCH(ws, st, kname<Cf>("pass_forward_outer"), (k_pass_forward<Cf, false>), dim3(N), dim3(WAVE), lf, m,
+                b.types, off, rec, h, ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
+    SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_outer"), (k_edge_backward<Cf, false>), dim3(N), dim3(WAVE), lb, m,
+                b.types, off, rec, h, ws.abar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
+    allow_lds(k_gather<Cf, false>, lg);
+    SUS2_LAUNCH(ws, st, kname<Cf>("gather_outer"), (k_gather<Cf, false>), dim3(N), dim3(WAVE), m.gated ? lg : 0, m,
+                b.types, off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
+                ws.gbar.get<T>(), ws.fw.get<T>());
     if (m.gated) {
-      allow_lds(k_gather<T, LL, KK, false>, lg);
-      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("gather_outer")), (k_gather<T, LL, KK, false>), dim3(N), dim3(WAVE), lg,
-                  m, b.types, off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
-                  ws.gbar.get<T>(), ws.fw.get<T>());
-      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("edge_backward_gate")), (k_edge_backward<T, LL, KK, true>), dim3(N),
-                  dim3(WAVE), 0, m, b.types, off, rec, h, ws.gbar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(),
+      SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_gate"), (k_edge_backward<Cf, true>), dim3(N), dim3(WAVE), lb, m,
+                  b.types, off, rec, h, ws.gbar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
+      SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(N), dim3(WAVE), 0, m, b.types, off,
+                  ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>(),
                   ws.fw.get<T>());
-      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("gather_gate")), (k_gather<T, LL, KK, true>), dim3(N), dim3(WAVE), 0, m,
-                  b.types, off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
-                  ws.gbar.get<T>(), ws.fw.get<T>());
-    } else {
-      SUS2_LAUNCH(ws, st, (kname<T, LL, KK>("gather_outer")), (k_gather<T, LL, KK, false>), dim3(N), dim3(WAVE), 0,
-                  m, b.types, off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
-                  ws.gbar.get<T>(), ws.fw.get<T>());
     }
   }
   finish_efs<T>(ws, b, out, st);

@@ -1,0 +1,9 @@
+// EFS kernel instances for L=1, K=2, B=3, warp sharing mode 3.
+#include "sus2_efs_impl.hpp"
+
+namespace sus2 {
+template void run_efs_shape<Cfg<float, 1, 2, 3, 3>>(const DevModelSet&, Workspace&, const Batch&,
+                                                          const EfsOut&, hipStream_t);
+template void run_efs_shape<Cfg<double, 1, 2, 3, 3>>(const DevModelSet&, Workspace&, const Batch&,
+                                                           const EfsOut&, hipStream_t);
+}  // namespace sus2

@@ -1,0 +1,9 @@
+// EFS kernel instances for L=3, K=4, B=10, warp sharing mode 2.
+#include "sus2_efs_impl.hpp"
+
+namespace sus2 {
+template void run_efs_shape<Cfg<float, 3, 4, 10, 2>>(const DevModelSet&, Workspace&, const Batch&,
+                                                          const EfsOut&, hipStream_t);
+template void run_efs_shape<Cfg<double, 3, 4, 10, 2>>(const DevModelSet&, Workspace&, const Batch&,
+                                                           const EfsOut&, hipStream_t);
+}  // namespace sus2

@@ -10,8 +10,9 @@
 #include "sus2_device.hpp"
 #include "sus2_model.hpp"
 
-// (L, K) shapes with compiled kernel instances
-#define SUS2_FOR_EACH_SHAPE(X) X(1, 2) X(2, 2) X(3, 4) X(4, 6) X(5, 6)
+// Model shapes with compiled kernel instances: (L, K, radial basis size B,
+// warp sharing 0 pair / 1 L / 2 K / 3 LK).  Adding a shape is one entry here.
+#define SUS2_FOR_EACH_SHAPE(X) X(1, 2, 3, 3) X(3, 4, 3, 3) X(3, 4, 10, 2) X(4, 6, 10, 2) X(5, 6, 10, 2)
 
 namespace sus2 {
 
