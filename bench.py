@@ -67,6 +67,24 @@ def cpu_baseline(model_path, frame, threads, budget_s=20.0):
                 sample=f"{copies} copies of the {n}-atom frame, FP64 oracle, OpenMP over frames, {dt:.1f} s")
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, each scaled by
+    the copy-kernel calibration) from the committed rocprofv3 counter summary,
+    or None when the summary does not cover this kernel."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_effect4096.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    import re
+    m = re.search(r"```json\n(.*?)\n```", text, re.S)
+    if not m:
+        return None
+    table = json.loads(m.group(1)).get("bytes_per_launch", {})
+    v = table.get(kernel)
+    return None if v is None else int(v)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +92,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the timed runs, launch a plain FP32 copy kernel (y = 2x) over this many MiB, "
+                         "three times: the calibration dispatch for rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (the path runs on one MI355X); --gpus {args.gpus} is not run",
@@ -138,6 +159,17 @@ def main():
                         unit="TFLOP/s", frac=round(ach / PEAK["fp32_tflops"], 5), traffic=None,
                         flops_per_launch=fl, avg_launch_us=round(avg_s * 1e6, 2))
     survey_rate = survey_efs_per_edge(dims) * nedges * args.steps / t32 / 1e12
+    if roof is not None:
+        roof["traffic"] = measured_traffic(roof["kernel"])
+
+    if args.copy_probe_mib > 0:
+        x = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 2.0, out=y)
+        torch.cuda.synchronize()
+        print(f"[bench] copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written per launch",
+              file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC,

@@ -108,7 +108,7 @@ int sus2_loss_grad(sus2_model* m, int precision, int nframes, int natoms, const 
  * Replaces the reference's env-gated wall-clock stage timers
  * (SUS2_SH_ACCUM_PROFILE, dev_src/mtpr_sh.cpp:1855-1957; pair_style
  * profile_every, interfaces/lammps/ML-SUS2/pair_sus2_embedded.cpp:320-361)
- * with HIP events recorded around every kernel launch on its stream.
+ * with HIP events attached to every kernel dispatch on its stream.
  * sus2_profile_read waits for pending events and fills up to `cap` entries:
  * names (cap * 64 chars, NUL-terminated), total milliseconds and launch
  * counts; *n receives the number of distinct kernels. */

@@ -2,6 +2,7 @@
 // macro with the optional per-kernel HIP-event profiler.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <string>
 
@@ -30,12 +31,19 @@ __device__ __forceinline__ double rec_jbits(int j, double) { return __longlong_a
   } while (0)
 
 // Launch with per-kernel event timing when the workspace profiler is on.
-#define SUS2_LAUNCH(ws, st, name, kern, grid, block, lds, ...)   \
-  do {                                                           \
-    const int ev_ = (ws).prof.begin((name), (st));               \
-    hipLaunchKernelGGL(kern, grid, block, lds, st, __VA_ARGS__); \
-    SUS2_CHECK(hipGetLastError());                               \
-    (ws).prof.end(ev_, (st));                                    \
+// Launch a kernel; with the profiler on, its start/stop events are attached to
+// the dispatch packet itself (hipExtLaunchKernel), so the timings are the
+// kernel's own execution span rather than that of marker packets around it.
+#define SUS2_LAUNCH(ws, st, name, kern, grid, block, lds, ...)                                      \
+  do {                                                                                              \
+    const int ev_ = (ws).prof.begin((name));                                                        \
+    if (ev_ < 0) {                                                                                  \
+      hipLaunchKernelGGL(kern, grid, block, lds, st, __VA_ARGS__);                                  \
+    } else {                                                                                        \
+      hipExtLaunchKernelGGL(kern, grid, block, lds, st, (ws).prof.start_event(ev_),                 \
+                            (ws).prof.stop_event(ev_), 0, __VA_ARGS__);                             \
+    }                                                                                               \
+    SUS2_CHECK(hipGetLastError());                                                                  \
   } while (0)
 
 template <class T> constexpr const char* tname();

@@ -32,7 +32,7 @@ Profiler::~Profiler() {
   for (auto e : pool) (void)hipEventDestroy(e);
 }
 
-int Profiler::begin(const std::string& name, hipStream_t st) {
+int Profiler::begin(const std::string& name) {
   if (!on) return -1;
   int id = -1;
   for (size_t k = 0; k < names.size(); ++k)
@@ -48,14 +48,8 @@ int Profiler::begin(const std::string& name, hipStream_t st) {
     if (!pool.empty()) { e = pool.back(); pool.pop_back(); }
     else RT_CHECK(hipEventCreate(&e));
   }
-  RT_CHECK(hipEventRecord(ev[0], st));
   pending.push_back({id, ev[0], ev[1]});
   return (int)pending.size() - 1;
-}
-
-void Profiler::end(int idx, hipStream_t st) {
-  if (idx < 0) return;
-  RT_CHECK(hipEventRecord(pending[idx].b, st));
 }
 
 void Profiler::collect_ready() {

@@ -52,8 +52,10 @@ struct Profiler {
   std::vector<Pending> pending;
   std::vector<hipEvent_t> pool;
   ~Profiler();
-  int begin(const std::string& name, hipStream_t st);
-  void end(int idx, hipStream_t st);
+  // returns -1 when off, else the index of a fresh start/stop event pair
+  int begin(const std::string& name);
+  hipEvent_t start_event(int idx) const { return pending[idx].a; }
+  hipEvent_t stop_event(int idx) const { return pending[idx].b; }
   void collect_ready();  // accumulate completed pairs (non-blocking)
   void collect_all();    // wait for and accumulate every pending pair
   void reset();
