@@ -49,6 +49,8 @@ def lib():
         L.orc_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_int, _lp, _ip, _dp, _dp, _ip, _dp, _dp, _dp,
                                     ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp,
                                     _dp, _dp, _dp, ctypes.c_int]
+        L.orc_efs_grad.argtypes = [ctypes.c_void_p, ctypes.c_int, _lp, _ip, _dp, _dp, _ip, _dp, _dp, _dp, _dp,
+                                   ctypes.c_int]
         L.orc_loss.argtypes = [ctypes.c_void_p, ctypes.c_int, _lp, _ip, _dp, _dp, _ip, _dp, _dp, _dp,
                                ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         _LIB = L
@@ -134,6 +136,21 @@ class OracleModel:
                                    _ptr(S0, _dp), mode, wE, wF, wS, _ptr(loss, _dp), _ptr(grad, _dp),
                                    _ptr(eb, _dp), _ptr(fb, _dp), _ptr(sb, _dp), nthreads))
         return dict(loss=float(loss[0]), grad=grad, ebar=eb, fbar=fb, sbar=sb)
+
+    def efs_grad(self, frames, ene_w, frc_w, str_w, nthreads=1):
+        """sum over frames of sum_i ene_w_i dE_i/dtheta + d/dtheta (sum_e w_e . dE/dd_e),
+        w_e = frc_w_i - frc_w_j + str_w(frame) d_e (str_w row-major 3x3 per frame)."""
+        b = pack(frames)
+        nf = len(frames)
+        n = int(b["off"][-1])
+        ew = np.ascontiguousarray(ene_w, np.float64).reshape(n)
+        fw = np.ascontiguousarray(frc_w, np.float64).reshape(n, 3)
+        sw = np.ascontiguousarray(str_w, np.float64).reshape(nf, 9)
+        grad = np.zeros(lib().orc_coeff_count(self.h))
+        _check(lib().orc_efs_grad(self.h, nf, _ptr(b["off"], _lp), _ptr(b["types"], _ip), _ptr(b["pos"], _dp),
+                                  _ptr(b["cell"], _dp), _ptr(b["periodic"], _ip), _ptr(ew, _dp), _ptr(fw, _dp),
+                                  _ptr(sw, _dp), _ptr(grad, _dp), nthreads))
+        return grad
 
     def loss(self, frames, labels, mode=1, wE=1.0, wF=0.01, wS=0.0):
         b = pack(frames)

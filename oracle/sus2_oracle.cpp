@@ -684,6 +684,42 @@ double frame_loss(int n, int mode, double wE, double wF, double wS, double E, co
   return loss;
 }
 
+// Parameter gradient of one frame:  sum_i dEi_i dE_i/dtheta + d/dtheta of
+// the directional derivative of E along w_e = dF_i - dF_j + dS d_e for every
+// directed edge e = (i -> j)  (src/basic_mlip.cpp:421-446).  Hyper-dual
+// numbers carry the coefficient direction in e1 and w in e2, one coefficient
+// at a time.  out[ncoeff] is overwritten.
+void frame_grad(const Model& m, int n, const int* type, const std::vector<Edge>& e, const double* dEi,
+                const double* dF, const double* dS, double* out, int nthreads) {
+  const int P = m.ncoeff;
+  std::vector<TEdge<HD>> te(e.size());
+  for (size_t k = 0; k < e.size(); ++k) {
+    const int i = e[k].i, j = e[k].j;
+    te[k].i = i; te[k].j = j;
+    for (int q = 0; q < 3; ++q) {
+      double w = dF[3 * i + q] - dF[3 * j + q];
+      for (int b = 0; b < 3; ++b) w += dS[3 * q + b] * e[k].d[b];
+      te[k].d[q] = HD(e[k].d[q], 0, w, 0);
+    }
+  }
+  // coefficients are independent forward passes: spread them over threads
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+  {
+    std::vector<HD> th(P);
+    for (int p = 0; p < P; ++p) th[p] = HD(m.theta[p]);
+    std::vector<HD> site;
+#pragma omp for schedule(dynamic, 4)
+    for (int p = 0; p < P; ++p) {
+      th[p].b = 1.0;
+      forward<HD>(m, th.data(), n, type, te, site);
+      double g = 0;
+      for (int i = 0; i < n; ++i) g += dEi[i] * site[i].b + site[i].d;
+      out[p] = g;
+      th[p].b = 0.0;
+    }
+  }
+}
+
 }  // namespace orc
 
 // ================================================================= C ABI
@@ -812,7 +848,7 @@ int orc_loss_grad(void* h, int nframes, const long* off, const int* type, const 
     std::vector<double> Lf(nframes, 0.0);
     int bad = 0;
     std::string err;
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
+    // frames in order; frame_grad spreads the coefficients over the threads
     for (int f = 0; f < nframes; ++f) {
       try {
         const long a = off[f];
@@ -829,28 +865,7 @@ int orc_loss_grad(void* h, int nframes, const long* off, const int* type, const 
         if (ebar_out) for (int i = 0; i < n; ++i) ebar_out[a + i] = dEi[i];
         if (fbar_out) for (int i = 0; i < 3 * n; ++i) fbar_out[3 * a + i] = dF[i];
         if (sbar_out) for (int q = 0; q < 9; ++q) sbar_out[9 * f + q] = dS[q];
-        // edges carrying the coordinate tangent w_e in the e2 slot
-        std::vector<TEdge<HD>> te(e.size());
-        for (size_t k = 0; k < e.size(); ++k) {
-          const int i = e[k].i, j = e[k].j;
-          te[k].i = i; te[k].j = j;
-          for (int q = 0; q < 3; ++q) {
-            double w = dF[3 * i + q] - dF[3 * j + q];
-            for (int b = 0; b < 3; ++b) w += dS[3 * q + b] * e[k].d[b];
-            te[k].d[q] = HD(e[k].d[q], 0, w, 0);
-          }
-        }
-        std::vector<HD> th(P);
-        for (int p = 0; p < P; ++p) th[p] = HD(m->theta[p]);
-        std::vector<HD> site;
-        for (int p = 0; p < P; ++p) {
-          th[p].b = 1.0;
-          forward<HD>(*m, th.data(), n, type + a, te, site);
-          double g = 0;
-          for (int i = 0; i < n; ++i) g += dEi[i] * site[i].b + site[i].d;
-          G[(size_t)f * P + p] = g;
-          th[p].b = 0.0;
-        }
+        frame_grad(*m, n, type + a, e, dEi.data(), dF.data(), dS, G.data() + (size_t)f * P, nthreads);
       } catch (const std::exception& ex) {
 #pragma omp critical
         { bad = 1; err = ex.what(); }
@@ -864,6 +879,40 @@ int orc_loss_grad(void* h, int nframes, const long* off, const int* type, const 
       double s = 0;
       for (int f = 0; f < nframes; ++f) s += G[(size_t)f * P + p];
       grad[p] = s / nframes;
+    }
+  });
+}
+
+// Parameter gradient with caller-given weights (AccumulateEFSCombinationGrad
+// over a batch, dev_src/mtpr.cpp:9188): grad[p] = sum over frames of
+// frame_grad with ene_w [natoms], frc_w [natoms*3], str_w [nframes*9].
+int orc_efs_grad(void* h, int nframes, const long* off, const int* type, const double* x, const double* cell,
+                 const int* periodic, const double* ene_w, const double* frc_w, const double* str_w, double* grad,
+                 int nthreads) {
+  auto* m = static_cast<Model*>(h);
+  return guard([&] {
+    const int P = m->ncoeff;
+    std::vector<double> G((size_t)nframes * P, 0.0);
+    int bad = 0;
+    std::string err;
+    // frames in order; frame_grad spreads the coefficients over the threads
+    for (int f = 0; f < nframes; ++f) {
+      try {
+        const long a = off[f];
+        const int n = (int)(off[f + 1] - a);
+        auto e = neighbours(n, x + 3 * a, cell + 9 * f, periodic[f] != 0, m->rc);
+        frame_grad(*m, n, type + a, e, ene_w + a, frc_w + 3 * a, str_w + 9 * f, G.data() + (size_t)f * P,
+                   nthreads);
+      } catch (const std::exception& ex) {
+#pragma omp critical
+        { bad = 1; err = ex.what(); }
+      }
+    }
+    if (bad) fail(err);
+    for (int p = 0; p < P; ++p) {
+      double s = 0;
+      for (int f = 0; f < nframes; ++f) s += G[(size_t)f * P + p];
+      grad[p] = s;
     }
   });
 }

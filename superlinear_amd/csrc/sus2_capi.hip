@@ -229,7 +229,23 @@ int sus2_efs_grad(sus2_model* m, int precision, int nframes, int natoms, const i
     ensure_device(m);
     sus2::grad_dispatch(m->dev, m->ws, precision,
                         make_batch(nframes, natoms, frame_off, types, pos, cell, periodic), ene_weight, frc_weight,
-                        str_weight, grad, (hipStream_t)stream);
+                        str_weight, grad, 1.0, true, false, (hipStream_t)stream);
+  });
+}
+
+int sus2_loss_grad(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                   const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+                   const double* e0, const double* f0, const double* s0, int weighting, double wE, double wF,
+                   double wS, double* loss_out, double* grad, void* stream) {
+  return guard([&] {
+    if (!m) throw sus2::Error(SUS2_E_INVALID, "null model");
+    check_batch(m, nframes, natoms, frame_off, types, pos, cell, periodic);
+    if (!loss_out || !grad) throw sus2::Error(SUS2_E_INVALID, "null loss or gradient buffer");
+    if (nframes > 0 && (!e0 || !s0 || (natoms > 0 && !f0))) throw sus2::Error(SUS2_E_INVALID, "null label buffer");
+    ensure_device(m);
+    sus2::loss_grad_dispatch(m->dev, m->ws, precision,
+                             make_batch(nframes, natoms, frame_off, types, pos, cell, periodic), e0, f0, s0,
+                             weighting, wE, wF, wS, loss_out, grad, (hipStream_t)stream);
   });
 }
 

@@ -24,6 +24,17 @@ __device__ __forceinline__ int rec_j(const Rec<double>& r) { return (int)__doubl
 __device__ __forceinline__ float rec_jbits(int j, float) { return __int_as_float(j); }
 __device__ __forceinline__ double rec_jbits(int j, double) { return __longlong_as_double((long long)j); }
 
+// Frame of atom i: the largest f with frame_off[f] <= i (binary search).
+__device__ __forceinline__ int frame_of(const int64_t* off, int nframes, int i) {
+  int lo = 0, hi = nframes;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= i) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 #define SUS2_CHECK(x)                                                                                 \
   do {                                                                                                \
     hipError_t e_ = (x);                                                                              \
@@ -56,5 +67,17 @@ template <class T>
 void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st);
 template <class Cf>
 void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st);
+
+// Weights of the EFS combination whose parameter gradient is taken (device).
+struct GradWeights {
+  const double* ene_w;  // [N]
+  const double* frc_w;  // [N*3]
+  const double* str_w;  // [nf*9] row-major
+  const int64_t* frame_off;
+  int nframes;
+};
+template <class Cf>
+void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const GradWeights& gw, double* grad,
+                    double scale, bool add, bool reuse_neighbors, hipStream_t st);
 
 }  // namespace sus2

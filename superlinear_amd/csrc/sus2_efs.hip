@@ -56,16 +56,6 @@ struct Grid {
   int per;
 };
 
-__device__ __forceinline__ int frame_of(const int64_t* off, int nframes, int i) {
-  int lo = 0, hi = nframes;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (off[mid] <= i) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
 __global__ void k_grid_setup(int nframes, const int64_t* __restrict__ frame_off, const double* __restrict__ pos,
                              const double* __restrict__ cell, const int32_t* __restrict__ periodic, double rc,
                              Grid* __restrict__ grid) {

@@ -394,8 +394,13 @@ std::string kname(const char* base) {
          std::to_string(Cf::KK) + "B" + std::to_string(Cf::BB) + ">";
 }
 
+constexpr size_t LDS_MAX = 160 * 1024;  // LDS of one CDNA4 compute unit
+
 template <class K>
 void allow_lds(K kern, size_t bytes) {
+  if (bytes > LDS_MAX)
+    throw Error(SUS2_E_UNSUPPORTED, "model needs " + std::to_string(bytes) + " bytes of LDS per workgroup (limit " +
+                                        std::to_string(LDS_MAX) + ")");
   if (bytes > 64 * 1024)
     SUS2_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)bytes));

@@ -1,9 +1,13 @@
-// EFS kernel instances for L=3, K=4, B=3, warp sharing mode 3.
-#include "sus2_efs_impl.hpp"
+// EFS and parameter-gradient kernel instances for L=3, K=4, B=3, warp sharing mode 3.
+#include "sus2_grad_impl.hpp"
 
 namespace sus2 {
 template void run_efs_shape<Cfg<float, 3, 4, 3, 3>>(const DevModelSet&, Workspace&, const Batch&,
                                                           const EfsOut&, hipStream_t);
 template void run_efs_shape<Cfg<double, 3, 4, 3, 3>>(const DevModelSet&, Workspace&, const Batch&,
                                                            const EfsOut&, hipStream_t);
+template void run_grad_shape<Cfg<float, 3, 4, 3, 3>>(const DevModelSet&, Workspace&, const Batch&, const GradWeights&,
+                                              double*, double, bool, bool, hipStream_t);
+template void run_grad_shape<Cfg<double, 3, 4, 3, 3>>(const DevModelSet&, Workspace&, const Batch&, const GradWeights&,
+                                              double*, double, bool, bool, hipStream_t);
 }  // namespace sus2

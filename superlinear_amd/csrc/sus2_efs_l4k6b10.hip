@@ -1,9 +1,13 @@
-// EFS kernel instances for L=4, K=6, B=10, warp sharing mode 2.
-#include "sus2_efs_impl.hpp"
+// EFS and parameter-gradient kernel instances for L=4, K=6, B=10, warp sharing mode 2.
+#include "sus2_grad_impl.hpp"
 
 namespace sus2 {
 template void run_efs_shape<Cfg<float, 4, 6, 10, 2>>(const DevModelSet&, Workspace&, const Batch&,
                                                           const EfsOut&, hipStream_t);
 template void run_efs_shape<Cfg<double, 4, 6, 10, 2>>(const DevModelSet&, Workspace&, const Batch&,
                                                            const EfsOut&, hipStream_t);
+template void run_grad_shape<Cfg<float, 4, 6, 10, 2>>(const DevModelSet&, Workspace&, const Batch&, const GradWeights&,
+                                              double*, double, bool, bool, hipStream_t);
+template void run_grad_shape<Cfg<double, 4, 6, 10, 2>>(const DevModelSet&, Workspace&, const Batch&, const GradWeights&,
+                                              double*, double, bool, bool, hipStream_t);
 }  // namespace sus2

@@ -1,18 +1,165 @@
-// Parameter adjoint (forward-over-reverse) of the SUS2-SH EFS path.
+// Parameter gradient and training objective of the SUS2-SH path: shape
+// dispatch of the forward-over-reverse kernels (sus2_grad_impl.hpp) and the
+// per-frame loss with its adjoints (dev_src/non_linear_regression.cpp:936-1065).
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+#include <string>
+
 #include "../../include/sus2amd.h"
-#include "sus2_runtime.hpp"
+#include "sus2_common.hpp"
 
 namespace sus2 {
-void grad_dispatch(const DevModelSet&, Workspace&, int, const Batch&, const double*, const double*, const double*,
-                   double*, hipStream_t) {
-  throw Error(SUS2_E_UNSUPPORTED, "parameter gradient kernels are not built yet");
-}
-}  // namespace sus2
 
-extern "C" int sus2_loss_grad(sus2_model*, int, int, int, const int64_t*, const int32_t*, const double*,
-                              const double*, const int32_t*, const double*, const double*, const double*, int, double,
-                              double, double, double*, double*, void*) {
-  return SUS2_E_UNSUPPORTED;
+#define SUS2_DECLARE_GRAD(LL, KK, BB, SM)                                                                         \
+  extern template void run_grad_shape<Cfg<float, LL, KK, BB, SM>>(const DevModelSet&, Workspace&, const Batch&,  \
+                                                                  const GradWeights&, double*, double, bool, bool, \
+                                                                  hipStream_t);                                   \
+  extern template void run_grad_shape<Cfg<double, LL, KK, BB, SM>>(const DevModelSet&, Workspace&, const Batch&, \
+                                                                   const GradWeights&, double*, double, bool,     \
+                                                                   bool, hipStream_t);
+SUS2_FOR_EACH_SHAPE(SUS2_DECLARE_GRAD)
+#undef SUS2_DECLARE_GRAD
+
+template <class T>
+void run_grad(const DevModelSet& dm, Workspace& ws, const Batch& b, const GradWeights& gw, double* grad,
+              double scale, bool add, bool reuse, hipStream_t st) {
+  const int L = dm.host->L, K = dm.host->K, B = dm.host->B, SM = dm.host->smode;
+#define SUS2_SHAPE(LL, KK, BB, SMM)                                                                       \
+  if (L == LL && K == KK && B == BB && SM == SMM)                                                         \
+    return run_grad_shape<Cfg<T, LL, KK, BB, SMM>>(dm, ws, b, gw, grad, scale, add, reuse, st);
+  SUS2_FOR_EACH_SHAPE(SUS2_SHAPE)
+#undef SUS2_SHAPE
+  throw Error(SUS2_E_UNSUPPORTED, "no kernel instance for L=" + std::to_string(L) + " K=" + std::to_string(K) +
+                                      " B=" + std::to_string(B) + " scaling_map mode " + std::to_string(SM));
 }
+
+void grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* ene_w,
+                   const double* frc_w, const double* str_w, double* grad, double scale, bool add,
+                   bool reuse_neighbors, hipStream_t st) {
+  const GradWeights gw{ene_w, frc_w, str_w, b.frame_off, b.nframes};
+  if (precision == SUS2_FP64) run_grad<double>(dm, ws, b, gw, grad, scale, add, reuse_neighbors, st);
+  else if (precision == SUS2_FP32) run_grad<float>(dm, ws, b, gw, grad, scale, add, reuse_neighbors, st);
+  else throw Error(SUS2_E_INVALID, "precision must be SUS2_FP64 or SUS2_FP32");
+  ws.prof.collect_ready();
+}
+
+// ------------------------------------------------------------------ loss
+// One 256-thread block per frame, FP64, fixed reduction order.  Follows
+// NonLinearRegression::AddLoss / AddLossGrad with norm_by_forces = 0 and the
+// L2 force residual: weighting 0 vibrations, 1 structures (energy, force and
+// stress weights / N), 2 molecules (energy and stress weights * N).  Writes the
+// adjoints handed to the model: ene_w_i = dL/dE (same for every atom of the
+// frame), frc_w_i = dL/dF_i, str_w = -2 w_S (S - S0) / N (the reference's
+// sign convention, non_linear_regression.cpp:1027), and the frame loss.
+__global__ __launch_bounds__(256) void k_loss_frames(int nframes, const int64_t* __restrict__ frame_off,
+                                                     const double* __restrict__ E, const double* __restrict__ F,
+                                                     const double* __restrict__ V, const double* __restrict__ e0,
+                                                     const double* __restrict__ f0, const double* __restrict__ s0,
+                                                     int mode, double wE, double wF, double wS,
+                                                     double* __restrict__ ene_w, double* __restrict__ frc_w,
+                                                     double* __restrict__ str_w, double* __restrict__ loss_f) {
+  __shared__ double red[256];
+  const int f = blockIdx.x, t = threadIdx.x;
+  const long b = frame_off[f], e = frame_off[f + 1];
+  const int n = (int)(e - b);
+  double w = wF;
+  if (mode == 1 && n > 0) w /= n;
+  double fl = 0;
+  for (long i = b + t; i < e; i += 256)
+    for (int a = 0; a < 3; ++a) {
+      const double d = F[3 * i + a] - f0[3 * i + a];
+      if (wF != 0) {
+        fl += d * d;
+        frc_w[3 * i + a] = w * 2 * d;
+      } else {
+        frc_w[3 * i + a] = 0;
+      }
+    }
+  red[t] = fl;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) red[t] += red[t + s];
+    __syncthreads();
+  }
+  double we = wE, ws = wS;
+  if (mode == 1 && n > 0) { we /= n; ws /= n; }
+  else if (mode == 2) { we *= n; ws *= n; }
+  const double dE = (we != 0 && n > 0) ? 2.0 * we * (E[f] - e0[f]) / n : 0.0;
+  for (long i = b + t; i < e; i += 256) ene_w[i] = dE;
+  if (t == 0) {
+    double loss = w * red[0];
+    // PlusStress as a full matrix from Voigt xx yy zz xy xz yz
+    const int vi[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};
+    double fro = 0;
+    for (int q = 0; q < 9; ++q) {
+      const double d = V[6 * f + vi[q]] - s0[6 * f + vi[q]];
+      fro += d * d;
+      str_w[9 * f + q] = (ws != 0 && n > 0) ? -2.0 * ws * d / n : 0.0;
+    }
+    if (ws != 0 && n > 0) loss += ws * fro / n;
+    if (we != 0 && n > 0) loss += we * (E[f] - e0[f]) * (E[f] - e0[f]) / n;
+    loss_f[f] = loss;
+  }
+}
+
+// out[0] = (1/nframes) sum_f loss_f (fixed order).
+__global__ __launch_bounds__(256) void k_loss_sum(int nframes, const double* __restrict__ loss_f,
+                                                  double* __restrict__ out) {
+  __shared__ double red[256];
+  const int t = threadIdx.x;
+  double s = 0;
+  for (int f = t; f < nframes; f += 256) s += loss_f[f];
+  red[t] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) out[0] = nframes > 0 ? red[0] / nframes : 0.0;
+}
+
+__global__ void k_fill(int n, double v, double* __restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+void loss_grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* e0,
+                        const double* f0, const double* s0, int mode, double wE, double wF, double wS,
+                        double* loss_host, double* grad, hipStream_t st) {
+  if (mode < 0 || mode > 2) throw Error(SUS2_E_INVALID, "weighting must be 0 (vibrations), 1 (structures) or 2 (molecules)");
+  const int nf = b.nframes, N = b.natoms;
+  if (nf == 0) {  // empty batch: zero objective and gradient
+    const int P = dm.host->ncoeff;
+    SUS2_LAUNCH(ws, st, "fill", k_fill, dim3((P + 255) / 256), dim3(256), 0, P, 0.0, grad);
+    *loss_host = 0.0;
+    return;
+  }
+  const size_t n1 = std::max(1, N), f1 = std::max(1, nf);
+  ws.lE.ensure(sizeof(double) * f1);
+  ws.lF.ensure(sizeof(double) * 3 * n1);
+  ws.lV.ensure(sizeof(double) * 6 * f1);
+  ws.lew.ensure(sizeof(double) * n1);
+  ws.lfw.ensure(sizeof(double) * 3 * n1);
+  ws.lsw.ensure(sizeof(double) * 9 * f1);
+  ws.lloss.ensure(sizeof(double) * (f1 + 1));
+  EfsOut out;
+  out.energy = ws.lE.get<double>();
+  out.force = ws.lF.get<double>();
+  out.virial = ws.lV.get<double>();
+  efs_dispatch(dm, ws, precision, b, out, st);
+  double* lf = ws.lloss.get<double>();
+  SUS2_LAUNCH(ws, st, "loss_frames", k_loss_frames, dim3(f1), dim3(256), 0, nf, b.frame_off, out.energy, out.force,
+              out.virial, e0, f0, s0, mode, wE, wF, wS, ws.lew.get<double>(), ws.lfw.get<double>(),
+              ws.lsw.get<double>(), lf + 1);
+  SUS2_LAUNCH(ws, st, "loss_sum", k_loss_sum, dim3(1), dim3(256), 0, nf, lf + 1, lf);
+  grad_dispatch(dm, ws, precision, b, ws.lew.get<double>(), ws.lfw.get<double>(), ws.lsw.get<double>(), grad,
+                1.0 / nf, false, true, st);
+  SUS2_CHECK(hipMemcpyAsync(ws.host_total, lf, sizeof(double), hipMemcpyDeviceToHost, st));
+  SUS2_CHECK(hipStreamSynchronize(st));
+  double v;
+  std::memcpy(&v, ws.host_total, sizeof(double));
+  *loss_host = v;
+}
+
+}  // namespace sus2

@@ -1,0 +1,825 @@
+// Shape-templated parameter-gradient kernels of the SUS2-SH path; instantiated
+// per shape in sus2_efs_l*k*.hip beside the EFS kernels.
+//
+// What is computed (AccumulateEFSCombinationGrad, dev_src/mtpr.cpp:9188 and
+// src/basic_mlip.cpp:421-446):
+//   grad += d/dtheta [ sum_i Ebar_i E_i  +  D_w E ],
+// where D_w E = sum_e w_e . dE/dd_e is the derivative of the total energy
+// along the edge displacement field w_e = frc_w_i - frc_w_j + str_w(frame) d_e
+// for every directed edge e = (i -> j).  This is forward-over-reverse: every
+// pass carries primal values and their tangent along w (moments A and A_dot,
+// graph nodes v and v_dot, gate signal h and h_dot), and the reverse sweep
+// carries two adjoints per quantity, P = dPhi/dX and Q = dPhi/dX_dot.  Per-edge
+// quantities are recomputed in every sweep; nothing is cached per edge except
+// the gate messages of the outer pass (2F values per edge), which the gate
+// reverse gathers over the sorted incoming-edge lists (no atomics).
+//
+// Kernels (one 64-lane workgroup = one wavefront, LDS-staged edges):
+//   k_grad_gate_forward   per atom: gate moments G, G_dot; gate graph forward +
+//                         tangent; h = W S, h_dot = W S_dot; stores the gate
+//                         node values / tangents and the type-free moments.
+//   k_grad_outer          grid-stride over atoms: outer moments + tangent,
+//                         graph forward + tangent, readout, graph reverse of
+//                         the tangent program (P, Q), then a second edge sweep
+//                         producing the radial / type / warp / alpha gradients
+//                         and the gate messages (h_bar, h_dot_bar parts).
+//   k_grad_gate_reverse   grid-stride: gathers h_bar, h_dot_bar, gate graph
+//                         reverse, gate edge sweep (gate radial / type / warp).
+//   k_grad_W              W_bar = sum_j h_bar_j S_j^T + h_dot_bar_j S_dot_j^T
+//                         (split over atom chunks, FP64 partials).
+//   k_grad_reduce         fixed-order FP64 sum of the per-workgroup partials.
+// Parameter gradients are accumulated in FP64 (per-lane registers for the
+// radial coefficients, an FP64 LDS vector for the rest); per-edge arithmetic
+// runs in the kernel precision T.
+#pragma once
+#include "sus2_efs_impl.hpp"
+
+namespace sus2 {
+
+// Chebyshev values and xi-derivatives of one warp block, scaling folded in:
+// Tb[q] = S T_q(xi), T1[q] = dTb/dxi, T2[q] = d2Tb/dxi2.
+template <int BB, class T>
+__device__ __forceinline__ void cheb3(T S, T xi, T* Tb, T* T1, T* T2) {
+  Tb[0] = S; T1[0] = T(0); T2[0] = T(0);
+  if (BB > 1) { Tb[1] = S * xi; T1[1] = S; T2[1] = T(0); }
+#pragma unroll
+  for (int q = 2; q < BB; ++q) {
+    Tb[q] = T(2) * xi * Tb[q - 1] - Tb[q - 2];
+    T1[q] = T(2) * (Tb[q - 1] + xi * T1[q - 1]) - T1[q - 2];
+    T2[q] = T(4) * T1[q - 1] + T(2) * xi * T2[q - 1] - T2[q - 2];
+  }
+}
+
+// Index of a coefficient in the per-workgroup partial vector, which skips the
+// gate-weight block W (reduced separately by k_grad_W).
+__host__ __device__ __forceinline__ int pidx(int p, int oW, int nW) { return (oW >= 0 && p >= oW) ? p - nW : p; }
+
+// LDS layout shared by host (sizes) and device (carving).
+template <class Cf>
+struct GradLds {
+  int stage_rows, M, sc, tabs, pn;
+  __host__ __device__ size_t floats() const {
+    return (size_t)stage_rows * LR + 6 * (size_t)Cf::KH + 4 * (size_t)M + 2 * (size_t)sc + tabs + 2 * Cf::F + 2;
+  }
+  __host__ __device__ size_t bytes() const {
+    // T area first, then the FP64 partial vector (8-byte aligned)
+    const size_t t = floats() * sizeof(typename Cf::T);
+    return ((t + 7) & ~(size_t)7) + sizeof(double) * (size_t)pn;
+  }
+};
+
+template <class Cf>
+__host__ __device__ inline int grad_stage_rows(bool sweep2, bool outer_gated) {
+  const int r1 = 2 * Cf::F + 2 * Cf::H;
+  const int r2 = 4 * Cf::NBLK + (outer_gated ? 4 : 3) * Cf::F + 1;
+  return sweep2 ? (r1 > r2 ? r1 : r2) : r1;
+}
+
+template <class Cf>
+GradLds<Cf> grad_lds(const DModel<typename Cf::T>& m, bool gate, bool sweep2, int pn) {
+  GradLds<Cf> g;
+  g.stage_rows = grad_stage_rows<Cf>(sweep2, !gate && m.gated);
+  g.M = m.M;
+  g.sc = std::max(graph_scratch(gate ? m.gp : m.op), 1);
+  g.tabs = species_tables_size<Cf>(m.C);
+  g.pn = pn;
+  return g;
+}
+
+// Pointers into the LDS block.
+template <class T>
+struct GradSmem {
+  T *stage, *At, *Atd, *A, *Ad, *Pc, *Qc, *v, *vd, *P, *Q, *s1, *s2, *tabs, *hbs;
+  double* part;
+};
+
+template <class Cf>
+__device__ GradSmem<typename Cf::T> carve(const GradLds<Cf>& g, unsigned char* raw) {
+  using T = typename Cf::T;
+  GradSmem<T> s;
+  T* p = reinterpret_cast<T*>(raw);
+  s.stage = p; p += (size_t)g.stage_rows * LR;
+  s.At = p; p += Cf::KH;
+  s.Atd = p; p += Cf::KH;
+  s.A = p; p += Cf::KH;
+  s.Ad = p; p += Cf::KH;
+  s.Pc = p; p += Cf::KH;
+  s.Qc = p; p += Cf::KH;
+  s.v = p; p += g.M;
+  s.vd = p; p += g.M;
+  s.P = p; p += g.M;
+  s.Q = p; p += g.M;
+  s.s1 = p; p += g.sc;
+  s.s2 = p; p += g.sc;
+  s.tabs = p; p += g.tabs;
+  s.hbs = p; p += 2 * Cf::F + 2;
+  const size_t t = g.floats() * sizeof(T);
+  s.part = reinterpret_cast<double*>(raw + ((t + 7) & ~(size_t)7));
+  return s;
+}
+
+// ------------------------------------------------------------------ graph with tangents
+// Contraction of A and A_dot into v and v_dot (both zeroed first).
+template <class T, int KK>
+__device__ void wave_contract_tan(const DModel<T>& m, const DPass<T>& p, bool gate, const T* A, const T* Ad, T* v,
+                                  T* vd) {
+  for (int n = threadIdx.x; n < m.M; n += WAVE) { v[n] = T(0); vd[n] = T(0); }
+  __syncthreads();
+  wave_contract<T, KK>(m, p, gate, A, v);
+  wave_contract<T, KK>(m, p, gate, Ad, vd);
+}
+
+// Product layers: v[out] = sum c v[a] v[b];  v_dot[out] = sum c (v_dot[a] v[b] + v[a] v_dot[b]).
+template <class T>
+__device__ void wave_graph_forward_tan(const DPass<T>& p, T* v, T* vd, T* s1, T* s2) {
+  const int lane = threadIdx.x;
+  for (int l = 0; l < p.nfl; ++l) {
+    const int o0 = p.fl_off[l], o1 = p.fl_off[l + 1];
+    const int r0 = p.f_roff[o0], r1 = p.f_roff[o1];
+    for (int r = r0 + lane; r < r1; r += WAVE) {
+      const T c = p.f_c[r];
+      const int a = p.f_a[r], b = p.f_b[r];
+      s1[r - r0] = c * v[a] * v[b];
+      s2[r - r0] = c * (vd[a] * v[b] + v[a] * vd[b]);
+    }
+    __syncthreads();
+    for (int o = o0 + lane; o < o1; o += WAVE) {
+      T x = T(0), y = T(0);
+      const int e = p.f_roff[o + 1] - r0;
+      for (int r = p.f_roff[o] - r0; r < e; ++r) { x += s1[r]; y += s2[r]; }
+      v[p.f_out[o]] = x;
+      vd[p.f_out[o]] = y;
+    }
+    __syncthreads();
+  }
+}
+
+// Reverse of the tangent program: for every consumer row (out, partner, c) of
+// node n,  Q[n] += c Q[out] v[partner],
+//          P[n] += c (P[out] v[partner] + Q[out] v_dot[partner]).
+template <class T>
+__device__ void wave_graph_reverse_tan(const DPass<T>& p, const T* v, const T* vd, T* P, T* Q, T* s1, T* s2) {
+  const int lane = threadIdx.x;
+  for (int l = 0; l < p.nrl; ++l) {
+    const int n0 = p.rl_off[l], n1 = p.rl_off[l + 1];
+    const int q0 = p.r_roff[n0], q1 = p.r_roff[n1];
+    for (int q = q0 + lane; q < q1; q += WAVE) {
+      const T c = p.r_c[q];
+      const int o = p.r_out[q], pt = p.r_partner[q];
+      s1[q - q0] = c * (P[o] * v[pt] + Q[o] * vd[pt]);
+      s2[q - q0] = c * Q[o] * v[pt];
+    }
+    __syncthreads();
+    for (int k = n0 + lane; k < n1; k += WAVE) {
+      const int n = p.r_node[k];
+      T x = P[n], y = Q[n];
+      const int e = p.r_roff[k + 1] - q0;
+      for (int q = p.r_roff[k] - q0; q < e; ++q) { x += s1[q]; y += s2[q]; }
+      P[n] = x;
+      Q[n] = y;
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ sweep 1
+// Moments and tangents of one atom: At = sum_e x1 R Y, Atd = sum_e [x1 (R' w_r Y
+// + R grad Y . w) + x1_dot R Y]  (the centre factor x0 is applied afterwards:
+// A = x0 At), with x1 = neighbour type factor (+ alpha h_j for the outer pass
+// of an Effect model) and x1_dot = alpha h_dot_j.
+template <class Cf, bool GATE>
+__device__ void grad_sweep_moments(const DModel<typename Cf::T>& m, const SpeciesTables<typename Cf::T>& tab,
+                                   const int32_t* __restrict__ types, const Rec<typename Cf::T>* __restrict__ rec,
+                                   const double* __restrict__ frc_w, const typename Cf::T* Sw, const double* fwi,
+                                   const typename Cf::T* __restrict__ h, const typename Cf::T* __restrict__ hd, int ti,
+                                   int e0, int e1, typename Cf::T* stage, typename Cf::T* At, typename Cf::T* Atd) {
+  using T = typename Cf::T;
+  constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, H = Cf::H, F = Cf::F, KH = Cf::KH, NM = Cf::NM;
+  const int lane = threadIdx.x;
+  const bool hg = !GATE && m.gated;
+  T* sS1 = stage;            // [F][LR]  x1 R
+  T* sS2 = sS1 + F * LR;     // [F][LR]  x1 R' w_r + x1_dot R
+  T* sY = sS2 + F * LR;      // [H][LR]
+  T* sYw = sY + H * LR;      // [H][LR]  grad Y . w
+  T acc[NM], accd[NM];
+#pragma unroll
+  for (int t = 0; t < NM; ++t) { acc[t] = T(0); accd[t] = T(0); }
+  for (int base = e0; base < e1; base += WAVE) {
+    const int e = base + lane;
+    if (e < e1) {
+      const Rec<T> q = rec[e];
+      const int j = rec_j(q);
+      const T dx = q.x, dy = q.y, dz = q.z;
+      const T r = dsqrt(dx * dx + dy * dy + dz * dz);
+      const T inv = T(1) / r;
+      const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
+      const int tj = types[j];
+      const double* fj = frc_w + 3 * (size_t)j;
+      const T wx = T(fwi[0] - fj[0]) + Sw[0] * dx + Sw[1] * dy + Sw[2] * dz;
+      const T wy = T(fwi[1] - fj[1]) + Sw[3] * dx + Sw[4] * dy + Sw[5] * dz;
+      const T wz = T(fwi[2] - fj[2]) + Sw[6] * dx + Sw[7] * dy + Sw[8] * dz;
+      const T wr = ux * wx + uy * wy + uz * wz;
+      T R[F], dR[F];
+      radial<Cf, true>(m, GATE, tab, ti * m.C + tj, r, R, dR);
+      const T* x0j = tab.x0 + tj * F;
+      const T* alj = tab.alpha + tj * F;
+#pragma unroll
+      for (int mu = 0; mu < F; ++mu) {
+        T x1 = x0j[mu], xd = T(0);
+        if (hg) {
+          x1 += alj[mu] * h[(size_t)j * F + mu];
+          xd = alj[mu] * hd[(size_t)j * F + mu];
+        }
+        sS1[mu * LR + lane] = x1 * R[mu];
+        sS2[mu * LR + lane] = x1 * dR[mu] * wr + xd * R[mu];
+      }
+      sh_stream_grad<T, LL>(ux, uy, uz, inv, [&](int l, int mm, T Y, T yx, T yy, T yz) {
+        const int hh = hidx(l, mm);
+        sY[hh * LR + lane] = Y;
+        sYw[hh * LR + lane] = yx * wx + yy * wy + yz * wz;
+      });
+    }
+    __syncthreads();
+    const int cnt = min(WAVE, e1 - base);
+#pragma unroll
+    for (int t = 0; t < NM; ++t) {
+      const int c = lane + WAVE * t;
+      if (c < KH) {
+        const int hh = c / KK, k = c % KK;
+        const int mu = k * L1 + l_of(hh);
+        const T* p1 = sS1 + mu * LR;
+        const T* p2 = sS2 + mu * LR;
+        const T* py = sY + hh * LR;
+        const T* pw = sYw + hh * LR;
+        T s = acc[t], sd = accd[t];
+        for (int ee = 0; ee < cnt; ++ee) {
+          s += p1[ee] * py[ee];
+          sd += p2[ee] * py[ee] + p1[ee] * pw[ee];
+        }
+        acc[t] = s;
+        accd[t] = sd;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int t = 0; t < NM; ++t) {
+    const int c = lane + WAVE * t;
+    if (c < KH) { At[c] = acc[t]; Atd[c] = accd[t]; }
+  }
+}
+
+// ------------------------------------------------------------------ sweep 2
+// Parameter sweep over the centre's edges with the canonical adjoints Pc, Qc:
+// per edge  p_mu = sum_m Pc Y, q_mu = sum_m Qc Y, q'_mu = sum_m Qc grad Y . w,
+//   Z = R (p + q') + R' w_r q,   Gv = x0 (x1 (p + q') + x1_dot q),  Gd = x0 x1 w_r q
+// and the radial coefficient, type, warp (slope/shift) and alpha gradients
+// follow from R = f C0, R' = g C0 + f C1 xi_r with C_k = sum_b c_b d^k Tb/dxi^k.
+// Outer pass of an Effect model: messages (alpha x0 Z, alpha x0 R q) per edge.
+template <class Cf, bool GATE>
+__device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const SpeciesTables<typename Cf::T>& tab,
+                                  const int32_t* __restrict__ types, const Rec<typename Cf::T>* __restrict__ rec,
+                                  const double* __restrict__ frc_w, const typename Cf::T* Sw, const double* fwi,
+                                  const typename Cf::T* __restrict__ h, const typename Cf::T* __restrict__ hd,
+                                  typename Cf::T* __restrict__ msg, int ti, int e0, int e1,
+                                  const GradSmem<typename Cf::T>& s, double* creg, int pn_oW, int pn_nW) {
+  using T = typename Cf::T;
+  constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, F = Cf::F, BB = Cf::BB, NBLK = Cf::NBLK;
+  const int lane = threadIdx.x, C = m.C, CC = C * C;
+  const bool hg = !GATE && m.gated;
+  T* sXi = s.stage;             // [NBLK][LR]
+  T* sXr = sXi + NBLK * LR;     // [NBLK][LR]
+  T* sDs = sXr + NBLK * LR;     // [NBLK][LR]
+  T* sDt = sDs + NBLK * LR;     // [NBLK][LR]
+  T* sCv = sDt + NBLK * LR;     // [F][LR]
+  T* sCd = sCv + F * LR;        // [F][LR]
+  T* sX1 = sCd + F * LR;        // [F][LR]  x0 Z  (d Phi / d x1)
+  T* sTj = sX1 + F * LR;        // [LR]
+  T* sAl = sTj + LR;            // [F][LR]  x0 (Z h_j + R q h_dot_j)  (outer, gated)
+  const bool own_rad = GATE && m.o_grad >= 0;
+  const T* __restrict__ coef = m.theta + (own_rad ? m.o_grad : m.o_rad);
+  const int stride = own_rad ? BB : BB + C;
+  const int so = (GATE && m.o_gscal >= 0) ? m.o_gscal : m.o_scal;
+  const T* x0 = tab.x0 + ti * F;
+  // c-gradient lanes: (mu, subset)
+  constexpr int NSUB = WAVE / F;
+  const int cmu = lane % F, csub = lane / F;
+  const bool cact = csub < NSUB;
+  const int cblk = Cf::blk(cmu / L1, cmu % L1);
+  for (int base = e0; base < e1; base += WAVE) {
+    const int e = base + lane;
+    if (e < e1) {
+      const Rec<T> q = rec[e];
+      const int j = rec_j(q);
+      const T dx = q.x, dy = q.y, dz = q.z;
+      const T r = dsqrt(dx * dx + dy * dy + dz * dz);
+      const T inv = T(1) / r;
+      const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
+      const int tj = types[j];
+      const double* fj = frc_w + 3 * (size_t)j;
+      const T wx = T(fwi[0] - fj[0]) + Sw[0] * dx + Sw[1] * dy + Sw[2] * dz;
+      const T wy = T(fwi[1] - fj[1]) + Sw[3] * dx + Sw[4] * dy + Sw[5] * dz;
+      const T wz = T(fwi[2] - fj[2]) + Sw[6] * dx + Sw[7] * dy + Sw[8] * dz;
+      const T wr = ux * wx + uy * wy + uz * wz;
+      T pp[F], qv[F], qw[F];
+#pragma unroll
+      for (int mu = 0; mu < F; ++mu) { pp[mu] = T(0); qv[mu] = T(0); qw[mu] = T(0); }
+      sh_stream_grad<T, LL>(ux, uy, uz, inv, [&](int l, int mm, T Y, T yx, T yy, T yz) {
+        const int hh = hidx(l, mm);
+        const T Yw = yx * wx + yy * wy + yz * wz;
+#pragma unroll
+        for (int k = 0; k < KK; ++k) {
+          const int mu = k * L1 + l;
+          const T pc = s.Pc[hh * KK + k], qc = s.Qc[hh * KK + k];
+          pp[mu] += pc * Y;
+          qv[mu] += qc * Y;
+          qw[mu] += qc * Yw;
+        }
+      });
+      const T d0 = r - m.rc, f0 = d0 * d0, g0 = T(2) * d0;
+      const bool act = r < m.rca;
+      const T d1 = r - m.rca, f1 = act ? d1 * d1 : T(0), g1 = act ? T(2) * d1 : T(0);
+      const int pair = ti * C + tj;
+      const T* x0j = tab.x0 + tj * F;
+      const T* alj = tab.alpha + tj * F;
+#pragma unroll
+      for (int b = 0; b < NBLK; ++b) {
+        const T sl = tab.slope[b * CC + pair], sh = tab.shift[b * CC + pair];
+        const T xi = dtanh(sl * (r - sh) * T(0.5));
+        const T xx = T(1) - xi * xi;
+        const T xr = xx * sl * T(0.5), xs = xx * (r - sh) * T(0.5), xt = -xr;
+        const T xrs = T(0.5) * xx - sl * xi * xs, xrt = -sl * xi * xt;
+        T Tb[BB], T1[BB], T2[BB];
+        cheb3<BB>(m.scaling, xi, Tb, T1, T2);
+        T ds = T(0), dt = T(0);
+#pragma unroll
+        for (int k = 0; k < KK; ++k) {
+#pragma unroll
+          for (int l = 0; l < L1; ++l) {
+            if (Cf::blk(k, l) != b) continue;
+            const int mu = k * L1 + l;
+            const T* c = coef + mu * stride;
+            T C0 = T(0), C1 = T(0), C2 = T(0);
+#pragma unroll
+            for (int qq = 0; qq < BB; ++qq) {
+              C0 += c[qq] * Tb[qq];
+              C1 += c[qq] * T1[qq];
+              C2 += c[qq] * T2[qq];
+            }
+            const T f = l == 0 ? f0 : f1, g = l == 0 ? g0 : g1;
+            const T R = f * C0, Rp = g * C0 + f * C1 * xr;
+            T x1 = x0j[mu], xd = T(0), hj = T(0), hdj = T(0);
+            if (hg) {
+              hj = h[(size_t)j * F + mu];
+              hdj = hd[(size_t)j * F + mu];
+              x1 += alj[mu] * hj;
+              xd = alj[mu] * hdj;
+            }
+            const T pq = pp[mu] + qw[mu];
+            const T Z = R * pq + Rp * wr * qv[mu];
+            const T Rq = R * qv[mu];
+            const T Gv = x0[mu] * (x1 * pq + xd * qv[mu]);
+            const T Gd = x0[mu] * x1 * wr * qv[mu];
+            const T Cv = Gv * f + Gd * g, Cd = Gd * f;
+            ds += xs * C1 * Cv + Cd * (C2 * xs * xr + C1 * xrs);
+            dt += xt * C1 * Cv + Cd * (C2 * xt * xr + C1 * xrt);
+            sCv[mu * LR + lane] = Cv;
+            sCd[mu * LR + lane] = Cd;
+            sX1[mu * LR + lane] = x0[mu] * Z;
+            if (hg) {
+              sAl[mu * LR + lane] = x0[mu] * (Z * hj + Rq * hdj);
+              T* mo = msg + (size_t)e * 2 * F;
+              mo[mu] = alj[mu] * x0[mu] * Z;
+              mo[F + mu] = alj[mu] * x0[mu] * Rq;
+            }
+          }
+        }
+        sXi[b * LR + lane] = xi;
+        sXr[b * LR + lane] = xr;
+        sDs[b * LR + lane] = ds;
+        sDt[b * LR + lane] = dt;
+      }
+      sTj[lane] = T(tj);
+    }
+    __syncthreads();
+    const int cnt = min(WAVE, e1 - base);
+    // radial coefficients: lane (mu, subset) re-runs the Chebyshev recurrence
+    if (cact) {
+      const T S = m.scaling;
+      for (int ee = csub; ee < cnt; ee += NSUB) {
+        const T xi = sXi[cblk * LR + ee], cv = sCv[cmu * LR + ee], cd = sCd[cmu * LR + ee] * sXr[cblk * LR + ee];
+        T ta = S, tb = S * xi, da = T(0), db = S;
+        creg[0] += (double)(cv * ta);
+        if (BB > 1) creg[1] += (double)(cv * tb + cd * db);
+#pragma unroll
+        for (int qq = 2; qq < BB; ++qq) {
+          const T tn = T(2) * xi * tb - ta, dn = T(2) * (tb + xi * db) - da;
+          creg[qq] += (double)(cv * tn + cd * dn);
+          ta = tb; tb = tn; da = db; db = dn;
+        }
+      }
+    }
+    // type-bucketed sums: neighbour type factor, alpha, warp slope/shift
+    const int n1 = C * F, n2 = hg ? C * F : 0, n3 = 2 * NBLK * C;
+    for (int idx = lane; idx < n1 + n2 + n3; idx += WAVE) {
+      double acc = 0;
+      int p;
+      if (idx < n1 + n2) {
+        const int k2 = idx < n1 ? idx : idx - n1;
+        const int t = k2 / F, mu = k2 % F;
+        const T* src = (idx < n1 ? sX1 : sAl) + mu * LR;
+        for (int ee = 0; ee < cnt; ++ee)
+          if ((int)sTj[ee] == t) acc += (double)src[ee];
+        if (idx < n1) p = GATE ? m.o_gtype + t * F + mu : m.o_rad + mu * (BB + C) + BB + t;
+        else p = m.o_alpha + t * F + mu;
+      } else {
+        const int k3 = idx - n1 - n2;
+        const int b = k3 / (2 * C), w = (k3 / C) % 2, t = k3 % C;
+        const T* src = (w ? sDt : sDs) + b * LR;
+        for (int ee = 0; ee < cnt; ++ee)
+          if ((int)sTj[ee] == t) acc += (double)src[ee];
+        p = so + (2 * b + w) * CC + ti * C + t;
+      }
+      s.part[pidx(p, pn_oW, pn_nW)] += acc;
+    }
+    __syncthreads();
+  }
+}
+
+// Per-atom, moment-space gradients after the reverse: rank weights
+// (sum_m P[rank node] A + Q[rank node] A_dot) and the centre type factor
+// (sum_m Pc At + Qc Atd).  Also scales At -> A beforehand (done by caller).
+template <class Cf, bool GATE>
+__device__ void grad_moment_space(const DModel<typename Cf::T>& m, const DPass<typename Cf::T>& p,
+                                  const GradSmem<typename Cf::T>& s, int ti, int pn_oW, int pn_nW) {
+  using T = typename Cf::T;
+  constexpr int KK = Cf::KK, L1 = Cf::L1, F = Cf::F, BB = Cf::BB;
+  const int lane = threadIdx.x, H = Cf::H, C = m.C;
+  const int rank = p.rank;
+  const int nr = L1 * rank * KK;
+  for (int idx = lane; idx < nr; idx += WAVE) {
+    const int l = idx / (rank * KK), a = (idx / KK) % rank, k = idx % KK;
+    double acc = 0;
+    for (int mm = -l; mm <= l; ++mm) {
+      const int c = hidx(l, mm) * KK + k;
+      if (m.basic_node[c] < 0) continue;
+      const int node = rank_node(p.begin, rank, H, l, mm, a);
+      acc += (double)(s.P[node] * s.A[c] + s.Q[node] * s.Ad[c]);
+    }
+    const int pp = (GATE ? m.o_gw : m.o_ow) + (l * rank + a) * KK + k;
+    s.part[pidx(pp, pn_oW, pn_nW)] += acc;
+  }
+  for (int mu = lane; mu < F; mu += WAVE) {
+    const int k = mu / L1, l = mu % L1;
+    double acc = 0;
+    for (int mm = -l; mm <= l; ++mm) {
+      const int c = hidx(l, mm) * KK + k;
+      acc += (double)(s.Pc[c] * s.At[c] + s.Qc[c] * s.Atd[c]);
+    }
+    const int pp = GATE ? m.o_gtype + ti * F + mu : m.o_rad + mu * (BB + C) + BB + ti;
+    s.part[pidx(pp, pn_oW, pn_nW)] += acc;
+  }
+}
+
+// A = x0 At, A_dot = x0 Atd (centre type factor per channel).
+template <class Cf>
+__device__ void scale_moments(const SpeciesTables<typename Cf::T>& tab, int ti, const GradSmem<typename Cf::T>& s) {
+  constexpr int KK = Cf::KK, L1 = Cf::L1, F = Cf::F, KH = Cf::KH;
+  for (int c = threadIdx.x; c < KH; c += WAVE) {
+    const int hh = c / KK, k = c % KK;
+    const auto x = tab.x0[ti * F + k * L1 + l_of(hh)];
+    s.A[c] = x * s.At[c];
+    s.Ad[c] = x * s.Atd[c];
+  }
+}
+
+// Per-atom frame weights: str_w of the atom's frame (T) and frc_w of the atom.
+template <class T>
+__device__ void load_atom_weights(const GradWeights& gw, int i, T* Sw, double* fwi) {
+  const int f = frame_of(gw.frame_off, gw.nframes, i);
+#pragma unroll
+  for (int q = 0; q < 9; ++q) Sw[q] = T(gw.str_w[9 * (size_t)f + q]);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) fwi[q] = gw.frc_w[3 * (size_t)i + q];
+}
+
+// ------------------------------------------------------------------ kernels
+template <class Cf, class T = typename Cf::T>
+__global__ __launch_bounds__(WAVE) void k_grad_gate_forward(DModel<T> m, GradLds<Cf> L, GradWeights gw,
+                                                            const int32_t* __restrict__ types,
+                                                            const int* __restrict__ nbr_off,
+                                                            const Rec<T>* __restrict__ rec, T* __restrict__ h,
+                                                            T* __restrict__ hd, T* __restrict__ vg,
+                                                            T* __restrict__ vgd, T* __restrict__ gmom,
+                                                            T* __restrict__ gmomd) {
+  constexpr int F = Cf::F, KH = Cf::KH;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const GradSmem<T> s = carve(L, smem_raw);
+  const int i = blockIdx.x, lane = threadIdx.x;
+  const int ti = types[i];
+  SpeciesTables<T> tab;
+  load_species_tables<Cf>(m, true, s.tabs, tab);
+  T Sw[9];
+  double fwi[3];
+  load_atom_weights(gw, i, Sw, fwi);
+  __syncthreads();
+  grad_sweep_moments<Cf, true>(m, tab, types, rec, gw.frc_w, Sw, fwi, nullptr, nullptr, ti, nbr_off[i],
+                               nbr_off[i + 1], s.stage, s.At, s.Atd);
+  __syncthreads();
+  for (int c = lane; c < KH; c += WAVE) {
+    gmom[(size_t)i * KH + c] = s.At[c];
+    gmomd[(size_t)i * KH + c] = s.Atd[c];
+  }
+  scale_moments<Cf>(tab, ti, s);
+  __syncthreads();
+  const DPass<T>& p = m.gp;
+  wave_contract_tan<T, Cf::KK>(m, p, true, s.A, s.Ad, s.v, s.vd);
+  __syncthreads();
+  wave_graph_forward_tan(p, s.v, s.vd, s.s1, s.s2);
+  for (int mu = lane; mu < F; mu += WAVE) {
+    T a = T(0), b = T(0);
+    for (int q = 0; q < m.Qg; ++q) {
+      const T w = m.W(mu, q);
+      a += w * s.v[m.gate_nodes[q]];
+      b += w * s.vd[m.gate_nodes[q]];
+    }
+    h[(size_t)i * F + mu] = a;
+    hd[(size_t)i * F + mu] = b;
+  }
+  for (int n = lane; n < p.node_cnt; n += WAVE) {
+    vg[(size_t)i * p.node_cnt + n] = s.v[p.node_lo + n];
+    vgd[(size_t)i * p.node_cnt + n] = s.vd[p.node_lo + n];
+  }
+}
+
+// Zero the workgroup's FP64 partial vector.
+__device__ __forceinline__ void zero_part(double* part, int pn) {
+  for (int k = threadIdx.x; k < pn; k += WAVE) part[k] = 0.0;
+}
+
+// Fold the per-lane radial-coefficient registers into the partial vector
+// (subsets one after another), then write (or add) the partial row.
+template <class Cf>
+__device__ void flush_partial(const DModel<typename Cf::T>& m, bool gate, const double* creg, double* part, int pn,
+                              int oW, int nW, double* __restrict__ out_row, bool add) {
+  constexpr int F = Cf::F, BB = Cf::BB, NSUB = WAVE / F;
+  const int lane = threadIdx.x, cmu = lane % F, csub = lane / F;
+  const bool own_rad = gate && m.o_grad >= 0;
+  const int base = own_rad ? m.o_grad : m.o_rad, stride = own_rad ? BB : BB + m.C;
+  for (int sub = 0; sub < NSUB; ++sub) {
+    if (csub == sub)
+#pragma unroll
+      for (int b = 0; b < BB; ++b) part[pidx(base + cmu * stride + b, oW, nW)] += creg[b];
+    __syncthreads();
+  }
+  for (int k = lane; k < pn; k += WAVE) out_row[k] = add ? out_row[k] + part[k] : part[k];
+}
+
+template <class Cf, class T = typename Cf::T>
+__global__ __launch_bounds__(WAVE) void k_grad_outer(DModel<T> m, GradLds<Cf> L, GradWeights gw, int natoms,
+                                                     const int32_t* __restrict__ types,
+                                                     const int* __restrict__ nbr_off, const Rec<T>* __restrict__ rec,
+                                                     const T* __restrict__ h, const T* __restrict__ hd,
+                                                     T* __restrict__ msg, double* __restrict__ partial, int oW,
+                                                     int nW) {
+  constexpr int KK = Cf::KK, BB = Cf::BB;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const GradSmem<T> s = carve(L, smem_raw);
+  const int lane = threadIdx.x;
+  SpeciesTables<T> tab;
+  load_species_tables<Cf>(m, false, s.tabs, tab);
+  zero_part(s.part, L.pn);
+  double creg[BB];
+#pragma unroll
+  for (int b = 0; b < BB; ++b) creg[b] = 0.0;
+  const DPass<T>& p = m.op;
+  __syncthreads();
+  for (int i = blockIdx.x; i < natoms; i += gridDim.x) {
+    const int ti = types[i];
+    T Sw[9];
+    double fwi[3];
+    load_atom_weights(gw, i, Sw, fwi);
+    const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
+    grad_sweep_moments<Cf, false>(m, tab, types, rec, gw.frc_w, Sw, fwi, h, hd, ti, e0, e1, s.stage, s.At, s.Atd);
+    __syncthreads();
+    scale_moments<Cf>(tab, ti, s);
+    __syncthreads();
+    wave_contract_tan<T, KK>(m, p, false, s.A, s.Ad, s.v, s.vd);
+    __syncthreads();
+    wave_graph_forward_tan(p, s.v, s.vd, s.s1, s.s2);
+    // readout and its gradients; seeds of the reverse
+    const double Eb = gw.ene_w[i];
+    double gsp = 0;  // d/d species[ti] through a shared readout
+    for (int q = lane; q < m.Q; q += WAVE) {
+      const int node = m.scalar_map[q];
+      const double g = Eb * (double)s.v[node] + (double)s.vd[node];
+      if (m.species_readout) {
+        s.part[pidx(m.o_read + ti * m.Q + q, oW, nW)] += g;
+      } else {
+        s.part[pidx(m.o_read + q, oW, nW)] += (double)m.theta[m.o_species + ti] * g;
+        gsp += (double)m.theta[m.o_read + q] * g;
+      }
+    }
+    gsp = wave_sum(gsp);
+    for (int n = lane; n < m.M; n += WAVE) { s.P[n] = T(0); s.Q[n] = T(0); }
+    __syncthreads();
+    if (lane == 0) {
+      s.part[pidx(m.o_species + ti, oW, nW)] += Eb + gsp;
+      s.part[pidx(m.o_shift + ti, oW, nW)] += Eb;
+      for (int q = 0; q < m.Q; ++q) {
+        const T rho = m.readout(ti, q);
+        s.P[m.scalar_map[q]] += T(Eb) * rho;
+        s.Q[m.scalar_map[q]] += rho;
+      }
+    }
+    __syncthreads();
+    wave_graph_reverse_tan(p, s.v, s.vd, s.P, s.Q, s.s1, s.s2);
+    wave_adjoint_to_basic<T, KK>(m, p, false, s.P, s.Pc);
+    wave_adjoint_to_basic<T, KK>(m, p, false, s.Q, s.Qc);
+    __syncthreads();
+    grad_moment_space<Cf, false>(m, p, s, ti, oW, nW);
+    __syncthreads();
+    grad_sweep_params<Cf, false>(m, tab, types, rec, gw.frc_w, Sw, fwi, h, hd, msg, ti, e0, e1, s, creg, oW, nW);
+  }
+  __syncthreads();
+  flush_partial<Cf>(m, false, creg, s.part, L.pn, oW, nW, partial + (size_t)blockIdx.x * L.pn, false);
+}
+
+template <class Cf, class T = typename Cf::T>
+__global__ __launch_bounds__(WAVE) void k_grad_gate_reverse(DModel<T> m, GradLds<Cf> L, GradWeights gw, int natoms,
+                                                            const int32_t* __restrict__ types,
+                                                            const int* __restrict__ nbr_off,
+                                                            const int* __restrict__ in_edges,
+                                                            const Rec<T>* __restrict__ rec,
+                                                            const T* __restrict__ vg, const T* __restrict__ vgd,
+                                                            const T* __restrict__ gmom, const T* __restrict__ gmomd,
+                                                            const T* __restrict__ msg, T* __restrict__ hbar,
+                                                            double* __restrict__ partial, int oW, int nW) {
+  constexpr int KK = Cf::KK, BB = Cf::BB, F = Cf::F, KH = Cf::KH;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const GradSmem<T> s = carve(L, smem_raw);
+  const int lane = threadIdx.x;
+  SpeciesTables<T> tab;
+  load_species_tables<Cf>(m, true, s.tabs, tab);
+  zero_part(s.part, L.pn);
+  double creg[BB];
+#pragma unroll
+  for (int b = 0; b < BB; ++b) creg[b] = 0.0;
+  const DPass<T>& p = m.gp;
+  __syncthreads();
+  for (int i = blockIdx.x; i < natoms; i += gridDim.x) {
+    const int ti = types[i];
+    T Sw[9];
+    double fwi[3];
+    load_atom_weights(gw, i, Sw, fwi);
+    const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
+    // gather the gate messages of the incoming edges
+    T hb[F], hbd[F];
+#pragma unroll
+    for (int mu = 0; mu < F; ++mu) { hb[mu] = T(0); hbd[mu] = T(0); }
+    for (int k = e0 + lane; k < e1; k += WAVE) {
+      const T* mi = msg + (size_t)in_edges[k] * 2 * F;
+#pragma unroll
+      for (int mu = 0; mu < F; ++mu) { hb[mu] += mi[mu]; hbd[mu] += mi[F + mu]; }
+    }
+#pragma unroll
+    for (int mu = 0; mu < F; ++mu) {
+      const T a = wave_sum(hb[mu]), b = wave_sum(hbd[mu]);
+      if (lane == 0) { s.hbs[mu] = a; s.hbs[F + mu] = b; }
+    }
+    const T* vi = vg + (size_t)i * p.node_cnt;
+    const T* vdi = vgd + (size_t)i * p.node_cnt;
+    for (int n = lane; n < m.M; n += WAVE) {
+      const bool in = n >= p.node_lo && n < p.node_lo + p.node_cnt;
+      s.v[n] = in ? vi[n - p.node_lo] : T(0);
+      s.vd[n] = in ? vdi[n - p.node_lo] : T(0);
+      s.P[n] = T(0);
+      s.Q[n] = T(0);
+    }
+    for (int c = lane; c < KH; c += WAVE) {
+      s.At[c] = gmom[(size_t)i * KH + c];
+      s.Atd[c] = gmomd[(size_t)i * KH + c];
+    }
+    __syncthreads();
+    for (int mu = lane; mu < 2 * F; mu += WAVE) hbar[(size_t)i * 2 * F + mu] = s.hbs[mu];
+    for (int q = lane; q < m.Qg; q += WAVE) {
+      T a = T(0), b = T(0);
+#pragma unroll
+      for (int mu = 0; mu < F; ++mu) {
+        const T w = m.W(mu, q);
+        a += w * s.hbs[mu];
+        b += w * s.hbs[F + mu];
+      }
+      s.P[m.gate_nodes[q]] = a;
+      s.Q[m.gate_nodes[q]] = b;
+    }
+    scale_moments<Cf>(tab, ti, s);
+    __syncthreads();
+    wave_graph_reverse_tan(p, s.v, s.vd, s.P, s.Q, s.s1, s.s2);
+    wave_adjoint_to_basic<T, KK>(m, p, true, s.P, s.Pc);
+    wave_adjoint_to_basic<T, KK>(m, p, true, s.Q, s.Qc);
+    __syncthreads();
+    grad_moment_space<Cf, true>(m, p, s, ti, oW, nW);
+    __syncthreads();
+    grad_sweep_params<Cf, true>(m, tab, types, rec, gw.frc_w, Sw, fwi, nullptr, nullptr, nullptr, ti, e0, e1, s,
+                                creg, oW, nW);
+  }
+  __syncthreads();
+  flush_partial<Cf>(m, true, creg, s.part, L.pn, oW, nW, partial + (size_t)blockIdx.x * L.pn, true);
+}
+
+// W_bar[mu][q] over a chunk of atoms: thread = one (mu, q) entry, FP64 sums.
+template <class T>
+__global__ __launch_bounds__(256) void k_grad_W(int natoms, int F, int Qg, int chunk, const T* __restrict__ hbar,
+                                                const T* __restrict__ vg, const T* __restrict__ vgd, int node_cnt,
+                                                int node_lo, const int* __restrict__ gate_nodes,
+                                                double* __restrict__ wpart) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= F * Qg) return;
+  const int mu = idx / Qg, q = idx % Qg;
+  const int col = gate_nodes[q] - node_lo;
+  const int a0 = blockIdx.y * chunk, a1 = min(natoms, a0 + chunk);
+  double acc = 0;
+  for (int a = a0; a < a1; ++a) {
+    const T* hb = hbar + (size_t)a * 2 * F;
+    acc += (double)hb[mu] * (double)vg[(size_t)a * node_cnt + col] +
+           (double)hb[F + mu] * (double)vgd[(size_t)a * node_cnt + col];
+  }
+  wpart[(size_t)blockIdx.y * F * Qg + idx] = acc;
+}
+
+// grad[p] (=|+=) scale * sum of the partial rows, fixed order.
+static __global__ __launch_bounds__(256) void k_grad_reduce(int ncoeff, int rows, int pn, const double* __restrict__ part,
+                                                     int oW, int nW, int wrows, const double* __restrict__ wpart,
+                                                     double scale, int add, double* __restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ncoeff) return;
+  double acc = 0;
+  if (oW >= 0 && p >= oW && p < oW + nW) {
+    for (int r = 0; r < wrows; ++r) acc += wpart[(size_t)r * nW + (p - oW)];
+  } else {
+    const int k = pidx(p, oW, nW);
+    for (int r = 0; r < rows; ++r) acc += part[(size_t)r * pn + k];
+  }
+  grad[p] = (add ? grad[p] : 0.0) + scale * acc;
+}
+
+// ------------------------------------------------------------------ driver
+template <class Cf>
+void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const GradWeights& gw, double* grad,
+                    double scale, bool add, bool reuse_neighbors, hipStream_t st) {
+  using T = typename Cf::T;
+  const DModel<T>& m = dm.get<T>();
+  const int N = b.natoms;
+  if (!reuse_neighbors) build_neighbors<T>(ws, b, dm.host->rc, st);
+  const int ncoeff = dm.host->ncoeff;
+  const int oW = m.gated ? m.o_W : -1, nW = m.gated ? Cf::F * m.Qg : 0;
+  const int pn = ncoeff - nW;
+  const size_t n1 = std::max(1, N), ne = std::max<long>(1, ws.nedges);
+  const int G = std::max(1, std::min(N, ws.grad_groups));
+  ws.gpart.ensure(sizeof(double) * (size_t)G * pn);
+  const int* off = ws.nbr_off.get<int>();
+  const Rec<T>* rec = ws.rec.get<Rec<T>>();
+  int wrows = 0;
+  if (N > 0) {
+    const GradLds<Cf> lo = grad_lds<Cf>(m, false, true, pn);
+    if (m.gated) {
+      const int nc = std::max(1, m.gp.node_cnt);
+      ws.h.ensure(sizeof(T) * n1 * Cf::F);
+      ws.hdot.ensure(sizeof(T) * n1 * Cf::F);
+      ws.vg.ensure(sizeof(T) * n1 * nc);
+      ws.vgd.ensure(sizeof(T) * n1 * nc);
+      ws.gmom.ensure(sizeof(T) * n1 * Cf::KH);
+      ws.gmomd.ensure(sizeof(T) * n1 * Cf::KH);
+      ws.msg.ensure(sizeof(T) * ne * 2 * Cf::F);
+      ws.hbar.ensure(sizeof(T) * n1 * 2 * Cf::F);
+      const GradLds<Cf> l1 = grad_lds<Cf>(m, true, false, 0);
+      allow_lds(k_grad_gate_forward<Cf>, l1.bytes());
+      SUS2_LAUNCH(ws, st, kname<Cf>("grad_gate_forward"), (k_grad_gate_forward<Cf>), dim3(N), dim3(WAVE), l1.bytes(),
+                  m, l1, gw, b.types, off, rec, ws.h.get<T>(), ws.hdot.get<T>(), ws.vg.get<T>(), ws.vgd.get<T>(),
+                  ws.gmom.get<T>(), ws.gmomd.get<T>());
+    }
+    allow_lds(k_grad_outer<Cf>, lo.bytes());
+    SUS2_LAUNCH(ws, st, kname<Cf>("grad_outer"), (k_grad_outer<Cf>), dim3(G), dim3(WAVE), lo.bytes(), m, lo, gw, N,
+                b.types, off, rec, ws.h.get<T>(), ws.hdot.get<T>(), ws.msg.get<T>(), ws.gpart.get<double>(), oW, nW);
+    if (m.gated) {
+      const GradLds<Cf> l3 = grad_lds<Cf>(m, true, true, pn);
+      allow_lds(k_grad_gate_reverse<Cf>, l3.bytes());
+      SUS2_LAUNCH(ws, st, kname<Cf>("grad_gate_reverse"), (k_grad_gate_reverse<Cf>), dim3(G), dim3(WAVE), l3.bytes(),
+                  m, l3, gw, N, b.types, off, ws.in_edges.get<int>(), rec, ws.vg.get<T>(), ws.vgd.get<T>(),
+                  ws.gmom.get<T>(), ws.gmomd.get<T>(), ws.msg.get<T>(), ws.hbar.get<T>(), ws.gpart.get<double>(), oW,
+                  nW);
+      const int chunk = 64;
+      wrows = (N + chunk - 1) / chunk;
+      ws.wpart.ensure(sizeof(double) * (size_t)wrows * nW);
+      SUS2_LAUNCH(ws, st, kname<Cf>("grad_W"), k_grad_W<T>, dim3((nW + 255) / 256, wrows), dim3(256), 0, N, Cf::F,
+                  m.Qg, chunk, ws.hbar.get<T>(), ws.vg.get<T>(), ws.vgd.get<T>(), m.gp.node_cnt, m.gp.node_lo,
+                  m.gate_nodes, ws.wpart.get<double>());
+    }
+  }
+  SUS2_LAUNCH(ws, st, "grad_reduce", k_grad_reduce, dim3((ncoeff + 255) / 256), dim3(256), 0, ncoeff,
+              N > 0 ? G : 0, pn, ws.gpart.get<double>(), oW, nW, wrows, ws.wpart.get<double>(), scale, add ? 1 : 0,
+              grad);
+}
+
+}  // namespace sus2

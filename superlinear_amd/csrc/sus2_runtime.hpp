@@ -69,8 +69,13 @@ struct Workspace {
   DevBuf gedge, uedge, in_edges, in_cur;
   // cell-list neighbour build
   DevBuf grid, cell_of, wrap, cell_cnt, cell_start, cell_cur, cell_atoms;
-  // parameter-gradient scratch
-  DevBuf gtan, hdot, ebar, wedge, tgt, gpart, gacc;
+  // parameter gradient: gate tangent h_dot, gate node tangents, type-free gate
+  // moments and tangents, per-edge gate messages, gathered h_bar, per-workgroup
+  // FP64 partials (coefficients; gate weights W)
+  DevBuf hdot, vgd, gmom, gmomd, msg, hbar, gpart, wpart;
+  // training objective: EFS outputs, loss adjoints and per-frame losses (FP64)
+  DevBuf lE, lF, lV, lew, lfw, lsw, lloss;
+  int grad_groups = 1024;  // workgroups of the grid-stride gradient kernels
   long nedges = 0;
   int64_t* host_total = nullptr;  // pinned
   ~Workspace();
@@ -103,7 +108,16 @@ template <> inline const DModel<float>& DevModelSet::get<float>() const { return
 
 void efs_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const EfsOut& out,
                   hipStream_t st);
+// grad (+= when add, else =) scale * parameter gradient of the weighted EFS
+// combination; reuse_neighbors skips the neighbour build when the batch's list
+// of this precision is current (built by the preceding EFS call).
 void grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* ene_w,
-                   const double* frc_w, const double* str_w, double* grad, hipStream_t st);
+                   const double* frc_w, const double* str_w, double* grad, double scale, bool add,
+                   bool reuse_neighbors, hipStream_t st);
+// Training objective (1/nframes) sum_f L_f of the batch with its parameter
+// gradient (grad overwritten); labels e0/f0/s0 on the device, loss to host.
+void loss_grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* e0,
+                        const double* f0, const double* s0, int mode, double wE, double wF, double wS,
+                        double* loss_host, double* grad, hipStream_t st);
 
 }  // namespace sus2

@@ -157,6 +157,114 @@ class Sus2Model:
         return out
 
 
+    # ------------------------------------------------------------ gradients
+    def efs_grad_device(self, batch, ene_w, frc_w, str_w, precision="fp64", grad=None, stream=None):
+        """grad += d/dtheta [sum_i ene_w_i E_i + sum_e w_e . dE/dd_e] with
+        w_e = frc_w_i - frc_w_j + str_w(frame) d_e  (AccumulateEFSCombinationGrad,
+        dev_src/mtpr.cpp:9188).  Weights are FP64 device tensors: ene_w [N],
+        frc_w [N,3], str_w [nf,3,3]; grad is an FP64 device tensor [ncoeff]."""
+        import torch
+        dev = batch.pos.device
+        if grad is None:
+            grad = torch.zeros(self.coeff_count, dtype=torch.float64, device=dev)
+        ew = ene_w.to(device=dev, dtype=torch.float64).contiguous()
+        fw = frc_w.to(device=dev, dtype=torch.float64).contiguous()
+        sw = str_w.to(device=dev, dtype=torch.float64).contiguous()
+        if ew.numel() != batch.natoms or fw.numel() != 3 * batch.natoms or sw.numel() != 9 * batch.nframes:
+            raise ValueError("weight shapes do not match the batch")
+        st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        _lib.check(_lib.lib().sus2_efs_grad(self._h, PRECISION[precision], batch.nframes, batch.natoms,
+                                            batch.off.data_ptr(), batch.types.data_ptr(), batch.pos.data_ptr(),
+                                            batch.cell.data_ptr(), batch.periodic.data_ptr(), ew.data_ptr(),
+                                            fw.data_ptr(), sw.data_ptr(), grad.data_ptr(), ctypes.c_void_p(st)))
+        return grad
+
+    def loss_grad_device(self, batch, e0, f0, s0=None, weighting="structures", wE=1.0, wF=0.01, wS=0.0,
+                         precision="fp32", grad=None, stream=None):
+        """Training objective (1/nframes) sum_f L_f and its gradient
+        (NonLinearRegression::CalcObjectiveFunctionGrad,
+        dev_src/non_linear_regression.cpp:1112).  Labels are device tensors:
+        e0 [nf], f0 [N,3], s0 [nf,6] (xx yy zz xy xz yz).  Returns
+        (loss, grad) with grad an FP64 device tensor (overwritten)."""
+        import torch
+        dev = batch.pos.device
+        mode = {"vibrations": 0, "structures": 1, "molecules": 2}.get(weighting, weighting)
+        if grad is None:
+            grad = torch.empty(self.coeff_count, dtype=torch.float64, device=dev)
+        e0 = e0.to(device=dev, dtype=torch.float64).contiguous()
+        f0 = f0.to(device=dev, dtype=torch.float64).contiguous()
+        s0 = (torch.zeros((batch.nframes, 6), dtype=torch.float64, device=dev) if s0 is None
+              else s0.to(device=dev, dtype=torch.float64).contiguous())
+        loss = ctypes.c_double()
+        st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        _lib.check(_lib.lib().sus2_loss_grad(self._h, PRECISION[precision], batch.nframes, batch.natoms,
+                                             batch.off.data_ptr(), batch.types.data_ptr(), batch.pos.data_ptr(),
+                                             batch.cell.data_ptr(), batch.periodic.data_ptr(), e0.data_ptr(),
+                                             f0.data_ptr(), s0.data_ptr(), int(mode), float(wE), float(wF),
+                                             float(wS), ctypes.byref(loss), grad.data_ptr(), ctypes.c_void_p(st)))
+        return loss.value, grad
+
+
+def _str_weights_from_virial_grad(gV):
+    """str_w (row-major 3x3 per frame) that turns sum_e w_e . g_e into
+    sum_k gV_k V_k for the PlusStress Voigt components V = -sum d (x) g
+    (xx yy zz xy xz yz = -sum d_x g_x, d_y g_y, d_z g_z, d_x g_y, d_x g_z, d_y g_z)."""
+    import torch
+    nf = gV.shape[0]
+    sw = torch.zeros((nf, 3, 3), dtype=torch.float64, device=gV.device)
+    # w_e[q] = sum_b sw[q][b] d_b contracts with g_q: V_ab = -sum d_a g_b -> sw[b][a] = -gV_ab
+    for k, (a, b) in enumerate(((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))):
+        sw[:, b, a] = -gV[:, k]
+    return sw
+
+
+class EFSFunction:
+    """torch.autograd.Function over the coefficient vector: forward = batch
+    EFS on the GPU, backward = the hand-written parameter adjoint
+    (sus2_efs_grad), so that torch optimisers and losses can train the model.
+
+        E, F, V = EFSFunction.apply(theta, model, batch, "fp32")
+
+    theta is an FP64 tensor [ncoeff] (host or device); E [nf], F [N,3],
+    V [nf,6] are FP64 device tensors.  Gradients flow to theta only."""
+
+    _fn = None
+
+    @classmethod
+    def apply(cls, theta, model, batch, precision="fp32"):
+        return cls._function().apply(theta, model, batch, precision)
+
+    @classmethod
+    def _function(cls):
+        if cls._fn is None:
+            import torch
+
+            class _F(torch.autograd.Function):
+                @staticmethod
+                def forward(ctx, theta, model, batch, precision):
+                    model.set_coeffs(theta.detach().to("cpu", torch.float64).numpy())
+                    out = model.efs_device(batch, precision)
+                    ctx.model, ctx.batch, ctx.precision = model, batch, precision
+                    ctx.theta_device = theta.device
+                    return out["energy"], out["forces"], out["virial"]
+
+                @staticmethod
+                def backward(ctx, gE, gF, gV):
+                    b = ctx.batch
+                    dev = b.pos.device
+                    gE = torch.zeros(b.nframes, dtype=torch.float64, device=dev) if gE is None else gE.double()
+                    gF = torch.zeros((b.natoms, 3), dtype=torch.float64, device=dev) if gF is None else gF.double()
+                    gV = torch.zeros((b.nframes, 6), dtype=torch.float64, device=dev) if gV is None else gV.double()
+                    counts = (b.off[1:] - b.off[:-1]).to(torch.long)
+                    ene_w = torch.repeat_interleave(gE, counts)
+                    grad = ctx.model.efs_grad_device(b, ene_w, gF, _str_weights_from_virial_grad(gV),
+                                                     ctx.precision)
+                    return grad.to(ctx.theta_device), None, None, None
+
+            cls._fn = _F
+        return cls._fn
+
+
 class DeviceBatch:
     """Frames packed into device-resident torch tensors (HBM), the layout the
     device entry points read."""

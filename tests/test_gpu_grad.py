@@ -1,0 +1,152 @@
+"""GPU parity of the parameter-gradient path (the forward-over-reverse HIP
+kernels behind sus2_efs_grad and sus2_loss_grad, called through the C ABI)
+against the FP64 oracle's hyper-dual gradient, which is itself pinned by
+central finite differences in tests/test_oracle.py."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# FP64 kernels: gradient within 1e-10 of the oracle (relative, vector norm);
+# FP32 bulk arithmetic with FP64 accumulation of every parameter sum.
+TOL64 = 1e-10
+TOL32 = 1e-5
+
+
+@pytest.fixture(scope="module")
+def lib(gpu_available):
+    from superlinear_amd import _lib
+    _lib.lib()
+    return _lib
+
+
+def _frames(kind, species, seed):
+    from superlinear_amd import workloads as W
+    if kind == "periodic":
+        return W.periodic_frames(2, 8, 14, species, 0.15, seed=seed)
+    if kind == "molecule":
+        return W.molecule_frames(2, 6, 10, species, seed=seed)
+    if kind == "mixed":
+        return W.periodic_frames(1, 8, 12, species, 0.15, seed=seed) + W.molecule_frames(1, 6, 9, species, seed=seed)
+    raise ValueError(kind)
+
+
+def _weights(n, nf, seed):
+    rng = np.random.default_rng(seed)
+    return rng.normal(size=n), rng.normal(size=(n, 3)), rng.normal(size=(nf, 9))
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+CASES = [
+    ("static_l3k4o2", "periodic", 1, 51),
+    ("effect_l3k4r2o2", "periodic", 2, 52),
+    ("effect_l3k4r2o2", "molecule", 2, 53),
+    ("effect_l3k4r2o2", "mixed", 2, 54),
+    ("effect_l4k6r2o3", "mixed", 3, 55),
+]
+
+
+@pytest.mark.parametrize("mname,kind,species,seed", CASES, ids=[f"{c[0]}-{c[1]}" for c in CASES])
+def test_efs_grad_matches_oracle(lib, oracle_lib, mname, kind, species, seed):
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    print(f"[case] efs_grad {mname} {kind}", flush=True)
+    path = W.model_path(mname)
+    frames = _frames(kind, species, seed)
+    n = sum(len(f["types"]) for f in frames)
+    ew, fw, sw = _weights(n, len(frames), seed)
+    ref = oracle_lib.OracleModel(path).efs_grad(frames, ew, fw, sw, nthreads=16)
+    m = Sus2Model(path)
+    batch = DeviceBatch(frames)
+    args = [torch.from_numpy(x).cuda() for x in (ew, fw, sw)]
+    g64 = m.efs_grad_device(batch, *args, precision="fp64").cpu().numpy()
+    e64 = _rel(g64, ref)
+    # accumulation: a second call adds the same gradient again
+    g2 = m.efs_grad_device(batch, *args, precision="fp64", grad=torch.from_numpy(g64).cuda()).cpu().numpy()
+    g32 = m.efs_grad_device(batch, *args, precision="fp32").cpu().numpy()
+    e32 = _rel(g32, ref)
+    print(f"[result] {mname} {kind}: fp64 rel {e64:.3e}  fp32 rel {e32:.3e}", flush=True)
+    assert e64 <= TOL64
+    assert np.abs(g64 - ref).max() <= 1e-9 * (1 + np.abs(ref).max())
+    assert _rel(g2, 2 * ref) <= TOL64
+    assert e32 <= TOL32
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_loss_grad_matches_oracle(lib, oracle_lib, mode):
+    """Training objective and gradient against the oracle's loss, with labels
+    from a teacher model (perturbed coefficients of the same model)."""
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    print(f"[case] loss_grad mode {mode}", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frames = _frames("mixed", 2, 60 + mode)
+    n = sum(len(f["types"]) for f in frames)
+    om = oracle_lib.OracleModel(path)
+    theta = om.coeffs()
+    rng = np.random.default_rng(70 + mode)
+    om.set_coeffs(theta * (1 + 0.05 * rng.normal(size=theta.size)))
+    teach = om.efs(frames)
+    om.set_coeffs(theta)
+    labels = dict(energy=teach["energy"], forces=teach["forces"], virial=teach["virial"])
+    kw = dict(mode=mode, wE=1.0, wF=0.01, wS=0.001)
+    ref = om.loss_grad(frames, labels, nthreads=16, **kw)
+    m = Sus2Model(path)
+    batch = DeviceBatch(frames)
+    e0 = torch.from_numpy(teach["energy"]).cuda()
+    f0 = torch.from_numpy(teach["forces"]).cuda()
+    s0 = torch.from_numpy(teach["virial"]).cuda()
+    for prec, tol in (("fp64", TOL64), ("fp32", TOL32)):
+        loss, g = m.loss_grad_device(batch, e0, f0, s0, weighting=mode, wE=1.0, wF=0.01, wS=0.001, precision=prec)
+        g = g.cpu().numpy()
+        print(f"[result] mode {mode} {prec}: loss rel {abs(loss - ref['loss']) / abs(ref['loss']):.3e} "
+              f"grad rel {_rel(g, ref['grad']):.3e}", flush=True)
+        assert abs(loss - ref["loss"]) <= (1e-12 if prec == "fp64" else 1e-6) * abs(ref["loss"])
+        assert _rel(g, ref["grad"]) <= tol
+
+
+def test_autograd_function_backward(lib, oracle_lib):
+    """EFSFunction: forward equals sus2_efs; backward of a linear functional of
+    (E, F, V) equals the oracle's EFS-combination gradient with the matching
+    weights."""
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import DeviceBatch, EFSFunction, Sus2Model
+    print("[case] autograd", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frames = _frames("mixed", 2, 80)
+    n, nf = sum(len(f["types"]) for f in frames), len(frames)
+    rng = np.random.default_rng(81)
+    a, b, c = rng.normal(size=nf), rng.normal(size=(n, 3)), rng.normal(size=(nf, 6))
+    m = Sus2Model(path)
+    batch = DeviceBatch(frames)
+    theta = torch.tensor(m.coeffs(), dtype=torch.float64, requires_grad=True)
+    E, F, V = EFSFunction.apply(theta, m, batch, "fp64")
+    J = (E * torch.from_numpy(a).cuda()).sum() + (F * torch.from_numpy(b).cuda()).sum() + \
+        (V * torch.from_numpy(c).cuda()).sum()
+    J.backward()
+    counts = [len(f["types"]) for f in frames]
+    ew = np.repeat(a, counts)
+    sw = np.zeros((nf, 3, 3))
+    for k, (p, q) in enumerate(((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))):
+        sw[:, q, p] = -c[:, k]
+    ref = oracle_lib.OracleModel(path).efs_grad(frames, ew, b, sw.reshape(nf, 9), nthreads=16)
+    assert _rel(theta.grad.numpy(), ref) <= TOL64
+
+
+def test_loss_grad_invalid_weighting(lib):
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd._lib import Sus2Error
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    frames = _frames("periodic", 2, 90)
+    m = Sus2Model(W.model_path("effect_l3k4r2o2"))
+    batch = DeviceBatch(frames)
+    n = batch.natoms
+    with pytest.raises(Sus2Error):
+        m.loss_grad_device(batch, torch.zeros(2).cuda(), torch.zeros(n, 3).cuda(), weighting=7)

@@ -220,3 +220,65 @@ def test_loss_gradient_by_finite_differences(oracle_lib):
         assert abs(fd - g[p]) <= 1e-5 * abs(g[p]) + 1e-9 * (1 + abs(out["loss"])), p
     fds = np.array(fds)
     assert np.linalg.norm(fds - g[idx]) <= 1e-6 * np.linalg.norm(g[idx])
+
+
+def _str_w_from_voigt(c):
+    """Edge-weight matrix (row-major, w_e = S d_e) whose sum_e w_e . dE/dd_e
+    equals sum_k c_k V_k for PlusStress Voigt V = -sum d (x) g
+    (xx, yy, zz, xy, xz, yz = -sum d_x g_x, d_y g_y, d_z g_z, d_x g_y, d_x g_z, d_y g_z)."""
+    S = np.zeros((3, 3))
+    for k, (a, b) in enumerate(((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))):
+        S[b, a] = -c[k]
+    return S.ravel()
+
+
+@pytest.mark.parametrize("mname,species", [("effect_l3k4r2o2", 2), ("static_l3k4o2", 1)])
+def test_efs_grad_weights_by_finite_differences(oracle_lib, mname, species):
+    """orc_efs_grad with arbitrary weights equals the central finite difference
+    of J(theta) = sum_i a_i E_i + sum_i b_i . F_i + sum_f c_f . V_f, so the edge
+    weights w_e = b_i - b_j + S_f d_e (S_f from c_f) reproduce the force and
+    virial terms (src/basic_mlip.cpp:421-446)."""
+    from superlinear_amd import workloads as W
+    m = oracle_lib.OracleModel(W.model_path(mname))
+    frames = W.periodic_frames(2, 5, 7, species, 0.15, seed=41)
+    n = sum(len(f["types"]) for f in frames)
+    rng = np.random.default_rng(5)
+    a, b, c = rng.normal(size=n), rng.normal(size=(n, 3)), rng.normal(size=(2, 6))
+    sw = np.stack([_str_w_from_voigt(c[f]) for f in range(2)])
+    g = m.efs_grad(frames, a, b, sw, nthreads=8)
+
+    def J():
+        r = m.efs(frames)
+        return float(a @ r["site"] + (b * r["forces"]).sum() + (c * r["virial"]).sum())
+
+    theta = m.coeffs()
+    idx = np.unique(np.concatenate([np.argsort(-np.abs(g))[:25], rng.choice(len(theta), 25, replace=False)]))
+    h = 1e-6
+    fds = []
+    for p in idx:
+        t = theta.copy()
+        t[p] += h
+        m.set_coeffs(t)
+        jp = J()
+        t[p] -= 2 * h
+        m.set_coeffs(t)
+        jm = J()
+        m.set_coeffs(theta)
+        fds.append((jp - jm) / (2 * h))
+        assert abs(fds[-1] - g[p]) <= 1e-5 * abs(g[p]) + 1e-8, p
+    fds = np.array(fds)
+    assert np.linalg.norm(fds - g[idx]) <= 1e-6 * np.linalg.norm(g[idx])
+
+
+def test_efs_grad_agrees_with_loss_grad(oracle_lib):
+    """The loss gradient is the EFS-combination gradient with the loss
+    adjoints as weights, divided by the frame count."""
+    from superlinear_amd import workloads as W
+    m = oracle_lib.OracleModel(W.model_path("effect_l3k4r2o2"))
+    frames = W.periodic_frames(2, 5, 7, 2, 0.15, seed=43)
+    n = sum(len(f["types"]) for f in frames)
+    rng = np.random.default_rng(7)
+    labels = dict(energy=rng.normal(size=2), forces=rng.normal(size=(n, 3)) * 0.1, virial=rng.normal(size=(2, 6)))
+    out = m.loss_grad(frames, labels, mode=2, wE=1.0, wF=0.01, wS=0.001, nthreads=8)
+    g = m.efs_grad(frames, out["ebar"], out["fbar"], out["sbar"], nthreads=8) / 2
+    close(g, out["grad"], 1e-12)
