@@ -11,6 +11,14 @@ A step = one EFS evaluation of the frame (device neighbour list, gate pass,
 outer pass, gather-based force/virial backward, FP64 per-frame reductions)
 with inputs resident in HBM.  `value` is FP32 atom-steps/s; the FP64 rate, the
 FP32/FP64 speed ratio and the FP32 errors against the FP64 oracle ride along.
+
+Training workload ("train256", reported beside the headline figure): the same
+model form, 256 periodic frames with atom counts uniform in [8, 216], random
+triclinic cells of density 0.17-0.19 atoms/A^3, sigma = 0.15 A, labels (E, F)
+from a teacher model (the same coefficients perturbed with a fixed seed).  A
+training step = the objective (weighting "structures", energy weight 1, force
+weight 0.01: E/atom MSE + 0.01 force MSE) and its full parameter gradient for
+the whole batch (EFS, loss adjoints, forward-over-reverse gradient kernels).
 """
 import argparse
 import json
@@ -42,6 +50,74 @@ def _timed(model, batch, precision, steps, warmup, profile):
     model.profile(False)
     prof = model.profile_read() if profile else {}
     return t1 - t0, out, prof
+
+
+def _timed_train(model, batch, labels, precision, steps, warmup, profile):
+    import torch
+    for _ in range(warmup):
+        model.loss_grad_device(batch, *labels, weighting="structures", wE=1.0, wF=0.01, wS=0.0, precision=precision)
+    torch.cuda.synchronize()
+    model.profile_reset()
+    model.profile(profile)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss, g = model.loss_grad_device(batch, *labels, weighting="structures", wE=1.0, wF=0.01, wS=0.0,
+                                         precision=precision)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    model.profile(False)
+    prof = model.profile_read() if profile else {}
+    return t1 - t0, loss, g.cpu().numpy(), prof
+
+
+def train_section(path, steps, warmup, profile):
+    """Loss + parameter gradient over the 256-frame training batch, FP32 and FP64."""
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd.flops import PEAK, kernel_flops, survey_grad_per_edge
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    frames = W.config_frames("train256")
+    nf = len(frames)
+    batch = DeviceBatch(frames)
+    teacher = Sus2Model(path)
+    th = teacher.coeffs()
+    rng = np.random.default_rng(3031)
+    teacher.set_coeffs(th + 0.05 * np.abs(th) * rng.standard_normal(th.size))
+    lab = teacher.efs_device(batch, "fp64")
+    labels = (lab["energy"], lab["forces"], None)
+    model = Sus2Model(path)
+    dims = model.dims()
+    print(f"[bench] train256: {nf} frames, {batch.natoms} atoms", file=sys.stderr, flush=True)
+    t32, l32, g32, prof = _timed_train(model, batch, labels, "fp32", steps, warmup, profile)
+    nedges = model.last_edge_count()
+    t64, l64, g64, _ = _timed_train(model, batch, labels, "fp64", steps, warmup, False)
+    out = dict(
+        train_grad_frames_per_s=round(nf * steps / t32, 2),
+        train_grad_fp64_frames_per_s=round(nf * steps / t64, 2),
+        train_fp32_over_fp64=round(t64 / t32, 3),
+        train_ms_per_step_fp32=round(1e3 * t32 / steps, 3),
+        train_ms_per_step_fp64=round(1e3 * t64 / steps, 3),
+        train_config={"workload": "train256", "frames": nf, "atoms": batch.natoms, "directed_edges": nedges,
+                      "loss": "weighting structures, wE 1, wF 0.01, wS 0", "coefficients": len(g32)},
+        train_errors_fp32_vs_fp64_gpu=dict(loss_rel=float(abs(l32 - l64) / abs(l64)),
+                                           grad_rel=float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))),
+        train_step_survey_tflops=round(survey_grad_per_edge(dims) * nedges * steps / t32 / 1e12, 4),
+    )
+    if prof:
+        out["kernels_train_fp32"] = {k: dict(avg_us=round(1e3 * ms / max(c, 1), 2), launches=c)
+                                     for k, (ms, c) in prof.items()}
+        grad = {k: v for k, v in prof.items() if k.startswith("grad_") and kernel_flops(k, dims, 1)}
+        if grad:
+            dom = max(grad.items(), key=lambda kv: kv[1][0])[0]
+            avg_s = prof[dom][0] / prof[dom][1] / 1e3
+            fl = kernel_flops(dom, dims, nedges)
+            ach = fl / avg_s / 1e12
+            out["train_roofline"] = dict(bound="mfma", pipe="valu", kernel=dom, achieved=round(ach, 4),
+                                         peak=PEAK["fp32_tflops"], unit="TFLOP/s",
+                                         frac=round(ach / PEAK["fp32_tflops"], 5), flops_per_launch=fl,
+                                         avg_launch_us=round(avg_s * 1e6, 2))
+    return out
 
 
 def cpu_baseline(model_path, frame, threads, budget_s=20.0):
@@ -92,6 +168,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-train", action="store_true", help="skip the training (loss + gradient) workload")
+    ap.add_argument("--train-steps", type=int, default=5)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed runs, launch a plain FP32 copy kernel (y = 2x) over this many MiB, "
                          "three times: the calibration dispatch for rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -161,6 +239,7 @@ def main():
     survey_rate = survey_efs_per_edge(dims) * nedges * args.steps / t32 / 1e12
     if roof is not None:
         roof["traffic"] = measured_traffic(roof["kernel"])
+    train = {} if args.no_train else train_section(path, args.train_steps, 1, not args.no_profile)
 
     if args.copy_probe_mib > 0:
         x = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
@@ -190,7 +269,7 @@ def main():
         "fp64_atom_steps_per_s": round(rate64, 1),
         "fp64_ms_per_step": round(1e3 * t64 / args.steps, 4),
         "fp32_over_fp64": round(rate32 / rate64, 3),
-        "train_grad_frames_per_s": None,
+        "train_grad_frames_per_s": train.get("train_grad_frames_per_s"),
         "step_survey_tflops": round(survey_rate, 4),
         "errors_vs_fp64_oracle": errors,
         "kernels_fp32": kern,
@@ -198,6 +277,7 @@ def main():
         "cpu_baseline": cpu,
         "gpu": torch.cuda.get_device_name(0),
     }
+    line.update({k: v for k, v in train.items() if k != "train_grad_frames_per_s"})
     print(json.dumps(line))
     return 0
 

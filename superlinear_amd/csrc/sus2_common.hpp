@@ -63,8 +63,14 @@ template <> constexpr const char* tname<double>() { return "f64"; }
 
 template <class T>
 void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st);
+// Per-species site-energy constant species[t] + shift[t] (FP64 coefficients).
+struct SiteConst {
+  const double* theta;
+  int o_species, o_shift;
+  __device__ __forceinline__ double operator()(int t) const { return theta[o_species + t] + theta[o_shift + t]; }
+};
 template <class T>
-void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st);
+void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, const SiteConst& sc, hipStream_t st);
 template <class Cf>
 void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st);
 

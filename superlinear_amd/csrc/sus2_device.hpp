@@ -45,7 +45,6 @@ struct DModel {
   __device__ __forceinline__ T readout(int t, int q) const {
     return species_readout ? theta[o_read + t * Q + q] : theta[o_species + t] * theta[o_read + q];
   }
-  __device__ __forceinline__ T site0(int t) const { return theta[o_species + t] + theta[o_shift + t]; }
   __device__ __forceinline__ T rankw(bool gate, int l, int a, int k) const {
     return gate ? theta[o_gw + (l * R + a) * K + k] : theta[o_ow + (l * O + a) * K + k];
   }

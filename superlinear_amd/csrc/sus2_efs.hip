@@ -183,9 +183,15 @@ __global__ void k_zero_int(int n, int* __restrict__ a) {
   if (i < n) a[i] = 0;
 }
 
-// One wavefront per atom: lanes test 64 candidates of a bin at a time; hits
-// are compacted in candidate order with a ballot.  Count pass when rec is
-// null, fill pass otherwise.
+// One wavefront per atom.  The candidate space of the atom is flattened:
+// bin offsets in (o2, o1, o0) order, then the atoms of each bin in bin order.
+// Up to 64 offsets at a time are resolved by the lanes (bin, image shift, bin
+// size), their sizes prefix-summed across the wave, and then every lane takes
+// one candidate of the flattened range (binary search for its offset), so a
+// step tests 64 candidates however small the bins are.  Hits are compacted in
+// candidate order with a ballot.  Count pass when rec is null, fill otherwise.
+__device__ __forceinline__ int floor_div(int a, int n) { return a >= 0 ? a / n : -((-a + n - 1) / n); }
+
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, int nframes, const int64_t* __restrict__ frame_off,
                                                     const double* __restrict__ pos, const double* __restrict__ cell,
@@ -194,6 +200,7 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, int nframes, con
                                                     const int* __restrict__ cell_atoms, double rc,
                                                     const int* __restrict__ off, int* __restrict__ cnt,
                                                     Rec<T>* __restrict__ rec) {
+  __shared__ int o_b[WAVE], o_cum[WAVE], o_s[3][WAVE];
   const int i = blockIdx.x, lane = threadIdx.x;
   if (i >= natoms) return;
   const int f = frame_of(frame_off, nframes, i);
@@ -210,46 +217,70 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, int nframes, con
   const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
   int count = 0;
   const int w0 = rec ? off[i] : 0;
-  for (int o2 = -g.S[2]; o2 <= g.S[2]; ++o2) {
-    int z = c2 + o2, s2 = 0;
-    if (g.per) { s2 = (int)floor((double)z / g.n[2]); z -= s2 * g.n[2]; }
-    else if (z < 0 || z >= g.n[2]) continue;
-    for (int o1 = -g.S[1]; o1 <= g.S[1]; ++o1) {
-      int y = c1 + o1, s1 = 0;
-      if (g.per) { s1 = (int)floor((double)y / g.n[1]); y -= s1 * g.n[1]; }
-      else if (y < 0 || y >= g.n[1]) continue;
-      for (int o0 = -g.S[0]; o0 <= g.S[0]; ++o0) {
-        int x = c0 + o0, s0 = 0;
-        if (g.per) { s0 = (int)floor((double)x / g.n[0]); x -= s0 * g.n[0]; }
-        else if (x < 0 || x >= g.n[0]) continue;
+  const int n0 = 2 * g.S[0] + 1, n1 = 2 * g.S[1] + 1, n2 = 2 * g.S[2] + 1, nof = n0 * n1 * n2;
+  for (int ob = 0; ob < nof; ob += WAVE) {
+    const int o = ob + lane;
+    int nb = 0, bs = 0, s0 = 0, s1 = 0, s2 = 0;
+    if (o < nof) {
+      int x = c0 + o % n0 - g.S[0], y = c1 + (o / n0) % n1 - g.S[1], z = c2 + o / (n0 * n1) - g.S[2];
+      bool ok = true;
+      if (g.per) {
+        s0 = floor_div(x, g.n[0]); x -= s0 * g.n[0];
+        s1 = floor_div(y, g.n[1]); y -= s1 * g.n[1];
+        s2 = floor_div(z, g.n[2]); z -= s2 * g.n[2];
+      } else {
+        ok = x >= 0 && x < g.n[0] && y >= 0 && y < g.n[1] && z >= 0 && z < g.n[2];
+      }
+      if (ok) {
         const int c = cbase + x + g.n[0] * (y + g.n[1] * z);
-        const int b = cell_start[c], e = cell_start[c + 1];
-        for (int k0 = b; k0 < e; k0 += WAVE) {
-          const int k = k0 + lane;
-          bool hit = false;
-          double d0 = 0, d1 = 0, d2 = 0;
-          int j = 0;
-          if (k < e) {
-            j = cell_atoms[k];
-            d0 = pos[3 * j] - xi; d1 = pos[3 * j + 1] - yi; d2 = pos[3 * j + 2] - zi;
-            bool self = false;
-            if (g.per) {
-              const int k0i = s0 + wi0 - wrap[3 * j], k1i = s1 + wi1 - wrap[3 * j + 1], k2i = s2 + wi2 - wrap[3 * j + 2];
-              self = j == i && k0i == 0 && k1i == 0 && k2i == 0;
-              d0 = d0 + k0i * a[0] + k1i * a[3] + k2i * a[6];
-              d1 = d1 + k0i * a[1] + k1i * a[4] + k2i * a[7];
-              d2 = d2 + k0i * a[2] + k1i * a[5] + k2i * a[8];
-            } else {
-              self = j == i;
-            }
-            hit = !self && d0 * d0 + d1 * d1 + d2 * d2 < rc2;
-          }
-          const unsigned long long mask = __ballot(hit);
-          if (rec && hit) rec[w0 + count + __popcll(mask & below)] = Rec<T>{T(d0), T(d1), T(d2), rec_jbits(j, T(0))};
-          count += __popcll(mask);
-        }
+        bs = cell_start[c];
+        nb = cell_start[c + 1] - bs;
       }
     }
+    int incl = nb;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+      const int t = __shfl_up(incl, d, WAVE);
+      if (lane >= d) incl += t;
+    }
+    const int total = __shfl(incl, WAVE - 1, WAVE);
+    o_b[lane] = bs;
+    o_cum[lane] = incl - nb;
+    o_s[0][lane] = s0; o_s[1][lane] = s1; o_s[2][lane] = s2;
+    __syncthreads();
+    const int nvalid = min(WAVE, nof - ob);
+    for (int k0 = 0; k0 < total; k0 += WAVE) {
+      const int k = k0 + lane;
+      bool hit = false;
+      double d0 = 0, d1 = 0, d2 = 0;
+      int j = 0;
+      if (k < total) {
+        int lo = 0, hi = nvalid;  // largest q with o_cum[q] <= k
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (o_cum[mid] <= k) lo = mid;
+          else hi = mid;
+        }
+        j = cell_atoms[o_b[lo] + k - o_cum[lo]];
+        d0 = pos[3 * j] - xi; d1 = pos[3 * j + 1] - yi; d2 = pos[3 * j + 2] - zi;
+        bool self;
+        if (g.per) {
+          const int k0i = o_s[0][lo] + wi0 - wrap[3 * j], k1i = o_s[1][lo] + wi1 - wrap[3 * j + 1],
+                    k2i = o_s[2][lo] + wi2 - wrap[3 * j + 2];
+          self = j == i && k0i == 0 && k1i == 0 && k2i == 0;
+          d0 = d0 + k0i * a[0] + k1i * a[3] + k2i * a[6];
+          d1 = d1 + k0i * a[1] + k1i * a[4] + k2i * a[7];
+          d2 = d2 + k0i * a[2] + k1i * a[5] + k2i * a[8];
+        } else {
+          self = j == i;
+        }
+        hit = !self && d0 * d0 + d1 * d1 + d2 * d2 < rc2;
+      }
+      const unsigned long long mask = __ballot(hit);
+      if (rec && hit) rec[w0 + count + __popcll(mask & below)] = Rec<T>{T(d0), T(d1), T(d2), rec_jbits(j, T(0))};
+      count += __popcll(mask);
+    }
+    __syncthreads();
   }
   if (!rec && lane == 0) cnt[i] = count;
 }
@@ -271,6 +302,7 @@ __global__ void k_incoming(long nedges, const Rec<T>* __restrict__ rec, const in
 // partials in a fixed order (strided per thread, then a fixed tree).
 template <class T>
 __global__ __launch_bounds__(256) void k_finalize(int nframes, const int64_t* __restrict__ frame_off,
+                                                  const int32_t* __restrict__ types, SiteConst sc,
                                                   const T* __restrict__ site, const T* __restrict__ fw,
                                                   double* __restrict__ energy, double* __restrict__ virial) {
   __shared__ double red[7][256];
@@ -278,7 +310,7 @@ __global__ __launch_bounds__(256) void k_finalize(int nframes, const int64_t* __
   const long b = frame_off[f], e = frame_off[f + 1];
   double s[7] = {0, 0, 0, 0, 0, 0, 0};
   for (long i = b + t; i < e; i += 256) {
-    s[0] += (double)site[i];
+    s[0] += (double)site[i] + sc(types[i]);
     for (int q = 0; q < 6; ++q) s[1 + q] += (double)fw[i * 9 + 3 + q];
   }
   for (int q = 0; q < 7; ++q) red[q][t] = s[q];
@@ -295,14 +327,14 @@ __global__ __launch_bounds__(256) void k_finalize(int nframes, const int64_t* __
 }
 
 template <class T>
-__global__ void k_atom_out(int natoms, const T* __restrict__ site, const T* __restrict__ fw,
-                           double* __restrict__ force, double* __restrict__ site_out) {
+__global__ void k_atom_out(int natoms, const int32_t* __restrict__ types, SiteConst sc, const T* __restrict__ site,
+                           const T* __restrict__ fw, double* __restrict__ force, double* __restrict__ site_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= natoms) return;
   force[3 * i] = (double)fw[9 * (size_t)i];
   force[3 * i + 1] = (double)fw[9 * (size_t)i + 1];
   force[3 * i + 2] = (double)fw[9 * (size_t)i + 2];
-  if (site_out) site_out[i] = (double)site[i];
+  if (site_out) site_out[i] = (double)site[i] + sc(types[i]);
 }
 
 // ------------------------------------------------------------------ driver
@@ -359,19 +391,19 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
 }
 
 template <class T>
-void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st) {
+void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, const SiteConst& sc, hipStream_t st) {
   const int N = b.natoms;
   if (N > 0)
-    SUS2_LAUNCH(ws, st, "atom_out", k_atom_out<T>, dim3((N + 255) / 256), dim3(256), 0, N, ws.site.get<T>(),
-                ws.fw.get<T>(), out.force, out.site);
-  SUS2_LAUNCH(ws, st, "finalize", k_finalize<T>, dim3(b.nframes), dim3(256), 0, b.nframes, b.frame_off,
+    SUS2_LAUNCH(ws, st, "atom_out", k_atom_out<T>, dim3((N + 255) / 256), dim3(256), 0, N, b.types, sc,
+                ws.site.get<T>(), ws.fw.get<T>(), out.force, out.site);
+  SUS2_LAUNCH(ws, st, "finalize", k_finalize<T>, dim3(b.nframes), dim3(256), 0, b.nframes, b.frame_off, b.types, sc,
               ws.site.get<T>(), ws.fw.get<T>(), out.energy, out.virial);
 }
 
 template void build_neighbors<float>(Workspace&, const Batch&, double, hipStream_t);
 template void build_neighbors<double>(Workspace&, const Batch&, double, hipStream_t);
-template void finish_efs<float>(Workspace&, const Batch&, const EfsOut&, hipStream_t);
-template void finish_efs<double>(Workspace&, const Batch&, const EfsOut&, hipStream_t);
+template void finish_efs<float>(Workspace&, const Batch&, const EfsOut&, const SiteConst&, hipStream_t);
+template void finish_efs<double>(Workspace&, const Batch&, const EfsOut&, const SiteConst&, hipStream_t);
 
 #define SUS2_DECLARE(LL, KK, BB, SM)                                                                  \
   extern template void run_efs_shape<Cfg<float, LL, KK, BB, SM>>(const DevModelSet&, Workspace&, const Batch&, \

@@ -217,7 +217,9 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
   }
   T part = T(0);
   for (int q = lane; q < m.Q; q += WAVE) part += m.readout(ti, q) * v[m.scalar_map[q]];
-  const T E = wave_sum(part) + m.site0(ti);
+  // the per-species constant (species + shift coefficients) is added in FP64
+  // when the site energies are written out
+  const T E = wave_sum(part);
   if (lane == 0) site[i] = E;
   for (int n = lane; n < m.M; n += WAVE) a[n] = T(0);
   __syncthreads();
@@ -448,7 +450,7 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
                   ws.fw.get<T>());
     }
   }
-  finish_efs<T>(ws, b, out, st);
+  finish_efs<T>(ws, b, out, SiteConst{dm.d.view.theta, m.o_species, m.o_shift}, st);
 }
 
 }  // namespace sus2

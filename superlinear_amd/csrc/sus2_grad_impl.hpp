@@ -748,20 +748,39 @@ __global__ __launch_bounds__(256) void k_grad_W(int natoms, int F, int Qg, int c
   wpart[(size_t)blockIdx.y * F * Qg + idx] = acc;
 }
 
-// grad[p] (=|+=) scale * sum of the partial rows, fixed order.
-static __global__ __launch_bounds__(256) void k_grad_reduce(int ncoeff, int rows, int pn, const double* __restrict__ part,
-                                                     int oW, int nW, int wrows, const double* __restrict__ wpart,
-                                                     double scale, int add, double* __restrict__ grad) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= ncoeff) return;
+// grad[p] (=|+=) scale * sum of the partial rows in a fixed order: a block
+// covers 64 coefficients; its 16 row groups sum contiguous row ranges, then
+// the 16 group sums are added in group order.
+constexpr int RED_X = 64, RED_Y = 16;
+
+static __global__ __launch_bounds__(RED_X * RED_Y) void k_grad_reduce(int ncoeff, int rows, int pn,
+                                                                      const double* __restrict__ part, int oW,
+                                                                      int nW, int wrows,
+                                                                      const double* __restrict__ wpart, double scale,
+                                                                      int add, double* __restrict__ grad) {
+  __shared__ double red[RED_Y][RED_X];
+  const int x = threadIdx.x, y = threadIdx.y;
+  const int p = blockIdx.x * RED_X + x;
   double acc = 0;
-  if (oW >= 0 && p >= oW && p < oW + nW) {
-    for (int r = 0; r < wrows; ++r) acc += wpart[(size_t)r * nW + (p - oW)];
-  } else {
-    const int k = pidx(p, oW, nW);
-    for (int r = 0; r < rows; ++r) acc += part[(size_t)r * pn + k];
+  if (p < ncoeff) {
+    const bool isw = oW >= 0 && p >= oW && p < oW + nW;
+    const int nr = isw ? wrows : rows;
+    const int chunk = (nr + RED_Y - 1) / RED_Y;
+    const int r0 = y * chunk, r1 = min(nr, r0 + chunk);
+    if (isw) {
+      for (int r = r0; r < r1; ++r) acc += wpart[(size_t)r * nW + (p - oW)];
+    } else {
+      const int k = pidx(p, oW, nW);
+      for (int r = r0; r < r1; ++r) acc += part[(size_t)r * pn + k];
+    }
   }
-  grad[p] = (add ? grad[p] : 0.0) + scale * acc;
+  red[y][x] = acc;
+  __syncthreads();
+  if (y == 0 && p < ncoeff) {
+    double s = 0;
+    for (int g = 0; g < RED_Y; ++g) s += red[g][x];
+    grad[p] = (add ? grad[p] : 0.0) + scale * s;
+  }
 }
 
 // ------------------------------------------------------------------ driver
@@ -817,7 +836,7 @@ void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const 
                   m.gate_nodes, ws.wpart.get<double>());
     }
   }
-  SUS2_LAUNCH(ws, st, "grad_reduce", k_grad_reduce, dim3((ncoeff + 255) / 256), dim3(256), 0, ncoeff,
+  SUS2_LAUNCH(ws, st, "grad_reduce", k_grad_reduce, dim3((ncoeff + RED_X - 1) / RED_X), dim3(RED_X, RED_Y), 0, ncoeff,
               N > 0 ? G : 0, pn, ws.gpart.get<double>(), oW, nW, wrows, ws.wpart.get<double>(), scale, add ? 1 : 0,
               grad);
 }

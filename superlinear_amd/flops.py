@@ -19,6 +19,19 @@ H = (L+1)^2, F = K(L+1), B radial basis size, Kw warp blocks, KH = K H.
 
 ``pass_forward_*`` kernels also run the per-atom product graph, whose cost
 (a few flops per product row per atom) is not counted.
+
+Parameter gradient (forward-over-reverse, per directed edge of one pass):
+
+* ``grad_sweep1`` (moments and their tangent along w):
+  geometry + edge weight 30 + radial basis and r-derivative Kw (13 + 7B)
+  + F (4B + 6) + harmonics and gradient 20H + grad Y . w 5H + staging 4F
+  (+ 2F for the outer pass of an Effect model) + accumulation 6 KH
+* ``grad_sweep2`` (parameter sweep): geometry + edge weight 30 + harmonics and
+  gradient 20H + 5H + adjoint contractions p, q, q' 6 KH + per warp block
+  Kw (20 + 9B) (tanh, Chebyshev values and two xi-derivatives) + per channel
+  F (6B + 30) (+10F outer Effect: gate messages) + radial-coefficient
+  accumulation 11 F B
+* ``survey_grad`` is the whole-step figure F_e^G = F_e^E + 16 KH + 8 FB + 8 Kw B + 4F.
 """
 
 PEAK = {  # MI355X published peaks
@@ -53,6 +66,21 @@ def survey_efs_per_edge(d):
     return s
 
 
+def grad_sweep1_per_edge(d, outer_effect=False):
+    L, K, B, Kw, H, F, KH = dims_of(d)
+    return 30 + Kw * (13 + 7 * B) + F * (4 * B + 6) + 25 * H + 4 * F + (2 * F if outer_effect else 0) + 6 * KH
+
+
+def grad_sweep2_per_edge(d, outer_effect=False):
+    L, K, B, Kw, H, F, KH = dims_of(d)
+    return 30 + 25 * H + 6 * KH + Kw * (20 + 9 * B) + F * (6 * B + 30) + (10 * F if outer_effect else 0) + 11 * F * B
+
+
+def survey_grad_per_edge(d):
+    L, K, B, Kw, H, F, KH = dims_of(d)
+    return survey_efs_per_edge(d) + 16 * KH + 8 * F * B + 8 * Kw * B + 4 * F
+
+
 def kernel_flops(name, d, nedges):
     """Algorithmic flops of one launch of the named kernel over nedges."""
     if name.startswith("pass_forward_"):
@@ -61,4 +89,11 @@ def kernel_flops(name, d, nedges):
         return force_per_edge(d, outer_effect=bool(d.get("gated"))) * nedges
     if name.startswith("edge_backward_gate"):
         return force_per_edge(d) * nedges
+    eff = bool(d.get("gated"))
+    if name.startswith("grad_gate_forward"):
+        return grad_sweep1_per_edge(d) * nedges
+    if name.startswith("grad_outer"):
+        return (grad_sweep1_per_edge(d, eff) + grad_sweep2_per_edge(d, eff)) * nedges
+    if name.startswith("grad_gate_reverse"):
+        return grad_sweep2_per_edge(d) * nedges
     return None

@@ -91,28 +91,28 @@ def random_cell(n, density, rng):
 
 
 def random_positions(n, cell, rng, min_sep=1.0, periodic=True, max_tries=200000):
-    """Uniform positions with rejection of pairs closer than min_sep."""
-    pts = []
+    """Uniform positions with rejection of candidates closer than min_sep to an
+    accepted point (minimum image for periodic cells)."""
+    pts = np.zeros((n, 3))
     inv = np.linalg.inv(cell)
+    k = 0
     tries = 0
-    while len(pts) < n:
+    while k < n:
         tries += 1
         if tries > max_tries:
             raise RuntimeError("could not place atoms with the requested separation")
         p = rng.uniform(0, 1, 3) @ cell
-        ok = True
-        for q in pts:
-            d = p - q
+        if k:
+            d = p - pts[:k]
             if periodic:
                 f = d @ inv
                 f -= np.round(f)
                 d = f @ cell
-            if d @ d < min_sep * min_sep:
-                ok = False
-                break
-        if ok:
-            pts.append(p)
-    return np.array(pts)
+            if np.any(np.einsum("ij,ij->i", d, d) < min_sep * min_sep):
+                continue
+        pts[k] = p
+        k += 1
+    return pts
 
 
 def periodic_frames(count, nmin, nmax, species, sigma, seed, density=(0.17, 0.19), log_uniform=False):
