@@ -352,12 +352,43 @@ struct Edge { int i, j; double d[3]; };
 
 // Full directed neighbour list with every periodic image inside rc
 // (interfaces/lammps/tests/embedded_efs.cpp:44-66 semantics; strict r < rc).
+// A lattice row of zeros marks a non-periodic direction, as in the
+// reference's Configuration (src/neighborhoods.cpp:216-220): images are only
+// taken along the non-zero rows.
 std::vector<Edge> neighbours(int n, const double* x, const double* cell, bool periodic, double rc) {
   std::vector<Edge> out;
   double inv[3][3] = {{0}};
   double reach[3] = {0, 0, 0};
-  if (periodic) {
-    const double* a = cell;  // rows are lattice vectors
+  bool act[3] = {false, false, false};
+  if (periodic)
+    for (int q = 0; q < 3; ++q) act[q] = cell[3 * q] != 0.0 || cell[3 * q + 1] != 0.0 || cell[3 * q + 2] != 0.0;
+  const bool any = act[0] || act[1] || act[2];
+  if (any) {
+    // basis: active lattice rows, completed by unit normals for the open axes
+    double a[9];
+    for (int k = 0; k < 9; ++k) a[k] = cell[k];
+    auto cross = [](const double* u, const double* v, double* w) {
+      w[0] = u[1] * v[2] - u[2] * v[1]; w[1] = u[2] * v[0] - u[0] * v[2]; w[2] = u[0] * v[1] - u[1] * v[0];
+    };
+    auto unit = [](double* w) {
+      const double s = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+      for (int k = 0; k < 3; ++k) w[k] /= s;
+    };
+    const int nact = act[0] + act[1] + act[2];
+    if (nact == 1) {
+      const int p = act[0] ? 0 : act[1] ? 1 : 2;
+      const double* v = a + 3 * p;
+      double e[3] = {0, 0, 0};
+      const int m = std::fabs(v[0]) <= std::fabs(v[1]) && std::fabs(v[0]) <= std::fabs(v[2]) ? 0
+                    : std::fabs(v[1]) <= std::fabs(v[2]) ? 1 : 2;
+      e[m] = 1;
+      const int q1 = (p + 1) % 3, q2 = (p + 2) % 3;
+      cross(v, e, a + 3 * q1); unit(a + 3 * q1);
+      cross(v, a + 3 * q1, a + 3 * q2); unit(a + 3 * q2);
+    } else if (nact == 2) {
+      const int q = !act[0] ? 0 : !act[1] ? 1 : 2;
+      cross(a + 3 * ((q + 1) % 3), a + 3 * ((q + 2) % 3), a + 3 * q); unit(a + 3 * q);
+    }
     double b[3][3];
     for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j)
       b[j][i] = a[((i + 1) % 3) * 3 + (j + 1) % 3] * a[((i + 2) % 3) * 3 + (j + 2) % 3] -
@@ -377,9 +408,10 @@ std::vector<Edge> neighbours(int n, const double* x, const double* cell, bool pe
     double d[3], f[3] = {0, 0, 0};
     for (int q = 0; q < 3; ++q) d[q] = x[3 * j + q] - x[3 * i + q];
     int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    if (periodic) {
+    if (any) {
       for (int q = 0; q < 3; ++q) for (int a2 = 0; a2 < 3; ++a2) f[q] += d[a2] * inv[a2][q];
-      for (int q = 0; q < 3; ++q) { lo[q] = (int)std::ceil(-f[q] - reach[q]); hi[q] = (int)std::floor(-f[q] + reach[q]); }
+      for (int q = 0; q < 3; ++q)
+        if (act[q]) { lo[q] = (int)std::ceil(-f[q] - reach[q]); hi[q] = (int)std::floor(-f[q] + reach[q]); }
     }
     for (int a2 = lo[0]; a2 <= hi[0]; ++a2) for (int b2 = lo[1]; b2 <= hi[1]; ++b2) for (int c2 = lo[2]; c2 <= hi[2]; ++c2) {
       if (i == j && a2 == 0 && b2 == 0 && c2 == 0) continue;

@@ -2,7 +2,8 @@
 
 Reader and writer for the reference's text CFG (src/configuration.cpp:429
 Load, :1093 Save): ``Size``, optional ``Supercell`` rows (absent = a
-non-periodic molecule, src/neighborhoods.cpp:223-227), ``AtomData`` with
+non-periodic molecule, src/neighborhoods.cpp:223-227; one or two rows, or a
+zero row, leave the remaining directions non-periodic, :490-511), ``AtomData`` with
 ``cartes_*`` or ``direct_*`` coordinates and optional ``fx fy fz`` and
 ``site_en`` columns, optional ``Energy`` and ``PlusStress`` (xx yy zz yz xz xy).
 """
@@ -34,11 +35,11 @@ def _frame_from_block(lines):
                 except (ValueError, IndexError):
                     break
                 j += 1
-            cell = np.zeros((3, 3))
+            if not rows:
+                raise ValueError("Supercell without lattice vectors")
+            cell = np.zeros((3, 3))  # missing or zero rows: non-periodic directions
             cell[:len(rows)] = rows
-            if len(rows) != 3:
-                raise ValueError("only fully periodic (3-row) supercells are supported")
-            f["cell"], f["periodic"] = cell, True
+            f["cell"], f["periodic"] = cell, bool(np.any(cell != 0))
             i = j
             continue
         if key == "AtomData:":
@@ -106,9 +107,12 @@ def write_cfg(fh, frame, lossless=False):
     n = len(frame["types"])
     out = ["BEGIN_CFG", " Size", f"    {n}"]
     if frame.get("periodic", False):
-        out.append(" Supercell")
-        for row in np.asarray(frame["cell"]).reshape(3, 3):
-            out.append("    " + " ".join(lat(v) for v in row))
+        cell = np.asarray(frame["cell"]).reshape(3, 3)
+        count = max([q + 1 for q in range(3) if np.any(cell[q] != 0)], default=0)
+        if count:  # rows up to the last non-zero one, as Configuration::Save
+            out.append(" Supercell")
+            for row in cell[:count]:
+                out.append("    " + " ".join(lat(v) for v in row))
     header = " AtomData:  id type       cartes_x      cartes_y      cartes_z"
     if "site_energy" in frame:
         header += "    site_en"

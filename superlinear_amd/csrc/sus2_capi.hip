@@ -84,12 +84,27 @@ void validate_host(const sus2::Model& m, int nframes, int natoms, const int64_t*
     if (types[i] < 0 || types[i] >= m.C) throw sus2::Error(SUS2_E_RANGE, "atom species index out of range");
   for (long i = 0; i < 3L * natoms; ++i)
     if (!std::isfinite(pos[i])) throw sus2::Error(SUS2_E_INVALID, "non-finite coordinate");
+  // periodic lattice rows (the non-zero ones) must be finite and independent
   for (int f = 0; f < nframes; ++f) {
     if (!periodic[f]) continue;
     const double* a = cell + 9 * f;
-    const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
-                       a[2] * (a[3] * a[7] - a[4] * a[6]);
-    if (!(std::fabs(det) > 1e-12) || !std::isfinite(det)) throw sus2::Error(SUS2_E_INVALID, "singular periodic cell");
+    double basis[3][3];
+    int nb = 0;
+    for (int q = 0; q < 3; ++q) {
+      double v[3] = {a[3 * q], a[3 * q + 1], a[3 * q + 2]};
+      if (!std::isfinite(v[0]) || !std::isfinite(v[1]) || !std::isfinite(v[2]))
+        throw sus2::Error(SUS2_E_INVALID, "non-finite lattice vector");
+      if (v[0] == 0.0 && v[1] == 0.0 && v[2] == 0.0) continue;
+      const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      for (int p = 0; p < nb; ++p) {
+        const double d = v[0] * basis[p][0] + v[1] * basis[p][1] + v[2] * basis[p][2];
+        for (int k = 0; k < 3; ++k) v[k] -= d * basis[p][k];
+      }
+      const double r = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      if (!(r > 1e-9 * len)) throw sus2::Error(SUS2_E_INVALID, "singular periodic cell (dependent lattice rows)");
+      for (int k = 0; k < 3; ++k) basis[nb][k] = v[k] / r;
+      ++nb;
+    }
   }
 }
 

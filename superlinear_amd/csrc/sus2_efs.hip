@@ -50,12 +50,16 @@ namespace sus2 {
 // (each (bin, shift) pair is a distinct image).  Bin contents are sorted by
 // atom index, so the edge order -- and every sum over it -- is deterministic.
 struct Grid {
-  double inv[9];  // fractional = cartesian * inv (row vector convention)
-  double org[3];  // molecule bounding-box origin
+  double inv[9];  // fractional = (cartesian - org) * inv (row vector convention)
+  double org[3];  // origin of the open (non-periodic) directions
   int n[3], S[3];
-  int per;
+  int per;  // bit q set: periodic along lattice row q
 };
 
+// Per frame: lattice rows that are non-zero are periodic (a zero row is a
+// non-periodic direction, as in the reference's Configuration); open
+// directions get unit normals scaled to the atoms' extent, so fractional
+// coordinates along them lie in [0, 1).  periodic == 0 frames are molecules.
 __global__ void k_grid_setup(int nframes, const int64_t* __restrict__ frame_off, const double* __restrict__ pos,
                              const double* __restrict__ cell, const int32_t* __restrict__ periodic, double rc,
                              Grid* __restrict__ grid) {
@@ -63,39 +67,69 @@ __global__ void k_grid_setup(int nframes, const int64_t* __restrict__ frame_off,
   if (f >= nframes) return;
   Grid g;
   const long b0 = frame_off[f], na = frame_off[f + 1] - b0;
-  g.per = periodic[f] != 0;
+  double a[9];
+  const double* c = cell + 9 * f;
+  g.per = 0;
+  for (int q = 0; q < 3; ++q) {
+    for (int k = 0; k < 3; ++k) a[3 * q + k] = 0.0;
+    if (periodic[f] && (c[3 * q] != 0.0 || c[3 * q + 1] != 0.0 || c[3 * q + 2] != 0.0)) {
+      g.per |= 1 << q;
+      for (int k = 0; k < 3; ++k) a[3 * q + k] = c[3 * q + k];
+    }
+  }
+  // complete the basis with unit normals for the open rows
+  const int nact = (g.per & 1) + ((g.per >> 1) & 1) + ((g.per >> 2) & 1);
+  auto cross = [](const double* u, const double* v, double* w) {
+    w[0] = u[1] * v[2] - u[2] * v[1]; w[1] = u[2] * v[0] - u[0] * v[2]; w[2] = u[0] * v[1] - u[1] * v[0];
+  };
+  auto unit = [](double* w) {
+    const double s = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    for (int k = 0; k < 3; ++k) w[k] /= s;
+  };
+  if (nact == 0) {
+    a[0] = a[4] = a[8] = 1.0;
+  } else if (nact == 1) {
+    const int p = (g.per & 1) ? 0 : (g.per & 2) ? 1 : 2;
+    const double* v = a + 3 * p;
+    double e[3] = {0, 0, 0};
+    const int m = fabs(v[0]) <= fabs(v[1]) && fabs(v[0]) <= fabs(v[2]) ? 0 : fabs(v[1]) <= fabs(v[2]) ? 1 : 2;
+    e[m] = 1;
+    const int q1 = (p + 1) % 3, q2 = (p + 2) % 3;
+    cross(v, e, a + 3 * q1); unit(a + 3 * q1);
+    cross(v, a + 3 * q1, a + 3 * q2); unit(a + 3 * q2);
+  } else if (nact == 2) {
+    const int q = !(g.per & 1) ? 0 : !(g.per & 2) ? 1 : 2;
+    cross(a + 3 * ((q + 1) % 3), a + 3 * ((q + 2) % 3), a + 3 * q); unit(a + 3 * q);
+  }
+  // open rows: scale the unit normal to the extent of the atoms along it
+  for (int k = 0; k < 3; ++k) g.org[k] = 0.0;
+  for (int q = 0; q < 3; ++q) {
+    if (g.per & (1 << q)) continue;
+    double lo = 0, hi = 0;
+    for (long k = 0; k < na; ++k) {
+      const double* x = pos + 3 * (b0 + k);
+      const double t = x[0] * a[3 * q] + x[1] * a[3 * q + 1] + x[2] * a[3 * q + 2];
+      if (k == 0 || t < lo) lo = t;
+      if (k == 0 || t > hi) hi = t;
+    }
+    const double ext = fmax(hi - lo, 1e-6) * (1 + 1e-9) + 1e-9;
+    for (int k = 0; k < 3; ++k) {
+      g.org[k] += lo * a[3 * q + k];
+      a[3 * q + k] *= ext;
+    }
+  }
+  double b[3][3];
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q)
+      b[q][p] = a[((p + 1) % 3) * 3 + (q + 1) % 3] * a[((p + 2) % 3) * 3 + (q + 2) % 3] -
+                a[((p + 1) % 3) * 3 + (q + 2) % 3] * a[((p + 2) % 3) * 3 + (q + 1) % 3];
+  const double det = a[0] * b[0][0] + a[1] * b[1][0] + a[2] * b[2][0];
   double reach[3];
-  if (g.per) {
-    const double* a = cell + 9 * f;
-    double b[3][3];
-    for (int p = 0; p < 3; ++p)
-      for (int q = 0; q < 3; ++q)
-        b[q][p] = a[((p + 1) % 3) * 3 + (q + 1) % 3] * a[((p + 2) % 3) * 3 + (q + 2) % 3] -
-                  a[((p + 1) % 3) * 3 + (q + 2) % 3] * a[((p + 2) % 3) * 3 + (q + 1) % 3];
-    const double det = a[0] * b[0][0] + a[1] * b[1][0] + a[2] * b[2][0];
-    for (int p = 0; p < 3; ++p)
-      for (int q = 0; q < 3; ++q) g.inv[3 * p + q] = b[p][q] / det;
-    for (int q = 0; q < 3; ++q) {
-      reach[q] = rc * sqrt(g.inv[q] * g.inv[q] + g.inv[3 + q] * g.inv[3 + q] + g.inv[6 + q] * g.inv[6 + q]);
-      g.n[q] = (int)fmin(1024.0, fmax(1.0, floor(1.0 / reach[q])));
-      g.org[q] = 0;
-    }
-  } else {
-    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    for (long k = 0; k < na; ++k)
-      for (int q = 0; q < 3; ++q) {
-        const double x = pos[3 * (b0 + k) + q];
-        if (k == 0 || x < lo[q]) lo[q] = x;
-        if (k == 0 || x > hi[q]) hi[q] = x;
-      }
-    for (int p = 0; p < 9; ++p) g.inv[p] = 0;
-    for (int q = 0; q < 3; ++q) {
-      const double ext = fmax(hi[q] - lo[q], 1e-6) * (1 + 1e-9) + 1e-9;
-      g.org[q] = lo[q];
-      g.inv[4 * q] = 1.0 / ext;
-      reach[q] = rc / ext;
-      g.n[q] = (int)fmin(1024.0, fmax(1.0, floor(ext / rc)));
-    }
+  for (int p = 0; p < 3; ++p)
+    for (int q = 0; q < 3; ++q) g.inv[3 * p + q] = b[p][q] / det;
+  for (int q = 0; q < 3; ++q) {
+    reach[q] = rc * sqrt(g.inv[q] * g.inv[q] + g.inv[3 + q] * g.inv[3 + q] + g.inv[6 + q] * g.inv[6 + q]);
+    g.n[q] = (int)fmin(1024.0, fmax(1.0, floor(1.0 / reach[q])));
   }
   // at most one bin per atom
   while ((long)g.n[0] * g.n[1] * g.n[2] > (na > 0 ? na : 1)) {
@@ -104,7 +138,7 @@ __global__ void k_grid_setup(int nframes, const int64_t* __restrict__ frame_off,
     if (g.n[2] > g.n[q]) q = 2;
     g.n[q] = max(1, g.n[q] - 1);
   }
-  for (int q = 0; q < 3; ++q) g.S[q] = g.per ? (int)floor(reach[q] * g.n[q]) + 1 : 1;
+  for (int q = 0; q < 3; ++q) g.S[q] = (g.per & (1 << q)) ? (int)floor(reach[q] * g.n[q]) + 1 : 1;
   grid[f] = g;
 }
 
@@ -124,7 +158,7 @@ __global__ void k_bin(int natoms, int nframes, const int64_t* __restrict__ frame
   frac_of(g, pos + 3 * i, fr);
   int c[3];
   for (int q = 0; q < 3; ++q) {
-    const double w = g.per ? floor(fr[q]) : 0.0;
+    const double w = (g.per & (1 << q)) ? floor(fr[q]) : 0.0;
     wrap[3 * i + q] = (int)w;
     c[q] = min(g.n[q] - 1, max(0, (int)((fr[q] - w) * g.n[q])));
   }
@@ -224,13 +258,12 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, int nframes, con
     if (o < nof) {
       int x = c0 + o % n0 - g.S[0], y = c1 + (o / n0) % n1 - g.S[1], z = c2 + o / (n0 * n1) - g.S[2];
       bool ok = true;
-      if (g.per) {
-        s0 = floor_div(x, g.n[0]); x -= s0 * g.n[0];
-        s1 = floor_div(y, g.n[1]); y -= s1 * g.n[1];
-        s2 = floor_div(z, g.n[2]); z -= s2 * g.n[2];
-      } else {
-        ok = x >= 0 && x < g.n[0] && y >= 0 && y < g.n[1] && z >= 0 && z < g.n[2];
-      }
+      if (g.per & 1) { s0 = floor_div(x, g.n[0]); x -= s0 * g.n[0]; }
+      else ok = ok && x >= 0 && x < g.n[0];
+      if (g.per & 2) { s1 = floor_div(y, g.n[1]); y -= s1 * g.n[1]; }
+      else ok = ok && y >= 0 && y < g.n[1];
+      if (g.per & 4) { s2 = floor_div(z, g.n[2]); z -= s2 * g.n[2]; }
+      else ok = ok && z >= 0 && z < g.n[2];
       if (ok) {
         const int c = cbase + x + g.n[0] * (y + g.n[1] * z);
         bs = cell_start[c];
@@ -264,7 +297,7 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, int nframes, con
         j = cell_atoms[o_b[lo] + k - o_cum[lo]];
         d0 = pos[3 * j] - xi; d1 = pos[3 * j + 1] - yi; d2 = pos[3 * j + 2] - zi;
         bool self;
-        if (g.per) {
+        if (g.per) {  // open directions have zero shifts and wraps
           const int k0i = o_s[0][lo] + wi0 - wrap[3 * j], k1i = o_s[1][lo] + wi1 - wrap[3 * j + 1],
                     k2i = o_s[2][lo] + wi2 - wrap[3 * j + 2];
           self = j == i && k0i == 0 && k1i == 0 && k2i == 0;

@@ -142,6 +142,22 @@ def molecule_frames(count, nmin, nmax, species, seed, density=0.1):
     return out
 
 
+def partial_frames(count, nmin, nmax, species, seed, axes=2, density=0.17, sigma=0.10):
+    """Frames periodic along `axes` (1 or 2) lattice rows only; the remaining
+    rows are zero (the reference's marking of a non-periodic direction), with
+    the atoms in a finite slab (axes=2) or wire (axes=1) along them."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        cell = random_cell(n, density, rng)
+        x = random_positions(n, cell, rng) + sigma * rng.standard_normal((n, 3))
+        cell = cell.copy()
+        cell[axes:] = 0.0
+        out.append(dict(types=rng.integers(0, species, n).astype(np.int32), positions=x, cell=cell, periodic=True))
+    return out
+
+
 def config_frames(config, scale=1.0):
     """Frames of a named config (`scale` < 1 shrinks frame counts for tests)."""
     if config == "static64":

@@ -154,3 +154,49 @@ def test_cfg_round_trip():
     buf = io.StringIO()
     write_cfg(buf, mol)
     assert "Supercell" not in buf.getvalue()
+
+
+def test_cfg_partial_supercell_round_trip(tmp_path):
+    """One or two Supercell rows (or a zero row) leave the other directions
+    non-periodic; the writer emits rows up to the last non-zero one."""
+    from superlinear_amd.cfg import read_cfg, write_cfg
+    from superlinear_amd import workloads as W
+    frames = W.partial_frames(1, 5, 7, 2, seed=3, axes=2) + W.partial_frames(1, 5, 7, 2, seed=4, axes=1)
+    p = tmp_path / "p.cfg"
+    with open(p, "w") as fh:
+        for f in frames:
+            write_cfg(fh, f, lossless=True)
+    text = p.read_text()
+    assert text.count("Supercell") == 2
+    back = list(read_cfg(p))
+    for f, g in zip(frames, back):
+        assert g["periodic"]
+        np.testing.assert_array_equal(g["cell"], f["cell"])
+        np.testing.assert_array_equal(g["positions"], f["positions"])
+
+
+def test_calculator_validates_before_evaluation():
+    """Species mapping and geometry checks of the Python entry point
+    (interfaces/python/bindings/core.cpp:26-74) raise before any GPU call."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.calculator import SuperlinearCalculator, SuperlinearModel
+    path = W.model_path("effect_l3k4r2o2")
+    with pytest.raises(ValueError):
+        SuperlinearModel(path, [6])  # the model has two species
+    with pytest.raises(ValueError):
+        SuperlinearModel(path, [6, 6])
+    with pytest.raises(ValueError):
+        SuperlinearModel(path, [])
+    m = SuperlinearModel(path, [6, 14])
+    assert m.metadata["species_count"] == 2 and m.metadata["kind"] == "coordination-effect"
+    cell = np.eye(3) * 10
+    with pytest.raises(ValueError):
+        m.calculate([6, 7], np.zeros((2, 3)), cell, [True] * 3)  # element outside the mapping
+    with pytest.raises(ValueError):
+        m.calculate([6], np.zeros((2, 3)), cell, [True] * 3)  # shape mismatch
+    with pytest.raises(ValueError):
+        m.calculate([6, 6], np.zeros((2, 3)), np.diag([10.0, 10.0, 0.2]), [True] * 3)  # thin periodic cell
+    with pytest.raises(ValueError):
+        SuperlinearCalculator(path, "CSi")
+    with pytest.raises(ValueError):
+        SuperlinearCalculator(path, ["C", "Xx"])

@@ -23,14 +23,15 @@ def test_oracle_matches_reference_fixtures(oracle_lib, case):
 
 
 def _brute_neighbors(x, cell, periodic, rc):
+    """Every image within rc; a zero lattice row is a non-periodic direction."""
     n = len(x)
     out = set()
-    rng = range(-4, 5) if periodic else range(0, 1)
+    rngs = [range(-4, 5) if periodic and np.any(cell[q] != 0) else range(0, 1) for q in range(3)]
     for i in range(n):
         for j in range(n):
-            for a in rng:
-                for b in rng:
-                    for c in rng:
+            for a in rngs[0]:
+                for b in rngs[1]:
+                    for c in rngs[2]:
                         if i == j and a == b == c == 0:
                             continue
                         d = x[j] - x[i] + (a * cell[0] + b * cell[1] + c * cell[2] if periodic else 0)
@@ -42,7 +43,9 @@ def _brute_neighbors(x, cell, periodic, rc):
 @pytest.mark.parametrize("seed", [5, 6, 7])
 def test_neighbors_match_brute_force(oracle_lib, seed):
     from superlinear_amd import workloads as W
-    for f in W.periodic_frames(3, 6, 14, 2, 0.15, seed=seed) + W.molecule_frames(1, 10, 14, 2, seed=seed):
+    frames = (W.periodic_frames(3, 6, 14, 2, 0.15, seed=seed) + W.molecule_frames(1, 10, 14, 2, seed=seed)
+              + W.partial_frames(1, 6, 12, 2, seed=seed, axes=2) + W.partial_frames(1, 5, 9, 2, seed=seed, axes=1))
+    for f in frames:
         ij, d = oracle_lib.neighbors(f["positions"], f["cell"], f["periodic"], 5.0)
         got = {(int(a), int(b), round(v[0], 9), round(v[1], 9), round(v[2], 9)) for (a, b), v in zip(ij, d)}
         want = _brute_neighbors(f["positions"], f["cell"], f["periodic"], 5.0)
