@@ -104,6 +104,35 @@ int sus2_loss_grad(sus2_model* m, int precision, int nframes, int natoms, const 
                    const double* e0, const double* f0, const double* s0, int weighting, double wE, double wF,
                    double wS, double* loss_out, double* grad, void* stream);
 
+/* --- pair_style sus2/embedded compute() contract ------------------------------
+ * Mirrors Kernel (interfaces/lammps/ML-SUS2/sus2_sl_kernel.h:12-43) as driven
+ * by PairSUS2Embedded::compute (pair_sus2_embedded.cpp:250-313):
+ *   sus2_pair_begin -> forward comm (pack/unpack, width sus2_pair_comm_width)
+ *   -> sus2_pair_outer -> reverse comm (pack_reverse/add_reverse)
+ *   -> sus2_pair_finish.
+ * Atoms [0, nowned) are owned centres, [nowned, nall) ghosts; edges are the
+ * owned atoms' full neighbour lists, i owned -> j any, d = x_j - x_i (edges at
+ * or beyond the cutoff are dropped).  All buffers are host memory (the LAMMPS
+ * atom and comm arrays); computation runs on the GPU.  The comm calls take
+ * the atom index lists of pack_forward_comm / unpack_forward_comm /
+ * pack_reverse_comm / unpack_reverse_comm with `width` doubles per atom
+ * (the gate signal h for Effect models, 0 for Static).  finish writes the
+ * energy and virial (xx yy zz xy xz yz) of the owned atoms, site energies of
+ * the owned atoms (site may be NULL) and forces on all nall atoms. */
+typedef struct sus2_pair sus2_pair;
+int sus2_pair_create(sus2_model* m, int precision, sus2_pair** out);
+void sus2_pair_free(sus2_pair* p);
+int sus2_pair_comm_width(const sus2_pair* p);
+int sus2_pair_begin(sus2_pair* p, int nowned, int nall, const int32_t* types /* [nall] */, int64_t nedges,
+                    const int32_t* ei, const int32_t* ej, const double* d /* [nedges*3] */);
+int sus2_pair_pack_forward(sus2_pair* p, int n, const int32_t* atoms, double* buf /* [n*width] */);
+int sus2_pair_unpack_forward(sus2_pair* p, int n, const int32_t* atoms, const double* buf);
+int sus2_pair_outer(sus2_pair* p);
+int sus2_pair_pack_reverse(sus2_pair* p, int n, const int32_t* atoms, double* buf);
+int sus2_pair_add_reverse(sus2_pair* p, int n, const int32_t* atoms, const double* buf);
+int sus2_pair_finish(sus2_pair* p, double* energy, double* site /* [nowned] */, double* force /* [nall*3] */,
+                     double* virial /* [6] */);
+
 /* --- per-kernel timing ------------------------------------------------------
  * Replaces the reference's env-gated wall-clock stage timers
  * (SUS2_SH_ACCUM_PROFILE, dev_src/mtpr_sh.cpp:1855-1957; pair_style

@@ -42,6 +42,16 @@ SIGNATURES = {
     "sus2_efs_grad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sus2_loss_grad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _dp, _vp,
                             _vp]),
+    "sus2_pair_create": (_i, [_vp, _i, ctypes.POINTER(_vp)]),
+    "sus2_pair_free": (None, [_vp]),
+    "sus2_pair_comm_width": (_i, [_vp]),
+    "sus2_pair_begin": (_i, [_vp, _i, _i, _vp, ctypes.c_int64, _vp, _vp, _vp]),
+    "sus2_pair_pack_forward": (_i, [_vp, _i, _vp, _vp]),
+    "sus2_pair_unpack_forward": (_i, [_vp, _i, _vp, _vp]),
+    "sus2_pair_outer": (_i, [_vp]),
+    "sus2_pair_pack_reverse": (_i, [_vp, _i, _vp, _vp]),
+    "sus2_pair_add_reverse": (_i, [_vp, _i, _vp, _vp]),
+    "sus2_pair_finish": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "sus2_profile_enable": (_i, [_vp, _i]),
     "sus2_profile_reset": (_i, [_vp]),
     "sus2_profile_read": (_i, [_vp, _i, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_int64), _ip]),

@@ -1,6 +1,7 @@
 // extern "C" boundary of the SUS2-SH MI355X path (declared in include/sus2amd.h).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstring>
 #include <exception>
 #include <functional>
@@ -10,6 +11,7 @@
 
 #include "../../include/sus2amd.h"
 #include "sus2_model.hpp"
+#include "sus2_common.hpp"
 #include "sus2_runtime.hpp"
 
 struct sus2_model {
@@ -261,6 +263,220 @@ int sus2_loss_grad(sus2_model* m, int precision, int nframes, int natoms, const 
     sus2::loss_grad_dispatch(m->dev, m->ws, precision,
                              make_batch(nframes, natoms, frame_off, types, pos, cell, periodic), e0, f0, s0,
                              weighting, wE, wF, wS, loss_out, grad, (hipStream_t)stream);
+  });
+}
+
+// ------------------------------------------------------------------ pair style
+}  // extern "C"
+
+struct sus2_pair {
+  sus2_model* model = nullptr;
+  int precision = SUS2_FP64;
+  sus2::Workspace ws;
+  sus2::DevBuf types, ei, ej, d, in_off, frame_off, outE, outF, outV, outS, rows, idx;
+  int nowned = 0, nall = 0;
+  int state = 0;  // 0 idle, 1 begun, 2 outer done
+  int width = 0;
+};
+
+namespace {
+// Rows of `src` (T, width F) for the listed atoms -> FP64 host buffer, or the
+// reverse (set or add).  mode 0 pack, 1 set, 2 add.
+template <class T>
+__global__ void k_rows(int n, int F, const int* __restrict__ atoms, T* __restrict__ dev, double* __restrict__ buf,
+                       int mode) {
+  const int k = blockIdx.x * blockDim.y + threadIdx.y, mu = threadIdx.x;
+  if (k >= n || mu >= F) return;
+  T* row = dev + (size_t)atoms[k] * F;
+  if (mode == 0) buf[(size_t)k * F + mu] = (double)row[mu];
+  else if (mode == 1) row[mu] = (T)buf[(size_t)k * F + mu];
+  else row[mu] = (T)((double)row[mu] + buf[(size_t)k * F + mu]);
+}
+
+void pair_rows(sus2_pair* p, bool adj, int n, const int32_t* atoms, double* hbuf, int mode) {
+  if (n < 0) throw sus2::Error(SUS2_E_INVALID, "negative atom count");
+  if (p->width == 0 || n == 0) return;
+  if (!atoms || !hbuf) throw sus2::Error(SUS2_E_INVALID, "null atom list or buffer");
+  const int F = p->width;
+  std::vector<int32_t> list(atoms, atoms + n);
+  std::vector<double> vals;
+  if (mode == 2) {  // aggregate repeated atoms on the host in list order, then one parallel add
+    std::vector<int32_t> uniq;
+    std::vector<int> slot(p->nall, -1);
+    for (int k = 0; k < n; ++k) {
+      const int a = list[k];
+      if (a < 0 || a >= p->nall) throw sus2::Error(SUS2_E_RANGE, "atom index out of range");
+      if (slot[a] < 0) {
+        slot[a] = (int)uniq.size();
+        uniq.push_back(a);
+        vals.insert(vals.end(), hbuf + (size_t)k * F, hbuf + (size_t)(k + 1) * F);
+      } else {
+        for (int mu = 0; mu < F; ++mu) vals[(size_t)slot[a] * F + mu] += hbuf[(size_t)k * F + mu];
+      }
+    }
+    list.swap(uniq);
+    n = (int)list.size();
+  } else {
+    for (int k = 0; k < n; ++k)
+      if (list[k] < 0 || list[k] >= p->nall) throw sus2::Error(SUS2_E_RANGE, "atom index out of range");
+    if (mode == 1) vals.assign(hbuf, hbuf + (size_t)n * F);
+  }
+  p->idx.ensure(sizeof(int) * n);
+  p->rows.ensure(sizeof(double) * (size_t)n * F);
+  h2d(p->idx.get<int>(), list.data(), sizeof(int) * n);
+  if (mode != 0) h2d(p->rows.get<double>(), vals.data(), sizeof(double) * (size_t)n * F);
+  const int per = std::max(1, 256 / F);
+  const dim3 block(F, per), grid((n + per - 1) / per);
+  if (p->precision == SUS2_FP32) {
+    float* dev = adj ? p->ws.hbar.get<float>() : p->ws.h.get<float>();
+    hipLaunchKernelGGL(k_rows<float>, grid, block, 0, nullptr, n, F, p->idx.get<int>(), dev, p->rows.get<double>(),
+                       mode);
+  } else {
+    double* dev = adj ? p->ws.hbar.get<double>() : p->ws.h.get<double>();
+    hipLaunchKernelGGL(k_rows<double>, grid, block, 0, nullptr, n, F, p->idx.get<int>(), dev, p->rows.get<double>(),
+                       mode);
+  }
+  if (hipGetLastError() != hipSuccess) throw sus2::Error(SUS2_E_DEVICE, "row copy kernel failed");
+  if (mode == 0) d2h(hbuf, p->rows.get<double>(), sizeof(double) * (size_t)n * F);
+  else if (hipStreamSynchronize(nullptr) != hipSuccess) throw sus2::Error(SUS2_E_DEVICE, "stream synchronize failed");
+}
+
+sus2::PairView pair_view(sus2_pair* p) {
+  sus2::PairView v;
+  v.nowned = p->nowned;
+  v.nall = p->nall;
+  v.types = p->types.get<int32_t>();
+  v.in_off = p->in_off.get<int>();
+  v.frame_off = p->frame_off.get<int64_t>();
+  v.energy = p->outE.get<double>();
+  v.force = p->outF.get<double>();
+  v.virial = p->outV.get<double>();
+  v.site = p->outS.get<double>();
+  return v;
+}
+}  // namespace
+
+extern "C" {
+
+int sus2_pair_create(sus2_model* m, int precision, sus2_pair** out) {
+  return guard([&] {
+    if (!m || !out) throw sus2::Error(SUS2_E_INVALID, "null argument");
+    if (precision != SUS2_FP64 && precision != SUS2_FP32) throw sus2::Error(SUS2_E_INVALID, "bad precision");
+    ensure_device(m);
+    std::unique_ptr<sus2_pair> p(new sus2_pair());
+    p->model = m;
+    p->precision = precision;
+    p->width = m->model.gated ? m->model.F : 0;
+    *out = p.release();
+  });
+}
+
+void sus2_pair_free(sus2_pair* p) { delete p; }
+
+int sus2_pair_comm_width(const sus2_pair* p) { return p ? p->width : 0; }
+
+int sus2_pair_begin(sus2_pair* p, int nowned, int nall, const int32_t* types, int64_t nedges, const int32_t* ei,
+                    const int32_t* ej, const double* d) {
+  return guard([&] {
+    if (!p) throw sus2::Error(SUS2_E_INVALID, "null pair handle");
+    const sus2::Model& M = p->model->model;
+    if (nowned < 0 || nall < nowned || nedges < 0) throw sus2::Error(SUS2_E_INVALID, "invalid atom layout");
+    if ((nall > 0 && !types) || (nedges > 0 && (!ei || !ej || !d))) throw sus2::Error(SUS2_E_INVALID, "null buffer");
+    if (nedges > INT32_MAX - 1) throw sus2::Error(SUS2_E_RANGE, "too many edges");
+    for (int k = 0; k < nall; ++k)
+      if (types[k] < 0 || types[k] >= M.C) throw sus2::Error(SUS2_E_RANGE, "atom species index out of range");
+    // validate every edge; edges at or beyond the cutoff contribute nothing and are dropped
+    // (Kernel::begin_impl and the r >= max_dist skips, sus2_sl_kernel.cpp:435, 173-176)
+    std::vector<int32_t> vi, vj;
+    std::vector<double> vd;
+    vi.reserve(nedges); vj.reserve(nedges); vd.reserve(3 * nedges);
+    const double rc2 = M.rc * M.rc;
+    for (int64_t k = 0; k < nedges; ++k) {
+      const double r2 = d[3 * k] * d[3 * k] + d[3 * k + 1] * d[3 * k + 1] + d[3 * k + 2] * d[3 * k + 2];
+      if (ei[k] < 0 || ei[k] >= nowned || ej[k] < 0 || ej[k] >= nall || !(r2 > 0) || !std::isfinite(r2))
+        throw sus2::Error(SUS2_E_INVALID, "distributed EFS neighborhood has an invalid edge");
+      if (r2 >= rc2) continue;
+      vi.push_back(ei[k]); vj.push_back(ej[k]);
+      vd.insert(vd.end(), d + 3 * k, d + 3 * k + 3);
+    }
+    const long ne = (long)vi.size();
+    ensure_device(p->model);
+    p->nowned = nowned;
+    p->nall = nall;
+    const size_t a1 = std::max(1, nall);
+    p->types.ensure(sizeof(int32_t) * a1);
+    p->ei.ensure(sizeof(int32_t) * std::max<long>(1, ne));
+    p->ej.ensure(sizeof(int32_t) * std::max<long>(1, ne));
+    p->d.ensure(sizeof(double) * 3 * std::max<long>(1, ne));
+    p->in_off.ensure(sizeof(int) * (a1 + 1));
+    p->frame_off.ensure(sizeof(int64_t) * 2);
+    p->outE.ensure(sizeof(double));
+    p->outF.ensure(sizeof(double) * 3 * a1);
+    p->outV.ensure(sizeof(double) * 6);
+    p->outS.ensure(sizeof(double) * a1);
+    h2d(p->types.get<int32_t>(), types, sizeof(int32_t) * nall);
+    h2d(p->ei.get<int32_t>(), vi.data(), sizeof(int32_t) * ne);
+    h2d(p->ej.get<int32_t>(), vj.data(), sizeof(int32_t) * ne);
+    h2d(p->d.get<double>(), vd.data(), sizeof(double) * 3 * ne);
+    const int64_t fo[2] = {0, nowned};
+    h2d(p->frame_off.get<int64_t>(), fo, sizeof(fo));
+    if (p->precision == SUS2_FP32)
+      sus2::build_pair_lists<float>(p->ws, nowned, nall, ne, p->ei.get<int32_t>(), p->ej.get<int32_t>(),
+                                    p->d.get<double>(), p->in_off.get<int>(), nullptr);
+    else
+      sus2::build_pair_lists<double>(p->ws, nowned, nall, ne, p->ei.get<int32_t>(), p->ej.get<int32_t>(),
+                                     p->d.get<double>(), p->in_off.get<int>(), nullptr);
+    sus2::pair_stage_dispatch(0, p->precision, p->model->dev, p->ws, pair_view(p), nullptr);
+    p->state = 1;
+  });
+}
+
+int sus2_pair_pack_forward(sus2_pair* p, int n, const int32_t* atoms, double* buf) {
+  return guard([&] {
+    if (!p || p->state < 1) throw sus2::Error(SUS2_E_INVALID, "pack_forward before begin");
+    pair_rows(p, false, n, atoms, buf, 0);
+  });
+}
+
+int sus2_pair_unpack_forward(sus2_pair* p, int n, const int32_t* atoms, const double* buf) {
+  return guard([&] {
+    if (!p || p->state != 1) throw sus2::Error(SUS2_E_INVALID, "unpack_forward outside begin .. outer");
+    pair_rows(p, false, n, atoms, const_cast<double*>(buf), 1);
+  });
+}
+
+int sus2_pair_outer(sus2_pair* p) {
+  return guard([&] {
+    if (!p || p->state != 1) throw sus2::Error(SUS2_E_INVALID, "outer before begin");
+    sus2::pair_stage_dispatch(1, p->precision, p->model->dev, p->ws, pair_view(p), nullptr);
+    p->state = 2;
+  });
+}
+
+int sus2_pair_pack_reverse(sus2_pair* p, int n, const int32_t* atoms, double* buf) {
+  return guard([&] {
+    if (!p || p->state != 2) throw sus2::Error(SUS2_E_INVALID, "pack_reverse before outer");
+    pair_rows(p, true, n, atoms, buf, 0);
+  });
+}
+
+int sus2_pair_add_reverse(sus2_pair* p, int n, const int32_t* atoms, const double* buf) {
+  return guard([&] {
+    if (!p || p->state != 2) throw sus2::Error(SUS2_E_INVALID, "add_reverse before outer");
+    pair_rows(p, true, n, atoms, const_cast<double*>(buf), 2);
+  });
+}
+
+int sus2_pair_finish(sus2_pair* p, double* energy, double* site, double* force, double* virial) {
+  return guard([&] {
+    if (!p || p->state != 2) throw sus2::Error(SUS2_E_INVALID, "finish before outer");
+    if (!energy || !force || !virial) throw sus2::Error(SUS2_E_INVALID, "null output buffer");
+    sus2::pair_stage_dispatch(2, p->precision, p->model->dev, p->ws, pair_view(p), nullptr);
+    d2h(energy, p->outE.get<double>(), sizeof(double));
+    d2h(virial, p->outV.get<double>(), sizeof(double) * 6);
+    d2h(force, p->outF.get<double>(), sizeof(double) * 3 * p->nall);
+    if (site) d2h(site, p->outS.get<double>(), sizeof(double) * p->nowned);
+    p->state = 0;
   });
 }
 

@@ -69,8 +69,31 @@ struct SiteConst {
   int o_species, o_shift;
   __device__ __forceinline__ double operator()(int t) const { return theta[o_species + t] + theta[o_shift + t]; }
 };
+// Forces of atoms [0, b.natoms), site energies of atoms [0, nsite) (all when
+// nsite < 0), per-frame energy and virial sums over the frames' atoms.
 template <class T>
-void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, const SiteConst& sc, hipStream_t st);
+void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, const SiteConst& sc, hipStream_t st,
+                int nsite = -1);
+// Zero n values of type T on the stream (fill kernel; no memset node).
+template <class T>
+void fill_zero(Workspace& ws, T* p, size_t n, hipStream_t st);
+
+// Device view of one pair-style evaluation (owned atoms [0, nowned), ghosts
+// [nowned, nall)).
+struct PairView {
+  int nowned = 0, nall = 0;
+  const int32_t* types = nullptr;   // [nall]
+  const int* in_off = nullptr;      // [nall + 1]
+  const int64_t* frame_off = nullptr;  // {0, nowned}
+  double *energy = nullptr, *force = nullptr, *virial = nullptr, *site = nullptr;  // device outputs
+};
+template <class Cf>
+void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const PairView& pv, hipStream_t st);
+template <class T>
+void build_pair_lists(Workspace& ws, int nowned, int nall, long nedges, const int32_t* ei, const int32_t* ej,
+                      const double* d, int* in_off, hipStream_t st);
+void pair_stage_dispatch(int stage, int precision, const DevModelSet& dm, Workspace& ws, const PairView& pv,
+                         hipStream_t st);
 template <class Cf>
 void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st);
 

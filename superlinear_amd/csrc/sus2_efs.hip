@@ -360,15 +360,31 @@ __global__ __launch_bounds__(256) void k_finalize(int nframes, const int64_t* __
 }
 
 template <class T>
-__global__ void k_atom_out(int natoms, const int32_t* __restrict__ types, SiteConst sc, const T* __restrict__ site,
-                           const T* __restrict__ fw, double* __restrict__ force, double* __restrict__ site_out) {
+__global__ void k_atom_out(int natoms, int nsite, const int32_t* __restrict__ types, SiteConst sc,
+                           const T* __restrict__ site, const T* __restrict__ fw, double* __restrict__ force,
+                           double* __restrict__ site_out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= natoms) return;
   force[3 * i] = (double)fw[9 * (size_t)i];
   force[3 * i + 1] = (double)fw[9 * (size_t)i + 1];
   force[3 * i + 2] = (double)fw[9 * (size_t)i + 2];
-  if (site_out) site_out[i] = (double)site[i] + sc(types[i]);
+  if (site_out && i < nsite) site_out[i] = (double)site[i] + sc(types[i]);
 }
+
+template <class T>
+__global__ void k_fill_zero(size_t n, T* __restrict__ a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = T(0);
+}
+
+template <class T>
+void fill_zero(Workspace& ws, T* p, size_t n, hipStream_t st) {
+  if (n == 0) return;
+  SUS2_LAUNCH(ws, st, "fill_zero", k_fill_zero<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, n, p);
+}
+template void fill_zero<float>(Workspace&, float*, size_t, hipStream_t);
+template void fill_zero<double>(Workspace&, double*, size_t, hipStream_t);
+template void fill_zero<int>(Workspace&, int*, size_t, hipStream_t);
 
 // ------------------------------------------------------------------ driver
 template <class T>
@@ -424,19 +440,122 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
 }
 
 template <class T>
-void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, const SiteConst& sc, hipStream_t st) {
+void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, const SiteConst& sc, hipStream_t st, int nsite) {
   const int N = b.natoms;
   if (N > 0)
-    SUS2_LAUNCH(ws, st, "atom_out", k_atom_out<T>, dim3((N + 255) / 256), dim3(256), 0, N, b.types, sc,
-                ws.site.get<T>(), ws.fw.get<T>(), out.force, out.site);
+    SUS2_LAUNCH(ws, st, "atom_out", k_atom_out<T>, dim3((N + 255) / 256), dim3(256), 0, N, nsite < 0 ? N : nsite,
+                b.types, sc, ws.site.get<T>(), ws.fw.get<T>(), out.force, out.site);
   SUS2_LAUNCH(ws, st, "finalize", k_finalize<T>, dim3(b.nframes), dim3(256), 0, b.nframes, b.frame_off, b.types, sc,
               ws.site.get<T>(), ws.fw.get<T>(), out.energy, out.virial);
 }
 
 template void build_neighbors<float>(Workspace&, const Batch&, double, hipStream_t);
 template void build_neighbors<double>(Workspace&, const Batch&, double, hipStream_t);
-template void finish_efs<float>(Workspace&, const Batch&, const EfsOut&, const SiteConst&, hipStream_t);
-template void finish_efs<double>(Workspace&, const Batch&, const EfsOut&, const SiteConst&, hipStream_t);
+template void finish_efs<float>(Workspace&, const Batch&, const EfsOut&, const SiteConst&, hipStream_t, int);
+template void finish_efs<double>(Workspace&, const Batch&, const EfsOut&, const SiteConst&, hipStream_t, int);
+
+// ------------------------------------------------------------------ pair-style lists
+// CSR of caller-supplied directed edges (i owned -> j any, d = x_j - x_i):
+// edges grouped by centre in their input order, then the incoming lists of all
+// atoms (CSR positions grouped by target, ascending).
+__global__ void k_count_keys(long n, const int32_t* __restrict__ keys, int* __restrict__ cnt) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) atomicAdd(cnt + keys[e], 1);
+}
+
+__global__ void k_place_keys(long n, const int32_t* __restrict__ keys, const int* __restrict__ off,
+                             int* __restrict__ cur, int* __restrict__ out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) out[off[keys[e]] + atomicAdd(cur + keys[e], 1)] = (int)e;
+}
+
+template <class T>
+__global__ void k_build_rec(long n, const int* __restrict__ order, const int32_t* __restrict__ ej,
+                            const double* __restrict__ d, Rec<T>* __restrict__ rec, int32_t* __restrict__ tgt) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int e = order[p];
+  const int j = ej[e];
+  rec[p] = Rec<T>{T(d[3 * (size_t)e]), T(d[3 * (size_t)e + 1]), T(d[3 * (size_t)e + 2]), rec_jbits(j, T(0))};
+  tgt[p] = j;
+}
+
+template <class T>
+void build_pair_lists(Workspace& ws, int nowned, int nall, long nedges, const int32_t* ei, const int32_t* ej,
+                      const double* d, int* in_off, hipStream_t st) {
+  const size_t n1 = std::max(1, nall), e1 = std::max<long>(1, nedges);
+  ws.ensure_atoms(nowned);
+  ws.cell_cnt.ensure(sizeof(int) * n1);
+  ws.cell_cur.ensure(sizeof(int) * n1);
+  ws.cell_atoms.ensure(sizeof(int) * e1);  // edge order by centre
+  ws.wrap.ensure(sizeof(int32_t) * e1);    // target of each CSR position
+  ws.scan_tmp.ensure(2 * sizeof(int64_t));
+  ws.rec.ensure(sizeof(Rec<T>) * e1);
+  ws.in_edges.ensure(sizeof(int) * e1);
+  ws.nedges = nedges;
+  const unsigned ge = (unsigned)((nedges + 255) / 256);
+  int* cnt = ws.cell_cnt.get<int>();
+  int* cur = ws.cell_cur.get<int>();
+  int* order = ws.cell_atoms.get<int>();
+  int32_t* tgt = ws.wrap.get<int32_t>();
+  // out lists (owned centres)
+  fill_zero(ws, cnt, (size_t)std::max(1, nowned), st);
+  fill_zero(ws, cur, (size_t)std::max(1, nowned), st);
+  if (nedges > 0) SUS2_LAUNCH(ws, st, "pair_count", k_count_keys, dim3(ge), dim3(256), 0, nedges, ei, cnt);
+  device_scan(cnt, ws.nbr_off.get<int>(), nowned, ws.scan_tmp.get<int64_t>(), st);
+  if (nedges > 0) {
+    SUS2_LAUNCH(ws, st, "pair_place", k_place_keys, dim3(ge), dim3(256), 0, nedges, ei, ws.nbr_off.get<int>(), cur,
+                order);
+    if (nowned > 0)
+      SUS2_LAUNCH(ws, st, "pair_sort", k_sort_lists, dim3(nowned), dim3(WAVE), 0, nowned, ws.nbr_off.get<int>(),
+                  order);
+    SUS2_LAUNCH(ws, st, "pair_rec", k_build_rec<T>, dim3(ge), dim3(256), 0, nedges, order, ej, d,
+                ws.rec.get<Rec<T>>(), tgt);
+  }
+  // in lists (all atoms) over CSR positions
+  fill_zero(ws, cnt, n1, st);
+  fill_zero(ws, cur, n1, st);
+  if (nedges > 0) SUS2_LAUNCH(ws, st, "pair_count", k_count_keys, dim3(ge), dim3(256), 0, nedges, tgt, cnt);
+  device_scan(cnt, in_off, nall, ws.scan_tmp.get<int64_t>() + 1, st);
+  if (nedges > 0) {
+    SUS2_LAUNCH(ws, st, "pair_place", k_place_keys, dim3(ge), dim3(256), 0, nedges, tgt, in_off, cur,
+                ws.in_edges.get<int>());
+    if (nall > 0)
+      SUS2_LAUNCH(ws, st, "pair_sort", k_sort_lists, dim3(nall), dim3(WAVE), 0, nall, in_off,
+                  ws.in_edges.get<int>());
+  }
+}
+template void build_pair_lists<float>(Workspace&, int, int, long, const int32_t*, const int32_t*, const double*,
+                                      int*, hipStream_t);
+template void build_pair_lists<double>(Workspace&, int, int, long, const int32_t*, const int32_t*, const double*,
+                                       int*, hipStream_t);
+
+#define SUS2_DECLARE_PAIR(LL, KK, BB, SM)                                                                        \
+  extern template void pair_stage_shape<Cfg<float, LL, KK, BB, SM>>(int, const DevModelSet&, Workspace&,        \
+                                                                    const PairView&, hipStream_t);              \
+  extern template void pair_stage_shape<Cfg<double, LL, KK, BB, SM>>(int, const DevModelSet&, Workspace&,       \
+                                                                     const PairView&, hipStream_t);
+SUS2_FOR_EACH_SHAPE(SUS2_DECLARE_PAIR)
+#undef SUS2_DECLARE_PAIR
+
+template <class T>
+void pair_stage_t(int stage, const DevModelSet& dm, Workspace& ws, const PairView& pv, hipStream_t st) {
+  const int L = dm.host->L, K = dm.host->K, B = dm.host->B, SM = dm.host->smode;
+#define SUS2_SHAPE(LL, KK, BB, SMM) \
+  if (L == LL && K == KK && B == BB && SM == SMM) return pair_stage_shape<Cfg<T, LL, KK, BB, SMM>>(stage, dm, ws, pv, st);
+  SUS2_FOR_EACH_SHAPE(SUS2_SHAPE)
+#undef SUS2_SHAPE
+  throw Error(SUS2_E_UNSUPPORTED, "no kernel instance for L=" + std::to_string(L) + " K=" + std::to_string(K) +
+                                      " B=" + std::to_string(B) + " scaling_map mode " + std::to_string(SM));
+}
+
+void pair_stage_dispatch(int stage, int precision, const DevModelSet& dm, Workspace& ws, const PairView& pv,
+                         hipStream_t st) {
+  if (precision == SUS2_FP64) pair_stage_t<double>(stage, dm, ws, pv, st);
+  else if (precision == SUS2_FP32) pair_stage_t<float>(stage, dm, ws, pv, st);
+  else throw Error(SUS2_E_INVALID, "precision must be SUS2_FP64 or SUS2_FP32");
+  ws.prof.collect_ready();
+}
 
 #define SUS2_DECLARE(LL, KK, BB, SM)                                                                  \
   extern template void run_efs_shape<Cfg<float, LL, KK, BB, SM>>(const DevModelSet&, Workspace&, const Batch&, \

@@ -311,40 +311,61 @@ __global__ __launch_bounds__(WAVE) void k_edge_backward(DModel<T> m, const int32
 }
 
 // ------------------------------------------------------------------ gather
-template <class Cf>
-size_t gather_lds(const DModel<typename Cf::T>& m) {
-  return sizeof(typename Cf::T) * ((size_t)2 * m.M + Cf::KH + Cf::F + std::max(graph_scratch(m.gp), 1));
+// Gate graph reverse of one atom from its gate-signal adjoint h_bar (LDS hbs,
+// alpha already applied): seeds S_bar_q = sum_mu h_bar_mu W_mu,q, reverse over
+// the stored gate node values, transposed contraction -> gate basic adjoints.
+template <class Cf, class T = typename Cf::T>
+__device__ void gate_reverse_atom(const DModel<T>& m, int i, const T* __restrict__ vg, T* __restrict__ gbar,
+                                  const T* hbs, T* v, T* a, T* scratch) {
+  constexpr int KK = Cf::KK, F = Cf::F;
+  const int lane = threadIdx.x;
+  const DPass<T>& p = m.gp;
+  const T* vi = vg + (size_t)i * p.node_cnt;
+  for (int n = lane; n < m.M; n += WAVE) {
+    v[n] = (n >= p.node_lo && n < p.node_lo + p.node_cnt) ? vi[n - p.node_lo] : T(0);
+    a[n] = T(0);
+  }
+  __syncthreads();
+  for (int q = lane; q < m.Qg; q += WAVE) {
+    T s = T(0);
+#pragma unroll
+    for (int mu = 0; mu < F; ++mu) s += hbs[mu] * m.W(mu, q);
+    a[m.gate_nodes[q]] = s;
+  }
+  __syncthreads();
+  wave_graph_reverse(p, v, a, scratch);
+  wave_adjoint_to_basic<T, KK>(m, p, true, a, gbar + (size_t)i * Cf::KH);
 }
 
-template <class Cf>
-size_t backward_lds(const DModel<typename Cf::T>& m) {
-  return sizeof(typename Cf::T) * ((size_t)Cf::KH + species_tables_size<Cf>(m.C));
-}
-
-// F_i = sum_{e in out(i)} g_e - sum_{e' in in(i)} g_e' ; the outer pass of an
-// Effect model also forms h_bar_i = alpha_{t_i} sum_{e' in in(i)} u_e' and runs
-// the gate graph reverse from the stored gate node values.
+// F_a = sum_{e in out(a)} g_e - sum_{e' in in(a)} g_e' over the centre's own
+// edges and its sorted incoming-edge list (no atomics).  The outer pass of an
+// Effect model also forms h_bar_a = alpha_{t_a} sum_{e' in in(a)} u_e' and
+// either runs the gate graph reverse at once (hbar_out == null) or stores
+// h_bar for a later reverse (the pair-style path, whose ghost adjoints are
+// communicated in between).  Atoms a >= nowned (ghosts) have no out edges.
 template <class Cf, bool GATE, class T = typename Cf::T>
-__global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __restrict__ types,
-                                                 const int* __restrict__ nbr_off, const int* __restrict__ in_edges,
-                                                 const T* __restrict__ gedge, const T* __restrict__ uedge,
-                                                 const T* __restrict__ vg, T* __restrict__ gbar,
+__global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __restrict__ types, int nowned,
+                                                 const int* __restrict__ out_off, const int* __restrict__ in_off,
+                                                 const int* __restrict__ in_edges, const T* __restrict__ gedge,
+                                                 const T* __restrict__ uedge, const T* __restrict__ vg,
+                                                 T* __restrict__ gbar, T* __restrict__ hbar_out,
                                                  T* __restrict__ fw) {
-  constexpr int KK = Cf::KK, F = Cf::F, KH = Cf::KH;
+  constexpr int F = Cf::F;
   const int i = blockIdx.x, lane = threadIdx.x;
-  const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
   const bool hgate = !GATE && m.gated;
   T fx = T(0), fy = T(0), fz = T(0);
   T hb[F];
 #pragma unroll
   for (int mu = 0; mu < F; ++mu) hb[mu] = T(0);
-  for (int k = e0 + lane; k < e1; k += WAVE) {
-    const T* go = gedge + 3 * (size_t)k;
+  if (i < nowned)
+    for (int k = out_off[i] + lane; k < out_off[i + 1]; k += WAVE) {
+      const T* go = gedge + 3 * (size_t)k;
+      fx += go[0]; fy += go[1]; fz += go[2];
+    }
+  for (int k = in_off[i] + lane; k < in_off[i + 1]; k += WAVE) {
     const int ein = in_edges[k];
     const T* gi = gedge + 3 * (size_t)ein;
-    fx += go[0] - gi[0];
-    fy += go[1] - gi[1];
-    fz += go[2] - gi[2];
+    fx -= gi[0]; fy -= gi[1]; fz -= gi[2];
     if (hgate) {
       const T* ue = uedge + (size_t)ein * F;
 #pragma unroll
@@ -361,32 +382,45 @@ __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __r
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* v = reinterpret_cast<T*>(smem_raw);  // [M]
   T* a = v + m.M;                          // [M]
-  T* mom = a + m.M;                        // [KH]
-  T* hbs = mom + KH;                       // [F]
+  T* hbs = a + m.M;                        // [F]
   T* scratch = hbs + F;
   const int ti = types[i];
 #pragma unroll
   for (int mu = 0; mu < F; ++mu) {
-    const T s = wave_sum(hb[mu]);
-    if (lane == 0) hbs[mu] = s * m.alpha(ti, mu);
+    const T s = wave_sum(hb[mu]) * m.alpha(ti, mu);
+    if (hbar_out) {
+      if (lane == 0) hbar_out[(size_t)i * F + mu] = s;
+    } else if (lane == 0) {
+      hbs[mu] = s;
+    }
   }
-  const DPass<T>& p = m.gp;
-  const T* vi = vg + (size_t)i * p.node_cnt;
-  for (int n = lane; n < m.M; n += WAVE) {
-    v[n] = (n >= p.node_lo && n < p.node_lo + p.node_cnt) ? vi[n - p.node_lo] : T(0);
-    a[n] = T(0);
-  }
-  __syncthreads();
-  // seeds S_bar_q = sum_mu h_bar_mu W_mu,q
-  for (int q = lane; q < m.Qg; q += WAVE) {
-    T s = T(0);
-#pragma unroll
-    for (int mu = 0; mu < F; ++mu) s += hbs[mu] * m.W(mu, q);
-    a[m.gate_nodes[q]] = s;
-  }
-  __syncthreads();
-  wave_graph_reverse(p, v, a, scratch);
-  wave_adjoint_to_basic<T, KK>(m, p, true, a, gbar + (size_t)i * KH);
+  if (hbar_out) return;
+  gate_reverse_atom<Cf>(m, i, vg, gbar, hbs, v, a, scratch);
+}
+
+// Gate reverse of owned atoms from stored h_bar (pair-style path).
+template <class Cf, class T = typename Cf::T>
+__global__ __launch_bounds__(WAVE) void k_gate_reverse(DModel<T> m, const T* __restrict__ hbar,
+                                                       const T* __restrict__ vg, T* __restrict__ gbar) {
+  constexpr int F = Cf::F;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* v = reinterpret_cast<T*>(smem_raw);
+  T* a = v + m.M;
+  T* hbs = a + m.M;
+  T* scratch = hbs + F;
+  const int i = blockIdx.x;
+  for (int mu = threadIdx.x; mu < F; mu += WAVE) hbs[mu] = hbar[(size_t)i * F + mu];
+  gate_reverse_atom<Cf>(m, i, vg, gbar, hbs, v, a, scratch);
+}
+
+template <class Cf>
+size_t gather_lds(const DModel<typename Cf::T>& m) {
+  return sizeof(typename Cf::T) * ((size_t)2 * m.M + Cf::F + std::max(graph_scratch(m.gp), 1));
+}
+
+template <class Cf>
+size_t backward_lds(const DModel<typename Cf::T>& m) {
+  return sizeof(typename Cf::T) * ((size_t)Cf::KH + species_tables_size<Cf>(m.C));
 }
 
 // ------------------------------------------------------------------ driver
@@ -440,17 +474,96 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
                 b.types, off, rec, h, ws.abar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
     allow_lds(k_gather<Cf, false>, lg);
     SUS2_LAUNCH(ws, st, kname<Cf>("gather_outer"), (k_gather<Cf, false>), dim3(N), dim3(WAVE), m.gated ? lg : 0, m,
-                b.types, off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
-                ws.gbar.get<T>(), ws.fw.get<T>());
+                b.types, N, off, off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
+                ws.gbar.get<T>(), (T*)nullptr, ws.fw.get<T>());
     if (m.gated) {
       SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_gate"), (k_edge_backward<Cf, true>), dim3(N), dim3(WAVE), lb, m,
                   b.types, off, rec, h, ws.gbar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
-      SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(N), dim3(WAVE), 0, m, b.types, off,
-                  ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>(),
-                  ws.fw.get<T>());
+      SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(N), dim3(WAVE), 0, m, b.types, N, off,
+                  off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>(),
+                  (T*)nullptr, ws.fw.get<T>());
     }
   }
   finish_efs<T>(ws, b, out, SiteConst{dm.d.view.theta, m.o_species, m.o_shift}, st);
+}
+
+// ------------------------------------------------------------------ pair-style stages
+// Owned atoms [0, nowned) are centres; ghosts [nowned, nall) appear only as
+// neighbours.  ws.nbr_off / ws.rec hold the owned atoms' edges (CSR), pv.in_off
+// / ws.in_edges the incoming lists of all atoms.  The stages follow
+// Kernel::begin / outer / finish (interfaces/lammps/ML-SUS2/sus2_sl_kernel.cpp:
+// 385-464, 487-546, 549-562).
+template <class Cf>
+void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const PairView& pv, hipStream_t st) {
+  using T = typename Cf::T;
+  const DModel<T>& m = dm.get<T>();
+  const int n = pv.nowned, all = pv.nall;
+  const size_t a1 = std::max(1, all), ne = std::max<long>(1, ws.nedges);
+  const int* off = ws.nbr_off.get<int>();
+  const Rec<T>* rec = ws.rec.get<Rec<T>>();
+  const size_t lf = forward_lds<Cf>(m), lg = gather_lds<Cf>(m), lb = backward_lds<Cf>(m);
+  if (stage == 0) {  // begin: gate pass over the owned atoms; ghost gate signals arrive by forward comm
+    ws.h.ensure(sizeof(T) * a1 * Cf::F);
+    ws.hbar.ensure(sizeof(T) * a1 * Cf::F);
+    ws.abar.ensure(sizeof(T) * a1 * Cf::KH);
+    ws.gbar.ensure(sizeof(T) * a1 * Cf::KH);
+    ws.vg.ensure(sizeof(T) * a1 * std::max(1, m.gp.node_cnt));
+    ws.site.ensure(sizeof(T) * a1);
+    ws.fw.ensure(sizeof(T) * a1 * 9);
+    ws.gedge.ensure(sizeof(T) * ne * 3);
+    if (m.gated) ws.uedge.ensure(sizeof(T) * ne * Cf::F);
+    if (m.gated && all > 0) {
+      fill_zero(ws, ws.h.get<T>(), (size_t)all * Cf::F, st);
+      if (n > 0) {
+        allow_lds(k_pass_forward<Cf, true>, lf);
+        SUS2_LAUNCH(ws, st, kname<Cf>("pass_forward_gate"), (k_pass_forward<Cf, true>), dim3(n), dim3(WAVE), lf, m,
+                    pv.types, off, rec, ws.h.get<T>(), ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
+      }
+    }
+    return;
+  }
+  if (stage == 1) {  // outer: outer pass, its coordinate reverse, forces and gate adjoints of all atoms
+    if (all > 0) fill_zero(ws, ws.fw.get<T>(), (size_t)all * 9, st);
+    if (n > 0) {
+      allow_lds(k_pass_forward<Cf, false>, lf);
+      SUS2_LAUNCH(ws, st, kname<Cf>("pass_forward_outer"), (k_pass_forward<Cf, false>), dim3(n), dim3(WAVE), lf, m,
+                  pv.types, off, rec, ws.h.get<T>(), ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
+      SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_outer"), (k_edge_backward<Cf, false>), dim3(n), dim3(WAVE), lb,
+                  m, pv.types, off, rec, ws.h.get<T>(), ws.abar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(),
+                  ws.fw.get<T>());
+    }
+    if (all > 0) {
+      allow_lds(k_gather<Cf, false>, lg);
+      SUS2_LAUNCH(ws, st, kname<Cf>("gather_outer"), (k_gather<Cf, false>), dim3(all), dim3(WAVE), 0, m, pv.types,
+                  n, off, pv.in_off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
+                  ws.gbar.get<T>(), m.gated ? ws.hbar.get<T>() : (T*)nullptr, ws.fw.get<T>());
+    }
+    return;
+  }
+  // finish: gate reverse of the owned atoms (their adjoints now include the
+  // ghosts' contributions), gate coordinate reverse, forces, reductions
+  if (m.gated && n > 0) {
+    allow_lds(k_gate_reverse<Cf>, lg);
+    SUS2_LAUNCH(ws, st, kname<Cf>("gate_reverse"), (k_gate_reverse<Cf>), dim3(n), dim3(WAVE), lg, m,
+                ws.hbar.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>());
+    SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_gate"), (k_edge_backward<Cf, true>), dim3(n), dim3(WAVE), lb, m,
+                pv.types, off, rec, ws.h.get<T>(), ws.gbar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(),
+                ws.fw.get<T>());
+    SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(all), dim3(WAVE), 0, m, pv.types, n,
+                off, pv.in_off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
+                ws.gbar.get<T>(), (T*)nullptr, ws.fw.get<T>());
+  }
+  Batch b;
+  b.nframes = 1;
+  b.natoms = all;
+  b.frame_off = pv.frame_off;  // {0, nowned}: energy and virial of the owned atoms
+  b.types = pv.types;
+  EfsOut out;
+  out.energy = pv.energy;
+  out.force = pv.force;
+  out.virial = pv.virial;
+  out.site = pv.site;
+  finish_efs<T>(ws, b, out, SiteConst{dm.d.view.theta, m.o_species, m.o_shift}, st, n);
 }
 
 }  // namespace sus2

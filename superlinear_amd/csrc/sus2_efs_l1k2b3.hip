@@ -10,4 +10,8 @@ template void run_grad_shape<Cfg<float, 1, 2, 3, 3>>(const DevModelSet&, Workspa
                                               double*, double, bool, bool, hipStream_t);
 template void run_grad_shape<Cfg<double, 1, 2, 3, 3>>(const DevModelSet&, Workspace&, const Batch&, const GradWeights&,
                                               double*, double, bool, bool, hipStream_t);
+template void pair_stage_shape<Cfg<float, 1, 2, 3, 3>>(int, const DevModelSet&, Workspace&, const PairView&,
+                                                 hipStream_t);
+template void pair_stage_shape<Cfg<double, 1, 2, 3, 3>>(int, const DevModelSet&, Workspace&, const PairView&,
+                                                 hipStream_t);
 }  // namespace sus2

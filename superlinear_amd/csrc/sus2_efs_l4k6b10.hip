@@ -10,4 +10,8 @@ template void run_grad_shape<Cfg<float, 4, 6, 10, 2>>(const DevModelSet&, Worksp
                                               double*, double, bool, bool, hipStream_t);
 template void run_grad_shape<Cfg<double, 4, 6, 10, 2>>(const DevModelSet&, Workspace&, const Batch&, const GradWeights&,
                                               double*, double, bool, bool, hipStream_t);
+template void pair_stage_shape<Cfg<float, 4, 6, 10, 2>>(int, const DevModelSet&, Workspace&, const PairView&,
+                                                 hipStream_t);
+template void pair_stage_shape<Cfg<double, 4, 6, 10, 2>>(int, const DevModelSet&, Workspace&, const PairView&,
+                                                 hipStream_t);
 }  // namespace sus2
