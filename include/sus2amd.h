@@ -104,6 +104,21 @@ int sus2_loss_grad(sus2_model* m, int precision, int nframes, int natoms, const 
                    const double* e0, const double* f0, const double* s0, int weighting, double wE, double wF,
                    double wS, double* loss_out, double* grad, void* stream);
 
+/* --- training driver ----------------------------------------------------------
+ * L-BFGS (host FP64, `history` pairs, Armijo backtracking) minimising the
+ * objective of sus2_loss_grad over one device-resident batch; replaces the
+ * BFGS/L-BFGS loop of MTPR_trainer::Train (dev_src/mtpr_trainer.cpp:4685-4760,
+ * src/common/lbfgs.h:114).  The model's coefficients end at the last accepted
+ * point.  loss_trace [max_iter+1] (host, may be NULL) receives the objective
+ * at the start and after each accepted step; *iters_out the steps taken.
+ * Stops when max |grad| <= gtol, after max_iter steps or when the line search
+ * fails. */
+int sus2_train_lbfgs(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                     const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+                     const double* e0, const double* f0, const double* s0, int weighting, double wE, double wF,
+                     double wS, int max_iter, int history, double gtol, double* loss_trace, int* iters_out,
+                     void* stream);
+
 /* --- pair_style sus2/embedded compute() contract ------------------------------
  * Mirrors Kernel (interfaces/lammps/ML-SUS2/sus2_sl_kernel.h:12-43) as driven
  * by PairSUS2Embedded::compute (pair_sus2_embedded.cpp:250-313):

@@ -42,6 +42,8 @@ SIGNATURES = {
     "sus2_efs_grad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sus2_loss_grad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _dp, _vp,
                             _vp]),
+    "sus2_train_lbfgs": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _i, _i, _d,
+                              _vp, _ip, _vp]),
     "sus2_pair_create": (_i, [_vp, _i, ctypes.POINTER(_vp)]),
     "sus2_pair_free": (None, [_vp]),
     "sus2_pair_comm_width": (_i, [_vp]),

@@ -204,6 +204,29 @@ class Sus2Model:
                                              float(wS), ctypes.byref(loss), grad.data_ptr(), ctypes.c_void_p(st)))
         return loss.value, grad
 
+    def train_lbfgs(self, batch, e0, f0, s0=None, weighting="structures", wE=1.0, wF=0.01, wS=0.0,
+                    precision="fp32", max_iter=50, history=20, gtol=0.0, stream=None):
+        """L-BFGS on the training objective (host FP64 optimiser, GPU loss and
+        gradient); the model's coefficients end at the last accepted point.
+        Returns (loss trace, steps taken)."""
+        import torch
+        dev = batch.pos.device
+        mode = {"vibrations": 0, "structures": 1, "molecules": 2}.get(weighting, weighting)
+        e0 = e0.to(device=dev, dtype=torch.float64).contiguous()
+        f0 = f0.to(device=dev, dtype=torch.float64).contiguous()
+        s0 = (torch.zeros((batch.nframes, 6), dtype=torch.float64, device=dev) if s0 is None
+              else s0.to(device=dev, dtype=torch.float64).contiguous())
+        trace = np.zeros(max_iter + 1)
+        iters = ctypes.c_int()
+        st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        _lib.check(_lib.lib().sus2_train_lbfgs(self._h, PRECISION[precision], batch.nframes, batch.natoms,
+                                               batch.off.data_ptr(), batch.types.data_ptr(), batch.pos.data_ptr(),
+                                               batch.cell.data_ptr(), batch.periodic.data_ptr(), e0.data_ptr(),
+                                               f0.data_ptr(), s0.data_ptr(), int(mode), float(wE), float(wF),
+                                               float(wS), int(max_iter), int(history), float(gtol),
+                                               _p(trace), ctypes.byref(iters), ctypes.c_void_p(st)))
+        return trace[:iters.value + 1], iters.value
+
 
 def _str_weights_from_virial_grad(gV):
     """str_w (row-major 3x3 per frame) that turns sum_e w_e . g_e into

@@ -150,3 +150,32 @@ def test_loss_grad_invalid_weighting(lib):
     n = batch.natoms
     with pytest.raises(Sus2Error):
         m.loss_grad_device(batch, torch.zeros(2).cuda(), torch.zeros(n, 3).cuda(), weighting=7)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_lbfgs_training_reduces_objective(lib, prec):
+    """The host L-BFGS over the GPU objective: Armijo-accepted steps never
+    increase the objective, it drops well below its start on a teacher-student
+    batch, and the model ends at the last accepted point."""
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    print(f"[case] lbfgs {prec}", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frames = W.periodic_frames(6, 8, 16, 2, 0.15, seed=97)
+    batch = DeviceBatch(frames)
+    teacher = Sus2Model(path)
+    th = teacher.coeffs()
+    rng = np.random.default_rng(98)
+    teacher.set_coeffs(th * (1 + 0.05 * rng.standard_normal(th.size)))
+    lab = teacher.efs_device(batch, "fp64")
+    m = Sus2Model(path)
+    trace, steps = m.train_lbfgs(batch, lab["energy"], lab["forces"], weighting="structures", wE=1.0, wF=0.01,
+                                 precision=prec, max_iter=25, history=10)
+    print(f"[result] lbfgs {prec}: {steps} steps, objective {trace[0]:.4e} -> {trace[-1]:.4e}", flush=True)
+    assert steps >= 5
+    assert np.all(np.diff(trace) <= 1e-12 * trace[0])
+    assert trace[-1] < 0.2 * trace[0]
+    loss, _ = m.loss_grad_device(batch, lab["energy"], lab["forces"], precision=prec)
+    assert abs(loss - trace[-1]) <= 1e-9 * trace[0]
+    assert not np.allclose(m.coeffs(), th)
