@@ -60,10 +60,16 @@ struct Grid {
 // non-periodic direction, as in the reference's Configuration); open
 // directions get unit normals scaled to the atoms' extent, so fractional
 // coordinates along them lie in [0, 1).  periodic == 0 frames are molecules.
-__global__ void k_grid_setup(int nframes, const int64_t* __restrict__ frame_off, const double* __restrict__ pos,
-                             const double* __restrict__ cell, const int32_t* __restrict__ periodic, double rc,
-                             Grid* __restrict__ grid) {
+__global__ void k_grid_setup(int nframes, int natoms, const int64_t* __restrict__ frame_off,
+                             const double* __restrict__ pos, const double* __restrict__ cell,
+                             const int32_t* __restrict__ periodic, double rc, Grid* __restrict__ grid,
+                             int* __restrict__ z0, int* __restrict__ z1, int* __restrict__ z2) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < natoms) {  // per-atom counters of the bin and incoming-list fills start at zero
+    z0[f] = 0;
+    z1[f] = 0;
+    z2[f] = 0;
+  }
   if (f >= nframes) return;
   Grid g;
   const long b0 = frame_off[f], na = frame_off[f + 1] - b0;
@@ -176,8 +182,9 @@ __global__ void k_fill_bins(int natoms, const int* __restrict__ cell_of, const i
 }
 
 // Sort the ids of one list [b, e) ascending (one wavefront per list; each
-// element's rank = number of smaller ids; ids are distinct).
-__device__ void wave_rank_sort(int* __restrict__ list, int b, int e, int* __restrict__ tmp) {
+// element's rank = number of smaller ids; ids are distinct).  With inv, also
+// inv[id] = its final position.
+__device__ void wave_rank_sort(int* __restrict__ list, int b, int e, int* __restrict__ tmp, int* __restrict__ inv) {
   const int lane = threadIdx.x, n = e - b;
   for (int k = lane; k < n; k += WAVE) tmp[k] = list[b + k];
   __syncthreads();
@@ -186,6 +193,7 @@ __device__ void wave_rank_sort(int* __restrict__ list, int b, int e, int* __rest
     int rank = 0;
     for (int q = 0; q < n; ++q) rank += tmp[q] < v;
     list[b + rank] = v;
+    if (inv) inv[v] = b + rank;
   }
   __syncthreads();
 }
@@ -193,23 +201,29 @@ __device__ void wave_rank_sort(int* __restrict__ list, int b, int e, int* __rest
 constexpr int SORT_MAX = 2048;
 
 __global__ __launch_bounds__(WAVE) void k_sort_lists(int nlists, const int* __restrict__ start,
-                                                     int* __restrict__ list) {
+                                                     int* __restrict__ list, int* __restrict__ inv) {
   __shared__ int tmp[SORT_MAX];
   const int c = blockIdx.x;
   if (c >= nlists) return;
   const int b = start[c], e = start[c + 1];
-  if (e - b <= 1) return;
-  if (e - b <= SORT_MAX) {
-    wave_rank_sort(list, b, e, tmp);
+  if (e - b <= 1) {
+    if (inv && e - b == 1 && threadIdx.x == 0) inv[list[b]] = b;
     return;
   }
-  if (threadIdx.x == 0)  // oversized list: serial insertion sort
+  if (e - b <= SORT_MAX) {
+    wave_rank_sort(list, b, e, tmp, inv);
+    return;
+  }
+  if (threadIdx.x == 0) {  // oversized list: serial insertion sort
     for (int k = b + 1; k < e; ++k) {
       const int v = list[k];
       int q = k - 1;
       while (q >= b && list[q] > v) { list[q + 1] = list[q]; --q; }
       list[q + 1] = v;
     }
+    if (inv)
+      for (int k = b; k < e; ++k) inv[list[k]] = k;
+  }
 }
 
 __global__ void k_zero_int(int n, int* __restrict__ a) {
@@ -223,7 +237,10 @@ __global__ void k_zero_int(int n, int* __restrict__ a) {
 // size), their sizes prefix-summed across the wave, and then every lane takes
 // one candidate of the flattened range (binary search for its offset), so a
 // step tests 64 candidates however small the bins are.  Hits are compacted in
-// candidate order with a ballot.  Count pass when rec is null, fill otherwise.
+// candidate order with a ballot.  Collect mode (cap > 0): hits go to the
+// atom's fixed-capacity slots out[i*cap + k] for k < cap and the count to
+// cnt[i].  Fill mode (cap == 0): hits go to out[off[i] + k]; only atoms whose
+// count exceeds `over` run (the ones whose hits did not fit their slots).
 __device__ __forceinline__ int floor_div(int a, int n) { return a >= 0 ? a / n : -((-a + n - 1) / n); }
 
 template <class T>
@@ -233,10 +250,11 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, int nframes, con
                                                     const int* __restrict__ wrap, const int* __restrict__ cell_start,
                                                     const int* __restrict__ cell_atoms, double rc,
                                                     const int* __restrict__ off, int* __restrict__ cnt,
-                                                    Rec<T>* __restrict__ rec) {
+                                                    Rec<T>* __restrict__ rec, int cap, int over) {
   __shared__ int o_b[WAVE], o_cum[WAVE], o_s[3][WAVE];
   const int i = blockIdx.x, lane = threadIdx.x;
   if (i >= natoms) return;
+  if (cap == 0 && cnt[i] <= over) return;
   const int f = frame_of(frame_off, nframes, i);
   const Grid g = grid[f];
   const double* a = cell + 9 * f;
@@ -250,7 +268,7 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, int nframes, con
   const double rc2 = rc * rc;
   const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
   int count = 0;
-  const int w0 = rec ? off[i] : 0;
+  const long w0 = cap ? (long)i * cap : (long)off[i];
   const int n0 = 2 * g.S[0] + 1, n1 = 2 * g.S[1] + 1, n2 = 2 * g.S[2] + 1, nof = n0 * n1 * n2;
   for (int ob = 0; ob < nof; ob += WAVE) {
     const int o = ob + lane;
@@ -310,12 +328,28 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, int nframes, con
         hit = !self && d0 * d0 + d1 * d1 + d2 * d2 < rc2;
       }
       const unsigned long long mask = __ballot(hit);
-      if (rec && hit) rec[w0 + count + __popcll(mask & below)] = Rec<T>{T(d0), T(d1), T(d2), rec_jbits(j, T(0))};
+      const int slot = count + __popcll(mask & below);
+      if (hit && (cap == 0 || slot < cap)) rec[w0 + slot] = Rec<T>{T(d0), T(d1), T(d2), rec_jbits(j, T(0))};
       count += __popcll(mask);
     }
     __syncthreads();
   }
-  if (!rec && lane == 0) cnt[i] = count;
+  if (cap && lane == 0) cnt[i] = count;
+}
+
+// Copy every atom's collected hits from its fixed-capacity slots to the CSR
+// records (atoms whose count exceeded the capacity are filled directly).
+template <class T>
+__global__ __launch_bounds__(WAVE) void k_compact(int natoms, int cap, const int* __restrict__ cnt,
+                                                  const int* __restrict__ off, const Rec<T>* __restrict__ slots,
+                                                  Rec<T>* __restrict__ rec) {
+  const int i = blockIdx.x;
+  if (i >= natoms) return;
+  const int n = cnt[i];
+  if (n > cap) return;
+  const Rec<T>* src = slots + (size_t)i * cap;
+  Rec<T>* dst = rec + off[i];
+  for (int k = threadIdx.x; k < n; k += WAVE) dst[k] = src[k];
 }
 
 // Incoming edges of every atom (edges e' = j -> i grouped by i).  The full
@@ -400,43 +434,52 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   ws.cell_cur.ensure(sizeof(int) * n1);
   ws.cell_atoms.ensure(sizeof(int) * n1);
   ws.in_cur.ensure(sizeof(int) * n1);
-  ws.scan_tmp.ensure(2 * sizeof(int64_t));
+  ws.scan_tmp.ensure(4 * sizeof(int64_t));
+  const int cap = ws.nbr_cap;
+  ws.nbr_slots.ensure(sizeof(Rec<T>) * n1 * cap);
   const int nb = (N + 255) / 256;
   const std::string tn = std::string("<") + tname<T>() + ">";
-  SUS2_LAUNCH(ws, st, "grid_setup", k_grid_setup, dim3((b.nframes + 63) / 64), dim3(64), 0, b.nframes, b.frame_off,
-              b.pos, b.cell, b.periodic, rc, ws.grid.get<Grid>());
+  SUS2_LAUNCH(ws, st, "grid_setup", k_grid_setup, dim3((std::max(b.nframes, N) + 63) / 64), dim3(64), 0, b.nframes,
+              N, b.frame_off, b.pos, b.cell, b.periodic, rc, ws.grid.get<Grid>(), ws.cell_cnt.get<int>(),
+              ws.cell_cur.get<int>(), ws.in_cur.get<int>());
   if (N > 0) {
-    SUS2_LAUNCH(ws, st, "zero", k_zero_int, dim3(nb), dim3(256), 0, N, ws.cell_cnt.get<int>());
-    SUS2_LAUNCH(ws, st, "zero", k_zero_int, dim3(nb), dim3(256), 0, N, ws.cell_cur.get<int>());
-    SUS2_LAUNCH(ws, st, "zero", k_zero_int, dim3(nb), dim3(256), 0, N, ws.in_cur.get<int>());
     SUS2_LAUNCH(ws, st, "bin_atoms", k_bin, dim3(nb), dim3(256), 0, N, b.nframes, b.frame_off, b.pos,
                 ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_cnt.get<int>());
-    device_scan(ws.cell_cnt.get<int>(), ws.cell_start.get<int>(), N, ws.scan_tmp.get<int64_t>() + 1, st);
+    device_scan(ws.cell_cnt.get<int>(), ws.cell_start.get<int>(), N, ws.scan_tmp.get<int64_t>() + 2, st);
     SUS2_LAUNCH(ws, st, "fill_bins", k_fill_bins, dim3(nb), dim3(256), 0, N, ws.cell_of.get<int>(),
                 ws.cell_start.get<int>(), ws.cell_cur.get<int>(), ws.cell_atoms.get<int>());
     SUS2_LAUNCH(ws, st, "sort_bins", k_sort_lists, dim3(N), dim3(WAVE), 0, N, ws.cell_start.get<int>(),
-                ws.cell_atoms.get<int>());
-    SUS2_LAUNCH(ws, st, "neighbors_count" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, b.nframes, b.frame_off,
-                b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
-                ws.cell_start.get<int>(), ws.cell_atoms.get<int>(), rc, nullptr, ws.nbr_cnt.get<int>(), nullptr);
+                ws.cell_atoms.get<int>(), (int*)nullptr);
+    SUS2_LAUNCH(ws, st, "neighbors" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, b.nframes, b.frame_off, b.pos,
+                b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(),
+                ws.cell_atoms.get<int>(), rc, (const int*)nullptr, ws.nbr_cnt.get<int>(),
+                ws.nbr_slots.get<Rec<T>>(), cap, 0);
   }
-  const long total = exclusive_scan(ws, N, st);
+  long maxcnt = 0;
+  const long total = exclusive_scan(ws, N, st, &maxcnt);
   if (total > INT32_MAX - 1) throw Error(SUS2_E_RANGE, "neighbour list exceeds 2^31 edges");
   ws.nedges = total;
   ws.rec.ensure(sizeof(Rec<T>) * std::max<long>(total, 1));
   ws.in_edges.ensure(sizeof(int) * std::max<long>(total, 1));
+  ws.in_pos.ensure(sizeof(int) * std::max<long>(total, 1));
   if (N > 0) {
-    SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, b.nframes, b.frame_off,
-                b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
-                ws.cell_start.get<int>(), ws.cell_atoms.get<int>(), rc, ws.nbr_off.get<int>(), nullptr,
-                ws.rec.get<Rec<T>>());
+    SUS2_LAUNCH(ws, st, "compact" + tn, k_compact<T>, dim3(N), dim3(WAVE), 0, N, cap, ws.nbr_cnt.get<int>(),
+                ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>());
+    if (maxcnt > cap)  // atoms with more neighbours than slots: fill their records directly
+      SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, b.nframes, b.frame_off,
+                  b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
+                  ws.cell_start.get<int>(), ws.cell_atoms.get<int>(), rc, ws.nbr_off.get<int>(),
+                  ws.nbr_cnt.get<int>(), ws.rec.get<Rec<T>>(), 0, cap);
     if (total > 0) {
       SUS2_LAUNCH(ws, st, "incoming" + tn, k_incoming<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, total,
                   ws.rec.get<Rec<T>>(), ws.nbr_off.get<int>(), ws.in_cur.get<int>(), ws.in_edges.get<int>());
       SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(WAVE), 0, N, ws.nbr_off.get<int>(),
-                  ws.in_edges.get<int>());
+                  ws.in_edges.get<int>(), ws.in_pos.get<int>());
     }
   }
+  // slots for the next call: the largest count seen, with headroom, a multiple of 16
+  const long want = std::max<long>(16, ((maxcnt + maxcnt / 4 + 15) / 16) * 16);
+  if (want > cap) ws.nbr_cap = (int)std::min<long>(want, 4096);
 }
 
 template <class T>
@@ -489,9 +532,10 @@ void build_pair_lists(Workspace& ws, int nowned, int nall, long nedges, const in
   ws.cell_cur.ensure(sizeof(int) * n1);
   ws.cell_atoms.ensure(sizeof(int) * e1);  // edge order by centre
   ws.wrap.ensure(sizeof(int32_t) * e1);    // target of each CSR position
-  ws.scan_tmp.ensure(2 * sizeof(int64_t));
+  ws.scan_tmp.ensure(4 * sizeof(int64_t));
   ws.rec.ensure(sizeof(Rec<T>) * e1);
   ws.in_edges.ensure(sizeof(int) * e1);
+  ws.in_pos.ensure(sizeof(int) * e1);
   ws.nedges = nedges;
   const unsigned ge = (unsigned)((nedges + 255) / 256);
   int* cnt = ws.cell_cnt.get<int>();
@@ -508,7 +552,7 @@ void build_pair_lists(Workspace& ws, int nowned, int nall, long nedges, const in
                 order);
     if (nowned > 0)
       SUS2_LAUNCH(ws, st, "pair_sort", k_sort_lists, dim3(nowned), dim3(WAVE), 0, nowned, ws.nbr_off.get<int>(),
-                  order);
+                  order, (int*)nullptr);
     SUS2_LAUNCH(ws, st, "pair_rec", k_build_rec<T>, dim3(ge), dim3(256), 0, nedges, order, ej, d,
                 ws.rec.get<Rec<T>>(), tgt);
   }
@@ -516,13 +560,13 @@ void build_pair_lists(Workspace& ws, int nowned, int nall, long nedges, const in
   fill_zero(ws, cnt, n1, st);
   fill_zero(ws, cur, n1, st);
   if (nedges > 0) SUS2_LAUNCH(ws, st, "pair_count", k_count_keys, dim3(ge), dim3(256), 0, nedges, tgt, cnt);
-  device_scan(cnt, in_off, nall, ws.scan_tmp.get<int64_t>() + 1, st);
+  device_scan(cnt, in_off, nall, ws.scan_tmp.get<int64_t>() + 2, st);
   if (nedges > 0) {
     SUS2_LAUNCH(ws, st, "pair_place", k_place_keys, dim3(ge), dim3(256), 0, nedges, tgt, in_off, cur,
                 ws.in_edges.get<int>());
     if (nall > 0)
       SUS2_LAUNCH(ws, st, "pair_sort", k_sort_lists, dim3(nall), dim3(WAVE), 0, nall, in_off,
-                  ws.in_edges.get<int>());
+                  ws.in_edges.get<int>(), ws.in_pos.get<int>());
   }
 }
 template void build_pair_lists<float>(Workspace&, int, int, long, const int32_t*, const int32_t*, const double*,

@@ -124,11 +124,22 @@ __host__ __device__ inline int graph_scratch(const DPass<T>& p) {
 }
 
 // ------------------------------------------------------------------ pass forward
+// LDS of k_pass_forward: the species tables, then one region used first for
+// the edge staging [F + H][LR] and afterwards (the staging is dead once the
+// moments are in registers) for moments [KH], node values and adjoints [2M]
+// and the graph scratch -- the overlay keeps the footprint small enough for
+// every workgroup of a 4096-atom frame to be resident at once.
+template <class Cf>
+__host__ __device__ inline size_t forward_region(int M, int sc) {
+  const size_t stage = (size_t)(Cf::F + Cf::H) * LR;
+  const size_t graph = (size_t)Cf::KH + 2 * (size_t)M + (size_t)(sc > 1 ? sc : 1);
+  return stage > graph ? stage : graph;
+}
+
 template <class Cf>
 size_t forward_lds(const DModel<typename Cf::T>& m) {
   const int sc = std::max(graph_scratch(m.gp), graph_scratch(m.op));
-  return sizeof(typename Cf::T) * ((size_t)(Cf::F + Cf::H) * LR + Cf::KH + 2 * (size_t)m.M + std::max(sc, 1) +
-                                   species_tables_size<Cf>(m.C));
+  return sizeof(typename Cf::T) * (species_tables_size<Cf>(m.C) + forward_region<Cf>(m.M, sc));
 }
 
 template <class Cf, bool GATE, class T = typename Cf::T>
@@ -139,13 +150,14 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
                                                        T* __restrict__ abar) {
   constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, H = Cf::H, F = Cf::F, KH = Cf::KH, NM = Cf::NM;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sR = reinterpret_cast<T*>(smem_raw);  // [F][LR]
+  T* tabs_base = reinterpret_cast<T*>(smem_raw);
+  T* region = tabs_base + species_tables_size<Cf>(m.C);
+  T* sR = region;                          // [F][LR]  edge phase
   T* sY = sR + F * LR;                     // [H][LR]
-  T* mom = sY + H * LR;                    // [KH]
+  T* mom = region;                         // [KH]     after the edge phase
   T* v = mom + KH;                         // [M]
   T* a = v + m.M;                          // [M]
-  T* tabs_base = a + m.M;
-  T* scratch = tabs_base + species_tables_size<Cf>(m.C);
+  T* scratch = a + m.M;
   const int i = blockIdx.x, lane = threadIdx.x;
   const int ti = types[i];
   SpeciesTables<T> tab;
@@ -194,6 +206,7 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
     }
     __syncthreads();
   }
+  // (the edge loop ends with a barrier, so the staging region is free here)
   for (int n = lane; n < m.M; n += WAVE) v[n] = T(0);
 #pragma unroll
   for (int t = 0; t < NM; ++t) {
@@ -242,11 +255,13 @@ template <class Cf, bool GATE, class T = typename Cf::T>
 __global__ __launch_bounds__(WAVE) void k_edge_backward(DModel<T> m, const int32_t* __restrict__ types,
                                                         const int* __restrict__ nbr_off,
                                                         const Rec<T>* __restrict__ rec, const T* __restrict__ h,
-                                                        const T* __restrict__ adj, T* __restrict__ gedge,
+                                                        const T* __restrict__ adj, const int* __restrict__ in_pos,
+                                                        T* __restrict__ gedge, T* __restrict__ gedge_in,
                                                         T* __restrict__ uedge, T* __restrict__ fw) {
   constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, F = Cf::F, KH = Cf::KH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* sA = reinterpret_cast<T*>(smem_raw);  // [KH]
+  T* sXR = sA + KH + species_tables_size<Cf>(m.C);  // [F][LR] lane-private columns: x0 R (gate message)
   SpeciesTables<T> tab;
   const int i = blockIdx.x, lane = threadIdx.x;
   const int ti = types[i];
@@ -265,16 +280,22 @@ __global__ __launch_bounds__(WAVE) void k_edge_backward(DModel<T> m, const int32
     const T inv = T(1) / r;
     const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
     const int tj = types[j];
-    T R[F], dR[F], wt[F], us[F];
+    // weighted radial values x0 x1 R and x0 x1 R' in registers; x0 R (the gate
+    // message factor) parked in this lane's own LDS column
+    T R[F], dR[F], us[F];
     radial<Cf, true>(m, GATE, tab, ti * m.C + tj, r, R, dR);
     const T* x0j = tab.x0 + tj * F;
     const T* alj = tab.alpha + tj * F;
 #pragma unroll
     for (int mu = 0; mu < F; ++mu) {
       T x1 = x0j[mu];
-      if (!GATE && m.gated) x1 += alj[mu] * h[(size_t)j * F + mu];
-      wt[mu] = x0[mu] * x1;
-      dR[mu] *= wt[mu];
+      if (hgate) {
+        x1 += alj[mu] * h[(size_t)j * F + mu];
+        sXR[mu * LR + lane] = x0[mu] * R[mu];
+      }
+      const T wt = x0[mu] * x1;
+      R[mu] *= wt;
+      dR[mu] *= wt;
       us[mu] = T(0);
     }
     T sY = T(0), gx = T(0), gy = T(0), gz = T(0);
@@ -285,20 +306,23 @@ __global__ __launch_bounds__(WAVE) void k_edge_backward(DModel<T> m, const int32
         const int mu = k * L1 + l;
         const T av = sA[hh * KK + k];
         sY += av * dR[mu] * Y;
-        const T g = av * wt[mu] * R[mu];
+        const T g = av * R[mu];
         gx += g * yx; gy += g * yy; gz += g * yz;
         if (hgate) us[mu] += av * Y;
       }
     });
     const T g0 = ux * sY + gx, g1 = uy * sY + gy, g2 = uz * sY + gz;
+    const int slot = in_pos[e];  // the edge's position in its target's incoming list
     T* ge = gedge + 3 * (size_t)e;
     ge[0] = g0; ge[1] = g1; ge[2] = g2;
+    T* gi = gedge_in + 3 * (size_t)slot;
+    gi[0] = g0; gi[1] = g1; gi[2] = g2;
     w0 -= dx * g0; w1 -= dy * g1; w2 -= dz * g2;
     w3 -= dx * g1; w4 -= dx * g2; w5 -= dy * g2;
     if (hgate) {
-      T* ue = uedge + (size_t)e * F;
+      T* ue = uedge + (size_t)slot * F;
 #pragma unroll
-      for (int mu = 0; mu < F; ++mu) ue[mu] = x0[mu] * R[mu] * us[mu];
+      for (int mu = 0; mu < F; ++mu) ue[mu] = sXR[mu * LR + lane] * us[mu];
     }
   }
   w0 = wave_sum(w0); w1 = wave_sum(w1); w2 = wave_sum(w2);
@@ -338,7 +362,8 @@ __device__ void gate_reverse_atom(const DModel<T>& m, int i, const T* __restrict
 }
 
 // F_a = sum_{e in out(a)} g_e - sum_{e' in in(a)} g_e' over the centre's own
-// edges and its sorted incoming-edge list (no atomics).  The outer pass of an
+// edges and its incoming-list slots (written there by the edge kernels; no
+// atomics, contiguous reads).  The outer pass of an
 // Effect model also forms h_bar_a = alpha_{t_a} sum_{e' in in(a)} u_e' and
 // either runs the gate graph reverse at once (hbar_out == null) or stores
 // h_bar for a later reverse (the pair-style path, whose ghost adjoints are
@@ -346,7 +371,7 @@ __device__ void gate_reverse_atom(const DModel<T>& m, int i, const T* __restrict
 template <class Cf, bool GATE, class T = typename Cf::T>
 __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __restrict__ types, int nowned,
                                                  const int* __restrict__ out_off, const int* __restrict__ in_off,
-                                                 const int* __restrict__ in_edges, const T* __restrict__ gedge,
+                                                 const T* __restrict__ gedge, const T* __restrict__ gedge_in,
                                                  const T* __restrict__ uedge, const T* __restrict__ vg,
                                                  T* __restrict__ gbar, T* __restrict__ hbar_out,
                                                  T* __restrict__ fw) {
@@ -362,12 +387,11 @@ __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __r
       const T* go = gedge + 3 * (size_t)k;
       fx += go[0]; fy += go[1]; fz += go[2];
     }
-  for (int k = in_off[i] + lane; k < in_off[i + 1]; k += WAVE) {
-    const int ein = in_edges[k];
-    const T* gi = gedge + 3 * (size_t)ein;
+  for (int k = in_off[i] + lane; k < in_off[i + 1]; k += WAVE) {  // incoming slots: contiguous
+    const T* gi = gedge_in + 3 * (size_t)k;
     fx -= gi[0]; fy -= gi[1]; fz -= gi[2];
     if (hgate) {
-      const T* ue = uedge + (size_t)ein * F;
+      const T* ue = uedge + (size_t)k * F;
 #pragma unroll
       for (int mu = 0; mu < F; ++mu) hb[mu] += ue[mu];
     }
@@ -420,7 +444,8 @@ size_t gather_lds(const DModel<typename Cf::T>& m) {
 
 template <class Cf>
 size_t backward_lds(const DModel<typename Cf::T>& m) {
-  return sizeof(typename Cf::T) * ((size_t)Cf::KH + species_tables_size<Cf>(m.C));
+  return sizeof(typename Cf::T) *
+         ((size_t)Cf::KH + species_tables_size<Cf>(m.C) + (m.gated ? (size_t)Cf::F * LR : 0));
 }
 
 // ------------------------------------------------------------------ driver
@@ -456,6 +481,7 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
   ws.site.ensure(sizeof(T) * n1);
   ws.fw.ensure(sizeof(T) * n1 * 9);
   ws.gedge.ensure(sizeof(T) * ne * 3);
+  ws.gedge_in.ensure(sizeof(T) * ne * 3);
   if (m.gated) ws.uedge.ensure(sizeof(T) * ne * Cf::F);
   const int* off = ws.nbr_off.get<int>();
   const Rec<T>* rec = ws.rec.get<Rec<T>>();
@@ -470,17 +496,20 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
     allow_lds(k_pass_forward<Cf, false>, lf);
     SUS2_LAUNCH(ws, st, kname<Cf>("pass_forward_outer"), (k_pass_forward<Cf, false>), dim3(N), dim3(WAVE), lf, m,
                 b.types, off, rec, h, ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
+    const int* ip = ws.in_pos.get<int>();
     SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_outer"), (k_edge_backward<Cf, false>), dim3(N), dim3(WAVE), lb, m,
-                b.types, off, rec, h, ws.abar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
+                b.types, off, rec, h, ws.abar.get<T>(), ip, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(),
+                ws.fw.get<T>());
     allow_lds(k_gather<Cf, false>, lg);
     SUS2_LAUNCH(ws, st, kname<Cf>("gather_outer"), (k_gather<Cf, false>), dim3(N), dim3(WAVE), m.gated ? lg : 0, m,
-                b.types, N, off, off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
+                b.types, N, off, off, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
                 ws.gbar.get<T>(), (T*)nullptr, ws.fw.get<T>());
     if (m.gated) {
       SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_gate"), (k_edge_backward<Cf, true>), dim3(N), dim3(WAVE), lb, m,
-                  b.types, off, rec, h, ws.gbar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
+                  b.types, off, rec, h, ws.gbar.get<T>(), ip, ws.gedge.get<T>(), ws.gedge_in.get<T>(),
+                  ws.uedge.get<T>(), ws.fw.get<T>());
       SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(N), dim3(WAVE), 0, m, b.types, N, off,
-                  off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>(),
+                  off, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>(),
                   (T*)nullptr, ws.fw.get<T>());
     }
   }
@@ -511,6 +540,7 @@ void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const Pai
     ws.site.ensure(sizeof(T) * a1);
     ws.fw.ensure(sizeof(T) * a1 * 9);
     ws.gedge.ensure(sizeof(T) * ne * 3);
+    ws.gedge_in.ensure(sizeof(T) * ne * 3);
     if (m.gated) ws.uedge.ensure(sizeof(T) * ne * Cf::F);
     if (m.gated && all > 0) {
       fill_zero(ws, ws.h.get<T>(), (size_t)all * Cf::F, st);
@@ -529,13 +559,13 @@ void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const Pai
       SUS2_LAUNCH(ws, st, kname<Cf>("pass_forward_outer"), (k_pass_forward<Cf, false>), dim3(n), dim3(WAVE), lf, m,
                   pv.types, off, rec, ws.h.get<T>(), ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
       SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_outer"), (k_edge_backward<Cf, false>), dim3(n), dim3(WAVE), lb,
-                  m, pv.types, off, rec, ws.h.get<T>(), ws.abar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(),
-                  ws.fw.get<T>());
+                  m, pv.types, off, rec, ws.h.get<T>(), ws.abar.get<T>(), ws.in_pos.get<int>(), ws.gedge.get<T>(),
+                  ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
     }
     if (all > 0) {
       allow_lds(k_gather<Cf, false>, lg);
       SUS2_LAUNCH(ws, st, kname<Cf>("gather_outer"), (k_gather<Cf, false>), dim3(all), dim3(WAVE), 0, m, pv.types,
-                  n, off, pv.in_off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
+                  n, off, pv.in_off, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
                   ws.gbar.get<T>(), m.gated ? ws.hbar.get<T>() : (T*)nullptr, ws.fw.get<T>());
     }
     return;
@@ -547,10 +577,10 @@ void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const Pai
     SUS2_LAUNCH(ws, st, kname<Cf>("gate_reverse"), (k_gate_reverse<Cf>), dim3(n), dim3(WAVE), lg, m,
                 ws.hbar.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>());
     SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_gate"), (k_edge_backward<Cf, true>), dim3(n), dim3(WAVE), lb, m,
-                pv.types, off, rec, ws.h.get<T>(), ws.gbar.get<T>(), ws.gedge.get<T>(), ws.uedge.get<T>(),
-                ws.fw.get<T>());
+                pv.types, off, rec, ws.h.get<T>(), ws.gbar.get<T>(), ws.in_pos.get<int>(), ws.gedge.get<T>(),
+                ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
     SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(all), dim3(WAVE), 0, m, pv.types, n,
-                off, pv.in_off, ws.in_edges.get<int>(), ws.gedge.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
+                off, pv.in_off, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
                 ws.gbar.get<T>(), (T*)nullptr, ws.fw.get<T>());
   }
   Batch b;

@@ -281,7 +281,8 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
                                   const int32_t* __restrict__ types, const Rec<typename Cf::T>* __restrict__ rec,
                                   const double* __restrict__ frc_w, const typename Cf::T* Sw, const double* fwi,
                                   const typename Cf::T* __restrict__ h, const typename Cf::T* __restrict__ hd,
-                                  typename Cf::T* __restrict__ msg, int ti, int e0, int e1,
+                                  typename Cf::T* __restrict__ msg, const int* __restrict__ in_pos, int ti,
+                                  int e0, int e1,
                                   const GradSmem<typename Cf::T>& s, double* creg, int pn_oW, int pn_nW) {
   using T = typename Cf::T;
   constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, F = Cf::F, BB = Cf::BB, NBLK = Cf::NBLK;
@@ -388,7 +389,7 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
             sX1[mu * LR + lane] = x0[mu] * Z;
             if (hg) {
               sAl[mu * LR + lane] = x0[mu] * (Z * hj + Rq * hdj);
-              T* mo = msg + (size_t)e * 2 * F;
+              T* mo = msg + (size_t)in_pos[e] * 2 * F;
               mo[mu] = alj[mu] * x0[mu] * Z;
               mo[F + mu] = alj[mu] * x0[mu] * Rq;
             }
@@ -580,8 +581,8 @@ __global__ __launch_bounds__(WAVE) void k_grad_outer(DModel<T> m, GradLds<Cf> L,
                                                      const int32_t* __restrict__ types,
                                                      const int* __restrict__ nbr_off, const Rec<T>* __restrict__ rec,
                                                      const T* __restrict__ h, const T* __restrict__ hd,
-                                                     T* __restrict__ msg, double* __restrict__ partial, int oW,
-                                                     int nW) {
+                                                     const int* __restrict__ in_pos, T* __restrict__ msg,
+                                                     double* __restrict__ partial, int oW, int nW) {
   constexpr int KK = Cf::KK, BB = Cf::BB;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const GradSmem<T> s = carve(L, smem_raw);
@@ -639,7 +640,8 @@ __global__ __launch_bounds__(WAVE) void k_grad_outer(DModel<T> m, GradLds<Cf> L,
     __syncthreads();
     grad_moment_space<Cf, false>(m, p, s, ti, oW, nW);
     __syncthreads();
-    grad_sweep_params<Cf, false>(m, tab, types, rec, gw.frc_w, Sw, fwi, h, hd, msg, ti, e0, e1, s, creg, oW, nW);
+    grad_sweep_params<Cf, false>(m, tab, types, rec, gw.frc_w, Sw, fwi, h, hd, msg, in_pos, ti, e0, e1, s, creg, oW,
+                                 nW);
   }
   __syncthreads();
   flush_partial<Cf>(m, false, creg, s.part, L.pn, oW, nW, partial + (size_t)blockIdx.x * L.pn, false);
@@ -677,8 +679,8 @@ __global__ __launch_bounds__(WAVE) void k_grad_gate_reverse(DModel<T> m, GradLds
     T hb[F], hbd[F];
 #pragma unroll
     for (int mu = 0; mu < F; ++mu) { hb[mu] = T(0); hbd[mu] = T(0); }
-    for (int k = e0 + lane; k < e1; k += WAVE) {
-      const T* mi = msg + (size_t)in_edges[k] * 2 * F;
+    for (int k = e0 + lane; k < e1; k += WAVE) {  // incoming slots (same range as the out edges)
+      const T* mi = msg + (size_t)k * 2 * F;
 #pragma unroll
       for (int mu = 0; mu < F; ++mu) { hb[mu] += mi[mu]; hbd[mu] += mi[F + mu]; }
     }
@@ -721,8 +723,8 @@ __global__ __launch_bounds__(WAVE) void k_grad_gate_reverse(DModel<T> m, GradLds
     __syncthreads();
     grad_moment_space<Cf, true>(m, p, s, ti, oW, nW);
     __syncthreads();
-    grad_sweep_params<Cf, true>(m, tab, types, rec, gw.frc_w, Sw, fwi, nullptr, nullptr, nullptr, ti, e0, e1, s,
-                                creg, oW, nW);
+    grad_sweep_params<Cf, true>(m, tab, types, rec, gw.frc_w, Sw, fwi, nullptr, nullptr, nullptr, nullptr, ti, e0,
+                                e1, s, creg, oW, nW);
   }
   __syncthreads();
   flush_partial<Cf>(m, true, creg, s.part, L.pn, oW, nW, partial + (size_t)blockIdx.x * L.pn, true);
@@ -820,7 +822,8 @@ void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const 
     }
     allow_lds(k_grad_outer<Cf>, lo.bytes());
     SUS2_LAUNCH(ws, st, kname<Cf>("grad_outer"), (k_grad_outer<Cf>), dim3(G), dim3(WAVE), lo.bytes(), m, lo, gw, N,
-                b.types, off, rec, ws.h.get<T>(), ws.hdot.get<T>(), ws.msg.get<T>(), ws.gpart.get<double>(), oW, nW);
+                b.types, off, rec, ws.h.get<T>(), ws.hdot.get<T>(), ws.in_pos.get<int>(), ws.msg.get<T>(),
+                ws.gpart.get<double>(), oW, nW);
     if (m.gated) {
       const GradLds<Cf> l3 = grad_lds<Cf>(m, true, true, pn);
       allow_lds(k_grad_gate_reverse<Cf>, l3.bytes());

@@ -84,7 +84,7 @@ Workspace::~Workspace() {
 void Workspace::ensure_atoms(int n) {
   nbr_cnt.ensure(sizeof(int) * (size_t)std::max(n, 1));
   nbr_off.ensure(sizeof(int) * ((size_t)n + 1));
-  if (!host_total) RT_CHECK(hipHostMalloc((void**)&host_total, sizeof(int64_t), hipHostMallocDefault));
+  if (!host_total) RT_CHECK(hipHostMalloc((void**)&host_total, 2 * sizeof(int64_t), hipHostMallocDefault));
 }
 
 // Single-workgroup exclusive scan: 1024 threads each own a contiguous chunk,
@@ -93,17 +93,25 @@ void Workspace::ensure_atoms(int n) {
 __global__ __launch_bounds__(1024) void k_scan(int n, const int* __restrict__ cnt, int* __restrict__ off,
                                                int64_t* __restrict__ total) {
   __shared__ long part[1024];
+  __shared__ int mx[1024];
   const int t = threadIdx.x;
   const int chunk = (n + 1023) / 1024;
   const int b = t * chunk, e = min(n, b + chunk);
   long s = 0;
-  for (int i = b; i < e; ++i) s += cnt[i];
+  int m = 0;
+  for (int i = b; i < e; ++i) {
+    s += cnt[i];
+    m = max(m, cnt[i]);
+  }
   part[t] = s;
+  mx[t] = m;
   __syncthreads();
   for (int w = 1; w < 1024; w <<= 1) {
     const long x = t >= w ? part[t - w] : 0;
+    const int y = t >= w ? mx[t - w] : 0;
     __syncthreads();
     part[t] += x;
+    mx[t] = max(mx[t], y);
     __syncthreads();
   }
   long run = t ? part[t - 1] : 0;
@@ -113,7 +121,8 @@ __global__ __launch_bounds__(1024) void k_scan(int n, const int* __restrict__ cn
   }
   if (t == 1023) {
     off[n] = (int)part[1023];
-    *total = part[1023];
+    total[0] = part[1023];
+    total[1] = mx[1023];
   }
 }
 
@@ -122,12 +131,13 @@ void device_scan(const int* cnt, int* off, int n, int64_t* total, hipStream_t st
   RT_CHECK(hipGetLastError());
 }
 
-long exclusive_scan(Workspace& ws, int n, hipStream_t st) {
-  ws.scan_tmp.ensure(2 * sizeof(int64_t));
+long exclusive_scan(Workspace& ws, int n, hipStream_t st, long* maxcnt) {
+  ws.scan_tmp.ensure(4 * sizeof(int64_t));
   device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), n, ws.scan_tmp.get<int64_t>(), st);
-  RT_CHECK(hipMemcpyAsync(ws.host_total, ws.scan_tmp.get<int64_t>(), sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  RT_CHECK(hipMemcpyAsync(ws.host_total, ws.scan_tmp.get<int64_t>(), 2 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
   RT_CHECK(hipStreamSynchronize(st));
-  return (long)*ws.host_total;
+  if (maxcnt) *maxcnt = (long)ws.host_total[1];
+  return (long)ws.host_total[0];
 }
 
 namespace {

@@ -66,7 +66,13 @@ struct Workspace {
   DevBuf nbr_cnt, nbr_off, scan_tmp, rec, h, abar, gbar, vg, site, fw;
   // per-edge backward buffers: dE/dd of each directed edge, gate-adjoint
   // messages, and the sorted incoming-edge lists
-  DevBuf gedge, uedge, in_edges, in_cur;
+  // gedge: dE/dd per edge in the centre's order; gedge_in, uedge: the same g
+  // and the gate messages at the edge's slot in its target's incoming list
+  // (in_pos), so that gathers read contiguous ranges
+  DevBuf gedge, gedge_in, uedge, in_edges, in_pos, in_cur;
+  // fixed-capacity per-atom neighbour slots of the single-pass list build
+  DevBuf nbr_slots;
+  int nbr_cap = 128;
   // cell-list neighbour build
   DevBuf grid, cell_of, wrap, cell_cnt, cell_start, cell_cur, cell_atoms;
   // parameter gradient: gate tangent h_dot, gate node tangents, type-free gate
@@ -82,10 +88,11 @@ struct Workspace {
   void ensure_atoms(int n);
 };
 
-// exclusive scan cnt[0..n) -> off[0..n] on the device (total also in *total)
+// exclusive scan cnt[0..n) -> off[0..n] on the device (total[0] = sum, total[1] = max)
 void device_scan(const int* cnt, int* off, int n, int64_t* total, hipStream_t st);
 // exclusive scan of ws.nbr_cnt[0..n) into ws.nbr_off[0..n]; returns the total
-long exclusive_scan(Workspace& ws, int n, hipStream_t st);
+// (and the largest count in *maxcnt)
+long exclusive_scan(Workspace& ws, int n, hipStream_t st, long* maxcnt = nullptr);
 
 // One precision's device view of a model.
 template <class T>
