@@ -148,6 +148,187 @@ __global__ void k_grid_setup(int nframes, int natoms, const int64_t* __restrict_
   grid[f] = g;
 }
 
+// One workgroup per frame (frames of at most FRAME_CELLS_MAX atoms): the grid
+// of k_grid_setup (open-direction extents by a block reduction), the bin of
+// every atom, bin counts and starts (LDS), the bin lists in ascending atom
+// order, and the zeroed incoming-list cursors -- the whole cell list in one
+// launch.  Bin ids of frame f are frame_off[f] + local bin, as in the global
+// path; ids past the frame's bins start at the next frame's first atom.
+constexpr int FRAME_CELLS_MAX = 8192;
+constexpr int FRAME_CELLS_THREADS = 512;
+
+__global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
+    int nframes, const int64_t* __restrict__ frame_off, const double* __restrict__ pos,
+    const double* __restrict__ cell, const int32_t* __restrict__ periodic, double rc, Grid* __restrict__ grid,
+    int* __restrict__ cell_of, int* __restrict__ wrap, int* __restrict__ cell_start, int* __restrict__ cell_atoms,
+    int* __restrict__ in_cur) {
+  extern __shared__ int lds_cells[];
+  int* bin = lds_cells;                      // [na] local bin of each atom
+  int* cnt = bin + FRAME_CELLS_MAX;          // [nbins + 1] counts, then exclusive starts
+  __shared__ double red_lo[FRAME_CELLS_THREADS], red_hi[FRAME_CELLS_THREADS];
+  __shared__ Grid g_sh;
+  const int f = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
+  const long b0 = frame_off[f];
+  const int na = (int)(frame_off[f + 1] - b0);
+  if (t == 0) {
+    Grid g;
+    const double* c = cell + 9 * f;
+    g.per = 0;
+    for (int q = 0; q < 3; ++q)
+      if (periodic[f] && (c[3 * q] != 0.0 || c[3 * q + 1] != 0.0 || c[3 * q + 2] != 0.0)) g.per |= 1 << q;
+    g_sh = g;
+  }
+  __syncthreads();
+  // basis rows: lattice rows for periodic directions, unit normals for open ones
+  double a[9];
+  {
+    const double* c = cell + 9 * f;
+    const int per = g_sh.per;
+    for (int q = 0; q < 3; ++q)
+      for (int k = 0; k < 3; ++k) a[3 * q + k] = (per & (1 << q)) ? c[3 * q + k] : 0.0;
+    const int nact = (per & 1) + ((per >> 1) & 1) + ((per >> 2) & 1);
+    auto cross = [](const double* u, const double* v, double* w) {
+      w[0] = u[1] * v[2] - u[2] * v[1]; w[1] = u[2] * v[0] - u[0] * v[2]; w[2] = u[0] * v[1] - u[1] * v[0];
+    };
+    auto unit = [](double* w) {
+      const double s = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+      for (int k = 0; k < 3; ++k) w[k] /= s;
+    };
+    if (nact == 0) {
+      a[0] = a[4] = a[8] = 1.0;
+    } else if (nact == 1) {
+      const int p = (per & 1) ? 0 : (per & 2) ? 1 : 2;
+      const double* v = a + 3 * p;
+      double e[3] = {0, 0, 0};
+      const int m = fabs(v[0]) <= fabs(v[1]) && fabs(v[0]) <= fabs(v[2]) ? 0 : fabs(v[1]) <= fabs(v[2]) ? 1 : 2;
+      e[m] = 1;
+      const int q1 = (p + 1) % 3, q2 = (p + 2) % 3;
+      cross(v, e, a + 3 * q1); unit(a + 3 * q1);
+      cross(v, a + 3 * q1, a + 3 * q2); unit(a + 3 * q2);
+    } else if (nact == 2) {
+      const int q = !(per & 1) ? 0 : !(per & 2) ? 1 : 2;
+      cross(a + 3 * ((q + 1) % 3), a + 3 * ((q + 2) % 3), a + 3 * q); unit(a + 3 * q);
+    }
+  }
+  // open directions: extent of the atoms along each normal (block min / max)
+  double org[3] = {0, 0, 0};
+  for (int q = 0; q < 3; ++q) {
+    if (g_sh.per & (1 << q)) continue;
+    double lo = 1e300, hi = -1e300;
+    for (int k = t; k < na; k += nt) {
+      const double* x = pos + 3 * (b0 + k);
+      const double v = x[0] * a[3 * q] + x[1] * a[3 * q + 1] + x[2] * a[3 * q + 2];
+      lo = fmin(lo, v);
+      hi = fmax(hi, v);
+    }
+    red_lo[t] = lo;
+    red_hi[t] = hi;
+    __syncthreads();
+    for (int w = nt / 2; w > 0; w >>= 1) {
+      if (t < w) {
+        red_lo[t] = fmin(red_lo[t], red_lo[t + w]);
+        red_hi[t] = fmax(red_hi[t], red_hi[t + w]);
+      }
+      __syncthreads();
+    }
+    lo = na > 0 ? red_lo[0] : 0.0;
+    hi = na > 0 ? red_hi[0] : 0.0;
+    __syncthreads();
+    const double ext = fmax(hi - lo, 1e-6) * (1 + 1e-9) + 1e-9;
+    for (int k = 0; k < 3; ++k) {
+      org[k] += lo * a[3 * q + k];
+      a[3 * q + k] *= ext;
+    }
+  }
+  if (t == 0) {
+    Grid g = g_sh;
+    double b[3][3];
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q)
+        b[q][p] = a[((p + 1) % 3) * 3 + (q + 1) % 3] * a[((p + 2) % 3) * 3 + (q + 2) % 3] -
+                  a[((p + 1) % 3) * 3 + (q + 2) % 3] * a[((p + 2) % 3) * 3 + (q + 1) % 3];
+    const double det = a[0] * b[0][0] + a[1] * b[1][0] + a[2] * b[2][0];
+    double reach[3];
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q) g.inv[3 * p + q] = b[p][q] / det;
+    for (int q = 0; q < 3; ++q) {
+      g.org[q] = org[q];
+      reach[q] = rc * sqrt(g.inv[q] * g.inv[q] + g.inv[3 + q] * g.inv[3 + q] + g.inv[6 + q] * g.inv[6 + q]);
+      g.n[q] = (int)fmin(1024.0, fmax(1.0, floor(1.0 / reach[q])));
+    }
+    while ((long)g.n[0] * g.n[1] * g.n[2] > (na > 0 ? na : 1)) {
+      int q = 0;
+      if (g.n[1] > g.n[q]) q = 1;
+      if (g.n[2] > g.n[q]) q = 2;
+      g.n[q] = max(1, g.n[q] - 1);
+    }
+    for (int q = 0; q < 3; ++q) g.S[q] = (g.per & (1 << q)) ? (int)floor(reach[q] * g.n[q]) + 1 : 1;
+    g_sh = g;
+    grid[f] = g;
+  }
+  __syncthreads();
+  const Grid g = g_sh;
+  const int nbins = g.n[0] * g.n[1] * g.n[2];
+  int* cur = cnt + FRAME_CELLS_MAX + 1;  // [nbins] placement cursors
+  int* tmp = cur + FRAME_CELLS_MAX;      // [na] atoms placed per bin (unordered)
+  for (int k = t; k <= nbins; k += nt) cnt[k] = 0;
+  __syncthreads();
+  for (int k = t; k < na; k += nt) {
+    double fr[3];
+    const double* x = pos + 3 * (b0 + k);
+    for (int q = 0; q < 3; ++q)
+      fr[q] = (x[0] - g.org[0]) * g.inv[q] + (x[1] - g.org[1]) * g.inv[3 + q] + (x[2] - g.org[2]) * g.inv[6 + q];
+    int c[3];
+    for (int q = 0; q < 3; ++q) {
+      const double w = (g.per & (1 << q)) ? floor(fr[q]) : 0.0;
+      wrap[3 * (b0 + k) + q] = (int)w;
+      c[q] = min(g.n[q] - 1, max(0, (int)((fr[q] - w) * g.n[q])));
+    }
+    const int id = c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
+    bin[k] = id;
+    cell_of[b0 + k] = (int)b0 + id;
+    in_cur[b0 + k] = 0;
+    atomicAdd(cnt + id, 1);
+  }
+  __syncthreads();
+  {  // exclusive scan of the bin counts: chunk sums per thread, then a block scan of the sums
+    const int chunk = (nbins + nt - 1) / nt;
+    const int c0 = t * chunk, c1 = min(nbins, c0 + chunk);
+    long s = 0;
+    for (int k = c0; k < c1; ++k) s += cnt[k];
+    red_lo[t] = (double)s;
+    __syncthreads();
+    for (int w = 1; w < nt; w <<= 1) {
+      const double x = t >= w ? red_lo[t - w] : 0.0;
+      __syncthreads();
+      red_lo[t] += x;
+      __syncthreads();
+    }
+    int run = t ? (int)red_lo[t - 1] : 0;
+    for (int k = c0; k < c1; ++k) {
+      const int v = cnt[k];
+      cnt[k] = run;
+      cur[k] = run;
+      run += v;
+    }
+    if (t == nt - 1) cnt[nbins] = (int)red_lo[nt - 1];
+  }
+  __syncthreads();
+  for (int k = t; k <= na; k += nt) cell_start[b0 + k] = (int)b0 + (k < nbins ? cnt[k] : na);
+  for (int k = t; k < na; k += nt) tmp[atomicAdd(cur + bin[k], 1)] = k;
+  __syncthreads();
+  // ascending atom order inside each bin: rank among the bin's members
+  for (int p = t; p < na; p += nt) {
+    const int k = tmp[p], id = bin[k];
+    const int s0 = cnt[id], s1 = cnt[id + 1];
+    int r = 0;
+    for (int q = s0; q < s1; ++q) r += tmp[q] < k;
+    cell_atoms[b0 + s0 + r] = (int)b0 + k;
+  }
+}
+
+constexpr size_t frame_cells_lds() { return sizeof(int) * (3 * (size_t)FRAME_CELLS_MAX + 1 + FRAME_CELLS_MAX); }
+
 __device__ __forceinline__ void frac_of(const Grid& g, const double* x, double* f) {
   for (int q = 0; q < 3; ++q)
     f[q] = (x[0] - g.org[0]) * g.inv[q] + (x[1] - g.org[1]) * g.inv[3 + q] + (x[2] - g.org[2]) * g.inv[6 + q];
@@ -371,14 +552,20 @@ template <class T>
 __global__ __launch_bounds__(256) void k_finalize(int nframes, const int64_t* __restrict__ frame_off,
                                                   const int32_t* __restrict__ types, SiteConst sc,
                                                   const T* __restrict__ site, const T* __restrict__ fw,
-                                                  double* __restrict__ energy, double* __restrict__ virial) {
+                                                  double* __restrict__ energy, double* __restrict__ virial,
+                                                  double* __restrict__ force, double* __restrict__ site_out) {
   __shared__ double red[7][256];
   const int f = blockIdx.x, t = threadIdx.x;
   const long b = frame_off[f], e = frame_off[f + 1];
   double s[7] = {0, 0, 0, 0, 0, 0, 0};
   for (long i = b + t; i < e; i += 256) {
-    s[0] += (double)site[i] + sc(types[i]);
+    const double se = (double)site[i] + sc(types[i]);
+    s[0] += se;
     for (int q = 0; q < 6; ++q) s[1 + q] += (double)fw[i * 9 + 3 + q];
+    force[3 * i] = (double)fw[9 * i];
+    force[3 * i + 1] = (double)fw[9 * i + 1];
+    force[3 * i + 2] = (double)fw[9 * i + 2];
+    if (site_out) site_out[i] = se;
   }
   for (int q = 0; q < 7; ++q) red[q][t] = s[q];
   __syncthreads();
@@ -393,16 +580,14 @@ __global__ __launch_bounds__(256) void k_finalize(int nframes, const int64_t* __
   }
 }
 
+// Forces of atoms [first, natoms) outside every frame (the ghosts of a pair-style call).
 template <class T>
-__global__ void k_atom_out(int natoms, int nsite, const int32_t* __restrict__ types, SiteConst sc,
-                           const T* __restrict__ site, const T* __restrict__ fw, double* __restrict__ force,
-                           double* __restrict__ site_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_atom_out(int first, int natoms, const T* __restrict__ fw, double* __restrict__ force) {
+  const int i = first + blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= natoms) return;
   force[3 * i] = (double)fw[9 * (size_t)i];
   force[3 * i + 1] = (double)fw[9 * (size_t)i + 1];
   force[3 * i + 2] = (double)fw[9 * (size_t)i + 2];
-  if (site_out && i < nsite) site_out[i] = (double)site[i] + sc(types[i]);
 }
 
 template <class T>
@@ -439,10 +624,35 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   ws.nbr_slots.ensure(sizeof(Rec<T>) * n1 * cap);
   const int nb = (N + 255) / 256;
   const std::string tn = std::string("<") + tname<T>() + ">";
-  SUS2_LAUNCH(ws, st, "grid_setup", k_grid_setup, dim3((std::max(b.nframes, N) + 63) / 64), dim3(64), 0, b.nframes,
-              N, b.frame_off, b.pos, b.cell, b.periodic, rc, ws.grid.get<Grid>(), ws.cell_cnt.get<int>(),
-              ws.cell_cur.get<int>(), ws.in_cur.get<int>());
-  if (N > 0) {
+  // frames of at most FRAME_CELLS_MAX atoms build their cell lists in one
+  // workgroup each; otherwise the multi-kernel global path
+  bool per_frame = N <= FRAME_CELLS_MAX;
+  if (!per_frame && b.nframes > 0) {
+    ws.host_frame_off.resize(b.nframes + 1);
+    SUS2_CHECK(hipMemcpyAsync(ws.host_frame_off.data(), b.frame_off, sizeof(int64_t) * (b.nframes + 1),
+                              hipMemcpyDeviceToHost, st));
+    SUS2_CHECK(hipStreamSynchronize(st));
+    int64_t mx = 0;
+    for (int f = 0; f < b.nframes; ++f) mx = std::max(mx, ws.host_frame_off[f + 1] - ws.host_frame_off[f]);
+    per_frame = mx <= FRAME_CELLS_MAX;
+  }
+  if (per_frame) {
+    if (b.nframes > 0) {
+      static bool lds_set = false;
+      if (!lds_set) {
+        SUS2_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(k_frame_cells),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)frame_cells_lds()));
+        lds_set = true;
+      }
+      SUS2_LAUNCH(ws, st, "frame_cells", k_frame_cells, dim3(b.nframes), dim3(FRAME_CELLS_THREADS),
+                  frame_cells_lds(), b.nframes, b.frame_off, b.pos, b.cell, b.periodic, rc, ws.grid.get<Grid>(),
+                  ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(), ws.cell_atoms.get<int>(),
+                  ws.in_cur.get<int>());
+    }
+  } else {
+    SUS2_LAUNCH(ws, st, "grid_setup", k_grid_setup, dim3((std::max(b.nframes, N) + 63) / 64), dim3(64), 0,
+                b.nframes, N, b.frame_off, b.pos, b.cell, b.periodic, rc, ws.grid.get<Grid>(), ws.cell_cnt.get<int>(),
+                ws.cell_cur.get<int>(), ws.in_cur.get<int>());
     SUS2_LAUNCH(ws, st, "bin_atoms", k_bin, dim3(nb), dim3(256), 0, N, b.nframes, b.frame_off, b.pos,
                 ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_cnt.get<int>());
     device_scan(ws.cell_cnt.get<int>(), ws.cell_start.get<int>(), N, ws.scan_tmp.get<int64_t>() + 2, st);
@@ -450,6 +660,8 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
                 ws.cell_start.get<int>(), ws.cell_cur.get<int>(), ws.cell_atoms.get<int>());
     SUS2_LAUNCH(ws, st, "sort_bins", k_sort_lists, dim3(N), dim3(WAVE), 0, N, ws.cell_start.get<int>(),
                 ws.cell_atoms.get<int>(), (int*)nullptr);
+  }
+  if (N > 0) {
     SUS2_LAUNCH(ws, st, "neighbors" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, b.nframes, b.frame_off, b.pos,
                 b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(),
                 ws.cell_atoms.get<int>(), rc, (const int*)nullptr, ws.nbr_cnt.get<int>(),
@@ -485,11 +697,13 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
 template <class T>
 void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, const SiteConst& sc, hipStream_t st, int nsite) {
   const int N = b.natoms;
-  if (N > 0)
-    SUS2_LAUNCH(ws, st, "atom_out", k_atom_out<T>, dim3((N + 255) / 256), dim3(256), 0, N, nsite < 0 ? N : nsite,
-                b.types, sc, ws.site.get<T>(), ws.fw.get<T>(), out.force, out.site);
-  SUS2_LAUNCH(ws, st, "finalize", k_finalize<T>, dim3(b.nframes), dim3(256), 0, b.nframes, b.frame_off, b.types, sc,
-              ws.site.get<T>(), ws.fw.get<T>(), out.energy, out.virial);
+  const int first = nsite < 0 ? N : nsite;  // atoms past the frames (pair-style ghosts)
+  if (b.nframes > 0)
+    SUS2_LAUNCH(ws, st, "finalize", k_finalize<T>, dim3(b.nframes), dim3(256), 0, b.nframes, b.frame_off, b.types, sc,
+                ws.site.get<T>(), ws.fw.get<T>(), out.energy, out.virial, out.force, out.site);
+  if (first < N)
+    SUS2_LAUNCH(ws, st, "atom_out", k_atom_out<T>, dim3((N - first + 255) / 256), dim3(256), 0, first, N,
+                ws.fw.get<T>(), out.force);
 }
 
 template void build_neighbors<float>(Workspace&, const Batch&, double, hipStream_t);

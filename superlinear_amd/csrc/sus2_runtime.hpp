@@ -73,6 +73,7 @@ struct Workspace {
   // fixed-capacity per-atom neighbour slots of the single-pass list build
   DevBuf nbr_slots;
   int nbr_cap = 128;
+  std::vector<int64_t> host_frame_off;  // host copy of a batch's frame offsets (frame sizes)
   // cell-list neighbour build
   DevBuf grid, cell_of, wrap, cell_cnt, cell_start, cell_cur, cell_atoms;
   // parameter gradient: gate tangent h_dot, gate node tangents, type-free gate
