@@ -37,6 +37,38 @@ __host__ __device__ __forceinline__ int l_of(int h) {
   return l;
 }
 
+// ------------------------------------------------------------------ staged sums
+// sum_{e < cnt} a[e] b[e] over staged LDS rows (four independent partial sums
+// in a fixed pattern, so that the row loads overlap).
+template <class T>
+__device__ __forceinline__ T lds_dot(const T* __restrict__ a, const T* __restrict__ b, int cnt) {
+  T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+  int e = 0;
+  for (; e + 4 <= cnt; e += 4) {
+    s0 += a[e] * b[e];
+    s1 += a[e + 1] * b[e + 1];
+    s2 += a[e + 2] * b[e + 2];
+    s3 += a[e + 3] * b[e + 3];
+  }
+  for (; e < cnt; ++e) s0 += a[e] * b[e];
+  return (s0 + s1) + (s2 + s3);
+}
+
+// sum_{e < cnt, key[e] == t} v[e] in FP64 over staged LDS rows.
+template <class T>
+__device__ __forceinline__ double lds_sum_where(const T* __restrict__ v, const T* __restrict__ key, T t, int cnt) {
+  double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  int e = 0;
+  for (; e + 4 <= cnt; e += 4) {
+    s0 += key[e] == t ? (double)v[e] : 0.0;
+    s1 += key[e + 1] == t ? (double)v[e + 1] : 0.0;
+    s2 += key[e + 2] == t ? (double)v[e + 2] : 0.0;
+    s3 += key[e + 3] == t ? (double)v[e + 3] : 0.0;
+  }
+  for (; e < cnt; ++e) s0 += key[e] == t ? (double)v[e] : 0.0;
+  return (s0 + s1) + (s2 + s3);
+}
+
 // ------------------------------------------------------------------ wave graph walks
 // Basic nodes take the moments; rank nodes the K -> rank contraction
 // (sus2_sl_kernel.cpp:104-105, 217-221).  v: LDS [M].
@@ -199,9 +231,7 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
         const int hh = c / KK, k = c % KK;
         const T* pr = sR + (k * L1 + l_of(hh)) * LR;
         const T* py = sY + hh * LR;
-        T s = acc[t];
-        for (int ee = 0; ee < cnt; ++ee) s += pr[ee] * py[ee];
-        acc[t] = s;
+        acc[t] += lds_dot(pr, py, cnt);
       }
     }
     __syncthreads();

@@ -54,13 +54,22 @@ __device__ __forceinline__ void cheb3(T S, T xi, T* Tb, T* T1, T* T2) {
 // gate-weight block W (reduced separately by k_grad_W).
 __host__ __device__ __forceinline__ int pidx(int p, int oW, int nW) { return (oW >= 0 && p >= oW) ? p - nW : p; }
 
-// LDS layout shared by host (sizes) and device (carving).
+// LDS layout shared by host (sizes) and device (carving): the species tables,
+// the canonical adjoints Pc, Qc, the type-free moments At, Atd and the gate
+// adjoint row, then one region used in turn by the edge staging of sweep 1,
+// the graph phase (A, Ad, v, vd, P, Q, scratch) and the staging of sweep 2
+// (their lifetimes do not overlap), then the FP64 partial vector.
 template <class Cf>
 struct GradLds {
   int stage_rows, M, sc, tabs, pn;
-  __host__ __device__ size_t floats() const {
-    return (size_t)stage_rows * LR + 6 * (size_t)Cf::KH + 4 * (size_t)M + 2 * (size_t)sc + tabs + 2 * Cf::F + 2;
+  __host__ __device__ size_t graph_floats() const {
+    return 2 * (size_t)Cf::KH + 4 * (size_t)M + 2 * (size_t)sc;
   }
+  __host__ __device__ size_t region() const {
+    const size_t st = (size_t)stage_rows * LR, gr = graph_floats();
+    return st > gr ? st : gr;
+  }
+  __host__ __device__ size_t floats() const { return (size_t)tabs + 4 * (size_t)Cf::KH + 2 * Cf::F + 2 + region(); }
   __host__ __device__ size_t bytes() const {
     // T area first, then the FP64 partial vector (8-byte aligned)
     const size_t t = floats() * sizeof(typename Cf::T);
@@ -98,21 +107,22 @@ __device__ GradSmem<typename Cf::T> carve(const GradLds<Cf>& g, unsigned char* r
   using T = typename Cf::T;
   GradSmem<T> s;
   T* p = reinterpret_cast<T*>(raw);
-  s.stage = p; p += (size_t)g.stage_rows * LR;
-  s.At = p; p += Cf::KH;
-  s.Atd = p; p += Cf::KH;
-  s.A = p; p += Cf::KH;
-  s.Ad = p; p += Cf::KH;
+  s.tabs = p; p += g.tabs;
   s.Pc = p; p += Cf::KH;
   s.Qc = p; p += Cf::KH;
-  s.v = p; p += g.M;
-  s.vd = p; p += g.M;
-  s.P = p; p += g.M;
-  s.Q = p; p += g.M;
-  s.s1 = p; p += g.sc;
-  s.s2 = p; p += g.sc;
-  s.tabs = p; p += g.tabs;
+  s.At = p; p += Cf::KH;
+  s.Atd = p; p += Cf::KH;
   s.hbs = p; p += 2 * Cf::F + 2;
+  T* region = p;
+  s.stage = region;  // edge staging (sweeps 1 and 2)
+  s.A = region;      // graph phase
+  s.Ad = s.A + Cf::KH;
+  s.v = s.Ad + Cf::KH;
+  s.vd = s.v + g.M;
+  s.P = s.vd + g.M;
+  s.Q = s.P + g.M;
+  s.s1 = s.Q + g.M;
+  s.s2 = s.s1 + g.sc;
   const size_t t = g.floats() * sizeof(T);
   s.part = reinterpret_cast<double*>(raw + ((t + 7) & ~(size_t)7));
   return s;
@@ -251,13 +261,8 @@ __device__ void grad_sweep_moments(const DModel<typename Cf::T>& m, const Specie
         const T* p2 = sS2 + mu * LR;
         const T* py = sY + hh * LR;
         const T* pw = sYw + hh * LR;
-        T s = acc[t], sd = accd[t];
-        for (int ee = 0; ee < cnt; ++ee) {
-          s += p1[ee] * py[ee];
-          sd += p2[ee] * py[ee] + p1[ee] * pw[ee];
-        }
-        acc[t] = s;
-        accd[t] = sd;
+        acc[t] += lds_dot(p1, py, cnt);
+        accd[t] += lds_dot(p2, py, cnt) + lds_dot(p1, pw, cnt);
       }
     }
     __syncthreads();
@@ -322,83 +327,105 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
       const T wy = T(fwi[1] - fj[1]) + Sw[3] * dx + Sw[4] * dy + Sw[5] * dz;
       const T wz = T(fwi[2] - fj[2]) + Sw[6] * dx + Sw[7] * dy + Sw[8] * dz;
       const T wr = ux * wx + uy * wy + uz * wz;
-      T pp[F], qv[F], qw[F];
-#pragma unroll
-      for (int mu = 0; mu < F; ++mu) { pp[mu] = T(0); qv[mu] = T(0); qw[mu] = T(0); }
-      sh_stream_grad<T, LL>(ux, uy, uz, inv, [&](int l, int mm, T Y, T yx, T yy, T yz) {
-        const int hh = hidx(l, mm);
-        const T Yw = yx * wx + yy * wy + yz * wz;
-#pragma unroll
-        for (int k = 0; k < KK; ++k) {
-          const int mu = k * L1 + l;
-          const T pc = s.Pc[hh * KK + k], qc = s.Qc[hh * KK + k];
-          pp[mu] += pc * Y;
-          qv[mu] += qc * Y;
-          qw[mu] += qc * Yw;
-        }
-      });
       const T d0 = r - m.rc, f0 = d0 * d0, g0 = T(2) * d0;
       const bool act = r < m.rca;
       const T d1 = r - m.rca, f1 = act ? d1 * d1 : T(0), g1 = act ? T(2) * d1 : T(0);
       const int pair = ti * C + tj;
       const T* x0j = tab.x0 + tj * F;
       const T* alj = tab.alpha + tj * F;
+      // per-channel work of one radial function mu = (k, l): p, q, q' from the
+      // harmonics, the radial values and their coefficient / warp gradients
+      auto channel = [&](int mu, int k, int l, const T* Yv, const T* Ywv, const T* Tb, const T* T1, const T* T2,
+                         T xr, T xs, T xt, T xrs, T xrt, T& ds, T& dt) {
+        T pp = T(0), qv = T(0), qw = T(0);
 #pragma unroll
-      for (int b = 0; b < NBLK; ++b) {
-        const T sl = tab.slope[b * CC + pair], sh = tab.shift[b * CC + pair];
-        const T xi = dtanh(sl * (r - sh) * T(0.5));
-        const T xx = T(1) - xi * xi;
-        const T xr = xx * sl * T(0.5), xs = xx * (r - sh) * T(0.5), xt = -xr;
-        const T xrs = T(0.5) * xx - sl * xi * xs, xrt = -sl * xi * xt;
-        T Tb[BB], T1[BB], T2[BB];
-        cheb3<BB>(m.scaling, xi, Tb, T1, T2);
-        T ds = T(0), dt = T(0);
-#pragma unroll
-        for (int k = 0; k < KK; ++k) {
-#pragma unroll
-          for (int l = 0; l < L1; ++l) {
-            if (Cf::blk(k, l) != b) continue;
-            const int mu = k * L1 + l;
-            const T* c = coef + mu * stride;
-            T C0 = T(0), C1 = T(0), C2 = T(0);
-#pragma unroll
-            for (int qq = 0; qq < BB; ++qq) {
-              C0 += c[qq] * Tb[qq];
-              C1 += c[qq] * T1[qq];
-              C2 += c[qq] * T2[qq];
-            }
-            const T f = l == 0 ? f0 : f1, g = l == 0 ? g0 : g1;
-            const T R = f * C0, Rp = g * C0 + f * C1 * xr;
-            T x1 = x0j[mu], xd = T(0), hj = T(0), hdj = T(0);
-            if (hg) {
-              hj = h[(size_t)j * F + mu];
-              hdj = hd[(size_t)j * F + mu];
-              x1 += alj[mu] * hj;
-              xd = alj[mu] * hdj;
-            }
-            const T pq = pp[mu] + qw[mu];
-            const T Z = R * pq + Rp * wr * qv[mu];
-            const T Rq = R * qv[mu];
-            const T Gv = x0[mu] * (x1 * pq + xd * qv[mu]);
-            const T Gd = x0[mu] * x1 * wr * qv[mu];
-            const T Cv = Gv * f + Gd * g, Cd = Gd * f;
-            ds += xs * C1 * Cv + Cd * (C2 * xs * xr + C1 * xrs);
-            dt += xt * C1 * Cv + Cd * (C2 * xt * xr + C1 * xrt);
-            sCv[mu * LR + lane] = Cv;
-            sCd[mu * LR + lane] = Cd;
-            sX1[mu * LR + lane] = x0[mu] * Z;
-            if (hg) {
-              sAl[mu * LR + lane] = x0[mu] * (Z * hj + Rq * hdj);
-              T* mo = msg + (size_t)in_pos[e] * 2 * F;
-              mo[mu] = alj[mu] * x0[mu] * Z;
-              mo[F + mu] = alj[mu] * x0[mu] * Rq;
-            }
-          }
+        for (int mm = -l; mm <= l; ++mm) {
+          const int hh = hidx(l, mm);
+          const T pc = s.Pc[hh * KK + k], qc = s.Qc[hh * KK + k];
+          pp += pc * Yv[hh];
+          qv += qc * Yv[hh];
+          qw += qc * Ywv[hh];
         }
-        sXi[b * LR + lane] = xi;
-        sXr[b * LR + lane] = xr;
-        sDs[b * LR + lane] = ds;
-        sDt[b * LR + lane] = dt;
+        const T* c = coef + mu * stride;
+        T C0 = T(0), C1 = T(0), C2 = T(0);
+#pragma unroll
+        for (int qq = 0; qq < BB; ++qq) {
+          C0 += c[qq] * Tb[qq];
+          C1 += c[qq] * T1[qq];
+          C2 += c[qq] * T2[qq];
+        }
+        const T f = l == 0 ? f0 : f1, g = l == 0 ? g0 : g1;
+        const T R = f * C0, Rp = g * C0 + f * C1 * xr;
+        T x1 = x0j[mu], xd = T(0), hj = T(0), hdj = T(0);
+        if (hg) {
+          hj = h[(size_t)j * F + mu];
+          hdj = hd[(size_t)j * F + mu];
+          x1 += alj[mu] * hj;
+          xd = alj[mu] * hdj;
+        }
+        const T pq = pp + qw;
+        const T Z = R * pq + Rp * wr * qv;
+        const T Rq = R * qv;
+        const T Gv = x0[mu] * (x1 * pq + xd * qv);
+        const T Gd = x0[mu] * x1 * wr * qv;
+        const T Cv = Gv * f + Gd * g, Cd = Gd * f;
+        ds += xs * C1 * Cv + Cd * (C2 * xs * xr + C1 * xrs);
+        dt += xt * C1 * Cv + Cd * (C2 * xt * xr + C1 * xrt);
+        sCv[mu * LR + lane] = Cv;
+        sCd[mu * LR + lane] = Cd;
+        sX1[mu * LR + lane] = x0[mu] * Z;
+        if (hg) {
+          sAl[mu * LR + lane] = x0[mu] * (Z * hj + Rq * hdj);
+          T* mo = msg + (size_t)in_pos[e] * 2 * F;
+          mo[mu] = alj[mu] * x0[mu] * Z;
+          mo[F + mu] = alj[mu] * x0[mu] * Rq;
+        }
+      };
+      auto block_basis = [&](int b, T* Tb, T* T1, T* T2, T& xi, T& xr, T& xs, T& xt, T& xrs, T& xrt) {
+        const T sl = tab.slope[b * CC + pair], sh = tab.shift[b * CC + pair];
+        xi = dtanh(sl * (r - sh) * T(0.5));
+        const T xx = T(1) - xi * xi;
+        xr = xx * sl * T(0.5); xs = xx * (r - sh) * T(0.5); xt = -xr;
+        xrs = T(0.5) * xx - sl * xi * xs; xrt = -sl * xi * xt;
+        cheb3<BB>(m.scaling, xi, Tb, T1, T2);
+      };
+      T Yv[Cf::H], Ywv[Cf::H];
+      sh_stream_grad<T, LL>(ux, uy, uz, inv, [&](int l, int mm, T Y, T yx, T yy, T yz) {
+        const int hh = hidx(l, mm);
+        Yv[hh] = Y;
+        Ywv[hh] = yx * wx + yy * wy + yz * wz;
+      });
+      if constexpr (Cf::SM == 2) {
+        // blocks are the channels k: one block's basis live at a time
+#pragma unroll 1
+        for (int k = 0; k < KK; ++k) {
+          T Tb[BB], T1[BB], T2[BB], xi, xr, xs, xt, xrs, xrt;
+          block_basis(k, Tb, T1, T2, xi, xr, xs, xt, xrs, xrt);
+          T ds = T(0), dt = T(0);
+#pragma unroll
+          for (int l = 0; l < L1; ++l)
+            channel(k * L1 + l, k, l, Yv, Ywv, Tb, T1, T2, xr, xs, xt, xrs, xrt, ds, dt);
+          sXi[k * LR + lane] = xi;
+          sXr[k * LR + lane] = xr;
+          sDs[k * LR + lane] = ds;
+          sDt[k * LR + lane] = dt;
+        }
+      } else {
+#pragma unroll
+        for (int b = 0; b < NBLK; ++b) {
+          T Tb[BB], T1[BB], T2[BB], xi, xr, xs, xt, xrs, xrt;
+          block_basis(b, Tb, T1, T2, xi, xr, xs, xt, xrs, xrt);
+          T ds = T(0), dt = T(0);
+#pragma unroll
+          for (int k = 0; k < KK; ++k)
+#pragma unroll
+            for (int l = 0; l < L1; ++l)
+              if (Cf::blk(k, l) == b) channel(k * L1 + l, k, l, Yv, Ywv, Tb, T1, T2, xr, xs, xt, xrs, xrt, ds, dt);
+          sXi[b * LR + lane] = xi;
+          sXr[b * LR + lane] = xr;
+          sDs[b * LR + lane] = ds;
+          sDt[b * LR + lane] = dt;
+        }
       }
       sTj[lane] = T(tj);
     }
@@ -429,16 +456,14 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
         const int k2 = idx < n1 ? idx : idx - n1;
         const int t = k2 / F, mu = k2 % F;
         const T* src = (idx < n1 ? sX1 : sAl) + mu * LR;
-        for (int ee = 0; ee < cnt; ++ee)
-          if ((int)sTj[ee] == t) acc += (double)src[ee];
+        acc = lds_sum_where(src, sTj, T(t), cnt);
         if (idx < n1) p = GATE ? m.o_gtype + t * F + mu : m.o_rad + mu * (BB + C) + BB + t;
         else p = m.o_alpha + t * F + mu;
       } else {
         const int k3 = idx - n1 - n2;
         const int b = k3 / (2 * C), w = (k3 / C) % 2, t = k3 % C;
         const T* src = (w ? sDt : sDs) + b * LR;
-        for (int ee = 0; ee < cnt; ++ee)
-          if ((int)sTj[ee] == t) acc += (double)src[ee];
+        acc = lds_sum_where(src, sTj, T(t), cnt);
         p = so + (2 * b + w) * CC + ti * C + t;
       }
       s.part[pidx(p, pn_oW, pn_nW)] += acc;
@@ -785,6 +810,22 @@ static __global__ __launch_bounds__(RED_X * RED_Y) void k_grad_reduce(int ncoeff
   }
 }
 
+// Workgroups of one wavefront that fit on the whole device at once, rounded
+// down to a multiple of the four SIMDs of a CU: the grid-stride loops give
+// every group the same share of atoms, and a fifth wave on a CU would share a
+// SIMD with another and finish last.
+template <class K>
+int resident_groups(K kern, size_t lds) {
+  constexpr int SIMDS = 4;
+  allow_lds(kern, lds);
+  int per_cu = 0, dev = 0, cus = 0;
+  SUS2_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), WAVE, lds));
+  SUS2_CHECK(hipGetDevice(&dev));
+  SUS2_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (per_cu > SIMDS) per_cu -= per_cu % SIMDS;
+  return std::max(1, per_cu) * std::max(1, cus);
+}
+
 // ------------------------------------------------------------------ driver
 template <class Cf>
 void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const GradWeights& gw, double* grad,
@@ -797,7 +838,12 @@ void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const 
   const int oW = m.gated ? m.o_W : -1, nW = m.gated ? Cf::F * m.Qg : 0;
   const int pn = ncoeff - nW;
   const size_t n1 = std::max(1, N), ne = std::max<long>(1, ws.nedges);
-  const int G = std::max(1, std::min(N, ws.grad_groups));
+  // grid-stride groups: as many workgroups as can be resident at once (every
+  // group holds its own FP64 partial row, so more would only add rows)
+  const GradLds<Cf> lo_ = grad_lds<Cf>(m, false, true, pn), l3_ = grad_lds<Cf>(m, true, true, pn);
+  const int G = std::max(1, std::min(N, std::min(resident_groups(k_grad_outer<Cf>, lo_.bytes()),
+                                                 m.gated ? resident_groups(k_grad_gate_reverse<Cf>, l3_.bytes())
+                                                         : 1 << 30)));
   ws.gpart.ensure(sizeof(double) * (size_t)G * pn);
   const int* off = ws.nbr_off.get<int>();
   const Rec<T>* rec = ws.rec.get<Rec<T>>();

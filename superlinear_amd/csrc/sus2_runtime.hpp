@@ -82,7 +82,6 @@ struct Workspace {
   DevBuf hdot, vgd, gmom, gmomd, msg, hbar, gpart, wpart;
   // training objective: EFS outputs, loss adjoints and per-frame losses (FP64)
   DevBuf lE, lF, lV, lew, lfw, lsw, lloss;
-  int grad_groups = 1024;  // workgroups of the grid-stride gradient kernels
   long nedges = 0;
   int64_t* host_total = nullptr;  // pinned
   ~Workspace();
