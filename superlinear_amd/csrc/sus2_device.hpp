@@ -28,6 +28,7 @@ struct DModel {
   int C, L, H, K, F, B, M, Q, Kw, smode, R, O, Qg;
   T rc, rca, scaling;
   int gated, species_readout;
+  int scalar_unique;  // no node appears twice in scalar_map
   const T* theta;
   int o_shift, o_scal, o_rad, o_grad, o_gscal, o_alpha, o_gtype, o_W, o_gw, o_ow, o_species, o_read;
   const int* basic_node;  // [H*K], canonical moment (h, k) -> node or -1

@@ -266,8 +266,11 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
   if (lane == 0) site[i] = E;
   for (int n = lane; n < m.M; n += WAVE) a[n] = T(0);
   __syncthreads();
-  if (lane == 0)
+  if (m.scalar_unique) {  // distinct nodes: one lane per scalar
+    for (int q = lane; q < m.Q; q += WAVE) a[m.scalar_map[q]] += m.readout(ti, q);
+  } else if (lane == 0) {  // repeated nodes: in scalar order
     for (int q = 0; q < m.Q; ++q) a[m.scalar_map[q]] += m.readout(ti, q);
+  }
   __syncthreads();
   wave_graph_reverse(p, v, a, scratch);
   wave_adjoint_to_basic<T, KK>(m, p, false, a, abar + (size_t)i * KH);

@@ -584,10 +584,10 @@ __device__ __forceinline__ void zero_part(double* part, int pn) {
 }
 
 // Fold the per-lane radial-coefficient registers into the partial vector
-// (subsets one after another), then write (or add) the partial row.
+// (subsets one after another), then write the partial row.
 template <class Cf>
 __device__ void flush_partial(const DModel<typename Cf::T>& m, bool gate, const double* creg, double* part, int pn,
-                              int oW, int nW, double* __restrict__ out_row, bool add) {
+                              int oW, int nW, double* __restrict__ out_row) {
   constexpr int F = Cf::F, BB = Cf::BB, NSUB = WAVE / F;
   const int lane = threadIdx.x, cmu = lane % F, csub = lane / F;
   const bool own_rad = gate && m.o_grad >= 0;
@@ -598,7 +598,7 @@ __device__ void flush_partial(const DModel<typename Cf::T>& m, bool gate, const 
       for (int b = 0; b < BB; ++b) part[pidx(base + cmu * stride + b, oW, nW)] += creg[b];
     __syncthreads();
   }
-  for (int k = lane; k < pn; k += WAVE) out_row[k] = add ? out_row[k] + part[k] : part[k];
+  for (int k = lane; k < pn; k += WAVE) out_row[k] = part[k];
 }
 
 template <class Cf, class T = typename Cf::T>
@@ -652,11 +652,16 @@ __global__ __launch_bounds__(WAVE) void k_grad_outer(DModel<T> m, GradLds<Cf> L,
     if (lane == 0) {
       s.part[pidx(m.o_species + ti, oW, nW)] += Eb + gsp;
       s.part[pidx(m.o_shift + ti, oW, nW)] += Eb;
-      for (int q = 0; q < m.Q; ++q) {
-        const T rho = m.readout(ti, q);
-        s.P[m.scalar_map[q]] += T(Eb) * rho;
-        s.Q[m.scalar_map[q]] += rho;
-      }
+    }
+    auto seed = [&](int q) {
+      const T rho = m.readout(ti, q);
+      s.P[m.scalar_map[q]] += T(Eb) * rho;
+      s.Q[m.scalar_map[q]] += rho;
+    };
+    if (m.scalar_unique) {  // distinct nodes: one lane per scalar
+      for (int q = lane; q < m.Q; q += WAVE) seed(q);
+    } else if (lane == 0) {  // repeated nodes: in scalar order
+      for (int q = 0; q < m.Q; ++q) seed(q);
     }
     __syncthreads();
     wave_graph_reverse_tan(p, s.v, s.vd, s.P, s.Q, s.s1, s.s2);
@@ -669,14 +674,13 @@ __global__ __launch_bounds__(WAVE) void k_grad_outer(DModel<T> m, GradLds<Cf> L,
                                  nW);
   }
   __syncthreads();
-  flush_partial<Cf>(m, false, creg, s.part, L.pn, oW, nW, partial + (size_t)blockIdx.x * L.pn, false);
+  flush_partial<Cf>(m, false, creg, s.part, L.pn, oW, nW, partial + (size_t)blockIdx.x * L.pn);
 }
 
 template <class Cf, class T = typename Cf::T>
 __global__ __launch_bounds__(WAVE) void k_grad_gate_reverse(DModel<T> m, GradLds<Cf> L, GradWeights gw, int natoms,
                                                             const int32_t* __restrict__ types,
                                                             const int* __restrict__ nbr_off,
-                                                            const int* __restrict__ in_edges,
                                                             const Rec<T>* __restrict__ rec,
                                                             const T* __restrict__ vg, const T* __restrict__ vgd,
                                                             const T* __restrict__ gmom, const T* __restrict__ gmomd,
@@ -752,7 +756,7 @@ __global__ __launch_bounds__(WAVE) void k_grad_gate_reverse(DModel<T> m, GradLds
                                 e1, s, creg, oW, nW);
   }
   __syncthreads();
-  flush_partial<Cf>(m, true, creg, s.part, L.pn, oW, nW, partial + (size_t)blockIdx.x * L.pn, true);
+  flush_partial<Cf>(m, true, creg, s.part, L.pn, oW, nW, partial + (size_t)blockIdx.x * L.pn);
 }
 
 // W_bar[mu][q] over a chunk of atoms: thread = one (mu, q) entry, FP64 sums.
@@ -775,25 +779,28 @@ __global__ __launch_bounds__(256) void k_grad_W(int natoms, int F, int Qg, int c
   wpart[(size_t)blockIdx.y * F * Qg + idx] = acc;
 }
 
-// grad[p] (=|+=) scale * sum of the partial rows in a fixed order: a block
-// covers 64 coefficients; its 16 row groups sum contiguous row ranges, then
-// the 16 group sums are added in group order.
-constexpr int RED_X = 64, RED_Y = 16;
+// grad[p] (=|+=) scale * sum of the partial rows, in a fixed order and in two
+// stages.  k_grad_fold: block (bx, s) covers 64 coefficients and row fold s
+// (of RED_S contiguous row ranges); its 16 thread rows sum contiguous
+// sub-ranges, added in thread-row order into fold[s][p].  k_grad_reduce adds
+// the RED_S folds in fold order.
+constexpr int RED_X = 64, RED_Y = 16, RED_S = 16;
 
-static __global__ __launch_bounds__(RED_X * RED_Y) void k_grad_reduce(int ncoeff, int rows, int pn,
-                                                                      const double* __restrict__ part, int oW,
-                                                                      int nW, int wrows,
-                                                                      const double* __restrict__ wpart, double scale,
-                                                                      int add, double* __restrict__ grad) {
+static __global__ __launch_bounds__(RED_X * RED_Y) void k_grad_fold(int ncoeff, int rows, int pn,
+                                                                    const double* __restrict__ part, int oW, int nW,
+                                                                    int wrows, const double* __restrict__ wpart,
+                                                                    double* __restrict__ fold) {
   __shared__ double red[RED_Y][RED_X];
-  const int x = threadIdx.x, y = threadIdx.y;
+  const int x = threadIdx.x, y = threadIdx.y, s = blockIdx.y;
   const int p = blockIdx.x * RED_X + x;
   double acc = 0;
   if (p < ncoeff) {
     const bool isw = oW >= 0 && p >= oW && p < oW + nW;
     const int nr = isw ? wrows : rows;
-    const int chunk = (nr + RED_Y - 1) / RED_Y;
-    const int r0 = y * chunk, r1 = min(nr, r0 + chunk);
+    const int fchunk = (nr + RED_S - 1) / RED_S;
+    const int f0 = s * fchunk, f1 = min(nr, f0 + fchunk);
+    const int chunk = (fchunk + RED_Y - 1) / RED_Y;
+    const int r0 = f0 + y * chunk, r1 = min(f1, r0 + chunk);
     if (isw) {
       for (int r = r0; r < r1; ++r) acc += wpart[(size_t)r * nW + (p - oW)];
     } else {
@@ -804,25 +811,35 @@ static __global__ __launch_bounds__(RED_X * RED_Y) void k_grad_reduce(int ncoeff
   red[y][x] = acc;
   __syncthreads();
   if (y == 0 && p < ncoeff) {
-    double s = 0;
-    for (int g = 0; g < RED_Y; ++g) s += red[g][x];
-    grad[p] = (add ? grad[p] : 0.0) + scale * s;
+    double t = 0;
+    for (int q = 0; q < RED_Y; ++q) t += red[q][x];
+    fold[(size_t)s * ncoeff + p] = t;
   }
 }
 
-// Workgroups of one wavefront that fit on the whole device at once, rounded
-// down to a multiple of the four SIMDs of a CU: the grid-stride loops give
-// every group the same share of atoms, and a fifth wave on a CU would share a
-// SIMD with another and finish last.
+static __global__ __launch_bounds__(256) void k_grad_reduce(int ncoeff, const double* __restrict__ fold, double scale,
+                                                            int add, double* __restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ncoeff) return;
+  double t = 0;
+  for (int s = 0; s < RED_S; ++s) t += fold[(size_t)s * ncoeff + p];
+  grad[p] = (add ? grad[p] : 0.0) + scale * t;
+}
+
+// Workgroups of one wavefront that fit on the whole device at once.  With
+// whole_simds the count per CU is rounded down to a multiple of its four
+// SIMDs: the grid-stride loops give every group the same share of atoms, and
+// for an issue-bound kernel a fifth wave on a CU shares a SIMD with another
+// and finishes last.
 template <class K>
-int resident_groups(K kern, size_t lds) {
+int resident_groups(K kern, size_t lds, bool whole_simds) {
   constexpr int SIMDS = 4;
   allow_lds(kern, lds);
   int per_cu = 0, dev = 0, cus = 0;
   SUS2_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), WAVE, lds));
   SUS2_CHECK(hipGetDevice(&dev));
   SUS2_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  if (per_cu > SIMDS) per_cu -= per_cu % SIMDS;
+  if (whole_simds && per_cu > SIMDS) per_cu -= per_cu % SIMDS;
   return std::max(1, per_cu) * std::max(1, cus);
 }
 
@@ -839,12 +856,12 @@ void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const 
   const int pn = ncoeff - nW;
   const size_t n1 = std::max(1, N), ne = std::max<long>(1, ws.nedges);
   // grid-stride groups: as many workgroups as can be resident at once (every
-  // group holds its own FP64 partial row, so more would only add rows)
+  // group holds its own FP64 partial row, so more would only add rows); the
+  // outer pass owns partial rows [0, Go), the gate reverse rows [Go, Go + Gg)
   const GradLds<Cf> lo_ = grad_lds<Cf>(m, false, true, pn), l3_ = grad_lds<Cf>(m, true, true, pn);
-  const int G = std::max(1, std::min(N, std::min(resident_groups(k_grad_outer<Cf>, lo_.bytes()),
-                                                 m.gated ? resident_groups(k_grad_gate_reverse<Cf>, l3_.bytes())
-                                                         : 1 << 30)));
-  ws.gpart.ensure(sizeof(double) * (size_t)G * pn);
+  const int Go = std::max(1, std::min(N, resident_groups(k_grad_outer<Cf>, lo_.bytes(), true)));
+  const int Gg = m.gated ? std::max(1, std::min(N, resident_groups(k_grad_gate_reverse<Cf>, l3_.bytes(), false))) : 0;
+  ws.gpart.ensure(sizeof(double) * (size_t)(Go + Gg) * pn);
   const int* off = ws.nbr_off.get<int>();
   const Rec<T>* rec = ws.rec.get<Rec<T>>();
   int wrows = 0;
@@ -867,15 +884,15 @@ void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const 
                   ws.gmom.get<T>(), ws.gmomd.get<T>());
     }
     allow_lds(k_grad_outer<Cf>, lo.bytes());
-    SUS2_LAUNCH(ws, st, kname<Cf>("grad_outer"), (k_grad_outer<Cf>), dim3(G), dim3(WAVE), lo.bytes(), m, lo, gw, N,
+    SUS2_LAUNCH(ws, st, kname<Cf>("grad_outer"), (k_grad_outer<Cf>), dim3(Go), dim3(WAVE), lo.bytes(), m, lo, gw, N,
                 b.types, off, rec, ws.h.get<T>(), ws.hdot.get<T>(), ws.in_pos.get<int>(), ws.msg.get<T>(),
                 ws.gpart.get<double>(), oW, nW);
     if (m.gated) {
       const GradLds<Cf> l3 = grad_lds<Cf>(m, true, true, pn);
       allow_lds(k_grad_gate_reverse<Cf>, l3.bytes());
-      SUS2_LAUNCH(ws, st, kname<Cf>("grad_gate_reverse"), (k_grad_gate_reverse<Cf>), dim3(G), dim3(WAVE), l3.bytes(),
-                  m, l3, gw, N, b.types, off, ws.in_edges.get<int>(), rec, ws.vg.get<T>(), ws.vgd.get<T>(),
-                  ws.gmom.get<T>(), ws.gmomd.get<T>(), ws.msg.get<T>(), ws.hbar.get<T>(), ws.gpart.get<double>(), oW,
+      SUS2_LAUNCH(ws, st, kname<Cf>("grad_gate_reverse"), (k_grad_gate_reverse<Cf>), dim3(Gg), dim3(WAVE),
+                  l3.bytes(), m, l3, gw, N, b.types, off, rec, ws.vg.get<T>(), ws.vgd.get<T>(), ws.gmom.get<T>(),
+                  ws.gmomd.get<T>(), ws.msg.get<T>(), ws.hbar.get<T>(), ws.gpart.get<double>() + (size_t)Go * pn, oW,
                   nW);
       const int chunk = 64;
       wrows = (N + chunk - 1) / chunk;
@@ -885,9 +902,12 @@ void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const 
                   m.gate_nodes, ws.wpart.get<double>());
     }
   }
-  SUS2_LAUNCH(ws, st, "grad_reduce", k_grad_reduce, dim3((ncoeff + RED_X - 1) / RED_X), dim3(RED_X, RED_Y), 0, ncoeff,
-              N > 0 ? G : 0, pn, ws.gpart.get<double>(), oW, nW, wrows, ws.wpart.get<double>(), scale, add ? 1 : 0,
-              grad);
+  ws.gfold.ensure(sizeof(double) * (size_t)RED_S * ncoeff);
+  SUS2_LAUNCH(ws, st, "grad_fold", k_grad_fold, dim3((ncoeff + RED_X - 1) / RED_X, RED_S), dim3(RED_X, RED_Y), 0,
+              ncoeff, N > 0 ? Go + Gg : 0, pn, ws.gpart.get<double>(), oW, nW, wrows, ws.wpart.get<double>(),
+              ws.gfold.get<double>());
+  SUS2_LAUNCH(ws, st, "grad_reduce", k_grad_reduce, dim3((ncoeff + 255) / 256), dim3(256), 0, ncoeff,
+              ws.gfold.get<double>(), scale, add ? 1 : 0, grad);
 }
 
 }  // namespace sus2

@@ -177,6 +177,9 @@ void fill_view(DevModelT<T>& dv, const Model& m, const int* ibase, const std::ve
   v.smode = m.smode; v.R = m.R; v.O = m.O; v.Qg = m.Qg;
   v.rc = T(m.rc); v.rca = T(m.rca); v.scaling = T(m.scaling);
   v.gated = m.gated; v.species_readout = m.species_readout;
+  v.scalar_unique = 1;
+  for (int n : m.scalar_map)
+    if (std::count(m.scalar_map.begin(), m.scalar_map.end(), n) != 1) v.scalar_unique = 0;
   v.theta = dv.theta.template get<T>();
   v.o_shift = m.o_shift; v.o_scal = m.o_scal; v.o_rad = m.o_rad; v.o_grad = m.o_grad; v.o_gscal = m.o_gscal;
   v.o_alpha = m.o_alpha; v.o_gtype = m.o_gtype; v.o_W = m.o_W; v.o_gw = m.o_gw; v.o_ow = m.o_ow;

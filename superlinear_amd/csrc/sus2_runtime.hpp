@@ -79,7 +79,7 @@ struct Workspace {
   // parameter gradient: gate tangent h_dot, gate node tangents, type-free gate
   // moments and tangents, per-edge gate messages, gathered h_bar, per-workgroup
   // FP64 partials (coefficients; gate weights W)
-  DevBuf hdot, vgd, gmom, gmomd, msg, hbar, gpart, wpart;
+  DevBuf hdot, vgd, gmom, gmomd, msg, hbar, gpart, wpart, gfold;
   // training objective: EFS outputs, loss adjoints and per-frame losses (FP64)
   DevBuf lE, lF, lV, lew, lfw, lsw, lloss;
   long nedges = 0;

@@ -179,3 +179,33 @@ def test_lbfgs_training_reduces_objective(lib, prec):
     loss, _ = m.loss_grad_device(batch, lab["energy"], lab["forces"], precision=prec)
     assert abs(loss - trace[-1]) <= 1e-9 * trace[0]
     assert not np.allclose(m.coeffs(), th)
+
+
+def test_repeated_scalar_node(lib, oracle_lib, tmp_path):
+    """A readout mapping that names one graph node twice (the file format
+    allows it): the energy seeds of that node add up in scalar order.  EFS and
+    its parameter gradient against the oracle on such a model."""
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    print("[case] repeated scalar node", flush=True)
+    text = open(W.model_path("effect_l3k4r2o2")).read()
+    dup = text.replace("alpha_moment_mapping = {238, 239,", "alpha_moment_mapping = {238, 238,")
+    assert dup != text
+    path = tmp_path / "dup.sus2"
+    path.write_text(dup)
+    frames = _frames("mixed", 2, 95)
+    n = sum(len(f["types"]) for f in frames)
+    om = oracle_lib.OracleModel(str(path))
+    ref = om.efs(frames)
+    m = Sus2Model(str(path))
+    batch = DeviceBatch(frames)
+    for prec, tol in (("fp64", 1e-10), ("fp32", 1e-5)):
+        out = m.efs_device(batch, prec)
+        assert _rel(out["energy"].cpu().numpy(), ref["energy"]) <= tol
+        assert _rel(out["forces"].cpu().numpy(), ref["forces"]) <= tol
+    ew, fw, sw = _weights(n, len(frames), 96)
+    gref = om.efs_grad(frames, ew, fw, sw, nthreads=16)
+    args = [torch.from_numpy(x).cuda() for x in (ew, fw, sw)]
+    g = m.efs_grad_device(batch, *args, precision="fp64").cpu().numpy()
+    assert _rel(g, gref) <= TOL64
