@@ -116,7 +116,8 @@ def train_section(path, steps, warmup, profile):
             ach = fl / avg_s / 1e12
             out["train_roofline"] = dict(bound="mfma", pipe="valu", kernel=dom, achieved=round(ach, 4),
                                          peak=PEAK["fp32_tflops"], unit="TFLOP/s",
-                                         frac=round(ach / PEAK["fp32_tflops"], 5), flops_per_launch=fl,
+                                         frac=round(ach / PEAK["fp32_tflops"], 5),
+                                         traffic=measured_traffic(dom, "train256"), flops_per_launch=fl,
                                          avg_launch_us=round(avg_s * 1e6, 2))
     return out
 
@@ -144,10 +145,10 @@ def cpu_baseline(model_path, frame, threads, budget_s=20.0):
                 sample=f"{copies} copies of the {n}-atom frame, FP64 oracle, OpenMP over frames, {dt:.1f} s")
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, each scaled by
-    the copy-kernel calibration) from the committed rocprofv3 counter summary,
-    or None when the summary does not cover this kernel."""
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` in `workload` (FETCH_SIZE + WRITE_SIZE,
+    each scaled by the copy-kernel calibration) from the committed rocprofv3
+    counter summary, or None when the summary does not cover this kernel."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic_effect4096.md")
     try:
         text = open(path).read()
@@ -157,7 +158,7 @@ def measured_traffic(kernel):
     m = re.search(r"```json\n(.*?)\n```", text, re.S)
     if not m:
         return None
-    table = json.loads(m.group(1)).get("bytes_per_launch", {})
+    table = json.loads(m.group(1)).get(workload, {})
     v = table.get(kernel)
     return None if v is None else int(v)
 
@@ -242,7 +243,7 @@ def main():
                         flops_per_launch=fl, avg_launch_us=round(avg_s * 1e6, 2))
     survey_rate = survey_efs_per_edge(dims) * nedges * args.steps / t32 / 1e12
     if roof is not None:
-        roof["traffic"] = measured_traffic(roof["kernel"])
+        roof["traffic"] = measured_traffic(roof["kernel"], "effect4096")
     train = {} if args.no_train else train_section(path, args.train_steps, 1, not args.no_profile)
 
     if args.copy_probe_mib > 0:

@@ -102,8 +102,7 @@ struct GradWeights {
   const double* ene_w;  // [N]
   const double* frc_w;  // [N*3]
   const double* str_w;  // [nf*9] row-major
-  const int64_t* frame_off;
-  int nframes;
+  const int* atom_frame;  // [N] frame of every atom
 };
 template <class Cf>
 void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const GradWeights& gw, double* grad,

@@ -161,7 +161,7 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
     int nframes, const int64_t* __restrict__ frame_off, const double* __restrict__ pos,
     const double* __restrict__ cell, const int32_t* __restrict__ periodic, double rc, Grid* __restrict__ grid,
     int* __restrict__ cell_of, int* __restrict__ wrap, int* __restrict__ cell_start, int* __restrict__ cell_atoms,
-    int* __restrict__ in_cur) {
+    int* __restrict__ in_cur, int* __restrict__ atom_frame) {
   extern __shared__ int lds_cells[];
   int* bin = lds_cells;                      // [na] local bin of each atom
   int* cnt = bin + FRAME_CELLS_MAX;          // [nbins + 1] counts, then exclusive starts
@@ -288,6 +288,7 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
     bin[k] = id;
     cell_of[b0 + k] = (int)b0 + id;
     in_cur[b0 + k] = 0;
+    atom_frame[b0 + k] = f;
     atomicAdd(cnt + id, 1);
   }
   __syncthreads();
@@ -336,10 +337,11 @@ __device__ __forceinline__ void frac_of(const Grid& g, const double* x, double* 
 
 __global__ void k_bin(int natoms, int nframes, const int64_t* __restrict__ frame_off, const double* __restrict__ pos,
                       const Grid* __restrict__ grid, int* __restrict__ cell_of, int* __restrict__ wrap,
-                      int* __restrict__ cell_cnt) {
+                      int* __restrict__ cell_cnt, int* __restrict__ atom_frame) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= natoms) return;
   const int f = frame_of(frame_off, nframes, i);
+  atom_frame[i] = f;
   const Grid g = grid[f];
   double fr[3];
   frac_of(g, pos + 3 * i, fr);
@@ -425,7 +427,8 @@ __global__ void k_zero_int(int n, int* __restrict__ a) {
 __device__ __forceinline__ int floor_div(int a, int n) { return a >= 0 ? a / n : -((-a + n - 1) / n); }
 
 template <class T>
-__global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, int nframes, const int64_t* __restrict__ frame_off,
+__global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __restrict__ atom_frame,
+                                                    const int64_t* __restrict__ frame_off,
                                                     const double* __restrict__ pos, const double* __restrict__ cell,
                                                     const Grid* __restrict__ grid, const int* __restrict__ cell_of,
                                                     const int* __restrict__ wrap, const int* __restrict__ cell_start,
@@ -436,7 +439,7 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, int nframes, con
   const int i = blockIdx.x, lane = threadIdx.x;
   if (i >= natoms) return;
   if (cap == 0 && cnt[i] <= over) return;
-  const int f = frame_of(frame_off, nframes, i);
+  const int f = atom_frame[i];
   const Grid g = grid[f];
   const double* a = cell + 9 * f;
   const int cbase = (int)frame_off[f];
@@ -613,6 +616,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   const size_t n1 = std::max(1, N);
   ws.grid.ensure(sizeof(Grid) * std::max(1, b.nframes));
   ws.cell_of.ensure(sizeof(int) * n1);
+  ws.atom_frame.ensure(sizeof(int) * n1);
   ws.wrap.ensure(sizeof(int) * 3 * n1);
   ws.cell_cnt.ensure(sizeof(int) * n1);
   ws.cell_start.ensure(sizeof(int) * (n1 + 1));
@@ -647,14 +651,15 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
       SUS2_LAUNCH(ws, st, "frame_cells", k_frame_cells, dim3(b.nframes), dim3(FRAME_CELLS_THREADS),
                   frame_cells_lds(), b.nframes, b.frame_off, b.pos, b.cell, b.periodic, rc, ws.grid.get<Grid>(),
                   ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(), ws.cell_atoms.get<int>(),
-                  ws.in_cur.get<int>());
+                  ws.in_cur.get<int>(), ws.atom_frame.get<int>());
     }
   } else {
     SUS2_LAUNCH(ws, st, "grid_setup", k_grid_setup, dim3((std::max(b.nframes, N) + 63) / 64), dim3(64), 0,
                 b.nframes, N, b.frame_off, b.pos, b.cell, b.periodic, rc, ws.grid.get<Grid>(), ws.cell_cnt.get<int>(),
                 ws.cell_cur.get<int>(), ws.in_cur.get<int>());
     SUS2_LAUNCH(ws, st, "bin_atoms", k_bin, dim3(nb), dim3(256), 0, N, b.nframes, b.frame_off, b.pos,
-                ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_cnt.get<int>());
+                ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_cnt.get<int>(),
+                ws.atom_frame.get<int>());
     device_scan(ws.cell_cnt.get<int>(), ws.cell_start.get<int>(), N, ws.scan_tmp.get<int64_t>() + 2, st);
     SUS2_LAUNCH(ws, st, "fill_bins", k_fill_bins, dim3(nb), dim3(256), 0, N, ws.cell_of.get<int>(),
                 ws.cell_start.get<int>(), ws.cell_cur.get<int>(), ws.cell_atoms.get<int>());
@@ -662,7 +667,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
                 ws.cell_atoms.get<int>(), (int*)nullptr);
   }
   if (N > 0) {
-    SUS2_LAUNCH(ws, st, "neighbors" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, b.nframes, b.frame_off, b.pos,
+    SUS2_LAUNCH(ws, st, "neighbors" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(), b.frame_off, b.pos,
                 b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(),
                 ws.cell_atoms.get<int>(), rc, (const int*)nullptr, ws.nbr_cnt.get<int>(),
                 ws.nbr_slots.get<Rec<T>>(), cap, 0);
@@ -678,7 +683,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     SUS2_LAUNCH(ws, st, "compact" + tn, k_compact<T>, dim3(N), dim3(WAVE), 0, N, cap, ws.nbr_cnt.get<int>(),
                 ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>());
     if (maxcnt > cap)  // atoms with more neighbours than slots: fill their records directly
-      SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, b.nframes, b.frame_off,
+      SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(), b.frame_off,
                   b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
                   ws.cell_start.get<int>(), ws.cell_atoms.get<int>(), rc, ws.nbr_off.get<int>(),
                   ws.nbr_cnt.get<int>(), ws.rec.get<Rec<T>>(), 0, cap);

@@ -522,7 +522,7 @@ __device__ void scale_moments(const SpeciesTables<typename Cf::T>& tab, int ti, 
 // Per-atom frame weights: str_w of the atom's frame (T) and frc_w of the atom.
 template <class T>
 __device__ void load_atom_weights(const GradWeights& gw, int i, T* Sw, double* fwi) {
-  const int f = frame_of(gw.frame_off, gw.nframes, i);
+  const int f = gw.atom_frame[i];
 #pragma unroll
   for (int q = 0; q < 9; ++q) Sw[q] = T(gw.str_w[9 * (size_t)f + q]);
 #pragma unroll
@@ -704,20 +704,38 @@ __global__ __launch_bounds__(WAVE) void k_grad_gate_reverse(DModel<T> m, GradLds
     double fwi[3];
     load_atom_weights(gw, i, Sw, fwi);
     const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
-    // gather the gate messages of the incoming edges
-    T hb[F], hbd[F];
-#pragma unroll
-    for (int mu = 0; mu < F; ++mu) { hb[mu] = T(0); hbd[mu] = T(0); }
-    for (int k = e0 + lane; k < e1; k += WAVE) {  // incoming slots (same range as the out edges)
-      const T* mi = msg + (size_t)k * 2 * F;
-#pragma unroll
-      for (int mu = 0; mu < F; ++mu) { hb[mu] += mi[mu]; hbd[mu] += mi[F + mu]; }
+    // gather the gate messages of the incoming edges: column c of the message
+    // rows (h_bar part, then h_dot_bar part) summed over the atom's incoming
+    // slots, coalesced along the rows; with 2F <= 64 the lanes form WAVE / 2F
+    // groups that take every group-th slot, and the group sums are added in
+    // group order
+    constexpr int NC = 2 * F;
+    auto column_sum = [&](int col, int k0, int step) {
+      T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0);
+      int k = k0;
+      for (; k + 3 * step < e1; k += 4 * step) {
+        a0 += msg[(size_t)k * NC + col];
+        a1 += msg[(size_t)(k + step) * NC + col];
+        a2 += msg[(size_t)(k + 2 * step) * NC + col];
+        a3 += msg[(size_t)(k + 3 * step) * NC + col];
+      }
+      for (; k < e1; k += step) a0 += msg[(size_t)k * NC + col];
+      return (a0 + a1) + (a2 + a3);
+    };
+    if constexpr (NC <= WAVE) {
+      constexpr int NG = WAVE / NC;
+      const int col = lane % NC, grp = lane / NC;
+      if (grp < NG) s.stage[grp * NC + col] = column_sum(col, e0 + grp, NG);
+      __syncthreads();
+      if (lane < NC) {
+        T t = T(0);
+        for (int q = 0; q < NG; ++q) t += s.stage[q * NC + lane];
+        s.hbs[lane] = t;
+      }
+    } else {
+      for (int col = lane; col < NC; col += WAVE) s.hbs[col] = column_sum(col, e0, 1);
     }
-#pragma unroll
-    for (int mu = 0; mu < F; ++mu) {
-      const T a = wave_sum(hb[mu]), b = wave_sum(hbd[mu]);
-      if (lane == 0) { s.hbs[mu] = a; s.hbs[F + mu] = b; }
-    }
+    __syncthreads();  // the group sums in the staging region are consumed
     const T* vi = vg + (size_t)i * p.node_cnt;
     const T* vdi = vgd + (size_t)i * p.node_cnt;
     for (int n = lane; n < m.M; n += WAVE) {
@@ -845,12 +863,14 @@ int resident_groups(K kern, size_t lds, bool whole_simds) {
 
 // ------------------------------------------------------------------ driver
 template <class Cf>
-void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const GradWeights& gw, double* grad,
+void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const GradWeights& gw_in, double* grad,
                     double scale, bool add, bool reuse_neighbors, hipStream_t st) {
   using T = typename Cf::T;
   const DModel<T>& m = dm.get<T>();
   const int N = b.natoms;
   if (!reuse_neighbors) build_neighbors<T>(ws, b, dm.host->rc, st);
+  GradWeights gw = gw_in;
+  gw.atom_frame = ws.atom_frame.get<int>();  // written by the neighbour build of this batch
   const int ncoeff = dm.host->ncoeff;
   const int oW = m.gated ? m.o_W : -1, nW = m.gated ? Cf::F * m.Qg : 0;
   const int pn = ncoeff - nW;
