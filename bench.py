@@ -19,6 +19,12 @@ from a teacher model (the same coefficients perturbed with a fixed seed).  A
 training step = the objective (weighting "structures", energy weight 1, force
 weight 0.01: E/atom MSE + 0.01 force MSE) and its full parameter gradient for
 the whole batch (EFS, loss adjoints, forward-over-reverse gradient kernels).
+
+Batched workload ("batch4096"): Effect L4/K6/R2/O3, 3 species, 4096 frames
+with N ~ logU[16, 512], half periodic cells and half molecules, all in one
+call; atom-steps/s, p50/p99 batch latency, FP64 ratio and the force-loss
+gradient rate.  Large-frame workload ("md2m"): Effect L5/K6/R2/O3, diamond
+64^3 cells = 2,097,152 atoms, FP32 and FP64 EFS.
 """
 import argparse
 import json
@@ -122,6 +128,115 @@ def train_section(path, steps, warmup, profile):
     return out
 
 
+def _call_latencies(fn, steps, warmup):
+    """Seconds of each of `steps` synchronised calls of fn (after `warmup`)."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    lat = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - t0)
+    return np.array(lat)
+
+
+def _roofline_of(prof, dims, nedges):
+    """Roofline entry of the kernel with the largest summed time among those
+    with an algorithmic flop count (HIP-event profile of the same calls)."""
+    from superlinear_amd.flops import PEAK, kernel_flops
+    timed = {k: v for k, v in prof.items() if kernel_flops(k, dims, 1) is not None}
+    if not timed:
+        return None
+    dom = max(timed.items(), key=lambda kv: kv[1][0])[0]
+    avg_s = prof[dom][0] / prof[dom][1] / 1e3
+    fl = kernel_flops(dom, dims, nedges)
+    ach = fl / avg_s / 1e12
+    return dict(bound="mfma", pipe="valu", kernel=dom, achieved=round(ach, 4), peak=PEAK["fp32_tflops"],
+                unit="TFLOP/s", frac=round(ach / PEAK["fp32_tflops"], 5), flops_per_launch=fl,
+                avg_launch_us=round(avg_s * 1e6, 2))
+
+
+def batch_section(steps, warmup, profile):
+    """Uneven batched EFS and its force-loss gradient: 4096 frames in one call."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.flops import survey_efs_per_edge
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    path = W.model_path("effect_l4k6r2o3")
+    frames = W.config_frames("batch4096")
+    nf, n = len(frames), sum(len(f["types"]) for f in frames)
+    batch = DeviceBatch(frames)
+    model = Sus2Model(path)
+    dims = model.dims()
+    print(f"[bench] batch4096: {nf} frames, {n} atoms", file=sys.stderr, flush=True)
+    l32 = _call_latencies(lambda: model.efs_device(batch, "fp32"), steps, warmup)
+    nedges = model.last_edge_count()
+    l64 = _call_latencies(lambda: model.efs_device(batch, "fp64"), steps, warmup)
+    out = dict(workload="batch4096", model="Effect L4/K6/R2/O3 body 4, B=10, 3 species", frames=nf, atoms=n,
+               directed_edges=nedges, atom_steps_per_s=round(n * steps / l32.sum(), 1),
+               frames_per_s=round(nf * steps / l32.sum(), 1), p50_ms=round(1e3 * float(np.percentile(l32, 50)), 3),
+               p99_ms=round(1e3 * float(np.percentile(l32, 99)), 3),
+               fp64_atom_steps_per_s=round(n * steps / l64.sum(), 1), fp32_over_fp64=round(l64.sum() / l32.sum(), 3),
+               step_survey_tflops=round(survey_efs_per_edge(dims) * nedges * steps / l32.sum() / 1e12, 4))
+    # force-loss gradient of the whole batch (labels from a perturbed teacher)
+    teacher = Sus2Model(path)
+    th = teacher.coeffs()
+    teacher.set_coeffs(th + 0.05 * np.abs(th) * np.random.default_rng(4041).standard_normal(th.size))
+    lab = teacher.efs_device(batch, "fp64")
+    gsteps = max(1, steps // 4)
+    lg = _call_latencies(lambda: model.loss_grad_device(batch, lab["energy"], lab["forces"], weighting="structures",
+                                                        wE=1.0, wF=0.01, precision="fp32"), gsteps, 1)
+    out["grad_frames_per_s"] = round(nf * gsteps / lg.sum(), 1)
+    out["grad_ms_per_step"] = round(1e3 * lg.mean(), 3)
+    if profile:
+        model.profile_reset()
+        model.profile(True)
+        model.efs_device(batch, "fp32")
+        import torch
+        torch.cuda.synchronize()
+        model.profile(False)
+        prof = model.profile_read()
+        out["kernels_fp32"] = {k: dict(avg_us=round(1e3 * ms / max(c, 1), 2), launches=c) for k, (ms, c) in prof.items()}
+        out["roofline"] = _roofline_of(prof, dims, nedges)
+    return out
+
+
+def md_section(steps, warmup, profile):
+    """One large frame: 2,097,152-atom diamond, Effect L5/K6, FP32 and FP64."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.flops import survey_efs_per_edge
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    path = W.model_path("effect_l5k6r2o3")
+    frame = W.config_frames("md2m")[0]
+    n = len(frame["types"])
+    batch = DeviceBatch([frame])
+    model = Sus2Model(path)
+    dims = model.dims()
+    print(f"[bench] md2m: {n} atoms", file=sys.stderr, flush=True)
+    l32 = _call_latencies(lambda: model.efs_device(batch, "fp32"), steps, warmup)
+    nedges = model.last_edge_count()
+    print(f"[bench] md2m fp32 {1e3 * l32.mean():.1f} ms/step", file=sys.stderr, flush=True)
+    l64 = _call_latencies(lambda: model.efs_device(batch, "fp64"), max(1, steps - 1), 1)
+    out = dict(workload="md2m", model="Effect L5/K6/R2/O3 body 4, B=10, 2 species", atoms=n, directed_edges=nedges,
+               atom_steps_per_s=round(n * steps / l32.sum(), 1), ms_per_step=round(1e3 * l32.mean(), 3),
+               fp64_atom_steps_per_s=round(n * len(l64) / l64.sum(), 1),
+               fp64_ms_per_step=round(1e3 * l64.mean(), 3), fp32_over_fp64=round(l64.mean() / l32.mean(), 3),
+               step_survey_tflops=round(survey_efs_per_edge(dims) * nedges / l32.mean() / 1e12, 4))
+    if profile:
+        model.profile_reset()
+        model.profile(True)
+        model.efs_device(batch, "fp32")
+        import torch
+        torch.cuda.synchronize()
+        model.profile(False)
+        prof = model.profile_read()
+        out["kernels_fp32"] = {k: dict(avg_us=round(1e3 * ms / max(c, 1), 2), launches=c) for k, (ms, c) in prof.items()}
+        out["roofline"] = _roofline_of(prof, dims, nedges)
+    return out
+
+
 def cpu_baseline(model_path, frame, threads, budget_s=20.0):
     """FP64 oracle EFS on the host cores: `threads` copies of the frame, one
     per OpenMP thread (frame-parallel, as the reference shards frames over
@@ -172,6 +287,10 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-train", action="store_true", help="skip the training (loss + gradient) workload")
     ap.add_argument("--train-steps", type=int, default=5)
+    ap.add_argument("--no-batch", action="store_true", help="skip the 4096-frame uneven batch workload")
+    ap.add_argument("--batch-steps", type=int, default=10)
+    ap.add_argument("--no-md", action="store_true", help="skip the 2,097,152-atom frame workload")
+    ap.add_argument("--md-steps", type=int, default=3)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed runs, launch a plain FP32 copy kernel (y = 2x) over this many MiB, "
                          "three times: the calibration dispatch for rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -245,6 +364,8 @@ def main():
     if roof is not None:
         roof["traffic"] = measured_traffic(roof["kernel"], "effect4096")
     train = {} if args.no_train else train_section(path, args.train_steps, 1, not args.no_profile)
+    batch4096 = None if args.no_batch else batch_section(args.batch_steps, 2, not args.no_profile)
+    md2m = None if args.no_md else md_section(args.md_steps, 1, not args.no_profile)
 
     if args.copy_probe_mib > 0:
         x = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
@@ -283,6 +404,8 @@ def main():
         "gpu": torch.cuda.get_device_name(0),
     }
     line.update({k: v for k, v in train.items() if k != "train_grad_frames_per_s"})
+    line["batch4096"] = batch4096
+    line["md2m"] = md2m
     print(json.dumps(line))
     return 0
 

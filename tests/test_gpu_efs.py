@@ -39,6 +39,7 @@ CONFIG_CASES = [
     ("effect_l3k4r2o2", "diamond2", dict(ncell=2, species=2, seed=8)),
     ("effect_l3k4r2o2", "diamond3", dict(ncell=3, species=2, seed=9)),
     ("effect_l4k6r2o3", "diamond2", dict(ncell=2, species=3, seed=10)),
+    ("effect_l5k6r2o3", "diamond2", dict(ncell=2, species=2, seed=11)),
 ]
 
 
@@ -74,6 +75,44 @@ def test_batch_mixed_frames(lib, oracle_lib):
     ref = _oracle_efs(oracle_lib, path, frames)
     r = Sus2Model(path).calc_efs(frames, "fp64")
     assert np.abs(r["energy"] - ref["energy"]).max() <= 1e-10 * (1 + np.abs(ref["energy"]).max())
+    assert np.abs(r["forces"] - ref["forces"]).max() <= 1e-10 * (1 + np.abs(ref["forces"]).max())
+    assert np.abs(r["virial"] - ref["virial"]).max() <= 1e-10 * (1 + np.abs(ref["virial"]).max())
+
+
+def test_batch_uneven_l4k6(lib, oracle_lib):
+    """The batched-EFS workload's form at a reduced count: three species,
+    L4/K6, log-uniform frame sizes, periodic frames and molecules mixed in one
+    call, FP64 and FP32 against the oracle."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    print("[case] uneven batch l4k6", flush=True)
+    path = W.model_path("effect_l4k6r2o3")
+    frames = (W.periodic_frames(4, 16, 160, 3, 0.10, seed=41, log_uniform=True)
+              + W.molecule_frames(4, 16, 160, 3, seed=42))
+    ref = _oracle_efs(oracle_lib, path, frames)
+    m = Sus2Model(path)
+    r = m.calc_efs(frames, "fp64")
+    assert np.abs(r["energy"] - ref["energy"]).max() <= 1e-10 * (1 + np.abs(ref["energy"]).max())
+    assert np.abs(r["forces"] - ref["forces"]).max() <= 1e-10 * (1 + np.abs(ref["forces"]).max())
+    assert np.abs(r["virial"] - ref["virial"]).max() <= 1e-10 * (1 + np.abs(ref["virial"]).max())
+    r32 = m.calc_efs(frames, "fp32")
+    counts = np.array([len(f["types"]) for f in frames])
+    assert np.abs(r32["forces"] - ref["forces"]).max() <= 1e-5
+    assert np.abs((r32["energy"] - ref["energy"]) / counts).max() <= 1e-6
+
+
+def test_large_frame_grid_path(lib, oracle_lib):
+    """A frame above the per-frame single-workgroup cell-list limit (8192
+    atoms) takes the device-wide binning path; sparse (density 0.03 A^-3) so
+    that the oracle stays quick."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    print("[case] large frame grid path", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frames = W.periodic_frames(1, 8300, 8300, 2, 0.10, seed=13, density=(0.03, 0.03))
+    ref = _oracle_efs(oracle_lib, path, frames)
+    r = Sus2Model(path).calc_efs(frames, "fp64")
+    assert abs(r["energy"][0] - ref["energy"][0]) <= 1e-10 * (1 + abs(ref["energy"][0]))
     assert np.abs(r["forces"] - ref["forces"]).max() <= 1e-10 * (1 + np.abs(ref["forces"]).max())
     assert np.abs(r["virial"] - ref["virial"]).max() <= 1e-10 * (1 + np.abs(ref["virial"]).max())
 
