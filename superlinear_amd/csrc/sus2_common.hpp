@@ -66,7 +66,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st);
 // Per-species site-energy constant species[t] + shift[t] (FP64 coefficients).
 struct SiteConst {
   const double* theta;
-  int o_species, o_shift;
+  int o_species, o_shift, C;
   __device__ __forceinline__ double operator()(int t) const { return theta[o_species + t] + theta[o_shift + t]; }
 };
 // Forces of atoms [0, b.natoms), site energies of atoms [0, nsite) (all when

@@ -52,9 +52,26 @@ namespace sus2 {
 struct Grid {
   double inv[9];  // fractional = (cartesian - org) * inv (row vector convention)
   double org[3];  // origin of the open (non-periodic) directions
+  double A[9];    // basis rows: lattice rows, open rows completed and scaled to the atoms' extent
+  double rbox;    // half the longest diagonal of one bin
   int n[3], S[3];
   int per;  // bit q set: periodic along lattice row q
 };
+
+// Basis rows and the bin half-diagonal, once the bin counts are fixed.
+__device__ inline void grid_box(Grid& g, const double* a) {
+  double rb = 0;
+  for (int k = 0; k < 9; ++k) g.A[k] = a[k];
+  for (int s = 0; s < 8; ++s) {
+    double v[3] = {0, 0, 0};
+    for (int q = 0; q < 3; ++q) {
+      const double w = ((s >> q) & 1 ? 1.0 : -1.0) / g.n[q];
+      for (int k = 0; k < 3; ++k) v[k] += w * a[3 * q + k];
+    }
+    rb = fmax(rb, sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+  }
+  g.rbox = 0.5 * rb;
+}
 
 // Per frame: lattice rows that are non-zero are periodic (a zero row is a
 // non-periodic direction, as in the reference's Configuration); open
@@ -145,6 +162,7 @@ __global__ void k_grid_setup(int nframes, int natoms, const int64_t* __restrict_
     g.n[q] = max(1, g.n[q] - 1);
   }
   for (int q = 0; q < 3; ++q) g.S[q] = (g.per & (1 << q)) ? (int)floor(reach[q] * g.n[q]) + 1 : 1;
+  grid_box(g, a);
   grid[f] = g;
 }
 
@@ -155,7 +173,7 @@ __global__ void k_grid_setup(int nframes, int natoms, const int64_t* __restrict_
 // launch.  Bin ids of frame f are frame_off[f] + local bin, as in the global
 // path; ids past the frame's bins start at the next frame's first atom.
 constexpr int FRAME_CELLS_MAX = 8192;
-constexpr int FRAME_CELLS_THREADS = 512;
+constexpr int FRAME_CELLS_THREADS = 1024;
 
 __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
     int nframes, const int64_t* __restrict__ frame_off, const double* __restrict__ pos,
@@ -263,6 +281,7 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
       g.n[q] = max(1, g.n[q] - 1);
     }
     for (int q = 0; q < 3; ++q) g.S[q] = (g.per & (1 << q)) ? (int)floor(reach[q] * g.n[q]) + 1 : 1;
+    grid_box(g, a);
     g_sh = g;
     grid[f] = g;
   }
@@ -322,9 +341,15 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
   for (int p = t; p < na; p += nt) {
     const int k = tmp[p], id = bin[k];
     const int s0 = cnt[id], s1 = cnt[id + 1];
-    int r = 0;
-    for (int q = s0; q < s1; ++q) r += tmp[q] < k;
-    cell_atoms[b0 + s0 + r] = (int)b0 + k;
+    int r0 = 0, r1 = 0, r2 = 0, r3 = 0, q = s0;
+    for (; q + 3 < s1; q += 4) {  // four independent LDS reads in flight
+      r0 += tmp[q] < k;
+      r1 += tmp[q + 1] < k;
+      r2 += tmp[q + 2] < k;
+      r3 += tmp[q + 3] < k;
+    }
+    for (; q < s1; ++q) r0 += tmp[q] < k;
+    cell_atoms[b0 + s0 + (r0 + r1) + (r2 + r3)] = (int)b0 + k;
   }
 }
 
@@ -450,6 +475,12 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
   const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
   const int wi0 = wrap[3 * i], wi1 = wrap[3 * i + 1], wi2 = wrap[3 * i + 2];
   const double rc2 = rc * rc;
+  // the atom in its home cell; a bin box whose centre is farther than
+  // rc + (bin half-diagonal) holds no neighbour (1e-6 A of slack)
+  const double hx = xi - wi0 * g.A[0] - wi1 * g.A[3] - wi2 * g.A[6];
+  const double hy = yi - wi0 * g.A[1] - wi1 * g.A[4] - wi2 * g.A[7];
+  const double hz = zi - wi0 * g.A[2] - wi1 * g.A[5] - wi2 * g.A[8];
+  const double far2 = (rc + g.rbox + 1e-6) * (rc + g.rbox + 1e-6);
   const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
   int count = 0;
   const long w0 = cap ? (long)i * cap : (long)off[i];
@@ -466,6 +497,13 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
       else ok = ok && y >= 0 && y < g.n[1];
       if (g.per & 4) { s2 = floor_div(z, g.n[2]); z -= s2 * g.n[2]; }
       else ok = ok && z >= 0 && z < g.n[2];
+      if (ok) {
+        const double u0 = (x + 0.5) / g.n[0] + s0, u1 = (y + 0.5) / g.n[1] + s1, u2 = (z + 0.5) / g.n[2] + s2;
+        const double ex = g.org[0] + u0 * g.A[0] + u1 * g.A[3] + u2 * g.A[6] - hx;
+        const double ey = g.org[1] + u0 * g.A[1] + u1 * g.A[4] + u2 * g.A[7] - hy;
+        const double ez = g.org[2] + u0 * g.A[2] + u1 * g.A[5] + u2 * g.A[8] - hz;
+        ok = ex * ex + ey * ey + ez * ez <= far2;
+      }
       if (ok) {
         const int c = cbase + x + g.n[0] * (y + g.n[1] * z);
         bs = cell_start[c];
@@ -551,36 +589,88 @@ __global__ void k_incoming(long nedges, const Rec<T>* __restrict__ rec, const in
 // ------------------------------------------------------------------ finalize
 // One 256-thread block per frame: FP64 sums of site energies and virial
 // partials in a fixed order (strided per thread, then a fixed tree).
+// Frame reduction in chunks of FIN_CHUNK atoms: block (f, s) covers atoms
+// [frame_off[f] + s FIN_CHUNK, ...) of frame f, writes their FP64 forces and
+// site energies, and sums energy and virial in a fixed order (per-thread
+// strided sums, a fixed wave-shuffle tree, then the waves in order).  With
+// one chunk per frame (nsub == 1) the block writes the frame's energy and
+// virial; otherwise its partial, and k_finalize_frames adds the chunks in
+// order.
+constexpr int FIN_THREADS = 1024, FIN_CHUNK = 8192;
+
 template <class T>
-__global__ __launch_bounds__(256) void k_finalize(int nframes, const int64_t* __restrict__ frame_off,
-                                                  const int32_t* __restrict__ types, SiteConst sc,
-                                                  const T* __restrict__ site, const T* __restrict__ fw,
-                                                  double* __restrict__ energy, double* __restrict__ virial,
-                                                  double* __restrict__ force, double* __restrict__ site_out) {
-  __shared__ double red[7][256];
-  const int f = blockIdx.x, t = threadIdx.x;
-  const long b = frame_off[f], e = frame_off[f + 1];
-  double s[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (long i = b + t; i < e; i += 256) {
-    const double se = (double)site[i] + sc(types[i]);
-    s[0] += se;
-    for (int q = 0; q < 6; ++q) s[1 + q] += (double)fw[i * 9 + 3 + q];
-    force[3 * i] = (double)fw[9 * i];
-    force[3 * i + 1] = (double)fw[9 * i + 1];
-    force[3 * i + 2] = (double)fw[9 * i + 2];
-    if (site_out) site_out[i] = se;
-  }
-  for (int q = 0; q < 7; ++q) red[q][t] = s[q];
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize(int nsub, const int64_t* __restrict__ frame_off,
+                                                          const int32_t* __restrict__ types, SiteConst sc,
+                                                          const T* __restrict__ site, const T* __restrict__ fw,
+                                                          double* __restrict__ energy, double* __restrict__ virial,
+                                                          double* __restrict__ force, double* __restrict__ site_out,
+                                                          double* __restrict__ part) {
+  constexpr int CMAX = 64;  // per-species site constants staged in LDS
+  __shared__ double red[7][FIN_THREADS / WAVE];
+  __shared__ double spc[CMAX];
+  const int f = blockIdx.x, s = blockIdx.y, t = threadIdx.x;
+  if (t < CMAX && t < sc.C) spc[t] = sc(t);
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (t < w)
-      for (int q = 0; q < 7; ++q) red[q][t] += red[q][t + w];
-    __syncthreads();
+  const long fe = frame_off[f + 1];
+  const long b = frame_off[f] + (long)s * FIN_CHUNK, e = min(fe, b + FIN_CHUNK);
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  // all of the thread's loads first (FIN_CHUNK / FIN_THREADS atoms), then
+  // the sums and stores in atom order
+  constexpr int PER = FIN_CHUNK / FIN_THREADS;
+  T wv[PER][9], sv[PER];
+  int tv[PER];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const long i = b + t + (long)p * FIN_THREADS;
+    if (i < e) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) wv[p][q] = fw[9 * i + q];
+      sv[p] = site[i];
+      tv[p] = types[i];
+    }
   }
-  if (t == 0) {
-    energy[f] = red[0][0];
-    for (int q = 0; q < 6; ++q) virial[6 * f + q] = red[1 + q][0];
+#pragma unroll
+  for (int p = 0; p < PER; ++p) {
+    const long i = b + t + (long)p * FIN_THREADS;
+    if (i < e) {
+      const int ty = tv[p];
+      const double se = (double)sv[p] + (ty < CMAX ? spc[ty] : sc(ty));
+      acc[0] += se;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) acc[1 + q] += (double)wv[p][3 + q];
+      force[3 * i] = (double)wv[p][0];
+      force[3 * i + 1] = (double)wv[p][1];
+      force[3 * i + 2] = (double)wv[p][2];
+      if (site_out) site_out[i] = se;
+    }
   }
+#pragma unroll
+  for (int q = 0; q < 7; ++q) acc[q] = wave_sum(acc[q]);
+  if (t % WAVE == 0)
+#pragma unroll
+    for (int q = 0; q < 7; ++q) red[q][t / WAVE] = acc[q];
+  __syncthreads();
+  if (t < 7) {
+    double v = 0;
+    for (int k = 0; k < FIN_THREADS / WAVE; ++k) v += red[t][k];
+    if (nsub == 1) {
+      if (t == 0) energy[f] = v;
+      else virial[6 * f + t - 1] = v;
+    } else {
+      part[((size_t)f * nsub + s) * 7 + t] = v;
+    }
+  }
+}
+
+__global__ void k_finalize_frames(int nframes, int nsub, const double* __restrict__ part,
+                                  double* __restrict__ energy, double* __restrict__ virial) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nframes * 7) return;
+  const int f = idx / 7, q = idx % 7;
+  double v = 0;
+  for (int s = 0; s < nsub; ++s) v += part[((size_t)f * nsub + s) * 7 + q];
+  if (q == 0) energy[f] = v;
+  else virial[6 * f + q - 1] = v;
 }
 
 // Forces of atoms [first, natoms) outside every frame (the ghosts of a pair-style call).
@@ -631,6 +721,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   // frames of at most FRAME_CELLS_MAX atoms build their cell lists in one
   // workgroup each; otherwise the multi-kernel global path
   bool per_frame = N <= FRAME_CELLS_MAX;
+  ws.max_frame_atoms = N;
   if (!per_frame && b.nframes > 0) {
     ws.host_frame_off.resize(b.nframes + 1);
     SUS2_CHECK(hipMemcpyAsync(ws.host_frame_off.data(), b.frame_off, sizeof(int64_t) * (b.nframes + 1),
@@ -638,6 +729,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     SUS2_CHECK(hipStreamSynchronize(st));
     int64_t mx = 0;
     for (int f = 0; f < b.nframes; ++f) mx = std::max(mx, ws.host_frame_off[f + 1] - ws.host_frame_off[f]);
+    ws.max_frame_atoms = mx;
     per_frame = mx <= FRAME_CELLS_MAX;
   }
   if (per_frame) {
@@ -703,9 +795,17 @@ template <class T>
 void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, const SiteConst& sc, hipStream_t st, int nsite) {
   const int N = b.natoms;
   const int first = nsite < 0 ? N : nsite;  // atoms past the frames (pair-style ghosts)
-  if (b.nframes > 0)
-    SUS2_LAUNCH(ws, st, "finalize", k_finalize<T>, dim3(b.nframes), dim3(256), 0, b.nframes, b.frame_off, b.types, sc,
-                ws.site.get<T>(), ws.fw.get<T>(), out.energy, out.virial, out.force, out.site);
+  if (b.nframes > 0) {
+    const long mx = std::max<long>(1, std::min<long>(ws.max_frame_atoms, N));
+    const int nsub = (int)((mx + FIN_CHUNK - 1) / FIN_CHUNK);
+    if (nsub > 1) ws.fin_part.ensure(sizeof(double) * 7 * (size_t)b.nframes * nsub);
+    SUS2_LAUNCH(ws, st, "finalize", k_finalize<T>, dim3(b.nframes, nsub), dim3(FIN_THREADS), 0, nsub, b.frame_off,
+                b.types, sc, ws.site.get<T>(), ws.fw.get<T>(), out.energy, out.virial, out.force, out.site,
+                ws.fin_part.get<double>());
+    if (nsub > 1)
+      SUS2_LAUNCH(ws, st, "finalize_frames", k_finalize_frames, dim3((7 * b.nframes + 255) / 256), dim3(256), 0,
+                  b.nframes, nsub, ws.fin_part.get<double>(), out.energy, out.virial);
+  }
   if (first < N)
     SUS2_LAUNCH(ws, st, "atom_out", k_atom_out<T>, dim3((N - first + 255) / 256), dim3(256), 0, first, N,
                 ws.fw.get<T>(), out.force);

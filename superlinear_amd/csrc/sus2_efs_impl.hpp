@@ -284,8 +284,13 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
 // outer pass of an Effect model the gate-adjoint message
 //   u_e,mu = tau_{t_i,mu} R_mu(r; t_i, t_j) sum_m abar_i[k,l,m] Y_lm(d_e),
 // which the neighbour j gathers into h_bar_j (times alpha_{t_j,mu}).
+// Outer pass, FP32, L <= 3: at most 168 VGPRs (no spills), so three waves
+// share each SIMD.
+template <class Cf, bool GATE>
+constexpr int edge_backward_waves() { return (!GATE && sizeof(typename Cf::T) == 4 && Cf::LL <= 3) ? 3 : 1; }
+
 template <class Cf, bool GATE, class T = typename Cf::T>
-__global__ __launch_bounds__(WAVE) void k_edge_backward(DModel<T> m, const int32_t* __restrict__ types,
+__global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edge_backward(DModel<T> m, const int32_t* __restrict__ types,
                                                         const int* __restrict__ nbr_off,
                                                         const Rec<T>* __restrict__ rec, const T* __restrict__ h,
                                                         const T* __restrict__ adj, const int* __restrict__ in_pos,
@@ -546,7 +551,7 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
                   (T*)nullptr, ws.fw.get<T>());
     }
   }
-  finish_efs<T>(ws, b, out, SiteConst{dm.d.view.theta, m.o_species, m.o_shift}, st);
+  finish_efs<T>(ws, b, out, SiteConst{dm.d.view.theta, m.o_species, m.o_shift, m.C}, st);
 }
 
 // ------------------------------------------------------------------ pair-style stages
@@ -626,7 +631,8 @@ void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const Pai
   out.force = pv.force;
   out.virial = pv.virial;
   out.site = pv.site;
-  finish_efs<T>(ws, b, out, SiteConst{dm.d.view.theta, m.o_species, m.o_shift}, st, n);
+  ws.max_frame_atoms = n;
+  finish_efs<T>(ws, b, out, SiteConst{dm.d.view.theta, m.o_species, m.o_shift, m.C}, st, n);
 }
 
 }  // namespace sus2

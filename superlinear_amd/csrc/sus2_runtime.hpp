@@ -84,6 +84,8 @@ struct Workspace {
   // training objective: EFS outputs, loss adjoints and per-frame losses (FP64)
   DevBuf lE, lF, lV, lew, lfw, lsw, lloss;
   long nedges = 0;
+  long max_frame_atoms = 0;  // upper bound on the atoms of one frame of the current batch
+  DevBuf fin_part;           // per-chunk partial frame sums of the frame reduction
   int64_t* host_total = nullptr;  // pinned
   ~Workspace();
   void ensure_atoms(int n);
