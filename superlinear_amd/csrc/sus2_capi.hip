@@ -483,6 +483,7 @@ int sus2_pair_begin(sus2_pair* p, int nowned, int nall, const int32_t* types, in
     if (nowned < 0 || nall < nowned || nedges < 0) throw sus2::Error(SUS2_E_INVALID, "invalid atom layout");
     if ((nall > 0 && !types) || (nedges > 0 && (!ei || !ej || !d))) throw sus2::Error(SUS2_E_INVALID, "null buffer");
     if (nedges > INT32_MAX - 1) throw sus2::Error(SUS2_E_RANGE, "too many edges");
+    sus2::check_rec_limits(p->precision, nall, M.C);
     for (int k = 0; k < nall; ++k)
       if (types[k] < 0 || types[k] >= M.C) throw sus2::Error(SUS2_E_RANGE, "atom species index out of range");
     // validate every edge; edges at or beyond the cutoff contribute nothing and are dropped
@@ -522,10 +523,10 @@ int sus2_pair_begin(sus2_pair* p, int nowned, int nall, const int32_t* types, in
     h2d(p->frame_off.get<int64_t>(), fo, sizeof(fo));
     if (p->precision == SUS2_FP32)
       sus2::build_pair_lists<float>(p->ws, nowned, nall, ne, p->ei.get<int32_t>(), p->ej.get<int32_t>(),
-                                    p->d.get<double>(), p->in_off.get<int>(), nullptr);
+                                    p->d.get<double>(), p->types.get<int32_t>(), p->in_off.get<int>(), nullptr);
     else
       sus2::build_pair_lists<double>(p->ws, nowned, nall, ne, p->ei.get<int32_t>(), p->ej.get<int32_t>(),
-                                     p->d.get<double>(), p->in_off.get<int>(), nullptr);
+                                     p->d.get<double>(), p->types.get<int32_t>(), p->in_off.get<int>(), nullptr);
     sus2::pair_stage_dispatch(0, p->precision, p->model->dev, p->ws, pair_view(p), nullptr);
     p->state = 1;
   });

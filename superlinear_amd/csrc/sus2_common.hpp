@@ -13,16 +13,39 @@
 
 namespace sus2 {
 
-// One directed edge i -> j: d = x_j - x_i (+ image) and the index j stored in
-// the bits of the fourth component (one 16 / 32-byte load per edge).
+// One directed edge i -> j: d = x_j - x_i (+ image), and the neighbour index
+// j with its species t_j packed into the bits of the fourth component (one
+// 16 / 32-byte load per edge, no dependent species lookup).  FP32: j in the
+// low 26 bits, t_j in the high 6; FP64: j in the low 32, t_j in the high 32.
 template <class T>
 struct alignas(4 * sizeof(T)) Rec {
   T x, y, z, jb;
 };
-__device__ __forceinline__ int rec_j(const Rec<float>& r) { return __float_as_int(r.jb); }
-__device__ __forceinline__ int rec_j(const Rec<double>& r) { return (int)__double_as_longlong(r.jb); }
-__device__ __forceinline__ float rec_jbits(int j, float) { return __int_as_float(j); }
-__device__ __forceinline__ double rec_jbits(int j, double) { return __longlong_as_double((long long)j); }
+constexpr int REC32_JBITS = 26;
+constexpr long REC32_MAX_ATOMS = 1L << REC32_JBITS;
+constexpr int REC32_MAX_SPECIES = 1 << (32 - REC32_JBITS);
+__device__ __forceinline__ int rec_j(const Rec<float>& r) {
+  return __float_as_int(r.jb) & ((1 << REC32_JBITS) - 1);
+}
+__device__ __forceinline__ int rec_t(const Rec<float>& r) {
+  return (int)((unsigned)__float_as_int(r.jb) >> REC32_JBITS);
+}
+__device__ __forceinline__ int rec_j(const Rec<double>& r) {
+  return (int)(__double_as_longlong(r.jb) & 0xffffffffLL);
+}
+__device__ __forceinline__ int rec_t(const Rec<double>& r) { return (int)(__double_as_longlong(r.jb) >> 32); }
+__device__ __forceinline__ float rec_jbits(int j, int t, float) {
+  return __int_as_float((int)((unsigned)j | ((unsigned)t << REC32_JBITS)));
+}
+__device__ __forceinline__ double rec_jbits(int j, int t, double) {
+  return __longlong_as_double(((long long)t << 32) | (unsigned)j);
+}
+// The FP32 packing bounds the batch (2^26 atoms, far beyond one GPU's memory
+// at FP32) and the species count (64).
+inline void check_rec_limits(int precision, long natoms, int species) {
+  if (precision == SUS2_FP32 && (natoms >= REC32_MAX_ATOMS || species > REC32_MAX_SPECIES))
+    throw Error(SUS2_E_UNSUPPORTED, "FP32 batches hold fewer than 2^26 atoms and at most 64 species (use SUS2_FP64)");
+}
 
 // Frame of atom i: the largest f with frame_off[f] <= i (binary search).
 __device__ __forceinline__ int frame_of(const int64_t* off, int nframes, int i) {
@@ -91,7 +114,7 @@ template <class Cf>
 void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const PairView& pv, hipStream_t st);
 template <class T>
 void build_pair_lists(Workspace& ws, int nowned, int nall, long nedges, const int32_t* ei, const int32_t* ej,
-                      const double* d, int* in_off, hipStream_t st);
+                      const double* d, const int32_t* types, int* in_off, hipStream_t st);
 void pair_stage_dispatch(int stage, int precision, const DevModelSet& dm, Workspace& ws, const PairView& pv,
                          hipStream_t st);
 template <class Cf>

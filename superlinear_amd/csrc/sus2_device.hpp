@@ -172,6 +172,29 @@ __device__ __forceinline__ void sh_stream_grad(T ux, T uy, T uz, T inv_r, Fn&& f
   }
 }
 
+// A row of F values (a neighbour's gate signal): 16-byte vector loads when
+// the rows are 16-byte aligned (F * sizeof(T) a multiple of 16 on a 256-byte
+// aligned buffer), else scalar loads.
+template <class T, int F>
+__device__ __forceinline__ void load_row(const T* __restrict__ p, T* out) {
+  if constexpr (sizeof(T) == 4 && F % 4 == 0) {
+#pragma unroll
+    for (int k = 0; k < F / 4; ++k) {
+      const float4 v = reinterpret_cast<const float4*>(p)[k];
+      out[4 * k] = v.x; out[4 * k + 1] = v.y; out[4 * k + 2] = v.z; out[4 * k + 3] = v.w;
+    }
+  } else if constexpr (sizeof(T) == 8 && F % 2 == 0) {
+#pragma unroll
+    for (int k = 0; k < F / 2; ++k) {
+      const double2 v = reinterpret_cast<const double2*>(p)[k];
+      out[2 * k] = v.x; out[2 * k + 1] = v.y;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < F; ++k) out[k] = p[k];
+  }
+}
+
 __device__ __forceinline__ float dtanh(float x) { return tanhf(x); }
 __device__ __forceinline__ double dtanh(double x) { return tanh(x); }
 __device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }

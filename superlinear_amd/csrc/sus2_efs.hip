@@ -453,6 +453,7 @@ __device__ __forceinline__ int floor_div(int a, int n) { return a >= 0 ? a / n :
 
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __restrict__ atom_frame,
+                                                    const int32_t* __restrict__ types,
                                                     const int64_t* __restrict__ frame_off,
                                                     const double* __restrict__ pos, const double* __restrict__ cell,
                                                     const Grid* __restrict__ grid, const int* __restrict__ cell_of,
@@ -551,7 +552,7 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
       }
       const unsigned long long mask = __ballot(hit);
       const int slot = count + __popcll(mask & below);
-      if (hit && (cap == 0 || slot < cap)) rec[w0 + slot] = Rec<T>{T(d0), T(d1), T(d2), rec_jbits(j, T(0))};
+      if (hit && (cap == 0 || slot < cap)) rec[w0 + slot] = Rec<T>{T(d0), T(d1), T(d2), rec_jbits(j, types[j], T(0))};
       count += __popcll(mask);
     }
     __syncthreads();
@@ -759,7 +760,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
                 ws.cell_atoms.get<int>(), (int*)nullptr);
   }
   if (N > 0) {
-    SUS2_LAUNCH(ws, st, "neighbors" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(), b.frame_off, b.pos,
+    SUS2_LAUNCH(ws, st, "neighbors" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(), b.types, b.frame_off, b.pos,
                 b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(),
                 ws.cell_atoms.get<int>(), rc, (const int*)nullptr, ws.nbr_cnt.get<int>(),
                 ws.nbr_slots.get<Rec<T>>(), cap, 0);
@@ -775,7 +776,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     SUS2_LAUNCH(ws, st, "compact" + tn, k_compact<T>, dim3(N), dim3(WAVE), 0, N, cap, ws.nbr_cnt.get<int>(),
                 ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>());
     if (maxcnt > cap)  // atoms with more neighbours than slots: fill their records directly
-      SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(), b.frame_off,
+      SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(), b.types, b.frame_off,
                   b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
                   ws.cell_start.get<int>(), ws.cell_atoms.get<int>(), rc, ws.nbr_off.get<int>(),
                   ws.nbr_cnt.get<int>(), ws.rec.get<Rec<T>>(), 0, cap);
@@ -833,18 +834,20 @@ __global__ void k_place_keys(long n, const int32_t* __restrict__ keys, const int
 
 template <class T>
 __global__ void k_build_rec(long n, const int* __restrict__ order, const int32_t* __restrict__ ej,
-                            const double* __restrict__ d, Rec<T>* __restrict__ rec, int32_t* __restrict__ tgt) {
+                            const double* __restrict__ d, const int32_t* __restrict__ types,
+                            Rec<T>* __restrict__ rec, int32_t* __restrict__ tgt) {
   const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const int e = order[p];
   const int j = ej[e];
-  rec[p] = Rec<T>{T(d[3 * (size_t)e]), T(d[3 * (size_t)e + 1]), T(d[3 * (size_t)e + 2]), rec_jbits(j, T(0))};
+  rec[p] = Rec<T>{T(d[3 * (size_t)e]), T(d[3 * (size_t)e + 1]), T(d[3 * (size_t)e + 2]),
+                  rec_jbits(j, types[j], T(0))};
   tgt[p] = j;
 }
 
 template <class T>
 void build_pair_lists(Workspace& ws, int nowned, int nall, long nedges, const int32_t* ei, const int32_t* ej,
-                      const double* d, int* in_off, hipStream_t st) {
+                      const double* d, const int32_t* types, int* in_off, hipStream_t st) {
   const size_t n1 = std::max(1, nall), e1 = std::max<long>(1, nedges);
   ws.ensure_atoms(nowned);
   ws.cell_cnt.ensure(sizeof(int) * n1);
@@ -872,7 +875,7 @@ void build_pair_lists(Workspace& ws, int nowned, int nall, long nedges, const in
     if (nowned > 0)
       SUS2_LAUNCH(ws, st, "pair_sort", k_sort_lists, dim3(nowned), dim3(WAVE), 0, nowned, ws.nbr_off.get<int>(),
                   order, (int*)nullptr);
-    SUS2_LAUNCH(ws, st, "pair_rec", k_build_rec<T>, dim3(ge), dim3(256), 0, nedges, order, ej, d,
+    SUS2_LAUNCH(ws, st, "pair_rec", k_build_rec<T>, dim3(ge), dim3(256), 0, nedges, order, ej, d, types,
                 ws.rec.get<Rec<T>>(), tgt);
   }
   // in lists (all atoms) over CSR positions
@@ -889,9 +892,9 @@ void build_pair_lists(Workspace& ws, int nowned, int nall, long nedges, const in
   }
 }
 template void build_pair_lists<float>(Workspace&, int, int, long, const int32_t*, const int32_t*, const double*,
-                                      int*, hipStream_t);
+                                      const int32_t*, int*, hipStream_t);
 template void build_pair_lists<double>(Workspace&, int, int, long, const int32_t*, const int32_t*, const double*,
-                                       int*, hipStream_t);
+                                       const int32_t*, int*, hipStream_t);
 
 #define SUS2_DECLARE_PAIR(LL, KK, BB, SM)                                                                        \
   extern template void pair_stage_shape<Cfg<float, LL, KK, BB, SM>>(int, const DevModelSet&, Workspace&,        \
@@ -942,6 +945,7 @@ void run_efs(const DevModelSet& dm, Workspace& ws, const Batch& b, const EfsOut&
 
 void efs_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const EfsOut& out,
                   hipStream_t st) {
+  check_rec_limits(precision, b.natoms, dm.host->C);
   if (precision == SUS2_FP64) run_efs<double>(dm, ws, b, out, st);
   else if (precision == SUS2_FP32) run_efs<float>(dm, ws, b, out, st);
   else throw Error(SUS2_E_INVALID, "precision must be SUS2_FP64 or SUS2_FP32");

@@ -206,8 +206,9 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
       const int j = rec_j(q);
       const T r = dsqrt(q.x * q.x + q.y * q.y + q.z * q.z);
       const T inv = T(1) / r;
-      const int tj = types[j];
-      T Rv[F], Y[H];
+      const int tj = rec_t(q);
+      T Rv[F], Y[H], hj[F];
+      if (!GATE && m.gated) load_row<T, F>(h + (size_t)j * F, hj);  // issued before the basis work
       radial<Cf, false>(m, GATE, tab, ti * m.C + tj, r, Rv, nullptr);
       sh_values<T, LL>(q.x * inv, q.y * inv, q.z * inv, Y);
       const T* x0i = tab.x0 + ti * F;
@@ -216,7 +217,7 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
 #pragma unroll
       for (int mu = 0; mu < F; ++mu) {
         T x1 = x0j[mu];
-        if (!GATE && m.gated) x1 += alj[mu] * h[(size_t)j * F + mu];
+        if (!GATE && m.gated) x1 += alj[mu] * hj[mu];
         sR[mu * LR + lane] = x0i[mu] * x1 * Rv[mu];
       }
 #pragma unroll
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
     const T r = dsqrt(dx * dx + dy * dy + dz * dz);
     const T inv = T(1) / r;
     const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
-    const int tj = types[j];
+    const int tj = rec_t(q);
     // weighted radial values x0 x1 R and x0 x1 R' in registers; x0 R (the gate
     // message factor) parked in this lane's own LDS column
     T R[F], dR[F], us[F];

@@ -37,6 +37,7 @@ void run_grad(const DevModelSet& dm, Workspace& ws, const Batch& b, const GradWe
 void grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* ene_w,
                    const double* frc_w, const double* str_w, double* grad, double scale, bool add,
                    bool reuse_neighbors, hipStream_t st) {
+  check_rec_limits(precision, b.natoms, dm.host->C);
   const GradWeights gw{ene_w, frc_w, str_w, nullptr};  // atom_frame is the workspace's
   if (precision == SUS2_FP64) run_grad<double>(dm, ws, b, gw, grad, scale, add, reuse_neighbors, st);
   else if (precision == SUS2_FP32) run_grad<float>(dm, ws, b, gw, grad, scale, add, reuse_neighbors, st);

@@ -223,13 +223,17 @@ __device__ void grad_sweep_moments(const DModel<typename Cf::T>& m, const Specie
       const T r = dsqrt(dx * dx + dy * dy + dz * dz);
       const T inv = T(1) / r;
       const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
-      const int tj = types[j];
+      const int tj = rec_t(q);
       const double* fj = frc_w + 3 * (size_t)j;
       const T wx = T(fwi[0] - fj[0]) + Sw[0] * dx + Sw[1] * dy + Sw[2] * dz;
       const T wy = T(fwi[1] - fj[1]) + Sw[3] * dx + Sw[4] * dy + Sw[5] * dz;
       const T wz = T(fwi[2] - fj[2]) + Sw[6] * dx + Sw[7] * dy + Sw[8] * dz;
       const T wr = ux * wx + uy * wy + uz * wz;
-      T R[F], dR[F];
+      T R[F], dR[F], hj[F], hdj[F];
+      if (hg) {  // issued before the basis work
+        load_row<T, F>(h + (size_t)j * F, hj);
+        load_row<T, F>(hd + (size_t)j * F, hdj);
+      }
       radial<Cf, true>(m, GATE, tab, ti * m.C + tj, r, R, dR);
       const T* x0j = tab.x0 + tj * F;
       const T* alj = tab.alpha + tj * F;
@@ -237,8 +241,8 @@ __device__ void grad_sweep_moments(const DModel<typename Cf::T>& m, const Specie
       for (int mu = 0; mu < F; ++mu) {
         T x1 = x0j[mu], xd = T(0);
         if (hg) {
-          x1 += alj[mu] * h[(size_t)j * F + mu];
-          xd = alj[mu] * hd[(size_t)j * F + mu];
+          x1 += alj[mu] * hj[mu];
+          xd = alj[mu] * hdj[mu];
         }
         sS1[mu * LR + lane] = x1 * R[mu];
         sS2[mu * LR + lane] = x1 * dR[mu] * wr + xd * R[mu];
@@ -321,7 +325,7 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
       const T r = dsqrt(dx * dx + dy * dy + dz * dz);
       const T inv = T(1) / r;
       const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
-      const int tj = types[j];
+      const int tj = rec_t(q);
       const double* fj = frc_w + 3 * (size_t)j;
       const T wx = T(fwi[0] - fj[0]) + Sw[0] * dx + Sw[1] * dy + Sw[2] * dz;
       const T wy = T(fwi[1] - fj[1]) + Sw[3] * dx + Sw[4] * dy + Sw[5] * dz;
