@@ -61,9 +61,12 @@ struct Grid {
 // Basis rows and the bin half-diagonal, once the bin counts are fixed.
 __device__ inline void grid_box(Grid& g, const double* a) {
   double rb = 0;
+#pragma unroll
   for (int k = 0; k < 9; ++k) g.A[k] = a[k];
+#pragma unroll
   for (int s = 0; s < 8; ++s) {
     double v[3] = {0, 0, 0};
+#pragma unroll
     for (int q = 0; q < 3; ++q) {
       const double w = ((s >> q) & 1 ? 1.0 : -1.0) / g.n[q];
       for (int k = 0; k < 3; ++k) v[k] += w * a[3 * q + k];
@@ -71,6 +74,97 @@ __device__ inline void grid_box(Grid& g, const double* a) {
     rb = fmax(rb, sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
   }
   g.rbox = 0.5 * rb;
+}
+
+// Periodic mask of frame f: lattice rows that are non-zero (periodic frames).
+__device__ inline int grid_mask(const double* c, int periodic) {
+  int per = 0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    if (periodic && (c[3 * q] != 0.0 || c[3 * q + 1] != 0.0 || c[3 * q + 2] != 0.0)) per |= 1 << q;
+  return per;
+}
+
+__device__ inline void cross3(const double* u, const double* v, double* w) {
+  w[0] = u[1] * v[2] - u[2] * v[1];
+  w[1] = u[2] * v[0] - u[0] * v[2];
+  w[2] = u[0] * v[1] - u[1] * v[0];
+}
+__device__ inline void unit3(double* w) {
+  const double s = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  w[0] /= s; w[1] /= s; w[2] /= s;
+}
+// One periodic row P: the open rows P+1, P+2 become unit normals (the first
+// perpendicular to row P and its smallest Cartesian axis).
+template <int P>
+__device__ inline void complete_one(double* a) {
+  const double* v = a + 3 * P;
+  const int m = fabs(v[0]) <= fabs(v[1]) && fabs(v[0]) <= fabs(v[2]) ? 0 : fabs(v[1]) <= fabs(v[2]) ? 1 : 2;
+  const double e[3] = {m == 0 ? 1.0 : 0.0, m == 1 ? 1.0 : 0.0, m == 2 ? 1.0 : 0.0};
+  constexpr int Q1 = (P + 1) % 3, Q2 = (P + 2) % 3;
+  cross3(v, e, a + 3 * Q1);
+  unit3(a + 3 * Q1);
+  cross3(v, a + 3 * Q1, a + 3 * Q2);
+  unit3(a + 3 * Q2);
+}
+// Two periodic rows: the open row Q becomes their unit normal.
+template <int Q>
+__device__ inline void complete_two(double* a) {
+  cross3(a + 3 * ((Q + 1) % 3), a + 3 * ((Q + 2) % 3), a + 3 * Q);
+  unit3(a + 3 * Q);
+}
+// Basis rows: lattice rows for periodic directions, unit normals for open
+// ones (scaled afterwards to the atoms' extent).  Compile-time indices only.
+__device__ inline void grid_basis(const double* c, int per, double* a) {
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) a[3 * q + k] = (per & (1 << q)) ? c[3 * q + k] : 0.0;
+  switch (per) {
+    case 0: a[0] = a[4] = a[8] = 1.0; break;
+    case 1: complete_one<0>(a); break;
+    case 2: complete_one<1>(a); break;
+    case 4: complete_one<2>(a); break;
+    case 3: complete_two<2>(a); break;
+    case 5: complete_two<1>(a); break;
+    case 6: complete_two<0>(a); break;
+    default: break;
+  }
+}
+// Inverse basis, bins (at least r_c wide, at most one per atom) and image
+// reach of a frame with basis rows a (open rows already scaled).
+__device__ inline void grid_finish(Grid& g, const double* a, double rc, long na) {
+  double b[3][3];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      b[q][p] = a[((p + 1) % 3) * 3 + (q + 1) % 3] * a[((p + 2) % 3) * 3 + (q + 2) % 3] -
+                a[((p + 1) % 3) * 3 + (q + 2) % 3] * a[((p + 2) % 3) * 3 + (q + 1) % 3];
+  const double det = a[0] * b[0][0] + a[1] * b[1][0] + a[2] * b[2][0];
+#pragma unroll
+  for (int p = 0; p < 3; ++p)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) g.inv[3 * p + q] = b[p][q] / det;
+  double reach[3];
+  int n[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    reach[q] = rc * sqrt(g.inv[q] * g.inv[q] + g.inv[3 + q] * g.inv[3 + q] + g.inv[6 + q] * g.inv[6 + q]);
+    n[q] = (int)fmin(1024.0, fmax(1.0, floor(1.0 / reach[q])));
+  }
+  // at most one bin per atom: shrink the largest count (lowest index on ties)
+  while ((long)n[0] * n[1] * n[2] > (na > 0 ? na : 1)) {
+    if (n[0] >= n[1] && n[0] >= n[2]) n[0] = max(1, n[0] - 1);
+    else if (n[1] >= n[2]) n[1] = max(1, n[1] - 1);
+    else n[2] = max(1, n[2] - 1);
+  }
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    g.n[q] = n[q];
+    g.S[q] = (g.per & (1 << q)) ? (int)floor(reach[q] * n[q]) + 1 : 1;
+  }
+  grid_box(g, a);
 }
 
 // Per frame: lattice rows that are non-zero are periodic (a zero row is a
@@ -90,42 +184,13 @@ __global__ void k_grid_setup(int nframes, int natoms, const int64_t* __restrict_
   if (f >= nframes) return;
   Grid g;
   const long b0 = frame_off[f], na = frame_off[f + 1] - b0;
+  g.per = grid_mask(cell + 9 * f, periodic[f]);
   double a[9];
-  const double* c = cell + 9 * f;
-  g.per = 0;
-  for (int q = 0; q < 3; ++q) {
-    for (int k = 0; k < 3; ++k) a[3 * q + k] = 0.0;
-    if (periodic[f] && (c[3 * q] != 0.0 || c[3 * q + 1] != 0.0 || c[3 * q + 2] != 0.0)) {
-      g.per |= 1 << q;
-      for (int k = 0; k < 3; ++k) a[3 * q + k] = c[3 * q + k];
-    }
-  }
-  // complete the basis with unit normals for the open rows
-  const int nact = (g.per & 1) + ((g.per >> 1) & 1) + ((g.per >> 2) & 1);
-  auto cross = [](const double* u, const double* v, double* w) {
-    w[0] = u[1] * v[2] - u[2] * v[1]; w[1] = u[2] * v[0] - u[0] * v[2]; w[2] = u[0] * v[1] - u[1] * v[0];
-  };
-  auto unit = [](double* w) {
-    const double s = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-    for (int k = 0; k < 3; ++k) w[k] /= s;
-  };
-  if (nact == 0) {
-    a[0] = a[4] = a[8] = 1.0;
-  } else if (nact == 1) {
-    const int p = (g.per & 1) ? 0 : (g.per & 2) ? 1 : 2;
-    const double* v = a + 3 * p;
-    double e[3] = {0, 0, 0};
-    const int m = fabs(v[0]) <= fabs(v[1]) && fabs(v[0]) <= fabs(v[2]) ? 0 : fabs(v[1]) <= fabs(v[2]) ? 1 : 2;
-    e[m] = 1;
-    const int q1 = (p + 1) % 3, q2 = (p + 2) % 3;
-    cross(v, e, a + 3 * q1); unit(a + 3 * q1);
-    cross(v, a + 3 * q1, a + 3 * q2); unit(a + 3 * q2);
-  } else if (nact == 2) {
-    const int q = !(g.per & 1) ? 0 : !(g.per & 2) ? 1 : 2;
-    cross(a + 3 * ((q + 1) % 3), a + 3 * ((q + 2) % 3), a + 3 * q); unit(a + 3 * q);
-  }
+  grid_basis(cell + 9 * f, g.per, a);
   // open rows: scale the unit normal to the extent of the atoms along it
+#pragma unroll
   for (int k = 0; k < 3; ++k) g.org[k] = 0.0;
+#pragma unroll
   for (int q = 0; q < 3; ++q) {
     if (g.per & (1 << q)) continue;
     double lo = 0, hi = 0;
@@ -136,33 +201,13 @@ __global__ void k_grid_setup(int nframes, int natoms, const int64_t* __restrict_
       if (k == 0 || t > hi) hi = t;
     }
     const double ext = fmax(hi - lo, 1e-6) * (1 + 1e-9) + 1e-9;
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
       g.org[k] += lo * a[3 * q + k];
       a[3 * q + k] *= ext;
     }
   }
-  double b[3][3];
-  for (int p = 0; p < 3; ++p)
-    for (int q = 0; q < 3; ++q)
-      b[q][p] = a[((p + 1) % 3) * 3 + (q + 1) % 3] * a[((p + 2) % 3) * 3 + (q + 2) % 3] -
-                a[((p + 1) % 3) * 3 + (q + 2) % 3] * a[((p + 2) % 3) * 3 + (q + 1) % 3];
-  const double det = a[0] * b[0][0] + a[1] * b[1][0] + a[2] * b[2][0];
-  double reach[3];
-  for (int p = 0; p < 3; ++p)
-    for (int q = 0; q < 3; ++q) g.inv[3 * p + q] = b[p][q] / det;
-  for (int q = 0; q < 3; ++q) {
-    reach[q] = rc * sqrt(g.inv[q] * g.inv[q] + g.inv[3 + q] * g.inv[3 + q] + g.inv[6 + q] * g.inv[6 + q]);
-    g.n[q] = (int)fmin(1024.0, fmax(1.0, floor(1.0 / reach[q])));
-  }
-  // at most one bin per atom
-  while ((long)g.n[0] * g.n[1] * g.n[2] > (na > 0 ? na : 1)) {
-    int q = 0;
-    if (g.n[1] > g.n[q]) q = 1;
-    if (g.n[2] > g.n[q]) q = 2;
-    g.n[q] = max(1, g.n[q] - 1);
-  }
-  for (int q = 0; q < 3; ++q) g.S[q] = (g.per & (1 << q)) ? (int)floor(reach[q] * g.n[q]) + 1 : 1;
-  grid_box(g, a);
+  grid_finish(g, a, rc, na);
   grid[f] = g;
 }
 
@@ -190,46 +235,16 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
   const int na = (int)(frame_off[f + 1] - b0);
   if (t == 0) {
     Grid g;
-    const double* c = cell + 9 * f;
-    g.per = 0;
-    for (int q = 0; q < 3; ++q)
-      if (periodic[f] && (c[3 * q] != 0.0 || c[3 * q + 1] != 0.0 || c[3 * q + 2] != 0.0)) g.per |= 1 << q;
+    g.per = grid_mask(cell + 9 * f, periodic[f]);
     g_sh = g;
   }
   __syncthreads();
   // basis rows: lattice rows for periodic directions, unit normals for open ones
   double a[9];
-  {
-    const double* c = cell + 9 * f;
-    const int per = g_sh.per;
-    for (int q = 0; q < 3; ++q)
-      for (int k = 0; k < 3; ++k) a[3 * q + k] = (per & (1 << q)) ? c[3 * q + k] : 0.0;
-    const int nact = (per & 1) + ((per >> 1) & 1) + ((per >> 2) & 1);
-    auto cross = [](const double* u, const double* v, double* w) {
-      w[0] = u[1] * v[2] - u[2] * v[1]; w[1] = u[2] * v[0] - u[0] * v[2]; w[2] = u[0] * v[1] - u[1] * v[0];
-    };
-    auto unit = [](double* w) {
-      const double s = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
-      for (int k = 0; k < 3; ++k) w[k] /= s;
-    };
-    if (nact == 0) {
-      a[0] = a[4] = a[8] = 1.0;
-    } else if (nact == 1) {
-      const int p = (per & 1) ? 0 : (per & 2) ? 1 : 2;
-      const double* v = a + 3 * p;
-      double e[3] = {0, 0, 0};
-      const int m = fabs(v[0]) <= fabs(v[1]) && fabs(v[0]) <= fabs(v[2]) ? 0 : fabs(v[1]) <= fabs(v[2]) ? 1 : 2;
-      e[m] = 1;
-      const int q1 = (p + 1) % 3, q2 = (p + 2) % 3;
-      cross(v, e, a + 3 * q1); unit(a + 3 * q1);
-      cross(v, a + 3 * q1, a + 3 * q2); unit(a + 3 * q2);
-    } else if (nact == 2) {
-      const int q = !(per & 1) ? 0 : !(per & 2) ? 1 : 2;
-      cross(a + 3 * ((q + 1) % 3), a + 3 * ((q + 2) % 3), a + 3 * q); unit(a + 3 * q);
-    }
-  }
+  grid_basis(cell + 9 * f, g_sh.per, a);
   // open directions: extent of the atoms along each normal (block min / max)
   double org[3] = {0, 0, 0};
+#pragma unroll
   for (int q = 0; q < 3; ++q) {
     if (g_sh.per & (1 << q)) continue;
     double lo = 1e300, hi = -1e300;
@@ -253,6 +268,7 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
     hi = na > 0 ? red_hi[0] : 0.0;
     __syncthreads();
     const double ext = fmax(hi - lo, 1e-6) * (1 + 1e-9) + 1e-9;
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
       org[k] += lo * a[3 * q + k];
       a[3 * q + k] *= ext;
@@ -260,28 +276,9 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
   }
   if (t == 0) {
     Grid g = g_sh;
-    double b[3][3];
-    for (int p = 0; p < 3; ++p)
-      for (int q = 0; q < 3; ++q)
-        b[q][p] = a[((p + 1) % 3) * 3 + (q + 1) % 3] * a[((p + 2) % 3) * 3 + (q + 2) % 3] -
-                  a[((p + 1) % 3) * 3 + (q + 2) % 3] * a[((p + 2) % 3) * 3 + (q + 1) % 3];
-    const double det = a[0] * b[0][0] + a[1] * b[1][0] + a[2] * b[2][0];
-    double reach[3];
-    for (int p = 0; p < 3; ++p)
-      for (int q = 0; q < 3; ++q) g.inv[3 * p + q] = b[p][q] / det;
-    for (int q = 0; q < 3; ++q) {
-      g.org[q] = org[q];
-      reach[q] = rc * sqrt(g.inv[q] * g.inv[q] + g.inv[3 + q] * g.inv[3 + q] + g.inv[6 + q] * g.inv[6 + q]);
-      g.n[q] = (int)fmin(1024.0, fmax(1.0, floor(1.0 / reach[q])));
-    }
-    while ((long)g.n[0] * g.n[1] * g.n[2] > (na > 0 ? na : 1)) {
-      int q = 0;
-      if (g.n[1] > g.n[q]) q = 1;
-      if (g.n[2] > g.n[q]) q = 2;
-      g.n[q] = max(1, g.n[q] - 1);
-    }
-    for (int q = 0; q < 3; ++q) g.S[q] = (g.per & (1 << q)) ? (int)floor(reach[q] * g.n[q]) + 1 : 1;
-    grid_box(g, a);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) g.org[q] = org[q];
+    grid_finish(g, a, rc, na);
     g_sh = g;
     grid[f] = g;
   }
@@ -597,7 +594,7 @@ __global__ void k_incoming(long nedges, const Rec<T>* __restrict__ rec, const in
 // one chunk per frame (nsub == 1) the block writes the frame's energy and
 // virial; otherwise its partial, and k_finalize_frames adds the chunks in
 // order.
-constexpr int FIN_THREADS = 1024, FIN_CHUNK = 8192;
+constexpr int FIN_THREADS = 1024, FIN_CHUNK = 4096;
 
 template <class T>
 __global__ __launch_bounds__(FIN_THREADS) void k_finalize(int nsub, const int64_t* __restrict__ frame_off,
