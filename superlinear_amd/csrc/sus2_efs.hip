@@ -559,17 +559,28 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
 
 // Copy every atom's collected hits from its fixed-capacity slots to the CSR
 // records (atoms whose count exceeded the capacity are filled directly).
+// With in_edges, also place each copied edge in its target's incoming list
+// (unordered; sorted afterwards), as k_incoming does.
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_compact(int natoms, int cap, const int* __restrict__ cnt,
                                                   const int* __restrict__ off, const Rec<T>* __restrict__ slots,
-                                                  Rec<T>* __restrict__ rec) {
+                                                  Rec<T>* __restrict__ rec, int* __restrict__ in_cur,
+                                                  int* __restrict__ in_edges) {
   const int i = blockIdx.x;
   if (i >= natoms) return;
   const int n = cnt[i];
   if (n > cap) return;
   const Rec<T>* src = slots + (size_t)i * cap;
-  Rec<T>* dst = rec + off[i];
-  for (int k = threadIdx.x; k < n; k += WAVE) dst[k] = src[k];
+  const int o = off[i];
+  Rec<T>* dst = rec + o;
+  for (int k = threadIdx.x; k < n; k += WAVE) {
+    const Rec<T> q = src[k];
+    dst[k] = q;
+    if (in_edges) {
+      const int t = rec_j(q);
+      in_edges[off[t] + atomicAdd(in_cur + t, 1)] = o + k;
+    }
+  }
 }
 
 // Incoming edges of every atom (edges e' = j -> i grouped by i).  The full
@@ -770,16 +781,22 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   ws.in_edges.ensure(sizeof(int) * std::max<long>(total, 1));
   ws.in_pos.ensure(sizeof(int) * std::max<long>(total, 1));
   if (N > 0) {
+    // without overflow the copy also places the incoming edges; otherwise the
+    // overflowed atoms are filled first and the incoming lists built after
+    const bool overflow = maxcnt > cap;
     SUS2_LAUNCH(ws, st, "compact" + tn, k_compact<T>, dim3(N), dim3(WAVE), 0, N, cap, ws.nbr_cnt.get<int>(),
-                ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>());
-    if (maxcnt > cap)  // atoms with more neighbours than slots: fill their records directly
-      SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(), b.types, b.frame_off,
-                  b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
-                  ws.cell_start.get<int>(), ws.cell_atoms.get<int>(), rc, ws.nbr_off.get<int>(),
-                  ws.nbr_cnt.get<int>(), ws.rec.get<Rec<T>>(), 0, cap);
+                ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>(), ws.in_cur.get<int>(),
+                overflow ? (int*)nullptr : ws.in_edges.get<int>());
+    if (overflow)  // atoms with more neighbours than slots: fill their records directly
+      SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(),
+                  b.types, b.frame_off, b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(),
+                  ws.wrap.get<int>(), ws.cell_start.get<int>(), ws.cell_atoms.get<int>(), rc,
+                  ws.nbr_off.get<int>(), ws.nbr_cnt.get<int>(), ws.rec.get<Rec<T>>(), 0, cap);
     if (total > 0) {
-      SUS2_LAUNCH(ws, st, "incoming" + tn, k_incoming<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, total,
-                  ws.rec.get<Rec<T>>(), ws.nbr_off.get<int>(), ws.in_cur.get<int>(), ws.in_edges.get<int>());
+      if (overflow)
+        SUS2_LAUNCH(ws, st, "incoming" + tn, k_incoming<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                    total, ws.rec.get<Rec<T>>(), ws.nbr_off.get<int>(), ws.in_cur.get<int>(),
+                    ws.in_edges.get<int>());
       SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(WAVE), 0, N, ws.nbr_off.get<int>(),
                   ws.in_edges.get<int>(), ws.in_pos.get<int>());
     }

@@ -117,6 +117,25 @@ def test_large_frame_grid_path(lib, oracle_lib):
     assert np.abs(r["virial"] - ref["virial"]).max() <= 1e-10 * (1 + np.abs(ref["virial"]).max())
 
 
+def test_neighbor_slot_overflow_then_adapted(lib, oracle_lib):
+    """A dense frame (about 180 neighbours per atom) overflows the default
+    128 per-atom neighbour slots: the overflowed atoms are filled directly and
+    the incoming lists built afterwards.  The next call uses the enlarged
+    slots (copy and incoming placement fused).  Both against the oracle."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    print("[case] neighbour slot overflow", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frames = W.periodic_frames(1, 64, 64, 2, 0.05, seed=17, density=(0.35, 0.35))
+    ref = _oracle_efs(oracle_lib, path, frames)
+    m = Sus2Model(path)
+    for call in range(2):
+        r = m.calc_efs(frames, "fp64")
+        assert abs(r["energy"][0] - ref["energy"][0]) <= 1e-10 * (1 + abs(ref["energy"][0])), call
+        assert np.abs(r["forces"] - ref["forces"]).max() <= 1e-10 * (1 + np.abs(ref["forces"]).max()), call
+        assert np.abs(r["virial"] - ref["virial"]).max() <= 1e-10 * (1 + np.abs(ref["virial"]).max()), call
+
+
 def test_device_entry_point_matches_host(lib):
     import torch
     from superlinear_amd import workloads as W
