@@ -61,7 +61,9 @@ __host__ __device__ __forceinline__ int pidx(int p, int oW, int nW) { return (oW
 // (their lifetimes do not overlap), then the FP64 partial vector.
 template <class Cf>
 struct GradLds {
-  int stage_rows, M, sc, tabs, pn;
+  int stage_rows, M, sc, tabs, pn, C;
+  // per-species edge masks of a staged batch (64-bit words, in units of T)
+  __host__ __device__ size_t mask_words() const { return (size_t)C * 8 / sizeof(typename Cf::T) + 2; }
   __host__ __device__ size_t graph_floats() const {
     return 2 * (size_t)Cf::KH + 4 * (size_t)M + 2 * (size_t)sc;
   }
@@ -69,7 +71,9 @@ struct GradLds {
     const size_t st = (size_t)stage_rows * LR, gr = graph_floats();
     return st > gr ? st : gr;
   }
-  __host__ __device__ size_t floats() const { return (size_t)tabs + 4 * (size_t)Cf::KH + 2 * Cf::F + 2 + region(); }
+  __host__ __device__ size_t floats() const {
+    return (size_t)tabs + 4 * (size_t)Cf::KH + 2 * Cf::F + 2 + mask_words() + region();
+  }
   __host__ __device__ size_t bytes() const {
     // T area first, then the FP64 partial vector (8-byte aligned)
     const size_t t = floats() * sizeof(typename Cf::T);
@@ -80,7 +84,7 @@ struct GradLds {
 template <class Cf>
 __host__ __device__ inline int grad_stage_rows(bool sweep2, bool outer_gated) {
   const int r1 = 2 * Cf::F + 2 * Cf::H;
-  const int r2 = 4 * Cf::NBLK + (outer_gated ? 4 : 3) * Cf::F + 1;
+  const int r2 = 4 * Cf::NBLK + (outer_gated ? 4 : 3) * Cf::F;
   return sweep2 ? (r1 > r2 ? r1 : r2) : r1;
 }
 
@@ -92,6 +96,7 @@ GradLds<Cf> grad_lds(const DModel<typename Cf::T>& m, bool gate, bool sweep2, in
   g.sc = std::max(graph_scratch(gate ? m.gp : m.op), 1);
   g.tabs = species_tables_size<Cf>(m.C);
   g.pn = pn;
+  g.C = m.C;
   return g;
 }
 
@@ -99,6 +104,7 @@ GradLds<Cf> grad_lds(const DModel<typename Cf::T>& m, bool gate, bool sweep2, in
 template <class T>
 struct GradSmem {
   T *stage, *At, *Atd, *A, *Ad, *Pc, *Qc, *v, *vd, *P, *Q, *s1, *s2, *tabs, *hbs;
+  unsigned long long* tmask;  // [C]
   double* part;
 };
 
@@ -113,6 +119,12 @@ __device__ GradSmem<typename Cf::T> carve(const GradLds<Cf>& g, unsigned char* r
   s.At = p; p += Cf::KH;
   s.Atd = p; p += Cf::KH;
   s.hbs = p; p += 2 * Cf::F + 2;
+  {  // 8-byte aligned mask words
+    unsigned char* q = reinterpret_cast<unsigned char*>(p);
+    const size_t off = (size_t)(q - raw);
+    s.tmask = reinterpret_cast<unsigned long long*>(raw + ((off + 7) & ~(size_t)7));
+    p += g.mask_words();
+  }
   T* region = p;
   s.stage = region;  // edge staging (sweeps 1 and 2)
   s.A = region;      // graph phase
@@ -304,8 +316,7 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
   T* sCv = sDt + NBLK * LR;     // [F][LR]
   T* sCd = sCv + F * LR;        // [F][LR]
   T* sX1 = sCd + F * LR;        // [F][LR]  x0 Z  (d Phi / d x1)
-  T* sTj = sX1 + F * LR;        // [LR]
-  T* sAl = sTj + LR;            // [F][LR]  x0 (Z h_j + R q h_dot_j)  (outer, gated)
+  T* sAl = sX1 + F * LR;        // [F][LR]  x0 (Z h_j + R q h_dot_j)  (outer, gated)
   const bool own_rad = GATE && m.o_grad >= 0;
   const T* __restrict__ coef = m.theta + (own_rad ? m.o_grad : m.o_rad);
   const int stride = own_rad ? BB : BB + C;
@@ -318,6 +329,7 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
   const int cblk = Cf::blk(cmu / L1, cmu % L1);
   for (int base = e0; base < e1; base += WAVE) {
     const int e = base + lane;
+    int lane_t = -1;  // species of this lane's edge (none past the list)
     if (e < e1) {
       const Rec<T> q = rec[e];
       const int j = rec_j(q);
@@ -326,6 +338,7 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
       const T inv = T(1) / r;
       const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
       const int tj = rec_t(q);
+      lane_t = tj;
       const double* fj = frc_w + 3 * (size_t)j;
       const T wx = T(fwi[0] - fj[0]) + Sw[0] * dx + Sw[1] * dy + Sw[2] * dz;
       const T wy = T(fwi[1] - fj[1]) + Sw[3] * dx + Sw[4] * dy + Sw[5] * dz;
@@ -431,44 +444,62 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
           sDt[b * LR + lane] = dt;
         }
       }
-      sTj[lane] = T(tj);
+    }
+    for (int t = 0; t < C; ++t) {  // which staged edges have species t
+      const unsigned long long mk = __ballot(lane_t == t);
+      if (lane == 0) s.tmask[t] = mk;
     }
     __syncthreads();
     const int cnt = min(WAVE, e1 - base);
     // radial coefficients: lane (mu, subset) re-runs the Chebyshev recurrence
+    // over its share of the batch (at most WAVE / NSUB edges, summed in T),
+    // then adds the batch sums to its FP64 registers
     if (cact) {
       const T S = m.scaling;
+      T cb[BB];
+#pragma unroll
+      for (int qq = 0; qq < BB; ++qq) cb[qq] = T(0);
       for (int ee = csub; ee < cnt; ee += NSUB) {
         const T xi = sXi[cblk * LR + ee], cv = sCv[cmu * LR + ee], cd = sCd[cmu * LR + ee] * sXr[cblk * LR + ee];
         T ta = S, tb = S * xi, da = T(0), db = S;
-        creg[0] += (double)(cv * ta);
-        if (BB > 1) creg[1] += (double)(cv * tb + cd * db);
+        cb[0] += cv * ta;
+        if (BB > 1) cb[1] += cv * tb + cd * db;
 #pragma unroll
         for (int qq = 2; qq < BB; ++qq) {
           const T tn = T(2) * xi * tb - ta, dn = T(2) * (tb + xi * db) - da;
-          creg[qq] += (double)(cv * tn + cd * dn);
+          cb[qq] += cv * tn + cd * dn;
           ta = tb; tb = tn; da = db; db = dn;
         }
       }
+#pragma unroll
+      for (int qq = 0; qq < BB; ++qq) creg[qq] += (double)cb[qq];
     }
-    // type-bucketed sums: neighbour type factor, alpha, warp slope/shift
+    // type-bucketed sums: neighbour type factor, alpha, warp slope/shift; each
+    // lane walks the staged edges of its species in edge order (mask bits)
     const int n1 = C * F, n2 = hg ? C * F : 0, n3 = 2 * NBLK * C;
     for (int idx = lane; idx < n1 + n2 + n3; idx += WAVE) {
-      double acc = 0;
-      int p;
+      const T* src;
+      int p, t;
       if (idx < n1 + n2) {
         const int k2 = idx < n1 ? idx : idx - n1;
-        const int t = k2 / F, mu = k2 % F;
-        const T* src = (idx < n1 ? sX1 : sAl) + mu * LR;
-        acc = lds_sum_where(src, sTj, T(t), cnt);
+        t = k2 / F;
+        const int mu = k2 % F;
+        src = (idx < n1 ? sX1 : sAl) + mu * LR;
         if (idx < n1) p = GATE ? m.o_gtype + t * F + mu : m.o_rad + mu * (BB + C) + BB + t;
         else p = m.o_alpha + t * F + mu;
       } else {
         const int k3 = idx - n1 - n2;
-        const int b = k3 / (2 * C), w = (k3 / C) % 2, t = k3 % C;
-        const T* src = (w ? sDt : sDs) + b * LR;
-        acc = lds_sum_where(src, sTj, T(t), cnt);
+        const int b = k3 / (2 * C), w = (k3 / C) % 2;
+        t = k3 % C;
+        src = (w ? sDt : sDs) + b * LR;
         p = so + (2 * b + w) * CC + ti * C + t;
+      }
+      unsigned long long mk = s.tmask[t];
+      double acc = 0;
+      while (mk) {
+        const int k = __ffsll((long long)mk) - 1;
+        acc += (double)src[k];
+        mk &= mk - 1;
       }
       s.part[pidx(p, pn_oW, pn_nW)] += acc;
     }
