@@ -98,6 +98,7 @@ def train_section(path, steps, warmup, profile):
     t32, l32, g32, _ = _timed_train(model, batch, labels, "fp32", steps, warmup, False)
     nedges = model.last_edge_count()
     t64, l64, g64, _ = _timed_train(model, batch, labels, "fp64", steps, warmup, False)
+    tmx, lmx, gmx, _ = _timed_train(model, batch, labels, "mixed", steps, warmup, False)
     prof = _timed_train(model, batch, labels, "fp32", steps, 0, True)[3] if profile else {}
     out = dict(
         train_grad_frames_per_s=round(nf * steps / t32, 2),
@@ -109,6 +110,10 @@ def train_section(path, steps, warmup, profile):
                       "loss": "weighting structures, wE 1, wF 0.01, wS 0", "coefficients": len(g32)},
         train_errors_fp32_vs_fp64_gpu=dict(loss_rel=float(abs(l32 - l64) / abs(l64)),
                                            grad_rel=float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))),
+        train_mixed_frames_per_s=round(nf * steps / tmx, 2),
+        train_mixed_over_fp64=round(t64 / tmx, 3),
+        train_errors_mixed_vs_fp64_gpu=dict(loss_rel=float(abs(lmx - l64) / abs(l64)),
+                                            grad_rel=float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64))),
         train_step_survey_tflops=round(survey_grad_per_edge(dims) * nedges * steps / t32 / 1e12, 4),
     )
     if prof:

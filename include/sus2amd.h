@@ -42,8 +42,10 @@ enum sus2_status {
 };
 
 enum sus2_precision {
-  SUS2_FP64 = 0, /* all arithmetic in FP64 */
-  SUS2_FP32 = 1  /* FP32 bulk arithmetic, FP64 per-frame energy/virial reductions */
+  SUS2_FP64 = 0,  /* all arithmetic in FP64 */
+  SUS2_FP32 = 1,  /* FP32 bulk arithmetic, FP64 per-frame energy/virial reductions */
+  SUS2_MIXED = 2  /* objective calls only (sus2_loss_grad, sus2_train_lbfgs): FP64 EFS
+                     and loss residuals, FP32 parameter-gradient sweeps with FP64 sums */
 };
 
 typedef struct sus2_model sus2_model;

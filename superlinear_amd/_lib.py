@@ -16,6 +16,7 @@ _LIB = None
 SUS2_OK = 0
 SUS2_FP64 = 0
 SUS2_FP32 = 1
+SUS2_MIXED = 2
 STATUS = {1: "invalid", 2: "io", 3: "unsupported", 4: "device", 5: "range"}
 
 _vp = ctypes.c_void_p

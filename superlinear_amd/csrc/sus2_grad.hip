@@ -125,10 +125,24 @@ __global__ void k_fill(int n, double v, double* __restrict__ a) {
   if (i < n) a[i] = v;
 }
 
+// FP64 edge records -> FP32 (same order, neighbour and species bits repacked).
+__global__ void k_rec_narrow(long n, const Rec<double>* __restrict__ in, Rec<float>* __restrict__ out) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const Rec<double> q = in[e];
+  out[e] = Rec<float>{(float)q.x, (float)q.y, (float)q.z, rec_jbits(rec_j(q), rec_t(q), 0.0f)};
+}
+
 void loss_grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* e0,
                         const double* f0, const double* s0, int mode, double wE, double wF, double wS,
                         double* loss_host, double* grad, hipStream_t st) {
   if (mode < 0 || mode > 2) throw Error(SUS2_E_INVALID, "weighting must be 0 (vibrations), 1 (structures) or 2 (molecules)");
+  if (precision != SUS2_FP64 && precision != SUS2_FP32 && precision != SUS2_MIXED)
+    throw Error(SUS2_E_INVALID, "precision must be SUS2_FP64, SUS2_FP32 or SUS2_MIXED");
+  // mixed: the loss residuals come from an FP64 EFS, the sweeps run in FP32
+  const int efs_prec = precision == SUS2_MIXED ? SUS2_FP64 : precision;
+  const int grad_prec = precision == SUS2_MIXED ? SUS2_FP32 : precision;
+  if (precision == SUS2_MIXED) check_rec_limits(SUS2_FP32, b.natoms, dm.host->C);
   const int nf = b.nframes, N = b.natoms;
   if (nf == 0) {  // empty batch: zero objective and gradient
     const int P = dm.host->ncoeff;
@@ -148,13 +162,19 @@ void loss_grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, con
   out.energy = ws.lE.get<double>();
   out.force = ws.lF.get<double>();
   out.virial = ws.lV.get<double>();
-  efs_dispatch(dm, ws, precision, b, out, st);
+  efs_dispatch(dm, ws, efs_prec, b, out, st);
   double* lf = ws.lloss.get<double>();
   SUS2_LAUNCH(ws, st, "loss_frames", k_loss_frames, dim3(f1), dim3(256), 0, nf, b.frame_off, out.energy, out.force,
               out.virial, e0, f0, s0, mode, wE, wF, wS, ws.lew.get<double>(), ws.lfw.get<double>(),
               ws.lsw.get<double>(), lf + 1);
   SUS2_LAUNCH(ws, st, "loss_sum", k_loss_sum, dim3(1), dim3(256), 0, nf, lf + 1, lf);
-  grad_dispatch(dm, ws, precision, b, ws.lew.get<double>(), ws.lfw.get<double>(), ws.lsw.get<double>(), grad,
+  if (precision == SUS2_MIXED && ws.nedges > 0) {  // the neighbour list of the FP64 EFS, narrowed
+    ws.rec_alt.ensure(sizeof(Rec<float>) * (size_t)ws.nedges);
+    SUS2_LAUNCH(ws, st, "rec_narrow", k_rec_narrow, dim3((unsigned)((ws.nedges + 255) / 256)), dim3(256), 0,
+                ws.nedges, ws.rec.get<Rec<double>>(), ws.rec_alt.get<Rec<float>>());
+    ws.rec.swap(ws.rec_alt);
+  }
+  grad_dispatch(dm, ws, grad_prec, b, ws.lew.get<double>(), ws.lfw.get<double>(), ws.lsw.get<double>(), grad,
                 1.0 / nf, false, true, st);
   SUS2_CHECK(hipMemcpyAsync(ws.host_total, lf, sizeof(double), hipMemcpyDeviceToHost, st));
   SUS2_CHECK(hipStreamSynchronize(st));

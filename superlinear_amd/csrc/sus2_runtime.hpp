@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "sus2_device.hpp"
@@ -25,6 +26,10 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
   ~DevBuf();
   void ensure(size_t n);
+  void swap(DevBuf& o) {
+    std::swap(p, o.p);
+    std::swap(bytes, o.bytes);
+  }
   template <class U> U* get() const { return static_cast<U*>(p); }
 };
 
@@ -81,6 +86,7 @@ struct Workspace {
   // moments and tangents, per-edge gate messages, gathered h_bar, per-workgroup
   // FP64 partials (coefficients; gate weights W)
   DevBuf hdot, vgd, gmom, gmomd, msg, hbar, gpart, wpart, gfold;
+  DevBuf rec_alt;  // FP32 copy of FP64 edge records (mixed-precision objective)
   // training objective: EFS outputs, loss adjoints and per-frame losses (FP64)
   DevBuf lE, lF, lV, lew, lfw, lsw, lloss;
   long nedges = 0;

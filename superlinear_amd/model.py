@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 
-PRECISION = {"fp64": _lib.SUS2_FP64, "fp32": _lib.SUS2_FP32, 0: 0, 1: 1}
+PRECISION = {"fp64": _lib.SUS2_FP64, "fp32": _lib.SUS2_FP32, "mixed": _lib.SUS2_MIXED, 0: 0, 1: 1, 2: 2}
 DIM_NAMES = ("C", "L", "K", "B", "F", "M", "Q", "R", "O", "Qg", "Kw", "nbasic", "nprod", "gated",
              "species_readout", "H")
 OFF_NAMES = ("shift", "scal", "rad", "grad", "gscal", "alpha", "gtype", "W", "gw", "ow", "species", "read", "ncoeff")
