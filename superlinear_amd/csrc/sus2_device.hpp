@@ -100,8 +100,10 @@ struct Cfg {
 // Real Y_lm(u) for a unit vector u, l <= LL, flat index l^2+l+m; Condon-
 // Shortley phase cancelled, Y_1{-1,0,1} ~ (y, z, x).  Associated Legendre
 // part by the three-term recurrence in l; azimuthal part (ux + i uy)^m.
-template <class T, int LL>
-__device__ __forceinline__ void sh_values(T ux, T uy, T uz, T* Y) {
+// Values streamed to fn(l, m, Y) in (m, l) order, so a caller can consume
+// each one at once instead of holding all H.
+template <class T, int LL, class Fn>
+__device__ __forceinline__ void sh_stream_values(T ux, T uy, T uz, Fn&& fn) {
   T cm = T(1), sm = T(0);
 #pragma unroll
   for (int m = 0; m <= LL; ++m) {
@@ -117,12 +119,17 @@ __device__ __forceinline__ void sh_values(T ux, T uy, T uz, T* Y) {
       if (l == m) p = T(kYT.df[m]);
       else p = (T(2 * l - 1) * uz * p1 - T(l + m - 1) * p2) * T(1.0 / (l - m));
       const T n = T(kYT.n[hidx(l, m)]);
-      Y[hidx(l, m)] = n * p * cm;
-      if (m > 0) Y[hidx(l, -m)] = n * p * sm;
+      fn(l, m, n * p * cm);
+      if (m > 0) fn(l, -m, n * p * sm);
       p2 = p1;
       p1 = p;
     }
   }
+}
+
+template <class T, int LL>
+__device__ __forceinline__ void sh_values(T ux, T uy, T uz, T* Y) {
+  sh_stream_values<T, LL>(ux, uy, uz, [&](int l, int m, T v) { Y[hidx(l, m)] = v; });
 }
 
 // Y_lm(d/|d|) and grad_d Y_lm streamed to fn(l, m, Y, gx, gy, gz) in (m, l)
@@ -244,9 +251,11 @@ __device__ void load_species_tables(const DModel<typename Cf::T>& m, bool gate, 
 // rc, rc_{l>0} = rca; channels with l > 0 vanish for r >= rca.  Warp blocks
 // and basis size are compile-time, so each block's tanh and Chebyshev
 // recurrence is evaluated once and only for the channels that share it.
-template <class Cf, bool DER>
-__device__ __forceinline__ void radial(const DModel<typename Cf::T>& m, bool gate, const SpeciesTables<typename Cf::T>& t,
-                                       int pair, typename Cf::T r, typename Cf::T* R, typename Cf::T* dR) {
+// Each channel goes to fn(mu, R_mu, dR_mu/dr) as soon as it is formed.
+template <class Cf, bool DER, class Fn>
+__device__ __forceinline__ void radial_each(const DModel<typename Cf::T>& m, bool gate,
+                                            const SpeciesTables<typename Cf::T>& t, int pair, typename Cf::T r,
+                                            Fn&& fn) {
   using T = typename Cf::T;
   constexpr int L1 = Cf::L1, BB = Cf::BB;
   const bool own_rad = gate && m.o_grad >= 0;
@@ -287,11 +296,21 @@ __device__ __forceinline__ void radial(const DModel<typename Cf::T>& m, bool gat
           if (DER) sd += c[q] * Db[q];
         }
         const T f = l == 0 ? f0 : f1, g = l == 0 ? g0 : g1;
-        R[mu] = f * s;
-        if (DER) dR[mu] = g * s + f * sd;
+        fn(mu, f * s, DER ? g * s + f * sd : T(0));
       }
     }
   }
+}
+
+// All channels into R[F] (and dR[F] when DER).
+template <class Cf, bool DER>
+__device__ __forceinline__ void radial(const DModel<typename Cf::T>& m, bool gate, const SpeciesTables<typename Cf::T>& t,
+                                       int pair, typename Cf::T r, typename Cf::T* R, typename Cf::T* dR) {
+  using T = typename Cf::T;
+  radial_each<Cf, DER>(m, gate, t, pair, r, [&](int mu, T v, T dv) {
+    R[mu] = v;
+    if (DER) dR[mu] = dv;
+  });
 }
 
 // ------------------------------------------------------------------ wave utils

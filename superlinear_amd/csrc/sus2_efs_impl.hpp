@@ -92,6 +92,21 @@ __device__ void wave_contract(const DModel<T>& m, const DPass<T>& p, bool gate, 
   __syncthreads();
 }
 
+// Row loop of a graph layer: fn(r) for r = r0 + lane, r0 + lane + WAVE, ...
+// below r1, four rows at a time so that their schedule loads are in flight
+// together (the row data are the same for every atom, L2-resident).
+template <class Fn>
+__device__ __forceinline__ void wave_rows(int r0, int r1, Fn&& fn) {
+  int r = r0 + (int)threadIdx.x;
+  for (; r + 3 * WAVE < r1; r += 4 * WAVE) {
+    fn(r);
+    fn(r + WAVE);
+    fn(r + 2 * WAVE);
+    fn(r + 3 * WAVE);
+  }
+  for (; r < r1; r += WAVE) fn(r);
+}
+
 // Product layers with rows spread over the lanes, then per-output sums of the
 // staged row products in serialised row order (sus2_sl_kernel.cpp:308-313).
 template <class T>
@@ -100,7 +115,7 @@ __device__ void wave_graph_forward(const DPass<T>& p, T* v, T* scratch) {
   for (int l = 0; l < p.nfl; ++l) {
     const int o0 = p.fl_off[l], o1 = p.fl_off[l + 1];
     const int r0 = p.f_roff[o0], r1 = p.f_roff[o1];
-    for (int r = r0 + lane; r < r1; r += WAVE) scratch[r - r0] = p.f_c[r] * v[p.f_a[r]] * v[p.f_b[r]];
+    wave_rows(r0, r1, [&](int r) { scratch[r - r0] = p.f_c[r] * v[p.f_a[r]] * v[p.f_b[r]]; });
     __syncthreads();
     for (int o = o0 + lane; o < o1; o += WAVE) {
       T s = T(0);
@@ -120,7 +135,7 @@ __device__ void wave_graph_reverse(const DPass<T>& p, const T* v, T* a, T* scrat
   for (int l = 0; l < p.nrl; ++l) {
     const int n0 = p.rl_off[l], n1 = p.rl_off[l + 1];
     const int q0 = p.r_roff[n0], q1 = p.r_roff[n1];
-    for (int q = q0 + lane; q < q1; q += WAVE) scratch[q - q0] = p.r_c[q] * a[p.r_out[q]] * v[p.r_partner[q]];
+    wave_rows(q0, q1, [&](int q) { scratch[q - q0] = p.r_c[q] * a[p.r_out[q]] * v[p.r_partner[q]]; });
     __syncthreads();
     for (int k = n0 + lane; k < n1; k += WAVE) {
       const int n = p.r_node[k];
@@ -170,7 +185,8 @@ __host__ __device__ inline size_t forward_region(int M, int sc) {
 
 template <class Cf>
 size_t forward_lds(const DModel<typename Cf::T>& m) {
-  const int sc = std::max(graph_scratch(m.gp), graph_scratch(m.op));
+  // the scratch also holds the gate scalars S [Qg] of the gate pass
+  const int sc = std::max({graph_scratch(m.gp), graph_scratch(m.op), m.Qg});
   return sizeof(typename Cf::T) * (species_tables_size<Cf>(m.C) + forward_region<Cf>(m.M, sc));
 }
 
@@ -207,21 +223,31 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
       const T r = dsqrt(q.x * q.x + q.y * q.y + q.z * q.z);
       const T inv = T(1) / r;
       const int tj = rec_t(q);
-      T Rv[F], Y[H], hj[F];
-      if (!GATE && m.gated) load_row<T, F>(h + (size_t)j * F, hj);  // issued before the basis work
-      radial<Cf, false>(m, GATE, tab, ti * m.C + tj, r, Rv, nullptr);
-      sh_values<T, LL>(q.x * inv, q.y * inv, q.z * inv, Y);
       const T* x0i = tab.x0 + ti * F;
       const T* x0j = tab.x0 + tj * F;
       const T* alj = tab.alpha + tj * F;
+      if constexpr (F > 16) {  // wide shapes: each channel and harmonic staged as soon as formed
+        radial_each<Cf, false>(m, GATE, tab, ti * m.C + tj, r, [&](int mu, T R, T) {
+          T x1 = x0j[mu];
+          if (!GATE && m.gated) x1 += alj[mu] * h[(size_t)j * F + mu];
+          sR[mu * LR + lane] = x0i[mu] * x1 * R;
+        });
+        sh_stream_values<T, LL>(q.x * inv, q.y * inv, q.z * inv,
+                                [&](int l, int mm, T Y) { sY[hidx(l, mm) * LR + lane] = Y; });
+      } else {
+        T Rv[F], Y[H], hj[F];
+        if (!GATE && m.gated) load_row<T, F>(h + (size_t)j * F, hj);  // issued before the basis work
+        radial<Cf, false>(m, GATE, tab, ti * m.C + tj, r, Rv, nullptr);
+        sh_values<T, LL>(q.x * inv, q.y * inv, q.z * inv, Y);
 #pragma unroll
-      for (int mu = 0; mu < F; ++mu) {
-        T x1 = x0j[mu];
-        if (!GATE && m.gated) x1 += alj[mu] * hj[mu];
-        sR[mu * LR + lane] = x0i[mu] * x1 * Rv[mu];
+        for (int mu = 0; mu < F; ++mu) {
+          T x1 = x0j[mu];
+          if (!GATE && m.gated) x1 += alj[mu] * hj[mu];
+          sR[mu * LR + lane] = x0i[mu] * x1 * Rv[mu];
+        }
+#pragma unroll
+        for (int hh = 0; hh < H; ++hh) sY[hh * LR + lane] = Y[hh];
       }
-#pragma unroll
-      for (int hh = 0; hh < H; ++hh) sY[hh * LR + lane] = Y[hh];
     }
     __syncthreads();
     const int cnt = min(WAVE, e1 - base);
@@ -249,11 +275,22 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
   wave_contract<T, KK>(m, p, GATE, mom, v);
   wave_graph_forward(p, v, scratch);
   if (GATE) {
-    // h_i = W S_i (lanes over mu), node values of the gate range for the reverse
+    // h_i = W S_i (lanes over mu; S gathered into the free scratch first, four
+    // partial sums in flight), node values of the gate range for the reverse
+    T* sS = scratch;
+    for (int q = lane; q < m.Qg; q += WAVE) sS[q] = v[m.gate_nodes[q]];
+    __syncthreads();
     for (int mu = lane; mu < F; mu += WAVE) {
-      T s = T(0);
-      for (int q = 0; q < m.Qg; ++q) s += m.W(mu, q) * v[m.gate_nodes[q]];
-      h[(size_t)i * F + mu] = s;
+      T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+      int q = 0;
+      for (; q + 3 < m.Qg; q += 4) {
+        s0 += m.W(mu, q) * sS[q];
+        s1 += m.W(mu, q + 1) * sS[q + 1];
+        s2 += m.W(mu, q + 2) * sS[q + 2];
+        s3 += m.W(mu, q + 3) * sS[q + 3];
+      }
+      for (; q < m.Qg; ++q) s0 += m.W(mu, q) * sS[q];
+      h[(size_t)i * F + mu] = (s0 + s1) + (s2 + s3);
     }
     T* out = vg + (size_t)i * p.node_cnt;
     for (int n = lane; n < p.node_cnt; n += WAVE) out[n] = v[p.node_lo + n];
@@ -390,10 +427,14 @@ __device__ void gate_reverse_atom(const DModel<T>& m, int i, const T* __restrict
   }
   __syncthreads();
   for (int q = lane; q < m.Qg; q += WAVE) {
-    T s = T(0);
+    T s0 = T(0), s1 = T(0);
 #pragma unroll
-    for (int mu = 0; mu < F; ++mu) s += hbs[mu] * m.W(mu, q);
-    a[m.gate_nodes[q]] = s;
+    for (int mu = 0; mu + 1 < F; mu += 2) {
+      s0 += hbs[mu] * m.W(mu, q);
+      s1 += hbs[mu + 1] * m.W(mu + 1, q);
+    }
+    if (F % 2) s0 += hbs[F - 1] * m.W(F - 1, q);
+    a[m.gate_nodes[q]] = s0 + s1;
   }
   __syncthreads();
   wave_graph_reverse(p, v, a, scratch);

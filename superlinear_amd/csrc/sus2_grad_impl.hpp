@@ -158,12 +158,12 @@ __device__ void wave_graph_forward_tan(const DPass<T>& p, T* v, T* vd, T* s1, T*
   for (int l = 0; l < p.nfl; ++l) {
     const int o0 = p.fl_off[l], o1 = p.fl_off[l + 1];
     const int r0 = p.f_roff[o0], r1 = p.f_roff[o1];
-    for (int r = r0 + lane; r < r1; r += WAVE) {
+    wave_rows(r0, r1, [&](int r) {
       const T c = p.f_c[r];
       const int a = p.f_a[r], b = p.f_b[r];
       s1[r - r0] = c * v[a] * v[b];
       s2[r - r0] = c * (vd[a] * v[b] + v[a] * vd[b]);
-    }
+    });
     __syncthreads();
     for (int o = o0 + lane; o < o1; o += WAVE) {
       T x = T(0), y = T(0);
@@ -185,12 +185,12 @@ __device__ void wave_graph_reverse_tan(const DPass<T>& p, const T* v, const T* v
   for (int l = 0; l < p.nrl; ++l) {
     const int n0 = p.rl_off[l], n1 = p.rl_off[l + 1];
     const int q0 = p.r_roff[n0], q1 = p.r_roff[n1];
-    for (int q = q0 + lane; q < q1; q += WAVE) {
+    wave_rows(q0, q1, [&](int q) {
       const T c = p.r_c[q];
       const int o = p.r_out[q], pt = p.r_partner[q];
       s1[q - q0] = c * (P[o] * v[pt] + Q[o] * vd[pt]);
       s2[q - q0] = c * Q[o] * v[pt];
-    }
+    });
     __syncthreads();
     for (int k = n0 + lane; k < n1; k += WAVE) {
       const int n = p.r_node[k];
