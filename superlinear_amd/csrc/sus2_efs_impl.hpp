@@ -322,10 +322,9 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
 // outer pass of an Effect model the gate-adjoint message
 //   u_e,mu = tau_{t_i,mu} R_mu(r; t_i, t_j) sum_m abar_i[k,l,m] Y_lm(d_e),
 // which the neighbour j gathers into h_bar_j (times alpha_{t_j,mu}).
-// Outer pass, FP32, L <= 3: at most 168 VGPRs (no spills), so three waves
-// share each SIMD.
+// FP32, L <= 3: at most 168 VGPRs (no spills), so three waves share each SIMD.
 template <class Cf, bool GATE>
-constexpr int edge_backward_waves() { return (!GATE && sizeof(typename Cf::T) == 4 && Cf::LL <= 3) ? 3 : 1; }
+constexpr int edge_backward_waves() { return (sizeof(typename Cf::T) == 4 && Cf::LL <= 3) ? 3 : 1; }
 
 template <class Cf, bool GATE, class T = typename Cf::T>
 __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edge_backward(DModel<T> m, const int32_t* __restrict__ types,
@@ -377,6 +376,9 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
     T sY = T(0), gx = T(0), gy = T(0), gz = T(0);
     sh_stream_grad<T, LL>(ux, uy, uz, inv, [&](int l, int mm, T Y, T yx, T yy, T yz) {
       const int hh = hidx(l, mm);
+      // gate pass: keep the moment-adjoint LDS reads of this (l, m) here;
+      // hoisted over the whole unrolled stream they would hold K H registers
+      if constexpr (GATE) asm volatile("" ::: "memory");
 #pragma unroll
       for (int k = 0; k < KK; ++k) {
         const int mu = k * L1 + l;
