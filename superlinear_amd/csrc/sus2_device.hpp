@@ -182,15 +182,15 @@ __device__ __forceinline__ void sh_stream_grad(T ux, T uy, T uz, T inv_r, Fn&& f
 // A row of F values (a neighbour's gate signal): 16-byte vector loads when
 // the rows are 16-byte aligned (F * sizeof(T) a multiple of 16 on a 256-byte
 // aligned buffer), else scalar loads.
-template <class T, int F>
+template <class T, int F, bool ALIGNED = true>
 __device__ __forceinline__ void load_row(const T* __restrict__ p, T* out) {
-  if constexpr (sizeof(T) == 4 && F % 4 == 0) {
+  if constexpr (ALIGNED && sizeof(T) == 4 && F % 4 == 0) {
 #pragma unroll
     for (int k = 0; k < F / 4; ++k) {
       const float4 v = reinterpret_cast<const float4*>(p)[k];
       out[4 * k] = v.x; out[4 * k + 1] = v.y; out[4 * k + 2] = v.z; out[4 * k + 3] = v.w;
     }
-  } else if constexpr (sizeof(T) == 8 && F % 2 == 0) {
+  } else if constexpr (ALIGNED && sizeof(T) == 8 && F % 2 == 0) {
 #pragma unroll
     for (int k = 0; k < F / 2; ++k) {
       const double2 v = reinterpret_cast<const double2*>(p)[k];
@@ -251,8 +251,9 @@ __device__ void load_species_tables(const DModel<typename Cf::T>& m, bool gate, 
 // rc, rc_{l>0} = rca; channels with l > 0 vanish for r >= rca.  Warp blocks
 // and basis size are compile-time, so each block's tanh and Chebyshev
 // recurrence is evaluated once and only for the channels that share it.
-// Each channel goes to fn(mu, R_mu, dR_mu/dr) as soon as it is formed.
-template <class Cf, bool DER, class Fn>
+// Each channel goes to fn(mu, R_mu, dR_mu/dr) as soon as it is formed; only
+// the warp blocks [B0, B1) are evaluated.
+template <class Cf, bool DER, int B0 = 0, int B1 = Cf::NBLK, class Fn>
 __device__ __forceinline__ void radial_each(const DModel<typename Cf::T>& m, bool gate,
                                             const SpeciesTables<typename Cf::T>& t, int pair, typename Cf::T r,
                                             Fn&& fn) {
@@ -266,7 +267,7 @@ __device__ __forceinline__ void radial_each(const DModel<typename Cf::T>& m, boo
   const bool act = r < m.rca;
   const T d1 = r - m.rca, f1 = act ? d1 * d1 : T(0), g1 = act ? T(2) * d1 : T(0);
 #pragma unroll
-  for (int b = 0; b < Cf::NBLK; ++b) {
+  for (int b = B0; b < B1; ++b) {
     const T slope = t.slope[b * CC + pair], shift = t.shift[b * CC + pair];
     const T xi = dtanh(slope * (r - shift) * T(0.5));
     const T dxi = (T(1) - xi * xi) * slope * T(0.5);

@@ -54,6 +54,19 @@ __device__ __forceinline__ void cheb3(T S, T xi, T* Tb, T* T1, T* T2) {
 // gate-weight block W (reduced separately by k_grad_W).
 __host__ __device__ __forceinline__ int pidx(int p, int oW, int nW) { return (oW >= 0 && p >= oW) ? p - nW : p; }
 
+// Channel halves of the edge sweeps: with one warp block per channel k
+// (scaling-map mode 2) and an even K, every sweep runs over an atom's edges
+// twice, for the channels k in [0, K/2) and [K/2, K).  That halves the staged
+// LDS rows (more workgroups resident per CU) and the live radial values per
+// lane, at the price of the harmonics being evaluated twice.
+template <class Cf>
+struct Halves {
+  static constexpr int NH = (Cf::SM == 2 && Cf::KK % 2 == 0) ? 2 : 1;
+  static constexpr int KP = Cf::KK / NH;           // channels k per half
+  static constexpr int FP = KP * Cf::L1;           // radial functions per half
+  static constexpr int BP = NH == 2 ? KP : Cf::NBLK;  // warp blocks per half
+};
+
 // LDS layout shared by host (sizes) and device (carving): the species tables,
 // the canonical adjoints Pc, Qc, the type-free moments At, Atd and the gate
 // adjoint row, then one region used in turn by the edge staging of sweep 1,
@@ -83,8 +96,9 @@ struct GradLds {
 
 template <class Cf>
 __host__ __device__ inline int grad_stage_rows(bool sweep2, bool outer_gated) {
-  const int r1 = 2 * Cf::F + 2 * Cf::H;
-  const int r2 = 4 * Cf::NBLK + (outer_gated ? 4 : 3) * Cf::F;
+  using Hv = Halves<Cf>;
+  const int r1 = 2 * Hv::FP + 2 * Cf::H;
+  const int r2 = 4 * Hv::BP + (outer_gated ? 4 : 3) * Hv::FP;
   return sweep2 ? (r1 > r2 ? r1 : r2) : r1;
 }
 
@@ -205,27 +219,37 @@ __device__ void wave_graph_reverse_tan(const DPass<T>& p, const T* v, const T* v
 }
 
 // ------------------------------------------------------------------ sweep 1
-// Moments and tangents of one atom: At = sum_e x1 R Y, Atd = sum_e [x1 (R' w_r Y
-// + R grad Y . w) + x1_dot R Y]  (the centre factor x0 is applied afterwards:
-// A = x0 At), with x1 = neighbour type factor (+ alpha h_j for the outer pass
-// of an Effect model) and x1_dot = alpha h_dot_j.
-template <class Cf, bool GATE>
+// Moments and tangents of one atom for the channels k of half HALF:
+// At = sum_e x1 R Y, Atd = sum_e [x1 (R' w_r Y + R grad Y . w) + x1_dot R Y]
+// (the centre factor x0 is applied afterwards: A = x0 At), with x1 =
+// neighbour type factor (+ alpha h_j for the outer pass of an Effect model)
+// and x1_dot = alpha h_dot_j.  The moment dot products over a staged batch
+// use two lanes per moment (one per half of the batch) when a half has at
+// most WAVE / 2 moments.
+template <class Cf, bool GATE, int HALF>
 __device__ void grad_sweep_moments(const DModel<typename Cf::T>& m, const SpeciesTables<typename Cf::T>& tab,
-                                   const int32_t* __restrict__ types, const Rec<typename Cf::T>* __restrict__ rec,
-                                   const double* __restrict__ frc_w, const typename Cf::T* Sw, const double* fwi,
+                                   const Rec<typename Cf::T>* __restrict__ rec, const double* __restrict__ frc_w,
+                                   const typename Cf::T* Sw, const double* fwi,
                                    const typename Cf::T* __restrict__ h, const typename Cf::T* __restrict__ hd, int ti,
                                    int e0, int e1, typename Cf::T* stage, typename Cf::T* At, typename Cf::T* Atd) {
   using T = typename Cf::T;
-  constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, H = Cf::H, F = Cf::F, KH = Cf::KH, NM = Cf::NM;
+  using Hv = Halves<Cf>;
+  constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, H = Cf::H, F = Cf::F;
+  constexpr int KP = Hv::KP, FP = Hv::FP, K0 = HALF * KP, MU0 = K0 * L1;
+  constexpr int B0 = Hv::NH == 2 ? K0 : 0, B1 = Hv::NH == 2 ? K0 + KP : Cf::NBLK;
+  constexpr bool ROW_ALIGNED = (MU0 * sizeof(T)) % 16 == 0;
+  constexpr int NMH = H * KP;                        // moments of the half
+  constexpr int SPLIT = NMH <= WAVE / 2 ? 2 : 1;     // lanes per moment: l and l + WAVE / 2
+  constexpr int NMT = SPLIT == 2 ? 1 : (NMH + WAVE - 1) / WAVE;
   const int lane = threadIdx.x;
   const bool hg = !GATE && m.gated;
-  T* sS1 = stage;            // [F][LR]  x1 R
-  T* sS2 = sS1 + F * LR;     // [F][LR]  x1 R' w_r + x1_dot R
-  T* sY = sS2 + F * LR;      // [H][LR]
-  T* sYw = sY + H * LR;      // [H][LR]  grad Y . w
-  T acc[NM], accd[NM];
+  T* sS1 = stage;             // [FP][LR]  x1 R
+  T* sS2 = sS1 + FP * LR;     // [FP][LR]  x1 R' w_r + x1_dot R
+  T* sY = sS2 + FP * LR;      // [H][LR]
+  T* sYw = sY + H * LR;       // [H][LR]  grad Y . w
+  T acc[NMT], accd[NMT];
 #pragma unroll
-  for (int t = 0; t < NM; ++t) { acc[t] = T(0); accd[t] = T(0); }
+  for (int t = 0; t < NMT; ++t) { acc[t] = T(0); accd[t] = T(0); }
   for (int base = e0; base < e1; base += WAVE) {
     const int e = base + lane;
     if (e < e1) {
@@ -241,23 +265,27 @@ __device__ void grad_sweep_moments(const DModel<typename Cf::T>& m, const Specie
       const T wy = T(fwi[1] - fj[1]) + Sw[3] * dx + Sw[4] * dy + Sw[5] * dz;
       const T wz = T(fwi[2] - fj[2]) + Sw[6] * dx + Sw[7] * dy + Sw[8] * dz;
       const T wr = ux * wx + uy * wy + uz * wz;
-      T R[F], dR[F], hj[F], hdj[F];
+      T R[FP], dR[FP], hj[FP], hdj[FP];
       if (hg) {  // issued before the basis work
-        load_row<T, F>(h + (size_t)j * F, hj);
-        load_row<T, F>(hd + (size_t)j * F, hdj);
+        load_row<T, FP, ROW_ALIGNED>(h + (size_t)j * F + MU0, hj);
+        load_row<T, FP, ROW_ALIGNED>(hd + (size_t)j * F + MU0, hdj);
       }
-      radial<Cf, true>(m, GATE, tab, ti * m.C + tj, r, R, dR);
-      const T* x0j = tab.x0 + tj * F;
-      const T* alj = tab.alpha + tj * F;
+      radial_each<Cf, true, B0, B1>(m, GATE, tab, ti * m.C + tj, r, [&](int mu, T v, T dv) {
+        if (mu < MU0 || mu >= MU0 + FP) return;
+        R[mu - MU0] = v;
+        dR[mu - MU0] = dv;
+      });
+      const T* x0j = tab.x0 + tj * F + MU0;
+      const T* alj = tab.alpha + tj * F + MU0;
 #pragma unroll
-      for (int mu = 0; mu < F; ++mu) {
-        T x1 = x0j[mu], xd = T(0);
+      for (int c = 0; c < FP; ++c) {
+        T x1 = x0j[c], xd = T(0);
         if (hg) {
-          x1 += alj[mu] * hj[mu];
-          xd = alj[mu] * hdj[mu];
+          x1 += alj[c] * hj[c];
+          xd = alj[c] * hdj[c];
         }
-        sS1[mu * LR + lane] = x1 * R[mu];
-        sS2[mu * LR + lane] = x1 * dR[mu] * wr + xd * R[mu];
+        sS1[c * LR + lane] = x1 * R[c];
+        sS2[c * LR + lane] = x1 * dR[c] * wr + xd * R[c];
       }
       sh_stream_grad<T, LL>(ux, uy, uz, inv, [&](int l, int mm, T Y, T yx, T yy, T yz) {
         const int hh = hidx(l, mm);
@@ -268,25 +296,37 @@ __device__ void grad_sweep_moments(const DModel<typename Cf::T>& m, const Specie
     __syncthreads();
     const int cnt = min(WAVE, e1 - base);
 #pragma unroll
-    for (int t = 0; t < NM; ++t) {
-      const int c = lane + WAVE * t;
-      if (c < KH) {
-        const int hh = c / KK, k = c % KK;
-        const int mu = k * L1 + l_of(hh);
-        const T* p1 = sS1 + mu * LR;
-        const T* p2 = sS2 + mu * LR;
-        const T* py = sY + hh * LR;
-        const T* pw = sYw + hh * LR;
-        acc[t] += lds_dot(p1, py, cnt);
-        accd[t] += lds_dot(p2, py, cnt) + lds_dot(p1, pw, cnt);
+    for (int t = 0; t < NMT; ++t) {
+      const int idx = lane + WAVE * t;
+      const int mloc = SPLIT == 2 ? lane % (WAVE / 2) : idx, prt = SPLIT == 2 ? lane / (WAVE / 2) : 0;
+      if (mloc < NMH) {
+        const int hh = mloc / KP, kk = mloc % KP;
+        const int c = kk * L1 + l_of(hh);
+        const int lo = (SPLIT == 2 && prt) ? (cnt + 1) / 2 : 0;
+        const int hi = (SPLIT == 2 && !prt) ? (cnt + 1) / 2 : cnt;
+        const T* p1 = sS1 + c * LR + lo;
+        const T* p2 = sS2 + c * LR + lo;
+        const T* py = sY + hh * LR + lo;
+        const T* pw = sYw + hh * LR + lo;
+        acc[t] += lds_dot(p1, py, hi - lo);
+        accd[t] += lds_dot(p2, py, hi - lo) + lds_dot(p1, pw, hi - lo);
       }
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int t = 0; t < NM; ++t) {
-    const int c = lane + WAVE * t;
-    if (c < KH) { At[c] = acc[t]; Atd[c] = accd[t]; }
+  for (int t = 0; t < NMT; ++t) {
+    T a = acc[t], ad = accd[t];
+    if constexpr (SPLIT == 2) {  // the two halves of the batches (lanes idx and idx + WAVE / 2)
+      a += __shfl_xor(a, WAVE / 2, WAVE);
+      ad += __shfl_xor(ad, WAVE / 2, WAVE);
+    }
+    const int idx = lane + WAVE * t;  // with SPLIT == 2 the lanes below WAVE / 2 write
+    if (idx < NMH) {
+      const int hh = idx / KP, kk = idx % KP;
+      At[hh * KK + K0 + kk] = a;
+      Atd[hh * KK + K0 + kk] = ad;
+    }
   }
 }
 
@@ -297,36 +337,40 @@ __device__ void grad_sweep_moments(const DModel<typename Cf::T>& m, const Specie
 // and the radial coefficient, type, warp (slope/shift) and alpha gradients
 // follow from R = f C0, R' = g C0 + f C1 xi_r with C_k = sum_b c_b d^k Tb/dxi^k.
 // Outer pass of an Effect model: messages (alpha x0 Z, alpha x0 R q) per edge.
-template <class Cf, bool GATE>
+template <class Cf, bool GATE, int HALF>
 __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const SpeciesTables<typename Cf::T>& tab,
-                                  const int32_t* __restrict__ types, const Rec<typename Cf::T>* __restrict__ rec,
+                                  const Rec<typename Cf::T>* __restrict__ rec,
                                   const double* __restrict__ frc_w, const typename Cf::T* Sw, const double* fwi,
                                   const typename Cf::T* __restrict__ h, const typename Cf::T* __restrict__ hd,
                                   typename Cf::T* __restrict__ msg, const int* __restrict__ in_pos, int ti,
                                   int e0, int e1,
                                   const GradSmem<typename Cf::T>& s, double* creg, int pn_oW, int pn_nW) {
   using T = typename Cf::T;
+  using Hv = Halves<Cf>;
   constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, F = Cf::F, BB = Cf::BB, NBLK = Cf::NBLK;
+  constexpr int KP = Hv::KP, FP = Hv::FP, BP = Hv::BP, K0 = HALF * KP, MU0 = K0 * L1;
+  constexpr int B0 = Hv::NH == 2 ? K0 : 0;  // first warp block of the half
   const int lane = threadIdx.x, C = m.C, CC = C * C;
   const bool hg = !GATE && m.gated;
-  T* sXi = s.stage;             // [NBLK][LR]
-  T* sXr = sXi + NBLK * LR;     // [NBLK][LR]
-  T* sDs = sXr + NBLK * LR;     // [NBLK][LR]
-  T* sDt = sDs + NBLK * LR;     // [NBLK][LR]
-  T* sCv = sDt + NBLK * LR;     // [F][LR]
-  T* sCd = sCv + F * LR;        // [F][LR]
-  T* sX1 = sCd + F * LR;        // [F][LR]  x0 Z  (d Phi / d x1)
-  T* sAl = sX1 + F * LR;        // [F][LR]  x0 (Z h_j + R q h_dot_j)  (outer, gated)
+  // staged rows of the half: blocks b - B0, channels mu - MU0
+  T* sXi = s.stage;             // [BP][LR]
+  T* sXr = sXi + BP * LR;       // [BP][LR]
+  T* sDs = sXr + BP * LR;       // [BP][LR]
+  T* sDt = sDs + BP * LR;       // [BP][LR]
+  T* sCv = sDt + BP * LR;       // [FP][LR]
+  T* sCd = sCv + FP * LR;       // [FP][LR]
+  T* sX1 = sCd + FP * LR;       // [FP][LR]  x0 Z  (d Phi / d x1)
+  T* sAl = sX1 + FP * LR;       // [FP][LR]  x0 (Z h_j + R q h_dot_j)  (outer, gated)
   const bool own_rad = GATE && m.o_grad >= 0;
   const T* __restrict__ coef = m.theta + (own_rad ? m.o_grad : m.o_rad);
   const int stride = own_rad ? BB : BB + C;
   const int so = (GATE && m.o_gscal >= 0) ? m.o_gscal : m.o_scal;
   const T* x0 = tab.x0 + ti * F;
-  // c-gradient lanes: (mu, subset)
-  constexpr int NSUB = WAVE / F;
-  const int cmu = lane % F, csub = lane / F;
+  // c-gradient lanes: (channel of the half, subset)
+  constexpr int NSUB = WAVE / FP;
+  const int cmu = lane % FP, csub = lane / FP;
   const bool cact = csub < NSUB;
-  const int cblk = Cf::blk(cmu / L1, cmu % L1);
+  const int cblk = Hv::NH == 2 ? cmu / L1 : Cf::blk(cmu / L1, cmu % L1);
   for (int base = e0; base < e1; base += WAVE) {
     const int e = base + lane;
     int lane_t = -1;  // species of this lane's edge (none past the list)
@@ -388,11 +432,11 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
         const T Cv = Gv * f + Gd * g, Cd = Gd * f;
         ds += xs * C1 * Cv + Cd * (C2 * xs * xr + C1 * xrs);
         dt += xt * C1 * Cv + Cd * (C2 * xt * xr + C1 * xrt);
-        sCv[mu * LR + lane] = Cv;
-        sCd[mu * LR + lane] = Cd;
-        sX1[mu * LR + lane] = x0[mu] * Z;
+        sCv[(mu - MU0) * LR + lane] = Cv;
+        sCd[(mu - MU0) * LR + lane] = Cd;
+        sX1[(mu - MU0) * LR + lane] = x0[mu] * Z;
         if (hg) {
-          sAl[mu * LR + lane] = x0[mu] * (Z * hj + Rq * hdj);
+          sAl[(mu - MU0) * LR + lane] = x0[mu] * (Z * hj + Rq * hdj);
           T* mo = msg + (size_t)in_pos[e] * 2 * F;
           mo[mu] = alj[mu] * x0[mu] * Z;
           mo[F + mu] = alj[mu] * x0[mu] * Rq;
@@ -413,19 +457,19 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
         Ywv[hh] = yx * wx + yy * wy + yz * wz;
       });
       if constexpr (Cf::SM == 2) {
-        // blocks are the channels k: one block's basis live at a time
+        // blocks are the channels k of the half: one block's basis live at a time
 #pragma unroll 1
-        for (int k = 0; k < KK; ++k) {
+        for (int k = K0; k < K0 + KP; ++k) {
           T Tb[BB], T1[BB], T2[BB], xi, xr, xs, xt, xrs, xrt;
           block_basis(k, Tb, T1, T2, xi, xr, xs, xt, xrs, xrt);
           T ds = T(0), dt = T(0);
 #pragma unroll
           for (int l = 0; l < L1; ++l)
             channel(k * L1 + l, k, l, Yv, Ywv, Tb, T1, T2, xr, xs, xt, xrs, xrt, ds, dt);
-          sXi[k * LR + lane] = xi;
-          sXr[k * LR + lane] = xr;
-          sDs[k * LR + lane] = ds;
-          sDt[k * LR + lane] = dt;
+          sXi[(k - B0) * LR + lane] = xi;
+          sXr[(k - B0) * LR + lane] = xr;
+          sDs[(k - B0) * LR + lane] = ds;
+          sDt[(k - B0) * LR + lane] = dt;
         }
       } else {
 #pragma unroll
@@ -476,22 +520,22 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
     }
     // type-bucketed sums: neighbour type factor, alpha, warp slope/shift; each
     // lane walks the staged edges of its species in edge order (mask bits)
-    const int n1 = C * F, n2 = hg ? C * F : 0, n3 = 2 * NBLK * C;
+    const int n1 = C * FP, n2 = hg ? C * FP : 0, n3 = 2 * BP * C;
     for (int idx = lane; idx < n1 + n2 + n3; idx += WAVE) {
       const T* src;
       int p, t;
       if (idx < n1 + n2) {
         const int k2 = idx < n1 ? idx : idx - n1;
-        t = k2 / F;
-        const int mu = k2 % F;
-        src = (idx < n1 ? sX1 : sAl) + mu * LR;
+        t = k2 / FP;
+        const int c = k2 % FP, mu = MU0 + c;
+        src = (idx < n1 ? sX1 : sAl) + c * LR;
         if (idx < n1) p = GATE ? m.o_gtype + t * F + mu : m.o_rad + mu * (BB + C) + BB + t;
         else p = m.o_alpha + t * F + mu;
       } else {
         const int k3 = idx - n1 - n2;
-        const int b = k3 / (2 * C), w = (k3 / C) % 2;
+        const int bl = k3 / (2 * C), w = (k3 / C) % 2, b = B0 + bl;
         t = k3 % C;
-        src = (w ? sDt : sDs) + b * LR;
+        src = (w ? sDt : sDs) + bl * LR;
         p = so + (2 * b + w) * CC + ti * C + t;
       }
       unsigned long long mk = s.tmask[t];
@@ -584,8 +628,11 @@ __global__ __launch_bounds__(WAVE) void k_grad_gate_forward(DModel<T> m, GradLds
   double fwi[3];
   load_atom_weights(gw, i, Sw, fwi);
   __syncthreads();
-  grad_sweep_moments<Cf, true>(m, tab, types, rec, gw.frc_w, Sw, fwi, nullptr, nullptr, ti, nbr_off[i],
-                               nbr_off[i + 1], s.stage, s.At, s.Atd);
+  grad_sweep_moments<Cf, true, 0>(m, tab, rec, gw.frc_w, Sw, fwi, nullptr, nullptr, ti, nbr_off[i], nbr_off[i + 1],
+                                  s.stage, s.At, s.Atd);
+  if constexpr (Halves<Cf>::NH > 1)
+    grad_sweep_moments<Cf, true, 1>(m, tab, rec, gw.frc_w, Sw, fwi, nullptr, nullptr, ti, nbr_off[i],
+                                    nbr_off[i + 1], s.stage, s.At, s.Atd);
   __syncthreads();
   for (int c = lane; c < KH; c += WAVE) {
     gmom[(size_t)i * KH + c] = s.At[c];
@@ -618,20 +665,25 @@ __device__ __forceinline__ void zero_part(double* part, int pn) {
   for (int k = threadIdx.x; k < pn; k += WAVE) part[k] = 0.0;
 }
 
-// Fold the per-lane radial-coefficient registers into the partial vector
-// (subsets one after another), then write the partial row.
+// Fold the per-lane radial-coefficient registers (creg[half][b]) into the
+// partial vector (subsets one after another), then write the partial row.
 template <class Cf>
 __device__ void flush_partial(const DModel<typename Cf::T>& m, bool gate, const double* creg, double* part, int pn,
                               int oW, int nW, double* __restrict__ out_row) {
-  constexpr int F = Cf::F, BB = Cf::BB, NSUB = WAVE / F;
-  const int lane = threadIdx.x, cmu = lane % F, csub = lane / F;
+  using Hv = Halves<Cf>;
+  constexpr int FP = Hv::FP, BB = Cf::BB, NSUB = WAVE / FP;
+  const int lane = threadIdx.x, cmu = lane % FP, csub = lane / FP;
   const bool own_rad = gate && m.o_grad >= 0;
   const int base = own_rad ? m.o_grad : m.o_rad, stride = own_rad ? BB : BB + m.C;
-  for (int sub = 0; sub < NSUB; ++sub) {
-    if (csub == sub)
 #pragma unroll
-      for (int b = 0; b < BB; ++b) part[pidx(base + cmu * stride + b, oW, nW)] += creg[b];
-    __syncthreads();
+  for (int hf = 0; hf < Hv::NH; ++hf) {
+    const int mu = hf * FP + cmu;
+    for (int sub = 0; sub < NSUB; ++sub) {
+      if (csub == sub)
+#pragma unroll
+        for (int b = 0; b < BB; ++b) part[pidx(base + mu * stride + b, oW, nW)] += creg[hf * BB + b];
+      __syncthreads();
+    }
   }
   for (int k = lane; k < pn; k += WAVE) out_row[k] = part[k];
 }
@@ -650,9 +702,10 @@ __global__ __launch_bounds__(WAVE) void k_grad_outer(DModel<T> m, GradLds<Cf> L,
   SpeciesTables<T> tab;
   load_species_tables<Cf>(m, false, s.tabs, tab);
   zero_part(s.part, L.pn);
-  double creg[BB];
+  constexpr int NH = Halves<Cf>::NH;
+  double creg[NH * BB];
 #pragma unroll
-  for (int b = 0; b < BB; ++b) creg[b] = 0.0;
+  for (int b = 0; b < NH * BB; ++b) creg[b] = 0.0;
   const DPass<T>& p = m.op;
   __syncthreads();
   for (int i = blockIdx.x; i < natoms; i += gridDim.x) {
@@ -661,7 +714,9 @@ __global__ __launch_bounds__(WAVE) void k_grad_outer(DModel<T> m, GradLds<Cf> L,
     double fwi[3];
     load_atom_weights(gw, i, Sw, fwi);
     const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
-    grad_sweep_moments<Cf, false>(m, tab, types, rec, gw.frc_w, Sw, fwi, h, hd, ti, e0, e1, s.stage, s.At, s.Atd);
+    grad_sweep_moments<Cf, false, 0>(m, tab, rec, gw.frc_w, Sw, fwi, h, hd, ti, e0, e1, s.stage, s.At, s.Atd);
+    if constexpr (NH > 1)
+      grad_sweep_moments<Cf, false, 1>(m, tab, rec, gw.frc_w, Sw, fwi, h, hd, ti, e0, e1, s.stage, s.At, s.Atd);
     __syncthreads();
     scale_moments<Cf>(tab, ti, s);
     __syncthreads();
@@ -705,8 +760,11 @@ __global__ __launch_bounds__(WAVE) void k_grad_outer(DModel<T> m, GradLds<Cf> L,
     __syncthreads();
     grad_moment_space<Cf, false>(m, p, s, ti, oW, nW);
     __syncthreads();
-    grad_sweep_params<Cf, false>(m, tab, types, rec, gw.frc_w, Sw, fwi, h, hd, msg, in_pos, ti, e0, e1, s, creg, oW,
-                                 nW);
+    grad_sweep_params<Cf, false, 0>(m, tab, rec, gw.frc_w, Sw, fwi, h, hd, msg, in_pos, ti, e0, e1, s, creg, oW,
+                                    nW);
+    if constexpr (NH > 1)
+      grad_sweep_params<Cf, false, 1>(m, tab, rec, gw.frc_w, Sw, fwi, h, hd, msg, in_pos, ti, e0, e1, s, creg + BB,
+                                      oW, nW);
   }
   __syncthreads();
   flush_partial<Cf>(m, false, creg, s.part, L.pn, oW, nW, partial + (size_t)blockIdx.x * L.pn);
@@ -728,9 +786,10 @@ __global__ __launch_bounds__(WAVE) void k_grad_gate_reverse(DModel<T> m, GradLds
   SpeciesTables<T> tab;
   load_species_tables<Cf>(m, true, s.tabs, tab);
   zero_part(s.part, L.pn);
-  double creg[BB];
+  constexpr int NH = Halves<Cf>::NH;
+  double creg[NH * BB];
 #pragma unroll
-  for (int b = 0; b < BB; ++b) creg[b] = 0.0;
+  for (int b = 0; b < NH * BB; ++b) creg[b] = 0.0;
   const DPass<T>& p = m.gp;
   __syncthreads();
   for (int i = blockIdx.x; i < natoms; i += gridDim.x) {
@@ -805,8 +864,11 @@ __global__ __launch_bounds__(WAVE) void k_grad_gate_reverse(DModel<T> m, GradLds
     __syncthreads();
     grad_moment_space<Cf, true>(m, p, s, ti, oW, nW);
     __syncthreads();
-    grad_sweep_params<Cf, true>(m, tab, types, rec, gw.frc_w, Sw, fwi, nullptr, nullptr, nullptr, nullptr, ti, e0,
-                                e1, s, creg, oW, nW);
+    grad_sweep_params<Cf, true, 0>(m, tab, rec, gw.frc_w, Sw, fwi, nullptr, nullptr, nullptr, nullptr, ti, e0, e1,
+                                   s, creg, oW, nW);
+    if constexpr (NH > 1)
+      grad_sweep_params<Cf, true, 1>(m, tab, rec, gw.frc_w, Sw, fwi, nullptr, nullptr, nullptr, nullptr, ti, e0, e1,
+                                     s, creg + BB, oW, nW);
   }
   __syncthreads();
   flush_partial<Cf>(m, true, creg, s.part, L.pn, oW, nW, partial + (size_t)blockIdx.x * L.pn);
