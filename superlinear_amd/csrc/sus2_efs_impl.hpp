@@ -322,12 +322,17 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
 // outer pass of an Effect model the gate-adjoint message
 //   u_e,mu = tau_{t_i,mu} R_mu(r; t_i, t_j) sum_m abar_i[k,l,m] Y_lm(d_e),
 // which the neighbour j gathers into h_bar_j (times alpha_{t_j,mu}).
-// FP32, L <= 3: at most 168 VGPRs (no spills), so three waves share each SIMD.
+// Centres per edge-backward workgroup: their edge lists are contiguous, so
+// two atoms' ~2 x 90 edges fill the lanes better than one atom's ~90.
+constexpr int EB_ATOMS = 2;
+
+// FP32, L <= 3: at most 168 VGPRs, so three waves share each SIMD.
 template <class Cf, bool GATE>
 constexpr int edge_backward_waves() { return (sizeof(typename Cf::T) == 4 && Cf::LL <= 3) ? 3 : 1; }
 
 template <class Cf, bool GATE, class T = typename Cf::T>
-__global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edge_backward(DModel<T> m, const int32_t* __restrict__ types,
+__global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edge_backward(DModel<T> m, int natoms,
+                                                        const int32_t* __restrict__ types,
                                                         const int* __restrict__ nbr_off,
                                                         const Rec<T>* __restrict__ rec, const T* __restrict__ h,
                                                         const T* __restrict__ adj, const int* __restrict__ in_pos,
@@ -335,19 +340,49 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
                                                         T* __restrict__ uedge, T* __restrict__ fw) {
   constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, F = Cf::F, KH = Cf::KH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sA = reinterpret_cast<T*>(smem_raw);  // [KH]
-  T* sXR = sA + KH + species_tables_size<Cf>(m.C);  // [F][LR] lane-private columns: x0 R (gate message)
+  constexpr int NA = EB_ATOMS;
+  T* sA = reinterpret_cast<T*>(smem_raw);  // [NA][KH]
+  T* sV = sA + NA * KH + species_tables_size<Cf>(m.C);  // [NA][6][WAVE] per-lane virial partials
+  T* sXR = sV + NA * 6 * WAVE;  // [F][LR] lane-private columns: x0 R (gate message)
   SpeciesTables<T> tab;
-  const int i = blockIdx.x, lane = threadIdx.x;
-  const int ti = types[i];
+  const int i0 = blockIdx.x * NA, lane = threadIdx.x;
+  const int na = min(NA, natoms - i0);
   const bool hgate = !GATE && m.gated;
-  load_species_tables<Cf>(m, GATE, sA + KH, tab);
-  for (int c = lane; c < KH; c += WAVE) sA[c] = adj[(size_t)i * KH + c];
+  load_species_tables<Cf>(m, GATE, sA + NA * KH, tab);
+  for (int c = lane; c < na * KH; c += WAVE) sA[c] = adj[(size_t)i0 * KH + c];
+#pragma unroll
+  for (int c = 0; c < NA * 6; ++c) sV[c * WAVE + lane] = T(0);
+  // the centres' edge ranges: [eb[a], eb[a + 1]) for atom i0 + a
+  int eb[NA + 1], ta[NA];
+#pragma unroll
+  for (int a = 0; a <= NA; ++a) eb[a] = nbr_off[i0 + min(a, na)];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) ta[a] = types[i0 + min(a, na - 1)];
   __syncthreads();
-  const T* x0 = tab.x0 + ti * F;
-  T w0 = T(0), w1 = T(0), w2 = T(0), w3 = T(0), w4 = T(0), w5 = T(0);
-  const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
-  for (int e = e0 + lane; e < e1; e += WAVE) {
+  // virial of the lane's current centre; a lane's edges visit the centres in
+  // order, so on a change of centre the partial goes to the lane's LDS slot
+  T w[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) w[k] = T(0);
+  int acur = 0;
+  for (int e = eb[0] + lane; e < eb[NA]; e += WAVE) {
+    int ai = 0;  // the edge's centre among the workgroup's atoms
+#pragma unroll
+    for (int a = 1; a < NA; ++a) ai += e >= eb[a];
+    if (ai != acur) {
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        sV[(acur * 6 + k) * WAVE + lane] = w[k];
+        w[k] = T(0);
+      }
+      acur = ai;
+    }
+    int ti = ta[0];
+#pragma unroll
+    for (int a = 1; a < NA; ++a)
+      if (ai == a) ti = ta[a];
+    const T* sAi = sA + ai * KH;
+    const T* x0 = tab.x0 + ti * F;
     const Rec<T> q = rec[e];
     const int j = rec_j(q);
     const T dx = q.x, dy = q.y, dz = q.z;
@@ -382,7 +417,7 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
 #pragma unroll
       for (int k = 0; k < KK; ++k) {
         const int mu = k * L1 + l;
-        const T av = sA[hh * KK + k];
+        const T av = sAi[hh * KK + k];
         sY += av * dR[mu] * Y;
         const T g = av * R[mu];
         gx += g * yx; gy += g * yy; gz += g * yz;
@@ -395,20 +430,27 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
     ge[0] = g0; ge[1] = g1; ge[2] = g2;
     T* gi = gedge_in + 3 * (size_t)slot;
     gi[0] = g0; gi[1] = g1; gi[2] = g2;
-    w0 -= dx * g0; w1 -= dy * g1; w2 -= dz * g2;
-    w3 -= dx * g1; w4 -= dx * g2; w5 -= dy * g2;
+    w[0] -= dx * g0; w[1] -= dy * g1; w[2] -= dz * g2;
+    w[3] -= dx * g1; w[4] -= dx * g2; w[5] -= dy * g2;
     if (hgate) {
       T* ue = uedge + (size_t)slot * F;
 #pragma unroll
       for (int mu = 0; mu < F; ++mu) ue[mu] = sXR[mu * LR + lane] * us[mu];
     }
   }
-  w0 = wave_sum(w0); w1 = wave_sum(w1); w2 = wave_sum(w2);
-  w3 = wave_sum(w3); w4 = wave_sum(w4); w5 = wave_sum(w5);
-  if (lane == 0) {
-    T* o = fw + (size_t)i * 9 + 3;
-    if (GATE) { o[0] += w0; o[1] += w1; o[2] += w2; o[3] += w3; o[4] += w4; o[5] += w5; }
-    else { o[0] = w0; o[1] = w1; o[2] = w2; o[3] = w3; o[4] = w4; o[5] = w5; }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) sV[(acur * 6 + k) * WAVE + lane] = w[k];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    if (a >= na) break;
+    T s[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s[k] = wave_sum(sV[(a * 6 + k) * WAVE + lane]);
+    if (lane == 0) {
+      T* o = fw + (size_t)(i0 + a) * 9 + 3;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) o[k] = GATE ? o[k] + s[k] : s[k];
+    }
   }
 }
 
@@ -527,7 +569,7 @@ size_t gather_lds(const DModel<typename Cf::T>& m) {
 template <class Cf>
 size_t backward_lds(const DModel<typename Cf::T>& m) {
   return sizeof(typename Cf::T) *
-         ((size_t)Cf::KH + species_tables_size<Cf>(m.C) + (m.gated ? (size_t)Cf::F * LR : 0));
+         ((size_t)EB_ATOMS * (Cf::KH + 6 * WAVE) + species_tables_size<Cf>(m.C) + (m.gated ? (size_t)Cf::F * LR : 0));
 }
 
 // ------------------------------------------------------------------ driver
@@ -579,7 +621,7 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
     SUS2_LAUNCH(ws, st, kname<Cf>("pass_forward_outer"), (k_pass_forward<Cf, false>), dim3(N), dim3(WAVE), lf, m,
                 b.types, off, rec, h, ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
     const int* ip = ws.in_pos.get<int>();
-    SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_outer"), (k_edge_backward<Cf, false>), dim3(N), dim3(WAVE), lb, m,
+    SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_outer"), (k_edge_backward<Cf, false>), dim3((N + EB_ATOMS - 1) / EB_ATOMS), dim3(WAVE), lb, m, N,
                 b.types, off, rec, h, ws.abar.get<T>(), ip, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(),
                 ws.fw.get<T>());
     allow_lds(k_gather<Cf, false>, lg);
@@ -587,7 +629,7 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
                 b.types, N, off, off, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
                 ws.gbar.get<T>(), (T*)nullptr, ws.fw.get<T>());
     if (m.gated) {
-      SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_gate"), (k_edge_backward<Cf, true>), dim3(N), dim3(WAVE), lb, m,
+      SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_gate"), (k_edge_backward<Cf, true>), dim3((N + EB_ATOMS - 1) / EB_ATOMS), dim3(WAVE), lb, m, N,
                   b.types, off, rec, h, ws.gbar.get<T>(), ip, ws.gedge.get<T>(), ws.gedge_in.get<T>(),
                   ws.uedge.get<T>(), ws.fw.get<T>());
       SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(N), dim3(WAVE), 0, m, b.types, N, off,
@@ -640,8 +682,8 @@ void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const Pai
       allow_lds(k_pass_forward<Cf, false>, lf);
       SUS2_LAUNCH(ws, st, kname<Cf>("pass_forward_outer"), (k_pass_forward<Cf, false>), dim3(n), dim3(WAVE), lf, m,
                   pv.types, off, rec, ws.h.get<T>(), ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
-      SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_outer"), (k_edge_backward<Cf, false>), dim3(n), dim3(WAVE), lb,
-                  m, pv.types, off, rec, ws.h.get<T>(), ws.abar.get<T>(), ws.in_pos.get<int>(), ws.gedge.get<T>(),
+      SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_outer"), (k_edge_backward<Cf, false>), dim3((n + EB_ATOMS - 1) / EB_ATOMS), dim3(WAVE),
+                  lb, m, n, pv.types, off, rec, ws.h.get<T>(), ws.abar.get<T>(), ws.in_pos.get<int>(), ws.gedge.get<T>(),
                   ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
     }
     if (all > 0) {
@@ -658,7 +700,7 @@ void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const Pai
     allow_lds(k_gate_reverse<Cf>, lg);
     SUS2_LAUNCH(ws, st, kname<Cf>("gate_reverse"), (k_gate_reverse<Cf>), dim3(n), dim3(WAVE), lg, m,
                 ws.hbar.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>());
-    SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_gate"), (k_edge_backward<Cf, true>), dim3(n), dim3(WAVE), lb, m,
+    SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_gate"), (k_edge_backward<Cf, true>), dim3((n + EB_ATOMS - 1) / EB_ATOMS), dim3(WAVE), lb, m, n,
                 pv.types, off, rec, ws.h.get<T>(), ws.gbar.get<T>(), ws.in_pos.get<int>(), ws.gedge.get<T>(),
                 ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
     SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(all), dim3(WAVE), 0, m, pv.types, n,
