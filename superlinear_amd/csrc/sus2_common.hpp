@@ -86,6 +86,10 @@ template <> constexpr const char* tname<double>() { return "f64"; }
 
 template <class T>
 void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st);
+// Reads the overflow report of a build that sized its edge buffers from earlier
+// calls (waits for the stream); true when the lists did not fit and the work
+// queued after the build has to be redone.  False when no report is pending.
+bool neighbors_verify(Workspace& ws, hipStream_t st);
 // Per-species site-energy constant species[t] + shift[t] (FP64 coefficients).
 struct SiteConst {
   const double* theta;

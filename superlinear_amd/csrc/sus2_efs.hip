@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -708,6 +709,8 @@ template void fill_zero<double>(Workspace&, double*, size_t, hipStream_t);
 template void fill_zero<int>(Workspace&, int*, size_t, hipStream_t);
 
 // ------------------------------------------------------------------ driver
+static void grow_slots(Workspace& ws, long maxcnt);
+
 template <class T>
 void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   const int N = b.natoms;
@@ -722,7 +725,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   ws.cell_cur.ensure(sizeof(int) * n1);
   ws.cell_atoms.ensure(sizeof(int) * n1);
   ws.in_cur.ensure(sizeof(int) * n1);
-  ws.scan_tmp.ensure(4 * sizeof(int64_t));
+  ws.scan_tmp.ensure(8 * sizeof(int64_t));
   const int cap = ws.nbr_cap;
   ws.nbr_slots.ensure(sizeof(Rec<T>) * n1 * cap);
   const int nb = (N + 255) / 256;
@@ -761,7 +764,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     SUS2_LAUNCH(ws, st, "bin_atoms", k_bin, dim3(nb), dim3(256), 0, N, b.nframes, b.frame_off, b.pos,
                 ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_cnt.get<int>(),
                 ws.atom_frame.get<int>());
-    device_scan(ws.cell_cnt.get<int>(), ws.cell_start.get<int>(), N, ws.scan_tmp.get<int64_t>() + 2, st);
+    device_scan(ws.cell_cnt.get<int>(), ws.cell_start.get<int>(), N, ws.scan_tmp.get<int64_t>() + 4, st);
     SUS2_LAUNCH(ws, st, "fill_bins", k_fill_bins, dim3(nb), dim3(256), 0, N, ws.cell_of.get<int>(),
                 ws.cell_start.get<int>(), ws.cell_cur.get<int>(), ws.cell_atoms.get<int>());
     SUS2_LAUNCH(ws, st, "sort_bins", k_sort_lists, dim3(N), dim3(WAVE), 0, N, ws.cell_start.get<int>(),
@@ -773,13 +776,38 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
                 ws.cell_atoms.get<int>(), rc, (const int*)nullptr, ws.nbr_cnt.get<int>(),
                 ws.nbr_slots.get<Rec<T>>(), cap, 0);
   }
+  // edge capacity left by earlier builds, in records of T
+  const long ecap = (long)std::min({ws.rec.bytes / sizeof(Rec<T>), ws.in_edges.bytes / sizeof(int),
+                                    ws.in_pos.bytes / sizeof(int), (size_t)INT32_MAX - 1});
+  if (ws.nbr_optimistic && ws.nbr_known && ecap > 0) {
+    // no host round trip: the scan reports (and empties every list) when the
+    // counts exceed the slots or the edge buffers; neighbors_verify reads the
+    // report once the call's work has been queued
+    int64_t* report = nullptr;
+    SUS2_CHECK(hipHostGetDevicePointer((void**)&report, ws.host_report, 0));
+    device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), N, ws.scan_tmp.get<int64_t>(), st, cap, ecap, report,
+                ++ws.report_seq);
+    ws.nedges = ecap;
+    ws.nbr_pending = true;
+    if (N > 0) {
+      SUS2_LAUNCH(ws, st, "compact" + tn, k_compact<T>, dim3(N), dim3(WAVE), 0, N, cap, ws.nbr_cnt.get<int>(),
+                  ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>(), ws.in_cur.get<int>(),
+                  ws.in_edges.get<int>());
+      SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(WAVE), 0, N, ws.nbr_off.get<int>(),
+                  ws.in_edges.get<int>(), ws.in_pos.get<int>());
+    }
+    return;
+  }
   long maxcnt = 0;
   const long total = exclusive_scan(ws, N, st, &maxcnt);
   if (total > INT32_MAX - 1) throw Error(SUS2_E_RANGE, "neighbour list exceeds 2^31 edges");
   ws.nedges = total;
-  ws.rec.ensure(sizeof(Rec<T>) * std::max<long>(total, 1));
-  ws.in_edges.ensure(sizeof(int) * std::max<long>(total, 1));
-  ws.in_pos.ensure(sizeof(int) * std::max<long>(total, 1));
+  ws.nbr_known = true;
+  // edge buffers with 1/8 headroom, so that later builds of similar frames fit
+  const long ebuf = std::max<long>(total + total / 8, 1);
+  ws.rec.ensure(sizeof(Rec<T>) * ebuf);
+  ws.in_edges.ensure(sizeof(int) * ebuf);
+  ws.in_pos.ensure(sizeof(int) * ebuf);
   if (N > 0) {
     // without overflow the copy also places the incoming edges; otherwise the
     // overflowed atoms are filled first and the incoming lists built after
@@ -801,9 +829,41 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
                   ws.in_edges.get<int>(), ws.in_pos.get<int>());
     }
   }
-  // slots for the next call: the largest count seen, with headroom, a multiple of 16
+  grow_slots(ws, maxcnt);
+}
+
+// slots for the next build: the largest count seen, with headroom, a multiple of 16
+static void grow_slots(Workspace& ws, long maxcnt) {
   const long want = std::max<long>(16, ((maxcnt + maxcnt / 4 + 15) / 16) * 16);
-  if (want > cap) ws.nbr_cap = (int)std::min<long>(want, 4096);
+  if (want > ws.nbr_cap) ws.nbr_cap = (int)std::min<long>(want, 4096);
+}
+
+bool neighbors_verify(Workspace& ws, hipStream_t st) {
+  if (!ws.nbr_pending) return false;
+  ws.nbr_pending = false;
+  // the scan runs early in the queued work: poll its report, checking the
+  // stream now and then so that a failed or drained stream ends the wait
+  volatile int64_t* rep = ws.host_report;
+  const int64_t want = ws.report_seq;
+  for (long polls = 1; rep[3] != want; ++polls) {
+    if (polls % 1024) continue;
+    const hipError_t e = hipStreamQuery(st);
+    if (e == hipErrorNotReady) continue;
+    if (e != hipSuccess) throw Error(SUS2_E_DEVICE, std::string("neighbour build failed: ") + hipGetErrorString(e));
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    if (rep[3] != want) throw Error(SUS2_E_DEVICE, "neighbour count report missing after the stream drained");
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  const long total = (long)rep[0], maxcnt = (long)rep[1];
+  const bool over = rep[2] != 0;
+  grow_slots(ws, maxcnt);
+  if (over) {  // the queued work ran on empty lists: drain it; the next build sizes on the host
+    SUS2_CHECK(hipStreamSynchronize(st));
+    ws.nbr_known = false;
+    return true;
+  }
+  ws.nedges = total;
+  return false;
 }
 
 template <class T>
@@ -960,9 +1020,23 @@ void run_efs(const DevModelSet& dm, Workspace& ws, const Batch& b, const EfsOut&
 void efs_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const EfsOut& out,
                   hipStream_t st) {
   check_rec_limits(precision, b.natoms, dm.host->C);
-  if (precision == SUS2_FP64) run_efs<double>(dm, ws, b, out, st);
-  else if (precision == SUS2_FP32) run_efs<float>(dm, ws, b, out, st);
-  else throw Error(SUS2_E_INVALID, "precision must be SUS2_FP64 or SUS2_FP32");
+  if (precision != SUS2_FP64 && precision != SUS2_FP32)
+    throw Error(SUS2_E_INVALID, "precision must be SUS2_FP64 or SUS2_FP32");
+  // first pass with the edge capacity of earlier calls (no host round trip
+  // inside the neighbour build); when the device reports that the lists did
+  // not fit, the outputs are recomputed with host-sized buffers
+  for (int pass = 0; pass < 2; ++pass) {
+    ws.nbr_optimistic = pass == 0;
+    try {
+      if (precision == SUS2_FP64) run_efs<double>(dm, ws, b, out, st);
+      else run_efs<float>(dm, ws, b, out, st);
+    } catch (...) {
+      ws.nbr_optimistic = ws.nbr_pending = false;
+      throw;
+    }
+    ws.nbr_optimistic = false;
+    if (!neighbors_verify(ws, st)) break;
+  }
   ws.prof.collect_ready();
 }
 

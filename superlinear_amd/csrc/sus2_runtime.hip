@@ -84,14 +84,19 @@ Workspace::~Workspace() {
 void Workspace::ensure_atoms(int n) {
   nbr_cnt.ensure(sizeof(int) * (size_t)std::max(n, 1));
   nbr_off.ensure(sizeof(int) * ((size_t)n + 1));
-  if (!host_total) RT_CHECK(hipHostMalloc((void**)&host_total, 2 * sizeof(int64_t), hipHostMallocDefault));
+  if (!host_total) RT_CHECK(hipHostMalloc((void**)&host_total, 4 * sizeof(int64_t), hipHostMallocDefault));
+  if (!host_report) {
+    RT_CHECK(hipHostMalloc((void**)&host_report, 4 * sizeof(int64_t), hipHostMallocCoherent));
+    for (int k = 0; k < 4; ++k) host_report[k] = 0;
+  }
 }
 
 // Single-workgroup exclusive scan: 1024 threads each own a contiguous chunk,
 // a workgroup scan of the chunk sums, then each chunk is rewritten.  Totals
 // above 2^31 are reported through the int64 total.
-__global__ __launch_bounds__(1024) void k_scan(int n, const int* __restrict__ cnt, int* __restrict__ off,
-                                               int64_t* __restrict__ total) {
+__global__ __launch_bounds__(1024) void k_scan(int n, int* __restrict__ cnt, int* __restrict__ off,
+                                               int64_t* __restrict__ total, int cap_lim, long edge_lim,
+                                               int64_t* report, int64_t seq) {
   __shared__ long part[1024];
   __shared__ int mx[1024];
   const int t = threadIdx.x;
@@ -114,25 +119,41 @@ __global__ __launch_bounds__(1024) void k_scan(int n, const int* __restrict__ cn
     mx[t] = max(mx[t], y);
     __syncthreads();
   }
+  const bool over = part[1023] > edge_lim || mx[1023] > cap_lim;
   long run = t ? part[t - 1] : 0;
   for (int i = b; i < e; ++i) {
-    off[i] = (int)run;
-    run += cnt[i];
+    if (over) {
+      off[i] = 0;
+      cnt[i] = 0;
+    } else {
+      off[i] = (int)run;
+      run += cnt[i];
+    }
   }
   if (t == 1023) {
-    off[n] = (int)part[1023];
+    off[n] = over ? 0 : (int)part[1023];
     total[0] = part[1023];
     total[1] = mx[1023];
+    total[2] = over;
+    if (report) {
+      report[0] = part[1023];
+      report[1] = mx[1023];
+      report[2] = over;
+      __threadfence_system();
+      report[3] = seq;
+      __threadfence_system();  // write the sequence back now, not at the stream's next system release
+    }
   }
 }
 
-void device_scan(const int* cnt, int* off, int n, int64_t* total, hipStream_t st) {
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, n, cnt, off, total);
+void device_scan(int* cnt, int* off, int n, int64_t* total, hipStream_t st, int cap_lim, long edge_lim,
+                 int64_t* report, int64_t seq) {
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, n, cnt, off, total, cap_lim, edge_lim, report, seq);
   RT_CHECK(hipGetLastError());
 }
 
 long exclusive_scan(Workspace& ws, int n, hipStream_t st, long* maxcnt) {
-  ws.scan_tmp.ensure(4 * sizeof(int64_t));
+  ws.scan_tmp.ensure(8 * sizeof(int64_t));
   device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), n, ws.scan_tmp.get<int64_t>(), st);
   RT_CHECK(hipMemcpyAsync(ws.host_total, ws.scan_tmp.get<int64_t>(), 2 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
   RT_CHECK(hipStreamSynchronize(st));

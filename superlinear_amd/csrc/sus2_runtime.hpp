@@ -92,13 +92,29 @@ struct Workspace {
   long nedges = 0;
   long max_frame_atoms = 0;  // upper bound on the atoms of one frame of the current batch
   DevBuf fin_part;           // per-chunk partial frame sums of the frame reduction
-  int64_t* host_total = nullptr;  // pinned
+  int64_t* host_total = nullptr;  // pinned [4]
+  // Sizing of the neighbour build without a host round trip: a build may take
+  // its edge capacity from earlier calls (nbr_known) when the caller verifies
+  // the device's overflow report afterwards (neighbors_verify); nbr_pending
+  // marks such a build whose report has not been read yet.
+  bool nbr_optimistic = false, nbr_known = false, nbr_pending = false;
+  // Count report of such a build: [total, max, overflow, sequence], written by
+  // the scan kernel into coherent pinned host memory and polled by the host.
+  // Kept until the process ends (a device-written host page is never freed).
+  int64_t* host_report = nullptr;
+  int64_t report_seq = 0;
   ~Workspace();
   void ensure_atoms(int n);
 };
 
 // exclusive scan cnt[0..n) -> off[0..n] on the device (total[0] = sum, total[1] = max)
-void device_scan(const int* cnt, int* off, int n, int64_t* total, hipStream_t st);
+// With cap_lim / edge_lim: when the largest count exceeds cap_lim or the sum
+// exceeds edge_lim, total[2] = 1 and every count and offset is written as 0
+// (an empty list for every atom); otherwise total[2] = 0.
+// With report: total[0..2] are also written to report[0..2] and then seq to
+// report[3] (system scope).
+void device_scan(int* cnt, int* off, int n, int64_t* total, hipStream_t st, int cap_lim = INT32_MAX,
+                 long edge_lim = INT64_MAX, int64_t* report = nullptr, int64_t seq = 0);
 // exclusive scan of ws.nbr_cnt[0..n) into ws.nbr_off[0..n]; returns the total
 // (and the largest count in *maxcnt)
 long exclusive_scan(Workspace& ws, int n, hipStream_t st, long* maxcnt = nullptr);

@@ -136,6 +136,50 @@ def test_neighbor_slot_overflow_then_adapted(lib, oracle_lib):
         assert np.abs(r["virial"] - ref["virial"]).max() <= 1e-10 * (1 + np.abs(ref["virial"]).max()), call
 
 
+def _directed_pairs(frame, rc):
+    """Brute-force count of directed pairs closer than rc over periodic images."""
+    x = np.asarray(frame["positions"], np.float64)
+    a = np.asarray(frame["cell"], np.float64)
+    recip = np.linalg.inv(a).T  # rows: reciprocal vectors (a_q . b_p = delta)
+    n = [int(np.ceil(rc * np.linalg.norm(recip[q]))) + 1 for q in range(3)]
+    d = x[None, :, :] - x[:, None, :]
+    count = 0
+    for s0 in range(-n[0], n[0] + 1):
+        for s1 in range(-n[1], n[1] + 1):
+            for s2 in range(-n[2], n[2] + 1):
+                r2 = ((d + s0 * a[0] + s1 * a[1] + s2 * a[2]) ** 2).sum(-1)
+                hit = r2 < rc * rc
+                if s0 == s1 == s2 == 0:
+                    np.fill_diagonal(hit, False)
+                count += int(hit.sum())
+    return count
+
+
+def test_edge_capacity_from_earlier_calls(lib, oracle_lib):
+    """Calls after the first size the neighbour lists from earlier calls and
+    read the device's overflow report at the end.  A sparse frame sets the
+    capacity; a dense frame then overflows the slots and the edge buffers, a
+    larger frame of normal density the edge buffers only, and a small frame
+    fits.  Each call's EFS and edge count against the oracle."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    print("[case] edge capacity from earlier calls", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    seq = [W.periodic_frames(1, 48, 48, 2, 0.05, seed=31, density=(0.08, 0.08)),
+           W.periodic_frames(1, 64, 64, 2, 0.05, seed=32, density=(0.35, 0.35)),
+           W.periodic_frames(1, 400, 400, 2, 0.05, seed=33),
+           W.periodic_frames(1, 40, 40, 2, 0.05, seed=34)]
+    m = Sus2Model(path)
+    for prec, tol in (("fp64", 1e-10), ("fp32", 1e-5)):
+        for call, frames in enumerate(seq):
+            ref = _oracle_efs(oracle_lib, path, frames)
+            r = m.calc_efs(frames, prec)
+            assert m.last_edge_count() == _directed_pairs(frames[0], m.cutoff), (prec, call)
+            assert abs(r["energy"][0] - ref["energy"][0]) <= tol * (1 + abs(ref["energy"][0])), (prec, call)
+            assert np.abs(r["forces"] - ref["forces"]).max() <= tol * (1 + np.abs(ref["forces"]).max()), (prec, call)
+            assert np.abs(r["virial"] - ref["virial"]).max() <= tol * (1 + np.abs(ref["virial"]).max()), (prec, call)
+
+
 def test_device_entry_point_matches_host(lib):
     import torch
     from superlinear_amd import workloads as W
