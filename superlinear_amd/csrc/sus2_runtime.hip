@@ -91,53 +91,81 @@ void Workspace::ensure_atoms(int n) {
   }
 }
 
-// Single-workgroup exclusive scan: 1024 threads each own a contiguous chunk,
-// a workgroup scan of the chunk sums, then each chunk is rewritten.  Totals
-// above 2^31 are reported through the int64 total.
-__global__ __launch_bounds__(1024) void k_scan(int n, int* __restrict__ cnt, int* __restrict__ off,
-                                               int64_t* __restrict__ total, int cap_lim, long edge_lim,
-                                               int64_t* report, int64_t seq) {
-  __shared__ long part[1024];
-  __shared__ int mx[1024];
-  const int t = threadIdx.x;
-  const int chunk = (n + 1023) / 1024;
-  const int b = t * chunk, e = min(n, b + chunk);
-  long s = 0;
-  int m = 0;
-  for (int i = b; i < e; ++i) {
-    s += cnt[i];
-    m = max(m, cnt[i]);
+// Single-workgroup exclusive scan over tiles of SCAN_TILE counts: each thread
+// loads four consecutive counts (coalesced across the workgroup), a wave
+// prefix sum and a scan of the 16 wave sums give the tile's offsets, and the
+// running total carries to the next tile.  Totals above 2^31 are reported
+// through the int64 total.
+constexpr int SCAN_THREADS = 1024, SCAN_PER = 4, SCAN_TILE = SCAN_THREADS * SCAN_PER;
+constexpr int SCAN_WAVES = SCAN_THREADS / 64;
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan(int n, int* __restrict__ cnt, int* __restrict__ off,
+                                                       int64_t* __restrict__ total, int cap_lim, long edge_lim,
+                                                       int64_t* report, int64_t seq) {
+  __shared__ long wsum[SCAN_WAVES];
+  __shared__ int wmax[SCAN_WAVES];
+  const int t = threadIdx.x, lane = t % 64, w = t / 64;
+  long carry = 0;
+  int mx = 0;
+  for (int base = 0; base < n; base += SCAN_TILE) {
+    const int i0 = base + t * SCAN_PER;
+    int v[SCAN_PER];
+    long s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k) {
+      v[k] = i0 + k < n ? cnt[i0 + k] : 0;
+      s += v[k];
+      mx = max(mx, v[k]);
+    }
+    long incl = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const long y = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    if (w == 0) {
+      long x = lane < SCAN_WAVES ? wsum[lane] : 0;
+#pragma unroll
+      for (int d = 1; d < SCAN_WAVES; d <<= 1) {
+        const long y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+      }
+      if (lane < SCAN_WAVES) wsum[lane] = x;
+    }
+    __syncthreads();
+    long run = carry + (w ? wsum[w - 1] : 0) + incl - s;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k)
+      if (i0 + k < n) {
+        off[i0 + k] = (int)run;
+        run += v[k];
+      }
+    carry += wsum[SCAN_WAVES - 1];
+    __syncthreads();  // wsum is rewritten by the next tile
   }
-  part[t] = s;
-  mx[t] = m;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) mx = max(mx, __shfl_xor(mx, d, 64));
+  if (lane == 0) wmax[w] = mx;
   __syncthreads();
-  for (int w = 1; w < 1024; w <<= 1) {
-    const long x = t >= w ? part[t - w] : 0;
-    const int y = t >= w ? mx[t - w] : 0;
-    __syncthreads();
-    part[t] += x;
-    mx[t] = max(mx[t], y);
-    __syncthreads();
-  }
-  const bool over = part[1023] > edge_lim || mx[1023] > cap_lim;
-  long run = t ? part[t - 1] : 0;
-  for (int i = b; i < e; ++i) {
-    if (over) {
+  int m = 0;
+#pragma unroll
+  for (int k = 0; k < SCAN_WAVES; ++k) m = max(m, wmax[k]);
+  const bool over = carry > edge_lim || m > cap_lim;
+  if (over)
+    for (int i = t; i < n; i += SCAN_THREADS) {
       off[i] = 0;
       cnt[i] = 0;
-    } else {
-      off[i] = (int)run;
-      run += cnt[i];
     }
-  }
-  if (t == 1023) {
-    off[n] = over ? 0 : (int)part[1023];
-    total[0] = part[1023];
-    total[1] = mx[1023];
+  if (t == 0) {
+    off[n] = over ? 0 : (int)carry;
+    total[0] = carry;
+    total[1] = m;
     total[2] = over;
     if (report) {
-      report[0] = part[1023];
-      report[1] = mx[1023];
+      report[0] = carry;
+      report[1] = m;
       report[2] = over;
       __threadfence_system();
       report[3] = seq;
@@ -148,7 +176,7 @@ __global__ __launch_bounds__(1024) void k_scan(int n, int* __restrict__ cnt, int
 
 void device_scan(int* cnt, int* off, int n, int64_t* total, hipStream_t st, int cap_lim, long edge_lim,
                  int64_t* report, int64_t seq) {
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, n, cnt, off, total, cap_lim, edge_lim, report, seq);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, st, n, cnt, off, total, cap_lim, edge_lim, report, seq);
   RT_CHECK(hipGetLastError());
 }
 
