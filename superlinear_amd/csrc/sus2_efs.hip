@@ -221,11 +221,38 @@ __global__ void k_grid_setup(int nframes, int natoms, const int64_t* __restrict_
 constexpr int FRAME_CELLS_MAX = 8192;
 constexpr int FRAME_CELLS_THREADS = 1024;
 
+// One atom of a bin-ordered cell list: its position moved into the home cell
+// (x - wrap . lattice rows), index and species, so that a neighbour candidate
+// is one 32-byte load.
+// Fractional coordinate q of x in the grid's basis, with every product and
+// sum rounded on its own (no contraction), so that two evaluations in one
+// kernel bin an atom identically.
+__device__ __forceinline__ double frac_coord(const Grid& g, const double* x, int q) {
+  return __dadd_rn(__dadd_rn(__dmul_rn(x[0] - g.org[0], g.inv[q]), __dmul_rn(x[1] - g.org[1], g.inv[3 + q])),
+                   __dmul_rn(x[2] - g.org[2], g.inv[6 + q]));
+}
+
+struct __align__(32) CellAtom {
+  double x, y, z;
+  int j, t;
+};
+
+__device__ __forceinline__ CellAtom cell_atom(const double* x, const int* w, const double* a, int j, int t) {
+  CellAtom c;
+  c.x = x[0] - (w[0] * a[0] + w[1] * a[3] + w[2] * a[6]);
+  c.y = x[1] - (w[0] * a[1] + w[1] * a[4] + w[2] * a[7]);
+  c.z = x[2] - (w[0] * a[2] + w[1] * a[5] + w[2] * a[8]);
+  c.j = j;
+  c.t = t;
+  return c;
+}
+
 __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
     int nframes, const int64_t* __restrict__ frame_off, const double* __restrict__ pos,
-    const double* __restrict__ cell, const int32_t* __restrict__ periodic, double rc, Grid* __restrict__ grid,
-    int* __restrict__ cell_of, int* __restrict__ wrap, int* __restrict__ cell_start, int* __restrict__ cell_atoms,
-    int* __restrict__ in_cur, int* __restrict__ atom_frame) {
+    const int32_t* __restrict__ types, const double* __restrict__ cell, const int32_t* __restrict__ periodic,
+    double rc, Grid* __restrict__ grid, int* __restrict__ cell_of, int* __restrict__ wrap,
+    int* __restrict__ cell_start, CellAtom* __restrict__ cell_pack, int* __restrict__ in_cur,
+    int* __restrict__ atom_frame) {
   extern __shared__ int lds_cells[];
   int* bin = lds_cells;                      // [na] local bin of each atom
   int* cnt = bin + FRAME_CELLS_MAX;          // [nbins + 1] counts, then exclusive starts
@@ -293,8 +320,7 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
   for (int k = t; k < na; k += nt) {
     double fr[3];
     const double* x = pos + 3 * (b0 + k);
-    for (int q = 0; q < 3; ++q)
-      fr[q] = (x[0] - g.org[0]) * g.inv[q] + (x[1] - g.org[1]) * g.inv[3 + q] + (x[2] - g.org[2]) * g.inv[6 + q];
+    for (int q = 0; q < 3; ++q) fr[q] = frac_coord(g, x, q);
     int c[3];
     for (int q = 0; q < 3; ++q) {
       const double w = (g.per & (1 << q)) ? floor(fr[q]) : 0.0;
@@ -336,6 +362,7 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
   for (int k = t; k < na; k += nt) tmp[atomicAdd(cur + bin[k], 1)] = k;
   __syncthreads();
   // ascending atom order inside each bin: rank among the bin's members
+  const double* a_rows = cell + 9 * f;
   for (int p = t; p < na; p += nt) {
     const int k = tmp[p], id = bin[k];
     const int s0 = cnt[id], s1 = cnt[id + 1];
@@ -347,7 +374,11 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
       r3 += tmp[q + 3] < k;
     }
     for (; q < s1; ++q) r0 += tmp[q] < k;
-    cell_atoms[b0 + s0 + (r0 + r1) + (r2 + r3)] = (int)b0 + k;
+    // the wrap of the binning pass, recomputed from the same expression
+    const double* x = pos + 3 * (b0 + k);
+    int w[3];
+    for (int d = 0; d < 3; ++d) w[d] = (g.per & (1 << d)) ? (int)floor(frac_coord(g, x, d)) : 0;
+    cell_pack[b0 + s0 + (r0 + r1) + (r2 + r3)] = cell_atom(x, w, a_rows, (int)b0 + k, types[b0 + k]);
   }
 }
 
@@ -385,6 +416,16 @@ __global__ void k_fill_bins(int natoms, const int* __restrict__ cell_of, const i
   if (i >= natoms) return;
   const int c = cell_of[i];
   cell_atoms[cell_start[c] + atomicAdd(cell_cur + c, 1)] = i;
+}
+
+__global__ void k_pack_cells(int natoms, const int* __restrict__ cell_atoms, const int* __restrict__ atom_frame,
+                             const double* __restrict__ pos, const int* __restrict__ wrap,
+                             const int32_t* __restrict__ types, const double* __restrict__ cell,
+                             CellAtom* __restrict__ cell_pack) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= natoms) return;
+  const int j = cell_atoms[s];
+  cell_pack[s] = cell_atom(pos + 3 * j, wrap + 3 * j, cell + 9 * atom_frame[j], j, types[j]);
 }
 
 // Sort the ids of one list [b, e) ascending (one wavefront per list; each
@@ -451,15 +492,15 @@ __device__ __forceinline__ int floor_div(int a, int n) { return a >= 0 ? a / n :
 
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __restrict__ atom_frame,
-                                                    const int32_t* __restrict__ types,
                                                     const int64_t* __restrict__ frame_off,
                                                     const double* __restrict__ pos, const double* __restrict__ cell,
                                                     const Grid* __restrict__ grid, const int* __restrict__ cell_of,
                                                     const int* __restrict__ wrap, const int* __restrict__ cell_start,
-                                                    const int* __restrict__ cell_atoms, double rc,
+                                                    const CellAtom* __restrict__ cell_pack, double rc,
                                                     const int* __restrict__ off, int* __restrict__ cnt,
                                                     Rec<T>* __restrict__ rec, int cap, int over) {
-  __shared__ int o_b[WAVE], o_cum[WAVE], o_s[3][WAVE];
+  __shared__ int o_b[WAVE], o_cum[WAVE], o_z[WAVE];
+  __shared__ double o_sh[3][WAVE];  // image shift of each bin offset (lattice rows); o_z: shift zero
   const int i = blockIdx.x, lane = threadIdx.x;
   if (i >= natoms) return;
   if (cap == 0 && cnt[i] <= over) return;
@@ -474,11 +515,15 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
   const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
   const int wi0 = wrap[3 * i], wi1 = wrap[3 * i + 1], wi2 = wrap[3 * i + 2];
   const double rc2 = rc * rc;
-  // the atom in its home cell; a bin box whose centre is farther than
-  // rc + (bin half-diagonal) holds no neighbour (1e-6 A of slack)
-  const double hx = xi - wi0 * g.A[0] - wi1 * g.A[3] - wi2 * g.A[6];
-  const double hy = yi - wi0 * g.A[1] - wi1 * g.A[4] - wi2 * g.A[7];
-  const double hz = zi - wi0 * g.A[2] - wi1 * g.A[5] - wi2 * g.A[8];
+  // the atom in its home cell, as the cell lists hold every atom; a bin box
+  // whose centre is farther than rc + (bin half-diagonal) holds no neighbour
+  // (1e-6 A of slack)
+  const double hx = xi - (wi0 * a[0] + wi1 * a[3] + wi2 * a[6]);
+  const double hy = yi - (wi0 * a[1] + wi1 * a[4] + wi2 * a[7]);
+  const double hz = zi - (wi0 * a[2] + wi1 * a[5] + wi2 * a[8]);
+  const double gx = xi - wi0 * g.A[0] - wi1 * g.A[3] - wi2 * g.A[6];
+  const double gy = yi - wi0 * g.A[1] - wi1 * g.A[4] - wi2 * g.A[7];
+  const double gz = zi - wi0 * g.A[2] - wi1 * g.A[5] - wi2 * g.A[8];
   const double far2 = (rc + g.rbox + 1e-6) * (rc + g.rbox + 1e-6);
   const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
   int count = 0;
@@ -498,9 +543,9 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
       else ok = ok && z >= 0 && z < g.n[2];
       if (ok) {
         const double u0 = (x + 0.5) / g.n[0] + s0, u1 = (y + 0.5) / g.n[1] + s1, u2 = (z + 0.5) / g.n[2] + s2;
-        const double ex = g.org[0] + u0 * g.A[0] + u1 * g.A[3] + u2 * g.A[6] - hx;
-        const double ey = g.org[1] + u0 * g.A[1] + u1 * g.A[4] + u2 * g.A[7] - hy;
-        const double ez = g.org[2] + u0 * g.A[2] + u1 * g.A[5] + u2 * g.A[8] - hz;
+        const double ex = g.org[0] + u0 * g.A[0] + u1 * g.A[3] + u2 * g.A[6] - gx;
+        const double ey = g.org[1] + u0 * g.A[1] + u1 * g.A[4] + u2 * g.A[7] - gy;
+        const double ez = g.org[2] + u0 * g.A[2] + u1 * g.A[5] + u2 * g.A[8] - gz;
         ok = ex * ex + ey * ey + ez * ez <= far2;
       }
       if (ok) {
@@ -518,40 +563,49 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
     const int total = __shfl(incl, WAVE - 1, WAVE);
     o_b[lane] = bs;
     o_cum[lane] = incl - nb;
-    o_s[0][lane] = s0; o_s[1][lane] = s1; o_s[2][lane] = s2;
+    // open directions have zero shifts
+    o_sh[0][lane] = s0 * a[0] + s1 * a[3] + s2 * a[6];
+    o_sh[1][lane] = s0 * a[1] + s1 * a[4] + s2 * a[7];
+    o_sh[2][lane] = s0 * a[2] + s1 * a[5] + s2 * a[8];
+    o_z[lane] = (s0 | s1 | s2) == 0;
     __syncthreads();
     const int nvalid = min(WAVE, nof - ob);
-    for (int k0 = 0; k0 < total; k0 += WAVE) {
-      const int k = k0 + lane;
-      bool hit = false;
-      double d0 = 0, d1 = 0, d2 = 0;
-      int j = 0;
-      if (k < total) {
-        int lo = 0, hi = nvalid;  // largest q with o_cum[q] <= k
-        while (hi - lo > 1) {
-          const int mid = (lo + hi) >> 1;
-          if (o_cum[mid] <= k) lo = mid;
-          else hi = mid;
-        }
-        j = cell_atoms[o_b[lo] + k - o_cum[lo]];
-        d0 = pos[3 * j] - xi; d1 = pos[3 * j + 1] - yi; d2 = pos[3 * j + 2] - zi;
-        bool self;
-        if (g.per) {  // open directions have zero shifts and wraps
-          const int k0i = o_s[0][lo] + wi0 - wrap[3 * j], k1i = o_s[1][lo] + wi1 - wrap[3 * j + 1],
-                    k2i = o_s[2][lo] + wi2 - wrap[3 * j + 2];
-          self = j == i && k0i == 0 && k1i == 0 && k2i == 0;
-          d0 = d0 + k0i * a[0] + k1i * a[3] + k2i * a[6];
-          d1 = d1 + k0i * a[1] + k1i * a[4] + k2i * a[7];
-          d2 = d2 + k0i * a[2] + k1i * a[5] + k2i * a[8];
-        } else {
-          self = j == i;
-        }
-        hit = !self && d0 * d0 + d1 * d1 + d2 * d2 < rc2;
+    // candidate k of the flattened range: its bin offset (binary search), the
+    // packed atom, and whether it is a neighbour (d = x_j + shift - x_i, both
+    // in the home cell; the atom itself at zero shift is not)
+    auto probe = [&](int k, bool& hit, double& d0, double& d1, double& d2, int& j, int& t) {
+      int lo = 0, hi = nvalid;  // largest q with o_cum[q] <= k
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (o_cum[mid] <= k) lo = mid;
+        else hi = mid;
       }
-      const unsigned long long mask = __ballot(hit);
-      const int slot = count + __popcll(mask & below);
-      if (hit && (cap == 0 || slot < cap)) rec[w0 + slot] = Rec<T>{T(d0), T(d1), T(d2), rec_jbits(j, types[j], T(0))};
-      count += __popcll(mask);
+      const CellAtom p = cell_pack[o_b[lo] + k - o_cum[lo]];
+      const double sx = o_sh[0][lo], sy = o_sh[1][lo], sz = o_sh[2][lo];
+      j = p.j;
+      t = p.t;
+      d0 = (p.x + sx) - hx;
+      d1 = (p.y + sy) - hy;
+      d2 = (p.z + sz) - hz;
+      const bool self = j == i && o_z[lo];
+      hit = !self && d0 * d0 + d1 * d1 + d2 * d2 < rc2;
+    };
+    // two candidates per lane per step: their loads overlap
+    for (int k0 = 0; k0 < total; k0 += 2 * WAVE) {
+      const int ka = k0 + lane, kb = k0 + WAVE + lane;
+      bool ha = false, hb = false;
+      double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+      int ja = 0, ta = 0, jb = 0, tb = 0;
+      if (ka < total) probe(ka, ha, a0, a1, a2, ja, ta);
+      if (kb < total) probe(kb, hb, b0, b1, b2, jb, tb);
+      const unsigned long long ma = __ballot(ha);
+      const int sa = count + __popcll(ma & below);
+      if (ha && (cap == 0 || sa < cap)) rec[w0 + sa] = Rec<T>{T(a0), T(a1), T(a2), rec_jbits(ja, ta, T(0))};
+      count += __popcll(ma);
+      const unsigned long long mb = __ballot(hb);
+      const int sb = count + __popcll(mb & below);
+      if (hb && (cap == 0 || sb < cap)) rec[w0 + sb] = Rec<T>{T(b0), T(b1), T(b2), rec_jbits(jb, tb, T(0))};
+      count += __popcll(mb);
     }
     __syncthreads();
   }
@@ -724,6 +778,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   ws.cell_start.ensure(sizeof(int) * (n1 + 1));
   ws.cell_cur.ensure(sizeof(int) * n1);
   ws.cell_atoms.ensure(sizeof(int) * n1);
+  ws.cell_pack.ensure(sizeof(CellAtom) * n1);
   ws.in_cur.ensure(sizeof(int) * n1);
   ws.scan_tmp.ensure(8 * sizeof(int64_t));
   const int cap = ws.nbr_cap;
@@ -753,9 +808,9 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
         lds_set = true;
       }
       SUS2_LAUNCH(ws, st, "frame_cells", k_frame_cells, dim3(b.nframes), dim3(FRAME_CELLS_THREADS),
-                  frame_cells_lds(), b.nframes, b.frame_off, b.pos, b.cell, b.periodic, rc, ws.grid.get<Grid>(),
-                  ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(), ws.cell_atoms.get<int>(),
-                  ws.in_cur.get<int>(), ws.atom_frame.get<int>());
+                  frame_cells_lds(), b.nframes, b.frame_off, b.pos, b.types, b.cell, b.periodic, rc,
+                  ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(),
+                  ws.cell_pack.get<CellAtom>(), ws.in_cur.get<int>(), ws.atom_frame.get<int>());
     }
   } else {
     SUS2_LAUNCH(ws, st, "grid_setup", k_grid_setup, dim3((std::max(b.nframes, N) + 63) / 64), dim3(64), 0,
@@ -769,11 +824,13 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
                 ws.cell_start.get<int>(), ws.cell_cur.get<int>(), ws.cell_atoms.get<int>());
     SUS2_LAUNCH(ws, st, "sort_bins", k_sort_lists, dim3(N), dim3(WAVE), 0, N, ws.cell_start.get<int>(),
                 ws.cell_atoms.get<int>(), (int*)nullptr);
+    SUS2_LAUNCH(ws, st, "pack_cells", k_pack_cells, dim3(nb), dim3(256), 0, N, ws.cell_atoms.get<int>(),
+                ws.atom_frame.get<int>(), b.pos, ws.wrap.get<int>(), b.types, b.cell, ws.cell_pack.get<CellAtom>());
   }
   if (N > 0) {
-    SUS2_LAUNCH(ws, st, "neighbors" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(), b.types, b.frame_off, b.pos,
+    SUS2_LAUNCH(ws, st, "neighbors" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(), b.frame_off, b.pos,
                 b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(),
-                ws.cell_atoms.get<int>(), rc, (const int*)nullptr, ws.nbr_cnt.get<int>(),
+                ws.cell_pack.get<CellAtom>(), rc, (const int*)nullptr, ws.nbr_cnt.get<int>(),
                 ws.nbr_slots.get<Rec<T>>(), cap, 0);
   }
   // edge capacity left by earlier builds, in records of T
@@ -817,8 +874,8 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
                 overflow ? (int*)nullptr : ws.in_edges.get<int>());
     if (overflow)  // atoms with more neighbours than slots: fill their records directly
       SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(),
-                  b.types, b.frame_off, b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(),
-                  ws.wrap.get<int>(), ws.cell_start.get<int>(), ws.cell_atoms.get<int>(), rc,
+                  b.frame_off, b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(),
+                  ws.wrap.get<int>(), ws.cell_start.get<int>(), ws.cell_pack.get<CellAtom>(), rc,
                   ws.nbr_off.get<int>(), ws.nbr_cnt.get<int>(), ws.rec.get<Rec<T>>(), 0, cap);
     if (total > 0) {
       if (overflow)

@@ -81,6 +81,7 @@ struct Workspace {
   std::vector<int64_t> host_frame_off;  // host copy of a batch's frame offsets (frame sizes)
   // cell-list neighbour build
   DevBuf grid, cell_of, wrap, cell_cnt, cell_start, cell_cur, cell_atoms;
+  DevBuf cell_pack;   // [N] bin-ordered atoms: home-cell position, index, species
   DevBuf atom_frame;  // [N] frame index of every atom (written by the binning kernels)
   // parameter gradient: gate tangent h_dot, gate node tangents, type-free gate
   // moments and tangents, per-edge gate messages, gathered h_bar, per-workgroup
