@@ -428,14 +428,16 @@ __global__ void k_pack_cells(int natoms, const int* __restrict__ cell_atoms, con
   cell_pack[s] = cell_atom(pos + 3 * j, wrap + 3 * j, cell + 9 * atom_frame[j], j, types[j]);
 }
 
-// Sort the ids of one list [b, e) ascending (one wavefront per list; each
-// element's rank = number of smaller ids; ids are distinct).  With inv, also
-// inv[id] = its final position.
-__device__ void wave_rank_sort(int* __restrict__ list, int b, int e, int* __restrict__ tmp, int* __restrict__ inv) {
+// Sort the ids of one list [b, e) ascending (one workgroup of LIST_THREADS
+// per list; each element's rank = number of smaller ids; ids are distinct).
+// With inv, also inv[id] = its final position.
+constexpr int LIST_THREADS = 128;  // per-list workgroups: a ~100-entry list in one step
+
+__device__ void list_rank_sort(int* __restrict__ list, int b, int e, int* __restrict__ tmp, int* __restrict__ inv) {
   const int lane = threadIdx.x, n = e - b;
-  for (int k = lane; k < n; k += WAVE) tmp[k] = list[b + k];
+  for (int k = lane; k < n; k += LIST_THREADS) tmp[k] = list[b + k];
   __syncthreads();
-  for (int k = lane; k < n; k += WAVE) {
+  for (int k = lane; k < n; k += LIST_THREADS) {
     const int v = tmp[k];
     int rank = 0;
     for (int q = 0; q < n; ++q) rank += tmp[q] < v;
@@ -447,7 +449,7 @@ __device__ void wave_rank_sort(int* __restrict__ list, int b, int e, int* __rest
 
 constexpr int SORT_MAX = 2048;
 
-__global__ __launch_bounds__(WAVE) void k_sort_lists(int nlists, const int* __restrict__ start,
+__global__ __launch_bounds__(LIST_THREADS) void k_sort_lists(int nlists, const int* __restrict__ start,
                                                      int* __restrict__ list, int* __restrict__ inv) {
   __shared__ int tmp[SORT_MAX];
   const int c = blockIdx.x;
@@ -458,7 +460,7 @@ __global__ __launch_bounds__(WAVE) void k_sort_lists(int nlists, const int* __re
     return;
   }
   if (e - b <= SORT_MAX) {
-    wave_rank_sort(list, b, e, tmp, inv);
+    list_rank_sort(list, b, e, tmp, inv);
     return;
   }
   if (threadIdx.x == 0) {  // oversized list: serial insertion sort
@@ -822,7 +824,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     device_scan(ws.cell_cnt.get<int>(), ws.cell_start.get<int>(), N, ws.scan_tmp.get<int64_t>() + 4, st);
     SUS2_LAUNCH(ws, st, "fill_bins", k_fill_bins, dim3(nb), dim3(256), 0, N, ws.cell_of.get<int>(),
                 ws.cell_start.get<int>(), ws.cell_cur.get<int>(), ws.cell_atoms.get<int>());
-    SUS2_LAUNCH(ws, st, "sort_bins", k_sort_lists, dim3(N), dim3(WAVE), 0, N, ws.cell_start.get<int>(),
+    SUS2_LAUNCH(ws, st, "sort_bins", k_sort_lists, dim3(N), dim3(LIST_THREADS), 0, N, ws.cell_start.get<int>(),
                 ws.cell_atoms.get<int>(), (int*)nullptr);
     SUS2_LAUNCH(ws, st, "pack_cells", k_pack_cells, dim3(nb), dim3(256), 0, N, ws.cell_atoms.get<int>(),
                 ws.atom_frame.get<int>(), b.pos, ws.wrap.get<int>(), b.types, b.cell, ws.cell_pack.get<CellAtom>());
@@ -850,7 +852,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
       SUS2_LAUNCH(ws, st, "compact" + tn, k_compact<T>, dim3(N), dim3(WAVE), 0, N, cap, ws.nbr_cnt.get<int>(),
                   ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>(), ws.in_cur.get<int>(),
                   ws.in_edges.get<int>());
-      SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(WAVE), 0, N, ws.nbr_off.get<int>(),
+      SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(LIST_THREADS), 0, N, ws.nbr_off.get<int>(),
                   ws.in_edges.get<int>(), ws.in_pos.get<int>());
     }
     return;
@@ -882,7 +884,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
         SUS2_LAUNCH(ws, st, "incoming" + tn, k_incoming<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                     total, ws.rec.get<Rec<T>>(), ws.nbr_off.get<int>(), ws.in_cur.get<int>(),
                     ws.in_edges.get<int>());
-      SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(WAVE), 0, N, ws.nbr_off.get<int>(),
+      SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(LIST_THREADS), 0, N, ws.nbr_off.get<int>(),
                   ws.in_edges.get<int>(), ws.in_pos.get<int>());
     }
   }
@@ -1004,7 +1006,7 @@ void build_pair_lists(Workspace& ws, int nowned, int nall, long nedges, const in
     SUS2_LAUNCH(ws, st, "pair_place", k_place_keys, dim3(ge), dim3(256), 0, nedges, ei, ws.nbr_off.get<int>(), cur,
                 order);
     if (nowned > 0)
-      SUS2_LAUNCH(ws, st, "pair_sort", k_sort_lists, dim3(nowned), dim3(WAVE), 0, nowned, ws.nbr_off.get<int>(),
+      SUS2_LAUNCH(ws, st, "pair_sort", k_sort_lists, dim3(nowned), dim3(LIST_THREADS), 0, nowned, ws.nbr_off.get<int>(),
                   order, (int*)nullptr);
     SUS2_LAUNCH(ws, st, "pair_rec", k_build_rec<T>, dim3(ge), dim3(256), 0, nedges, order, ej, d, types,
                 ws.rec.get<Rec<T>>(), tgt);
@@ -1018,7 +1020,7 @@ void build_pair_lists(Workspace& ws, int nowned, int nall, long nedges, const in
     SUS2_LAUNCH(ws, st, "pair_place", k_place_keys, dim3(ge), dim3(256), 0, nedges, tgt, in_off, cur,
                 ws.in_edges.get<int>());
     if (nall > 0)
-      SUS2_LAUNCH(ws, st, "pair_sort", k_sort_lists, dim3(nall), dim3(WAVE), 0, nall, in_off,
+      SUS2_LAUNCH(ws, st, "pair_sort", k_sort_lists, dim3(nall), dim3(LIST_THREADS), 0, nall, in_off,
                   ws.in_edges.get<int>(), ws.in_pos.get<int>());
   }
 }
