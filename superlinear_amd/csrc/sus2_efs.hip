@@ -237,14 +237,28 @@ struct __align__(32) CellAtom {
   int j, t;
 };
 
+// n . lattice rows, component q, every product and sum rounded on its own: the
+// same value in every kernel, and exactly negated for -n.
+__device__ __forceinline__ double lattice_comb(const int* n, const double* a, int q) {
+  return __dadd_rn(__dadd_rn(__dmul_rn((double)n[0], a[q]), __dmul_rn((double)n[1], a[3 + q])),
+                   __dmul_rn((double)n[2], a[6 + q]));
+}
+
 __device__ __forceinline__ CellAtom cell_atom(const double* x, const int* w, const double* a, int j, int t) {
   CellAtom c;
-  c.x = x[0] - (w[0] * a[0] + w[1] * a[3] + w[2] * a[6]);
-  c.y = x[1] - (w[0] * a[1] + w[1] * a[4] + w[2] * a[7]);
-  c.z = x[2] - (w[0] * a[2] + w[1] * a[5] + w[2] * a[8]);
+  c.x = x[0] - lattice_comb(w, a, 0);
+  c.y = x[1] - lattice_comb(w, a, 1);
+  c.z = x[2] - lattice_comb(w, a, 2);
   c.j = j;
   c.t = t;
   return c;
+}
+
+// Separation of a candidate from the centre, (x_j - x_i) + shift over home-cell
+// positions: exactly antisymmetric, so that i -> j and j -> i pass the cutoff
+// test together and the directed list stays symmetric.
+__device__ __forceinline__ double pair_dist2(double d0, double d1, double d2) {
+  return __dadd_rn(__dadd_rn(__dmul_rn(d0, d0), __dmul_rn(d1, d1)), __dmul_rn(d2, d2));
 }
 
 __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
@@ -517,12 +531,11 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
   const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
   const int wi0 = wrap[3 * i], wi1 = wrap[3 * i + 1], wi2 = wrap[3 * i + 2];
   const double rc2 = rc * rc;
-  // the atom in its home cell, as the cell lists hold every atom; a bin box
-  // whose centre is farther than rc + (bin half-diagonal) holds no neighbour
-  // (1e-6 A of slack)
-  const double hx = xi - (wi0 * a[0] + wi1 * a[3] + wi2 * a[6]);
-  const double hy = yi - (wi0 * a[1] + wi1 * a[4] + wi2 * a[7]);
-  const double hz = zi - (wi0 * a[2] + wi1 * a[5] + wi2 * a[8]);
+  // the atom in its home cell, as the cell lists hold every atom (the same
+  // expression as the cell lists' records); a bin box whose centre is farther
+  // than rc + (bin half-diagonal) holds no neighbour (1e-6 A of slack)
+  const CellAtom home = cell_atom(pos + 3 * i, wrap + 3 * i, a, i, 0);
+  const double hx = home.x, hy = home.y, hz = home.z;
   const double gx = xi - wi0 * g.A[0] - wi1 * g.A[3] - wi2 * g.A[6];
   const double gy = yi - wi0 * g.A[1] - wi1 * g.A[4] - wi2 * g.A[7];
   const double gz = zi - wi0 * g.A[2] - wi1 * g.A[5] - wi2 * g.A[8];
@@ -566,14 +579,15 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
     o_b[lane] = bs;
     o_cum[lane] = incl - nb;
     // open directions have zero shifts
-    o_sh[0][lane] = s0 * a[0] + s1 * a[3] + s2 * a[6];
-    o_sh[1][lane] = s0 * a[1] + s1 * a[4] + s2 * a[7];
-    o_sh[2][lane] = s0 * a[2] + s1 * a[5] + s2 * a[8];
+    const int sv[3] = {s0, s1, s2};
+    o_sh[0][lane] = lattice_comb(sv, a, 0);
+    o_sh[1][lane] = lattice_comb(sv, a, 1);
+    o_sh[2][lane] = lattice_comb(sv, a, 2);
     o_z[lane] = (s0 | s1 | s2) == 0;
     __syncthreads();
     const int nvalid = min(WAVE, nof - ob);
     // candidate k of the flattened range: its bin offset (binary search), the
-    // packed atom, and whether it is a neighbour (d = x_j + shift - x_i, both
+    // packed atom, and whether it is a neighbour (d = (x_j - x_i) + shift, both
     // in the home cell; the atom itself at zero shift is not)
     auto probe = [&](int k, bool& hit, double& d0, double& d1, double& d2, int& j, int& t) {
       int lo = 0, hi = nvalid;  // largest q with o_cum[q] <= k
@@ -586,11 +600,11 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
       const double sx = o_sh[0][lo], sy = o_sh[1][lo], sz = o_sh[2][lo];
       j = p.j;
       t = p.t;
-      d0 = (p.x + sx) - hx;
-      d1 = (p.y + sy) - hy;
-      d2 = (p.z + sz) - hz;
+      d0 = (p.x - hx) + sx;
+      d1 = (p.y - hy) + sy;
+      d2 = (p.z - hz) + sz;
       const bool self = j == i && o_z[lo];
-      hit = !self && d0 * d0 + d1 * d1 + d2 * d2 < rc2;
+      hit = !self && pair_dist2(d0, d1, d2) < rc2;
     };
     // two candidates per lane per step: their loads overlap
     for (int k0 = 0; k0 < total; k0 += 2 * WAVE) {

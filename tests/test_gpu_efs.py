@@ -180,6 +180,32 @@ def test_edge_capacity_from_earlier_calls(lib, oracle_lib):
             assert np.abs(r["virial"] - ref["virial"]).max() <= tol * (1 + np.abs(ref["virial"]).max()), (prec, call)
 
 
+def test_cutoff_ties_keep_lists_symmetric(lib, oracle_lib):
+    """Simple cubic lattice with spacing rc/3: many separations round to within
+    an ulp of the cutoff.  Both directions of a pair must pass the cutoff test
+    together (the incoming lists rely on a symmetric directed list): the edge
+    count is even and the EFS matches the oracle (the smooth cutoff makes the
+    contributions of pairs at rc vanish, so the oracle's own tie decisions do
+    not matter)."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    print("[case] cutoff ties", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    m = Sus2Model(path)
+    a = m.cutoff / 3.0
+    n = 6
+    grid = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(23)
+    frames = [dict(types=rng.integers(0, 2, n ** 3).astype(np.int32), positions=grid * a + 0.3,
+                   cell=np.eye(3) * (n * a), periodic=True)]
+    ref = _oracle_efs(oracle_lib, path, frames)
+    for prec, tol in (("fp64", 1e-10), ("fp32", 1e-5)):
+        r = m.calc_efs(frames, prec)
+        assert m.last_edge_count() % 2 == 0, prec
+        assert abs(r["energy"][0] - ref["energy"][0]) <= tol * (1 + abs(ref["energy"][0])), prec
+        assert np.abs(r["forces"] - ref["forces"]).max() <= tol * (1 + np.abs(ref["forces"]).max()), prec
+
+
 def test_device_entry_point_matches_host(lib):
     import torch
     from superlinear_amd import workloads as W
