@@ -134,6 +134,10 @@ __device__ inline void grid_basis(const double* c, int per, double* a) {
 }
 // Inverse basis, bins (at least r_c wide, at most one per atom) and image
 // reach of a frame with basis rows a (open rows already scaled).
+// Bins of about rc / BIN_DIV along each direction, so that a neighbour search
+// covers (2 BIN_DIV + 1)^3 bins (far ones culled) of few atoms each.
+constexpr double BIN_DIV = 2.0;
+
 __device__ inline void grid_finish(Grid& g, const double* a, double rc, long na) {
   double b[3][3];
 #pragma unroll
@@ -152,7 +156,7 @@ __device__ inline void grid_finish(Grid& g, const double* a, double rc, long na)
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     reach[q] = rc * sqrt(g.inv[q] * g.inv[q] + g.inv[3 + q] * g.inv[3 + q] + g.inv[6 + q] * g.inv[6 + q]);
-    n[q] = (int)fmin(1024.0, fmax(1.0, floor(1.0 / reach[q])));
+    n[q] = (int)fmin(1024.0, fmax(1.0, floor(BIN_DIV / reach[q])));
   }
   // at most one bin per atom: shrink the largest count (lowest index on ties)
   while ((long)n[0] * n[1] * n[2] > (na > 0 ? na : 1)) {
@@ -163,7 +167,9 @@ __device__ inline void grid_finish(Grid& g, const double* a, double rc, long na)
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     g.n[q] = n[q];
-    g.S[q] = (g.per & (1 << q)) ? (int)floor(reach[q] * n[q]) + 1 : 1;
+    // bins within reach; along an open direction no offset beyond n - 1 exists
+    const int s = (int)floor(reach[q] * n[q]) + 1;
+    g.S[q] = (g.per & (1 << q)) ? s : min(s, n[q] - 1);
   }
   grid_box(g, a);
 }
