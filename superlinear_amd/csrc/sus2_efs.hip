@@ -520,7 +520,8 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
                                                     const int* __restrict__ wrap, const int* __restrict__ cell_start,
                                                     const CellAtom* __restrict__ cell_pack, double rc,
                                                     const int* __restrict__ off, int* __restrict__ cnt,
-                                                    Rec<T>* __restrict__ rec, int cap, int over) {
+                                                    Rec<T>* __restrict__ rec, int* __restrict__ oslot, int cap,
+                                                    int over) {
   __shared__ int o_b[WAVE], o_cum[WAVE], o_z[WAVE];
   __shared__ double o_sh[3][WAVE];  // image shift of each bin offset (lattice rows); o_z: shift zero
   const int i = blockIdx.x, lane = threadIdx.x;
@@ -595,7 +596,7 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
     // candidate k of the flattened range: its bin offset (binary search), the
     // packed atom, and whether it is a neighbour (d = (x_j - x_i) + shift, both
     // in the home cell; the atom itself at zero shift is not)
-    auto probe = [&](int k, bool& hit, double& d0, double& d1, double& d2, int& j, int& t) {
+    auto probe = [&](int k, bool& hit, double& d0, double& d1, double& d2, int& j, int& t, int& oi) {
       int lo = 0, hi = nvalid;  // largest q with o_cum[q] <= k
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -611,22 +612,29 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
       d2 = (p.z - hz) + sz;
       const bool self = j == i && o_z[lo];
       hit = !self && pair_dist2(d0, d1, d2) < rc2;
+      oi = ob + lo;
     };
     // two candidates per lane per step: their loads overlap
     for (int k0 = 0; k0 < total; k0 += 2 * WAVE) {
       const int ka = k0 + lane, kb = k0 + WAVE + lane;
       bool ha = false, hb = false;
       double a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
-      int ja = 0, ta = 0, jb = 0, tb = 0;
-      if (ka < total) probe(ka, ha, a0, a1, a2, ja, ta);
-      if (kb < total) probe(kb, hb, b0, b1, b2, jb, tb);
+      int ja = 0, ta = 0, jb = 0, tb = 0, oa = 0, obb = 0;
+      if (ka < total) probe(ka, ha, a0, a1, a2, ja, ta, oa);
+      if (kb < total) probe(kb, hb, b0, b1, b2, jb, tb, obb);
       const unsigned long long ma = __ballot(ha);
       const int sa = count + __popcll(ma & below);
-      if (ha && (cap == 0 || sa < cap)) rec[w0 + sa] = Rec<T>{T(a0), T(a1), T(a2), rec_jbits(ja, ta, T(0))};
+      if (ha && (cap == 0 || sa < cap)) {
+        rec[w0 + sa] = Rec<T>{T(a0), T(a1), T(a2), rec_jbits(ja, ta, T(0))};
+        if (oslot) oslot[w0 + sa] = oa;
+      }
       count += __popcll(ma);
       const unsigned long long mb = __ballot(hb);
       const int sb = count + __popcll(mb & below);
-      if (hb && (cap == 0 || sb < cap)) rec[w0 + sb] = Rec<T>{T(b0), T(b1), T(b2), rec_jbits(jb, tb, T(0))};
+      if (hb && (cap == 0 || sb < cap)) {
+        rec[w0 + sb] = Rec<T>{T(b0), T(b1), T(b2), rec_jbits(jb, tb, T(0))};
+        if (oslot) oslot[w0 + sb] = obb;
+      }
       count += __popcll(mb);
     }
     __syncthreads();
@@ -637,7 +645,7 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
 // Copy every atom's collected hits from its fixed-capacity slots to the CSR
 // records (atoms whose count exceeded the capacity are filled directly).
 // With in_edges, also place each copied edge in its target's incoming list
-// (unordered; sorted afterwards), as k_incoming does.
+// (atomic cursors, so unordered; sorted afterwards), as k_incoming does.
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_compact(int natoms, int cap, const int* __restrict__ cnt,
                                                   const int* __restrict__ off, const Rec<T>* __restrict__ slots,
@@ -649,14 +657,87 @@ __global__ __launch_bounds__(WAVE) void k_compact(int natoms, int cap, const int
   if (n > cap) return;
   const Rec<T>* src = slots + (size_t)i * cap;
   const int o = off[i];
-  Rec<T>* dst = rec + o;
   for (int k = threadIdx.x; k < n; k += WAVE) {
     const Rec<T> q = src[k];
-    dst[k] = q;
+    rec[o + k] = q;
     if (in_edges) {
       const int t = rec_j(q);
       in_edges[off[t] + atomicAdd(in_cur + t, 1)] = o + k;
     }
+  }
+}
+
+// Copy of every atom's hits from its slots to the CSR records, and the
+// position of each edge's reverse, which places the edge in its target's
+// incoming list without atomics: the lists are exactly symmetric, so for
+// e = i -> t found at bin offset o, t found i at the mirrored offset
+// nof - 1 - o (image shift negated), in i's bin.  Its position in t's list
+// is the number of t's hits at earlier offsets (binary search over t's
+// ascending hit offsets) plus t's hits among the atoms of i's bin before i,
+// recomputed with the neighbour search's own expressions.  Incoming lists
+// are thereby in the target's outgoing order.  Requires every count within
+// the slots.
+template <class T>
+__global__ __launch_bounds__(WAVE) void k_compact_rev(int natoms, int cap, const int* __restrict__ cnt,
+                                                      const int* __restrict__ off,
+                                                      const Rec<T>* __restrict__ slots,
+                                                      const int* __restrict__ oslot, Rec<T>* __restrict__ rec,
+                                                      int* __restrict__ in_pos, const int* __restrict__ atom_frame,
+                                                      const int64_t* __restrict__ frame_off,
+                                                      const Grid* __restrict__ grid, const double* __restrict__ cell,
+                                                      const double* __restrict__ pos, const int* __restrict__ wrap,
+                                                      const int* __restrict__ cell_of,
+                                                      const int* __restrict__ cell_start,
+                                                      const CellAtom* __restrict__ cell_pack, double rc) {
+  const int i = blockIdx.x;
+  if (i >= natoms) return;
+  const int n = cnt[i];
+  const int f = atom_frame[i];
+  const Grid& g = grid[f];
+  const double* a = cell + 9 * f;
+  const int cbase = (int)frame_off[f];
+  const int n0 = 2 * g.S[0] + 1, n1 = 2 * g.S[1] + 1, n2 = 2 * g.S[2] + 1, nof = n0 * n1 * n2;
+  const int bin_start = cell_start[cell_of[i]], bin_end = cell_start[cell_of[i] + 1];
+  const double rc2 = rc * rc;
+  const Rec<T>* src = slots + (size_t)i * cap;
+  const int* osrc = oslot + (size_t)i * cap;
+  const int o_i = off[i];
+  for (int k = threadIdx.x; k < n; k += WAVE) {
+    const Rec<T> q = src[k];
+    rec[o_i + k] = q;
+    const int t = rec_j(q), os = nof - 1 - osrc[k];
+    // t's hits at offsets before os
+    const int* ot = oslot + (size_t)t * cap;
+    const int nt = cnt[t];
+    int lo = 0, hi = nt;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (ot[mid] < os) lo = mid + 1;
+      else hi = mid;
+    }
+    // t's bin coordinates and the image shift of offset os, as t's search forms them
+    int ct = cell_of[t] - cbase;
+    const int c0 = ct % g.n[0];
+    ct /= g.n[0];
+    const int c1 = ct % g.n[1], c2 = ct / g.n[1];
+    int x = c0 + os % n0 - g.S[0], y = c1 + (os / n0) % n1 - g.S[1], z = c2 + os / (n0 * n1) - g.S[2];
+    int sv[3] = {0, 0, 0};
+    if (g.per & 1) sv[0] = floor_div(x, g.n[0]);
+    if (g.per & 2) sv[1] = floor_div(y, g.n[1]);
+    if (g.per & 4) sv[2] = floor_div(z, g.n[2]);
+    const double sx = lattice_comb(sv, a, 0), sy = lattice_comb(sv, a, 1), sz = lattice_comb(sv, a, 2);
+    const bool zero = (sv[0] | sv[1] | sv[2]) == 0;
+    const CellAtom ht = cell_atom(pos + 3 * t, wrap + 3 * t, a, t, 0);
+    int before = 0;
+    for (int p = bin_start; p < bin_end; ++p) {  // i is in this bin: the loop ends at it
+      const CellAtom c = cell_pack[p];
+      if (c.j == i) break;
+      if (c.j == t && zero) continue;
+      const double d0 = (c.x - ht.x) + sx, d1 = (c.y - ht.y) + sy, d2 = (c.z - ht.z) + sz;
+      before += pair_dist2(d0, d1, d2) < rc2;
+    }
+    // (clamped into t's list: a position outside it would mean a broken symmetry)
+    in_pos[o_i + k] = off[t] + max(0, min(lo + before, nt - 1));
   }
 }
 
@@ -787,6 +868,33 @@ template void fill_zero<int>(Workspace&, int*, size_t, hipStream_t);
 // ------------------------------------------------------------------ driver
 static void grow_slots(Workspace& ws, long maxcnt);
 
+// Records from the slots and the incoming positions, when every count fits
+// the slots.  Up to REV_MAX_ATOMS atoms (a launch whose workgroups are all
+// resident, bound by the latency of each list) the reverse positions are
+// computed; beyond, where the per-edge reads of that computation bound the
+// kernel, the copy places the incoming edges with atomic cursors and the
+// lists are sorted into a fixed order.
+constexpr int REV_MAX_ATOMS = 16384;
+
+template <class T>
+static void compact_rev(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
+  const int N = b.natoms;
+  const std::string kn = std::string("compact<") + tname<T>() + ">";
+  if (N > REV_MAX_ATOMS) {
+    SUS2_LAUNCH(ws, st, kn, k_compact<T>, dim3(N), dim3(WAVE), 0, N, ws.nbr_cap, ws.nbr_cnt.get<int>(),
+                ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>(), ws.in_cur.get<int>(),
+                ws.in_edges.get<int>());
+    SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(LIST_THREADS), 0, N, ws.nbr_off.get<int>(),
+                ws.in_edges.get<int>(), ws.in_pos.get<int>());
+    return;
+  }
+  SUS2_LAUNCH(ws, st, kn, k_compact_rev<T>, dim3(N), dim3(WAVE), 0, N,
+              ws.nbr_cap, ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(),
+              ws.nbr_oslot.get<int>(), ws.rec.get<Rec<T>>(), ws.in_pos.get<int>(), ws.atom_frame.get<int>(),
+              b.frame_off, ws.grid.get<Grid>(), b.cell, b.pos, ws.wrap.get<int>(), ws.cell_of.get<int>(),
+              ws.cell_start.get<int>(), ws.cell_pack.get<CellAtom>(), rc);
+}
+
 template <class T>
 void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   const int N = b.natoms;
@@ -805,6 +913,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   ws.scan_tmp.ensure(8 * sizeof(int64_t));
   const int cap = ws.nbr_cap;
   ws.nbr_slots.ensure(sizeof(Rec<T>) * n1 * cap);
+  ws.nbr_oslot.ensure(sizeof(int) * n1 * cap);
   const int nb = (N + 255) / 256;
   const std::string tn = std::string("<") + tname<T>() + ">";
   // frames of at most FRAME_CELLS_MAX atoms build their cell lists in one
@@ -853,7 +962,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     SUS2_LAUNCH(ws, st, "neighbors" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(), b.frame_off, b.pos,
                 b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(),
                 ws.cell_pack.get<CellAtom>(), rc, (const int*)nullptr, ws.nbr_cnt.get<int>(),
-                ws.nbr_slots.get<Rec<T>>(), cap, 0);
+                ws.nbr_slots.get<Rec<T>>(), ws.nbr_oslot.get<int>(), cap, 0);
   }
   // edge capacity left by earlier builds, in records of T
   const long ecap = (long)std::min({ws.rec.bytes / sizeof(Rec<T>), ws.in_edges.bytes / sizeof(int),
@@ -868,13 +977,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
                 ++ws.report_seq);
     ws.nedges = ecap;
     ws.nbr_pending = true;
-    if (N > 0) {
-      SUS2_LAUNCH(ws, st, "compact" + tn, k_compact<T>, dim3(N), dim3(WAVE), 0, N, cap, ws.nbr_cnt.get<int>(),
-                  ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>(), ws.in_cur.get<int>(),
-                  ws.in_edges.get<int>());
-      SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(LIST_THREADS), 0, N, ws.nbr_off.get<int>(),
-                  ws.in_edges.get<int>(), ws.in_pos.get<int>());
-    }
+    if (N > 0) compact_rev<T>(ws, b, rc, st);
     return;
   }
   long maxcnt = 0;
@@ -887,26 +990,23 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   ws.rec.ensure(sizeof(Rec<T>) * ebuf);
   ws.in_edges.ensure(sizeof(int) * ebuf);
   ws.in_pos.ensure(sizeof(int) * ebuf);
-  if (N > 0) {
-    // without overflow the copy also places the incoming edges; otherwise the
-    // overflowed atoms are filled first and the incoming lists built after
-    const bool overflow = maxcnt > cap;
+  if (N > 0 && maxcnt <= cap) {
+    compact_rev<T>(ws, b, rc, st);
+  } else if (N > 0) {
+    // atoms with more neighbours than slots: the others are copied, these are
+    // filled directly, and the incoming lists are placed (atomic cursors) and
+    // sorted into a fixed order afterwards
     SUS2_LAUNCH(ws, st, "compact" + tn, k_compact<T>, dim3(N), dim3(WAVE), 0, N, cap, ws.nbr_cnt.get<int>(),
                 ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>(), ws.in_cur.get<int>(),
-                overflow ? (int*)nullptr : ws.in_edges.get<int>());
-    if (overflow)  // atoms with more neighbours than slots: fill their records directly
-      SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(),
-                  b.frame_off, b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(),
-                  ws.wrap.get<int>(), ws.cell_start.get<int>(), ws.cell_pack.get<CellAtom>(), rc,
-                  ws.nbr_off.get<int>(), ws.nbr_cnt.get<int>(), ws.rec.get<Rec<T>>(), 0, cap);
-    if (total > 0) {
-      if (overflow)
-        SUS2_LAUNCH(ws, st, "incoming" + tn, k_incoming<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                    total, ws.rec.get<Rec<T>>(), ws.nbr_off.get<int>(), ws.in_cur.get<int>(),
-                    ws.in_edges.get<int>());
-      SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(LIST_THREADS), 0, N, ws.nbr_off.get<int>(),
-                  ws.in_edges.get<int>(), ws.in_pos.get<int>());
-    }
+                (int*)nullptr);
+    SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(),
+                b.frame_off, b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
+                ws.cell_start.get<int>(), ws.cell_pack.get<CellAtom>(), rc, ws.nbr_off.get<int>(),
+                ws.nbr_cnt.get<int>(), ws.rec.get<Rec<T>>(), (int*)nullptr, 0, cap);
+    SUS2_LAUNCH(ws, st, "incoming" + tn, k_incoming<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, total,
+                ws.rec.get<Rec<T>>(), ws.nbr_off.get<int>(), ws.in_cur.get<int>(), ws.in_edges.get<int>());
+    SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(LIST_THREADS), 0, N, ws.nbr_off.get<int>(),
+                ws.in_edges.get<int>(), ws.in_pos.get<int>());
   }
   grow_slots(ws, maxcnt);
 }

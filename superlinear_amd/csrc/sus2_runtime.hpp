@@ -77,6 +77,7 @@ struct Workspace {
   DevBuf gedge, gedge_in, uedge, in_edges, in_pos, in_cur;
   // fixed-capacity per-atom neighbour slots of the single-pass list build
   DevBuf nbr_slots;
+  DevBuf nbr_oslot;  // bin-offset index of each slot's hit (ascending per atom)
   int nbr_cap = 128;
   std::vector<int64_t> host_frame_off;  // host copy of a batch's frame offsets (frame sizes)
   // cell-list neighbour build

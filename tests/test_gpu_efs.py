@@ -101,6 +101,26 @@ def test_batch_uneven_l4k6(lib, oracle_lib):
     assert np.abs((r32["energy"] - ref["energy"]) / counts).max() <= 1e-6
 
 
+def test_batch_above_reverse_placement_limit(lib, oracle_lib):
+    """A batch of more than 16384 atoms: the incoming lists are placed with
+    atomic cursors and sorted (smaller launches compute each edge's reverse
+    position instead).  FP64 and FP32 against the oracle."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    print("[case] batch above reverse placement limit", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frames = W.periodic_frames(48, 320, 440, 2, 0.05, seed=51)
+    assert sum(len(f["types"]) for f in frames) > 16384
+    ref = _oracle_efs(oracle_lib, path, frames)
+    m = Sus2Model(path)
+    r = m.calc_efs(frames, "fp64")
+    assert np.abs(r["energy"] - ref["energy"]).max() <= 1e-10 * (1 + np.abs(ref["energy"]).max())
+    assert np.abs(r["forces"] - ref["forces"]).max() <= 1e-10 * (1 + np.abs(ref["forces"]).max())
+    assert np.abs(r["virial"] - ref["virial"]).max() <= 1e-10 * (1 + np.abs(ref["virial"]).max())
+    r32 = m.calc_efs(frames, "fp32")
+    assert np.abs(r32["forces"] - ref["forces"]).max() <= 1e-5
+
+
 def test_large_frame_grid_path(lib, oracle_lib):
     """A frame above the per-frame single-workgroup cell-list limit (8192
     atoms) takes the device-wide binning path; sparse (density 0.03 A^-3) so
