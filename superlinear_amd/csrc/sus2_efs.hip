@@ -137,6 +137,9 @@ __device__ inline void grid_basis(const double* c, int per, double* a) {
 // Bins of about rc / BIN_DIV along each direction, so that a neighbour search
 // covers (2 BIN_DIV + 1)^3 bins (far ones culled) of few atoms each.
 constexpr double BIN_DIV = 2.0;
+// Per-atom table of hits before each bin offset (reverse positions), for
+// grids of at most NOF_TAB offsets (search extents up to 3 bins).
+constexpr int NOF_TAB = 343;
 
 __device__ inline void grid_finish(Grid& g, const double* a, double rc, long na) {
   double b[3][3];
@@ -229,7 +232,9 @@ constexpr int FRAME_CELLS_THREADS = 1024;
 
 // One atom of a bin-ordered cell list: its position moved into the home cell
 // (x - wrap . lattice rows), index and species, so that a neighbour candidate
-// is one 32-byte load.
+// is one 32-byte load.  The per-atom copy (atom_home, in atom order, written by
+// the neighbour search for its centre) holds the global bin id in place of
+// the index.
 // Fractional coordinate q of x in the grid's basis, with every product and
 // sum rounded on its own (no contraction), so that two evaluations in one
 // kernel bin an atom identically.
@@ -520,9 +525,10 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
                                                     const int* __restrict__ wrap, const int* __restrict__ cell_start,
                                                     const CellAtom* __restrict__ cell_pack, double rc,
                                                     const int* __restrict__ off, int* __restrict__ cnt,
-                                                    Rec<T>* __restrict__ rec, int* __restrict__ oslot, int cap,
-                                                    int over) {
-  __shared__ int o_b[WAVE], o_cum[WAVE], o_z[WAVE];
+                                                    Rec<T>* __restrict__ rec, int* __restrict__ oslot,
+                                                    uint16_t* __restrict__ otab, CellAtom* __restrict__ atom_home,
+                                                    int cap, int over) {
+  __shared__ int o_b[WAVE], o_cum[WAVE], o_z[WAVE], o_hits[WAVE];
   __shared__ double o_sh[3][WAVE];  // image shift of each bin offset (lattice rows); o_z: shift zero
   const int i = blockIdx.x, lane = threadIdx.x;
   if (i >= natoms) return;
@@ -543,6 +549,7 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
   // than rc + (bin half-diagonal) holds no neighbour (1e-6 A of slack)
   const CellAtom home = cell_atom(pos + 3 * i, wrap + 3 * i, a, i, 0);
   const double hx = home.x, hy = home.y, hz = home.z;
+  if (atom_home && lane == 0) atom_home[i] = CellAtom{hx, hy, hz, cell_of[i], 0};
   const double gx = xi - wi0 * g.A[0] - wi1 * g.A[3] - wi2 * g.A[6];
   const double gy = yi - wi0 * g.A[1] - wi1 * g.A[4] - wi2 * g.A[7];
   const double gz = zi - wi0 * g.A[2] - wi1 * g.A[5] - wi2 * g.A[8];
@@ -591,8 +598,9 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
     o_sh[1][lane] = lattice_comb(sv, a, 1);
     o_sh[2][lane] = lattice_comb(sv, a, 2);
     o_z[lane] = (s0 | s1 | s2) == 0;
+    o_hits[lane] = 0;
     __syncthreads();
-    const int nvalid = min(WAVE, nof - ob);
+    const int nvalid = min(WAVE, nof - ob), chunk_start = count;
     // candidate k of the flattened range: its bin offset (binary search), the
     // packed atom, and whether it is a neighbour (d = (x_j - x_i) + shift, both
     // in the home cell; the atom itself at zero shift is not)
@@ -628,6 +636,7 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
         rec[w0 + sa] = Rec<T>{T(a0), T(a1), T(a2), rec_jbits(ja, ta, T(0))};
         if (oslot) oslot[w0 + sa] = oa;
       }
+      if (ha && otab) atomicAdd(&o_hits[oa - ob], 1);
       count += __popcll(ma);
       const unsigned long long mb = __ballot(hb);
       const int sb = count + __popcll(mb & below);
@@ -635,9 +644,19 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
         rec[w0 + sb] = Rec<T>{T(b0), T(b1), T(b2), rec_jbits(jb, tb, T(0))};
         if (oslot) oslot[w0 + sb] = obb;
       }
+      if (hb && otab) atomicAdd(&o_hits[obb - ob], 1);
       count += __popcll(mb);
     }
     __syncthreads();
+    if (otab && nof <= NOF_TAB) {  // hits before each offset of the chunk
+      int h = o_hits[lane], inc = h;
+#pragma unroll
+      for (int d = 1; d < WAVE; d <<= 1) {
+        const int y = __shfl_up(inc, d, WAVE);
+        if (lane >= d) inc += y;
+      }
+      if (ob + lane < nof) otab[(size_t)i * NOF_TAB + ob + lane] = (uint16_t)(chunk_start + inc - h);
+    }
   }
   if (cap && lane == 0) cnt[i] = count;
 }
@@ -681,12 +700,12 @@ template <class T>
 __global__ __launch_bounds__(WAVE) void k_compact_rev(int natoms, int cap, const int* __restrict__ cnt,
                                                       const int* __restrict__ off,
                                                       const Rec<T>* __restrict__ slots,
-                                                      const int* __restrict__ oslot, Rec<T>* __restrict__ rec,
+                                                      const int* __restrict__ oslot,
+                                                      const uint16_t* __restrict__ otab, Rec<T>* __restrict__ rec,
                                                       int* __restrict__ in_pos, const int* __restrict__ atom_frame,
                                                       const int64_t* __restrict__ frame_off,
                                                       const Grid* __restrict__ grid, const double* __restrict__ cell,
-                                                      const double* __restrict__ pos, const int* __restrict__ wrap,
-                                                      const int* __restrict__ cell_of,
+                                                      const CellAtom* __restrict__ atom_home,
                                                       const int* __restrict__ cell_start,
                                                       const CellAtom* __restrict__ cell_pack, double rc) {
   const int i = blockIdx.x;
@@ -697,7 +716,9 @@ __global__ __launch_bounds__(WAVE) void k_compact_rev(int natoms, int cap, const
   const double* a = cell + 9 * f;
   const int cbase = (int)frame_off[f];
   const int n0 = 2 * g.S[0] + 1, n1 = 2 * g.S[1] + 1, n2 = 2 * g.S[2] + 1, nof = n0 * n1 * n2;
-  const int bin_start = cell_start[cell_of[i]], bin_end = cell_start[cell_of[i] + 1];
+  const bool tabled = nof <= NOF_TAB;
+  const int bin_i = atom_home[i].j;
+  const int bin_start = cell_start[bin_i], bin_end = cell_start[bin_i + 1];
   const double rc2 = rc * rc;
   const Rec<T>* src = slots + (size_t)i * cap;
   const int* osrc = oslot + (size_t)i * cap;
@@ -706,17 +727,24 @@ __global__ __launch_bounds__(WAVE) void k_compact_rev(int natoms, int cap, const
     const Rec<T> q = src[k];
     rec[o_i + k] = q;
     const int t = rec_j(q), os = nof - 1 - osrc[k];
+    const CellAtom ht = atom_home[t];  // home position and bin of t
+    const int t0 = off[t], nt = off[t + 1] - t0;
     // t's hits at offsets before os
-    const int* ot = oslot + (size_t)t * cap;
-    const int nt = cnt[t];
-    int lo = 0, hi = nt;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (ot[mid] < os) lo = mid + 1;
-      else hi = mid;
+    int lo;
+    if (tabled) {
+      lo = otab[(size_t)t * NOF_TAB + os];
+    } else {  // binary search over t's ascending hit offsets
+      const int* ot = oslot + (size_t)t * cap;
+      int hi = nt;
+      lo = 0;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ot[mid] < os) lo = mid + 1;
+        else hi = mid;
+      }
     }
     // t's bin coordinates and the image shift of offset os, as t's search forms them
-    int ct = cell_of[t] - cbase;
+    int ct = ht.j - cbase;
     const int c0 = ct % g.n[0];
     ct /= g.n[0];
     const int c1 = ct % g.n[1], c2 = ct / g.n[1];
@@ -727,7 +755,6 @@ __global__ __launch_bounds__(WAVE) void k_compact_rev(int natoms, int cap, const
     if (g.per & 4) sv[2] = floor_div(z, g.n[2]);
     const double sx = lattice_comb(sv, a, 0), sy = lattice_comb(sv, a, 1), sz = lattice_comb(sv, a, 2);
     const bool zero = (sv[0] | sv[1] | sv[2]) == 0;
-    const CellAtom ht = cell_atom(pos + 3 * t, wrap + 3 * t, a, t, 0);
     int before = 0;
     for (int p = bin_start; p < bin_end; ++p) {  // i is in this bin: the loop ends at it
       const CellAtom c = cell_pack[p];
@@ -737,7 +764,7 @@ __global__ __launch_bounds__(WAVE) void k_compact_rev(int natoms, int cap, const
       before += pair_dist2(d0, d1, d2) < rc2;
     }
     // (clamped into t's list: a position outside it would mean a broken symmetry)
-    in_pos[o_i + k] = off[t] + max(0, min(lo + before, nt - 1));
+    in_pos[o_i + k] = t0 + max(0, min(lo + before, nt - 1));
   }
 }
 
@@ -890,8 +917,8 @@ static void compact_rev(Workspace& ws, const Batch& b, double rc, hipStream_t st
   }
   SUS2_LAUNCH(ws, st, kn, k_compact_rev<T>, dim3(N), dim3(WAVE), 0, N,
               ws.nbr_cap, ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(),
-              ws.nbr_oslot.get<int>(), ws.rec.get<Rec<T>>(), ws.in_pos.get<int>(), ws.atom_frame.get<int>(),
-              b.frame_off, ws.grid.get<Grid>(), b.cell, b.pos, ws.wrap.get<int>(), ws.cell_of.get<int>(),
+              ws.nbr_oslot.get<int>(), ws.nbr_otab.get<uint16_t>(), ws.rec.get<Rec<T>>(), ws.in_pos.get<int>(),
+              ws.atom_frame.get<int>(), b.frame_off, ws.grid.get<Grid>(), b.cell, ws.atom_home.get<CellAtom>(),
               ws.cell_start.get<int>(), ws.cell_pack.get<CellAtom>(), rc);
 }
 
@@ -909,6 +936,10 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   ws.cell_cur.ensure(sizeof(int) * n1);
   ws.cell_atoms.ensure(sizeof(int) * n1);
   ws.cell_pack.ensure(sizeof(CellAtom) * n1);
+  if (N <= REV_MAX_ATOMS) {
+    ws.atom_home.ensure(sizeof(CellAtom) * n1);
+    ws.nbr_otab.ensure(sizeof(uint16_t) * n1 * NOF_TAB);
+  }
   ws.in_cur.ensure(sizeof(int) * n1);
   ws.scan_tmp.ensure(8 * sizeof(int64_t));
   const int cap = ws.nbr_cap;
@@ -962,7 +993,9 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     SUS2_LAUNCH(ws, st, "neighbors" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(), b.frame_off, b.pos,
                 b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(),
                 ws.cell_pack.get<CellAtom>(), rc, (const int*)nullptr, ws.nbr_cnt.get<int>(),
-                ws.nbr_slots.get<Rec<T>>(), ws.nbr_oslot.get<int>(), cap, 0);
+                ws.nbr_slots.get<Rec<T>>(), ws.nbr_oslot.get<int>(),
+                N <= REV_MAX_ATOMS ? ws.nbr_otab.get<uint16_t>() : (uint16_t*)nullptr,
+                N <= REV_MAX_ATOMS ? ws.atom_home.get<CellAtom>() : (CellAtom*)nullptr, cap, 0);
   }
   // edge capacity left by earlier builds, in records of T
   const long ecap = (long)std::min({ws.rec.bytes / sizeof(Rec<T>), ws.in_edges.bytes / sizeof(int),
@@ -1002,7 +1035,8 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(),
                 b.frame_off, b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
                 ws.cell_start.get<int>(), ws.cell_pack.get<CellAtom>(), rc, ws.nbr_off.get<int>(),
-                ws.nbr_cnt.get<int>(), ws.rec.get<Rec<T>>(), (int*)nullptr, 0, cap);
+                ws.nbr_cnt.get<int>(), ws.rec.get<Rec<T>>(), (int*)nullptr, (uint16_t*)nullptr, (CellAtom*)nullptr,
+                0, cap);
     SUS2_LAUNCH(ws, st, "incoming" + tn, k_incoming<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, total,
                 ws.rec.get<Rec<T>>(), ws.nbr_off.get<int>(), ws.in_cur.get<int>(), ws.in_edges.get<int>());
     SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(LIST_THREADS), 0, N, ws.nbr_off.get<int>(),
