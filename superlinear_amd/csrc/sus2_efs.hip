@@ -61,20 +61,22 @@ struct Grid {
 
 // Basis rows and the bin half-diagonal, once the bin counts are fixed.
 __device__ inline void grid_box(Grid& g, const double* a) {
-  double rb = 0;
+  double rb2 = 0;
 #pragma unroll
   for (int k = 0; k < 9; ++k) g.A[k] = a[k];
+  const double in[3] = {1.0 / g.n[0], 1.0 / g.n[1], 1.0 / g.n[2]};
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     double v[3] = {0, 0, 0};
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const double w = ((s >> q) & 1 ? 1.0 : -1.0) / g.n[q];
+      const double w = (s >> q) & 1 ? in[q] : -in[q];
+#pragma unroll
       for (int k = 0; k < 3; ++k) v[k] += w * a[3 * q + k];
     }
-    rb = fmax(rb, sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+    rb2 = fmax(rb2, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
   }
-  g.rbox = 0.5 * rb;
+  g.rbox = 0.5 * sqrt(rb2);
 }
 
 // Periodic mask of frame f: lattice rows that are non-zero (periodic frames).
@@ -149,11 +151,11 @@ __device__ inline void grid_finish(Grid& g, const double* a, double rc, long na)
     for (int q = 0; q < 3; ++q)
       b[q][p] = a[((p + 1) % 3) * 3 + (q + 1) % 3] * a[((p + 2) % 3) * 3 + (q + 2) % 3] -
                 a[((p + 1) % 3) * 3 + (q + 2) % 3] * a[((p + 2) % 3) * 3 + (q + 1) % 3];
-  const double det = a[0] * b[0][0] + a[1] * b[1][0] + a[2] * b[2][0];
+  const double det = a[0] * b[0][0] + a[1] * b[1][0] + a[2] * b[2][0], rdet = 1.0 / det;
 #pragma unroll
   for (int p = 0; p < 3; ++p)
 #pragma unroll
-    for (int q = 0; q < 3; ++q) g.inv[3 * p + q] = b[p][q] / det;
+    for (int q = 0; q < 3; ++q) g.inv[3 * p + q] = b[p][q] * rdet;
   double reach[3];
   int n[3];
 #pragma unroll
@@ -229,6 +231,22 @@ __global__ void k_grid_setup(int nframes, int natoms, const int64_t* __restrict_
 // path; ids past the frame's bins start at the next frame's first atom.
 constexpr int FRAME_CELLS_MAX = 8192;
 constexpr int FRAME_CELLS_THREADS = 1024;
+constexpr int FC_PER = 4;  // atoms per thread whose loads are issued together
+// LDS bin word of an atom: local bin id (13 bits, FRAME_CELLS_MAX bins at
+// most), then its wrap + 32 in 6 bits per direction; the sign bit marks a
+// wrap outside [-32, 31] (recomputed where needed).
+constexpr int BIN_ID_BITS = 13, BIN_ID_MASK = (1 << BIN_ID_BITS) - 1;
+static_assert(FRAME_CELLS_MAX <= (1 << BIN_ID_BITS), "bin ids must fit the bin word");
+__device__ __forceinline__ int bin_word(int id, const int* w) {
+  bool fit = true;
+  int word = id;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    fit = fit && w[d] >= -32 && w[d] < 32;
+    word |= ((w[d] + 32) & 63) << (BIN_ID_BITS + 6 * d);
+  }
+  return fit ? word : (id | INT32_MIN);
+}
 
 // One atom of a bin-ordered cell list: its position moved into the home cell
 // (x - wrap . lattice rows), index and species, so that a neighbour candidate
@@ -342,68 +360,115 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
   int* tmp = cur + FRAME_CELLS_MAX;      // [na] atoms placed per bin (unordered)
   for (int k = t; k <= nbins; k += nt) cnt[k] = 0;
   __syncthreads();
-  for (int k = t; k < na; k += nt) {
-    double fr[3];
-    const double* x = pos + 3 * (b0 + k);
-    for (int q = 0; q < 3; ++q) fr[q] = frac_coord(g, x, q);
-    int c[3];
-    for (int q = 0; q < 3; ++q) {
-      const double w = (g.per & (1 << q)) ? floor(fr[q]) : 0.0;
-      wrap[3 * (b0 + k) + q] = (int)w;
-      c[q] = min(g.n[q] - 1, max(0, (int)((fr[q] - w) * g.n[q])));
+  // the thread's atoms FC_PER at a time: every position loaded before any is binned
+  for (int k0 = t; k0 < na; k0 += FC_PER * nt) {
+    double xs[FC_PER][3];
+#pragma unroll
+    for (int r = 0; r < FC_PER; ++r) {
+      const int k = k0 + r * nt;
+      if (k < na)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) xs[r][q] = pos[3 * (b0 + k) + q];
     }
-    const int id = c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
-    bin[k] = id;
-    cell_of[b0 + k] = (int)b0 + id;
-    in_cur[b0 + k] = 0;
-    atom_frame[b0 + k] = f;
-    atomicAdd(cnt + id, 1);
+#pragma unroll
+    for (int r = 0; r < FC_PER; ++r) {
+      const int k = k0 + r * nt;
+      if (k >= na) break;
+      double fr[3];
+      for (int q = 0; q < 3; ++q) fr[q] = frac_coord(g, xs[r], q);
+      int c[3], wi[3];
+      for (int q = 0; q < 3; ++q) {
+        const double w = (g.per & (1 << q)) ? floor(fr[q]) : 0.0;
+        wi[q] = (int)w;
+        wrap[3 * (b0 + k) + q] = wi[q];
+        c[q] = min(g.n[q] - 1, max(0, (int)((fr[q] - w) * g.n[q])));
+      }
+      const int id = c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
+      bin[k] = bin_word(id, wi);
+      cell_of[b0 + k] = (int)b0 + id;
+      in_cur[b0 + k] = 0;
+      atom_frame[b0 + k] = f;
+      atomicAdd(cnt + id, 1);
+    }
   }
   __syncthreads();
-  {  // exclusive scan of the bin counts: chunk sums per thread, then a block scan of the sums
-    const int chunk = (nbins + nt - 1) / nt;
+  {  // exclusive scan of the bin counts: chunk sums per thread, wave prefix
+     // sums, then a scan of the wave totals
+    __shared__ int wtot[FRAME_CELLS_THREADS / WAVE];
+    const int chunk = (nbins + nt - 1) / nt, lane = t % WAVE, wv = t / WAVE;
     const int c0 = t * chunk, c1 = min(nbins, c0 + chunk);
-    long s = 0;
+    int s = 0;
     for (int k = c0; k < c1; ++k) s += cnt[k];
-    red_lo[t] = (double)s;
-    __syncthreads();
-    for (int w = 1; w < nt; w <<= 1) {
-      const double x = t >= w ? red_lo[t - w] : 0.0;
-      __syncthreads();
-      red_lo[t] += x;
-      __syncthreads();
+    int incl = s;
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+      const int y = __shfl_up(incl, d, WAVE);
+      if (lane >= d) incl += y;
     }
-    int run = t ? (int)red_lo[t - 1] : 0;
+    if (lane == WAVE - 1) wtot[wv] = incl;
+    __syncthreads();
+    if (wv == 0) {
+      constexpr int NW = FRAME_CELLS_THREADS / WAVE;
+      int x = lane < NW ? wtot[lane] : 0;
+#pragma unroll
+      for (int d = 1; d < NW; d <<= 1) {
+        const int y = __shfl_up(x, d, WAVE);
+        if (lane >= d) x += y;
+      }
+      if (lane < NW) wtot[lane] = x;
+    }
+    __syncthreads();
+    int run = (wv ? wtot[wv - 1] : 0) + incl - s;
     for (int k = c0; k < c1; ++k) {
       const int v = cnt[k];
       cnt[k] = run;
       cur[k] = run;
       run += v;
     }
-    if (t == nt - 1) cnt[nbins] = (int)red_lo[nt - 1];
+    if (t == nt - 1) cnt[nbins] = wtot[FRAME_CELLS_THREADS / WAVE - 1];
   }
   __syncthreads();
   for (int k = t; k <= na; k += nt) cell_start[b0 + k] = (int)b0 + (k < nbins ? cnt[k] : na);
-  for (int k = t; k < na; k += nt) tmp[atomicAdd(cur + bin[k], 1)] = k;
+  for (int k = t; k < na; k += nt) tmp[atomicAdd(cur + (bin[k] & BIN_ID_MASK), 1)] = k;
   __syncthreads();
   // ascending atom order inside each bin: rank among the bin's members
   const double* a_rows = cell + 9 * f;
-  for (int p = t; p < na; p += nt) {
-    const int k = tmp[p], id = bin[k];
-    const int s0 = cnt[id], s1 = cnt[id + 1];
-    int r0 = 0, r1 = 0, r2 = 0, r3 = 0, q = s0;
-    for (; q + 3 < s1; q += 4) {  // four independent LDS reads in flight
-      r0 += tmp[q] < k;
-      r1 += tmp[q + 1] < k;
-      r2 += tmp[q + 2] < k;
-      r3 += tmp[q + 3] < k;
+  for (int p0 = t; p0 < na; p0 += FC_PER * nt) {
+    double xs[FC_PER][3];
+    int ks[FC_PER], ts[FC_PER];
+#pragma unroll
+    for (int r = 0; r < FC_PER; ++r) {
+      const int p = p0 + r * nt;
+      if (p < na) {
+        const int k = tmp[p];
+        ks[r] = k;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) xs[r][q] = pos[3 * (b0 + k) + q];
+        ts[r] = types[b0 + k];
+      }
     }
-    for (; q < s1; ++q) r0 += tmp[q] < k;
-    // the wrap of the binning pass, recomputed from the same expression
-    const double* x = pos + 3 * (b0 + k);
-    int w[3];
-    for (int d = 0; d < 3; ++d) w[d] = (g.per & (1 << d)) ? (int)floor(frac_coord(g, x, d)) : 0;
-    cell_pack[b0 + s0 + (r0 + r1) + (r2 + r3)] = cell_atom(x, w, a_rows, (int)b0 + k, types[b0 + k]);
+#pragma unroll
+    for (int r = 0; r < FC_PER; ++r) {
+      if (p0 + r * nt >= na) break;
+      const int k = ks[r], bw = bin[k], id = bw & BIN_ID_MASK;
+      const int s0 = cnt[id], s1 = cnt[id + 1];
+      int r0 = 0, r1 = 0, r2 = 0, r3 = 0, q = s0;
+      for (; q + 3 < s1; q += 4) {  // four independent LDS reads in flight
+        r0 += tmp[q] < k;
+        r1 += tmp[q + 1] < k;
+        r2 += tmp[q + 2] < k;
+        r3 += tmp[q + 3] < k;
+      }
+      for (; q < s1; ++q) r0 += tmp[q] < k;
+      // the wrap of the binning pass: from the bin word, or recomputed from
+      // the same expression when it did not fit there
+      int w[3];
+      if (bw >= 0)
+        for (int d = 0; d < 3; ++d) w[d] = ((bw >> (BIN_ID_BITS + 6 * d)) & 63) - 32;
+      else
+        for (int d = 0; d < 3; ++d) w[d] = (g.per & (1 << d)) ? (int)floor(frac_coord(g, xs[r], d)) : 0;
+      cell_pack[b0 + s0 + (r0 + r1) + (r2 + r3)] = cell_atom(xs[r], w, a_rows, (int)b0 + k, ts[r]);
+    }
   }
 }
 

@@ -226,6 +226,29 @@ def test_cutoff_ties_keep_lists_symmetric(lib, oracle_lib):
         assert np.abs(r["forces"] - ref["forces"]).max() <= tol * (1 + np.abs(ref["forces"]).max()), prec
 
 
+def test_atoms_far_outside_the_cell(lib, oracle_lib):
+    """Atoms moved by whole lattice vectors far outside the cell (wraps up to
+    60 cells, beyond what the per-frame binning keeps in its LDS bin word):
+    the same EFS as the wrapped frame and the oracle."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    print("[case] atoms far outside the cell", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frame = W.diamond(ncell=2, species=2, seed=12)
+    a = np.asarray(frame["cell"], np.float64)
+    rng = np.random.default_rng(5)
+    shifts = rng.integers(-60, 61, (len(frame["types"]), 3))
+    shifts[::3] = 0
+    far = dict(frame, positions=np.asarray(frame["positions"]) + shifts @ a)
+    ref = _oracle_efs(oracle_lib, path, [frame])
+    m = Sus2Model(path)
+    for fr in (frame, far):
+        r = m.calc_efs([fr], "fp64")
+        assert abs(r["energy"][0] - ref["energy"][0]) <= 1e-9 * (1 + abs(ref["energy"][0]))
+        assert np.abs(r["forces"] - ref["forces"]).max() <= 1e-9 * (1 + np.abs(ref["forces"]).max())
+        assert np.abs(r["virial"] - ref["virial"]).max() <= 1e-9 * (1 + np.abs(ref["virial"]).max())
+
+
 def test_device_entry_point_matches_host(lib):
     import torch
     from superlinear_amd import workloads as W
