@@ -275,22 +275,45 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
   wave_contract<T, KK>(m, p, GATE, mom, v);
   wave_graph_forward(p, v, scratch);
   if (GATE) {
-    // h_i = W S_i (lanes over mu; S gathered into the free scratch first, four
-    // partial sums in flight), node values of the gate range for the reverse
+    // h_i = W S_i (S gathered into the free scratch first; narrow shapes
+    // spread each channel's sum over several lanes), node values of the gate
+    // range for the reverse
     T* sS = scratch;
     for (int q = lane; q < m.Qg; q += WAVE) sS[q] = v[m.gate_nodes[q]];
     __syncthreads();
-    for (int mu = lane; mu < F; mu += WAVE) {
-      T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
-      int q = 0;
-      for (; q + 3 < m.Qg; q += 4) {
-        s0 += m.W(mu, q) * sS[q];
-        s1 += m.W(mu, q + 1) * sS[q + 1];
-        s2 += m.W(mu, q + 2) * sS[q + 2];
-        s3 += m.W(mu, q + 3) * sS[q + 3];
+    if constexpr (F <= WAVE / 2) {
+      // HP lanes per channel, each over a contiguous part of the scalars; the
+      // parts added in order by the part-0 lane
+      constexpr int HP = F <= WAVE / 8 ? 8 : F <= WAVE / 4 ? 4 : 2;
+      const int mu = lane % F, part = lane / F;
+      const int qp = (m.Qg + HP - 1) / HP, q0 = min(m.Qg, part * qp), q1 = min(m.Qg, q0 + qp);
+      T s0 = T(0), s1 = T(0);
+      if (part < HP) {
+        int q = q0;
+        for (; q + 1 < q1; q += 2) {
+          s0 += m.W(mu, q) * sS[q];
+          s1 += m.W(mu, q + 1) * sS[q + 1];
+        }
+        if (q < q1) s0 += m.W(mu, q) * sS[q];
       }
-      for (; q < m.Qg; ++q) s0 += m.W(mu, q) * sS[q];
-      h[(size_t)i * F + mu] = (s0 + s1) + (s2 + s3);
+      T s = s0 + s1;
+      T tot = s;
+#pragma unroll
+      for (int k = 1; k < HP; ++k) tot += __shfl(s, min(lane + k * F, WAVE - 1), WAVE);
+      if (lane < F) h[(size_t)i * F + mu] = tot;
+    } else {
+      for (int mu = lane; mu < F; mu += WAVE) {
+        T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+        int q = 0;
+        for (; q + 3 < m.Qg; q += 4) {
+          s0 += m.W(mu, q) * sS[q];
+          s1 += m.W(mu, q + 1) * sS[q + 1];
+          s2 += m.W(mu, q + 2) * sS[q + 2];
+          s3 += m.W(mu, q + 3) * sS[q + 3];
+        }
+        for (; q < m.Qg; ++q) s0 += m.W(mu, q) * sS[q];
+        h[(size_t)i * F + mu] = (s0 + s1) + (s2 + s3);
+      }
     }
     T* out = vg + (size_t)i * p.node_cnt;
     for (int n = lane; n < p.node_cnt; n += WAVE) out[n] = v[p.node_lo + n];
