@@ -140,8 +140,9 @@ __device__ inline void grid_basis(const double* c, int per, double* a) {
 // covers (2 BIN_DIV + 1)^3 bins (far ones culled) of few atoms each.
 constexpr double BIN_DIV = 2.0;
 // Per-atom table of hits before each bin offset (reverse positions), for
-// grids of at most NOF_TAB offsets (search extents up to 3 bins).
-constexpr int NOF_TAB = 343;
+// grids of at most NOF_TAB offsets (search extents up to 2 bins, the usual
+// case with bins of r_c / 2; wider searches take a binary search instead).
+constexpr int NOF_TAB = 125;
 
 __device__ inline void grid_finish(Grid& g, const double* a, double rc, long na) {
   double b[3][3];
