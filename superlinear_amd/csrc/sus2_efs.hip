@@ -729,27 +729,17 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
 
 // Copy every atom's collected hits from its fixed-capacity slots to the CSR
 // records (atoms whose count exceeded the capacity are filled directly).
-// With in_edges, also place each copied edge in its target's incoming list
-// (atomic cursors, so unordered; sorted afterwards), as k_incoming does.
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_compact(int natoms, int cap, const int* __restrict__ cnt,
                                                   const int* __restrict__ off, const Rec<T>* __restrict__ slots,
-                                                  Rec<T>* __restrict__ rec, int* __restrict__ in_cur,
-                                                  int* __restrict__ in_edges) {
+                                                  Rec<T>* __restrict__ rec) {
   const int i = blockIdx.x;
   if (i >= natoms) return;
   const int n = cnt[i];
   if (n > cap) return;
   const Rec<T>* src = slots + (size_t)i * cap;
-  const int o = off[i];
-  for (int k = threadIdx.x; k < n; k += WAVE) {
-    const Rec<T> q = src[k];
-    rec[o + k] = q;
-    if (in_edges) {
-      const int t = rec_j(q);
-      in_edges[off[t] + atomicAdd(in_cur + t, 1)] = o + k;
-    }
-  }
+  Rec<T>* dst = rec + off[i];
+  for (int k = threadIdx.x; k < n; k += WAVE) dst[k] = src[k];
 }
 
 // Copy of every atom's hits from its slots to the CSR records, and the
@@ -962,25 +952,11 @@ template void fill_zero<int>(Workspace&, int*, size_t, hipStream_t);
 static void grow_slots(Workspace& ws, long maxcnt);
 
 // Records from the slots and the incoming positions, when every count fits
-// the slots.  Up to REV_MAX_ATOMS atoms (a launch whose workgroups are all
-// resident, bound by the latency of each list) the reverse positions are
-// computed; beyond, where the per-edge reads of that computation bound the
-// kernel, the copy places the incoming edges with atomic cursors and the
-// lists are sorted into a fixed order.
-constexpr int REV_MAX_ATOMS = 16384;
-
+// the slots.
 template <class T>
 static void compact_rev(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   const int N = b.natoms;
   const std::string kn = std::string("compact<") + tname<T>() + ">";
-  if (N > REV_MAX_ATOMS) {
-    SUS2_LAUNCH(ws, st, kn, k_compact<T>, dim3(N), dim3(WAVE), 0, N, ws.nbr_cap, ws.nbr_cnt.get<int>(),
-                ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>(), ws.in_cur.get<int>(),
-                ws.in_edges.get<int>());
-    SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(LIST_THREADS), 0, N, ws.nbr_off.get<int>(),
-                ws.in_edges.get<int>(), ws.in_pos.get<int>());
-    return;
-  }
   SUS2_LAUNCH(ws, st, kn, k_compact_rev<T>, dim3(N), dim3(WAVE), 0, N,
               ws.nbr_cap, ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(),
               ws.nbr_oslot.get<int>(), ws.nbr_otab.get<uint16_t>(), ws.rec.get<Rec<T>>(), ws.in_pos.get<int>(),
@@ -1002,10 +978,8 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   ws.cell_cur.ensure(sizeof(int) * n1);
   ws.cell_atoms.ensure(sizeof(int) * n1);
   ws.cell_pack.ensure(sizeof(CellAtom) * n1);
-  if (N <= REV_MAX_ATOMS) {
-    ws.atom_home.ensure(sizeof(CellAtom) * n1);
-    ws.nbr_otab.ensure(sizeof(uint16_t) * n1 * NOF_TAB);
-  }
+  ws.atom_home.ensure(sizeof(CellAtom) * n1);
+  ws.nbr_otab.ensure(sizeof(uint16_t) * n1 * NOF_TAB);
   ws.in_cur.ensure(sizeof(int) * n1);
   ws.scan_tmp.ensure(8 * sizeof(int64_t));
   const int cap = ws.nbr_cap;
@@ -1059,9 +1033,8 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     SUS2_LAUNCH(ws, st, "neighbors" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(), b.frame_off, b.pos,
                 b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(),
                 ws.cell_pack.get<CellAtom>(), rc, (const int*)nullptr, ws.nbr_cnt.get<int>(),
-                ws.nbr_slots.get<Rec<T>>(), ws.nbr_oslot.get<int>(),
-                N <= REV_MAX_ATOMS ? ws.nbr_otab.get<uint16_t>() : (uint16_t*)nullptr,
-                N <= REV_MAX_ATOMS ? ws.atom_home.get<CellAtom>() : (CellAtom*)nullptr, cap, 0);
+                ws.nbr_slots.get<Rec<T>>(), ws.nbr_oslot.get<int>(), ws.nbr_otab.get<uint16_t>(),
+                ws.atom_home.get<CellAtom>(), cap, 0);
   }
   // edge capacity left by earlier builds, in records of T
   const long ecap = (long)std::min({ws.rec.bytes / sizeof(Rec<T>), ws.in_edges.bytes / sizeof(int),
@@ -1096,8 +1069,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     // filled directly, and the incoming lists are placed (atomic cursors) and
     // sorted into a fixed order afterwards
     SUS2_LAUNCH(ws, st, "compact" + tn, k_compact<T>, dim3(N), dim3(WAVE), 0, N, cap, ws.nbr_cnt.get<int>(),
-                ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>(), ws.in_cur.get<int>(),
-                (int*)nullptr);
+                ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>());
     SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(),
                 b.frame_off, b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
                 ws.cell_start.get<int>(), ws.cell_pack.get<CellAtom>(), rc, ws.nbr_off.get<int>(),

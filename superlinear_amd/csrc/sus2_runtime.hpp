@@ -83,8 +83,8 @@ struct Workspace {
   // cell-list neighbour build
   DevBuf grid, cell_of, wrap, cell_cnt, cell_start, cell_cur, cell_atoms;
   DevBuf cell_pack;   // [N] bin-ordered atoms: home-cell position, index, species
-  DevBuf atom_home;   // [N] home-cell position and bin id per atom (launches of at most 16384 atoms)
-  DevBuf nbr_otab;    // [N][125] uint16 hits before each bin offset (launches of at most 16384 atoms)
+  DevBuf atom_home;   // [N] home-cell position and bin id per atom
+  DevBuf nbr_otab;    // [N][125] uint16 hits before each bin offset
   DevBuf atom_frame;  // [N] frame index of every atom (written by the binning kernels)
   // parameter gradient: gate tangent h_dot, gate node tangents, type-free gate
   // moments and tangents, per-edge gate messages, gathered h_bar, per-workgroup

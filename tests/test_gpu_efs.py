@@ -101,13 +101,12 @@ def test_batch_uneven_l4k6(lib, oracle_lib):
     assert np.abs((r32["energy"] - ref["energy"]) / counts).max() <= 1e-6
 
 
-def test_batch_above_reverse_placement_limit(lib, oracle_lib):
-    """A batch of more than 16384 atoms: the incoming lists are placed with
-    atomic cursors and sorted (smaller launches compute each edge's reverse
-    position instead).  FP64 and FP32 against the oracle."""
+def test_batch_of_many_frames(lib, oracle_lib):
+    """A batch of 48 periodic frames (more than 16384 atoms, several
+    thousand neighbour-search workgroups): FP64 and FP32 against the oracle."""
     from superlinear_amd import workloads as W
     from superlinear_amd.model import Sus2Model
-    print("[case] batch above reverse placement limit", flush=True)
+    print("[case] batch of many frames", flush=True)
     path = W.model_path("effect_l3k4r2o2")
     frames = W.periodic_frames(48, 320, 440, 2, 0.05, seed=51)
     assert sum(len(f["types"]) for f in frames) > 16384
