@@ -776,6 +776,11 @@ __global__ __launch_bounds__(WAVE) void k_compact_rev(int natoms, int cap, const
   const int bin_i = atom_home[i].j;
   const int bin_start = cell_start[bin_i], bin_end = cell_start[bin_i + 1];
   const double rc2 = rc * rc;
+  // i's bin (the same for all of i's edges) staged once when it is small
+  __shared__ CellAtom sbin[WAVE];
+  const bool staged = bin_end - bin_start <= WAVE;
+  if (staged && bin_start + (int)threadIdx.x < bin_end) sbin[threadIdx.x] = cell_pack[bin_start + threadIdx.x];
+  __syncthreads();
   const Rec<T>* src = slots + (size_t)i * cap;
   const int* osrc = oslot + (size_t)i * cap;
   const int o_i = off[i];
@@ -813,7 +818,7 @@ __global__ __launch_bounds__(WAVE) void k_compact_rev(int natoms, int cap, const
     const bool zero = (sv[0] | sv[1] | sv[2]) == 0;
     int before = 0;
     for (int p = bin_start; p < bin_end; ++p) {  // i is in this bin: the loop ends at it
-      const CellAtom c = cell_pack[p];
+      const CellAtom c = staged ? sbin[p - bin_start] : cell_pack[p];
       if (c.j == i) break;
       if (c.j == t && zero) continue;
       const double d0 = (c.x - ht.x) + sx, d1 = (c.y - ht.y) + sy, d2 = (c.z - ht.z) + sz;
