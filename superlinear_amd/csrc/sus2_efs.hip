@@ -594,8 +594,8 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
                                                     Rec<T>* __restrict__ rec, int* __restrict__ oslot,
                                                     uint16_t* __restrict__ otab, CellAtom* __restrict__ atom_home,
                                                     int cap, int over) {
-  __shared__ int o_b[WAVE], o_cum[WAVE], o_z[WAVE], o_hits[WAVE];
-  __shared__ double o_sh[3][WAVE];  // image shift of each bin offset (lattice rows); o_z: shift zero
+  __shared__ int o_b[2 * WAVE], o_cum[2 * WAVE], o_z[2 * WAVE], o_hits[2 * WAVE];
+  __shared__ double o_sh[3][2 * WAVE];  // image shift of each bin offset (lattice rows); o_z: shift zero
   const int i = blockIdx.x, lane = threadIdx.x;
   if (i >= natoms) return;
   if (cap == 0 && cnt[i] <= over) return;
@@ -624,49 +624,63 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
   int count = 0;
   const long w0 = cap ? (long)i * cap : (long)off[i];
   const int n0 = 2 * g.S[0] + 1, n1 = 2 * g.S[1] + 1, n2 = 2 * g.S[2] + 1, nof = n0 * n1 * n2;
-  for (int ob = 0; ob < nof; ob += WAVE) {
-    const int o = ob + lane;
-    int nb = 0, bs = 0, s0 = 0, s1 = 0, s2 = 0;
-    if (o < nof) {
-      int x = c0 + o % n0 - g.S[0], y = c1 + (o / n0) % n1 - g.S[1], z = c2 + o / (n0 * n1) - g.S[2];
-      bool ok = true;
-      if (g.per & 1) { s0 = floor_div(x, g.n[0]); x -= s0 * g.n[0]; }
-      else ok = ok && x >= 0 && x < g.n[0];
-      if (g.per & 2) { s1 = floor_div(y, g.n[1]); y -= s1 * g.n[1]; }
-      else ok = ok && y >= 0 && y < g.n[1];
-      if (g.per & 4) { s2 = floor_div(z, g.n[2]); z -= s2 * g.n[2]; }
-      else ok = ok && z >= 0 && z < g.n[2];
-      if (ok) {
-        const double u0 = (x + 0.5) / g.n[0] + s0, u1 = (y + 0.5) / g.n[1] + s1, u2 = (z + 0.5) / g.n[2] + s2;
-        const double ex = g.org[0] + u0 * g.A[0] + u1 * g.A[3] + u2 * g.A[6] - gx;
-        const double ey = g.org[1] + u0 * g.A[1] + u1 * g.A[4] + u2 * g.A[7] - gy;
-        const double ez = g.org[2] + u0 * g.A[2] + u1 * g.A[5] + u2 * g.A[8] - gz;
-        ok = ex * ex + ey * ey + ez * ez <= far2;
+  // offsets OPL at a time, two adjacent ones per lane (o = ob + 2 lane, +1),
+  // so that a 5 x 5 x 5 search resolves all its bins in one chunk
+  constexpr int OPL = 2 * WAVE;
+  for (int ob = 0; ob < nof; ob += OPL) {
+    int nb2[2], bs2[2], sv2[2][3];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int o = ob + 2 * lane + u;
+      int nb = 0, bs = 0, s0 = 0, s1 = 0, s2 = 0;
+      if (o < nof) {
+        int x = c0 + o % n0 - g.S[0], y = c1 + (o / n0) % n1 - g.S[1], z = c2 + o / (n0 * n1) - g.S[2];
+        bool ok = true;
+        if (g.per & 1) { s0 = floor_div(x, g.n[0]); x -= s0 * g.n[0]; }
+        else ok = ok && x >= 0 && x < g.n[0];
+        if (g.per & 2) { s1 = floor_div(y, g.n[1]); y -= s1 * g.n[1]; }
+        else ok = ok && y >= 0 && y < g.n[1];
+        if (g.per & 4) { s2 = floor_div(z, g.n[2]); z -= s2 * g.n[2]; }
+        else ok = ok && z >= 0 && z < g.n[2];
+        if (ok) {
+          const double u0 = (x + 0.5) / g.n[0] + s0, u1 = (y + 0.5) / g.n[1] + s1, u2 = (z + 0.5) / g.n[2] + s2;
+          const double ex = g.org[0] + u0 * g.A[0] + u1 * g.A[3] + u2 * g.A[6] - gx;
+          const double ey = g.org[1] + u0 * g.A[1] + u1 * g.A[4] + u2 * g.A[7] - gy;
+          const double ez = g.org[2] + u0 * g.A[2] + u1 * g.A[5] + u2 * g.A[8] - gz;
+          ok = ex * ex + ey * ey + ez * ez <= far2;
+        }
+        if (ok) {
+          const int c = cbase + x + g.n[0] * (y + g.n[1] * z);
+          bs = cell_start[c];
+          nb = cell_start[c + 1] - bs;
+        }
       }
-      if (ok) {
-        const int c = cbase + x + g.n[0] * (y + g.n[1] * z);
-        bs = cell_start[c];
-        nb = cell_start[c + 1] - bs;
-      }
+      nb2[u] = nb;
+      bs2[u] = bs;
+      sv2[u][0] = s0; sv2[u][1] = s1; sv2[u][2] = s2;
     }
-    int incl = nb;
+    const int pair = nb2[0] + nb2[1];
+    int incl = pair;
 #pragma unroll
     for (int d = 1; d < WAVE; d <<= 1) {
       const int t = __shfl_up(incl, d, WAVE);
       if (lane >= d) incl += t;
     }
     const int total = __shfl(incl, WAVE - 1, WAVE);
-    o_b[lane] = bs;
-    o_cum[lane] = incl - nb;
-    // open directions have zero shifts
-    const int sv[3] = {s0, s1, s2};
-    o_sh[0][lane] = lattice_comb(sv, a, 0);
-    o_sh[1][lane] = lattice_comb(sv, a, 1);
-    o_sh[2][lane] = lattice_comb(sv, a, 2);
-    o_z[lane] = (s0 | s1 | s2) == 0;
-    o_hits[lane] = 0;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int q = 2 * lane + u;
+      o_b[q] = bs2[u];
+      o_cum[q] = incl - pair + (u ? nb2[0] : 0);
+      // open directions have zero shifts
+      o_sh[0][q] = lattice_comb(sv2[u], a, 0);
+      o_sh[1][q] = lattice_comb(sv2[u], a, 1);
+      o_sh[2][q] = lattice_comb(sv2[u], a, 2);
+      o_z[q] = (sv2[u][0] | sv2[u][1] | sv2[u][2]) == 0;
+      o_hits[q] = 0;
+    }
     __syncthreads();
-    const int nvalid = min(WAVE, nof - ob), chunk_start = count;
+    const int nvalid = min(OPL, nof - ob), chunk_start = count;
     // candidate k of the flattened range: its bin offset (binary search), the
     // packed atom, and whether it is a neighbour (d = (x_j - x_i) + shift, both
     // in the home cell; the atom itself at zero shift is not)
@@ -715,13 +729,17 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
     }
     __syncthreads();
     if (otab && nof <= NOF_TAB) {  // hits before each offset of the chunk
-      int h = o_hits[lane], inc = h;
+      const int h0 = o_hits[2 * lane], h1 = o_hits[2 * lane + 1], hp = h0 + h1;
+      int inc = hp;
 #pragma unroll
       for (int d = 1; d < WAVE; d <<= 1) {
         const int y = __shfl_up(inc, d, WAVE);
         if (lane >= d) inc += y;
       }
-      if (ob + lane < nof) otab[(size_t)i * NOF_TAB + ob + lane] = (uint16_t)(chunk_start + inc - h);
+      const int e0 = chunk_start + inc - hp;
+      const size_t trow = (size_t)i * NOF_TAB + ob + 2 * lane;
+      if (ob + 2 * lane < nof) otab[trow] = (uint16_t)e0;
+      if (ob + 2 * lane + 1 < nof) otab[trow + 1] = (uint16_t)(e0 + h0);
     }
   }
   if (cap && lane == 0) cnt[i] = count;
