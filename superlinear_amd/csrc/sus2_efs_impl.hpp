@@ -215,10 +215,13 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
   T acc[NM];
 #pragma unroll
   for (int t = 0; t < NM; ++t) acc[t] = T(0);
+  // the next batch's record loaded while this one is worked on
+  Rec<T> qn = e0 + lane < e1 ? rec[e0 + lane] : Rec<T>{T(1), T(0), T(0), T(0)};
   for (int base = e0; base < e1; base += WAVE) {
     const int e = base + lane;
+    const Rec<T> q = qn;
+    if (e + WAVE < e1) qn = rec[e + WAVE];
     if (e < e1) {
-      const Rec<T> q = rec[e];
       const int j = rec_j(q);
       const T r = dsqrt(q.x * q.x + q.y * q.y + q.z * q.z);
       const T inv = T(1) / r;
