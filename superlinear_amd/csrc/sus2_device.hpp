@@ -322,6 +322,21 @@ __device__ __forceinline__ T wave_sum(T v) {
   return v;
 }
 
+// N wave sums at once, level by level: each butterfly level issues all N
+// shuffles before any is added, so their latencies overlap.  Every lane ends
+// with the N totals (the same association as wave_sum).
+template <class T, int N>
+__device__ __forceinline__ void wave_sum_n(T (&v)[N]) {
+#pragma unroll
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    T t[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) t[k] = __shfl_xor(v[k], o, WAVE);
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] += t[k];
+  }
+}
+
 __device__ __forceinline__ int lane_id() { return threadIdx.x; }
 
 // rank node of channel a at (l, m) for a pass

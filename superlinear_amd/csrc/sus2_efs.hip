@@ -917,8 +917,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(int nsub, const int64_
       if (site_out) site_out[i] = se;
     }
   }
-#pragma unroll
-  for (int q = 0; q < 7; ++q) acc[q] = wave_sum(acc[q]);
+  wave_sum_n(acc);
   if (t % WAVE == 0)
 #pragma unroll
     for (int q = 0; q < 7; ++q) red[q][t / WAVE] = acc[q];

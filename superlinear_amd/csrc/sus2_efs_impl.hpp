@@ -466,16 +466,17 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
   }
 #pragma unroll
   for (int k = 0; k < 6; ++k) sV[(acur * 6 + k) * WAVE + lane] = w[k];
+  T s[NA * 6];
+#pragma unroll
+  for (int k = 0; k < NA * 6; ++k) s[k] = sV[k * WAVE + lane];
+  wave_sum_n(s);
 #pragma unroll
   for (int a = 0; a < NA; ++a) {
     if (a >= na) break;
-    T s[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) s[k] = wave_sum(sV[(a * 6 + k) * WAVE + lane]);
     if (lane == 0) {
       T* o = fw + (size_t)(i0 + a) * 9 + 3;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) o[k] = GATE ? o[k] + s[k] : s[k];
+      for (int k = 0; k < 6; ++k) o[k] = GATE ? o[k] + s[a * 6 + k] : s[a * 6 + k];
     }
   }
 }
@@ -491,10 +492,10 @@ __device__ void gate_reverse_atom(const DModel<T>& m, int i, const T* __restrict
   const int lane = threadIdx.x;
   const DPass<T>& p = m.gp;
   const T* vi = vg + (size_t)i * p.node_cnt;
-  for (int n = lane; n < m.M; n += WAVE) {
+  wave_rows(0, m.M, [&](int n) {  // four node loads in flight per lane
     v[n] = (n >= p.node_lo && n < p.node_lo + p.node_cnt) ? vi[n - p.node_lo] : T(0);
     a[n] = T(0);
-  }
+  });
   __syncthreads();
   for (int q = lane; q < m.Qg; q += WAVE) {
     T s0 = T(0), s1 = T(0);
@@ -537,16 +538,38 @@ __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __r
       const T* go = gedge + 3 * (size_t)k;
       fx += go[0]; fy += go[1]; fz += go[2];
     }
-  for (int k = in_off[i] + lane; k < in_off[i + 1]; k += WAVE) {  // incoming slots: contiguous
-    const T* gi = gedge_in + 3 * (size_t)k;
-    fx -= gi[0]; fy -= gi[1]; fz -= gi[2];
-    if (hgate) {
-      const T* ue = uedge + (size_t)k * F;
+  // incoming slots (contiguous); narrow shapes take two batches per step so
+  // that their loads are in flight together
+  constexpr int IN_STEP = F <= 16 ? 2 : 1;
+  const int k1 = in_off[i + 1];
+  for (int k0 = in_off[i] + lane; k0 < k1; k0 += IN_STEP * WAVE) {
 #pragma unroll
-      for (int mu = 0; mu < F; ++mu) hb[mu] += ue[mu];
+    for (int u = 0; u < IN_STEP; ++u) {
+      const int k = k0 + u * WAVE;
+      if (k >= k1) break;
+      const T* gi = gedge_in + 3 * (size_t)k;
+      fx -= gi[0]; fy -= gi[1]; fz -= gi[2];
+      if (hgate) {
+        const T* ue = uedge + (size_t)k * F;
+#pragma unroll
+        for (int mu = 0; mu < F; ++mu) hb[mu] += ue[mu];
+      }
     }
   }
-  fx = wave_sum(fx); fy = wave_sum(fy); fz = wave_sum(fz);
+  if (!hgate) {
+    T f3[3] = {fx, fy, fz};
+    wave_sum_n(f3);
+    fx = f3[0]; fy = f3[1]; fz = f3[2];
+  } else {  // the force and message sums together
+    T fh[3 + F];
+    fh[0] = fx; fh[1] = fy; fh[2] = fz;
+#pragma unroll
+    for (int mu = 0; mu < F; ++mu) fh[3 + mu] = hb[mu];
+    wave_sum_n(fh);
+    fx = fh[0]; fy = fh[1]; fz = fh[2];
+#pragma unroll
+    for (int mu = 0; mu < F; ++mu) hb[mu] = fh[3 + mu];
+  }
   if (lane == 0) {
     T* o = fw + (size_t)i * 9;
     if (GATE) { o[0] += fx; o[1] += fy; o[2] += fz; }
@@ -561,7 +584,7 @@ __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __r
   const int ti = types[i];
 #pragma unroll
   for (int mu = 0; mu < F; ++mu) {
-    const T s = wave_sum(hb[mu]) * m.alpha(ti, mu);
+    const T s = hb[mu] * m.alpha(ti, mu);
     if (hbar_out) {
       if (lane == 0) hbar_out[(size_t)i * F + mu] = s;
     } else if (lane == 0) {
