@@ -248,6 +248,29 @@ def test_atoms_far_outside_the_cell(lib, oracle_lib):
         assert np.abs(r["virial"] - ref["virial"]).max() <= 1e-9 * (1 + np.abs(ref["virial"]).max())
 
 
+def test_tiny_cells_wide_search(lib, oracle_lib):
+    """Periodic cells of one and two atoms, 2.2-2.6 A wide (one triclinic): the
+    5 A cutoff reaches three cells out, a 7 x 7 x 7 bin search whose incoming
+    positions come from a binary search over each target's hit offsets.
+    FP64 and FP32 against the oracle."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    print("[case] tiny cells", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frames = [dict(types=np.array([0], np.int32), positions=np.array([[0.1, 0.2, 0.3]]),
+                   cell=np.eye(3) * 2.2, periodic=True),
+              dict(types=np.array([0, 1], np.int32), positions=np.array([[0.0, 0.0, 0.0], [1.2, 1.1, 1.3]]),
+                   cell=np.array([[2.6, 0.0, 0.0], [0.7, 2.4, 0.0], [0.3, 0.5, 2.5]]), periodic=True)]
+    ref = _oracle_efs(oracle_lib, path, frames)
+    m = Sus2Model(path)
+    r = m.calc_efs(frames, "fp64")
+    assert np.abs(r["energy"] - ref["energy"]).max() <= 1e-10 * (1 + np.abs(ref["energy"]).max())
+    assert np.abs(r["forces"] - ref["forces"]).max() <= 1e-10 * (1 + np.abs(ref["forces"]).max())
+    assert np.abs(r["virial"] - ref["virial"]).max() <= 1e-10 * (1 + np.abs(ref["virial"]).max())
+    r32 = m.calc_efs(frames, "fp32")
+    assert np.abs(r32["forces"] - ref["forces"]).max() <= 1e-5
+
+
 def test_device_entry_point_matches_host(lib):
     import torch
     from superlinear_amd import workloads as W
