@@ -136,6 +136,27 @@ def test_large_frame_grid_path(lib, oracle_lib):
     assert np.abs(r["virial"] - ref["virial"]).max() <= 1e-10 * (1 + np.abs(ref["virial"]).max())
 
 
+def test_large_open_frames_grid_path(lib, oracle_lib):
+    """A 9000-atom molecule and an 8600-atom slab (periodic along two lattice
+    rows, the third zero): both above the per-frame cell-list limit, so the
+    device-wide binning with open-direction extents builds their lists.
+    FP64 and FP32 against the oracle."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    print("[case] large open frames grid path", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frames = (W.molecule_frames(1, 9000, 9000, 2, seed=61, density=0.04)
+              + W.partial_frames(1, 8600, 8600, 2, seed=62, axes=2, density=0.04))
+    ref = _oracle_efs(oracle_lib, path, frames)
+    m = Sus2Model(path)
+    r = m.calc_efs(frames, "fp64")
+    assert np.abs(r["energy"] - ref["energy"]).max() <= 1e-10 * (1 + np.abs(ref["energy"]).max())
+    assert np.abs(r["forces"] - ref["forces"]).max() <= 1e-10 * (1 + np.abs(ref["forces"]).max())
+    assert np.abs(r["virial"] - ref["virial"]).max() <= 1e-10 * (1 + np.abs(ref["virial"]).max())
+    r32 = m.calc_efs(frames, "fp32")
+    assert np.abs(r32["forces"] - ref["forces"]).max() <= 1e-5
+
+
 def test_neighbor_slot_overflow_then_adapted(lib, oracle_lib):
     """A dense frame (about 180 neighbours per atom) overflows the default
     128 per-atom neighbour slots: the overflowed atoms are filled directly and
