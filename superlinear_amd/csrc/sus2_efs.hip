@@ -310,6 +310,15 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
     g.per = grid_mask(cell + 9 * f, periodic[f]);
     g_sh = g;
   }
+  // the thread's first FC_PER positions, loaded while the grid is formed
+  double xs[FC_PER][3];
+#pragma unroll
+  for (int r = 0; r < FC_PER; ++r) {
+    const int k = t + r * nt;
+    if (k < na)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) xs[r][q] = pos[3 * (b0 + k) + q];
+  }
   __syncthreads();
   // basis rows: lattice rows for periodic directions, unit normals for open ones
   double a[9];
@@ -361,16 +370,17 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
   int* tmp = cur + FRAME_CELLS_MAX;      // [na] atoms placed per bin (unordered)
   for (int k = t; k <= nbins; k += nt) cnt[k] = 0;
   __syncthreads();
-  // the thread's atoms FC_PER at a time: every position loaded before any is binned
+  // the thread's atoms FC_PER at a time: every position loaded before any is
+  // binned (the first FC_PER are already in registers)
   for (int k0 = t; k0 < na; k0 += FC_PER * nt) {
-    double xs[FC_PER][3];
+    if (k0 != t)
 #pragma unroll
-    for (int r = 0; r < FC_PER; ++r) {
-      const int k = k0 + r * nt;
-      if (k < na)
+      for (int r = 0; r < FC_PER; ++r) {
+        const int k = k0 + r * nt;
+        if (k < na)
 #pragma unroll
-        for (int q = 0; q < 3; ++q) xs[r][q] = pos[3 * (b0 + k) + q];
-    }
+          for (int q = 0; q < 3; ++q) xs[r][q] = pos[3 * (b0 + k) + q];
+      }
 #pragma unroll
     for (int r = 0; r < FC_PER; ++r) {
       const int k = k0 + r * nt;
