@@ -50,6 +50,7 @@ CASES = [
     ("effect_l3k4r2o2", "molecule", 2, 53),
     ("effect_l3k4r2o2", "mixed", 2, 54),
     ("effect_l4k6r2o3", "mixed", 3, 55),
+    ("effect_l5k6r2o3", "periodic", 2, 56),
 ]
 
 
