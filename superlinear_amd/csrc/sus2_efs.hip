@@ -1127,18 +1127,23 @@ bool neighbors_verify(Workspace& ws, hipStream_t st) {
   // the scan runs early in the queued work: poll its report, checking the
   // stream now and then so that a failed or drained stream ends the wait
   volatile int64_t* rep = ws.host_report;
-  const int64_t want = ws.report_seq;
-  for (long polls = 1; rep[3] != want; ++polls) {
+  const uint64_t want = (uint64_t)(ws.report_seq & 0x7fffffff);
+  uint64_t w0 = 0, w1 = 0;
+  auto arrived = [&] {
+    w0 = (uint64_t)rep[0];
+    w1 = (uint64_t)rep[1];
+    return (w0 >> 32) == want && (w1 >> 32) == want;
+  };
+  for (long polls = 1; !arrived(); ++polls) {
     if (polls % 1024) continue;
     const hipError_t e = hipStreamQuery(st);
     if (e == hipErrorNotReady) continue;
     if (e != hipSuccess) throw Error(SUS2_E_DEVICE, std::string("neighbour build failed: ") + hipGetErrorString(e));
     std::atomic_thread_fence(std::memory_order_seq_cst);
-    if (rep[3] != want) throw Error(SUS2_E_DEVICE, "neighbour count report missing after the stream drained");
+    if (!arrived()) throw Error(SUS2_E_DEVICE, "neighbour count report missing after the stream drained");
   }
-  std::atomic_thread_fence(std::memory_order_acquire);
-  const long total = (long)rep[0], maxcnt = (long)rep[1];
-  const bool over = rep[2] != 0;
+  const long total = (long)(w1 & 0x7fffffff), maxcnt = (long)(w0 & 0xffffffff);
+  const bool over = (w1 >> 31) & 1;
   grow_slots(ws, maxcnt);
   if (over) {  // the queued work ran on empty lists: drain it; the next build sizes on the host
     SUS2_CHECK(hipStreamSynchronize(st));

@@ -163,13 +163,10 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(int n, int* __restrict__ 
     total[0] = carry;
     total[1] = m;
     total[2] = over;
-    if (report) {
-      report[0] = carry;
-      report[1] = m;
-      report[2] = over;
-      __threadfence_system();
-      report[3] = seq;
-      __threadfence_system();  // write the sequence back now, not at the stream's next system release
+    if (report) {  // two self-tagged words, each one 64-bit store (no fence)
+      const uint64_t tag = (uint64_t)(seq & 0x7fffffff) << 32;
+      report[0] = (int64_t)(tag | (uint64_t)(uint32_t)m);
+      report[1] = (int64_t)(tag | ((uint64_t)over << 31) | (uint64_t)(uint32_t)min(carry, (long)0x7fffffff));
     }
   }
 }

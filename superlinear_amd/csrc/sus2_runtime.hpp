@@ -102,9 +102,11 @@ struct Workspace {
   // the device's overflow report afterwards (neighbors_verify); nbr_pending
   // marks such a build whose report has not been read yet.
   bool nbr_optimistic = false, nbr_known = false, nbr_pending = false;
-  // Count report of such a build: [total, max, overflow, sequence], written by
-  // the scan kernel into coherent pinned host memory and polled by the host.
-  // Kept until the process ends (a device-written host page is never freed).
+  // Count report of such a build, written by the scan kernel into coherent
+  // pinned host memory and polled by the host: two 64-bit words, each tagged
+  // with the build's sequence number in its upper half (word 0: largest count;
+  // word 1: overflow flag in bit 31, total in bits 0-30, saturated).  Kept
+  // until the process ends (a device-written host page is never freed).
   int64_t* host_report = nullptr;
   int64_t report_seq = 0;
   ~Workspace();
@@ -115,8 +117,8 @@ struct Workspace {
 // With cap_lim / edge_lim: when the largest count exceeds cap_lim or the sum
 // exceeds edge_lim, total[2] = 1 and every count and offset is written as 0
 // (an empty list for every atom); otherwise total[2] = 0.
-// With report: total[0..2] are also written to report[0..2] and then seq to
-// report[3] (system scope).
+// With report: the largest count and the total with the overflow flag are
+// also written to report[0..1], each word tagged with seq in its upper half.
 void device_scan(int* cnt, int* off, int n, int64_t* total, hipStream_t st, int cap_lim = INT32_MAX,
                  long edge_lim = INT64_MAX, int64_t* report = nullptr, int64_t seq = 0);
 // exclusive scan of ws.nbr_cnt[0..n) into ws.nbr_off[0..n]; returns the total
