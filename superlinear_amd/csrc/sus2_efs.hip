@@ -1079,6 +1079,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     SUS2_CHECK(hipHostGetDevicePointer((void**)&report, ws.host_report, 0));
     device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), N, ws.scan_tmp.get<int64_t>(), st, cap, ecap, report,
                 ++ws.report_seq);
+    SUS2_CHECK(hipEventRecord(ws.report_ev, st));
     ws.nedges = ecap;
     ws.nbr_pending = true;
     if (N > 0) compact_rev<T>(ws, b, rc, st);
@@ -1121,6 +1122,9 @@ static void grow_slots(Workspace& ws, long maxcnt) {
   if (want > ws.nbr_cap) ws.nbr_cap = (int)std::min<long>(want, 4096);
 }
 
+// polls of the count report before a blocking wait (tens of microseconds)
+constexpr long SPIN_POLLS = 1L << 14;
+
 bool neighbors_verify(Workspace& ws, hipStream_t st) {
   if (!ws.nbr_pending) return false;
   ws.nbr_pending = false;
@@ -1135,6 +1139,12 @@ bool neighbors_verify(Workspace& ws, hipStream_t st) {
     return (w0 >> 32) == want && (w1 >> 32) == want;
   };
   for (long polls = 1; !arrived(); ++polls) {
+    if (polls == SPIN_POLLS) {
+      // a long build: sleep until the scan has finished instead of spinning
+      const hipError_t e = hipEventSynchronize(ws.report_ev);
+      if (e != hipSuccess) throw Error(SUS2_E_DEVICE, std::string("neighbour build failed: ") + hipGetErrorString(e));
+      continue;
+    }
     if (polls % 1024) continue;
     const hipError_t e = hipStreamQuery(st);
     if (e == hipErrorNotReady) continue;
