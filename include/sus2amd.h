@@ -36,7 +36,7 @@ enum sus2_status {
   SUS2_OK = 0,
   SUS2_E_INVALID = 1,     /* malformed model or arguments */
   SUS2_E_IO = 2,          /* file cannot be opened / written */
-  SUS2_E_UNSUPPORTED = 3, /* a model family or shape this build does not run */
+  SUS2_E_UNSUPPORTED = 3, /* a model family or shape without a compiled kernel instance */
   SUS2_E_DEVICE = 4,      /* HIP runtime failure */
   SUS2_E_RANGE = 5        /* species / index / size out of range */
 };
