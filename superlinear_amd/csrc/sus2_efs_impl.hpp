@@ -230,9 +230,11 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
       const T* x0j = tab.x0 + tj * F;
       const T* alj = tab.alpha + tj * F;
       if constexpr (F > 16) {  // wide shapes: each channel and harmonic staged as soon as formed
+        T hj[F];
+        if (!GATE && m.gated) load_row<T, F>(h + (size_t)j * F, hj);  // issued before the basis work
         radial_each<Cf, false>(m, GATE, tab, ti * m.C + tj, r, [&](int mu, T R, T) {
           T x1 = x0j[mu];
-          if (!GATE && m.gated) x1 += alj[mu] * h[(size_t)j * F + mu];
+          if (!GATE && m.gated) x1 += alj[mu] * hj[mu];
           sR[mu * LR + lane] = x0i[mu] * x1 * R;
         });
         sh_stream_values<T, LL>(q.x * inv, q.y * inv, q.z * inv,
