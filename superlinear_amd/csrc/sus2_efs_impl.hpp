@@ -558,6 +558,55 @@ __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __r
       }
     }
   }
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* v = reinterpret_cast<T*>(smem_raw);  // [M]
+  T* a = v + m.M;                          // [M]
+  T* hbs = a + m.M;                        // [F]
+  T* scratch = hbs + F;
+  if constexpr (F > 16) {
+    if (hgate) {
+      // wide shapes: the 3 + F lane sums through an LDS transpose (lane
+      // columns, then one row per lane summed in order), before v / a are used
+      T* tr = v;  // [3 + F][LR]
+      tr[lane] = fx;
+      tr[LR + lane] = fy;
+      tr[2 * LR + lane] = fz;
+#pragma unroll
+      for (int mu = 0; mu < F; ++mu) tr[(3 + mu) * LR + lane] = hb[mu];
+      __syncthreads();
+      T tot = T(0);
+      if (lane < 3 + F) {
+        const T* row = tr + lane * LR;
+        T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
+#pragma unroll 4
+        for (int e = 0; e < WAVE; e += 4) {
+          s0 += row[e];
+          s1 += row[e + 1];
+          s2 += row[e + 2];
+          s3 += row[e + 3];
+        }
+        tot = (s0 + s1) + (s2 + s3);
+      }
+      __syncthreads();
+      fx = __shfl(tot, 0, WAVE);
+      fy = __shfl(tot, 1, WAVE);
+      fz = __shfl(tot, 2, WAVE);
+      if (lane == 0) {
+        T* o = fw + (size_t)i * 9;
+        o[0] = fx; o[1] = fy; o[2] = fz;
+      }
+      const int ti = types[i];
+      if (lane >= 3 && lane < 3 + F) {
+        const int mu = lane - 3;
+        const T s = tot * m.alpha(ti, mu);
+        if (hbar_out) hbar_out[(size_t)i * F + mu] = s;
+        else hbs[mu] = s;
+      }
+      if (hbar_out) return;
+      gate_reverse_atom<Cf>(m, i, vg, gbar, hbs, v, a, scratch);
+      return;
+    }
+  }
   if (!hgate) {
     T f3[3] = {fx, fy, fz};
     wave_sum_n(f3);
@@ -578,11 +627,6 @@ __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __r
     else { o[0] = fx; o[1] = fy; o[2] = fz; }
   }
   if (!hgate) return;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* v = reinterpret_cast<T*>(smem_raw);  // [M]
-  T* a = v + m.M;                          // [M]
-  T* hbs = a + m.M;                        // [F]
-  T* scratch = hbs + F;
   const int ti = types[i];
 #pragma unroll
   for (int mu = 0; mu < F; ++mu) {
@@ -614,7 +658,10 @@ __global__ __launch_bounds__(WAVE) void k_gate_reverse(DModel<T> m, const T* __r
 
 template <class Cf>
 size_t gather_lds(const DModel<typename Cf::T>& m) {
-  return sizeof(typename Cf::T) * ((size_t)2 * m.M + Cf::F + std::max(graph_scratch(m.gp), 1));
+  const size_t walk = (size_t)2 * m.M + Cf::F + std::max(graph_scratch(m.gp), 1);
+  // wide shapes also transpose the 3 + F lane sums through the same region
+  const size_t sums = Cf::F > 16 ? (size_t)(3 + Cf::F) * LR : 0;
+  return sizeof(typename Cf::T) * std::max(walk, sums);
 }
 
 template <class Cf>
@@ -738,8 +785,9 @@ void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const Pai
                   ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
     }
     if (all > 0) {
-      allow_lds(k_gather<Cf, false>, lg);
-      SUS2_LAUNCH(ws, st, kname<Cf>("gather_outer"), (k_gather<Cf, false>), dim3(all), dim3(WAVE), 0, m, pv.types,
+      allow_lds(k_gather<Cf, false>, lg);  // (wide gated shapes sum their messages through LDS)
+      SUS2_LAUNCH(ws, st, kname<Cf>("gather_outer"), (k_gather<Cf, false>), dim3(all), dim3(WAVE),
+                  (Cf::F > 16 && m.gated) ? lg : 0, m, pv.types,
                   n, off, pv.in_off, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
                   ws.gbar.get<T>(), m.gated ? ws.hbar.get<T>() : (T*)nullptr, ws.fw.get<T>());
     }
