@@ -563,9 +563,9 @@ __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __r
   T* a = v + m.M;                          // [M]
   T* hbs = a + m.M;                        // [F]
   T* scratch = hbs + F;
-  if constexpr (F > 16) {
+  if constexpr (F >= 16) {
     if (hgate) {
-      // wide shapes: the 3 + F lane sums through an LDS transpose (lane
+      // F >= 16: the 3 + F lane sums through an LDS transpose (lane
       // columns, then one row per lane summed in order), before v / a are used
       T* tr = v;  // [3 + F][LR]
       tr[lane] = fx;
@@ -659,8 +659,8 @@ __global__ __launch_bounds__(WAVE) void k_gate_reverse(DModel<T> m, const T* __r
 template <class Cf>
 size_t gather_lds(const DModel<typename Cf::T>& m) {
   const size_t walk = (size_t)2 * m.M + Cf::F + std::max(graph_scratch(m.gp), 1);
-  // wide shapes also transpose the 3 + F lane sums through the same region
-  const size_t sums = Cf::F > 16 ? (size_t)(3 + Cf::F) * LR : 0;
+  // F >= 16 also transposes the 3 + F lane sums through the same region
+  const size_t sums = Cf::F >= 16 ? (size_t)(3 + Cf::F) * LR : 0;
   return sizeof(typename Cf::T) * std::max(walk, sums);
 }
 
@@ -785,9 +785,9 @@ void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const Pai
                   ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
     }
     if (all > 0) {
-      allow_lds(k_gather<Cf, false>, lg);  // (wide gated shapes sum their messages through LDS)
+      allow_lds(k_gather<Cf, false>, lg);  // (gated shapes with F >= 16 sum their messages through LDS)
       SUS2_LAUNCH(ws, st, kname<Cf>("gather_outer"), (k_gather<Cf, false>), dim3(all), dim3(WAVE),
-                  (Cf::F > 16 && m.gated) ? lg : 0, m, pv.types,
+                  (Cf::F >= 16 && m.gated) ? lg : 0, m, pv.types,
                   n, off, pv.in_off, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
                   ws.gbar.get<T>(), m.gated ? ws.hbar.get<T>() : (T*)nullptr, ws.fw.get<T>());
     }
