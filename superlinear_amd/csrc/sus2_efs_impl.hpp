@@ -442,15 +442,18 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
       // gate pass: keep the moment-adjoint LDS reads of this (l, m) here;
       // hoisted over the whole unrolled stream they would hold K H registers
       if constexpr (GATE) asm volatile("" ::: "memory");
+      // channel sums first (sum_k abar R', sum_k abar R), then Y and grad Y once
+      T adR = T(0), aR = T(0);
 #pragma unroll
       for (int k = 0; k < KK; ++k) {
         const int mu = k * L1 + l;
         const T av = sAi[hh * KK + k];
-        sY += av * dR[mu] * Y;
-        const T g = av * R[mu];
-        gx += g * yx; gy += g * yy; gz += g * yz;
+        adR += av * dR[mu];
+        aR += av * R[mu];
         if (hgate) us[mu] += av * Y;
       }
+      sY += adR * Y;
+      gx += aR * yx; gy += aR * yy; gz += aR * yz;
     });
     const T g0 = ux * sY + gx, g1 = uy * sY + gy, g2 = uz * sY + gz;
     const int slot = in_pos[e];  // the edge's position in its target's incoming list

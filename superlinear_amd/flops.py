@@ -10,7 +10,8 @@ H = (L+1)^2, F = K(L+1), B radial basis size, Kw warp blocks, KH = K H.
 * ``force``    (one pass, coordinate reverse):
   geometry 10 + radial basis and r-derivative Kw (13 + 7B)
   + contraction/envelope of R and R' F (4B + 6) + weights 4F
-  + harmonics and gradient 20H + adjoint contraction 11 KH
+  + harmonics and gradient 20H + adjoint contraction 4 KH + 8H (the channel
+  sums sum_k abar R and sum_k abar R' per harmonic, then Y and grad Y once)
   (+ 4 KH for the outer pass of an Effect model: the gate adjoint h_bar)
   + force / virial accumulation 21
 * ``survey_efs`` is the whole-step figure F_e^E (Effect) / F_e^S (Static)
@@ -55,7 +56,8 @@ def moments_per_edge(d):
 
 def force_per_edge(d, outer_effect=False):
     L, K, B, Kw, H, F, KH = dims_of(d)
-    return 10 + Kw * (13 + 7 * B) + F * (4 * B + 6) + 4 * F + 20 * H + 11 * KH + (4 * KH if outer_effect else 0) + 21
+    return (10 + Kw * (13 + 7 * B) + F * (4 * B + 6) + 4 * F + 20 * H + 4 * KH + 8 * H
+            + (4 * KH if outer_effect else 0) + 21)
 
 
 def survey_efs_per_edge(d):
