@@ -115,9 +115,22 @@ __device__ void wave_graph_forward(const DPass<T>& p, T* v, T* scratch) {
   for (int l = 0; l < p.nfl; ++l) {
     const int o0 = p.fl_off[l], o1 = p.fl_off[l + 1];
     const int r0 = p.f_roff[o0], r1 = p.f_roff[o1];
+    // this lane's first output of the layer, loaded before the row products
+    const int oa = o0 + lane;
+    int ob = 0, oe = 0, on = 0;
+    if (oa < o1) {
+      ob = p.f_roff[oa];
+      oe = p.f_roff[oa + 1];
+      on = p.f_out[oa];
+    }
     wave_rows(r0, r1, [&](int r) { scratch[r - r0] = p.f_c[r] * v[p.f_a[r]] * v[p.f_b[r]]; });
     __syncthreads();
-    for (int o = o0 + lane; o < o1; o += WAVE) {
+    if (oa < o1) {
+      T s = T(0);
+      for (int r = ob - r0; r < oe - r0; ++r) s += scratch[r];
+      v[on] = s;
+    }
+    for (int o = oa + WAVE; o < o1; o += WAVE) {
       T s = T(0);
       const int e = p.f_roff[o + 1] - r0;
       for (int r = p.f_roff[o] - r0; r < e; ++r) s += scratch[r];
@@ -135,9 +148,22 @@ __device__ void wave_graph_reverse(const DPass<T>& p, const T* v, T* a, T* scrat
   for (int l = 0; l < p.nrl; ++l) {
     const int n0 = p.rl_off[l], n1 = p.rl_off[l + 1];
     const int q0 = p.r_roff[n0], q1 = p.r_roff[n1];
+    // this lane's first node of the layer, loaded before the row products
+    const int ka = n0 + lane;
+    int kb = 0, ke = 0, kn = 0;
+    if (ka < n1) {
+      kb = p.r_roff[ka];
+      ke = p.r_roff[ka + 1];
+      kn = p.r_node[ka];
+    }
     wave_rows(q0, q1, [&](int q) { scratch[q - q0] = p.r_c[q] * a[p.r_out[q]] * v[p.r_partner[q]]; });
     __syncthreads();
-    for (int k = n0 + lane; k < n1; k += WAVE) {
+    if (ka < n1) {
+      T s = a[kn];
+      for (int q = kb - q0; q < ke - q0; ++q) s += scratch[q];
+      a[kn] = s;
+    }
+    for (int k = ka + WAVE; k < n1; k += WAVE) {
       const int n = p.r_node[k];
       T s = a[n];
       const int e = p.r_roff[k + 1] - q0;
