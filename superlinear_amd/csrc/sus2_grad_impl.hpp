@@ -172,6 +172,14 @@ __device__ void wave_graph_forward_tan(const DPass<T>& p, T* v, T* vd, T* s1, T*
   for (int l = 0; l < p.nfl; ++l) {
     const int o0 = p.fl_off[l], o1 = p.fl_off[l + 1];
     const int r0 = p.f_roff[o0], r1 = p.f_roff[o1];
+    // this lane's first output of the layer, loaded before the row products
+    const int oa = o0 + lane;
+    int ob = 0, oe = 0, on = 0;
+    if (oa < o1) {
+      ob = p.f_roff[oa];
+      oe = p.f_roff[oa + 1];
+      on = p.f_out[oa];
+    }
     wave_rows(r0, r1, [&](int r) {
       const T c = p.f_c[r];
       const int a = p.f_a[r], b = p.f_b[r];
@@ -179,7 +187,13 @@ __device__ void wave_graph_forward_tan(const DPass<T>& p, T* v, T* vd, T* s1, T*
       s2[r - r0] = c * (vd[a] * v[b] + v[a] * vd[b]);
     });
     __syncthreads();
-    for (int o = o0 + lane; o < o1; o += WAVE) {
+    if (oa < o1) {
+      T x = T(0), y = T(0);
+      for (int r = ob - r0; r < oe - r0; ++r) { x += s1[r]; y += s2[r]; }
+      v[on] = x;
+      vd[on] = y;
+    }
+    for (int o = oa + WAVE; o < o1; o += WAVE) {
       T x = T(0), y = T(0);
       const int e = p.f_roff[o + 1] - r0;
       for (int r = p.f_roff[o] - r0; r < e; ++r) { x += s1[r]; y += s2[r]; }
@@ -199,6 +213,14 @@ __device__ void wave_graph_reverse_tan(const DPass<T>& p, const T* v, const T* v
   for (int l = 0; l < p.nrl; ++l) {
     const int n0 = p.rl_off[l], n1 = p.rl_off[l + 1];
     const int q0 = p.r_roff[n0], q1 = p.r_roff[n1];
+    // this lane's first node of the layer, loaded before the row products
+    const int ka = n0 + lane;
+    int kb = 0, ke = 0, kn = 0;
+    if (ka < n1) {
+      kb = p.r_roff[ka];
+      ke = p.r_roff[ka + 1];
+      kn = p.r_node[ka];
+    }
     wave_rows(q0, q1, [&](int q) {
       const T c = p.r_c[q];
       const int o = p.r_out[q], pt = p.r_partner[q];
@@ -206,7 +228,13 @@ __device__ void wave_graph_reverse_tan(const DPass<T>& p, const T* v, const T* v
       s2[q - q0] = c * Q[o] * v[pt];
     });
     __syncthreads();
-    for (int k = n0 + lane; k < n1; k += WAVE) {
+    if (ka < n1) {
+      T x = P[kn], y = Q[kn];
+      for (int q = kb - q0; q < ke - q0; ++q) { x += s1[q]; y += s2[q]; }
+      P[kn] = x;
+      Q[kn] = y;
+    }
+    for (int k = ka + WAVE; k < n1; k += WAVE) {
       const int n = p.r_node[k];
       T x = P[n], y = Q[n];
       const int e = p.r_roff[k + 1] - q0;
