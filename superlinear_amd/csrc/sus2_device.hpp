@@ -21,6 +21,8 @@ struct DPass {  // layered product-graph schedule of one pass (see PassPlan)
   const T* f_c = nullptr;
   const int *rl_off = nullptr, *r_node = nullptr, *r_roff = nullptr, *r_out = nullptr, *r_partner = nullptr;
   const T* r_c = nullptr;
+  // per layer {first out / node, end, first row, end row}: one 16-byte load
+  const int4 *fl_bnd = nullptr, *rl_bnd = nullptr;
 };
 
 template <class T>

@@ -113,8 +113,8 @@ template <class T>
 __device__ void wave_graph_forward(const DPass<T>& p, T* v, T* scratch) {
   const int lane = threadIdx.x;
   for (int l = 0; l < p.nfl; ++l) {
-    const int o0 = p.fl_off[l], o1 = p.fl_off[l + 1];
-    const int r0 = p.f_roff[o0], r1 = p.f_roff[o1];
+    const int4 bd = p.fl_bnd[l];
+    const int o0 = bd.x, o1 = bd.y, r0 = bd.z, r1 = bd.w;
     // this lane's first output of the layer, loaded before the row products
     const int oa = o0 + lane;
     int ob = 0, oe = 0, on = 0;
@@ -146,8 +146,8 @@ template <class T>
 __device__ void wave_graph_reverse(const DPass<T>& p, const T* v, T* a, T* scratch) {
   const int lane = threadIdx.x;
   for (int l = 0; l < p.nrl; ++l) {
-    const int n0 = p.rl_off[l], n1 = p.rl_off[l + 1];
-    const int q0 = p.r_roff[n0], q1 = p.r_roff[n1];
+    const int4 bd = p.rl_bnd[l];
+    const int n0 = bd.x, n1 = bd.y, q0 = bd.z, q1 = bd.w;
     // this lane's first node of the layer, loaded before the row products
     const int ka = n0 + lane;
     int kb = 0, ke = 0, kn = 0;

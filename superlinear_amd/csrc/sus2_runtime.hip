@@ -255,6 +255,8 @@ void fill_view(DevModelT<T>& dv, const Model& m, const int* ibase, const std::ve
     dp.f_a = ibase + io[k++]; dp.f_b = ibase + io[k++];
     dp.rl_off = ibase + io[k++]; dp.r_node = ibase + io[k++]; dp.r_roff = ibase + io[k++];
     dp.r_out = ibase + io[k++]; dp.r_partner = ibase + io[k++];
+    dp.fl_bnd = reinterpret_cast<const int4*>(ibase + io[k++]);
+    dp.rl_bnd = reinterpret_cast<const int4*>(ibase + io[k++]);
     dp.f_c = dv.fc.template get<T>() + (pass == 0 ? gfc : ofc);
     dp.r_c = dv.rc.template get<T>() + (pass == 0 ? grc : orc);
   }
@@ -276,6 +278,19 @@ void DevModelSet::upload(const Model& m) {
     for (const auto* vec : {&pp.fw_layer_off, &pp.fw_out, &pp.fw_row_off, &pp.fw_a, &pp.fw_b, &pp.rv_layer_off,
                             &pp.rv_node, &pp.rv_row_off, &pp.rv_out, &pp.rv_partner})
       io.push_back(pk.add(*vec));
+    // layer bounds, 16-byte aligned in the packed array
+    auto bounds = [](const std::vector<int>& lay, const std::vector<int>& roff) {
+      std::vector<int> b;
+      for (size_t l = 0; l + 1 < lay.size(); ++l) {
+        const int a0 = lay[l], a1 = lay[l + 1];
+        b.insert(b.end(), {a0, a1, roff.empty() ? 0 : roff[a0], roff.empty() ? 0 : roff[a1]});
+      }
+      return b;
+    };
+    for (const auto& b : {bounds(pp.fw_layer_off, pp.fw_row_off), bounds(pp.rv_layer_off, pp.rv_row_off)}) {
+      while (pk.data.size() % 4) pk.data.push_back(0);
+      io.push_back(pk.add(b));
+    }
   }
   ints.ensure(sizeof(int) * pk.data.size());
   RT_CHECK(hipMemcpy(ints.p, pk.data.data(), sizeof(int) * pk.data.size(), hipMemcpyHostToDevice));
