@@ -81,7 +81,7 @@ def train_section(path, steps, warmup, profile):
     """Loss + parameter gradient over the 256-frame training batch, FP32 and FP64."""
     import torch
     from superlinear_amd import workloads as W
-    from superlinear_amd.flops import PEAK, kernel_flops, survey_grad_per_edge
+    from superlinear_amd.flops import PEAK, kernel_flops, step_grad_per_edge
     from superlinear_amd.model import DeviceBatch, Sus2Model
     frames = W.config_frames("train256")
     nf = len(frames)
@@ -114,7 +114,7 @@ def train_section(path, steps, warmup, profile):
         train_mixed_over_fp64=round(t64 / tmx, 3),
         train_errors_mixed_vs_fp64_gpu=dict(loss_rel=float(abs(lmx - l64) / abs(l64)),
                                             grad_rel=float(np.linalg.norm(gmx - g64) / np.linalg.norm(g64))),
-        train_step_survey_tflops=round(survey_grad_per_edge(dims) * nedges * steps / t32 / 1e12, 4),
+        train_step_formula_tflops=round(step_grad_per_edge(dims) * nedges * steps / t32 / 1e12, 4),
     )
     if prof:
         out["kernels_train_fp32"] = {k: dict(avg_us=round(1e3 * ms / max(c, 1), 2), launches=c)
@@ -167,7 +167,7 @@ def _roofline_of(prof, dims, nedges):
 def batch_section(steps, warmup, profile):
     """Uneven batched EFS and its force-loss gradient: 4096 frames in one call."""
     from superlinear_amd import workloads as W
-    from superlinear_amd.flops import survey_efs_per_edge
+    from superlinear_amd.flops import step_efs_per_edge
     from superlinear_amd.model import DeviceBatch, Sus2Model
     path = W.model_path("effect_l4k6r2o3")
     frames = W.config_frames("batch4096")
@@ -184,7 +184,7 @@ def batch_section(steps, warmup, profile):
                frames_per_s=round(nf * steps / l32.sum(), 1), p50_ms=round(1e3 * float(np.percentile(l32, 50)), 3),
                p99_ms=round(1e3 * float(np.percentile(l32, 99)), 3),
                fp64_atom_steps_per_s=round(n * steps / l64.sum(), 1), fp32_over_fp64=round(l64.sum() / l32.sum(), 3),
-               step_survey_tflops=round(survey_efs_per_edge(dims) * nedges * steps / l32.sum() / 1e12, 4))
+               step_formula_tflops=round(step_efs_per_edge(dims) * nedges * steps / l32.sum() / 1e12, 4))
     # force-loss gradient of the whole batch (labels from a perturbed teacher)
     teacher = Sus2Model(path)
     th = teacher.coeffs()
@@ -211,7 +211,7 @@ def batch_section(steps, warmup, profile):
 def md_section(steps, warmup, profile):
     """One large frame: 2,097,152-atom diamond, Effect L5/K6, FP32 and FP64."""
     from superlinear_amd import workloads as W
-    from superlinear_amd.flops import survey_efs_per_edge
+    from superlinear_amd.flops import step_efs_per_edge
     from superlinear_amd.model import DeviceBatch, Sus2Model
     path = W.model_path("effect_l5k6r2o3")
     frame = W.config_frames("md2m")[0]
@@ -228,7 +228,7 @@ def md_section(steps, warmup, profile):
                atom_steps_per_s=round(n * steps / l32.sum(), 1), ms_per_step=round(1e3 * l32.mean(), 3),
                fp64_atom_steps_per_s=round(n * len(l64) / l64.sum(), 1),
                fp64_ms_per_step=round(1e3 * l64.mean(), 3), fp32_over_fp64=round(l64.mean() / l32.mean(), 3),
-               step_survey_tflops=round(survey_efs_per_edge(dims) * nedges / l32.mean() / 1e12, 4))
+               step_formula_tflops=round(step_efs_per_edge(dims) * nedges / l32.mean() / 1e12, 4))
     if profile:
         model.profile_reset()
         model.profile(True)
@@ -307,7 +307,7 @@ def main():
 
     import torch
     from superlinear_amd import workloads as W
-    from superlinear_amd.flops import PEAK, kernel_flops, survey_efs_per_edge
+    from superlinear_amd.flops import PEAK, kernel_flops, step_efs_per_edge
     from superlinear_amd.model import DeviceBatch, Sus2Model
 
     path = W.model_path("effect_l3k4r2o2")
@@ -365,7 +365,7 @@ def main():
             roof = dict(bound="mfma", pipe="valu", kernel=dom, achieved=round(ach, 4), peak=PEAK["fp32_tflops"],
                         unit="TFLOP/s", frac=round(ach / PEAK["fp32_tflops"], 5), traffic=None,
                         flops_per_launch=fl, avg_launch_us=round(avg_s * 1e6, 2))
-    survey_rate = survey_efs_per_edge(dims) * nedges * args.steps / t32 / 1e12
+    formula_rate = step_efs_per_edge(dims) * nedges * args.steps / t32 / 1e12
     if roof is not None:
         roof["traffic"] = measured_traffic(roof["kernel"], "effect4096")
     train = {} if args.no_train else train_section(path, args.train_steps, 1, not args.no_profile)
@@ -401,7 +401,7 @@ def main():
         "fp64_ms_per_step": round(1e3 * t64 / args.steps, 4),
         "fp32_over_fp64": round(rate32 / rate64, 3),
         "train_grad_frames_per_s": train.get("train_grad_frames_per_s"),
-        "step_survey_tflops": round(survey_rate, 4),
+        "step_formula_tflops": round(formula_rate, 4),
         "errors_vs_fp64_oracle": errors,
         "kernels_fp32": kern,
         "roofline": roof,

@@ -14,8 +14,8 @@ H = (L+1)^2, F = K(L+1), B radial basis size, Kw warp blocks, KH = K H.
   sums sum_k abar R and sum_k abar R' per harmonic, then Y and grad Y once)
   (+ 4 KH for the outer pass of an Effect model: the gate adjoint h_bar)
   + force / virial accumulation 21
-* ``survey_efs`` is the whole-step figure F_e^E (Effect) / F_e^S (Static)
-  of the path survey: F_e^S = 11 + 8H + Kw (10B + 12) + 4FB + 11 KH + F + 3,
+* ``step_efs`` is the closed-form whole-step count F_e^E (Effect) / F_e^S
+  (Static): F_e^S = 11 + 8H + Kw (10B + 12) + 4FB + 11 KH + F + 3,
   F_e^E = F_e^S + 4FB + 13 KH + 2F.
 
 ``pass_forward_*`` kernels also run the per-atom product graph, whose cost
@@ -32,7 +32,7 @@ Parameter gradient (forward-over-reverse, per directed edge of one pass):
   Kw (20 + 9B) (tanh, Chebyshev values and two xi-derivatives) + per channel
   F (6B + 30) (+10F outer Effect: gate messages) + radial-coefficient
   accumulation 11 F B
-* ``survey_grad`` is the whole-step figure F_e^G = F_e^E + 16 KH + 8 FB + 8 Kw B + 4F.
+* ``step_grad`` is the closed-form whole-step count F_e^G = F_e^E + 16 KH + 8 FB + 8 Kw B + 4F.
 """
 
 PEAK = {  # MI355X published peaks
@@ -60,7 +60,7 @@ def force_per_edge(d, outer_effect=False):
             + (4 * KH if outer_effect else 0) + 21)
 
 
-def survey_efs_per_edge(d):
+def step_efs_per_edge(d):
     L, K, B, Kw, H, F, KH = dims_of(d)
     s = 11 + 8 * H + Kw * (10 * B + 12) + 4 * F * B + 11 * KH + F + 3
     if d.get("gated"):
@@ -78,9 +78,9 @@ def grad_sweep2_per_edge(d, outer_effect=False):
     return 30 + 25 * H + 6 * KH + Kw * (20 + 9 * B) + F * (6 * B + 30) + (10 * F if outer_effect else 0) + 11 * F * B
 
 
-def survey_grad_per_edge(d):
+def step_grad_per_edge(d):
     L, K, B, Kw, H, F, KH = dims_of(d)
-    return survey_efs_per_edge(d) + 16 * KH + 8 * F * B + 8 * Kw * B + 4 * F
+    return step_efs_per_edge(d) + 16 * KH + 8 * F * B + 8 * Kw * B + 4 * F
 
 
 def kernel_flops(name, d, nedges):
