@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -1079,7 +1080,6 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     SUS2_CHECK(hipHostGetDevicePointer((void**)&report, ws.host_report, 0));
     device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), N, ws.scan_tmp.get<int64_t>(), st, cap, ecap, report,
                 ++ws.report_seq);
-    SUS2_CHECK(hipEventRecord(ws.report_ev, st));
     ws.nedges = ecap;
     ws.nbr_pending = true;
     if (N > 0) compact_rev<T>(ws, b, rc, st);
@@ -1122,8 +1122,11 @@ static void grow_slots(Workspace& ws, long maxcnt) {
   if (want > ws.nbr_cap) ws.nbr_cap = (int)std::min<long>(want, 4096);
 }
 
-// polls of the count report before a blocking wait (tens of microseconds)
-constexpr long SPIN_POLLS = 1L << 14;
+// how long the host polls the count report before it waits for the stream:
+// longer than a queued call of ordinary size, so that a host one call ahead
+// of the device does not drain the stream (which would leave the device idle
+// while the next call is launched)
+constexpr auto SPIN_LIMIT = std::chrono::microseconds(1000);
 
 bool neighbors_verify(Workspace& ws, hipStream_t st) {
   if (!ws.nbr_pending) return false;
@@ -1138,10 +1141,14 @@ bool neighbors_verify(Workspace& ws, hipStream_t st) {
     w1 = (uint64_t)rep[1];
     return (w0 >> 32) == want && (w1 >> 32) == want;
   };
+  const auto t0 = std::chrono::steady_clock::now();
+  bool waited = false;
   for (long polls = 1; !arrived(); ++polls) {
-    if (polls == SPIN_POLLS) {
-      // a long build: sleep until the scan has finished instead of spinning
-      const hipError_t e = hipEventSynchronize(ws.report_ev);
+    if (polls % 256) continue;
+    if (!waited && std::chrono::steady_clock::now() - t0 > SPIN_LIMIT) {
+      // a long call: wait for the queued work instead of spinning
+      waited = true;
+      const hipError_t e = hipStreamSynchronize(st);
       if (e != hipSuccess) throw Error(SUS2_E_DEVICE, std::string("neighbour build failed: ") + hipGetErrorString(e));
       continue;
     }

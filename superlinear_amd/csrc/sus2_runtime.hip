@@ -79,7 +79,6 @@ void Profiler::reset() {
 
 Workspace::~Workspace() {
   if (host_total) (void)hipHostFree(host_total);
-  if (report_ev) (void)hipEventDestroy(report_ev);
 }
 
 void Workspace::ensure_atoms(int n) {
@@ -90,7 +89,6 @@ void Workspace::ensure_atoms(int n) {
     RT_CHECK(hipHostMalloc((void**)&host_report, 4 * sizeof(int64_t), hipHostMallocCoherent));
     for (int k = 0; k < 4; ++k) host_report[k] = 0;
   }
-  if (!report_ev) RT_CHECK(hipEventCreateWithFlags(&report_ev, hipEventDisableTiming | hipEventBlockingSync));
 }
 
 // Single-workgroup exclusive scan over tiles of SCAN_TILE counts: each thread

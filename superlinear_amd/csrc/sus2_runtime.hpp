@@ -109,7 +109,6 @@ struct Workspace {
   // until the process ends (a device-written host page is never freed).
   int64_t* host_report = nullptr;
   int64_t report_seq = 0;
-  hipEvent_t report_ev = nullptr;  // recorded after the scan: a blocking wait for long builds
   ~Workspace();
   void ensure_atoms(int n);
 };
