@@ -92,11 +92,12 @@ void Workspace::ensure_atoms(int n) {
 }
 
 // Single-workgroup exclusive scan over tiles of SCAN_TILE counts: each thread
-// loads four consecutive counts (coalesced across the workgroup), a wave
+// loads 16 consecutive counts (four 16-byte loads), a wave
 // prefix sum and a scan of the 16 wave sums give the tile's offsets, and the
 // running total carries to the next tile.  Totals above 2^31 are reported
 // through the int64 total.
-constexpr int SCAN_THREADS = 1024, SCAN_PER = 4, SCAN_TILE = SCAN_THREADS * SCAN_PER;
+constexpr int SCAN_THREADS = 1024, SCAN_PER = 16, SCAN_TILE = SCAN_THREADS * SCAN_PER;
+static_assert(SCAN_PER % 4 == 0, "whole int4 groups per thread");
 constexpr int SCAN_WAVES = SCAN_THREADS / 64;
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan(int n, int* __restrict__ cnt, int* __restrict__ off,
@@ -105,15 +106,31 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(int n, int* __restrict__ 
   __shared__ long wsum[SCAN_WAVES];
   __shared__ int wmax[SCAN_WAVES];
   const int t = threadIdx.x, lane = t % 64, w = t / 64;
+  // 16-byte loads and stores where both arrays are 16-byte aligned (i0 is a
+  // multiple of 16 elements)
+  const bool vec = ((reinterpret_cast<uintptr_t>(cnt) | reinterpret_cast<uintptr_t>(off)) & 15) == 0;
   long carry = 0;
   int mx = 0;
   for (int base = 0; base < n; base += SCAN_TILE) {
     const int i0 = base + t * SCAN_PER;
     int v[SCAN_PER];
+    const bool whole = vec && i0 + SCAN_PER <= n;
+    if (whole) {
+#pragma unroll
+      for (int q = 0; q < SCAN_PER / 4; ++q) {
+        const int4 x = reinterpret_cast<const int4*>(cnt + i0)[q];
+        v[4 * q] = x.x;
+        v[4 * q + 1] = x.y;
+        v[4 * q + 2] = x.z;
+        v[4 * q + 3] = x.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < SCAN_PER; ++k) v[k] = i0 + k < n ? cnt[i0 + k] : 0;
+    }
     long s = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_PER; ++k) {
-      v[k] = i0 + k < n ? cnt[i0 + k] : 0;
       s += v[k];
       mx = max(mx, v[k]);
     }
@@ -136,12 +153,28 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(int n, int* __restrict__ 
     }
     __syncthreads();
     long run = carry + (w ? wsum[w - 1] : 0) + incl - s;
+    if (whole) {
 #pragma unroll
-    for (int k = 0; k < SCAN_PER; ++k)
-      if (i0 + k < n) {
-        off[i0 + k] = (int)run;
-        run += v[k];
+      for (int q = 0; q < SCAN_PER / 4; ++q) {
+        int4 y;
+        y.x = (int)run;
+        run += v[4 * q];
+        y.y = (int)run;
+        run += v[4 * q + 1];
+        y.z = (int)run;
+        run += v[4 * q + 2];
+        y.w = (int)run;
+        run += v[4 * q + 3];
+        reinterpret_cast<int4*>(off + i0)[q] = y;
       }
+    } else {
+#pragma unroll
+      for (int k = 0; k < SCAN_PER; ++k)
+        if (i0 + k < n) {
+          off[i0 + k] = (int)run;
+          run += v[k];
+        }
+    }
     carry += wsum[SCAN_WAVES - 1];
     __syncthreads();  // wsum is rewritten by the next tile
   }
