@@ -997,6 +997,62 @@ static void compact_rev(Workspace& ws, const Batch& b, double rc, hipStream_t st
               ws.cell_start.get<int>(), ws.cell_pack.get<CellAtom>(), rc);
 }
 
+// how long the host polls a report word before it waits for the stream:
+// longer than a queued call of ordinary size, so that a host one call ahead
+// of the device does not drain the stream (which would leave the device idle
+// while the next call is launched)
+constexpr auto SPIN_LIMIT = std::chrono::microseconds(1000);
+
+// Largest frame of a batch (atoms), written with its tag (the sequence
+// number in the upper 32 bits) into word 2 of the coherent host report.
+__global__ __launch_bounds__(256) void k_frame_max(int nframes, const int64_t* __restrict__ frame_off,
+                                                   int64_t* report, int64_t seq) {
+  __shared__ int64_t red[256 / WAVE];
+  const int t = threadIdx.x;
+  int64_t m = 0;
+  for (int f = t; f < nframes; f += 256) m = max(m, frame_off[f + 1] - frame_off[f]);
+#pragma unroll
+  for (int d = WAVE / 2; d > 0; d >>= 1) m = max(m, (int64_t)__shfl_xor(m, d, WAVE));
+  if (t % WAVE == 0) red[t / WAVE] = m;
+  __syncthreads();
+  if (t == 0) {
+    for (int k = 1; k < 256 / WAVE; ++k) m = max(m, red[k]);
+    const uint64_t tag = (uint64_t)(seq & 0x7fffffff) << 32;
+    __hip_atomic_store(report, (int64_t)(tag | (uint64_t)min(m, (int64_t)0x7fffffff)), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
+// Largest frame of the batch, read by polling the host report (no copy and
+// stream synchronisation on the call's critical path).
+static int64_t largest_frame(Workspace& ws, const Batch& b, hipStream_t st) {
+  int64_t* report = nullptr;
+  SUS2_CHECK(hipHostGetDevicePointer((void**)&report, ws.host_report, 0));
+  const int64_t seq = ++ws.report_seq;
+  SUS2_LAUNCH(ws, st, "frame_max", k_frame_max, dim3(1), dim3(256), 0, b.nframes, b.frame_off, report + 2, seq);
+  volatile int64_t* rep = ws.host_report + 2;
+  const uint64_t want = (uint64_t)(seq & 0x7fffffff);
+  const auto t0 = std::chrono::steady_clock::now();
+  bool waited = false;
+  uint64_t w = 0;
+  for (long polls = 1; ((w = (uint64_t)*rep) >> 32) != want; ++polls) {
+    if (polls % 256) continue;
+    if (!waited && std::chrono::steady_clock::now() - t0 > SPIN_LIMIT) {
+      waited = true;  // queued work ahead of it: wait for the stream
+      const hipError_t e = hipStreamSynchronize(st);
+      if (e != hipSuccess) throw Error(SUS2_E_DEVICE, std::string("frame size query failed: ") + hipGetErrorString(e));
+      continue;
+    }
+    if (polls % 1024) continue;
+    const hipError_t e = hipStreamQuery(st);
+    if (e == hipErrorNotReady) continue;
+    if (e != hipSuccess) throw Error(SUS2_E_DEVICE, std::string("frame size query failed: ") + hipGetErrorString(e));
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    if (((uint64_t)*rep >> 32) != want) throw Error(SUS2_E_DEVICE, "frame size report missing after the stream drained");
+  }
+  return (int64_t)(w & 0x7fffffff);
+}
+
 template <class T>
 void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   const int N = b.natoms;
@@ -1025,12 +1081,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   bool per_frame = N <= FRAME_CELLS_MAX;
   ws.max_frame_atoms = N;
   if (!per_frame && b.nframes > 0) {
-    ws.host_frame_off.resize(b.nframes + 1);
-    SUS2_CHECK(hipMemcpyAsync(ws.host_frame_off.data(), b.frame_off, sizeof(int64_t) * (b.nframes + 1),
-                              hipMemcpyDeviceToHost, st));
-    SUS2_CHECK(hipStreamSynchronize(st));
-    int64_t mx = 0;
-    for (int f = 0; f < b.nframes; ++f) mx = std::max(mx, ws.host_frame_off[f + 1] - ws.host_frame_off[f]);
+    const int64_t mx = largest_frame(ws, b, st);
     ws.max_frame_atoms = mx;
     per_frame = mx <= FRAME_CELLS_MAX;
   }
@@ -1122,11 +1173,6 @@ static void grow_slots(Workspace& ws, long maxcnt) {
   if (want > ws.nbr_cap) ws.nbr_cap = (int)std::min<long>(want, 4096);
 }
 
-// how long the host polls the count report before it waits for the stream:
-// longer than a queued call of ordinary size, so that a host one call ahead
-// of the device does not drain the stream (which would leave the device idle
-// while the next call is launched)
-constexpr auto SPIN_LIMIT = std::chrono::microseconds(1000);
 
 bool neighbors_verify(Workspace& ws, hipStream_t st) {
   if (!ws.nbr_pending) return false;

@@ -79,7 +79,6 @@ struct Workspace {
   DevBuf nbr_slots;
   DevBuf nbr_oslot;  // bin-offset index of each slot's hit (ascending per atom)
   int nbr_cap = 128;
-  std::vector<int64_t> host_frame_off;  // host copy of a batch's frame offsets (frame sizes)
   // cell-list neighbour build
   DevBuf grid, cell_of, wrap, cell_cnt, cell_start, cell_cur, cell_atoms;
   DevBuf cell_pack;   // [N] bin-ordered atoms: home-cell position, index, species
@@ -105,8 +104,9 @@ struct Workspace {
   // Count report of such a build, written by the scan kernel into coherent
   // pinned host memory and polled by the host: two 64-bit words, each tagged
   // with the build's sequence number in its upper half (word 0: largest count;
-  // word 1: overflow flag in bit 31, total in bits 0-30, saturated).  Kept
-  // until the process ends (a device-written host page is never freed).
+  // word 1: overflow flag in bit 31, total in bits 0-30, saturated); word 2
+  // holds the largest frame of a batch, tagged the same way.  Kept until the
+  // process ends (a device-written host page is never freed).
   int64_t* host_report = nullptr;
   int64_t report_seq = 0;
   ~Workspace();
