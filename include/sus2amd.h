@@ -66,6 +66,8 @@ int sus2_coeff_count(const sus2_model* m);
  * alpha[CF] | gate type[CF] | gate W[F Qg] | gate rank W | outer rank W |
  * species[C] | readout[Q or CQ]   (dev_src/mtpr.cpp:925-975,1586-1740,2108-2210) */
 int sus2_coeff_get(const sus2_model* m, double* theta /* host [ncoeff] */);
+/* sus2_coeff_set waits for the device's queued work before it replaces the
+ * coefficients on the device. */
 int sus2_coeff_set(sus2_model* m, const double* theta /* host [ncoeff] */);
 /* offsets[13]: shift scal rad grad gscal alpha gtype W gw ow species read ncoeff (-1 = absent) */
 int sus2_coeff_offsets(const sus2_model* m, int* offsets);
@@ -100,7 +102,8 @@ int sus2_efs_grad(sus2_model* m, int precision, int nframes, int natoms, const i
  * Labels: e0 [nframes], f0 [natoms*3], s0 [nframes*6] (xx yy zz xy xz yz).
  * weighting: 0 vibrations, 1 structures, 2 molecules; weights wE wF wS.
  * loss_out (host double) = (1/nframes) sum_f L_f; grad [ncoeff] (dev FP64)
- * is overwritten with the gradient of loss_out. */
+ * is overwritten with the gradient of loss_out.  Returns once loss_out is
+ * known; the gradient kernels finish on `stream` after the return. */
 int sus2_loss_grad(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
                    const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
                    const double* e0, const double* f0, const double* s0, int weighting, double wE, double wF,

@@ -295,6 +295,8 @@ int sus2_train_lbfgs(sus2_model* m, int precision, int nframes, int natoms, cons
       double f = 0;
       sus2::loss_grad_dispatch(m->dev, m->ws, precision, b, e0, f0, s0, weighting, wE, wF, wS, &f, gdev.get<double>(),
                                st);
+      // the gradient kernels finish on the stream after the objective is known
+      if (hipStreamSynchronize(st) != hipSuccess) throw sus2::Error(SUS2_E_DEVICE, "stream synchronize failed");
       d2h(grad.data(), gdev.get<double>(), sizeof(double) * P);
       return f;
     };

@@ -25,7 +25,6 @@
 
 #include <algorithm>
 #include <atomic>
-#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -997,12 +996,6 @@ static void compact_rev(Workspace& ws, const Batch& b, double rc, hipStream_t st
               ws.cell_start.get<int>(), ws.cell_pack.get<CellAtom>(), rc);
 }
 
-// how long the host polls a report word before it waits for the stream:
-// longer than a queued call of ordinary size, so that a host one call ahead
-// of the device does not drain the stream (which would leave the device idle
-// while the next call is launched)
-constexpr auto SPIN_LIMIT = std::chrono::microseconds(1000);
-
 // Largest frame of a batch (atoms), written with its tag (the sequence
 // number in the upper 32 bits) into word 2 of the coherent host report.
 __global__ __launch_bounds__(256) void k_frame_max(int nframes, const int64_t* __restrict__ frame_off,
@@ -1026,30 +1019,11 @@ __global__ __launch_bounds__(256) void k_frame_max(int nframes, const int64_t* _
 // Largest frame of the batch, read by polling the host report (no copy and
 // stream synchronisation on the call's critical path).
 static int64_t largest_frame(Workspace& ws, const Batch& b, hipStream_t st) {
-  int64_t* report = nullptr;
-  SUS2_CHECK(hipHostGetDevicePointer((void**)&report, ws.host_report, 0));
   const int64_t seq = ++ws.report_seq;
-  SUS2_LAUNCH(ws, st, "frame_max", k_frame_max, dim3(1), dim3(256), 0, b.nframes, b.frame_off, report + 2, seq);
-  volatile int64_t* rep = ws.host_report + 2;
-  const uint64_t want = (uint64_t)(seq & 0x7fffffff);
-  const auto t0 = std::chrono::steady_clock::now();
-  bool waited = false;
+  SUS2_LAUNCH(ws, st, "frame_max", k_frame_max, dim3(1), dim3(256), 0, b.nframes, b.frame_off, report_word(ws, 2),
+              seq);
   uint64_t w = 0;
-  for (long polls = 1; ((w = (uint64_t)*rep) >> 32) != want; ++polls) {
-    if (polls % 256) continue;
-    if (!waited && std::chrono::steady_clock::now() - t0 > SPIN_LIMIT) {
-      waited = true;  // queued work ahead of it: wait for the stream
-      const hipError_t e = hipStreamSynchronize(st);
-      if (e != hipSuccess) throw Error(SUS2_E_DEVICE, std::string("frame size query failed: ") + hipGetErrorString(e));
-      continue;
-    }
-    if (polls % 1024) continue;
-    const hipError_t e = hipStreamQuery(st);
-    if (e == hipErrorNotReady) continue;
-    if (e != hipSuccess) throw Error(SUS2_E_DEVICE, std::string("frame size query failed: ") + hipGetErrorString(e));
-    std::atomic_thread_fence(std::memory_order_seq_cst);
-    if (((uint64_t)*rep >> 32) != want) throw Error(SUS2_E_DEVICE, "frame size report missing after the stream drained");
-  }
+  poll_report(ws, 2, 1, seq, st, "frame size query", &w);
   return (int64_t)(w & 0x7fffffff);
 }
 
@@ -1127,10 +1101,8 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     // no host round trip: the scan reports (and empties every list) when the
     // counts exceed the slots or the edge buffers; neighbors_verify reads the
     // report once the call's work has been queued
-    int64_t* report = nullptr;
-    SUS2_CHECK(hipHostGetDevicePointer((void**)&report, ws.host_report, 0));
-    device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), N, ws.scan_tmp.get<int64_t>(), st, cap, ecap, report,
-                ++ws.report_seq);
+    device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), N, ws.scan_tmp.get<int64_t>(), st, cap, ecap,
+                report_word(ws, 0), ws.scan_seq = ++ws.report_seq);
     ws.nedges = ecap;
     ws.nbr_pending = true;
     if (N > 0) compact_rev<T>(ws, b, rc, st);
@@ -1177,34 +1149,10 @@ static void grow_slots(Workspace& ws, long maxcnt) {
 bool neighbors_verify(Workspace& ws, hipStream_t st) {
   if (!ws.nbr_pending) return false;
   ws.nbr_pending = false;
-  // the scan runs early in the queued work: poll its report, checking the
-  // stream now and then so that a failed or drained stream ends the wait
-  volatile int64_t* rep = ws.host_report;
-  const uint64_t want = (uint64_t)(ws.report_seq & 0x7fffffff);
-  uint64_t w0 = 0, w1 = 0;
-  auto arrived = [&] {
-    w0 = (uint64_t)rep[0];
-    w1 = (uint64_t)rep[1];
-    return (w0 >> 32) == want && (w1 >> 32) == want;
-  };
-  const auto t0 = std::chrono::steady_clock::now();
-  bool waited = false;
-  for (long polls = 1; !arrived(); ++polls) {
-    if (polls % 256) continue;
-    if (!waited && std::chrono::steady_clock::now() - t0 > SPIN_LIMIT) {
-      // a long call: wait for the queued work instead of spinning
-      waited = true;
-      const hipError_t e = hipStreamSynchronize(st);
-      if (e != hipSuccess) throw Error(SUS2_E_DEVICE, std::string("neighbour build failed: ") + hipGetErrorString(e));
-      continue;
-    }
-    if (polls % 1024) continue;
-    const hipError_t e = hipStreamQuery(st);
-    if (e == hipErrorNotReady) continue;
-    if (e != hipSuccess) throw Error(SUS2_E_DEVICE, std::string("neighbour build failed: ") + hipGetErrorString(e));
-    std::atomic_thread_fence(std::memory_order_seq_cst);
-    if (!arrived()) throw Error(SUS2_E_DEVICE, "neighbour count report missing after the stream drained");
-  }
+  // the scan runs early in the queued work: poll its report
+  uint64_t w[2];
+  poll_report(ws, 0, 2, ws.scan_seq, st, "neighbour build", w);
+  const uint64_t w0 = w[0], w1 = w[1];
   const long total = (long)(w1 & 0x7fffffff), maxcnt = (long)(w0 & 0xffffffff);
   const bool over = (w1 >> 31) & 1;
   grow_slots(ws, maxcnt);

@@ -104,9 +104,10 @@ __global__ __launch_bounds__(256) void k_loss_frames(int nframes, const int64_t*
   }
 }
 
-// out[0] = (1/nframes) sum_f loss_f (fixed order).
+// out[0] = (1/nframes) sum_f loss_f (fixed order); with report, also its low
+// and high 32 bits in report[0..1], each word tagged with seq in its upper half.
 __global__ __launch_bounds__(256) void k_loss_sum(int nframes, const double* __restrict__ loss_f,
-                                                  double* __restrict__ out) {
+                                                  double* __restrict__ out, int64_t* report, int64_t seq) {
   __shared__ double red[256];
   const int t = threadIdx.x;
   double s = 0;
@@ -117,7 +118,15 @@ __global__ __launch_bounds__(256) void k_loss_sum(int nframes, const double* __r
     if (t < w) red[t] += red[t + w];
     __syncthreads();
   }
-  if (t == 0) out[0] = nframes > 0 ? red[0] / nframes : 0.0;
+  if (t == 0) {
+    const double v = nframes > 0 ? red[0] / nframes : 0.0;
+    out[0] = v;
+    if (report) {
+      const uint64_t bits = (uint64_t)__double_as_longlong(v), tag = (uint64_t)(seq & 0x7fffffff) << 32;
+      __hip_atomic_store(report, (int64_t)(tag | (bits & 0xffffffffu)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(report + 1, (int64_t)(tag | (bits >> 32)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
 }
 
 __global__ void k_fill(int n, double v, double* __restrict__ a) {
@@ -167,7 +176,8 @@ void loss_grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, con
   SUS2_LAUNCH(ws, st, "loss_frames", k_loss_frames, dim3(f1), dim3(256), 0, nf, b.frame_off, out.energy, out.force,
               out.virial, e0, f0, s0, mode, wE, wF, wS, ws.lew.get<double>(), ws.lfw.get<double>(),
               ws.lsw.get<double>(), lf + 1);
-  SUS2_LAUNCH(ws, st, "loss_sum", k_loss_sum, dim3(1), dim3(256), 0, nf, lf + 1, lf);
+  const int64_t seq = ++ws.report_seq;
+  SUS2_LAUNCH(ws, st, "loss_sum", k_loss_sum, dim3(1), dim3(256), 0, nf, lf + 1, lf, report_word(ws, 4), seq);
   if (precision == SUS2_MIXED && ws.nedges > 0) {  // the neighbour list of the FP64 EFS, narrowed
     ws.rec_alt.ensure(sizeof(Rec<float>) * (size_t)ws.nedges);
     SUS2_LAUNCH(ws, st, "rec_narrow", k_rec_narrow, dim3((unsigned)((ws.nedges + 255) / 256)), dim3(256), 0,
@@ -176,10 +186,13 @@ void loss_grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, con
   }
   grad_dispatch(dm, ws, grad_prec, b, ws.lew.get<double>(), ws.lfw.get<double>(), ws.lsw.get<double>(), grad,
                 1.0 / nf, false, true, st);
-  SUS2_CHECK(hipMemcpyAsync(ws.host_total, lf, sizeof(double), hipMemcpyDeviceToHost, st));
-  SUS2_CHECK(hipStreamSynchronize(st));
+  // the objective is known once the EFS and the loss kernels have run: the
+  // call returns then, and the gradient kernels finish on the stream
+  uint64_t w[2];
+  poll_report(ws, 4, 2, seq, st, "training objective", w);
+  const uint64_t bits = (w[0] & 0xffffffffu) | (w[1] << 32);
   double v;
-  std::memcpy(&v, ws.host_total, sizeof(double));
+  std::memcpy(&v, &bits, sizeof(double));
   *loss_host = v;
 }
 

@@ -2,12 +2,16 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <string>
 
 #include "../../include/sus2amd.h"
 #include "sus2_runtime.hpp"
 
 namespace sus2 {
+
+constexpr int REPORT_WORDS = 8;  // 64-bit words of the coherent host report
 
 #define RT_CHECK(x)                                                                                \
   do {                                                                                             \
@@ -86,8 +90,8 @@ void Workspace::ensure_atoms(int n) {
   nbr_off.ensure(sizeof(int) * ((size_t)n + 1));
   if (!host_total) RT_CHECK(hipHostMalloc((void**)&host_total, 4 * sizeof(int64_t), hipHostMallocDefault));
   if (!host_report) {
-    RT_CHECK(hipHostMalloc((void**)&host_report, 4 * sizeof(int64_t), hipHostMallocCoherent));
-    for (int k = 0; k < 4; ++k) host_report[k] = 0;
+    RT_CHECK(hipHostMalloc((void**)&host_report, REPORT_WORDS * sizeof(int64_t), hipHostMallocCoherent));
+    for (int k = 0; k < REPORT_WORDS; ++k) host_report[k] = 0;
   }
 }
 
@@ -210,6 +214,49 @@ void device_scan(int* cnt, int* off, int n, int64_t* total, hipStream_t st, int 
   RT_CHECK(hipGetLastError());
 }
 
+int64_t* report_word(Workspace& ws, int k) {
+  if (!ws.host_report) ws.ensure_atoms(0);
+  int64_t* report = nullptr;
+  RT_CHECK(hipHostGetDevicePointer((void**)&report, ws.host_report, 0));
+  return report + k;
+}
+
+// How long the host spins on a report word before it waits for the stream:
+// longer than a queued call of ordinary size, so that a host one call ahead
+// of the device does not drain the stream (which would leave the device idle
+// while the next call is launched).
+constexpr auto SPIN_LIMIT = std::chrono::milliseconds(50);
+
+void poll_report(Workspace& ws, int k, int n, int64_t seq, hipStream_t st, const char* what, uint64_t* w) {
+  volatile int64_t* rep = ws.host_report + k;
+  const uint64_t want = (uint64_t)(seq & 0x7fffffff);
+  auto arrived = [&] {
+    bool ok = true;
+    for (int q = 0; q < n; ++q) {
+      w[q] = (uint64_t)rep[q];
+      ok = ok && (w[q] >> 32) == want;
+    }
+    return ok;
+  };
+  const auto t0 = std::chrono::steady_clock::now();
+  bool waited = false;
+  for (long polls = 1; !arrived(); ++polls) {
+    if (polls % 256) continue;
+    if (!waited && std::chrono::steady_clock::now() - t0 > SPIN_LIMIT) {
+      waited = true;  // a long queue: wait for the stream instead of spinning
+      const hipError_t e = hipStreamSynchronize(st);
+      if (e != hipSuccess) throw Error(SUS2_E_DEVICE, std::string(what) + " failed: " + hipGetErrorString(e));
+      continue;
+    }
+    if (polls % 1024) continue;
+    const hipError_t e = hipStreamQuery(st);
+    if (e == hipErrorNotReady) continue;
+    if (e != hipSuccess) throw Error(SUS2_E_DEVICE, std::string(what) + " failed: " + hipGetErrorString(e));
+    std::atomic_thread_fence(std::memory_order_seq_cst);
+    if (!arrived()) throw Error(SUS2_E_DEVICE, std::string(what) + ": report missing after the stream drained");
+  }
+}
+
 long exclusive_scan(Workspace& ws, int n, hipStream_t st, long* maxcnt) {
   ws.scan_tmp.ensure(8 * sizeof(int64_t));
   device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), n, ws.scan_tmp.get<int64_t>(), st);
@@ -330,6 +377,8 @@ void DevModelSet::upload(const Model& m) {
 }
 
 void DevModelSet::upload_theta(const Model& m) {
+  // kernels queued by earlier calls (on any stream) may still read them
+  RT_CHECK(hipDeviceSynchronize());
   std::vector<double> td(m.theta.begin(), m.theta.end());
   std::vector<float> tf(m.theta.begin(), m.theta.end());
   RT_CHECK(hipMemcpy(d.theta.p, td.data(), sizeof(double) * td.size(), hipMemcpyHostToDevice));

@@ -105,10 +105,11 @@ struct Workspace {
   // pinned host memory and polled by the host: two 64-bit words, each tagged
   // with the build's sequence number in its upper half (word 0: largest count;
   // word 1: overflow flag in bit 31, total in bits 0-30, saturated); word 2
-  // holds the largest frame of a batch, tagged the same way.  Kept until the
-  // process ends (a device-written host page is never freed).
+  // holds the largest frame of a batch, words 4-5 the low and high halves of
+  // a training objective, tagged the same way.  Kept until the process ends
+  // (a device-written host page is never freed).
   int64_t* host_report = nullptr;
-  int64_t report_seq = 0;
+  int64_t report_seq = 0, scan_seq = 0;  // last tag issued; tag of the pending scan report
   ~Workspace();
   void ensure_atoms(int n);
 };
@@ -124,6 +125,12 @@ void device_scan(int* cnt, int* off, int n, int64_t* total, hipStream_t st, int 
 // exclusive scan of ws.nbr_cnt[0..n) into ws.nbr_off[0..n]; returns the total
 // (and the largest count in *maxcnt)
 long exclusive_scan(Workspace& ws, int n, hipStream_t st, long* maxcnt = nullptr);
+// Device address of the host report's word k (for the kernels that write it).
+int64_t* report_word(Workspace& ws, int k);
+// Polls words [k, k + n) of the host report until each carries tag seq in its
+// upper half and returns them in w; spins for up to 50 ms, then waits for the
+// stream.  Throws when the stream fails or drains without the report.
+void poll_report(Workspace& ws, int k, int n, int64_t seq, hipStream_t st, const char* what, uint64_t* w);
 
 // One precision's device view of a model.
 template <class T>

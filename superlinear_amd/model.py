@@ -193,8 +193,13 @@ class Sus2Model:
             grad = torch.empty(self.coeff_count, dtype=torch.float64, device=dev)
         e0 = e0.to(device=dev, dtype=torch.float64).contiguous()
         f0 = f0.to(device=dev, dtype=torch.float64).contiguous()
-        s0 = (torch.zeros((batch.nframes, 6), dtype=torch.float64, device=dev) if s0 is None
-              else s0.to(device=dev, dtype=torch.float64).contiguous())
+        if s0 is None:  # zero stress labels, kept between calls (only read)
+            z = getattr(self, "_s0_zero", None)
+            if z is None or z.device != dev or z.shape[0] != batch.nframes:
+                z = self._s0_zero = torch.zeros((batch.nframes, 6), dtype=torch.float64, device=dev)
+            s0 = z
+        else:
+            s0 = s0.to(device=dev, dtype=torch.float64).contiguous()
         loss = ctypes.c_double()
         st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
         _lib.check(_lib.lib().sus2_loss_grad(self._h, PRECISION[precision], batch.nframes, batch.natoms,
