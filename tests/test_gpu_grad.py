@@ -229,3 +229,42 @@ def test_repeated_scalar_node(lib, oracle_lib, tmp_path):
     args = [torch.from_numpy(x).cuda() for x in (ew, fw, sw)]
     g = m.efs_grad_device(batch, *args, precision="fp64").cpu().numpy()
     assert _rel(g, gref) <= TOL64
+
+
+def test_back_to_back_objectives_with_new_coefficients(lib):
+    """sus2_loss_grad returns once the objective is known while its gradient
+    kernels still run on the stream; the next call and a coefficient update
+    in between must not disturb them.  Three objectives queued back to back
+    (coefficients replaced between them, no synchronisation by the caller)
+    against the same three evaluated one at a time with a synchronisation
+    after each: identical losses and gradients (same kernels, same order)."""
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    print("[case] back-to-back objectives, effect_l3k4r2o2", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frames = W.periodic_frames(96, 60, 200, 2, 0.15, seed=61)  # > 8192 atoms: the frame-size query runs
+    batch = DeviceBatch(frames)
+    assert batch.natoms > 8192
+    teacher = Sus2Model(path)
+    th0 = teacher.coeffs()
+    rng = np.random.default_rng(62)
+    teacher.set_coeffs(th0 * (1 + 0.05 * rng.standard_normal(th0.size)))
+    lab = teacher.efs_device(batch, "fp64")
+    thetas = [th0 * (1 + 0.02 * k * rng.standard_normal(th0.size)) for k in range(3)]
+    for prec in ("fp32", "fp64", "mixed"):
+        m = Sus2Model(path)
+        queued = []
+        for th in thetas:
+            m.set_coeffs(th)
+            loss, g = m.loss_grad_device(batch, lab["energy"], lab["forces"], precision=prec)
+            queued.append((loss, g.clone()))
+        torch.cuda.synchronize()
+        for th, (loss_q, g_q) in zip(thetas, queued):
+            m.set_coeffs(th)
+            loss, g = m.loss_grad_device(batch, lab["energy"], lab["forces"], precision=prec)
+            torch.cuda.synchronize()
+            print(f"[result] {prec}: loss {loss:.6e}", flush=True)
+            assert loss == loss_q
+            assert torch.equal(g, g_q)
+        assert queued[0][0] != queued[1][0]
