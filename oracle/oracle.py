@@ -53,6 +53,9 @@ def lib():
                                    ctypes.c_int]
         L.orc_loss.argtypes = [ctypes.c_void_p, ctypes.c_int, _lp, _ip, _dp, _dp, _ip, _dp, _dp, _dp,
                                ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_frame_loss.restype = ctypes.c_double
+        L.orc_frame_loss.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, _dp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -164,6 +167,22 @@ class OracleModel:
                               _ptr(b["cell"], _dp), _ptr(b["periodic"], _ip), _ptr(E0, _dp), _ptr(F0, _dp),
                               _ptr(S0, _dp), mode, wE, wF, wS, _ptr(loss, _dp)))
         return float(loss[0])
+
+
+def frame_loss(mode, wE, wF, wS, E, F, S, E0, F0, S0):
+    """One frame's loss and adjoints from given predictions (E, F [n,3], S
+    Voigt [6]) and labels: returns (loss, dL/dE_i [n], dL/dF [n,3], dL/dS [3,3])."""
+    F = np.ascontiguousarray(F, np.float64).reshape(-1, 3)
+    F0 = np.ascontiguousarray(F0, np.float64).reshape(-1, 3)
+    S = np.ascontiguousarray(S, np.float64)
+    S0 = np.ascontiguousarray(S0, np.float64)
+    n = len(F)
+    dE = np.zeros(n)
+    dF = np.zeros((n, 3))
+    dS = np.zeros(9)
+    loss = lib().orc_frame_loss(n, mode, wE, wF, wS, E, _ptr(F, _dp), _ptr(S, _dp), E0, _ptr(F0, _dp),
+                                _ptr(S0, _dp), _ptr(dE, _dp), _ptr(dF, _dp), _ptr(dS, _dp))
+    return loss, dE, dF, dS.reshape(3, 3)
 
 
 def pack(frames):

@@ -305,7 +305,7 @@ template <class T>
 void real_sh(const T& x, const T& y, const T& z, int L, T* Y) {
   const T r = sqrt(x * x + y * y + z * z);
   const T ux = x / r, uy = y / r, uz = z / r;
-  T cm[8], sm[8];  // Re/Im of (ux + i uy)^m
+  T cm[16], sm[16];  // Re/Im of (ux + i uy)^m (L <= 15)
   cm[0] = T(1.0); sm[0] = T(0.0);
   for (int m = 1; m <= L; ++m) {
     cm[m] = ux * cm[m - 1] - uy * sm[m - 1];
@@ -795,6 +795,7 @@ void orc_offsets(void* h, int* d) {
 
 // Real Y_lm and its Cartesian gradient for one vector (test hook).
 void orc_sh(int L, const double* d, double* Y, double* dY) {
+  if (L < 0 || L > 15) return;
   Dual3 x(d[0]), y(d[1]), z(d[2]);
   x.d[0] = 1; y.d[1] = 1; z.d[2] = 1;
   std::vector<Dual3> v((L + 1) * (L + 1));
@@ -970,6 +971,18 @@ int orc_loss(void* h, int nframes, const long* off, const int* type, const doubl
     }
     *loss_out = L / nframes;
   });
+}
+
+// One frame's loss and adjoints from given predictions and labels (no model):
+// E, E0 scalars, F, F0 [n*3], S, S0 Voigt xx yy zz xy xz yz.  Outputs dEi [n],
+// dF [n*3], dS [9] row-major; returns the frame loss
+// (dev_src/non_linear_regression.cpp:986-1037 with norm_by_forces = 0).
+double orc_frame_loss(int n, int mode, double wE, double wF, double wS, double E, const double* F, const double* S,
+                      double E0, const double* F0, const double* S0, double* dEi, double* dF, double* dS) {
+  double Sf[9], S0f[9];
+  voigt_to_full(S, Sf);
+  voigt_to_full(S0, S0f);
+  return frame_loss(n, mode, wE, wF, wS, E, F, Sf, E0, F0, S0f, true, true, true, dEi, dF, dS);
 }
 
 }  // extern "C"

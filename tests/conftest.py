@@ -47,6 +47,28 @@ def close(actual, expected, tol):
     return float(err.max(initial=0))
 
 
+def real_sh_scipy(L, d):
+    """Real Y_lm (l <= L, flat index l^2 + l + m) from scipy's complex
+    harmonics: m > 0 cos, m < 0 sin parts with the Condon-Shortley phase
+    cancelled, so that Y_1,(-1, 0, 1) is proportional to (y, z, x)."""
+    from scipy.special import sph_harm_y
+    r = np.linalg.norm(d)
+    theta, phi = np.arccos(d[2] / r), np.arctan2(d[1], d[0])
+    out = np.zeros((L + 1) ** 2)
+    for l in range(L + 1):
+        for m in range(-l, l + 1):
+            y = sph_harm_y(l, abs(m), theta, phi)  # includes the Condon-Shortley phase
+            cs = (-1) ** abs(m)
+            if m == 0:
+                v = y.real
+            elif m > 0:
+                v = np.sqrt(2) * cs * y.real
+            else:
+                v = np.sqrt(2) * cs * y.imag
+            out[l * l + l + m] = v
+    return out
+
+
 @pytest.fixture(scope="session")
 def oracle_lib():
     from oracle import oracle

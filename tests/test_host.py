@@ -8,7 +8,7 @@ import re
 import numpy as np
 import pytest
 
-from conftest import FIXTURES, ROOT
+from conftest import FIXTURES, ROOT, real_sh_scipy
 
 
 def _declared_symbols():
@@ -122,28 +122,55 @@ def test_topology_reproduces_reference_graphs(fixture, kw, keys):
 
 def test_real_cg_orthogonality():
     """Real coupling coefficients form an orthogonal transform of the
-    (l1 x l2) product space (sum over L, M of C C = delta)."""
+    (l1 x l2) product space (sum over L, M of C C = delta), for l1, l2 <= 6."""
     from superlinear_amd.topology import real_cg
-    for l1, l2 in ((1, 1), (2, 1), (2, 2), (3, 2)):
-        n = (2 * l1 + 1) * (2 * l2 + 1)
-        U = np.zeros((n, n))
-        row = 0
-        for L in range(abs(l1 - l2), l1 + l2 + 1):
-            for M in range(-L, L + 1):
-                U[row] = [real_cg(l1, a, l2, b, L, M) for a in range(-l1, l1 + 1) for b in range(-l2, l2 + 1)]
-                row += 1
-        assert np.allclose(U @ U.T, np.eye(n), atol=1e-12)
+    for l1 in range(1, 7):
+        for l2 in range(1, l1 + 1):
+            n = (2 * l1 + 1) * (2 * l2 + 1)
+            U = np.zeros((n, n))
+            row = 0
+            for L in range(abs(l1 - l2), l1 + l2 + 1):
+                for M in range(-L, L + 1):
+                    U[row] = [real_cg(l1, a, l2, b, L, M) for a in range(-l1, l1 + 1) for b in range(-l2, l2 + 1)]
+                    row += 1
+            assert np.allclose(U @ U.T, np.eye(n), atol=1e-12), (l1, l2)
 
 
-def test_cfg_round_trip():
+def test_real_cg_couples_harmonics_of_one_direction():
+    """sum_{r1 r2} C(l1 r1, l2 r2 | L R) Y_{l1 r1}(u) Y_{l2 r2}(u) equals
+    i^(l1+l2-L) g Y_{L R}(u) with the Gaunt factor
+    g = sqrt((2l1+1)(2l2+1) / (4 pi (2L+1))) <l1 0 l2 0 | L 0> for even
+    l1 + l2 - L (the phase of the real coupling, dev_src/mtpr_sh.cpp:265-326)
+    and vanishes for odd l1 + l2 - L.
+    Y from scipy in the kernels' real basis: this fixes every coefficient's
+    sign and the real-basis phase, which orthogonality alone does not."""
+    from math import pi, sqrt
+    from superlinear_amd.topology import clebsch_gordan, real_cg
+    rng = np.random.default_rng(8)
+    us = [rng.normal(size=3) for _ in range(3)]
+    Ys = [real_sh_scipy(12, u) for u in us]
+    for l1 in range(1, 7):
+        for l2 in range(1, l1 + 1):
+            for L in range(l1 - l2, l1 + l2 + 1):
+                C = np.array([[real_cg(l1, a, l2, b, L, R) for a in range(-l1, l1 + 1) for b in range(-l2, l2 + 1)]
+                              for R in range(-L, L + 1)])
+                g = sqrt((2 * l1 + 1) * (2 * l2 + 1) / (4 * pi * (2 * L + 1))) * clebsch_gordan(l1, 0, l2, 0, L, 0)
+                for Y in Ys:
+                    y1, y2 = Y[l1 * l1:(l1 + 1) ** 2], Y[l2 * l2:(l2 + 1) ** 2]
+                    got = C @ np.outer(y1, y2).ravel()
+                    k = l1 + l2 - L
+                    want = (-1) ** (k // 2) * g * Y[L * L:(L + 1) ** 2] if k % 2 == 0 else np.zeros(2 * L + 1)
+                    assert np.allclose(got, want, atol=1e-12), (l1, l2, L)
+
+
+def test_cfg_round_trip(tmp_path):
     from superlinear_amd.cfg import read_cfg, write_cfg
     frames = list(read_cfg(FIXTURES / "fixtures_v2" / "geometries.cfg"))
     assert len(frames) == 7
     f = dict(frames[0], forces=np.arange(9.0).reshape(3, 3) * 0.1, energy=-1.5, virial=np.arange(6.0))
     buf = io.StringIO()
     write_cfg(buf, f, lossless=True)
-    path = ROOT / "_work" / "rt.cfg"
-    path.parent.mkdir(exist_ok=True)
+    path = tmp_path / "rt.cfg"
     path.write_text(buf.getvalue())
     g = next(read_cfg(path))
     assert np.array_equal(g["positions"], f["positions"])

@@ -6,7 +6,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import FIXTURES, close, fixture_cases, voigt_to_matrix
+from conftest import FIXTURES, close, fixture_cases, real_sh_scipy, voigt_to_matrix
 
 
 @pytest.mark.parametrize("case", fixture_cases(), ids=lambda c: c[0])
@@ -53,31 +53,12 @@ def test_neighbors_match_brute_force(oracle_lib, seed):
         assert got == want
 
 
-def _real_sh_scipy(L, d):
-    from scipy.special import sph_harm_y
-    r = np.linalg.norm(d)
-    theta, phi = np.arccos(d[2] / r), np.arctan2(d[1], d[0])
-    out = np.zeros((L + 1) ** 2)
-    for l in range(L + 1):
-        for m in range(-l, l + 1):
-            y = sph_harm_y(l, abs(m), theta, phi)  # includes the Condon-Shortley phase
-            cs = (-1) ** abs(m)
-            if m == 0:
-                v = y.real
-            elif m > 0:
-                v = np.sqrt(2) * cs * y.real
-            else:
-                v = np.sqrt(2) * cs * y.imag
-            out[l * l + l + m] = v
-    return out
-
-
 def test_real_sh_against_scipy_and_fd(oracle_lib):
     rng = np.random.default_rng(0)
     for _ in range(20):
         d = rng.normal(size=3) * 2
         Y, dY = oracle_lib.sh(6, d)
-        assert np.allclose(Y, _real_sh_scipy(6, d), atol=1e-13)
+        assert np.allclose(Y, real_sh_scipy(6, d), atol=1e-13)
         h = 1e-6
         for q in range(3):
             e = np.zeros(3)
@@ -285,3 +266,136 @@ def test_efs_grad_agrees_with_loss_grad(oracle_lib):
     out = m.loss_grad(frames, labels, mode=2, wE=1.0, wF=0.01, wS=0.001, nthreads=8)
     g = m.efs_grad(frames, out["ebar"], out["fbar"], out["sbar"], nthreads=8) / 2
     close(g, out["grad"], 1e-12)
+
+
+# Hand-derived loss values, dev_src/non_linear_regression.cpp:986-1037
+# (norm_by_forces = 0, L2 force residual).  Two atoms, wE = 1, wF = 0.01,
+# wS = 0.001; residuals dE = 0.5, dF = [(0.1, 0, 0), (0, 0.2, 0)],
+# dS (Voigt xx yy zz xy xz yz) = (0.3, 0, 0, 0.1, 0, 0), so that
+# sum |dF|^2 = 0.05 and |dS|_F^2 = 0.09 + 2 * 0.01 = 0.11 (the off-diagonal
+# counted twice in the full 3x3 matrix).  Per mode:
+#   vibrations  L = 1*0.25/2 + 0.01*0.05 + 0.001*0.11/2           = 0.125555
+#               dL/dE_i = 2*1*0.5/2 = 0.5, dL/dF = 0.02 dF, dL/dS = -2*0.001 dS/2
+#   structures  weights / N:  L = 0.5*0.25/2 + 0.005*0.05 + 0.0005*0.11/2 = 0.0627775
+#               dL/dE_i = 0.25, dL/dF = 0.01 dF, dL/dS = -0.0005 dS
+#   molecules   energy, stress weights * N (force weight kept):
+#               L = 2*0.25/2 + 0.01*0.05 + 0.002*0.11/2        = 0.25061
+#               dL/dE_i = 1.0, dL/dF = 0.02 dF, dL/dS = -0.002 dS
+LOSS_F0 = np.array([[1.0, 2.0, -1.0], [0.5, -0.5, 0.25]])
+LOSS_DF = np.array([[0.1, 0.0, 0.0], [0.0, 0.2, 0.0]])
+LOSS_S0 = np.array([1.0, 2.0, 3.0, 0.5, 0.25, -0.25])
+LOSS_DS = np.array([0.3, 0.0, 0.0, 0.1, 0.0, 0.0])
+LOSS_CASES = [
+    # mode, loss, dL/dE_i, dL/dF factor, dL/dS factor (times the full dS matrix)
+    (0, 0.125555, 0.5, 0.02, -0.001),
+    (1, 0.0627775, 0.25, 0.01, -0.0005),
+    (2, 0.25061, 1.0, 0.02, -0.002),
+]
+
+
+@pytest.mark.parametrize("mode,loss,dE,fF,fS", LOSS_CASES)
+def test_frame_loss_hand_derived(oracle_lib, mode, loss, dE, fF, fS):
+    L, dEi, dF, dS = oracle_lib.frame_loss(mode, 1.0, 0.01, 0.001, 1.5, LOSS_F0 + LOSS_DF, LOSS_S0 + LOSS_DS, 1.0,
+                                           LOSS_F0, LOSS_S0)
+    assert abs(L - loss) <= 1e-15
+    assert np.allclose(dEi, [dE, dE], rtol=0, atol=1e-15)
+    assert np.allclose(dF, fF * LOSS_DF, rtol=0, atol=1e-15)
+    dS_full = np.array([[0.3, 0.1, 0.0], [0.1, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    assert np.allclose(dS, fS * dS_full, rtol=0, atol=1e-15)
+
+
+def test_frame_loss_energy_sign_and_zero_weights(oracle_lib):
+    """Three atoms, energy only (wF = wS = 0), structures: dE = -0.6 gives
+    L = (1/3) 0.36 / 3 = 0.04 and dL/dE_i = 2 (1/3) (-0.6) / 3 = -2/15; the
+    force and stress adjoints are exactly zero."""
+    F = np.arange(9.0).reshape(3, 3)
+    L, dEi, dF, dS = oracle_lib.frame_loss(1, 1.0, 0.0, 0.0, -2.0, F, np.ones(6), -1.4, F * 0.5, np.zeros(6))
+    assert abs(L - 0.04) <= 1e-16
+    assert np.allclose(dEi, -2.0 / 15.0, rtol=0, atol=1e-16)
+    assert not dF.any() and not dS.any()
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_frame_loss_invariants(oracle_lib, mode):
+    """Labels equal to predictions give zero loss and zero adjoints; each term
+    of `structures` is exactly 1/N of `vibrations`, and `molecules` scales
+    the energy and stress terms by N and leaves the force term."""
+    rng = np.random.default_rng(mode)
+    n = 5
+    E, F, S = rng.normal(), rng.normal(size=(n, 3)), rng.normal(size=6)
+    L, dEi, dF, dS = oracle_lib.frame_loss(mode, 1.0, 0.01, 0.001, E, F, S, E, F, S)
+    assert L == 0.0 and not dEi.any() and not dF.any() and not dS.any()
+    E0, F0, S0 = rng.normal(), rng.normal(size=(n, 3)), rng.normal(size=6)
+    scale = {0: (1, 1, 1), 1: (1 / n, 1 / n, 1 / n), 2: (n, 1, n)}[mode]
+    for k, w in enumerate(((1.0, 0, 0), (0, 0.01, 0), (0, 0, 0.001))):
+        lv = oracle_lib.frame_loss(0, *w, E, F, S, E0, F0, S0)[0]
+        lm = oracle_lib.frame_loss(mode, *w, E, F, S, E0, F0, S0)[0]
+        assert lv > 0
+        assert abs(lm - scale[k] * lv) <= 1e-14 * lv
+
+
+def test_objective_is_mean_of_frame_losses(oracle_lib):
+    """orc_loss = (1/K) sum_f L_f over the frames (the trainer's /K,
+    dev_src/mtpr_trainer.cpp:4737-4744), L_f from each frame's EFS."""
+    from superlinear_amd import workloads as W
+    m = oracle_lib.OracleModel(W.model_path("effect_l3k4r2o2"))
+    frames = W.periodic_frames(3, 5, 8, 2, 0.15, seed=17)
+    rng = np.random.default_rng(9)
+    n = sum(len(f["types"]) for f in frames)
+    labels = dict(energy=rng.normal(size=3), forces=rng.normal(size=(n, 3)), virial=rng.normal(size=(3, 6)))
+    r = m.efs(frames)
+    tot, a = 0.0, 0
+    for f, fr in enumerate(frames):
+        k = len(fr["types"])
+        tot += oracle_lib.frame_loss(1, 1.0, 0.01, 0.001, r["energy"][f], r["forces"][a:a + k], r["virial"][f],
+                                     labels["energy"][f], labels["forces"][a:a + k], labels["virial"][f])[0]
+        a += k
+    assert abs(m.loss(frames, labels, 1, 1.0, 0.01, 0.001) - tot / 3) <= 1e-13 * tot
+
+
+HI_L = [("effect_l4k6r2o3", 3), ("effect_l5k6r2o3", 2)]
+
+
+@pytest.mark.parametrize("mname,species", HI_L)
+def test_rotation_invariance_high_l(oracle_lib, mname, species):
+    """L = 4 and 5 models: energy invariant and forces / virial covariant under
+    a random rotation of a periodic triclinic frame (a wrong l >= 4 coupling
+    coefficient or harmonic breaks this)."""
+    from superlinear_amd import workloads as W
+    m = oracle_lib.OracleModel(W.model_path(mname))
+    frame = W.periodic_frames(1, 10, 14, species, 0.15, seed=23)[0]
+    base = m.efs([frame])
+    q, _ = np.linalg.qr(np.random.default_rng(4).normal(size=(3, 3)))
+    rot = dict(frame, positions=frame["positions"] @ q.T, cell=frame["cell"] @ q.T)
+    r = m.efs([rot])
+    close(r["energy"], base["energy"], 1e-12)
+    close(r["forces"], base["forces"] @ q.T, 1e-12)
+    close(voigt_to_matrix(r["virial"][0]), q @ voigt_to_matrix(base["virial"][0]) @ q.T, 1e-12)
+    close(r["site"], base["site"], 1e-12)
+
+
+@pytest.mark.parametrize("mname,species", HI_L)
+def test_forces_and_virial_by_finite_differences_high_l(oracle_lib, mname, species):
+    from superlinear_amd import workloads as W
+    m = oracle_lib.OracleModel(W.model_path(mname))
+    frame = W.periodic_frames(1, 6, 8, species, 0.15, seed=13)[0]
+    r = m.efs([frame])
+    h = 1e-5
+    x = frame["positions"]
+    for i in range(0, len(x), 2):
+        for q in range(3):
+            fp = dict(frame, positions=x.copy())
+            fm = dict(frame, positions=x.copy())
+            fp["positions"][i, q] += h
+            fm["positions"][i, q] -= h
+            fd = -(_energy(m, fp) - _energy(m, fm)) / (2 * h)
+            assert abs(fd - r["forces"][i, q]) <= 1e-7 * (1 + abs(fd))
+    W6 = voigt_to_matrix(r["virial"][0])
+    for a, b in ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)):
+        def strained(s):
+            f = dict(frame, positions=x.copy(), cell=frame["cell"].copy())
+            f["positions"][:, a] += s * h * x[:, b]
+            f["cell"][:, a] += s * h * frame["cell"][:, b]
+            return _energy(m, f)
+        fd = -(strained(1) - strained(-1)) / (2 * h)
+        assert abs(fd - W6[a, b]) <= 1e-7 * (1 + abs(fd))
