@@ -20,6 +20,10 @@ training step = the objective (weighting "structures", energy weight 1, force
 weight 0.01: E/atom MSE + 0.01 force MSE) and its full parameter gradient for
 the whole batch (EFS, loss adjoints, forward-over-reverse gradient kernels).
 
+Static workload ("static64"): Static L3/K4/O2, one species, the 64-atom
+diamond (2x2x2 cells, sigma 0.10 A), plain FP64 EFS (and FP32) with the
+reference's embedded CPU engine timed beside it.
+
 Batched workload ("batch4096"): Effect L4/K6/R2/O3, 3 species, 4096 frames
 with N ~ logU[16, 512], half periodic cells and half molecules, all in one
 call; atom-steps/s, p50/p99 batch latency, FP64 ratio and the force-loss
@@ -77,11 +81,35 @@ def _timed_train(model, batch, labels, precision, steps, warmup, profile):
     return t1 - t0, loss, g.cpu().numpy(), prof
 
 
-def train_section(path, steps, warmup, profile):
+def labels_host(lab, batch):
+    return dict(energy=lab["energy"].cpu().numpy(), forces=lab["forces"].cpu().numpy())
+
+
+def train_cpu_baseline(path, frames, labels, threads=None, count=1):
+    """The FP64 oracle's objective + full gradient on the first `count`
+    training frames (hyper-dual numbers: one forward pass per coefficient,
+    OpenMP over coefficients).  There is no reference gradient engine that
+    builds here, so this is the port's figure, not a reference-equivalent."""
+    from oracle import oracle
+    _oracle_ready()
+    threads = threads or min(16, os.cpu_count() or 1)
+    sub = frames[:count]
+    n = sum(len(f["types"]) for f in sub)
+    lab = dict(energy=labels["energy"][:count], forces=labels["forces"][:n])
+    m = oracle.OracleModel(path)
+    t0 = time.perf_counter()
+    m.loss_grad(sub, lab, mode=1, wE=1.0, wF=0.01, wS=0.0, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=round(count / dt, 4), unit="frame-gradients/s", cores=threads, kind="port",
+                sample=f"first {count} training frames ({n} atoms), objective + all {m.offsets()['ncoeff']} "
+                       f"gradient entries by hyper-dual forward passes, {dt:.1f} s")
+
+
+def train_section(path, steps, warmup, profile, no_cpu=False):
     """Loss + parameter gradient over the 256-frame training batch, FP32 and FP64."""
     import torch
     from superlinear_amd import workloads as W
-    from superlinear_amd.flops import PEAK, kernel_flops, step_grad_per_edge
+    from superlinear_amd.flops import kernel_flops, step_grad_per_edge
     from superlinear_amd.model import DeviceBatch, Sus2Model
     frames = W.config_frames("train256")
     nf = len(frames)
@@ -119,17 +147,13 @@ def train_section(path, steps, warmup, profile):
     if prof:
         out["kernels_train_fp32"] = {k: dict(avg_us=round(1e3 * ms / max(c, 1), 2), launches=c)
                                      for k, (ms, c) in prof.items()}
-        grad = {k: v for k, v in prof.items() if k.startswith("grad_") and kernel_flops(k, dims, 1)}
+        grad = {k: v for k, v in prof.items() if k.startswith("grad_") and kernel_flops(k, dims, 1) is not None}
         if grad:
             dom = max(grad.items(), key=lambda kv: kv[1][0])[0]
             avg_s = prof[dom][0] / prof[dom][1] / 1e3
-            fl = kernel_flops(dom, dims, nedges)
-            ach = fl / avg_s / 1e12
-            out["train_roofline"] = dict(bound="mfma", pipe="valu", kernel=dom, achieved=round(ach, 4),
-                                         peak=PEAK["fp32_tflops"], unit="TFLOP/s",
-                                         frac=round(ach / PEAK["fp32_tflops"], 5),
-                                         traffic=measured_traffic(dom, "train256"), flops_per_launch=fl,
-                                         avg_launch_us=round(avg_s * 1e6, 2))
+            out["train_roofline"] = _roofline(dom, avg_s, kernel_flops(dom, dims, nedges), "fp32",
+                                              measured_traffic(dom, "train256"))
+    out["train_cpu_baseline"] = None if no_cpu else train_cpu_baseline(path, frames, labels_host(lab, batch))
     return out
 
 
@@ -148,26 +172,81 @@ def _call_latencies(fn, steps, warmup):
     return np.array(lat)
 
 
-def _roofline_of(prof, dims, nedges):
+def _roofline(kernel, avg_s, flops_, precision, traffic=None):
+    """Roofline entry of one kernel: algorithmic flops per launch over its
+    average launch time, against MI355X's vector peak of the precision."""
+    from superlinear_amd.flops import PEAK
+    ach = flops_ / avg_s / 1e12
+    if precision == "fp32":
+        peak, peak_np = PEAK["fp32_tflops"], PEAK["fp32_nonpacked_tflops"]
+        basis = ("FP32 vector peak 157.3 TF counts packed two-wide FMA; this build emits no packed FP32, "
+                 "so its reachable FP32 rate is the one-FMA-per-lane 78.6 TF (frac_nonpacked)")
+    else:
+        peak = peak_np = PEAK["fp64_tflops"]
+        basis = "FP64 vector peak 78.6 TF"
+    return dict(bound="valu", kernel=kernel, achieved=round(ach, 4), peak=peak, unit="TFLOP/s",
+                frac=round(ach / peak, 5), frac_nonpacked=round(ach / peak_np, 5), peak_basis=basis,
+                traffic=traffic, flops_per_launch=int(flops_), avg_launch_us=round(avg_s * 1e6, 2))
+
+
+def _roofline_of(prof, dims, nedges, natoms, graph, precision, workload=None):
     """Roofline entry of the kernel with the largest summed time among those
     with an algorithmic flop count (HIP-event profile of the same calls)."""
-    from superlinear_amd.flops import PEAK, kernel_flops
-    timed = {k: v for k, v in prof.items() if kernel_flops(k, dims, 1) is not None}
+    from superlinear_amd.flops import kernel_flops
+    timed = {k: v for k, v in prof.items() if kernel_flops(k, dims, 1, 1, graph) is not None}
     if not timed:
         return None
     dom = max(timed.items(), key=lambda kv: kv[1][0])[0]
     avg_s = prof[dom][0] / prof[dom][1] / 1e3
-    fl = kernel_flops(dom, dims, nedges)
-    ach = fl / avg_s / 1e12
-    return dict(bound="mfma", pipe="valu", kernel=dom, achieved=round(ach, 4), peak=PEAK["fp32_tflops"],
-                unit="TFLOP/s", frac=round(ach / PEAK["fp32_tflops"], 5), flops_per_launch=fl,
-                avg_launch_us=round(avg_s * 1e6, 2))
+    fl = kernel_flops(dom, dims, nedges, natoms, graph)
+    return _roofline(dom, avg_s, fl, precision, measured_traffic(dom, workload) if workload else None)
+
+
+def static_section(steps, warmup, profile, no_cpu):
+    """The 64-atom Static L3/K4/O2 diamond frame (1 species, FP64 plain path
+    and FP32), errors against the oracle, the reference CPU engine beside it."""
+    from oracle import oracle
+    from superlinear_amd import workloads as W
+    from superlinear_amd.flops import graph_stats, step_efs_per_atom, step_efs_per_edge
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    path = W.model_path("static_l3k4o2")
+    frames = W.config_frames("static64")
+    n = len(frames[0]["types"])
+    batch = DeviceBatch(frames)
+    model = Sus2Model(path)
+    dims = model.dims()
+    print(f"[bench] static64: {n} atoms", file=sys.stderr, flush=True)
+    t64, o64, _ = _timed(model, batch, "fp64", steps, warmup, False)
+    nedges = model.last_edge_count()
+    t32, o32, _ = _timed(model, batch, "fp32", steps, warmup, False)
+    graph = graph_stats(path)
+    out = dict(workload="static64", model="Static L3/K4/O2 body 4, B=10, 1 species", atoms=n, directed_edges=nedges,
+               atom_steps_per_s=round(n * steps / t64, 1), ms_per_step=round(1e3 * t64 / steps, 4),
+               dtype="f64", fp32_atom_steps_per_s=round(n * steps / t32, 1),
+               fp32_ms_per_step=round(1e3 * t32 / steps, 4),
+               step_formula_tflops=round((step_efs_per_edge(dims) * nedges + step_efs_per_atom(dims, graph) * n)
+                                         * steps / t64 / 1e12, 5))
+    if profile:
+        prof = _timed(model, batch, "fp64", steps, 1, True)[2]
+        out["kernels_fp64"] = {k: dict(avg_us=round(1e3 * ms / max(c, 1), 2), launches=c) for k, (ms, c) in prof.items()}
+        out["roofline"] = _roofline_of(prof, dims, nedges, n, graph, "fp64", "static64")
+    if not no_cpu:
+        _oracle_ready()
+        ref = oracle.OracleModel(path).efs(frames)
+        e64 = {k: v.cpu().numpy() for k, v in o64.items()}
+        e32 = {k: v.cpu().numpy() for k, v in o32.items()}
+        out["errors_vs_fp64_oracle"] = dict(
+            fp64_energy_rel=float(abs(e64["energy"][0] - ref["energy"][0]) / abs(ref["energy"][0])),
+            fp64_force_maxabs=float(np.abs(e64["forces"] - ref["forces"]).max()),
+            fp32_force_maxabs=float(np.abs(e32["forces"] - ref["forces"]).max()))
+        out["cpu_baseline"] = cpu_baseline(path, frames, "static64")
+    return out
 
 
 def batch_section(steps, warmup, profile):
     """Uneven batched EFS and its force-loss gradient: 4096 frames in one call."""
     from superlinear_amd import workloads as W
-    from superlinear_amd.flops import step_efs_per_edge
+    from superlinear_amd.flops import graph_stats, step_efs_per_edge
     from superlinear_amd.model import DeviceBatch, Sus2Model
     path = W.model_path("effect_l4k6r2o3")
     frames = W.config_frames("batch4096")
@@ -204,14 +283,14 @@ def batch_section(steps, warmup, profile):
         model.profile(False)
         prof = model.profile_read()
         out["kernels_fp32"] = {k: dict(avg_us=round(1e3 * ms / max(c, 1), 2), launches=c) for k, (ms, c) in prof.items()}
-        out["roofline"] = _roofline_of(prof, dims, nedges)
+        out["roofline"] = _roofline_of(prof, dims, nedges, n, graph_stats(path), "fp32", "batch4096")
     return out
 
 
 def md_section(steps, warmup, profile):
     """One large frame: 2,097,152-atom diamond, Effect L5/K6, FP32 and FP64."""
     from superlinear_amd import workloads as W
-    from superlinear_amd.flops import step_efs_per_edge
+    from superlinear_amd.flops import graph_stats, step_efs_per_edge
     from superlinear_amd.model import DeviceBatch, Sus2Model
     path = W.model_path("effect_l5k6r2o3")
     frame = W.config_frames("md2m")[0]
@@ -238,49 +317,117 @@ def md_section(steps, warmup, profile):
         model.profile(False)
         prof = model.profile_read()
         out["kernels_fp32"] = {k: dict(avg_us=round(1e3 * ms / max(c, 1), 2), launches=c) for k, (ms, c) in prof.items()}
-        out["roofline"] = _roofline_of(prof, dims, nedges)
+        out["roofline"] = _roofline_of(prof, dims, nedges, n, graph_stats(path), "fp32", "md2m")
     return out
 
 
-def cpu_baseline(model_path, frame, threads, budget_s=20.0):
-    """FP64 oracle EFS on the host cores: `threads` copies of the frame, one
-    per OpenMP thread (frame-parallel, as the reference shards frames over
-    MPI ranks).  Returns atom-steps/s."""
+def _oracle_ready():
     from oracle import oracle
     try:
         oracle.lib()
     except RuntimeError:
         oracle.build()
+
+
+def _calibration(workload):
+    """reference_over_oracle of a workload from tests/golden/cpu_calibration.json."""
+    try:
+        cal = json.load(open(os.path.join(ROOT, "tests", "golden", "cpu_calibration.json")))
+        return cal["workloads"][workload]["reference_over_oracle"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def reference_cpu_baseline(model_path, frames, procs, budget_s=15.0):
+    """The reference's embedded EFS engine (built from its own sources into
+    oracle/_ref/ref-timing) on the host cores: `procs` independent processes,
+    each evaluating the frames repeatedly for about budget_s (throughput mode,
+    as independent MPI ranks on copies of the work).  Neighbour lists are
+    built before each process's timed region.  None when the binary is absent
+    or fails on this host."""
+    import subprocess
+    import tempfile
+    from oracle import oracle
+    binary = oracle.ref_timing_binary()
+    if binary is None:
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from make_cpu_calibration import write_frames
+    n = sum(len(f["types"]) for f in frames)
+    with tempfile.TemporaryDirectory() as tmp:
+        fp = os.path.join(tmp, "frames.txt")
+        write_frames(fp, frames)
+        try:
+            one = subprocess.run([str(binary), model_path, fp, "1"], capture_output=True, text=True, check=True,
+                                 timeout=300)
+            single = json.loads(one.stdout.split("REF_TIMING_JSON ")[1])["seconds_per_rep"]
+            reps = max(1, int(budget_s / max(single, 1e-6)))
+            ps = [subprocess.Popen([str(binary), model_path, fp, str(reps)], stdout=subprocess.PIPE, text=True)
+                  for _ in range(procs)]
+            secs = [json.loads(p.communicate(timeout=budget_s * 10 + 300)[0].split("REF_TIMING_JSON ")[1])
+                    ["seconds_per_rep"] for p in ps]
+        except (OSError, subprocess.SubprocessError, IndexError, ValueError) as e:
+            print(f"[bench] reference CPU engine unavailable here: {e}", file=sys.stderr, flush=True)
+            return None
+    value = sum(n / t for t in secs)
+    return dict(value=round(value, 1), unit="atom-steps/s", cores=procs, kind="reference",
+                sample=f"{procs} processes x {reps} evaluations of {len(frames)} frame(s) of {n} atoms by the "
+                       f"reference embedded engine (interfaces/lammps/ML-SUS2, -O3, direct radial mode, "
+                       f"prebuilt edge lists), {max(secs) * reps:.1f} s each")
+
+
+def oracle_cpu_baseline(model_path, frames, threads, workload, budget_s=15.0):
+    """FP64 oracle EFS on the host cores: copies of the frames, one per OpenMP
+    thread (frame-parallel), with the reference-equivalent rate from the
+    committed single-thread calibration."""
+    from oracle import oracle
+    _oracle_ready()
     m = oracle.OracleModel(model_path)
     t0 = time.perf_counter()
-    m.efs([frame], nthreads=1)
+    m.efs(frames, nthreads=1)
     single = time.perf_counter() - t0
     copies = max(1, int(round(budget_s / max(single, 1e-3))))
     copies = threads * max(1, -(-copies // threads))  # whole rounds of `threads` frames
     t0 = time.perf_counter()
-    m.efs([frame] * copies, nthreads=threads)
+    m.efs(frames * copies, nthreads=threads)
     dt = time.perf_counter() - t0
-    n = len(frame["types"])
-    return dict(value=n * copies / dt, unit="atom-steps/s", cores=threads, kind="port",
-                sample=f"{copies} copies of the {n}-atom frame, FP64 oracle, OpenMP over frames, {dt:.1f} s")
+    n = sum(len(f["types"]) for f in frames)
+    out = dict(value=round(n * copies / dt, 1), unit="atom-steps/s", cores=threads, kind="port",
+               sample=f"{copies} copies of {len(frames)} frame(s) of {n} atoms, FP64 oracle, OpenMP over copies, "
+                      f"{dt:.1f} s")
+    r = _calibration(workload)
+    if r:
+        out["reference_equivalent_value"] = round(out["value"] / r, 1)
+        out["calibration"] = f"reference/oracle single-thread time {r:.3f} (tests/golden/cpu_calibration.json)"
+    return out
+
+
+def cpu_baseline(model_path, frames, workload, threads=None):
+    """The reference engine on the host cores when its prebuilt binary runs
+    here, else the oracle port with its calibrated reference-equivalent rate."""
+    threads = threads or min(16, os.cpu_count() or 1)
+    ref = reference_cpu_baseline(model_path, frames, threads)
+    port = oracle_cpu_baseline(model_path, frames, threads, workload, budget_s=8.0 if ref else 15.0)
+    if ref is None:
+        return port
+    ref["port"] = port
+    return ref
 
 
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` in `workload` (FETCH_SIZE + WRITE_SIZE,
     each scaled by the copy-kernel calibration) from the committed rocprofv3
-    counter summary, or None when the summary does not cover this kernel."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_effect4096.md")
-    try:
-        text = open(path).read()
-    except OSError:
-        return None
+    counter summaries (profiles/hbm_traffic_*.md), or None when none covers it."""
+    import glob
     import re
-    m = re.search(r"```json\n(.*?)\n```", text, re.S)
-    if not m:
-        return None
-    table = json.loads(m.group(1)).get(workload, {})
-    v = table.get(kernel)
-    return None if v is None else int(v)
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "hbm_traffic_*.md"))):
+        m = re.search(r"```json\n(.*?)\n```", open(path).read(), re.S)
+        if not m:
+            continue
+        v = json.loads(m.group(1)).get(workload, {}).get(kernel)
+        if v is not None:
+            return int(v)
+    return None
 
 
 def main():
@@ -296,6 +443,11 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=10)
     ap.add_argument("--no-md", action="store_true", help="skip the 2,097,152-atom frame workload")
     ap.add_argument("--md-steps", type=int, default=3)
+    ap.add_argument("--no-static", action="store_true", help="skip the 64-atom Static FP64 workload")
+    ap.add_argument("--static-steps", type=int, default=200)
+    ap.add_argument("--only", choices=["effect4096", "static64", "train256", "batch4096", "md2m"],
+                    help="run one workload section only (for per-workload profiles); the headline line still "
+                         "reports effect4096 unless it is skipped")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed runs, launch a plain FP32 copy kernel (y = 2x) over this many MiB, "
                          "three times: the calibration dispatch for rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
@@ -305,9 +457,12 @@ def main():
               file=sys.stderr)
         return 2
 
+    if args.only:
+        for k in ("static", "train", "batch", "md"):
+            setattr(args, "no_" + k, not args.only.startswith(k))
     import torch
     from superlinear_amd import workloads as W
-    from superlinear_amd.flops import PEAK, kernel_flops, step_efs_per_edge
+    from superlinear_amd.flops import graph_stats, step_efs_per_atom, step_efs_per_edge
     from superlinear_amd.model import DeviceBatch, Sus2Model
 
     path = W.model_path("effect_l3k4r2o2")
@@ -336,10 +491,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle
-        try:
-            oracle.lib()
-        except RuntimeError:
-            oracle.build()
+        _oracle_ready()
         ref = oracle.OracleModel(path).efs([frame], nthreads=1)
         errors = dict(
             fp64_gpu_energy_rel=float(abs(e64["energy"][0] - ref["energy"][0]) / abs(ref["energy"][0])),
@@ -348,27 +500,20 @@ def main():
             fp32_force_maxabs=float(np.abs(e32["forces"] - ref["forces"]).max()),
             fp32_virial_maxabs=float(np.abs(e32["virial"] - ref["virial"]).max()),
             force_scale_maxabs=float(np.abs(ref["forces"]).max()))
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(path, frame, threads)
+        cpu = cpu_baseline(path, [frame], "effect4096")
 
+    graph = graph_stats(path)
     roof = None
     kern = {}
     if prof32:
         for name, (ms, cnt) in prof32.items():
-            kern[name] = dict(avg_us=1e3 * ms / max(cnt, 1), launches=cnt)
-        timed = {k: v for k, v in prof32.items() if kernel_flops(k, dims, 1) is not None}
-        dom = max(timed.items(), key=lambda kv: kv[1][0])[0]
-        avg_s = prof32[dom][0] / prof32[dom][1] / 1e3
-        fl = kernel_flops(dom, dims, nedges)
-        if fl is not None:
-            ach = fl / avg_s / 1e12
-            roof = dict(bound="mfma", pipe="valu", kernel=dom, achieved=round(ach, 4), peak=PEAK["fp32_tflops"],
-                        unit="TFLOP/s", frac=round(ach / PEAK["fp32_tflops"], 5), traffic=None,
-                        flops_per_launch=fl, avg_launch_us=round(avg_s * 1e6, 2))
-    formula_rate = step_efs_per_edge(dims) * nedges * args.steps / t32 / 1e12
-    if roof is not None:
-        roof["traffic"] = measured_traffic(roof["kernel"], "effect4096")
-    train = {} if args.no_train else train_section(path, args.train_steps, 1, not args.no_profile)
+            kern[name] = dict(avg_us=round(1e3 * ms / max(cnt, 1), 2), launches=cnt)
+        roof = _roofline_of(prof32, dims, nedges, n, graph, "fp32", "effect4096")
+    formula_rate = (step_efs_per_edge(dims) * nedges + step_efs_per_atom(dims, graph) * n) * args.steps / t32 / 1e12
+    static64 = None if args.no_static else static_section(args.static_steps, 20, not args.no_profile,
+                                                           args.no_cpu_baseline)
+    train = {} if args.no_train else train_section(path, args.train_steps, 1, not args.no_profile,
+                                                     args.no_cpu_baseline)
     batch4096 = None if args.no_batch else batch_section(args.batch_steps, 2, not args.no_profile)
     md2m = None if args.no_md else md_section(args.md_steps, 1, not args.no_profile)
 
@@ -409,6 +554,7 @@ def main():
         "gpu": torch.cuda.get_device_name(0),
     }
     line.update({k: v for k, v in train.items() if k != "train_grad_frames_per_s"})
+    line["static64"] = static64
     line["batch4096"] = batch4096
     line["md2m"] = md2m
     print(json.dumps(line))

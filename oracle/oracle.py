@@ -53,6 +53,7 @@ def lib():
                                    ctypes.c_int]
         L.orc_loss.argtypes = [ctypes.c_void_p, ctypes.c_int, _lp, _ip, _dp, _dp, _ip, _dp, _dp, _dp,
                                ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_set_grad_subset.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
         L.orc_frame_loss.restype = ctypes.c_double
         L.orc_frame_loss.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_double, _dp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]
@@ -122,6 +123,12 @@ class OracleModel:
                                    _ptr(W, _dp), _ptr(S, _dp), nthreads))
         return dict(energy=E, forces=F, virial=W, site=S)
 
+    def grad_subset(self, idx=None):
+        """Compute only the listed gradient entries (None: all); the others
+        come back as zero."""
+        a = np.ascontiguousarray(np.zeros(0) if idx is None else idx, np.int32)
+        _check(lib().orc_set_grad_subset(self.h, a.size, _ptr(a, _ip)))
+
     def loss_grad(self, frames, labels, mode=1, wE=1.0, wF=0.01, wS=0.0, nthreads=1):
         b = pack(frames)
         nf = len(frames)
@@ -154,6 +161,21 @@ class OracleModel:
                                   _ptr(b["cell"], _dp), _ptr(b["periodic"], _ip), _ptr(ew, _dp), _ptr(fw, _dp),
                                   _ptr(sw, _dp), _ptr(grad, _dp), nthreads))
         return grad
+
+    def efs_edges(self, types, ij, d):
+        """EFS of one domain owning every atom from a given directed edge list
+        (ij [E,2], d [E,3] = x_j - x_i): returns (energy, forces [n,3])."""
+        types = np.ascontiguousarray(types, np.int32)
+        ij = np.ascontiguousarray(ij, np.int32)
+        d = np.ascontiguousarray(d, np.float64)
+        n = len(types)
+        E = np.zeros(1)
+        F = np.zeros((n, 3))
+        W = np.zeros(6)
+        S = np.zeros(n)
+        _check(lib().orc_efs_edges(self.h, n, n, _ptr(types, _ip), len(ij), _ptr(ij, _ip), _ptr(d, _dp),
+                                   _ptr(E, _dp), _ptr(F, _dp), _ptr(W, _dp), _ptr(S, _dp)))
+        return float(E[0]), F
 
     def loss(self, frames, labels, mode=1, wE=1.0, wF=0.01, wS=0.0):
         b = pack(frames)
@@ -226,6 +248,11 @@ def build():
     subprocess.run(["make", "-C", str(_HERE), "liboracle.so"], check=True, stdout=subprocess.DEVNULL)
     if Path("/root/reference/interfaces/lammps/ML-SUS2").is_dir():
         subprocess.run(["make", "-C", str(_HERE), "ref"], check=True, stdout=subprocess.DEVNULL)
+
+
+def ref_timing_binary():
+    p = _HERE / "_ref" / "ref-timing"
+    return p if p.exists() and os.access(p, os.X_OK) else None
 
 
 def ref_binary():

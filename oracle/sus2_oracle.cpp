@@ -115,6 +115,9 @@ struct Model {
   int o_shift = 0, o_scal = 0, o_rad = 0, o_grad = -1, o_gscal = -1, o_alpha = -1, o_gtype = -1,
       o_W = -1, o_gw = -1, o_ow = -1, o_species = 0, o_read = 0, ncoeff = 0;
   int rad_stride_gate = 0;
+  // coefficients whose gradient entries are computed (empty: all); the others
+  // are left zero (lets large batches be checked on a sample of entries)
+  std::vector<int> grad_subset;
 };
 
 static std::string trim(const std::string& s) {
@@ -740,8 +743,13 @@ void frame_grad(const Model& m, int n, const int* type, const std::vector<Edge>&
     std::vector<HD> th(P);
     for (int p = 0; p < P; ++p) th[p] = HD(m.theta[p]);
     std::vector<HD> site;
+    const bool sub = !m.grad_subset.empty();
+    const int np = sub ? (int)m.grad_subset.size() : P;
 #pragma omp for schedule(dynamic, 4)
-    for (int p = 0; p < P; ++p) {
+    for (int p = 0; p < P; ++p) out[p] = 0.0;
+#pragma omp for schedule(dynamic, 4)
+    for (int k = 0; k < np; ++k) {
+      const int p = sub ? m.grad_subset[k] : k;
       th[p].b = 1.0;
       forward<HD>(m, th.data(), n, type, te, site);
       double g = 0;
@@ -970,6 +978,18 @@ int orc_loss(void* h, int nframes, const long* off, const int* type, const doubl
                       dEi.data(), dF.data(), dS);
     }
     *loss_out = L / nframes;
+  });
+}
+
+// Restrict the gradient entries of orc_loss_grad / orc_efs_grad to the n
+// listed coefficients (n = 0: all); the other entries are returned as zero.
+int orc_set_grad_subset(void* h, int n, const int* idx) {
+  auto* m = static_cast<Model*>(h);
+  return guard([&] {
+    std::vector<int> v(idx, idx + n);
+    for (int p : v)
+      if (p < 0 || p >= m->ncoeff) fail("gradient subset index out of range");
+    m->grad_subset = v;
   });
 }
 

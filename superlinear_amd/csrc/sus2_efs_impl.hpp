@@ -11,13 +11,16 @@
 //                         h_i = W S_i and the gate node values for the reverse;
 //                         outer: site energy and the basic-moment adjoints.
 //   k_edge_backward<gate> lane = edge: g_e = dE_i/dd_e of the centre's own
-//                         energy for each directed edge, stored per edge, and
-//                         (outer pass of an Effect model) the gate-adjoint
-//                         message u_e of the neighbour's gate signal.
-//   k_gather<gate>        F_i = sum_{out edges} g_e - sum_{in edges} g_e' over
-//                         the sorted incoming-edge list (no atomics), and for
-//                         the outer pass of an Effect model h_bar_i followed by
-//                         the gate graph reverse to the gate basic adjoints.
+//                         energy for each directed edge; the centre's sum over
+//                         its outgoing edges and its virial are reduced in the
+//                         workgroup, g_e is stored at the edge's slot in its
+//                         target's incoming list, and (outer pass of an
+//                         Effect model) so is the gate-adjoint message u_e of
+//                         the neighbour's gate signal.
+//   k_gather<gate>        F_i -= sum_{in edges} g_e' over the contiguous
+//                         incoming slots (no atomics), and for the outer pass
+//                         of an Effect model h_bar_i followed by the gate graph
+//                         reverse to the gate basic adjoints.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -371,8 +374,9 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
 // ------------------------------------------------------------------ edge backward
 // One directed edge e = i -> j of the centre's own energy:
 //   g_e = sum_b abar_i[b] w_mu (R'_mu u Y_b + R_mu grad Y_b),
-// written to gedge[e] (each pass overwrites it after the previous pass's
-// gather consumed it), and for the
+// summed over the centre's edges into F_i (with the virial -sum d (x) g) and
+// written to gedge_in at the edge's incoming slot (each pass overwrites it
+// after the previous pass's gather consumed it), and for the
 // outer pass of an Effect model the gate-adjoint message
 //   u_e,mu = tau_{t_i,mu} R_mu(r; t_i, t_j) sum_m abar_i[k,l,m] Y_lm(d_e),
 // which the neighbour j gathers into h_bar_j (times alpha_{t_j,mu}).
@@ -390,14 +394,16 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
                                                         const int* __restrict__ nbr_off,
                                                         const Rec<T>* __restrict__ rec, const T* __restrict__ h,
                                                         const T* __restrict__ adj, const int* __restrict__ in_pos,
-                                                        T* __restrict__ gedge, T* __restrict__ gedge_in,
-                                                        T* __restrict__ uedge, T* __restrict__ fw) {
+                                                        T* __restrict__ gedge_in, T* __restrict__ uedge,
+                                                        T* __restrict__ fw) {
   constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, F = Cf::F, KH = Cf::KH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr int NA = EB_ATOMS;
   T* sA = reinterpret_cast<T*>(smem_raw);  // [NA][KH]
-  T* sV = sA + NA * KH + species_tables_size<Cf>(m.C);  // [NA][6][WAVE] per-lane virial partials
-  T* sXR = sV + NA * 6 * WAVE;  // [F][LR] lane-private columns: x0 R (gate message)
+  // [NA][9][LR] per-lane partials of each centre: outgoing force sum, virial
+  // (padded rows, so that the row sums at the end read distinct banks)
+  T* sV = sA + NA * KH + species_tables_size<Cf>(m.C);
+  T* sXR = sV + NA * 9 * LR;  // [F][LR] lane-private columns: x0 R (gate message)
   SpeciesTables<T> tab;
   const int i0 = blockIdx.x * NA, lane = threadIdx.x;
   const int na = min(NA, natoms - i0);
@@ -405,7 +411,7 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
   load_species_tables<Cf>(m, GATE, sA + NA * KH, tab);
   for (int c = lane; c < na * KH; c += WAVE) sA[c] = adj[(size_t)i0 * KH + c];
 #pragma unroll
-  for (int c = 0; c < NA * 6; ++c) sV[c * WAVE + lane] = T(0);
+  for (int c = 0; c < NA * 9; ++c) sV[c * LR + lane] = T(0);
   // the centres' edge ranges: [eb[a], eb[a + 1]) for atom i0 + a
   int eb[NA + 1], ta[NA];
 #pragma unroll
@@ -413,11 +419,12 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
 #pragma unroll
   for (int a = 0; a < NA; ++a) ta[a] = types[i0 + min(a, na - 1)];
   __syncthreads();
-  // virial of the lane's current centre; a lane's edges visit the centres in
-  // order, so on a change of centre the partial goes to the lane's LDS slot
-  T w[6];
+  // outgoing force sum and virial of the lane's current centre; a lane's edges
+  // visit the centres in order, so on a change of centre the partial goes to
+  // the lane's LDS slot
+  T w[9];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) w[k] = T(0);
+  for (int k = 0; k < 9; ++k) w[k] = T(0);
   int acur = 0;
   for (int e = eb[0] + lane; e < eb[NA]; e += WAVE) {
     int ai = 0;  // the edge's centre among the workgroup's atoms
@@ -425,8 +432,8 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
     for (int a = 1; a < NA; ++a) ai += e >= eb[a];
     if (ai != acur) {
 #pragma unroll
-      for (int k = 0; k < 6; ++k) {
-        sV[(acur * 6 + k) * WAVE + lane] = w[k];
+      for (int k = 0; k < 9; ++k) {
+        sV[(acur * 9 + k) * LR + lane] = w[k];
         w[k] = T(0);
       }
       acur = ai;
@@ -483,12 +490,11 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
     });
     const T g0 = ux * sY + gx, g1 = uy * sY + gy, g2 = uz * sY + gz;
     const int slot = in_pos[e];  // the edge's position in its target's incoming list
-    T* ge = gedge + 3 * (size_t)e;
-    ge[0] = g0; ge[1] = g1; ge[2] = g2;
     T* gi = gedge_in + 3 * (size_t)slot;
     gi[0] = g0; gi[1] = g1; gi[2] = g2;
-    w[0] -= dx * g0; w[1] -= dy * g1; w[2] -= dz * g2;
-    w[3] -= dx * g1; w[4] -= dx * g2; w[5] -= dy * g2;
+    w[0] += g0; w[1] += g1; w[2] += g2;
+    w[3] -= dx * g0; w[4] -= dy * g1; w[5] -= dz * g2;
+    w[6] -= dx * g1; w[7] -= dx * g2; w[8] -= dy * g2;
     if (hgate) {
       T* ue = uedge + (size_t)slot * F;
 #pragma unroll
@@ -496,19 +502,24 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
     }
   }
 #pragma unroll
-  for (int k = 0; k < 6; ++k) sV[(acur * 6 + k) * WAVE + lane] = w[k];
-  T s[NA * 6];
+  for (int k = 0; k < 9; ++k) sV[(acur * 9 + k) * LR + lane] = w[k];
+  __syncthreads();
+  // one lane per (centre, component): its row of lane partials summed in a
+  // fixed order; the outer pass starts the atom's row, the gate pass adds
+  static_assert(NA * 9 <= WAVE, "one lane per centre component");
+  if (lane < NA * 9 && lane / 9 < na) {
+    const T* row = sV + lane * LR;
+    T s0 = T(0), s1 = T(0), s2 = T(0), s3 = T(0);
 #pragma unroll
-  for (int k = 0; k < NA * 6; ++k) s[k] = sV[k * WAVE + lane];
-  wave_sum_n(s);
-#pragma unroll
-  for (int a = 0; a < NA; ++a) {
-    if (a >= na) break;
-    if (lane == 0) {
-      T* o = fw + (size_t)(i0 + a) * 9 + 3;
-#pragma unroll
-      for (int k = 0; k < 6; ++k) o[k] = GATE ? o[k] + s[a * 6 + k] : s[a * 6 + k];
+    for (int e = 0; e < WAVE; e += 4) {
+      s0 += row[e];
+      s1 += row[e + 1];
+      s2 += row[e + 2];
+      s3 += row[e + 3];
     }
+    const T tot = (s0 + s1) + (s2 + s3);
+    T* o = fw + (size_t)(i0 + lane / 9) * 9 + lane % 9;
+    *o = GATE ? *o + tot : tot;
   }
 }
 
@@ -543,17 +554,16 @@ __device__ void gate_reverse_atom(const DModel<T>& m, int i, const T* __restrict
   wave_adjoint_to_basic<T, KK>(m, p, true, a, gbar + (size_t)i * Cf::KH);
 }
 
-// F_a = sum_{e in out(a)} g_e - sum_{e' in in(a)} g_e' over the centre's own
-// edges and its incoming-list slots (written there by the edge kernels; no
+// F_a -= sum_{e' in in(a)} g_e' over the atom's incoming-list slots (written
+// there by the edge kernels, which also added the atom's outgoing sum; no
 // atomics, contiguous reads).  The outer pass of an
 // Effect model also forms h_bar_a = alpha_{t_a} sum_{e' in in(a)} u_e' and
 // either runs the gate graph reverse at once (hbar_out == null) or stores
 // h_bar for a later reverse (the pair-style path, whose ghost adjoints are
-// communicated in between).  Atoms a >= nowned (ghosts) have no out edges.
+// communicated in between).  Ghost atoms (a >= nowned) start from a zeroed row.
 template <class Cf, bool GATE, class T = typename Cf::T>
-__global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __restrict__ types, int nowned,
-                                                 const int* __restrict__ out_off, const int* __restrict__ in_off,
-                                                 const T* __restrict__ gedge, const T* __restrict__ gedge_in,
+__global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __restrict__ types,
+                                                 const int* __restrict__ in_off, const T* __restrict__ gedge_in,
                                                  const T* __restrict__ uedge, const T* __restrict__ vg,
                                                  T* __restrict__ gbar, T* __restrict__ hbar_out,
                                                  T* __restrict__ fw) {
@@ -564,11 +574,6 @@ __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __r
   T hb[F];
 #pragma unroll
   for (int mu = 0; mu < F; ++mu) hb[mu] = T(0);
-  if (i < nowned)
-    for (int k = out_off[i] + lane; k < out_off[i + 1]; k += WAVE) {
-      const T* go = gedge + 3 * (size_t)k;
-      fx += go[0]; fy += go[1]; fz += go[2];
-    }
   // incoming slots (contiguous); narrow shapes take two batches per step so
   // that their loads are in flight together
   constexpr int IN_STEP = F <= 16 ? 2 : 1;
@@ -622,7 +627,7 @@ __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __r
       fz = __shfl(tot, 2, WAVE);
       if (lane == 0) {
         T* o = fw + (size_t)i * 9;
-        o[0] = fx; o[1] = fy; o[2] = fz;
+        o[0] += fx; o[1] += fy; o[2] += fz;
       }
       const int ti = types[i];
       if (lane >= 3 && lane < 3 + F) {
@@ -652,8 +657,7 @@ __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __r
   }
   if (lane == 0) {
     T* o = fw + (size_t)i * 9;
-    if (GATE) { o[0] += fx; o[1] += fy; o[2] += fz; }
-    else { o[0] = fx; o[1] = fy; o[2] = fz; }
+    o[0] += fx; o[1] += fy; o[2] += fz;
   }
   if (!hgate) return;
   const int ti = types[i];
@@ -696,7 +700,7 @@ size_t gather_lds(const DModel<typename Cf::T>& m) {
 template <class Cf>
 size_t backward_lds(const DModel<typename Cf::T>& m) {
   return sizeof(typename Cf::T) *
-         ((size_t)EB_ATOMS * (Cf::KH + 6 * WAVE) + species_tables_size<Cf>(m.C) + (m.gated ? (size_t)Cf::F * LR : 0));
+         ((size_t)EB_ATOMS * (Cf::KH + 9 * LR) + species_tables_size<Cf>(m.C) + (m.gated ? (size_t)Cf::F * LR : 0));
 }
 
 // ------------------------------------------------------------------ driver
@@ -731,7 +735,6 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
   ws.vg.ensure(sizeof(T) * n1 * std::max(1, m.gp.node_cnt));
   ws.site.ensure(sizeof(T) * n1);
   ws.fw.ensure(sizeof(T) * n1 * 9);
-  ws.gedge.ensure(sizeof(T) * ne * 3);
   ws.gedge_in.ensure(sizeof(T) * ne * 3);
   if (m.gated) ws.uedge.ensure(sizeof(T) * ne * Cf::F);
   const int* off = ws.nbr_off.get<int>();
@@ -749,19 +752,18 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
                 b.types, off, rec, h, ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
     const int* ip = ws.in_pos.get<int>();
     SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_outer"), (k_edge_backward<Cf, false>), dim3((N + EB_ATOMS - 1) / EB_ATOMS), dim3(WAVE), lb, m, N,
-                b.types, off, rec, h, ws.abar.get<T>(), ip, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(),
-                ws.fw.get<T>());
+                b.types, off, rec, h, ws.abar.get<T>(), ip, ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
     allow_lds(k_gather<Cf, false>, lg);
     SUS2_LAUNCH(ws, st, kname<Cf>("gather_outer"), (k_gather<Cf, false>), dim3(N), dim3(WAVE), m.gated ? lg : 0, m,
-                b.types, N, off, off, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
+                b.types, off, ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
                 ws.gbar.get<T>(), (T*)nullptr, ws.fw.get<T>());
     if (m.gated) {
       SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_gate"), (k_edge_backward<Cf, true>), dim3((N + EB_ATOMS - 1) / EB_ATOMS), dim3(WAVE), lb, m, N,
-                  b.types, off, rec, h, ws.gbar.get<T>(), ip, ws.gedge.get<T>(), ws.gedge_in.get<T>(),
-                  ws.uedge.get<T>(), ws.fw.get<T>());
-      SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(N), dim3(WAVE), 0, m, b.types, N, off,
-                  off, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>(),
-                  (T*)nullptr, ws.fw.get<T>());
+                  b.types, off, rec, h, ws.gbar.get<T>(), ip, ws.gedge_in.get<T>(), ws.uedge.get<T>(),
+                  ws.fw.get<T>());
+      SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(N), dim3(WAVE), 0, m, b.types, off,
+                  ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>(), (T*)nullptr,
+                  ws.fw.get<T>());
     }
   }
   finish_efs<T>(ws, b, out, SiteConst{dm.d.view.theta, m.o_species, m.o_shift, m.C}, st);
@@ -790,7 +792,6 @@ void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const Pai
     ws.vg.ensure(sizeof(T) * a1 * std::max(1, m.gp.node_cnt));
     ws.site.ensure(sizeof(T) * a1);
     ws.fw.ensure(sizeof(T) * a1 * 9);
-    ws.gedge.ensure(sizeof(T) * ne * 3);
     ws.gedge_in.ensure(sizeof(T) * ne * 3);
     if (m.gated) ws.uedge.ensure(sizeof(T) * ne * Cf::F);
     if (m.gated && all > 0) {
@@ -810,15 +811,15 @@ void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const Pai
       SUS2_LAUNCH(ws, st, kname<Cf>("pass_forward_outer"), (k_pass_forward<Cf, false>), dim3(n), dim3(WAVE), lf, m,
                   pv.types, off, rec, ws.h.get<T>(), ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
       SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_outer"), (k_edge_backward<Cf, false>), dim3((n + EB_ATOMS - 1) / EB_ATOMS), dim3(WAVE),
-                  lb, m, n, pv.types, off, rec, ws.h.get<T>(), ws.abar.get<T>(), ws.in_pos.get<int>(), ws.gedge.get<T>(),
+                  lb, m, n, pv.types, off, rec, ws.h.get<T>(), ws.abar.get<T>(), ws.in_pos.get<int>(),
                   ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
     }
     if (all > 0) {
       allow_lds(k_gather<Cf, false>, lg);  // (gated shapes with F >= 16 sum their messages through LDS)
       SUS2_LAUNCH(ws, st, kname<Cf>("gather_outer"), (k_gather<Cf, false>), dim3(all), dim3(WAVE),
-                  (Cf::F >= 16 && m.gated) ? lg : 0, m, pv.types,
-                  n, off, pv.in_off, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
-                  ws.gbar.get<T>(), m.gated ? ws.hbar.get<T>() : (T*)nullptr, ws.fw.get<T>());
+                  (Cf::F >= 16 && m.gated) ? lg : 0, m, pv.types, pv.in_off, ws.gedge_in.get<T>(),
+                  ws.uedge.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>(), m.gated ? ws.hbar.get<T>() : (T*)nullptr,
+                  ws.fw.get<T>());
     }
     return;
   }
@@ -829,11 +830,11 @@ void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const Pai
     SUS2_LAUNCH(ws, st, kname<Cf>("gate_reverse"), (k_gate_reverse<Cf>), dim3(n), dim3(WAVE), lg, m,
                 ws.hbar.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>());
     SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_gate"), (k_edge_backward<Cf, true>), dim3((n + EB_ATOMS - 1) / EB_ATOMS), dim3(WAVE), lb, m, n,
-                pv.types, off, rec, ws.h.get<T>(), ws.gbar.get<T>(), ws.in_pos.get<int>(), ws.gedge.get<T>(),
-                ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
-    SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(all), dim3(WAVE), 0, m, pv.types, n,
-                off, pv.in_off, ws.gedge.get<T>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
-                ws.gbar.get<T>(), (T*)nullptr, ws.fw.get<T>());
+                pv.types, off, rec, ws.h.get<T>(), ws.gbar.get<T>(), ws.in_pos.get<int>(), ws.gedge_in.get<T>(),
+                ws.uedge.get<T>(), ws.fw.get<T>());
+    SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(all), dim3(WAVE), 0, m, pv.types,
+                pv.in_off, ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>(), (T*)nullptr,
+                ws.fw.get<T>());
   }
   Batch b;
   b.nframes = 1;

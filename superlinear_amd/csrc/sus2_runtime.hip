@@ -200,10 +200,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(int n, int* __restrict__ 
     total[0] = carry;
     total[1] = m;
     total[2] = over;
-    if (report) {  // two self-tagged words, each one 64-bit store (no fence)
+    if (report) {  // two self-tagged words, each one atomic 64-bit store at system scope (no fence)
       const uint64_t tag = (uint64_t)(seq & 0x7fffffff) << 32;
-      report[0] = (int64_t)(tag | (uint64_t)(uint32_t)m);
-      report[1] = (int64_t)(tag | ((uint64_t)over << 31) | (uint64_t)(uint32_t)min(carry, (long)0x7fffffff));
+      __hip_atomic_store(report, (int64_t)(tag | (uint64_t)(uint32_t)m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      __hip_atomic_store(report + 1,
+                         (int64_t)(tag | ((uint64_t)over << 31) | (uint64_t)(uint32_t)min(carry, (long)0x7fffffff)),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
 }

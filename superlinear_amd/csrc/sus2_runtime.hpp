@@ -69,12 +69,11 @@ struct Profiler {
 struct Workspace {
   Profiler prof;
   DevBuf nbr_cnt, nbr_off, scan_tmp, rec, h, abar, gbar, vg, site, fw;
-  // per-edge backward buffers: dE/dd of each directed edge, gate-adjoint
-  // messages, and the sorted incoming-edge lists
-  // gedge: dE/dd per edge in the centre's order; gedge_in, uedge: the same g
-  // and the gate messages at the edge's slot in its target's incoming list
-  // (in_pos), so that gathers read contiguous ranges
-  DevBuf gedge, gedge_in, uedge, in_edges, in_pos, in_cur;
+  // per-edge backward buffers: gedge_in, uedge hold dE/dd of each directed
+  // edge and its gate-adjoint message at the edge's slot in its target's
+  // incoming list (in_pos), so that gathers read contiguous ranges; in_edges
+  // the sorted incoming-edge lists of the atomic placement path
+  DevBuf gedge_in, uedge, in_edges, in_pos, in_cur;
   // fixed-capacity per-atom neighbour slots of the single-pass list build
   DevBuf nbr_slots;
   DevBuf nbr_oslot;  // bin-offset index of each slot's hit (ascending per atom)

@@ -40,6 +40,25 @@ def pack_frames(frames):
                 natoms=int(off[-1]))
 
 
+def _stream_of(stream, dev):
+    """The HIP stream handle a call runs on: the caller's, else torch's current."""
+    import torch
+    return stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+
+
+def _join(stream, dev):
+    """After a call on a caller-given stream: torch's current stream waits for
+    it, so that the returned tensors (allocated on the current stream) are not
+    read or recycled before the kernels writing them have finished."""
+    import torch
+    if stream is None:
+        return
+    ext = torch.cuda.ExternalStream(stream, device=dev)
+    cur = torch.cuda.current_stream(dev)
+    if ext.cuda_stream != cur.cuda_stream:
+        cur.wait_stream(ext)
+
+
 class Sus2Model:
     def __init__(self, path):
         self._h = ctypes.c_void_p()
@@ -147,13 +166,14 @@ class Sus2Model:
                        forces=torch.empty((n, 3), dtype=torch.float64, device=dev),
                        virial=torch.empty((nf, 6), dtype=torch.float64, device=dev),
                        site=torch.empty(n, dtype=torch.float64, device=dev))
-        st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        st = _stream_of(stream, dev)
         _lib.check(_lib.lib().sus2_efs(self._h, PRECISION[precision], nf, n, batch.off.data_ptr(),
                                        batch.types.data_ptr(), batch.pos.data_ptr(), batch.cell.data_ptr(),
                                        batch.periodic.data_ptr(), out["energy"].data_ptr(),
                                        out["forces"].data_ptr(), out["virial"].data_ptr(),
                                        out["site"].data_ptr() if out.get("site") is not None else None,
                                        ctypes.c_void_p(st)))
+        _join(stream, dev)
         return out
 
 
@@ -172,11 +192,12 @@ class Sus2Model:
         sw = str_w.to(device=dev, dtype=torch.float64).contiguous()
         if ew.numel() != batch.natoms or fw.numel() != 3 * batch.natoms or sw.numel() != 9 * batch.nframes:
             raise ValueError("weight shapes do not match the batch")
-        st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        st = _stream_of(stream, dev)
         _lib.check(_lib.lib().sus2_efs_grad(self._h, PRECISION[precision], batch.nframes, batch.natoms,
                                             batch.off.data_ptr(), batch.types.data_ptr(), batch.pos.data_ptr(),
                                             batch.cell.data_ptr(), batch.periodic.data_ptr(), ew.data_ptr(),
                                             fw.data_ptr(), sw.data_ptr(), grad.data_ptr(), ctypes.c_void_p(st)))
+        _join(stream, dev)
         return grad
 
     def loss_grad_device(self, batch, e0, f0, s0=None, weighting="structures", wE=1.0, wF=0.01, wS=0.0,
@@ -185,7 +206,9 @@ class Sus2Model:
         (NonLinearRegression::CalcObjectiveFunctionGrad,
         dev_src/non_linear_regression.cpp:1112).  Labels are device tensors:
         e0 [nf], f0 [N,3], s0 [nf,6] (xx yy zz xy xz yz).  Returns
-        (loss, grad) with grad an FP64 device tensor (overwritten)."""
+        (loss, grad) with grad an FP64 device tensor (overwritten).  The call
+        returns once the loss is known; the gradient kernels finish on the
+        call's stream, which torch's current stream is made to wait for."""
         import torch
         dev = batch.pos.device
         mode = {"vibrations": 0, "structures": 1, "molecules": 2}.get(weighting, weighting)
@@ -201,12 +224,13 @@ class Sus2Model:
         else:
             s0 = s0.to(device=dev, dtype=torch.float64).contiguous()
         loss = ctypes.c_double()
-        st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        st = _stream_of(stream, dev)
         _lib.check(_lib.lib().sus2_loss_grad(self._h, PRECISION[precision], batch.nframes, batch.natoms,
                                              batch.off.data_ptr(), batch.types.data_ptr(), batch.pos.data_ptr(),
                                              batch.cell.data_ptr(), batch.periodic.data_ptr(), e0.data_ptr(),
                                              f0.data_ptr(), s0.data_ptr(), int(mode), float(wE), float(wF),
                                              float(wS), ctypes.byref(loss), grad.data_ptr(), ctypes.c_void_p(st)))
+        _join(stream, dev)
         return loss.value, grad
 
     def train_lbfgs(self, batch, e0, f0, s0=None, weighting="structures", wE=1.0, wF=0.01, wS=0.0,
@@ -223,7 +247,7 @@ class Sus2Model:
               else s0.to(device=dev, dtype=torch.float64).contiguous())
         trace = np.zeros(max_iter + 1)
         iters = ctypes.c_int()
-        st = stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream
+        st = _stream_of(stream, dev)
         _lib.check(_lib.lib().sus2_train_lbfgs(self._h, PRECISION[precision], batch.nframes, batch.natoms,
                                                batch.off.data_ptr(), batch.types.data_ptr(), batch.pos.data_ptr(),
                                                batch.cell.data_ptr(), batch.periodic.data_ptr(), e0.data_ptr(),

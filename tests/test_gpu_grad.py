@@ -268,3 +268,44 @@ def test_back_to_back_objectives_with_new_coefficients(lib):
             assert loss == loss_q
             assert torch.equal(g, g_q)
         assert queued[0][0] != queued[1][0]
+
+
+def test_back_to_back_objectives_on_growing_batches(lib):
+    """Objectives queued back to back with no coefficient update and no
+    synchronisation in between, alternating a small and a large batch on a
+    fresh handle: the large batch's first call outgrows the edge capacity
+    sized by the small one (device-reported overflow and a rerun) and every
+    workspace buffer grows while earlier gradient kernels are still queued.
+    Each loss and gradient equals, bit for bit, the same call sequence run
+    with a synchronisation after every call."""
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    print("[case] back-to-back objectives on growing batches", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    small = DeviceBatch(W.periodic_frames(6, 8, 30, 2, 0.15, seed=63))
+    large = DeviceBatch(W.periodic_frames(80, 100, 216, 2, 0.15, seed=64))
+    assert large.natoms > 8192
+    teacher = Sus2Model(path)
+    th0 = teacher.coeffs()
+    teacher.set_coeffs(th0 * (1 + 0.05 * np.random.default_rng(65).standard_normal(th0.size)))
+    labels = {id(b): teacher.efs_device(b, "fp64") for b in (small, large)}
+    seq = [small, large, small, large, large]
+    for prec in ("fp32", "fp64", "mixed"):
+        def run(sync):
+            m = Sus2Model(path)
+            out = []
+            for b in seq:
+                lab = labels[id(b)]
+                loss, g = m.loss_grad_device(b, lab["energy"], lab["forces"], precision=prec)
+                out.append((loss, g.clone()))
+                if sync:
+                    torch.cuda.synchronize()
+            torch.cuda.synchronize()
+            return out
+        queued, serial = run(False), run(True)
+        for k, ((lq, gq), (ls, gs)) in enumerate(zip(queued, serial)):
+            print(f"[result] {prec} call {k}: loss {lq:.6e}", flush=True)
+            assert lq == ls
+            assert torch.equal(gq, gs)
+        assert queued[0][0] != queued[1][0]
