@@ -82,6 +82,24 @@ int sus2_efs(sus2_model* m, int precision, int nframes, int natoms, const int64_
              const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
              double* energy, double* force, double* virial, double* site, void* stream);
 
+/* --- allocation-free EFS ------------------------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as sus2_efs, but every scratch buffer (neighbour lists, per-atom and
+ * per-edge state) is carved from one static device array of
+ * SUS2_NOALLOC_SCRATCH_BYTES per GPU instead of the handle's growable
+ * workspace; the call allocates nothing.  A batch whose scratch exceeds the
+ * array fails with SUS2_E_RANGE (the 4096-atom L3/K4 Effect frame fits in
+ * both precisions).  The model's device copy must already exist on the
+ * current device (made by a first sus2_efs call).  The neighbour counts are
+ * read back with plain copies, so the call synchronises `stream`.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams or threads) race and must be serialised by the caller. */
+#define SUS2_NOALLOC_SCRATCH_BYTES (128u << 20)
+int sus2_efs_noalloc(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                     const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+                     double* energy, double* force, double* virial, double* site, void* stream);
+
 /* Same with host buffers (copies in and out; synchronous). */
 int sus2_efs_host(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
                   const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,

@@ -6,6 +6,7 @@
 #include <exception>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -200,6 +201,61 @@ int sus2_efs(sus2_model* m, int precision, int nframes, int natoms, const int64_
     ensure_device(m);
     sus2::EfsOut out{energy, force, virial, site};
     sus2::efs_dispatch(m->dev, m->ws, precision, make_batch(nframes, natoms, frame_off, types, pos, cell, periodic),
+                       out, (hipStream_t)stream);
+  });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ allocation-free EFS
+// Device scratch of sus2_efs_noalloc: one static array per GPU (each device
+// holds its own copy once this module is loaded there).  Host code never
+// passes the array's name to a kernel: the per-device address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[SUS2_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Address of this device's scratch array (set up on the device's first call).
+void* scratch_of(int dev) {
+  if (dev < 0 || dev >= MAX_DEVICES) throw sus2::Error(SUS2_E_DEVICE, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      throw sus2::Error(SUS2_E_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int sus2_efs_noalloc(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                     const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+                     double* energy, double* force, double* virial, double* site, void* stream) {
+  return guard([&] {
+    if (!m) throw sus2::Error(SUS2_E_INVALID, "null model");
+    check_batch(m, nframes, natoms, frame_off, types, pos, cell, periodic);
+    if (nframes == 0) return;
+    if (!energy || !virial || (natoms > 0 && !force)) throw sus2::Error(SUS2_E_INVALID, "null output buffer");
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) throw sus2::Error(SUS2_E_DEVICE, "no HIP device available");
+    if (!m->uploaded || dev != m->device)
+      throw sus2::Error(SUS2_E_INVALID, "sus2_efs_noalloc needs the model's device copy on the current device: "
+                                        "make one sus2_efs call (which may allocate) on this device first");
+    sus2::Arena arena;
+    arena.base = static_cast<char*>(scratch_of(dev));
+    arena.cap = SUS2_NOALLOC_SCRATCH_BYTES;
+    sus2::ArenaScope scope(&arena);
+    sus2::Workspace ws;  // every buffer carved from the scratch array; nothing outlives the call
+    sus2::EfsOut out{energy, force, virial, site};
+    sus2::efs_dispatch(m->dev, ws, precision, make_batch(nframes, natoms, frame_off, types, pos, cell, periodic),
                        out, (hipStream_t)stream);
   });
 }

@@ -1019,6 +1019,15 @@ __global__ __launch_bounds__(256) void k_frame_max(int nframes, const int64_t* _
 // Largest frame of the batch, read by polling the host report (no copy and
 // stream synchronisation on the call's critical path).
 static int64_t largest_frame(Workspace& ws, const Batch& b, hipStream_t st) {
+  if (current_arena()) {  // allocation-free call: a device word and a plain copy
+    ws.scan_tmp.ensure(8 * sizeof(int64_t));
+    int64_t* word = ws.scan_tmp.get<int64_t>() + 7;
+    SUS2_LAUNCH(ws, st, "frame_max", k_frame_max, dim3(1), dim3(256), 0, b.nframes, b.frame_off, word, int64_t(0));
+    int64_t v = 0;
+    SUS2_CHECK(hipMemcpyAsync(&v, word, sizeof(v), hipMemcpyDeviceToHost, st));
+    SUS2_CHECK(hipStreamSynchronize(st));
+    return v & 0x7fffffff;
+  }
   const int64_t seq = ++ws.report_seq;
   SUS2_LAUNCH(ws, st, "frame_max", k_frame_max, dim3(1), dim3(256), 0, b.nframes, b.frame_off, report_word(ws, 2),
               seq);

@@ -19,13 +19,36 @@ constexpr int REPORT_WORDS = 8;  // 64-bit words of the coherent host report
     if (e_ != hipSuccess) throw Error(SUS2_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+namespace {
+thread_local Arena* t_arena = nullptr;
+}
+Arena* current_arena() { return t_arena; }
+ArenaScope::ArenaScope(Arena* a) : prev(t_arena) { t_arena = a; }
+ArenaScope::~ArenaScope() { t_arena = prev; }
+
 DevBuf::~DevBuf() {
-  if (p) (void)hipFree(p);
+  if (p && owned) (void)hipFree(p);
 }
 
 void DevBuf::ensure(size_t n) {
   if (n <= bytes) return;
-  if (p) { RT_CHECK(hipFree(p)); p = nullptr; bytes = 0; }
+  if (Arena* a = t_arena) {  // carve (256-byte aligned) from the arena; nothing is freed
+    const size_t want = ((std::max<size_t>(n, 1) + 255) / 256) * 256;
+    if (want > a->cap - a->used)
+      throw Error(SUS2_E_RANGE, "batch needs more device scratch than the " + std::to_string(a->cap >> 20) +
+                                    " MiB static region of the allocation-free entry point (" +
+                                    std::to_string((a->used + want) >> 20) + " MiB requested so far)");
+    if (p && owned) RT_CHECK(hipFree(p));
+    p = a->base + a->used;
+    a->used += want;
+    bytes = want;
+    owned = false;
+    return;
+  }
+  if (p && owned) RT_CHECK(hipFree(p));
+  p = nullptr;
+  bytes = 0;
+  owned = true;
   const size_t want = std::max<size_t>(n + n / 4, 256);
   RT_CHECK(hipMalloc(&p, want));
   bytes = want;
@@ -88,6 +111,7 @@ Workspace::~Workspace() {
 void Workspace::ensure_atoms(int n) {
   nbr_cnt.ensure(sizeof(int) * (size_t)std::max(n, 1));
   nbr_off.ensure(sizeof(int) * ((size_t)n + 1));
+  if (t_arena) return;  // allocation-free calls read results by plain copies instead of pinned pages
   if (!host_total) RT_CHECK(hipHostMalloc((void**)&host_total, 4 * sizeof(int64_t), hipHostMallocDefault));
   if (!host_report) {
     RT_CHECK(hipHostMalloc((void**)&host_report, REPORT_WORDS * sizeof(int64_t), hipHostMallocCoherent));
@@ -262,10 +286,12 @@ void poll_report(Workspace& ws, int k, int n, int64_t seq, hipStream_t st, const
 long exclusive_scan(Workspace& ws, int n, hipStream_t st, long* maxcnt) {
   ws.scan_tmp.ensure(8 * sizeof(int64_t));
   device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), n, ws.scan_tmp.get<int64_t>(), st);
-  RT_CHECK(hipMemcpyAsync(ws.host_total, ws.scan_tmp.get<int64_t>(), 2 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  int64_t local[2];
+  int64_t* dst = ws.host_total ? ws.host_total : local;
+  RT_CHECK(hipMemcpyAsync(dst, ws.scan_tmp.get<int64_t>(), 2 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
   RT_CHECK(hipStreamSynchronize(st));
-  if (maxcnt) *maxcnt = (long)ws.host_total[1];
-  return (long)ws.host_total[0];
+  if (maxcnt) *maxcnt = (long)dst[1];
+  return (long)dst[0];
 }
 
 namespace {

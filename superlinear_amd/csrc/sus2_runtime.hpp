@@ -17,10 +17,28 @@
 
 namespace sus2 {
 
-// Growable device allocation (never shrinks; freed with its owner).
+// A caller-provided region of device memory that buffers are carved from in
+// order (no allocation): while a thread has an arena installed (ArenaScope),
+// every DevBuf::ensure of that thread takes its bytes from it.
+struct Arena {
+  char* base = nullptr;
+  size_t cap = 0, used = 0;
+};
+Arena* current_arena();
+struct ArenaScope {
+  explicit ArenaScope(Arena* a);
+  ~ArenaScope();
+  ArenaScope(const ArenaScope&) = delete;
+  ArenaScope& operator=(const ArenaScope&) = delete;
+  Arena* prev;
+};
+
+// Growable device allocation (never shrinks; freed with its owner), or a
+// region carved from the installed arena (not freed).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool owned = true;
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
@@ -29,6 +47,7 @@ struct DevBuf {
   void swap(DevBuf& o) {
     std::swap(p, o.p);
     std::swap(bytes, o.bytes);
+    std::swap(owned, o.owned);
   }
   template <class U> U* get() const { return static_cast<U*>(p); }
 };
