@@ -244,9 +244,12 @@ def static_section(steps, warmup, profile, no_cpu):
 
 
 def batch_section(steps, warmup, profile):
-    """Uneven batched EFS and its force-loss gradient: 4096 frames in one call."""
+    """Uneven batched EFS and its force-loss gradient: 4096 frames, the launch
+    shape picked by the autotuner, eager calls and the same work captured in
+    one HIP graph per step."""
+    import torch
     from superlinear_amd import workloads as W
-    from superlinear_amd.flops import graph_stats, step_efs_per_edge
+    from superlinear_amd.flops import graph_stats, step_efs_per_atom, step_efs_per_edge
     from superlinear_amd.model import DeviceBatch, Sus2Model
     path = W.model_path("effect_l4k6r2o3")
     frames = W.config_frames("batch4096")
@@ -254,36 +257,60 @@ def batch_section(steps, warmup, profile):
     batch = DeviceBatch(frames)
     model = Sus2Model(path)
     dims = model.dims()
+    graph = graph_stats(path)
     print(f"[bench] batch4096: {nf} frames, {n} atoms", file=sys.stderr, flush=True)
+    tuned = {}
+    for prec in ("fp32", "fp64"):
+        best, table = model.autotune(batch, prec, reps=3)
+        tuned[prec] = dict(choice=best, median_ms={f"eb{k[0]}_grid{k[1]}": round(v, 3) for k, v in table.items()})
+    print(f"[bench] batch4096 tuned: fp32 {tuned['fp32']['choice']} fp64 {tuned['fp64']['choice']}",
+          file=sys.stderr, flush=True)
     l32 = _call_latencies(lambda: model.efs_device(batch, "fp32"), steps, warmup)
     nedges = model.last_edge_count()
     l64 = _call_latencies(lambda: model.efs_device(batch, "fp64"), steps, warmup)
+    # the same EFS captured once in a HIP graph, one graph launch per step
+    g32 = model.graph_efs(batch, "fp32")
+    lgr = _call_latencies(g32.launch, steps, warmup)
+    assert not g32.overflow()
+    per_step = step_efs_per_edge(dims) * nedges + step_efs_per_atom(dims, graph) * n
     out = dict(workload="batch4096", model="Effect L4/K6/R2/O3 body 4, B=10, 3 species", frames=nf, atoms=n,
-               directed_edges=nedges, atom_steps_per_s=round(n * steps / l32.sum(), 1),
+               directed_edges=nedges, launch_shape=tuned,
+               atom_steps_per_s=round(n * steps / l32.sum(), 1),
                frames_per_s=round(nf * steps / l32.sum(), 1), p50_ms=round(1e3 * float(np.percentile(l32, 50)), 3),
                p99_ms=round(1e3 * float(np.percentile(l32, 99)), 3),
+               graph_atom_steps_per_s=round(n * steps / lgr.sum(), 1),
+               graph_p50_ms=round(1e3 * float(np.percentile(lgr, 50)), 3),
+               graph_p99_ms=round(1e3 * float(np.percentile(lgr, 99)), 3),
                fp64_atom_steps_per_s=round(n * steps / l64.sum(), 1), fp32_over_fp64=round(l64.sum() / l32.sum(), 3),
-               step_formula_tflops=round(step_efs_per_edge(dims) * nedges * steps / l32.sum() / 1e12, 4))
+               step_formula_tflops=round(per_step * steps / l32.sum() / 1e12, 4))
+    del g32
     # force-loss gradient of the whole batch (labels from a perturbed teacher)
     teacher = Sus2Model(path)
     th = teacher.coeffs()
     teacher.set_coeffs(th + 0.05 * np.abs(th) * np.random.default_rng(4041).standard_normal(th.size))
     lab = teacher.efs_device(batch, "fp64")
-    gsteps = max(1, steps // 4)
+    gsteps = max(1, steps // 2)
     lg = _call_latencies(lambda: model.loss_grad_device(batch, lab["energy"], lab["forces"], weighting="structures",
                                                         wE=1.0, wF=0.01, precision="fp32"), gsteps, 1)
     out["grad_frames_per_s"] = round(nf * gsteps / lg.sum(), 1)
     out["grad_ms_per_step"] = round(1e3 * lg.mean(), 3)
+    gl = model.graph_loss_grad(batch, lab["energy"], lab["forces"], weighting="structures", wE=1.0, wF=0.01,
+                               precision="fp32")
+    lgl = _call_latencies(gl.launch, gsteps, 1)
+    assert not gl.overflow()
+    out["graph_grad_frames_per_s"] = round(nf * gsteps / lgl.sum(), 1)
+    out["graph_grad_p50_ms"] = round(1e3 * float(np.percentile(lgl, 50)), 3)
+    out["graph_grad_p99_ms"] = round(1e3 * float(np.percentile(lgl, 99)), 3)
+    del gl
     if profile:
         model.profile_reset()
         model.profile(True)
         model.efs_device(batch, "fp32")
-        import torch
         torch.cuda.synchronize()
         model.profile(False)
         prof = model.profile_read()
         out["kernels_fp32"] = {k: dict(avg_us=round(1e3 * ms / max(c, 1), 2), launches=c) for k, (ms, c) in prof.items()}
-        out["roofline"] = _roofline_of(prof, dims, nedges, n, graph_stats(path), "fp32", "batch4096")
+        out["roofline"] = _roofline_of(prof, dims, nedges, n, graph, "fp32", "batch4096")
     return out
 
 

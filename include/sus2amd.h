@@ -171,6 +171,55 @@ int sus2_pair_add_reverse(sus2_pair* p, int n, const int32_t* atoms, const doubl
 int sus2_pair_finish(sus2_pair* p, double* energy, double* site /* [nowned] */, double* force /* [nall*3] */,
                      double* virial /* [6] */);
 
+/* --- batches captured in a HIP graph ------------------------------------------
+ * One batch's whole EFS (or objective + gradient) call chain captured once
+ * and replayed with one hipGraphLaunch: no host waits inside a launch.  The
+ * device pointers given at creation are baked in: between launches the
+ * caller rewrites positions, cells, species and labels in place (same frame
+ * offsets); sus2_coeff_set updates the coefficients the graph reads.  Creation
+ * runs the batch three times (host-sized, capture-safe, captured) and
+ * synchronises `stream`.  A launch whose neighbour lists outgrow the capacity
+ * sized at creation (atoms moved much closer) empties the lists and sets a
+ * sticky device flag: its outputs are invalid; sus2_graph_overflow waits for
+ * `stream`, returns and clears the flag, and the caller re-creates the graph.
+ * The launch shape (sus2_tune_set / sus2_autotune) in force at creation is
+ * used.  A graph owns its buffers; do not launch one graph on two streams at
+ * once.  No reference counterpart: the reference's batched path is a CPU
+ * loop over frames (dev_src/mlp/mtpr_train.cpp:731-790). */
+typedef struct sus2_graph sus2_graph;
+int sus2_graph_efs_create(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                          const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+                          double* energy, double* force, double* virial, double* site, void* stream,
+                          sus2_graph** out);
+/* loss: one device double (the objective); grad [ncoeff] overwritten; energy,
+ * force, virial (may be NULL): the batch's EFS outputs. */
+int sus2_graph_loss_grad_create(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                                const int32_t* types, const double* pos, const double* cell,
+                                const int32_t* periodic, const double* e0, const double* f0, const double* s0,
+                                int weighting, double wE, double wF, double wS, double* loss, double* grad,
+                                double* energy, double* force, double* virial, void* stream, sus2_graph** out);
+int sus2_graph_launch(sus2_graph* g, void* stream);
+int sus2_graph_overflow(sus2_graph* g, void* stream, int* overflow);
+void sus2_graph_free(sus2_graph* g);
+
+/* --- launch shape of the per-atom EFS kernels and its autotuner -------------
+ * The forward passes, edge backwards and gathers run one workgroup per atom
+ * (edge backward: per group of eb_atoms = 1, 2 or 4 centres) when grid_mult is
+ * 0, else a persistent grid of grid_mult x the resident workgroups striding
+ * over the atoms.  Per handle and precision; results agree to rounding
+ * (eb_atoms changes the order of per-lane partial sums).  The reference picks
+ * its team/vector sizes per kernel at compile time
+ * (interfaces/lammps/ML-SUS2-KK/sus2_sl_kokkos_kernel.h:230,277); here
+ * sus2_autotune times every candidate (SUS2_TUNE_CANDIDATES: eb_atoms in
+ * {1, 2, 4} x grid_mult in {0, 1, 2, 4}, candidate c = (c / 4, c % 4)) on the
+ * caller's batch and keeps the fastest. */
+#define SUS2_TUNE_CANDIDATES 12
+int sus2_tune_set(sus2_model* m, int precision, int eb_atoms, int grid_mult);
+int sus2_tune_get(sus2_model* m, int precision, int* eb_atoms, int* grid_mult);
+int sus2_autotune(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                  const int32_t* types, const double* pos, const double* cell, const int32_t* periodic, int reps,
+                  void* stream, int* best /* [2] eb_atoms, grid_mult */, double* ms /* [12] or NULL */);
+
 /* --- per-kernel timing ------------------------------------------------------
  * Replaces the reference's env-gated wall-clock stage timers
  * (SUS2_SH_ACCUM_PROFILE, dev_src/mtpr_sh.cpp:1855-1957; pair_style

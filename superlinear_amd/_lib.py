@@ -41,6 +41,15 @@ SIGNATURES = {
     "sus2_efs": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sus2_efs_host": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sus2_efs_noalloc": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sus2_graph_efs_create": (_i, [_vp, _i, _i, _i] + [_vp] * 10 + [ctypes.POINTER(_vp)]),
+    "sus2_graph_loss_grad_create": (_i, [_vp, _i, _i, _i] + [_vp] * 8 + [_i, _d, _d, _d] + [_vp] * 6
+                                    + [ctypes.POINTER(_vp)]),
+    "sus2_graph_launch": (_i, [_vp, _vp]),
+    "sus2_graph_overflow": (_i, [_vp, _vp, ctypes.POINTER(_i)]),
+    "sus2_graph_free": (None, [_vp]),
+    "sus2_tune_set": (_i, [_vp, _i, _i, _i]),
+    "sus2_tune_get": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "sus2_autotune": (_i, [_vp, _i, _i, _i] + [_vp] * 5 + [_i, _vp, ctypes.POINTER(_i), _dp]),
     "sus2_efs_grad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sus2_loss_grad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _d, _d, _d, _dp, _vp,
                             _vp]),

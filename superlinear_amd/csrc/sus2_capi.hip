@@ -1,6 +1,7 @@
 // extern "C" boundary of the SUS2-SH MI355X path (declared in include/sus2amd.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -25,6 +26,11 @@ struct sus2_model {
 
 namespace {
 thread_local std::string g_error;
+// Candidate launch shapes of the autotuner: centres per edge-backward group x
+// grid multiple (0 = one workgroup per atom / group).
+constexpr int kTuneEb[3] = {1, 2, 4};
+constexpr int kTuneGrid[4] = {0, 1, 2, 4};
+static_assert(SUS2_TUNE_CANDIDATES == 12, "candidate table");
 
 int guard(const std::function<void()>& f) {
   try {
@@ -421,6 +427,217 @@ int sus2_train_lbfgs(sus2_model* m, int precision, int nframes, int natoms, cons
     std::memcpy(m->model.theta.data(), x.data(), sizeof(double) * P);
     m->dev.upload_theta(m->model);
     if (iters_out) *iters_out = it;
+  });
+}
+
+// ------------------------------------------------------------------ HIP graphs
+}  // extern "C"
+
+struct sus2_graph {
+  sus2_model* model = nullptr;
+  sus2::Workspace ws;  // the graph's own buffers, sized before the capture
+  sus2::DevBuf flag;   // sticky neighbour-overflow flag (int)
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~sus2_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+};
+
+namespace {
+// Three runs of `run` on the graph's private stream: host-sized (eager), then
+// capture-safe (eager: sizes every buffer of that path and checks that the
+// batch fits the edge capacity), then the same calls captured into a graph.
+template <class Fn>
+void build_graph(sus2_graph* g, int nframes, const int64_t* frame_off, hipStream_t user, Fn&& run) {
+  SUS2_CHECK(hipStreamSynchronize(user));  // inputs written on the caller's stream are complete
+  SUS2_CHECK(hipStreamCreate(&g->cap));
+  std::vector<int64_t> off(nframes + 1);
+  SUS2_CHECK(hipMemcpy(off.data(), frame_off, sizeof(int64_t) * (nframes + 1), hipMemcpyDeviceToHost));
+  long mx = 0;
+  for (int f = 0; f < nframes; ++f) mx = std::max<long>(mx, off[f + 1] - off[f]);
+  g->ws.known_max_frame = mx;
+  g->flag.ensure(sizeof(int));
+  SUS2_CHECK(hipMemset(g->flag.p, 0, sizeof(int)));
+  g->ws.overflow_flag = g->flag.get<int>();
+  run(g->cap);
+  SUS2_CHECK(hipStreamSynchronize(g->cap));
+  g->ws.capture = true;
+  run(g->cap);
+  SUS2_CHECK(hipStreamSynchronize(g->cap));
+  int over = 0;
+  SUS2_CHECK(hipMemcpy(&over, g->flag.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (over) throw sus2::Error(SUS2_E_RANGE, "neighbour lists outgrew the capacity sized for the graph");
+  {
+    sus2::CaptureScope scope;
+    SUS2_CHECK(hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal));
+    try {
+      run(g->cap);
+    } catch (...) {
+      hipGraph_t dead = nullptr;
+      (void)hipStreamEndCapture(g->cap, &dead);
+      if (dead) (void)hipGraphDestroy(dead);
+      throw;
+    }
+    SUS2_CHECK(hipStreamEndCapture(g->cap, &g->graph));
+  }
+  SUS2_CHECK(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+}
+}  // namespace
+
+extern "C" {
+
+int sus2_graph_efs_create(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                          const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+                          double* energy, double* force, double* virial, double* site, void* stream,
+                          sus2_graph** out) {
+  return guard([&] {
+    if (!m || !out) throw sus2::Error(SUS2_E_INVALID, "null argument");
+    check_batch(m, nframes, natoms, frame_off, types, pos, cell, periodic);
+    if (nframes == 0) throw sus2::Error(SUS2_E_INVALID, "empty batch");
+    if (!energy || !virial || (natoms > 0 && !force)) throw sus2::Error(SUS2_E_INVALID, "null output buffer");
+    ensure_device(m);
+    std::unique_ptr<sus2_graph> g(new sus2_graph());
+    g->model = m;
+    const sus2::Batch b = make_batch(nframes, natoms, frame_off, types, pos, cell, periodic);
+    const sus2::EfsOut o{energy, force, virial, site};
+    build_graph(g.get(), nframes, frame_off, (hipStream_t)stream,
+                [&](hipStream_t st) { sus2::efs_dispatch(m->dev, g->ws, precision, b, o, st); });
+    *out = g.release();
+  });
+}
+
+int sus2_graph_loss_grad_create(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                                const int32_t* types, const double* pos, const double* cell,
+                                const int32_t* periodic, const double* e0, const double* f0, const double* s0,
+                                int weighting, double wE, double wF, double wS, double* loss, double* grad,
+                                double* energy, double* force, double* virial, void* stream, sus2_graph** out) {
+  return guard([&] {
+    if (!m || !out || !loss || !grad) throw sus2::Error(SUS2_E_INVALID, "null argument");
+    check_batch(m, nframes, natoms, frame_off, types, pos, cell, periodic);
+    if (nframes == 0) throw sus2::Error(SUS2_E_INVALID, "empty batch");
+    if (!e0 || !s0 || (natoms > 0 && !f0)) throw sus2::Error(SUS2_E_INVALID, "null label buffer");
+    ensure_device(m);
+    std::unique_ptr<sus2_graph> g(new sus2_graph());
+    g->model = m;
+    const sus2::Batch b = make_batch(nframes, natoms, frame_off, types, pos, cell, periodic);
+    const sus2::EfsOut o{energy, force, virial, nullptr};
+    build_graph(g.get(), nframes, frame_off, (hipStream_t)stream, [&](hipStream_t st) {
+      sus2::loss_grad_dispatch(m->dev, g->ws, precision, b, e0, f0, s0, weighting, wE, wF, wS, nullptr, grad, st,
+                               loss, &o);
+    });
+    *out = g.release();
+  });
+}
+
+int sus2_graph_launch(sus2_graph* g, void* stream) {
+  return guard([&] {
+    if (!g || !g->exec) throw sus2::Error(SUS2_E_INVALID, "null graph");
+    SUS2_CHECK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  });
+}
+
+int sus2_graph_overflow(sus2_graph* g, void* stream, int* overflow) {
+  return guard([&] {
+    if (!g || !overflow) throw sus2::Error(SUS2_E_INVALID, "null argument");
+    SUS2_CHECK(hipStreamSynchronize((hipStream_t)stream));
+    int v = 0;
+    SUS2_CHECK(hipMemcpy(&v, g->flag.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (v) SUS2_CHECK(hipMemset(g->flag.p, 0, sizeof(int)));
+    *overflow = v;
+  });
+}
+
+void sus2_graph_free(sus2_graph* g) { delete g; }
+
+// ------------------------------------------------------------------ launch shape and autotuner
+namespace {
+sus2::EfsTune& tune_slot(sus2_model* m, int precision) {
+  if (precision != SUS2_FP64 && precision != SUS2_FP32) throw sus2::Error(SUS2_E_INVALID, "precision must be FP64 or FP32");
+  return m->dev.tune[precision == SUS2_FP32 ? 1 : 0];
+}
+}  // namespace
+
+int sus2_tune_set(sus2_model* m, int precision, int eb_atoms, int grid_mult) {
+  return guard([&] {
+    if (!m) throw sus2::Error(SUS2_E_INVALID, "null model");
+    if (eb_atoms != 1 && eb_atoms != 2 && eb_atoms != 4) throw sus2::Error(SUS2_E_INVALID, "eb_atoms must be 1, 2 or 4");
+    if (grid_mult < 0 || grid_mult > 64) throw sus2::Error(SUS2_E_INVALID, "grid_mult must be in [0, 64]");
+    tune_slot(m, precision) = sus2::EfsTune{eb_atoms, grid_mult};
+  });
+}
+
+int sus2_tune_get(sus2_model* m, int precision, int* eb_atoms, int* grid_mult) {
+  return guard([&] {
+    if (!m || !eb_atoms || !grid_mult) throw sus2::Error(SUS2_E_INVALID, "null argument");
+    const sus2::EfsTune& t = tune_slot(m, precision);
+    *eb_atoms = t.eb_atoms;
+    *grid_mult = t.grid_mult;
+  });
+}
+
+// Times the batch EFS under every launch shape of the candidate table
+// (centres per edge-backward group x grid multiple), `reps` calls each after
+// one warm-up call, with HIP events on `stream`, and keeps the fastest
+// (median call time).  ms_out [SUS2_TUNE_CANDIDATES] receives each shape's
+// median (may be NULL); best [2] the chosen eb_atoms, grid_mult.
+int sus2_autotune(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                  const int32_t* types, const double* pos, const double* cell, const int32_t* periodic, int reps,
+                  void* stream, int* best, double* ms_out) {
+  return guard([&] {
+    if (!m || !best) throw sus2::Error(SUS2_E_INVALID, "null argument");
+    check_batch(m, nframes, natoms, frame_off, types, pos, cell, periodic);
+    if (reps < 1) throw sus2::Error(SUS2_E_INVALID, "reps >= 1 required");
+    ensure_device(m);
+    sus2::EfsTune& slot = tune_slot(m, precision);
+    const sus2::EfsTune keep = slot;
+    const hipStream_t st = (hipStream_t)stream;
+    Dev de(sizeof(double) * std::max(1, nframes)), df(sizeof(double) * 3 * std::max(1, natoms)),
+        dv(sizeof(double) * 6 * std::max(1, nframes));
+    const sus2::Batch b = make_batch(nframes, natoms, frame_off, types, pos, cell, periodic);
+    const sus2::EfsOut out{de.get<double>(), df.get<double>(), dv.get<double>(), nullptr};
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+      throw sus2::Error(SUS2_E_DEVICE, "hipEventCreate failed");
+    double best_ms = 1e300;
+    sus2::EfsTune chosen = keep;
+    try {
+      for (int c = 0; c < SUS2_TUNE_CANDIDATES; ++c) {
+        const sus2::EfsTune t{kTuneEb[c / 4], kTuneGrid[c % 4]};
+        slot = t;
+        sus2::efs_dispatch(m->dev, m->ws, precision, b, out, st);  // warm-up (sizes, compiles nothing)
+        std::vector<double> ms(reps);
+        for (int r = 0; r < reps; ++r) {
+          SUS2_CHECK(hipEventRecord(e0, st));
+          sus2::efs_dispatch(m->dev, m->ws, precision, b, out, st);
+          SUS2_CHECK(hipEventRecord(e1, st));
+          SUS2_CHECK(hipEventSynchronize(e1));
+          float v = 0;
+          SUS2_CHECK(hipEventElapsedTime(&v, e0, e1));
+          ms[r] = v;
+        }
+        std::sort(ms.begin(), ms.end());
+        const double med = ms[reps / 2];
+        if (ms_out) ms_out[c] = med;
+        if (med < best_ms) {
+          best_ms = med;
+          chosen = t;
+        }
+      }
+    } catch (...) {
+      slot = keep;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    slot = chosen;
+    best[0] = chosen.eb_atoms;
+    best[1] = chosen.grid_mult;
   });
 }
 

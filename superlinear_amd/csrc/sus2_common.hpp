@@ -61,7 +61,7 @@ __device__ __forceinline__ int frame_of(const int64_t* off, int nframes, int i) 
 #define SUS2_CHECK(x)                                                                                 \
   do {                                                                                                \
     hipError_t e_ = (x);                                                                              \
-    if (e_ != hipSuccess) throw Error(SUS2_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    if (e_ != hipSuccess) throw ::sus2::Error(SUS2_E_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
 // Launch with per-kernel event timing when the workspace profiler is on.

@@ -1064,7 +1064,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   bool per_frame = N <= FRAME_CELLS_MAX;
   ws.max_frame_atoms = N;
   if (!per_frame && b.nframes > 0) {
-    const int64_t mx = largest_frame(ws, b, st);
+    const int64_t mx = ws.known_max_frame >= 0 ? ws.known_max_frame : largest_frame(ws, b, st);
     ws.max_frame_atoms = mx;
     per_frame = mx <= FRAME_CELLS_MAX;
   }
@@ -1106,14 +1106,22 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   // edge capacity left by earlier builds, in records of T
   const long ecap = (long)std::min({ws.rec.bytes / sizeof(Rec<T>), ws.in_edges.bytes / sizeof(int),
                                     ws.in_pos.bytes / sizeof(int), (size_t)INT32_MAX - 1});
-  if (ws.nbr_optimistic && ws.nbr_known && ecap > 0) {
+  if (ws.capture && !(ws.nbr_known && ecap > 0))
+    throw Error(SUS2_E_INVALID, "capture-safe neighbour build needs the edge capacity of an earlier call");
+  if ((ws.nbr_optimistic || ws.capture) && ws.nbr_known && ecap > 0) {
     // no host round trip: the scan reports (and empties every list) when the
     // counts exceed the slots or the edge buffers; neighbors_verify reads the
-    // report once the call's work has been queued
-    device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), N, ws.scan_tmp.get<int64_t>(), st, cap, ecap,
-                report_word(ws, 0), ws.scan_seq = ++ws.report_seq);
+    // report once the call's work has been queued.  Capture-safe calls set the
+    // sticky device flag instead and nothing is read back.
+    if (ws.capture) {
+      device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), N, ws.scan_tmp.get<int64_t>(), st, cap, ecap, nullptr,
+                  0, ws.overflow_flag);
+    } else {
+      device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), N, ws.scan_tmp.get<int64_t>(), st, cap, ecap,
+                  report_word(ws, 0), ws.scan_seq = ++ws.report_seq);
+      ws.nbr_pending = true;
+    }
     ws.nedges = ecap;
-    ws.nbr_pending = true;
     if (N > 0) compact_rev<T>(ws, b, rc, st);
     return;
   }

@@ -219,27 +219,23 @@ size_t forward_lds(const DModel<typename Cf::T>& m) {
   return sizeof(typename Cf::T) * (species_tables_size<Cf>(m.C) + forward_region<Cf>(m.M, sc));
 }
 
+// One centre atom i of a forward pass (the workgroup's LDS: species tables
+// already loaded into tab, then `region`).
 template <class Cf, bool GATE, class T = typename Cf::T>
-__global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_t* __restrict__ types,
-                                                       const int* __restrict__ nbr_off,
-                                                       const Rec<T>* __restrict__ rec, T* __restrict__ h,
-                                                       T* __restrict__ vg, T* __restrict__ site,
-                                                       T* __restrict__ abar) {
+__device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const SpeciesTables<T>& tab, T* region, int i,
+                                                  const int32_t* __restrict__ types,
+                                                  const int* __restrict__ nbr_off, const Rec<T>* __restrict__ rec,
+                                                  T* __restrict__ h, T* __restrict__ vg, T* __restrict__ site,
+                                                  T* __restrict__ abar) {
   constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, H = Cf::H, F = Cf::F, KH = Cf::KH, NM = Cf::NM;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* tabs_base = reinterpret_cast<T*>(smem_raw);
-  T* region = tabs_base + species_tables_size<Cf>(m.C);
   T* sR = region;                          // [F][LR]  edge phase
   T* sY = sR + F * LR;                     // [H][LR]
   T* mom = region;                         // [KH]     after the edge phase
   T* v = mom + KH;                         // [M]
   T* a = v + m.M;                          // [M]
   T* scratch = a + m.M;
-  const int i = blockIdx.x, lane = threadIdx.x;
+  const int lane = threadIdx.x;
   const int ti = types[i];
-  SpeciesTables<T> tab;
-  load_species_tables<Cf>(m, GATE, tabs_base, tab);
-  __syncthreads();
   const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
   T acc[NM];
 #pragma unroll
@@ -371,6 +367,27 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
   wave_adjoint_to_basic<T, KK>(m, p, false, a, abar + (size_t)i * KH);
 }
 
+// Forward pass over atoms [0, natoms): one atom per workgroup when the grid
+// covers them all, else a persistent grid striding over the atoms (the
+// species tables are loaded once per workgroup).
+template <class Cf, bool GATE, class T = typename Cf::T>
+__global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, int natoms, const int32_t* __restrict__ types,
+                                                       const int* __restrict__ nbr_off,
+                                                       const Rec<T>* __restrict__ rec, T* __restrict__ h,
+                                                       T* __restrict__ vg, T* __restrict__ site,
+                                                       T* __restrict__ abar) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* tabs_base = reinterpret_cast<T*>(smem_raw);
+  T* region = tabs_base + species_tables_size<Cf>(m.C);
+  SpeciesTables<T> tab;
+  load_species_tables<Cf>(m, GATE, tabs_base, tab);
+  __syncthreads();
+  for (int i = blockIdx.x; i < natoms; i += gridDim.x) {
+    pass_forward_atom<Cf, GATE>(m, tab, region, i, types, nbr_off, rec, h, vg, site, abar);
+    __syncthreads();  // the next atom overwrites the region
+  }
+}
+
 // ------------------------------------------------------------------ edge backward
 // One directed edge e = i -> j of the centre's own energy:
 //   g_e = sum_b abar_i[b] w_mu (R'_mu u Y_b + R_mu grad Y_b),
@@ -380,15 +397,16 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, const int32_
 // outer pass of an Effect model the gate-adjoint message
 //   u_e,mu = tau_{t_i,mu} R_mu(r; t_i, t_j) sum_m abar_i[k,l,m] Y_lm(d_e),
 // which the neighbour j gathers into h_bar_j (times alpha_{t_j,mu}).
-// Centres per edge-backward workgroup: their edge lists are contiguous, so
-// two atoms' ~2 x 90 edges fill the lanes better than one atom's ~90.
-constexpr int EB_ATOMS = 2;
 
 // FP32, L <= 3: at most 168 VGPRs, so three waves share each SIMD.
 template <class Cf, bool GATE>
 constexpr int edge_backward_waves() { return (sizeof(typename Cf::T) == 4 && Cf::LL <= 3) ? 3 : 1; }
 
-template <class Cf, bool GATE, class T = typename Cf::T>
+// NA centres per group (their edge lists are contiguous, so two or four
+// atoms' edges fill the lanes better than one atom's ~90); one group per
+// workgroup when the grid covers them all, else a persistent grid striding
+// over the groups.
+template <class Cf, bool GATE, int NA, class T = typename Cf::T>
 __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edge_backward(DModel<T> m, int natoms,
                                                         const int32_t* __restrict__ types,
                                                         const int* __restrict__ nbr_off,
@@ -398,17 +416,17 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
                                                         T* __restrict__ fw) {
   constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, F = Cf::F, KH = Cf::KH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  constexpr int NA = EB_ATOMS;
   T* sA = reinterpret_cast<T*>(smem_raw);  // [NA][KH]
   // [NA][9][LR] per-lane partials of each centre: outgoing force sum, virial
   // (padded rows, so that the row sums at the end read distinct banks)
   T* sV = sA + NA * KH + species_tables_size<Cf>(m.C);
   T* sXR = sV + NA * 9 * LR;  // [F][LR] lane-private columns: x0 R (gate message)
   SpeciesTables<T> tab;
-  const int i0 = blockIdx.x * NA, lane = threadIdx.x;
-  const int na = min(NA, natoms - i0);
+  const int lane = threadIdx.x;
   const bool hgate = !GATE && m.gated;
   load_species_tables<Cf>(m, GATE, sA + NA * KH, tab);
+  for (int i0 = blockIdx.x * NA; i0 < natoms; i0 += gridDim.x * NA) {
+  const int na = min(NA, natoms - i0);
   for (int c = lane; c < na * KH; c += WAVE) sA[c] = adj[(size_t)i0 * KH + c];
 #pragma unroll
   for (int c = 0; c < NA * 9; ++c) sV[c * LR + lane] = T(0);
@@ -521,6 +539,8 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
     T* o = fw + (size_t)(i0 + lane / 9) * 9 + lane % 9;
     *o = GATE ? *o + tot : tot;
   }
+  __syncthreads();  // the next group overwrites sA and sV
+  }
 }
 
 // ------------------------------------------------------------------ gather
@@ -562,13 +582,12 @@ __device__ void gate_reverse_atom(const DModel<T>& m, int i, const T* __restrict
 // h_bar for a later reverse (the pair-style path, whose ghost adjoints are
 // communicated in between).  Ghost atoms (a >= nowned) start from a zeroed row.
 template <class Cf, bool GATE, class T = typename Cf::T>
-__global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __restrict__ types,
-                                                 const int* __restrict__ in_off, const T* __restrict__ gedge_in,
-                                                 const T* __restrict__ uedge, const T* __restrict__ vg,
-                                                 T* __restrict__ gbar, T* __restrict__ hbar_out,
-                                                 T* __restrict__ fw) {
+__device__ __forceinline__ void gather_atom(const DModel<T>& m, int i, const int32_t* __restrict__ types,
+                                            const int* __restrict__ in_off, const T* __restrict__ gedge_in,
+                                            const T* __restrict__ uedge, const T* __restrict__ vg,
+                                            T* __restrict__ gbar, T* __restrict__ hbar_out, T* __restrict__ fw) {
   constexpr int F = Cf::F;
-  const int i = blockIdx.x, lane = threadIdx.x;
+  const int lane = threadIdx.x;
   const bool hgate = !GATE && m.gated;
   T fx = T(0), fy = T(0), fz = T(0);
   T hb[F];
@@ -674,6 +693,20 @@ __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, const int32_t* __r
   gate_reverse_atom<Cf>(m, i, vg, gbar, hbs, v, a, scratch);
 }
 
+// Gather over atoms [0, natoms): one atom per workgroup, or a persistent grid
+// striding over the atoms.
+template <class Cf, bool GATE, class T = typename Cf::T>
+__global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, int natoms, const int32_t* __restrict__ types,
+                                                 const int* __restrict__ in_off, const T* __restrict__ gedge_in,
+                                                 const T* __restrict__ uedge, const T* __restrict__ vg,
+                                                 T* __restrict__ gbar, T* __restrict__ hbar_out,
+                                                 T* __restrict__ fw) {
+  for (int i = blockIdx.x; i < natoms; i += gridDim.x) {
+    gather_atom<Cf, GATE>(m, i, types, in_off, gedge_in, uedge, vg, gbar, hbar_out, fw);
+    __syncthreads();  // the next atom reuses the LDS
+  }
+}
+
 // Gate reverse of owned atoms from stored h_bar (pair-style path).
 template <class Cf, class T = typename Cf::T>
 __global__ __launch_bounds__(WAVE) void k_gate_reverse(DModel<T> m, const T* __restrict__ hbar,
@@ -698,9 +731,9 @@ size_t gather_lds(const DModel<typename Cf::T>& m) {
 }
 
 template <class Cf>
-size_t backward_lds(const DModel<typename Cf::T>& m) {
+size_t backward_lds(const DModel<typename Cf::T>& m, int na) {
   return sizeof(typename Cf::T) *
-         ((size_t)EB_ATOMS * (Cf::KH + 9 * LR) + species_tables_size<Cf>(m.C) + (m.gated ? (size_t)Cf::F * LR : 0));
+         ((size_t)na * (Cf::KH + 9 * LR) + species_tables_size<Cf>(m.C) + (m.gated ? (size_t)Cf::F * LR : 0));
 }
 
 // ------------------------------------------------------------------ driver
@@ -722,6 +755,91 @@ void allow_lds(K kern, size_t bytes) {
                                    (int)bytes));
 }
 
+// Workgroups of one wavefront of `kern` that fit on the whole device at once.
+// With whole_simds the count per CU is rounded down to a multiple of its four
+// SIMDs: grid-stride loops give every group the same share of atoms, and for
+// an issue-bound kernel a fifth wave on a CU shares a SIMD with another and
+// finishes last.
+template <class K>
+int resident_groups(K kern, size_t lds, bool whole_simds) {
+  constexpr int SIMDS = 4;
+  allow_lds(kern, lds);
+  int per_cu = 0, dev = 0, cus = 0;
+  SUS2_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), WAVE, lds));
+  SUS2_CHECK(hipGetDevice(&dev));
+  SUS2_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (whole_simds && per_cu > SIMDS) per_cu -= per_cu % SIMDS;
+  return std::max(1, per_cu) * std::max(1, cus);
+}
+
+// Grid of a per-atom (per-group) kernel over `units` work items under the
+// launch shape: one workgroup per item, or grid_mult times the resident
+// workgroups striding over them (persistent).
+template <class K>
+unsigned tuned_grid(const EfsTune& t, K kern, size_t lds, long units) {
+  if (units <= 0) return 0;
+  if (t.grid_mult <= 0) return (unsigned)units;
+  return (unsigned)std::min<long>(units, (long)t.grid_mult * resident_groups(kern, lds, false));
+}
+
+template <class Cf, bool GATE>
+void launch_pass_forward(const DevModelSet& dm, Workspace& ws, int n, const int32_t* types, const int* off,
+                         const Rec<typename Cf::T>* rec, typename Cf::T* h, typename Cf::T* vg, typename Cf::T* site,
+                         typename Cf::T* abar, hipStream_t st) {
+  using T = typename Cf::T;
+  const DModel<T>& m = dm.get<T>();
+  const size_t lf = forward_lds<Cf>(m);
+  allow_lds(k_pass_forward<Cf, GATE>, lf);
+  const unsigned g = tuned_grid(dm.tune_of<T>(), k_pass_forward<Cf, GATE>, lf, n);
+  if (g == 0) return;
+  SUS2_LAUNCH(ws, st, kname<Cf>(GATE ? "pass_forward_gate" : "pass_forward_outer"), (k_pass_forward<Cf, GATE>),
+              dim3(g), dim3(WAVE), lf, m, n, types, off, rec, h, vg, site, abar);
+}
+
+template <class Cf, bool GATE, int NA>
+void launch_edge_backward_na(const DevModelSet& dm, Workspace& ws, int n, const int32_t* types, const int* off,
+                             const Rec<typename Cf::T>* rec, const typename Cf::T* h, const typename Cf::T* adj,
+                             const int* in_pos, typename Cf::T* gedge_in, typename Cf::T* uedge,
+                             typename Cf::T* fw, hipStream_t st) {
+  using T = typename Cf::T;
+  const DModel<T>& m = dm.get<T>();
+  const size_t lb = backward_lds<Cf>(m, NA);
+  allow_lds(k_edge_backward<Cf, GATE, NA>, lb);
+  const unsigned g = tuned_grid(dm.tune_of<T>(), k_edge_backward<Cf, GATE, NA>, lb, (n + NA - 1) / NA);
+  if (g == 0) return;
+  SUS2_LAUNCH(ws, st, kname<Cf>(GATE ? "edge_backward_gate" : "edge_backward_outer"), (k_edge_backward<Cf, GATE, NA>),
+              dim3(g), dim3(WAVE), lb, m, n, types, off, rec, h, adj, in_pos, gedge_in, uedge, fw);
+}
+
+// Edge backward with the tuned centres per group (1, 2 or 4).
+template <class Cf, bool GATE>
+void launch_edge_backward(const DevModelSet& dm, Workspace& ws, int n, const int32_t* types, const int* off,
+                          const Rec<typename Cf::T>* rec, const typename Cf::T* h, const typename Cf::T* adj,
+                          const int* in_pos, typename Cf::T* gedge_in, typename Cf::T* uedge, typename Cf::T* fw,
+                          hipStream_t st) {
+  switch (dm.tune_of<typename Cf::T>().eb_atoms) {
+    case 1: return launch_edge_backward_na<Cf, GATE, 1>(dm, ws, n, types, off, rec, h, adj, in_pos, gedge_in, uedge, fw, st);
+    case 4: return launch_edge_backward_na<Cf, GATE, 4>(dm, ws, n, types, off, rec, h, adj, in_pos, gedge_in, uedge, fw, st);
+    default: return launch_edge_backward_na<Cf, GATE, 2>(dm, ws, n, types, off, rec, h, adj, in_pos, gedge_in, uedge, fw, st);
+  }
+}
+
+template <class Cf, bool GATE>
+void launch_gather(const DevModelSet& dm, Workspace& ws, int nall, const int32_t* types, const int* in_off,
+                   const typename Cf::T* gedge_in, const typename Cf::T* uedge, const typename Cf::T* vg,
+                   typename Cf::T* gbar, typename Cf::T* hbar_out, typename Cf::T* fw, hipStream_t st) {
+  using T = typename Cf::T;
+  const DModel<T>& m = dm.get<T>();
+  // gated outer gathers with F >= 16 sum their messages through LDS; the gate
+  // graph reverse of the outer gather needs its walk LDS
+  const size_t lg = (!GATE && m.gated) ? gather_lds<Cf>(m) : 0;
+  allow_lds(k_gather<Cf, GATE>, lg);
+  const unsigned g = tuned_grid(dm.tune_of<T>(), k_gather<Cf, GATE>, lg, nall);
+  if (g == 0) return;
+  SUS2_LAUNCH(ws, st, kname<Cf>(GATE ? "gather_gate" : "gather_outer"), (k_gather<Cf, GATE>), dim3(g), dim3(WAVE),
+              lg, m, nall, types, in_off, gedge_in, uedge, vg, gbar, hbar_out, fw);
+}
+
 template <class Cf>
 void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st) {
   using T = typename Cf::T;
@@ -741,29 +859,20 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
   const Rec<T>* rec = ws.rec.get<Rec<T>>();
   T* h = ws.h.get<T>();
   if (N > 0) {
-    const size_t lf = forward_lds<Cf>(m), lg = gather_lds<Cf>(m), lb = backward_lds<Cf>(m);
-    if (m.gated) {
-      allow_lds(k_pass_forward<Cf, true>, lf);
-      SUS2_LAUNCH(ws, st, kname<Cf>("pass_forward_gate"), (k_pass_forward<Cf, true>), dim3(N), dim3(WAVE), lf, m,
-                  b.types, off, rec, h, ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
-    }
-    allow_lds(k_pass_forward<Cf, false>, lf);
-    SUS2_LAUNCH(ws, st, kname<Cf>("pass_forward_outer"), (k_pass_forward<Cf, false>), dim3(N), dim3(WAVE), lf, m,
-                b.types, off, rec, h, ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
+    T* vg = ws.vg.get<T>();
+    T* abar = ws.abar.get<T>();
+    T* gbar = ws.gbar.get<T>();
+    T* fw = ws.fw.get<T>();
+    T* gi = ws.gedge_in.get<T>();
+    T* ue = ws.uedge.get<T>();
     const int* ip = ws.in_pos.get<int>();
-    SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_outer"), (k_edge_backward<Cf, false>), dim3((N + EB_ATOMS - 1) / EB_ATOMS), dim3(WAVE), lb, m, N,
-                b.types, off, rec, h, ws.abar.get<T>(), ip, ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
-    allow_lds(k_gather<Cf, false>, lg);
-    SUS2_LAUNCH(ws, st, kname<Cf>("gather_outer"), (k_gather<Cf, false>), dim3(N), dim3(WAVE), m.gated ? lg : 0, m,
-                b.types, off, ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(),
-                ws.gbar.get<T>(), (T*)nullptr, ws.fw.get<T>());
+    if (m.gated) launch_pass_forward<Cf, true>(dm, ws, N, b.types, off, rec, h, vg, ws.site.get<T>(), abar, st);
+    launch_pass_forward<Cf, false>(dm, ws, N, b.types, off, rec, h, vg, ws.site.get<T>(), abar, st);
+    launch_edge_backward<Cf, false>(dm, ws, N, b.types, off, rec, h, abar, ip, gi, ue, fw, st);
+    launch_gather<Cf, false>(dm, ws, N, b.types, off, gi, ue, vg, gbar, (T*)nullptr, fw, st);
     if (m.gated) {
-      SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_gate"), (k_edge_backward<Cf, true>), dim3((N + EB_ATOMS - 1) / EB_ATOMS), dim3(WAVE), lb, m, N,
-                  b.types, off, rec, h, ws.gbar.get<T>(), ip, ws.gedge_in.get<T>(), ws.uedge.get<T>(),
-                  ws.fw.get<T>());
-      SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(N), dim3(WAVE), 0, m, b.types, off,
-                  ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>(), (T*)nullptr,
-                  ws.fw.get<T>());
+      launch_edge_backward<Cf, true>(dm, ws, N, b.types, off, rec, h, gbar, ip, gi, ue, fw, st);
+      launch_gather<Cf, true>(dm, ws, N, b.types, off, gi, ue, vg, gbar, (T*)nullptr, fw, st);
     }
   }
   finish_efs<T>(ws, b, out, SiteConst{dm.d.view.theta, m.o_species, m.o_shift, m.C}, st);
@@ -783,7 +892,6 @@ void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const Pai
   const size_t a1 = std::max(1, all), ne = std::max<long>(1, ws.nedges);
   const int* off = ws.nbr_off.get<int>();
   const Rec<T>* rec = ws.rec.get<Rec<T>>();
-  const size_t lf = forward_lds<Cf>(m), lg = gather_lds<Cf>(m), lb = backward_lds<Cf>(m);
   if (stage == 0) {  // begin: gate pass over the owned atoms; ghost gate signals arrive by forward comm
     ws.h.ensure(sizeof(T) * a1 * Cf::F);
     ws.hbar.ensure(sizeof(T) * a1 * Cf::F);
@@ -796,45 +904,33 @@ void pair_stage_shape(int stage, const DevModelSet& dm, Workspace& ws, const Pai
     if (m.gated) ws.uedge.ensure(sizeof(T) * ne * Cf::F);
     if (m.gated && all > 0) {
       fill_zero(ws, ws.h.get<T>(), (size_t)all * Cf::F, st);
-      if (n > 0) {
-        allow_lds(k_pass_forward<Cf, true>, lf);
-        SUS2_LAUNCH(ws, st, kname<Cf>("pass_forward_gate"), (k_pass_forward<Cf, true>), dim3(n), dim3(WAVE), lf, m,
-                    pv.types, off, rec, ws.h.get<T>(), ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
-      }
+      launch_pass_forward<Cf, true>(dm, ws, n, pv.types, off, rec, ws.h.get<T>(), ws.vg.get<T>(), ws.site.get<T>(),
+                                    ws.abar.get<T>(), st);
     }
     return;
   }
   if (stage == 1) {  // outer: outer pass, its coordinate reverse, forces and gate adjoints of all atoms
     if (all > 0) fill_zero(ws, ws.fw.get<T>(), (size_t)all * 9, st);
-    if (n > 0) {
-      allow_lds(k_pass_forward<Cf, false>, lf);
-      SUS2_LAUNCH(ws, st, kname<Cf>("pass_forward_outer"), (k_pass_forward<Cf, false>), dim3(n), dim3(WAVE), lf, m,
-                  pv.types, off, rec, ws.h.get<T>(), ws.vg.get<T>(), ws.site.get<T>(), ws.abar.get<T>());
-      SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_outer"), (k_edge_backward<Cf, false>), dim3((n + EB_ATOMS - 1) / EB_ATOMS), dim3(WAVE),
-                  lb, m, n, pv.types, off, rec, ws.h.get<T>(), ws.abar.get<T>(), ws.in_pos.get<int>(),
-                  ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>());
-    }
-    if (all > 0) {
-      allow_lds(k_gather<Cf, false>, lg);  // (gated shapes with F >= 16 sum their messages through LDS)
-      SUS2_LAUNCH(ws, st, kname<Cf>("gather_outer"), (k_gather<Cf, false>), dim3(all), dim3(WAVE),
-                  (Cf::F >= 16 && m.gated) ? lg : 0, m, pv.types, pv.in_off, ws.gedge_in.get<T>(),
-                  ws.uedge.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>(), m.gated ? ws.hbar.get<T>() : (T*)nullptr,
-                  ws.fw.get<T>());
-    }
+    launch_pass_forward<Cf, false>(dm, ws, n, pv.types, off, rec, ws.h.get<T>(), ws.vg.get<T>(), ws.site.get<T>(),
+                                   ws.abar.get<T>(), st);
+    launch_edge_backward<Cf, false>(dm, ws, n, pv.types, off, rec, ws.h.get<T>(), ws.abar.get<T>(),
+                                    ws.in_pos.get<int>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>(), st);
+    launch_gather<Cf, false>(dm, ws, all, pv.types, pv.in_off, ws.gedge_in.get<T>(), ws.uedge.get<T>(),
+                             ws.vg.get<T>(), ws.gbar.get<T>(), m.gated ? ws.hbar.get<T>() : (T*)nullptr,
+                             ws.fw.get<T>(), st);
     return;
   }
   // finish: gate reverse of the owned atoms (their adjoints now include the
   // ghosts' contributions), gate coordinate reverse, forces, reductions
   if (m.gated && n > 0) {
+    const size_t lg = gather_lds<Cf>(m);
     allow_lds(k_gate_reverse<Cf>, lg);
     SUS2_LAUNCH(ws, st, kname<Cf>("gate_reverse"), (k_gate_reverse<Cf>), dim3(n), dim3(WAVE), lg, m,
                 ws.hbar.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>());
-    SUS2_LAUNCH(ws, st, kname<Cf>("edge_backward_gate"), (k_edge_backward<Cf, true>), dim3((n + EB_ATOMS - 1) / EB_ATOMS), dim3(WAVE), lb, m, n,
-                pv.types, off, rec, ws.h.get<T>(), ws.gbar.get<T>(), ws.in_pos.get<int>(), ws.gedge_in.get<T>(),
-                ws.uedge.get<T>(), ws.fw.get<T>());
-    SUS2_LAUNCH(ws, st, kname<Cf>("gather_gate"), (k_gather<Cf, true>), dim3(all), dim3(WAVE), 0, m, pv.types,
-                pv.in_off, ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.vg.get<T>(), ws.gbar.get<T>(), (T*)nullptr,
-                ws.fw.get<T>());
+    launch_edge_backward<Cf, true>(dm, ws, n, pv.types, off, rec, ws.h.get<T>(), ws.gbar.get<T>(),
+                                   ws.in_pos.get<int>(), ws.gedge_in.get<T>(), ws.uedge.get<T>(), ws.fw.get<T>(), st);
+    launch_gather<Cf, true>(dm, ws, all, pv.types, pv.in_off, ws.gedge_in.get<T>(), ws.uedge.get<T>(),
+                            ws.vg.get<T>(), ws.gbar.get<T>(), (T*)nullptr, ws.fw.get<T>(), st);
   }
   Batch b;
   b.nframes = 1;

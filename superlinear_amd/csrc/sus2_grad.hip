@@ -144,7 +144,7 @@ __global__ void k_rec_narrow(long n, const Rec<double>* __restrict__ in, Rec<flo
 
 void loss_grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* e0,
                         const double* f0, const double* s0, int mode, double wE, double wF, double wS,
-                        double* loss_host, double* grad, hipStream_t st) {
+                        double* loss_host, double* grad, hipStream_t st, double* loss_dev, const EfsOut* efs_out) {
   if (mode < 0 || mode > 2) throw Error(SUS2_E_INVALID, "weighting must be 0 (vibrations), 1 (structures) or 2 (molecules)");
   if (precision != SUS2_FP64 && precision != SUS2_FP32 && precision != SUS2_MIXED)
     throw Error(SUS2_E_INVALID, "precision must be SUS2_FP64, SUS2_FP32 or SUS2_MIXED");
@@ -156,7 +156,8 @@ void loss_grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, con
   if (nf == 0) {  // empty batch: zero objective and gradient
     const int P = dm.host->ncoeff;
     SUS2_LAUNCH(ws, st, "fill", k_fill, dim3((P + 255) / 256), dim3(256), 0, P, 0.0, grad);
-    *loss_host = 0.0;
+    if (loss_dev) SUS2_LAUNCH(ws, st, "fill", k_fill, dim3(1), dim3(256), 0, 1, 0.0, loss_dev);
+    if (loss_host) *loss_host = 0.0;
     return;
   }
   const size_t n1 = std::max(1, N), f1 = std::max(1, nf);
@@ -168,17 +169,22 @@ void loss_grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, con
   ws.lsw.ensure(sizeof(double) * 9 * f1);
   ws.lloss.ensure(sizeof(double) * (f1 + 1));
   EfsOut out;
-  out.energy = ws.lE.get<double>();
-  out.force = ws.lF.get<double>();
-  out.virial = ws.lV.get<double>();
+  out.energy = efs_out && efs_out->energy ? efs_out->energy : ws.lE.get<double>();
+  out.force = efs_out && efs_out->force ? efs_out->force : ws.lF.get<double>();
+  out.virial = efs_out && efs_out->virial ? efs_out->virial : ws.lV.get<double>();
+  out.site = efs_out ? efs_out->site : nullptr;
   efs_dispatch(dm, ws, efs_prec, b, out, st);
   double* lf = ws.lloss.get<double>();
   SUS2_LAUNCH(ws, st, "loss_frames", k_loss_frames, dim3(f1), dim3(256), 0, nf, b.frame_off, out.energy, out.force,
               out.virial, e0, f0, s0, mode, wE, wF, wS, ws.lew.get<double>(), ws.lfw.get<double>(),
               ws.lsw.get<double>(), lf + 1);
-  const int64_t seq = ++ws.report_seq;
-  SUS2_LAUNCH(ws, st, "loss_sum", k_loss_sum, dim3(1), dim3(256), 0, nf, lf + 1, lf, report_word(ws, 4), seq);
-  if (precision == SUS2_MIXED && ws.nedges > 0) {  // the neighbour list of the FP64 EFS, narrowed
+  // capture-safe calls write the objective to the caller's device word only
+  const bool polled = !ws.capture && loss_host;
+  const int64_t seq = polled ? ++ws.report_seq : 0;
+  SUS2_LAUNCH(ws, st, "loss_sum", k_loss_sum, dim3(1), dim3(256), 0, nf, lf + 1, loss_dev ? loss_dev : lf,
+              polled ? report_word(ws, 4) : nullptr, seq);
+  const bool narrow = precision == SUS2_MIXED && ws.nedges > 0;
+  if (narrow) {  // the neighbour list of the FP64 EFS, narrowed for the FP32 sweeps
     ws.rec_alt.ensure(sizeof(Rec<float>) * (size_t)ws.nedges);
     SUS2_LAUNCH(ws, st, "rec_narrow", k_rec_narrow, dim3((unsigned)((ws.nedges + 255) / 256)), dim3(256), 0,
                 ws.nedges, ws.rec.get<Rec<double>>(), ws.rec_alt.get<Rec<float>>());
@@ -186,6 +192,8 @@ void loss_grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, con
   }
   grad_dispatch(dm, ws, grad_prec, b, ws.lew.get<double>(), ws.lfw.get<double>(), ws.lsw.get<double>(), grad,
                 1.0 / nf, false, true, st);
+  if (narrow) ws.rec.swap(ws.rec_alt);  // the FP64 records stay in place for the next build
+  if (!polled) return;
   // the objective is known once the EFS and the loss kernels have run: the
   // call returns then, and the gradient kernels finish on the stream
   uint64_t w[2];

@@ -969,23 +969,6 @@ static __global__ __launch_bounds__(256) void k_grad_reduce(int ncoeff, const do
   grad[p] = (add ? grad[p] : 0.0) + scale * t;
 }
 
-// Workgroups of one wavefront that fit on the whole device at once.  With
-// whole_simds the count per CU is rounded down to a multiple of its four
-// SIMDs: the grid-stride loops give every group the same share of atoms, and
-// for an issue-bound kernel a fifth wave on a CU shares a SIMD with another
-// and finishes last.
-template <class K>
-int resident_groups(K kern, size_t lds, bool whole_simds) {
-  constexpr int SIMDS = 4;
-  allow_lds(kern, lds);
-  int per_cu = 0, dev = 0, cus = 0;
-  SUS2_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), WAVE, lds));
-  SUS2_CHECK(hipGetDevice(&dev));
-  SUS2_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  if (whole_simds && per_cu > SIMDS) per_cu -= per_cu % SIMDS;
-  return std::max(1, per_cu) * std::max(1, cus);
-}
-
 // ------------------------------------------------------------------ driver
 template <class Cf>
 void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const GradWeights& gw_in, double* grad,

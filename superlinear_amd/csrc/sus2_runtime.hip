@@ -21,8 +21,12 @@ constexpr int REPORT_WORDS = 8;  // 64-bit words of the coherent host report
 
 namespace {
 thread_local Arena* t_arena = nullptr;
+thread_local bool t_capturing = false;
 }
 Arena* current_arena() { return t_arena; }
+bool capturing() { return t_capturing; }
+CaptureScope::CaptureScope() { t_capturing = true; }
+CaptureScope::~CaptureScope() { t_capturing = false; }
 ArenaScope::ArenaScope(Arena* a) : prev(t_arena) { t_arena = a; }
 ArenaScope::~ArenaScope() { t_arena = prev; }
 
@@ -45,6 +49,7 @@ void DevBuf::ensure(size_t n) {
     owned = false;
     return;
   }
+  if (t_capturing) throw Error(SUS2_E_INVALID, "device buffer growth during HIP graph capture");
   if (p && owned) RT_CHECK(hipFree(p));
   p = nullptr;
   bytes = 0;
@@ -60,7 +65,7 @@ Profiler::~Profiler() {
 }
 
 int Profiler::begin(const std::string& name) {
-  if (!on) return -1;
+  if (!on || t_capturing) return -1;
   int id = -1;
   for (size_t k = 0; k < names.size(); ++k)
     if (names[k] == name) { id = (int)k; break; }
@@ -130,7 +135,7 @@ constexpr int SCAN_WAVES = SCAN_THREADS / 64;
 
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan(int n, int* __restrict__ cnt, int* __restrict__ off,
                                                        int64_t* __restrict__ total, int cap_lim, long edge_lim,
-                                                       int64_t* report, int64_t seq) {
+                                                       int64_t* report, int64_t seq, int* sticky) {
   __shared__ long wsum[SCAN_WAVES];
   __shared__ int wmax[SCAN_WAVES];
   const int t = threadIdx.x, lane = t % 64, w = t / 64;
@@ -224,6 +229,7 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(int n, int* __restrict__ 
     total[0] = carry;
     total[1] = m;
     total[2] = over;
+    if (sticky && over) *sticky = 1;
     if (report) {  // two self-tagged words, each one atomic 64-bit store at system scope (no fence)
       const uint64_t tag = (uint64_t)(seq & 0x7fffffff) << 32;
       __hip_atomic_store(report, (int64_t)(tag | (uint64_t)(uint32_t)m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -235,8 +241,9 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan(int n, int* __restrict__ 
 }
 
 void device_scan(int* cnt, int* off, int n, int64_t* total, hipStream_t st, int cap_lim, long edge_lim,
-                 int64_t* report, int64_t seq) {
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, st, n, cnt, off, total, cap_lim, edge_lim, report, seq);
+                 int64_t* report, int64_t seq, int* sticky) {
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(SCAN_THREADS), 0, st, n, cnt, off, total, cap_lim, edge_lim, report, seq,
+                     sticky);
   RT_CHECK(hipGetLastError());
 }
 
@@ -254,6 +261,7 @@ int64_t* report_word(Workspace& ws, int k) {
 constexpr auto SPIN_LIMIT = std::chrono::milliseconds(50);
 
 void poll_report(Workspace& ws, int k, int n, int64_t seq, hipStream_t st, const char* what, uint64_t* w) {
+  if (t_capturing || ws.capture) throw Error(SUS2_E_INVALID, std::string(what) + ": host wait in capture-safe mode");
   volatile int64_t* rep = ws.host_report + k;
   const uint64_t want = (uint64_t)(seq & 0x7fffffff);
   auto arrived = [&] {
@@ -284,6 +292,7 @@ void poll_report(Workspace& ws, int k, int n, int64_t seq, hipStream_t st, const
 }
 
 long exclusive_scan(Workspace& ws, int n, hipStream_t st, long* maxcnt) {
+  if (t_capturing || ws.capture) throw Error(SUS2_E_INVALID, "neighbour sizing needs a host wait in capture-safe mode");
   ws.scan_tmp.ensure(8 * sizeof(int64_t));
   device_scan(ws.nbr_cnt.get<int>(), ws.nbr_off.get<int>(), n, ws.scan_tmp.get<int64_t>(), st);
   int64_t local[2];

@@ -25,6 +25,15 @@ struct Arena {
   size_t cap = 0, used = 0;
 };
 Arena* current_arena();
+// While a thread captures a HIP graph, buffer growth (hipMalloc / hipFree)
+// and host waits are errors: every buffer is sized before the capture.
+bool capturing();
+struct CaptureScope {
+  CaptureScope();
+  ~CaptureScope();
+  CaptureScope(const CaptureScope&) = delete;
+  CaptureScope& operator=(const CaptureScope&) = delete;
+};
 struct ArenaScope {
   explicit ArenaScope(Arena* a);
   ~ArenaScope();
@@ -119,6 +128,14 @@ struct Workspace {
   // the device's overflow report afterwards (neighbors_verify); nbr_pending
   // marks such a build whose report has not been read yet.
   bool nbr_optimistic = false, nbr_known = false, nbr_pending = false;
+  // Capture-safe mode (HIP graph capture): no host waits inside a call.  The
+  // neighbour build takes its edge capacity from earlier calls and reports an
+  // overflow only through the sticky device flag overflow_flag (the lists are
+  // then empty and the outputs invalid); the largest frame of the batch is
+  // known_max_frame; a training objective goes to a device word only.
+  bool capture = false;
+  int* overflow_flag = nullptr;
+  long known_max_frame = -1;
   // Count report of such a build, written by the scan kernel into coherent
   // pinned host memory and polled by the host: two 64-bit words, each tagged
   // with the build's sequence number in its upper half (word 0: largest count;
@@ -138,8 +155,9 @@ struct Workspace {
 // (an empty list for every atom); otherwise total[2] = 0.
 // With report: the largest count and the total with the overflow flag are
 // also written to report[0..1], each word tagged with seq in its upper half.
+// With sticky: set to 1 (and never cleared here) when the lists overflowed.
 void device_scan(int* cnt, int* off, int n, int64_t* total, hipStream_t st, int cap_lim = INT32_MAX,
-                 long edge_lim = INT64_MAX, int64_t* report = nullptr, int64_t seq = 0);
+                 long edge_lim = INT64_MAX, int64_t* report = nullptr, int64_t seq = 0, int* sticky = nullptr);
 // exclusive scan of ws.nbr_cnt[0..n) into ws.nbr_off[0..n]; returns the total
 // (and the largest count in *maxcnt)
 long exclusive_scan(Workspace& ws, int n, hipStream_t st, long* maxcnt = nullptr);
@@ -157,8 +175,20 @@ struct DevModelT {
   DModel<T> view{};
 };
 
+// Launch shape of the per-atom EFS kernels (forward passes, edge backwards,
+// gathers): centres per edge-backward group (1, 2 or 4) and the grid, one
+// workgroup per atom / group (grid_mult 0) or grid_mult times the resident
+// workgroups striding over them (persistent).  Set by the autotuner or by the
+// caller; the default is the untuned shape.
+struct EfsTune {
+  int eb_atoms = 2;
+  int grid_mult = 0;
+};
+
 struct DevModelSet {
   const Model* host = nullptr;
+  EfsTune tune[2];  // [0] FP64, [1] FP32 kernels
+  template <class T> const EfsTune& tune_of() const { return tune[sizeof(T) == 4 ? 1 : 0]; }
   DevBuf ints;  // all integer schedule arrays
   DevModelT<double> d;
   DevModelT<float> f;
@@ -179,8 +209,12 @@ void grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Ba
                    bool reuse_neighbors, hipStream_t st);
 // Training objective (1/nframes) sum_f L_f of the batch with its parameter
 // gradient (grad overwritten); labels e0/f0/s0 on the device, loss to host.
+// loss_dev (optional): the objective is also written to this device word;
+// efs_out (optional): the EFS outputs of the batch go to these buffers.  With
+// loss_host null or in capture-safe mode nothing is read back.
 void loss_grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* e0,
                         const double* f0, const double* s0, int mode, double wE, double wF, double wS,
-                        double* loss_host, double* grad, hipStream_t st);
+                        double* loss_host, double* grad, hipStream_t st, double* loss_dev = nullptr,
+                        const EfsOut* efs_out = nullptr);
 
 }  // namespace sus2

@@ -233,6 +233,76 @@ class Sus2Model:
         _join(stream, dev)
         return loss.value, grad
 
+    # ------------------------------------------------------------ graphs and launch shape
+    def graph_efs(self, batch, precision="fp32", stream=None):
+        """Capture the batch EFS in a HIP graph (outputs allocated here)."""
+        import torch
+        nf, n = batch.nframes, batch.natoms
+        dev = batch.pos.device
+        out = dict(energy=torch.zeros(nf, dtype=torch.float64, device=dev),
+                   forces=torch.zeros((n, 3), dtype=torch.float64, device=dev),
+                   virial=torch.zeros((nf, 6), dtype=torch.float64, device=dev),
+                   site=torch.zeros(n, dtype=torch.float64, device=dev))
+        h = ctypes.c_void_p()
+        st = _stream_of(stream, dev)
+        _lib.check(_lib.lib().sus2_graph_efs_create(
+            self._h, PRECISION[precision], nf, n, batch.off.data_ptr(), batch.types.data_ptr(),
+            batch.pos.data_ptr(), batch.cell.data_ptr(), batch.periodic.data_ptr(), out["energy"].data_ptr(),
+            out["forces"].data_ptr(), out["virial"].data_ptr(), out["site"].data_ptr(), ctypes.c_void_p(st),
+            ctypes.byref(h)))
+        return Sus2Graph(h, [batch], out)
+
+    def graph_loss_grad(self, batch, e0, f0, s0=None, weighting="structures", wE=1.0, wF=0.01, wS=0.0,
+                        precision="fp32", stream=None):
+        """Capture the training objective + gradient (and the batch's EFS) in a
+        HIP graph; outputs: loss [1], grad [ncoeff], energy, forces, virial."""
+        import torch
+        dev = batch.pos.device
+        nf, n = batch.nframes, batch.natoms
+        mode = {"vibrations": 0, "structures": 1, "molecules": 2}.get(weighting, weighting)
+        e0 = e0.to(device=dev, dtype=torch.float64).contiguous()
+        f0 = f0.to(device=dev, dtype=torch.float64).contiguous()
+        s0 = (torch.zeros((nf, 6), dtype=torch.float64, device=dev) if s0 is None
+              else s0.to(device=dev, dtype=torch.float64).contiguous())
+        out = dict(loss=torch.zeros(1, dtype=torch.float64, device=dev),
+                   grad=torch.zeros(self.coeff_count, dtype=torch.float64, device=dev),
+                   energy=torch.zeros(nf, dtype=torch.float64, device=dev),
+                   forces=torch.zeros((n, 3), dtype=torch.float64, device=dev),
+                   virial=torch.zeros((nf, 6), dtype=torch.float64, device=dev))
+        h = ctypes.c_void_p()
+        st = _stream_of(stream, dev)
+        _lib.check(_lib.lib().sus2_graph_loss_grad_create(
+            self._h, PRECISION[precision], nf, n, batch.off.data_ptr(), batch.types.data_ptr(),
+            batch.pos.data_ptr(), batch.cell.data_ptr(), batch.periodic.data_ptr(), e0.data_ptr(), f0.data_ptr(),
+            s0.data_ptr(), int(mode), float(wE), float(wF), float(wS), out["loss"].data_ptr(),
+            out["grad"].data_ptr(), out["energy"].data_ptr(), out["forces"].data_ptr(), out["virial"].data_ptr(),
+            ctypes.c_void_p(st), ctypes.byref(h)))
+        return Sus2Graph(h, [batch, e0, f0, s0], out)
+
+    def tune(self, precision, eb_atoms=2, grid_mult=0):
+        """Launch shape of the per-atom EFS kernels (see sus2_tune_set)."""
+        _lib.check(_lib.lib().sus2_tune_set(self._h, PRECISION[precision], int(eb_atoms), int(grid_mult)))
+
+    def get_tune(self, precision):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _lib.check(_lib.lib().sus2_tune_get(self._h, PRECISION[precision], ctypes.byref(a), ctypes.byref(b)))
+        return dict(eb_atoms=a.value, grid_mult=b.value)
+
+    def autotune(self, batch, precision="fp32", reps=3, stream=None):
+        """Time every launch shape on this batch and keep the fastest; returns
+        (chosen shape, {(eb_atoms, grid_mult): median ms})."""
+        import torch
+        dev = batch.pos.device
+        best = (ctypes.c_int * 2)()
+        ms = np.zeros(12)
+        st = _stream_of(stream, dev)
+        _lib.check(_lib.lib().sus2_autotune(self._h, PRECISION[precision], batch.nframes, batch.natoms,
+                                            batch.off.data_ptr(), batch.types.data_ptr(), batch.pos.data_ptr(),
+                                            batch.cell.data_ptr(), batch.periodic.data_ptr(), int(reps),
+                                            ctypes.c_void_p(st), best, ms.ctypes.data_as(_lib._dp)))
+        table = {((1, 2, 4)[c // 4], (0, 1, 2, 4)[c % 4]): float(ms[c]) for c in range(12)}
+        return dict(eb_atoms=best[0], grid_mult=best[1]), table
+
     def train_lbfgs(self, batch, e0, f0, s0=None, weighting="structures", wE=1.0, wF=0.01, wS=0.0,
                     precision="fp32", max_iter=50, history=20, gtol=0.0, stream=None):
         """L-BFGS on the training objective (host FP64 optimiser, GPU loss and
@@ -255,6 +325,44 @@ class Sus2Model:
                                                float(wS), int(max_iter), int(history), float(gtol),
                                                _p(trace), ctypes.byref(iters), ctypes.c_void_p(st)))
         return trace[:iters.value + 1], iters.value
+
+
+class Sus2Graph:
+    """A batch's EFS or objective + gradient captured in a HIP graph
+    (sus2_graph_*_create); launch() replays it on a stream.  Inputs and
+    outputs are the device tensors given at creation: rewrite them in place
+    between launches.  overflow() waits for the stream and reports (and
+    clears) whether a launch outgrew the neighbour capacity sized at creation,
+    in which case its outputs are invalid and the graph must be re-created."""
+
+    def __init__(self, handle, keep, outputs):
+        self._g = handle
+        self._keep = keep  # tensors whose device pointers the graph holds
+        self.outputs = outputs
+
+    def launch(self, stream=None):
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _lib.check(_lib.lib().sus2_graph_launch(self._g, ctypes.c_void_p(st)))
+        return self.outputs
+
+    def overflow(self, stream=None):
+        import torch
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        v = ctypes.c_int()
+        _lib.check(_lib.lib().sus2_graph_overflow(self._g, ctypes.c_void_p(st), ctypes.byref(v)))
+        return bool(v.value)
+
+    def close(self):
+        if getattr(self, "_g", None) and self._g.value:
+            _lib.lib().sus2_graph_free(self._g)
+            self._g = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _str_weights_from_virial_grad(gV):
