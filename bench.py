@@ -501,11 +501,31 @@ def main():
     print(f"[bench] {n} atoms, model {os.path.basename(path)}, steps={args.steps} warmup={args.warmup}",
           file=sys.stderr, flush=True)
 
+    # launch shape of the per-atom kernels from the autotuner, per precision
+    tuned = {}
+    for prec in ("fp32", "fp64"):
+        best, table = model.autotune(batch, prec, reps=5)
+        tuned[prec] = dict(choice=best, median_ms={f"eb{k[0]}_grid{k[1]}": round(v, 4) for k, v in table.items()})
+    print(f"[bench] tuned: fp32 {tuned['fp32']['choice']} fp64 {tuned['fp64']['choice']}", file=sys.stderr, flush=True)
     # both precisions timed without the per-kernel timers; a separate profiled
     # run of the same steps gives the kernel averages behind the roofline
     t32, out32, _ = _timed(model, batch, "fp32", args.steps, args.warmup, False)
     nedges = model.last_edge_count()
     t64, out64, _ = _timed(model, batch, "fp64", args.steps, args.warmup, False)
+    # the same step captured in a HIP graph: one graph launch per step
+    graph_ms = {}
+    for prec in ("fp32", "fp64"):
+        g = model.graph_efs(batch, prec)
+        for _ in range(args.warmup):
+            g.launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.launch()
+        torch.cuda.synchronize()
+        graph_ms[prec] = 1e3 * (time.perf_counter() - t0) / args.steps
+        assert not g.overflow()
+        del g
     prof32 = {} if args.no_profile else _timed(model, batch, "fp32", args.steps, 1, True)[2]
     rate32 = n * args.steps / t32
     rate64 = n * args.steps / t64
@@ -569,6 +589,10 @@ def main():
         "config": {"workload": "effect4096", "atoms": n, "species": 2, "sus2": "Effect L3/K4/R2/O2 body 4, B=10",
                    "cutoff_A": 5.0, "directed_edges": nedges, "neighbors_per_atom": round(nedges / n, 2),
                    "step": "neighbour list + gate pass + outer pass + gather force/virial backward"},
+        "launch_shape": tuned,
+        "graph_ms_per_step": round(graph_ms["fp32"], 4),
+        "graph_atom_steps_per_s": round(n / graph_ms["fp32"] * 1e3, 1),
+        "graph_fp64_ms_per_step": round(graph_ms["fp64"], 4),
         "fp64_atom_steps_per_s": round(rate64, 1),
         "fp64_ms_per_step": round(1e3 * t64 / args.steps, 4),
         "fp32_over_fp64": round(rate32 / rate64, 3),

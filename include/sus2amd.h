@@ -142,6 +142,19 @@ int sus2_train_lbfgs(sus2_model* m, int precision, int nframes, int natoms, cons
                      double wS, int max_iter, int history, double gtol, double* loss_trace, int* iters_out,
                      void* stream);
 
+/* BFGS (host FP64 dense inverse Hessian from the identity, strong-Wolfe line
+ * search with c1 = 0.1, c2 = 0.5, the reference's update and curvature guard)
+ * over the same objective; replaces BFGS::Iterate as used by the trainer
+ * (src/common/bfgs.cpp:251, dev_src/mtpr_trainer.cpp:4685-4760, the default
+ * optimizer of example/train-effect.yaml).  loss_trace [max_iter+1] (host,
+ * may be NULL) receives the objective at the start and after each accepted
+ * step; *iters_out the steps taken.  Stops when max |grad| <= gtol, after
+ * max_iter steps or when the line search finds no acceptable step. */
+int sus2_train_bfgs(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                    const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+                    const double* e0, const double* f0, const double* s0, int weighting, double wE, double wF,
+                    double wS, int max_iter, double gtol, double* loss_trace, int* iters_out, void* stream);
+
 /* --- pair_style sus2/embedded compute() contract ------------------------------
  * Mirrors Kernel (interfaces/lammps/ML-SUS2/sus2_sl_kernel.h:12-43) as driven
  * by PairSUS2Embedded::compute (pair_sus2_embedded.cpp:250-313):

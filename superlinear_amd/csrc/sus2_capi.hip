@@ -13,6 +13,7 @@
 
 #include "../../include/sus2amd.h"
 #include "sus2_model.hpp"
+#include "sus2_optim.hpp"
 #include "sus2_common.hpp"
 #include "sus2_runtime.hpp"
 
@@ -638,6 +639,44 @@ int sus2_autotune(sus2_model* m, int precision, int nframes, int natoms, const i
     slot = chosen;
     best[0] = chosen.eb_atoms;
     best[1] = chosen.grid_mult;
+  });
+}
+
+// BFGS over the training objective: host FP64 dense inverse Hessian and
+// strong-Wolfe line search (sus2_optim.hpp; protocol of BFGS::Iterate,
+// src/common/bfgs.cpp:251-330, as driven by MTPR_trainer::Train,
+// dev_src/mtpr_trainer.cpp:4685-4760); every objective and gradient runs on
+// the GPU (sus2_loss_grad).
+int sus2_train_bfgs(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                    const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+                    const double* e0, const double* f0, const double* s0, int weighting, double wE, double wF,
+                    double wS, int max_iter, double gtol, double* loss_trace, int* iters_out, void* stream) {
+  return guard([&] {
+    if (!m) throw sus2::Error(SUS2_E_INVALID, "null model");
+    check_batch(m, nframes, natoms, frame_off, types, pos, cell, periodic);
+    if (max_iter < 0) throw sus2::Error(SUS2_E_INVALID, "max_iter >= 0 required");
+    ensure_device(m);
+    const hipStream_t st = (hipStream_t)stream;
+    const sus2::Batch b = make_batch(nframes, natoms, frame_off, types, pos, cell, periodic);
+    const int P = m->model.ncoeff;
+    Dev gdev(sizeof(double) * P);
+    sus2::Objective fn = [&](const std::vector<double>& th, std::vector<double>& grad) {
+      std::memcpy(m->model.theta.data(), th.data(), sizeof(double) * P);
+      m->dev.upload_theta(m->model);
+      double f = 0;
+      sus2::loss_grad_dispatch(m->dev, m->ws, precision, b, e0, f0, s0, weighting, wE, wF, wS, &f, gdev.get<double>(),
+                               st);
+      if (hipStreamSynchronize(st) != hipSuccess) throw sus2::Error(SUS2_E_DEVICE, "stream synchronize failed");
+      d2h(grad.data(), gdev.get<double>(), sizeof(double) * P);
+      return f;
+    };
+    std::vector<double> x = m->model.theta;
+    const sus2::BfgsResult r = sus2::bfgs_minimize(fn, x, max_iter, gtol);
+    std::memcpy(m->model.theta.data(), x.data(), sizeof(double) * P);  // the last accepted point
+    m->dev.upload_theta(m->model);
+    if (loss_trace)
+      for (size_t k = 0; k < r.trace.size() && k <= (size_t)max_iter; ++k) loss_trace[k] = r.trace[k];
+    if (iters_out) *iters_out = (int)r.trace.size() - 1;
   });
 }
 

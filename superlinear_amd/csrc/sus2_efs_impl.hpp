@@ -367,10 +367,11 @@ __device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const Spec
   wave_adjoint_to_basic<T, KK>(m, p, false, a, abar + (size_t)i * KH);
 }
 
-// Forward pass over atoms [0, natoms): one atom per workgroup when the grid
-// covers them all, else a persistent grid striding over the atoms (the
-// species tables are loaded once per workgroup).
-template <class Cf, bool GATE, class T = typename Cf::T>
+// Forward pass over atoms [0, natoms): one atom per workgroup (grid = natoms),
+// or (PERSIST) a persistent grid striding over the atoms with the species
+// tables loaded once per workgroup.  The two are separate instances so that
+// the one-atom form keeps its straight-line register allocation.
+template <class Cf, bool GATE, bool PERSIST, class T = typename Cf::T>
 __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, int natoms, const int32_t* __restrict__ types,
                                                        const int* __restrict__ nbr_off,
                                                        const Rec<T>* __restrict__ rec, T* __restrict__ h,
@@ -382,9 +383,13 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, int natoms, 
   SpeciesTables<T> tab;
   load_species_tables<Cf>(m, GATE, tabs_base, tab);
   __syncthreads();
-  for (int i = blockIdx.x; i < natoms; i += gridDim.x) {
-    pass_forward_atom<Cf, GATE>(m, tab, region, i, types, nbr_off, rec, h, vg, site, abar);
-    __syncthreads();  // the next atom overwrites the region
+  if constexpr (!PERSIST) {
+    pass_forward_atom<Cf, GATE>(m, tab, region, blockIdx.x, types, nbr_off, rec, h, vg, site, abar);
+  } else {
+    for (int i = blockIdx.x; i < natoms; i += gridDim.x) {
+      pass_forward_atom<Cf, GATE>(m, tab, region, i, types, nbr_off, rec, h, vg, site, abar);
+      __syncthreads();  // the next atom overwrites the region
+    }
   }
 }
 
@@ -402,30 +407,20 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, int natoms, 
 template <class Cf, bool GATE>
 constexpr int edge_backward_waves() { return (sizeof(typename Cf::T) == 4 && Cf::LL <= 3) ? 3 : 1; }
 
-// NA centres per group (their edge lists are contiguous, so two or four
-// atoms' edges fill the lanes better than one atom's ~90); one group per
-// workgroup when the grid covers them all, else a persistent grid striding
-// over the groups.
+// NA centres i0 .. i0 + NA - 1 of an edge-backward pass (their edge lists
+// are contiguous, so two or four atoms' edges fill the lanes better than one
+// atom's ~90).  sA: [NA][KH] adjoints, sV: [NA][9][LR] lane partials, sXR:
+// [F][LR] gate-message columns; tab already loaded.
 template <class Cf, bool GATE, int NA, class T = typename Cf::T>
-__global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edge_backward(DModel<T> m, int natoms,
-                                                        const int32_t* __restrict__ types,
-                                                        const int* __restrict__ nbr_off,
-                                                        const Rec<T>* __restrict__ rec, const T* __restrict__ h,
-                                                        const T* __restrict__ adj, const int* __restrict__ in_pos,
-                                                        T* __restrict__ gedge_in, T* __restrict__ uedge,
-                                                        T* __restrict__ fw) {
+__device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const SpeciesTables<T>& tab, T* sA, T* sV,
+                                                    T* sXR, int i0, int natoms, const int32_t* __restrict__ types,
+                                                    const int* __restrict__ nbr_off, const Rec<T>* __restrict__ rec,
+                                                    const T* __restrict__ h, const T* __restrict__ adj,
+                                                    const int* __restrict__ in_pos, T* __restrict__ gedge_in,
+                                                    T* __restrict__ uedge, T* __restrict__ fw) {
   constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, F = Cf::F, KH = Cf::KH;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* sA = reinterpret_cast<T*>(smem_raw);  // [NA][KH]
-  // [NA][9][LR] per-lane partials of each centre: outgoing force sum, virial
-  // (padded rows, so that the row sums at the end read distinct banks)
-  T* sV = sA + NA * KH + species_tables_size<Cf>(m.C);
-  T* sXR = sV + NA * 9 * LR;  // [F][LR] lane-private columns: x0 R (gate message)
-  SpeciesTables<T> tab;
   const int lane = threadIdx.x;
   const bool hgate = !GATE && m.gated;
-  load_species_tables<Cf>(m, GATE, sA + NA * KH, tab);
-  for (int i0 = blockIdx.x * NA; i0 < natoms; i0 += gridDim.x * NA) {
   const int na = min(NA, natoms - i0);
   for (int c = lane; c < na * KH; c += WAVE) sA[c] = adj[(size_t)i0 * KH + c];
 #pragma unroll
@@ -539,7 +534,35 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
     T* o = fw + (size_t)(i0 + lane / 9) * 9 + lane % 9;
     *o = GATE ? *o + tot : tot;
   }
-  __syncthreads();  // the next group overwrites sA and sV
+}
+
+// Edge backward over groups of NA centres: one group per workgroup, or
+// (PERSIST) a persistent grid striding over the groups.
+template <class Cf, bool GATE, int NA, bool PERSIST, class T = typename Cf::T>
+__global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edge_backward(DModel<T> m, int natoms,
+                                                        const int32_t* __restrict__ types,
+                                                        const int* __restrict__ nbr_off,
+                                                        const Rec<T>* __restrict__ rec, const T* __restrict__ h,
+                                                        const T* __restrict__ adj, const int* __restrict__ in_pos,
+                                                        T* __restrict__ gedge_in, T* __restrict__ uedge,
+                                                        T* __restrict__ fw) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sA = reinterpret_cast<T*>(smem_raw);  // [NA][KH]
+  // [NA][9][LR] per-lane partials of each centre: outgoing force sum, virial
+  // (padded rows, so that the row sums at the end read distinct banks)
+  T* sV = sA + NA * Cf::KH + species_tables_size<Cf>(m.C);
+  T* sXR = sV + NA * 9 * LR;  // [F][LR] lane-private columns: x0 R (gate message)
+  SpeciesTables<T> tab;
+  load_species_tables<Cf>(m, GATE, sA + NA * Cf::KH, tab);
+  if constexpr (!PERSIST) {
+    edge_backward_group<Cf, GATE, NA>(m, tab, sA, sV, sXR, blockIdx.x * NA, natoms, types, nbr_off, rec, h, adj,
+                                      in_pos, gedge_in, uedge, fw);
+  } else {
+    for (int i0 = blockIdx.x * NA; i0 < natoms; i0 += gridDim.x * NA) {
+      edge_backward_group<Cf, GATE, NA>(m, tab, sA, sV, sXR, i0, natoms, types, nbr_off, rec, h, adj, in_pos,
+                                        gedge_in, uedge, fw);
+      __syncthreads();  // the next group overwrites sA and sV
+    }
   }
 }
 
@@ -693,17 +716,21 @@ __device__ __forceinline__ void gather_atom(const DModel<T>& m, int i, const int
   gate_reverse_atom<Cf>(m, i, vg, gbar, hbs, v, a, scratch);
 }
 
-// Gather over atoms [0, natoms): one atom per workgroup, or a persistent grid
-// striding over the atoms.
-template <class Cf, bool GATE, class T = typename Cf::T>
+// Gather over atoms [0, natoms): one atom per workgroup, or (PERSIST) a
+// persistent grid striding over the atoms.
+template <class Cf, bool GATE, bool PERSIST, class T = typename Cf::T>
 __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, int natoms, const int32_t* __restrict__ types,
                                                  const int* __restrict__ in_off, const T* __restrict__ gedge_in,
                                                  const T* __restrict__ uedge, const T* __restrict__ vg,
                                                  T* __restrict__ gbar, T* __restrict__ hbar_out,
                                                  T* __restrict__ fw) {
-  for (int i = blockIdx.x; i < natoms; i += gridDim.x) {
-    gather_atom<Cf, GATE>(m, i, types, in_off, gedge_in, uedge, vg, gbar, hbar_out, fw);
-    __syncthreads();  // the next atom reuses the LDS
+  if constexpr (!PERSIST) {
+    gather_atom<Cf, GATE>(m, blockIdx.x, types, in_off, gedge_in, uedge, vg, gbar, hbar_out, fw);
+  } else {
+    for (int i = blockIdx.x; i < natoms; i += gridDim.x) {
+      gather_atom<Cf, GATE>(m, i, types, in_off, gedge_in, uedge, vg, gbar, hbar_out, fw);
+      __syncthreads();  // the next atom reuses the LDS
+    }
   }
 }
 
@@ -789,11 +816,17 @@ void launch_pass_forward(const DevModelSet& dm, Workspace& ws, int n, const int3
   using T = typename Cf::T;
   const DModel<T>& m = dm.get<T>();
   const size_t lf = forward_lds<Cf>(m);
-  allow_lds(k_pass_forward<Cf, GATE>, lf);
-  const unsigned g = tuned_grid(dm.tune_of<T>(), k_pass_forward<Cf, GATE>, lf, n);
-  if (g == 0) return;
-  SUS2_LAUNCH(ws, st, kname<Cf>(GATE ? "pass_forward_gate" : "pass_forward_outer"), (k_pass_forward<Cf, GATE>),
-              dim3(g), dim3(WAVE), lf, m, n, types, off, rec, h, vg, site, abar);
+  if (n <= 0) return;
+  const char* name = GATE ? "pass_forward_gate" : "pass_forward_outer";
+  if (dm.tune_of<T>().grid_mult <= 0) {
+    allow_lds(k_pass_forward<Cf, GATE, false>, lf);
+    SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_pass_forward<Cf, GATE, false>), dim3(n), dim3(WAVE), lf, m, n, types, off,
+                rec, h, vg, site, abar);
+    return;
+  }
+  const unsigned g = tuned_grid(dm.tune_of<T>(), k_pass_forward<Cf, GATE, true>, lf, n);
+  SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_pass_forward<Cf, GATE, true>), dim3(g), dim3(WAVE), lf, m, n, types, off,
+              rec, h, vg, site, abar);
 }
 
 template <class Cf, bool GATE, int NA>
@@ -804,11 +837,18 @@ void launch_edge_backward_na(const DevModelSet& dm, Workspace& ws, int n, const 
   using T = typename Cf::T;
   const DModel<T>& m = dm.get<T>();
   const size_t lb = backward_lds<Cf>(m, NA);
-  allow_lds(k_edge_backward<Cf, GATE, NA>, lb);
-  const unsigned g = tuned_grid(dm.tune_of<T>(), k_edge_backward<Cf, GATE, NA>, lb, (n + NA - 1) / NA);
-  if (g == 0) return;
-  SUS2_LAUNCH(ws, st, kname<Cf>(GATE ? "edge_backward_gate" : "edge_backward_outer"), (k_edge_backward<Cf, GATE, NA>),
-              dim3(g), dim3(WAVE), lb, m, n, types, off, rec, h, adj, in_pos, gedge_in, uedge, fw);
+  if (n <= 0) return;
+  const char* name = GATE ? "edge_backward_gate" : "edge_backward_outer";
+  const int groups = (n + NA - 1) / NA;
+  if (dm.tune_of<T>().grid_mult <= 0) {
+    allow_lds(k_edge_backward<Cf, GATE, NA, false>, lb);
+    SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_edge_backward<Cf, GATE, NA, false>), dim3(groups), dim3(WAVE), lb, m, n,
+                types, off, rec, h, adj, in_pos, gedge_in, uedge, fw);
+    return;
+  }
+  const unsigned g = tuned_grid(dm.tune_of<T>(), k_edge_backward<Cf, GATE, NA, true>, lb, groups);
+  SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_edge_backward<Cf, GATE, NA, true>), dim3(g), dim3(WAVE), lb, m, n, types,
+              off, rec, h, adj, in_pos, gedge_in, uedge, fw);
 }
 
 // Edge backward with the tuned centres per group (1, 2 or 4).
@@ -833,11 +873,17 @@ void launch_gather(const DevModelSet& dm, Workspace& ws, int nall, const int32_t
   // gated outer gathers with F >= 16 sum their messages through LDS; the gate
   // graph reverse of the outer gather needs its walk LDS
   const size_t lg = (!GATE && m.gated) ? gather_lds<Cf>(m) : 0;
-  allow_lds(k_gather<Cf, GATE>, lg);
-  const unsigned g = tuned_grid(dm.tune_of<T>(), k_gather<Cf, GATE>, lg, nall);
-  if (g == 0) return;
-  SUS2_LAUNCH(ws, st, kname<Cf>(GATE ? "gather_gate" : "gather_outer"), (k_gather<Cf, GATE>), dim3(g), dim3(WAVE),
-              lg, m, nall, types, in_off, gedge_in, uedge, vg, gbar, hbar_out, fw);
+  if (nall <= 0) return;
+  const char* name = GATE ? "gather_gate" : "gather_outer";
+  if (dm.tune_of<T>().grid_mult <= 0) {
+    allow_lds(k_gather<Cf, GATE, false>, lg);
+    SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_gather<Cf, GATE, false>), dim3(nall), dim3(WAVE), lg, m, nall, types,
+                in_off, gedge_in, uedge, vg, gbar, hbar_out, fw);
+    return;
+  }
+  const unsigned g = tuned_grid(dm.tune_of<T>(), k_gather<Cf, GATE, true>, lg, nall);
+  SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_gather<Cf, GATE, true>), dim3(g), dim3(WAVE), lg, m, nall, types, in_off,
+              gedge_in, uedge, vg, gbar, hbar_out, fw);
 }
 
 template <class Cf>

@@ -303,6 +303,31 @@ class Sus2Model:
         table = {((1, 2, 4)[c // 4], (0, 1, 2, 4)[c % 4]): float(ms[c]) for c in range(12)}
         return dict(eb_atoms=best[0], grid_mult=best[1]), table
 
+    def train_bfgs(self, batch, e0, f0, s0=None, weighting="structures", wE=1.0, wF=0.01, wS=0.0,
+                   precision="fp32", max_iter=50, gtol=0.0, stream=None):
+        """BFGS on the training objective (host FP64 dense inverse Hessian and
+        strong-Wolfe line search, GPU loss and gradient); the model's
+        coefficients end at the last accepted point.  Returns (loss trace,
+        steps taken)."""
+        import torch
+        dev = batch.pos.device
+        mode = {"vibrations": 0, "structures": 1, "molecules": 2}.get(weighting, weighting)
+        e0 = e0.to(device=dev, dtype=torch.float64).contiguous()
+        f0 = f0.to(device=dev, dtype=torch.float64).contiguous()
+        s0 = (torch.zeros((batch.nframes, 6), dtype=torch.float64, device=dev) if s0 is None
+              else s0.to(device=dev, dtype=torch.float64).contiguous())
+        trace = np.zeros(max_iter + 1)
+        iters = ctypes.c_int()
+        st = _stream_of(stream, dev)
+        _lib.check(_lib.lib().sus2_train_bfgs(self._h, PRECISION[precision], batch.nframes, batch.natoms,
+                                              batch.off.data_ptr(), batch.types.data_ptr(), batch.pos.data_ptr(),
+                                              batch.cell.data_ptr(), batch.periodic.data_ptr(), e0.data_ptr(),
+                                              f0.data_ptr(), s0.data_ptr(), int(mode), float(wE), float(wF),
+                                              float(wS), int(max_iter), float(gtol),
+                                              trace.ctypes.data_as(_lib._dp), ctypes.byref(iters),
+                                              ctypes.c_void_p(st)))
+        return trace[:iters.value + 1], iters.value
+
     def train_lbfgs(self, batch, e0, f0, s0=None, weighting="structures", wE=1.0, wF=0.01, wS=0.0,
                     precision="fp32", max_iter=50, history=20, gtol=0.0, stream=None):
         """L-BFGS on the training objective (host FP64 optimiser, GPU loss and

@@ -201,6 +201,32 @@ def test_lbfgs_training_reduces_objective(lib, prec):
     assert not np.allclose(m.coeffs(), th)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_bfgs_training_reduces_objective(lib, prec):
+    """Host BFGS over the GPU objective: accepted steps never raise the
+    objective (Wolfe steps satisfy sufficient decrease), it drops well below
+    its start, and the model ends at the last accepted point."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    print(f"[case] bfgs {prec}", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frames = W.periodic_frames(6, 8, 16, 2, 0.15, seed=97)
+    batch = DeviceBatch(frames)
+    teacher = Sus2Model(path)
+    th = teacher.coeffs()
+    teacher.set_coeffs(th * (1 + 0.05 * np.random.default_rng(98).standard_normal(th.size)))
+    lab = teacher.efs_device(batch, "fp64")
+    m = Sus2Model(path)
+    trace, steps = m.train_bfgs(batch, lab["energy"], lab["forces"], weighting="structures", wE=1.0, wF=0.01,
+                                precision=prec, max_iter=25)
+    print(f"[result] bfgs {prec}: {steps} steps, objective {trace[0]:.4e} -> {trace[-1]:.4e}", flush=True)
+    assert steps >= 5
+    assert np.all(np.diff(trace) <= 0)
+    assert trace[-1] < 0.2 * trace[0]
+    loss, _ = m.loss_grad_device(batch, lab["energy"], lab["forces"], precision=prec)
+    assert abs(loss - trace[-1]) <= 1e-9 * trace[0]
+
+
 def test_repeated_scalar_node(lib, oracle_lib, tmp_path):
     """A readout mapping that names one graph node twice (the file format
     allows it): the energy seeds of that node add up in scalar order.  EFS and

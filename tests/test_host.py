@@ -227,3 +227,45 @@ def test_calculator_validates_before_evaluation():
         SuperlinearCalculator(path, "CSi")
     with pytest.raises(ValueError):
         SuperlinearCalculator(path, ["C", "Xx"])
+
+
+@pytest.fixture(scope="module")
+def optim_driver(tmp_path_factory):
+    import subprocess
+    exe = tmp_path_factory.mktemp("optim") / "optim_driver"
+    subprocess.run(["g++", "-O2", "-std=c++17", str(ROOT / "tests" / "native" / "optim_driver.cpp"), "-o", str(exe)],
+                   check=True)
+    return exe
+
+
+def _run_optim(exe, *args):
+    import subprocess
+    out = subprocess.run([str(exe), *map(str, args)], capture_output=True, text=True, check=True).stdout.split()
+    head = dict(zip(out[0:8:2], out[1:8:2]))
+    return head, np.array([float(v) for v in out[8:]])
+
+
+def test_bfgs_quadratic_minimiser_and_secant(optim_driver, tmp_path):
+    """Host BFGS on a convex quadratic with condition number 1e3: converges to
+    A^-1 b, and every inverse-Hessian update satisfies the secant equation
+    H y = s (the defining property of the BFGS update)."""
+    rng = np.random.default_rng(17)
+    n = 12
+    q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    A = q @ np.diag(np.logspace(0, 3, n)) @ q.T
+    b = rng.normal(size=n)
+    f = tmp_path / "quad.txt"
+    f.write_text(f"{n}\n" + " ".join(repr(float(v)) for v in A.ravel()) + "\n" + " ".join(repr(float(v)) for v in b))
+    head, x = _run_optim(optim_driver, "quad", f)
+    xs = np.linalg.solve(A, b)
+    assert head["converged"] == "1"
+    assert int(head["iters"]) <= 3 * n
+    assert np.abs(x - xs).max() <= 1e-8 * (1 + np.abs(xs).max())
+    assert float(head["secant"]) <= 1e-9 * (1 + np.abs(xs).max())
+
+
+def test_bfgs_rosenbrock(optim_driver):
+    head, x = _run_optim(optim_driver, "rosen")
+    assert head["converged"] == "1"
+    assert np.allclose(x, [1.0, 1.0], atol=1e-8)
+    assert int(head["iters"]) < 100
