@@ -155,6 +155,22 @@ int sus2_train_bfgs(sus2_model* m, int precision, int nframes, int natoms, const
                     const double* e0, const double* f0, const double* s0, int weighting, double wE, double wF,
                     double wS, int max_iter, double gtol, double* loss_trace, int* iters_out, void* stream);
 
+/* Linear readout solve: with every other coefficient fixed, the species
+ * constants and per-element readout weights enter E, F and the virial
+ * linearly; their least-squares fit under the objective's weights (the
+ * normal equations of MTPR_trainer::TrainLinear / AddToSLAE,
+ * dev_src/mtpr_trainer.cpp:1130-1455, norm_by_forces = 0) is formed on the GPU
+ * (design columns from EFS evaluations, J^T J on FP64 MFMA tiles) and solved
+ * on the host (Cholesky, `ridge` added to the readout diagonal).  The
+ * solution replaces the model's species and readout coefficients;
+ * *objective_out (host, may be NULL) receives the objective there (ridge
+ * excluded).  Per-element readout models only (else SUS2_E_UNSUPPORTED).
+ * Synchronises `stream`. */
+int sus2_train_linear(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                      const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+                      const double* e0, const double* f0, const double* s0, int weighting, double wE, double wF,
+                      double wS, double ridge, double* objective_out, void* stream);
+
 /* --- pair_style sus2/embedded compute() contract ------------------------------
  * Mirrors Kernel (interfaces/lammps/ML-SUS2/sus2_sl_kernel.h:12-43) as driven
  * by PairSUS2Embedded::compute (pair_sus2_embedded.cpp:250-313):

@@ -680,6 +680,25 @@ int sus2_train_bfgs(sus2_model* m, int precision, int nframes, int natoms, const
   });
 }
 
+// Linear readout solve (TrainLinear, dev_src/mtpr_trainer.cpp:1130-1455).
+int sus2_train_linear(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
+                      const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
+                      const double* e0, const double* f0, const double* s0, int weighting, double wE, double wF,
+                      double wS, double ridge, double* objective_out, void* stream) {
+  return guard([&] {
+    if (!m) throw sus2::Error(SUS2_E_INVALID, "null model");
+    check_batch(m, nframes, natoms, frame_off, types, pos, cell, periodic);
+    if (nframes == 0) throw sus2::Error(SUS2_E_INVALID, "empty batch");
+    if (!e0 || !s0 || (natoms > 0 && !f0)) throw sus2::Error(SUS2_E_INVALID, "null label buffer");
+    if (precision != SUS2_FP64 && precision != SUS2_FP32) throw sus2::Error(SUS2_E_INVALID, "precision must be FP64 or FP32");
+    ensure_device(m);
+    const double v = sus2::train_linear(m->model, m->dev, m->ws, precision,
+                                        make_batch(nframes, natoms, frame_off, types, pos, cell, periodic), e0, f0,
+                                        s0, weighting, wE, wF, wS, ridge, (hipStream_t)stream);
+    if (objective_out) *objective_out = v;
+  });
+}
+
 // ------------------------------------------------------------------ pair style
 }  // extern "C"
 

@@ -217,4 +217,11 @@ void loss_grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, con
                         double* loss_host, double* grad, hipStream_t st, double* loss_dev = nullptr,
                         const EfsOut* efs_out = nullptr);
 
+// Linear readout solve (species constants and per-element readout weights)
+// of the training objective; writes the solution into host.theta and the
+// device copies, returns the objective at the solution.
+double train_linear(Model& host, DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* e0,
+                    const double* f0, const double* s0, int mode, double wE, double wF, double wS, double ridge,
+                    hipStream_t st);
+
 }  // namespace sus2

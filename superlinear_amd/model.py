@@ -328,6 +328,27 @@ class Sus2Model:
                                               ctypes.c_void_p(st)))
         return trace[:iters.value + 1], iters.value
 
+    def train_linear(self, batch, e0, f0, s0=None, weighting="structures", wE=1.0, wF=0.01, wS=0.0,
+                     precision="fp64", ridge=0.0, stream=None):
+        """Linear readout solve (species constants and per-element readout
+        weights, the reference's TrainLinear); returns the objective at the
+        solution, which replaces those coefficients of the model."""
+        import torch
+        dev = batch.pos.device
+        mode = {"vibrations": 0, "structures": 1, "molecules": 2}.get(weighting, weighting)
+        e0 = e0.to(device=dev, dtype=torch.float64).contiguous()
+        f0 = f0.to(device=dev, dtype=torch.float64).contiguous()
+        s0 = (torch.zeros((batch.nframes, 6), dtype=torch.float64, device=dev) if s0 is None
+              else s0.to(device=dev, dtype=torch.float64).contiguous())
+        obj = ctypes.c_double()
+        st = _stream_of(stream, dev)
+        _lib.check(_lib.lib().sus2_train_linear(self._h, PRECISION[precision], batch.nframes, batch.natoms,
+                                                batch.off.data_ptr(), batch.types.data_ptr(), batch.pos.data_ptr(),
+                                                batch.cell.data_ptr(), batch.periodic.data_ptr(), e0.data_ptr(),
+                                                f0.data_ptr(), s0.data_ptr(), int(mode), float(wE), float(wF),
+                                                float(wS), float(ridge), ctypes.byref(obj), ctypes.c_void_p(st)))
+        return obj.value
+
     def train_lbfgs(self, batch, e0, f0, s0=None, weighting="structures", wE=1.0, wF=0.01, wS=0.0,
                     precision="fp32", max_iter=50, history=20, gtol=0.0, stream=None):
         """L-BFGS on the training objective (host FP64 optimiser, GPU loss and

@@ -335,3 +335,39 @@ def test_back_to_back_objectives_on_growing_batches(lib):
             assert lq == ls
             assert torch.equal(gq, gs)
         assert queued[0][0] != queued[1][0]
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_linear_readout_solve_is_stationary(lib, mode):
+    """sus2_train_linear: the species constants and per-element readout
+    weights it returns make the objective stationary in those coefficients
+    (their gradient entries, by the independent gradient kernels, vanish to
+    1e-7 of their size before the solve), its reported objective equals the
+    objective evaluated there, and the objective does not increase."""
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    print(f"[case] linear readout mode {mode}", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frames = W.periodic_frames(8, 8, 40, 2, 0.15, seed=131) + W.molecule_frames(2, 8, 20, 2, seed=132)
+    batch = DeviceBatch(frames)
+    teacher = Sus2Model(path)
+    th = teacher.coeffs()
+    teacher.set_coeffs(th * (1 + 0.05 * np.random.default_rng(133).standard_normal(th.size)))
+    lab = teacher.efs_device(batch, "fp64")
+    m = Sus2Model(path)
+    off = m.offsets()
+    lin = np.r_[off["species"]:off["species"] + 2, off["read"]:off["ncoeff"]]
+    kw = dict(weighting=mode, wE=1.0, wF=0.01, wS=0.001, precision="fp64")
+    loss0, g0 = m.loss_grad_device(batch, lab["energy"], lab["forces"], lab["virial"], **kw)
+    g0 = g0.cpu().numpy()
+    obj = m.train_linear(batch, lab["energy"], lab["forces"], lab["virial"], **kw)
+    loss1, g1 = m.loss_grad_device(batch, lab["energy"], lab["forces"], lab["virial"], **kw)
+    g1 = g1.cpu().numpy()
+    print(f"[result] linear mode {mode}: objective {loss0:.6e} -> {loss1:.6e} (reported {obj:.6e}); "
+          f"|g_lin| {np.linalg.norm(g0[lin]):.3e} -> {np.linalg.norm(g1[lin]):.3e}", flush=True)
+    assert loss1 <= loss0
+    assert abs(obj - loss1) <= 1e-8 * loss1 + 1e-15
+    assert np.linalg.norm(g1[lin]) <= 1e-7 * np.linalg.norm(g0[lin])
+    rest = np.setdiff1d(np.arange(len(th)), lin)
+    assert np.array_equal(m.coeffs()[rest], th[rest])
