@@ -457,6 +457,20 @@ def measured_traffic(kernel, workload):
     return None
 
 
+def _copy_probe(mib):
+    """A plain FP32 copy kernel (y = 2x) over `mib` MiB, three launches: the
+    calibration dispatch of the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    if mib <= 0:
+        return
+    import torch
+    x = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        torch.mul(x, 2.0, out=y)
+    torch.cuda.synchronize()
+    print(f"[bench] copy probe: {mib} MiB read + {mib} MiB written per launch", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -491,6 +505,16 @@ def main():
     from superlinear_amd import workloads as W
     from superlinear_amd.flops import graph_stats, step_efs_per_atom, step_efs_per_edge
     from superlinear_amd.model import DeviceBatch, Sus2Model
+
+    if args.only and args.only != "effect4096":  # one non-headline section (per-workload profiles)
+        path = W.model_path("effect_l3k4r2o2")
+        sec = {"static64": lambda: static_section(args.static_steps, 20, not args.no_profile, args.no_cpu_baseline),
+               "train256": lambda: train_section(path, args.train_steps, 1, not args.no_profile, args.no_cpu_baseline),
+               "batch4096": lambda: batch_section(args.batch_steps, 2, not args.no_profile),
+               "md2m": lambda: md_section(args.md_steps, 1, not args.no_profile)}[args.only]()
+        _copy_probe(args.copy_probe_mib)
+        print(json.dumps({"metric": METRIC, "section": args.only, "result": sec}))
+        return 0
 
     path = W.model_path("effect_l3k4r2o2")
     frame = W.config_frames("effect4096")[0]
@@ -564,14 +588,7 @@ def main():
     batch4096 = None if args.no_batch else batch_section(args.batch_steps, 2, not args.no_profile)
     md2m = None if args.no_md else md_section(args.md_steps, 1, not args.no_profile)
 
-    if args.copy_probe_mib > 0:
-        x = torch.ones(args.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        for _ in range(3):
-            torch.mul(x, 2.0, out=y)
-        torch.cuda.synchronize()
-        print(f"[bench] copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written per launch",
-              file=sys.stderr, flush=True)
+    _copy_probe(args.copy_probe_mib)
 
     line = {
         "metric": METRIC,

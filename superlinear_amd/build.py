@@ -9,6 +9,8 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 SRC = HERE / "csrc"
 OUT = HERE / "libsus2amd.so"
+CLI = HERE / "mlp-sus2"
+ROCM_LIB = Path(os.environ.get("ROCM_PATH", "/opt/rocm")) / "lib"
 OBJ = HERE / "_build"
 ARCH = os.environ.get("SUS2_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -43,8 +45,16 @@ def build(jobs=None, verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    # the C++ calc-efs / calc-errors driver over the C ABI
+    exe = CLI.with_name(CLI.name + f".tmp{os.getpid()}")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", str(SRC / "cli" / "mlp_sus2.cpp"), f"-L{HERE}", "-lsus2amd",
+           "-Wl,-rpath,$ORIGIN", f"-L{ROCM_LIB}", f"-Wl,-rpath-link,{ROCM_LIB}", "-o", str(exe)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(exe, CLI)
     if verbose:
-        print(f"built {OUT}")
+        print(f"built {OUT} and {CLI}")
     return OUT
 
 

@@ -112,3 +112,44 @@ def test_cli_calc_efs_and_errors(lib, oracle_lib, tmp_path):
     close(m["force_mae_mev_a"], 2.0, 1e-6)
     close(m["stress_rmse_ev"], 0.5, 1e-9)
     assert calc_efs(str(path), str(inp), str(tmp_path / "o32.cfg"), precision="fp32") == len(frames)
+
+
+def test_cpp_cli_matches_python_cli(lib, tmp_path):
+    """The C++ driver (superlinear_amd/mlp-sus2, over the C ABI) writes the
+    same lossless configurations as the Python calc-efs on the same batches,
+    and prints the same calc-errors summary block."""
+    import subprocess
+    from superlinear_amd import workloads as W
+    from superlinear_amd.build import CLI
+    from superlinear_amd.cfg import read_cfg, write_cfg
+    from superlinear_amd.cli import calc_efs, calc_errors
+    print("[case] cpp cli", flush=True)
+    if not CLI.exists():
+        pytest.fail("mlp-sus2 is not built (run superlinear_amd.build first)")
+    path = W.model_path("effect_l3k4r2o2")
+    frames = W.periodic_frames(3, 6, 12, 2, 0.1, seed=71) + W.molecule_frames(1, 6, 9, 2, seed=72) + \
+        W.partial_frames(1, 6, 10, 2, seed=73, axes=2)
+    inp = tmp_path / "in.cfg"
+    with open(inp, "w") as fh:
+        for k, f in enumerate(frames):
+            n = len(f["types"])
+            g = dict(f, energy=-3.0 * n + 0.1 * k, forces=np.full((n, 3), 0.01 * (k + 1)),
+                     virial=np.linspace(-1.0, 1.0, 6) * (k + 1))
+            write_cfg(fh, g, lossless=True)
+    out_py, out_cc = tmp_path / "py.cfg", tmp_path / "cc.cfg"
+    calc_efs(str(path), str(inp), str(out_py), lossless=True, max_atoms=20)
+    r = subprocess.run([str(CLI), "calc-efs", str(path), str(inp), str(out_cc), "--no-loss", "--batch-atoms", "20"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    a, b = list(read_cfg(out_py)), list(read_cfg(out_cc))
+    assert len(a) == len(b) == len(frames)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x["types"], y["types"])
+        for k in ("energy", "forces", "virial", "site_energy", "positions"):
+            close(y[k], x[k], 1e-14)
+    buf = io.StringIO()
+    calc_errors(str(path), str(inp), max_atoms=20, stream=buf)
+    r = subprocess.run([str(CLI), "calc-errors", str(path), str(inp), "--batch-atoms", "20"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == buf.getvalue().strip()
