@@ -83,7 +83,20 @@ struct Fields {
 
 // Layered schedule of one pass: forward layers by output dependency depth,
 // reverse layers by consumer depth; both in fixed (serialised) row order so
-// every accumulation is deterministic.
+// every accumulation is deterministic.  A depth level with more than
+// kLayerRows rows is cut into consecutive sub-layers at node boundaries (the
+// nodes of one level are independent, so every node's sum is unchanged):
+// the kernels stage one (sub-)layer's row products in LDS, and the cap keeps
+// that scratch -- and so the LDS footprint per atom -- bounded at large L.
+constexpr int kLayerRows = 1024;
+
+// Rows per sub-layer when a level of `total` rows is cut: equal shares of the
+// fewest sub-layers under the cap.
+int split_target(int total) {
+  const int parts = std::max(1, (total + kLayerRows - 1) / kLayerRows);
+  return (total + parts - 1) / parts;
+}
+
 PassPlan make_plan(const Model& m, bool gate) {
   PassPlan p;
   int lo = m.M, hi = -1;
@@ -112,7 +125,16 @@ PassPlan make_plan(const Model& m, bool gate) {
   p.fw_layer_off.push_back(0);
   p.fw_row_off.push_back(0);
   for (int l = 1; l <= nl; ++l) {
+    int total = 0, rows = 0;
+    for (int n = 0; n < m.M; ++n) if (is_out[n] && layer[n] == l) total += (int)rows_of[n].size();
+    const int target = split_target(total);
     for (int n = 0; n < m.M; ++n) if (is_out[n] && layer[n] == l) {
+      const int nr = (int)rows_of[n].size();
+      if (rows > 0 && (rows + nr > kLayerRows || rows >= target)) {
+        p.fw_layer_off.push_back((int)p.fw_out.size());
+        rows = 0;
+      }
+      rows += nr;
       p.fw_out.push_back(n);
       for (int q : rows_of[n]) { p.fw_a.push_back(m.prod[q].a); p.fw_b.push_back(m.prod[q].b); p.fw_c.push_back(m.prod[q].c); }
       p.fw_row_off.push_back((int)p.fw_a.size());
@@ -131,7 +153,16 @@ PassPlan make_plan(const Model& m, bool gate) {
   p.rv_layer_off.push_back(0);
   p.rv_row_off.push_back(0);
   for (int l = 1; l <= nr; ++l) {
+    int total = 0, rows = 0;
+    for (int n = 0; n < m.M; ++n) if (!cons[n].empty() && rl[n] == l) total += (int)cons[n].size();
+    const int target = split_target(total);
     for (int n = 0; n < m.M; ++n) if (!cons[n].empty() && rl[n] == l) {
+      const int nc = (int)cons[n].size();
+      if (rows > 0 && (rows + nc > kLayerRows || rows >= target)) {
+        p.rv_layer_off.push_back((int)p.rv_node.size());
+        rows = 0;
+      }
+      rows += nc;
       p.rv_node.push_back(n);
       for (const auto& c : cons[n]) {
         const auto& r = m.prod[c[0]];
