@@ -1103,9 +1103,17 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
                 ws.nbr_slots.get<Rec<T>>(), ws.nbr_oslot.get<int>(), ws.nbr_otab.get<uint16_t>(),
                 ws.atom_home.get<CellAtom>(), cap, 0);
   }
-  // edge capacity left by earlier builds, in records of T
+  // edge capacity left by earlier builds: the host-sized one, in records of T
+  // (outside capture the edge buffers are widened to it first, as after a
+  // change of precision)
+  if (!ws.capture && ws.nbr_known && ws.edge_cap > 0) {
+    ws.rec.ensure(sizeof(Rec<T>) * ws.edge_cap);
+    ws.in_edges.ensure(sizeof(int) * ws.edge_cap);
+    ws.in_pos.ensure(sizeof(int) * ws.edge_cap);
+  }
   const long ecap = (long)std::min({ws.rec.bytes / sizeof(Rec<T>), ws.in_edges.bytes / sizeof(int),
-                                    ws.in_pos.bytes / sizeof(int), (size_t)INT32_MAX - 1});
+                                    ws.in_pos.bytes / sizeof(int), (size_t)INT32_MAX - 1,
+                                    (size_t)(ws.edge_cap > 0 ? ws.edge_cap : INT32_MAX - 1)});
   if (ws.capture && !(ws.nbr_known && ecap > 0))
     throw Error(SUS2_E_INVALID, "capture-safe neighbour build needs the edge capacity of an earlier call");
   if ((ws.nbr_optimistic || ws.capture) && ws.nbr_known && ecap > 0) {
@@ -1132,6 +1140,7 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
   ws.nbr_known = true;
   // edge buffers with 1/8 headroom, so that later builds of similar frames fit
   const long ebuf = std::max<long>(total + total / 8, 1);
+  ws.edge_cap = ebuf;
   ws.rec.ensure(sizeof(Rec<T>) * ebuf);
   ws.in_edges.ensure(sizeof(int) * ebuf);
   ws.in_pos.ensure(sizeof(int) * ebuf);

@@ -120,6 +120,11 @@ struct Workspace {
   // training objective: EFS outputs, loss adjoints and per-frame losses (FP64)
   DevBuf lE, lF, lV, lew, lfw, lsw, lloss;
   long nedges = 0;
+  // edge capacity (edges, not bytes) fixed by the last host-sized build: the
+  // per-edge buffers of either precision are sized from it, so that a build
+  // sized from earlier calls neither creeps past their slack nor overflows
+  // after a change of precision (FP64 records are twice as wide)
+  long edge_cap = 0;
   long max_frame_atoms = 0;  // upper bound on the atoms of one frame of the current batch
   DevBuf fin_part;           // per-chunk partial frame sums of the frame reduction
   int64_t* host_total = nullptr;  // pinned [4]
