@@ -72,6 +72,19 @@ int sus2_coeff_set(sus2_model* m, const double* theta /* host [ncoeff] */);
 /* offsets[13]: shift scal rad grad gscal alpha gtype W gw ow species read ncoeff (-1 = absent) */
 int sus2_coeff_offsets(const sus2_model* m, int* offsets);
 
+/* --- model shapes ------------------------------------------------------------
+ * No reference counterpart: the reference's kernels take any K and B at run
+ * time (interfaces/lammps/ML-SUS2/sus2_sl_model.cpp:276).  Here kernels are
+ * compile-time instances per shape (L, K, B = radial basis size, scaling_map
+ * mode 0 pair / 1 L / 2 K / 3 LK); shapes without a prebuilt instance are
+ * compiled with hipcc at their first evaluation into a cached shape library
+ * (SUS2_JIT_DIR, default _jit/ beside the library; SUS2_JIT=0 forbids the
+ * compiler run).  sus2_shape_build does that ahead of time (no GPU needed)
+ * and writes the shape library's path into path[cap] ("" for a prebuilt
+ * shape).  Limits: L <= 6, K (L + 1) <= 64, B <= 16; other shapes fail with
+ * SUS2_E_UNSUPPORTED. */
+int sus2_shape_build(int L, int K, int B, int scaling_mode, char* path, int cap);
+
 /* --- frame-batch EFS (CalcEFS over a batch; dev pointers) -------------------
  * frame_off [nframes+1] atom offsets (int64), types [natoms] model species,
  * pos [natoms*3] Cartesian, cell [nframes*9] lattice rows, periodic [nframes]

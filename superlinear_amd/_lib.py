@@ -38,6 +38,7 @@ SIGNATURES = {
     "sus2_coeff_get": (_i, [_vp, _dp]),
     "sus2_coeff_set": (_i, [_vp, _dp]),
     "sus2_coeff_offsets": (_i, [_vp, _ip]),
+    "sus2_shape_build": (_i, [_i, _i, _i, _i, ctypes.c_char_p, _i]),
     "sus2_efs": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sus2_efs_host": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sus2_efs_noalloc": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

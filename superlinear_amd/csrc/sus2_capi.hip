@@ -168,6 +168,18 @@ int sus2_model_dims(const sus2_model* m, int* d) {
 }
 
 double sus2_model_cutoff(const sus2_model* m) { return m ? m->model.rc : 0.0; }
+
+int sus2_shape_build(int L, int K, int B, int scaling_mode, char* path, int cap) {
+  return guard([&] {
+    if (!path || cap <= 0) throw sus2::Error(SUS2_E_INVALID, "null argument");
+    const sus2::ShapeOps& ops = sus2::shape_ops(L, K, B, scaling_mode);  // builds and loads when not prebuilt
+    (void)ops;
+    std::string p = sus2::is_prebuilt_shape(L, K, B, scaling_mode) ? std::string()
+                                                                     : sus2::shape_library_path(L, K, B, scaling_mode);
+    if ((int)p.size() + 1 > cap) throw sus2::Error(SUS2_E_RANGE, "path buffer too small");
+    std::memcpy(path, p.c_str(), p.size() + 1);
+  });
+}
 int sus2_coeff_count(const sus2_model* m) { return m ? m->model.ncoeff : -1; }
 
 int sus2_coeff_get(const sus2_model* m, double* theta) {

@@ -135,4 +135,43 @@ template <class Cf>
 void run_grad_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const GradWeights& gw, double* grad,
                     double scale, bool add, bool reuse_neighbors, hipStream_t st);
 
+// Host entry points of one model shape (L, K, B, warp-sharing mode), index 0
+// FP32 and 1 FP64.  The shapes of SUS2_FOR_EACH_SHAPE are compiled into the
+// library; any other shape's are compiled at first use into a shape library
+// that is loaded beside it (shape_ops).
+struct ShapeOps {
+  void (*efs[2])(const DevModelSet&, Workspace&, const Batch&, const EfsOut&, hipStream_t);
+  void (*grad[2])(const DevModelSet&, Workspace&, const Batch&, const GradWeights&, double*, double, bool, bool,
+                  hipStream_t);
+  void (*pair[2])(int, const DevModelSet&, Workspace&, const PairView&, hipStream_t);
+};
+// Interface version between the library and its shape libraries.
+constexpr int SHAPE_ABI = 1;
+
+template <int L, int K, int B, int SM>
+ShapeOps make_shape_ops() {
+  using C32 = Cfg<float, L, K, B, SM>;
+  using C64 = Cfg<double, L, K, B, SM>;
+  ShapeOps o;
+  o.efs[0] = &run_efs_shape<C32>;
+  o.efs[1] = &run_efs_shape<C64>;
+  o.grad[0] = &run_grad_shape<C32>;
+  o.grad[1] = &run_grad_shape<C64>;
+  o.pair[0] = &pair_stage_shape<C32>;
+  o.pair[1] = &pair_stage_shape<C64>;
+  return o;
+}
+
+// The entry points of a model's shape: prebuilt, from a shape library loaded
+// earlier, or compiled now (hipcc, cached on disk); throws SUS2_E_UNSUPPORTED
+// when the shape cannot be built.
+const ShapeOps& shape_ops(int L, int K, int B, int SM);
+inline const ShapeOps& shape_ops(const Model& m) { return shape_ops(m.L, m.K, m.B, m.smode); }
+bool is_prebuilt_shape(int L, int K, int B, int SM);
+// Cache path of a shape library, and its compilation when absent (no GPU use).
+std::string shape_library_path(int L, int K, int B, int SM);
+std::string build_shape_library(int L, int K, int B, int SM);
+template <class T>
+constexpr int prec_index() { return sizeof(T) == 8 ? 1 : 0; }
+
 }  // namespace sus2

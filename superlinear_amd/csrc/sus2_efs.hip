@@ -1295,23 +1295,9 @@ template void build_pair_lists<float>(Workspace&, int, int, long, const int32_t*
 template void build_pair_lists<double>(Workspace&, int, int, long, const int32_t*, const int32_t*, const double*,
                                        const int32_t*, int*, hipStream_t);
 
-#define SUS2_DECLARE_PAIR(LL, KK, BB, SM)                                                                        \
-  extern template void pair_stage_shape<Cfg<float, LL, KK, BB, SM>>(int, const DevModelSet&, Workspace&,        \
-                                                                    const PairView&, hipStream_t);              \
-  extern template void pair_stage_shape<Cfg<double, LL, KK, BB, SM>>(int, const DevModelSet&, Workspace&,       \
-                                                                     const PairView&, hipStream_t);
-SUS2_FOR_EACH_SHAPE(SUS2_DECLARE_PAIR)
-#undef SUS2_DECLARE_PAIR
-
 template <class T>
 void pair_stage_t(int stage, const DevModelSet& dm, Workspace& ws, const PairView& pv, hipStream_t st) {
-  const int L = dm.host->L, K = dm.host->K, B = dm.host->B, SM = dm.host->smode;
-#define SUS2_SHAPE(LL, KK, BB, SMM) \
-  if (L == LL && K == KK && B == BB && SM == SMM) return pair_stage_shape<Cfg<T, LL, KK, BB, SMM>>(stage, dm, ws, pv, st);
-  SUS2_FOR_EACH_SHAPE(SUS2_SHAPE)
-#undef SUS2_SHAPE
-  throw Error(SUS2_E_UNSUPPORTED, "no kernel instance for L=" + std::to_string(L) + " K=" + std::to_string(K) +
-                                      " B=" + std::to_string(B) + " scaling_map mode " + std::to_string(SM));
+  shape_ops(*dm.host).pair[prec_index<T>()](stage, dm, ws, pv, st);
 }
 
 void pair_stage_dispatch(int stage, int precision, const DevModelSet& dm, Workspace& ws, const PairView& pv,
@@ -1322,24 +1308,9 @@ void pair_stage_dispatch(int stage, int precision, const DevModelSet& dm, Worksp
   ws.prof.collect_ready();
 }
 
-#define SUS2_DECLARE(LL, KK, BB, SM)                                                                  \
-  extern template void run_efs_shape<Cfg<float, LL, KK, BB, SM>>(const DevModelSet&, Workspace&, const Batch&, \
-                                                                 const EfsOut&, hipStream_t);               \
-  extern template void run_efs_shape<Cfg<double, LL, KK, BB, SM>>(const DevModelSet&, Workspace&, const Batch&, \
-                                                                  const EfsOut&, hipStream_t);
-SUS2_FOR_EACH_SHAPE(SUS2_DECLARE)
-#undef SUS2_DECLARE
-
 template <class T>
 void run_efs(const DevModelSet& dm, Workspace& ws, const Batch& b, const EfsOut& out, hipStream_t st) {
-  const int L = dm.host->L, K = dm.host->K, B = dm.host->B, SM = dm.host->smode;
-#define SUS2_SHAPE(LL, KK, BB, SMM) \
-  if (L == LL && K == KK && B == BB && SM == SMM) return run_efs_shape<Cfg<T, LL, KK, BB, SMM>>(dm, ws, b, out, st);
-  SUS2_FOR_EACH_SHAPE(SUS2_SHAPE)
-#undef SUS2_SHAPE
-  throw Error(SUS2_E_UNSUPPORTED, "no kernel instance for L=" + std::to_string(L) + " K=" + std::to_string(K) +
-                                      " B=" + std::to_string(B) + " scaling_map mode " + std::to_string(SM) +
-                                      " (instances are listed in SUS2_FOR_EACH_SHAPE)");
+  shape_ops(*dm.host).efs[prec_index<T>()](dm, ws, b, out, st);
 }
 
 void efs_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const EfsOut& out,

@@ -11,27 +11,10 @@
 
 namespace sus2 {
 
-#define SUS2_DECLARE_GRAD(LL, KK, BB, SM)                                                                         \
-  extern template void run_grad_shape<Cfg<float, LL, KK, BB, SM>>(const DevModelSet&, Workspace&, const Batch&,  \
-                                                                  const GradWeights&, double*, double, bool, bool, \
-                                                                  hipStream_t);                                   \
-  extern template void run_grad_shape<Cfg<double, LL, KK, BB, SM>>(const DevModelSet&, Workspace&, const Batch&, \
-                                                                   const GradWeights&, double*, double, bool,     \
-                                                                   bool, hipStream_t);
-SUS2_FOR_EACH_SHAPE(SUS2_DECLARE_GRAD)
-#undef SUS2_DECLARE_GRAD
-
 template <class T>
 void run_grad(const DevModelSet& dm, Workspace& ws, const Batch& b, const GradWeights& gw, double* grad,
               double scale, bool add, bool reuse, hipStream_t st) {
-  const int L = dm.host->L, K = dm.host->K, B = dm.host->B, SM = dm.host->smode;
-#define SUS2_SHAPE(LL, KK, BB, SMM)                                                                       \
-  if (L == LL && K == KK && B == BB && SM == SMM)                                                         \
-    return run_grad_shape<Cfg<T, LL, KK, BB, SMM>>(dm, ws, b, gw, grad, scale, add, reuse, st);
-  SUS2_FOR_EACH_SHAPE(SUS2_SHAPE)
-#undef SUS2_SHAPE
-  throw Error(SUS2_E_UNSUPPORTED, "no kernel instance for L=" + std::to_string(L) + " K=" + std::to_string(K) +
-                                      " B=" + std::to_string(B) + " scaling_map mode " + std::to_string(SM));
+  shape_ops(*dm.host).grad[prec_index<T>()](dm, ws, b, gw, grad, scale, add, reuse, st);
 }
 
 void grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* ene_w,

@@ -269,3 +269,27 @@ def test_bfgs_rosenbrock(optim_driver):
     assert head["converged"] == "1"
     assert np.allclose(x, [1.0, 1.0], atol=1e-8)
     assert int(head["iters"]) < 100
+
+
+def test_shape_build_prebuilt_cached_and_limits(monkeypatch):
+    """sus2_shape_build: prebuilt shapes need no shape library (""), the
+    shapes compiled by build() are found in the cache (SUS2_JIT=0: no compiler
+    run here), and shapes beyond the kernels' limits or absent from the cache
+    with compilation disabled fail with SUS2_E_UNSUPPORTED."""
+    import ctypes
+    import os
+    from superlinear_amd import _lib
+    from superlinear_amd.build import EXTRA_SHAPES
+    L = _lib.lib()
+    monkeypatch.setenv("SUS2_JIT", "0")
+    buf = ctypes.create_string_buffer(4096)
+    assert L.sus2_shape_build(3, 4, 10, 2, buf, len(buf)) == 0 and buf.value == b""
+    for shape in EXTRA_SHAPES:
+        assert L.sus2_shape_build(*shape, buf, len(buf)) == 0, L.sus2_last_error()
+        p = buf.value.decode()
+        assert os.path.exists(p) and "sus2shape_l{}k{}b{}m{}_".format(*shape) in p
+    assert L.sus2_shape_build(7, 2, 8, 2, buf, len(buf)) == 3  # L > 6
+    assert L.sus2_shape_build(6, 10, 8, 2, buf, len(buf)) == 3  # K (L + 1) > 64
+    assert L.sus2_shape_build(2, 3, 17, 2, buf, len(buf)) == 3  # B > 16
+    assert L.sus2_shape_build(2, 3, 7, 1, buf, len(buf)) == 3  # not cached, compiler disabled
+    assert b"SUS2_JIT=0" in L.sus2_last_error()

@@ -103,6 +103,28 @@ def test_oracle_matches_compiled_reference(oracle_lib, mname, species, seed):
         close(r["site"], ref["site_energies"], 1e-10)
 
 
+@pytest.mark.parametrize("gate", [True, False], ids=["effect", "static"])
+def test_oracle_matches_compiled_reference_lk_shape(oracle_lib, tmp_path, gate):
+    """The shape of the runtime-compiled GPU kernels (L2/K3, radial basis 8,
+    one warp block per channel: scaling_map LK) against the reference engine."""
+    from superlinear_amd import workloads as W
+    from superlinear_amd.topology import write_model
+    if oracle_lib.ref_binary() is None:
+        pytest.skip("reference engine not built (reference tree absent)")
+    path = tmp_path / "lk.sus2"
+    kw = dict(gate=True, gate_rank=2, gate_pruning="q-total", outer_pruning="same-k", species=2) if gate else \
+        dict(gate=False, outer_pruning="q-total", species=1)
+    write_model(str(path), lmax=2, kmax=3, body_order=4, outer_rank=2, rb_size=8, scaling_map="LK", cutoff=5.0,
+                seed=91 if gate else 92, **kw)
+    species = 2 if gate else 1
+    for frame in W.periodic_frames(2, 12, 30, species, 0.12, seed=93):  # the engine's checker takes periodic cells
+        ref = _ref_efs(path, frame)
+        r = oracle_lib.OracleModel(path).efs([frame])
+        close(r["energy"][0], ref["energy"], 1e-10)
+        close(r["forces"], ref["forces"], 1e-10)
+        close(voigt_to_matrix(r["virial"][0]), ref["virial"], 1e-10)
+
+
 def _energy(m, frame):
     return m.efs([frame])["energy"][0]
 
