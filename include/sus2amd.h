@@ -141,10 +141,12 @@ int sus2_loss_grad(sus2_model* m, int precision, int nframes, int natoms, const 
                    double wS, double* loss_out, double* grad, void* stream);
 
 /* --- training driver ----------------------------------------------------------
- * L-BFGS (host FP64, `history` pairs, Armijo backtracking) minimising the
- * objective of sus2_loss_grad over one device-resident batch; replaces the
- * BFGS/L-BFGS loop of MTPR_trainer::Train (dev_src/mtpr_trainer.cpp:4685-4760,
- * src/common/lbfgs.h:114).  The model's coefficients end at the last accepted
+ * L-BFGS (host FP64, `history` pairs; the reference's update rules: Armijo
+ * halving with adaptive initial step, Powell-damped curvature pairs, curvature
+ * skips, H0 = gamma I) minimising the objective of sus2_loss_grad over one
+ * device-resident batch; replaces the L-BFGS loop of MTPR_trainer::Train
+ * (dev_src/mtpr_trainer.cpp:4685-4760, src/common/lbfgs.h:114,
+ * src/common/lbfgs.cpp:129-327).  The model's coefficients end at the last accepted
  * point.  loss_trace [max_iter+1] (host, may be NULL) receives the objective
  * at the start and after each accepted step; *iters_out the steps taken.
  * Stops when max |grad| <= gtol, after max_iter steps or when the line search

@@ -363,8 +363,8 @@ int sus2_train_lbfgs(sus2_model* m, int precision, int nframes, int natoms, cons
     const sus2::Batch b = make_batch(nframes, natoms, frame_off, types, pos, cell, periodic);
     const int P = m->model.ncoeff;
     Dev gdev(sizeof(double) * P);
-    std::vector<double> x = m->model.theta, g(P), xn(P), gn(P), p(P), q(P);
-    auto eval = [&](const std::vector<double>& th, std::vector<double>& grad) {
+    std::vector<double> x = m->model.theta;
+    const sus2::Objective obj = [&](const std::vector<double>& th, std::vector<double>& grad) {
       std::memcpy(m->model.theta.data(), th.data(), sizeof(double) * P);
       m->dev.upload_theta(m->model);
       double f = 0;
@@ -375,67 +375,10 @@ int sus2_train_lbfgs(sus2_model* m, int precision, int nframes, int natoms, cons
       d2h(grad.data(), gdev.get<double>(), sizeof(double) * P);
       return f;
     };
-    auto dot = [&](const std::vector<double>& a, const std::vector<double>& c) {
-      double s = 0;
-      for (int k = 0; k < P; ++k) s += a[k] * c[k];
-      return s;
-    };
-    std::vector<std::vector<double>> S, Y;
-    std::vector<double> rho;
-    double f = eval(x, g);
-    if (loss_trace) loss_trace[0] = f;
-    int it = 0;
-    for (; it < max_iter; ++it) {
-      double gmax = 0;
-      for (double v : g) gmax = std::max(gmax, std::fabs(v));
-      if (gmax <= gtol) break;
-      // two-loop recursion: p = -H g
-      q = g;
-      std::vector<double> alpha(S.size());
-      for (int k = (int)S.size() - 1; k >= 0; --k) {
-        alpha[k] = rho[k] * dot(S[k], q);
-        for (int t = 0; t < P; ++t) q[t] -= alpha[k] * Y[k][t];
-      }
-      double gamma = 1.0;
-      if (!S.empty()) gamma = dot(S.back(), Y.back()) / dot(Y.back(), Y.back());
-      for (int t = 0; t < P; ++t) q[t] *= gamma;
-      for (size_t k = 0; k < S.size(); ++k) {
-        const double beta = rho[k] * dot(Y[k], q);
-        for (int t = 0; t < P; ++t) q[t] += (alpha[k] - beta) * S[k][t];
-      }
-      for (int t = 0; t < P; ++t) p[t] = -q[t];
-      double pg = dot(p, g);
-      if (!(pg < 0)) {  // not a descent direction: restart from steepest descent
-        S.clear(); Y.clear(); rho.clear();
-        const double gn2 = dot(g, g);
-        for (int t = 0; t < P; ++t) p[t] = -g[t] / std::sqrt(gn2);
-        pg = dot(p, g);
-      }
-      double step = S.empty() ? std::min(1.0, 1.0 / std::max(std::sqrt(dot(g, g)), 1e-300)) : 1.0;
-      double fn = 0;
-      bool ok = false;
-      for (int bt = 0; bt < 40; ++bt) {
-        for (int t = 0; t < P; ++t) xn[t] = x[t] + step * p[t];
-        fn = eval(xn, gn);
-        if (std::isfinite(fn) && fn <= f + 1e-4 * step * pg) { ok = true; break; }
-        step *= 0.5;
-      }
-      if (!ok) {  // line search failed: keep the last accepted point
-        eval(x, g);
-        break;
-      }
-      std::vector<double> s(P), y(P);
-      for (int t = 0; t < P; ++t) { s[t] = xn[t] - x[t]; y[t] = gn[t] - g[t]; }
-      const double sy = dot(s, y);
-      if (sy > 1e-10 * std::sqrt(dot(s, s) * dot(y, y))) {
-        S.push_back(s); Y.push_back(y); rho.push_back(1.0 / sy);
-        if ((int)S.size() > history) { S.erase(S.begin()); Y.erase(Y.begin()); rho.erase(rho.begin()); }
-      }
-      x.swap(xn);
-      g.swap(gn);
-      f = fn;
-      if (loss_trace) loss_trace[it + 1] = f;
-    }
+    const sus2::LbfgsResult r = sus2::lbfgs_minimize(obj, x, max_iter, history, gtol);
+    const int it = r.iters;
+    if (loss_trace)
+      for (size_t k = 0; k < r.trace.size(); ++k) loss_trace[k] = r.trace[k];
     // leave the model at the final accepted point
     std::memcpy(m->model.theta.data(), x.data(), sizeof(double) * P);
     m->dev.upload_theta(m->model);

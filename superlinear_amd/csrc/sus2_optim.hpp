@@ -208,4 +208,136 @@ inline BfgsResult bfgs_minimize(const Objective& fn, std::vector<double>& x, int
   return res;
 }
 
+// L-BFGS with the update rules of the reference's LBFGS
+// (src/common/lbfgs.cpp:129-327, constants src/common/lbfgs.h:58,93-98):
+// direction from the two-loop recursion with H0 = gamma I (gamma = s.y / y.y
+// of the last accepted pair, clamped to [1e-12, 1e12]); a non-descent
+// direction clears the history and falls back to -g.  Armijo backtracking by
+// halving (c1 = 1e-4) from the next initial step: 1 after an accepted history
+// update, else the last step (times 1.5 when it needed no backtracking).  The
+// pair (s, y) is Powell-damped when s.y < 0.2 s.B0.s (B0 = I / gamma):
+//   y <- theta y + (1 - theta) B0 s,  theta = 0.8 s.B0.s / (s.B0.s - s.y),
+// and skipped unless s.y > 64 eps sqrt(s.B0.s y.H0.y) and its relative
+// curvature s.y / sqrt(s.B0.s y.H0.y) >= 1e-10.  The line search stops
+// (stagnates) when halving no longer changes the step or it falls below 1e-20.
+struct LbfgsResult {
+  int iters = 0, evals = 0, damped = 0, skipped = 0, resets = 0;
+  bool converged = false;
+  std::vector<double> trace;  // objective at the start and after each accepted step
+  std::vector<double> gammas;  // H0 scale after each accepted step
+};
+
+inline LbfgsResult lbfgs_minimize(const Objective& fn, std::vector<double>& x, int max_iter, int history,
+                                  double gtol) {
+  using optim_detail::dot;
+  constexpr double c1 = 1e-4, powell = 0.2, rel_tol = 1e-10, min_step = 1e-20, max_step = 1e20, growth = 1.5;
+  const size_t n = x.size();
+  LbfgsResult res;
+  std::vector<std::vector<double>> S, Y;
+  std::vector<double> rho, alpha;
+  double gamma = 1.0, next_step = 1.0;
+  std::vector<double> g(n), p(n), q(n), xn(n), gn(n), s(n), y(n);
+  double f = fn(x, g);
+  ++res.evals;
+  if (!std::isfinite(f)) throw std::runtime_error("L-BFGS: non-finite objective at the starting point");
+  res.trace.push_back(f);
+  for (int it = 0; it < max_iter; ++it) {
+    double gmax = 0;
+    for (double v : g) gmax = std::max(gmax, std::fabs(v));
+    if (gmax <= gtol) {
+      res.converged = true;
+      break;
+    }
+    // two-loop recursion: p = -H g
+    q = g;
+    alpha.assign(S.size(), 0.0);
+    for (int k = (int)S.size() - 1; k >= 0; --k) {
+      alpha[k] = rho[k] * dot(S[k], q);
+      for (size_t t = 0; t < n; ++t) q[t] -= alpha[k] * Y[k][t];
+    }
+    for (size_t t = 0; t < n; ++t) q[t] *= gamma;
+    for (size_t k = 0; k < S.size(); ++k) {
+      const double beta = rho[k] * dot(Y[k], q);
+      for (size_t t = 0; t < n; ++t) q[t] += (alpha[k] - beta) * S[k][t];
+    }
+    for (size_t t = 0; t < n; ++t) p[t] = -q[t];
+    double pg = dot(p, g);
+    if (!(pg < 0)) {  // not a descent direction: clear the history, steepest descent
+      S.clear(); Y.clear(); rho.clear();
+      gamma = 1.0;
+      ++res.resets;
+      for (size_t t = 0; t < n; ++t) p[t] = -g[t];
+      pg = dot(p, g);
+      if (!(pg < 0)) break;
+    }
+    // Armijo backtracking by halving
+    double a = std::min(std::max(next_step, min_step), max_step), fn_new = 0;
+    int backtracks = 0;
+    bool ok = false;
+    for (;;) {
+      for (size_t t = 0; t < n; ++t) xn[t] = x[t] + a * p[t];
+      fn_new = fn(xn, gn);
+      ++res.evals;
+      if (std::isfinite(fn_new) && fn_new <= f + c1 * a * pg) {
+        ok = true;
+        break;
+      }
+      const double a2 = 0.5 * a;
+      if (std::fabs(a2 - a) <= 64.0 * std::numeric_limits<double>::epsilon() * std::max(1.0, a) || a2 < min_step)
+        break;  // stagnated
+      a = a2;
+      ++backtracks;
+    }
+    if (!ok) break;  // x, g, f stay at the last accepted point
+    next_step = std::min(std::max(backtracks == 0 ? a * growth : a, min_step), max_step);
+    for (size_t t = 0; t < n; ++t) {
+      s[t] = xn[t] - x[t];
+      y[t] = gn[t] - g[t];
+    }
+    // curvature pair, Powell-damped against B0 = I / gamma
+    const double sy_raw = dot(s, y), sBs = dot(s, s) / gamma, yHy_raw = gamma * dot(y, y);
+    bool accepted = false;
+    if (sBs > 0 && yHy_raw > 0 && std::isfinite(sBs * yHy_raw)) {
+      std::vector<double> yd = y;
+      bool damped = false;
+      if (sy_raw < powell * sBs) {
+        const double den = sBs - sy_raw;
+        if (den > 0 && std::isfinite(den)) {
+          const double theta = std::min(1.0, std::max(0.0, (1.0 - powell) * sBs / den));
+          for (size_t t = 0; t < n; ++t) yd[t] = theta * y[t] + (1.0 - theta) * s[t] / gamma;
+          damped = true;
+        } else {
+          yd.clear();
+        }
+      }
+      if (!yd.empty()) {
+        const double sy = dot(s, yd), yHy = gamma * dot(yd, yd);
+        const double scale = std::sqrt(sBs * yHy);
+        if (yHy > 0 && std::isfinite(yHy) && sy > 64.0 * std::numeric_limits<double>::epsilon() * scale &&
+            sy / scale >= rel_tol && std::isfinite(1.0 / sy)) {
+          if ((int)S.size() == history) {
+            S.erase(S.begin()); Y.erase(Y.begin()); rho.erase(rho.begin());
+          }
+          S.push_back(s);
+          Y.push_back(yd);
+          rho.push_back(1.0 / sy);
+          const double gc = sy / dot(yd, yd);
+          if (gc > 0 && std::isfinite(gc)) gamma = std::min(1e12, std::max(1e-12, gc));
+          accepted = true;
+          res.damped += damped ? 1 : 0;
+        }
+      }
+    }
+    if (accepted) next_step = 1.0;
+    else ++res.skipped;
+    x.swap(xn);
+    g.swap(gn);
+    f = fn_new;
+    res.iters = it + 1;
+    res.trace.push_back(f);
+    res.gammas.push_back(gamma);
+  }
+  return res;
+}
+
 }  // namespace sus2

@@ -271,6 +271,48 @@ def test_bfgs_rosenbrock(optim_driver):
     assert int(head["iters"]) < 100
 
 
+def test_lbfgs_quadratic_and_rosenbrock(optim_driver, tmp_path):
+    """L-BFGS (reference update rules) on convex quadratics: with condition
+    number 10 it converges (max |g| <= 1e-9) to A^-1 b for a short and a full
+    history; with condition number 1e3, where the reference's damping against
+    B0 = I / gamma also fires on convex steps, every accepted step still lowers
+    f and the iterate gets within 1e-6 of A^-1 b.  It minimises the
+    Rosenbrock function."""
+    n = 12
+    for cond, hists, tol, conv in ((1, (3, 20), 1e-8, True), (3, (20,), 1e-6, False)):
+        rng = np.random.default_rng(19)
+        q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+        A = q @ np.diag(np.logspace(0, cond, n)) @ q.T
+        b = rng.normal(size=n)
+        f = tmp_path / f"quad{cond}.txt"
+        f.write_text(f"{n}\n" + " ".join(repr(float(v)) for v in A.ravel()) + "\n" +
+                     " ".join(repr(float(v)) for v in b))
+        xs = np.linalg.solve(A, b)
+        for hist in hists:
+            head, x = _run_optim(optim_driver, "lbfgs-quad", f, hist)  # exit status 3 if f ever rose
+            if conv:
+                assert head["converged"] == "1" and head["damped"] == "0"
+            assert np.abs(x - xs).max() <= tol * (1 + np.abs(xs).max())
+    head, x = _run_optim(optim_driver, "lbfgs-rosen")
+    assert head["converged"] == "1"
+    assert np.allclose(x, [1.0, 1.0], atol=1e-7)
+
+
+def test_lbfgs_gamma_and_powell_damping(optim_driver):
+    """H0 = gamma I with gamma = s.y / y.y: on f = c x^2 / 2 (c = 1/2) the
+    first pair gives gamma = 1/c and the next unit step lands on the minimum.
+    From the concave middle of a double well the first pair has s.y < 0: it is
+    Powell-damped (not dropped) and the iteration still reaches a minimum
+    (|x| = 1, y = 0)."""
+    head, x = _run_optim(optim_driver, "lbfgs-1d")
+    assert float(head["gamma"]) == pytest.approx(2.0, rel=1e-14)
+    assert head["converged"] == "1" and int(head["iters"]) == 2 and x[0] == 0.0
+    head, x = _run_optim(optim_driver, "lbfgs-dw")
+    assert int(head["damped"]) >= 1
+    assert head["converged"] == "1"
+    assert abs(abs(x[0]) - 1.0) <= 1e-9 and abs(x[1]) <= 1e-9
+
+
 def test_shape_build_prebuilt_cached_and_limits(monkeypatch):
     """sus2_shape_build: prebuilt shapes need no shape library (""), the
     shapes compiled by build() are found in the cache (SUS2_JIT=0: no compiler
