@@ -1019,6 +1019,7 @@ __global__ __launch_bounds__(256) void k_frame_max(int nframes, const int64_t* _
 // Largest frame of the batch, read by polling the host report (no copy and
 // stream synchronisation on the call's critical path).
 static int64_t largest_frame(Workspace& ws, const Batch& b, hipStream_t st) {
+  if (b.nframes == 1) return b.natoms;  // one frame: no device round trip
   if (current_arena()) {  // allocation-free call: a device word and a plain copy
     ws.scan_tmp.ensure(8 * sizeof(int64_t));
     int64_t* word = ws.scan_tmp.get<int64_t>() + 7;
