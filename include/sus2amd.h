@@ -258,6 +258,18 @@ void sus2_graph_free(sus2_graph* g);
  * {1, 2, 4} x grid_mult in {0, 1, 2, 4}, candidate c = (c / 4, c % 4)) on the
  * caller's batch and keeps the fastest. */
 #define SUS2_TUNE_CANDIDATES 12
+/* Gate-adjoint messages of the batch EFS of Effect models: mode 0 has the
+ * outer edge backward write them per edge (E x F values, read by the outer
+ * gather); 2 writes them from a messages-only pass after a forces-only one
+ * (two lighter kernels instead of one register-bound one); 1 has the outer
+ * gather recompute them from the atom's own records and its neighbours'
+ * moment adjoints (no per-edge message buffer: 27 GB FP32 at 1.9e8 edges and
+ * F = 36); 3 (the default) is automatic: 2 from L = 4 on, where the combined
+ * pass is register-bound, else 0; -1 leaves the mode.  *current (may be NULL)
+ * receives the mode set.  Modes 0 and 2 give identical results, mode 1 agrees
+ * to rounding.  Pair-style stages always use mode 0 (ghost atoms have no
+ * records of their own). */
+int sus2_tune_messages(sus2_model* m, int precision, int mode, int* current);
 int sus2_tune_set(sus2_model* m, int precision, int eb_atoms, int grid_mult);
 int sus2_tune_get(sus2_model* m, int precision, int* eb_atoms, int* grid_mult);
 int sus2_autotune(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,

@@ -51,6 +51,7 @@ SIGNATURES = {
     "sus2_train_bfgs": (_i, [_vp, _i, _i, _i] + [_vp] * 8 + [_i, _d, _d, _d, _i, _d, _dp, ctypes.POINTER(_i), _vp]),
     "sus2_train_linear": (_i, [_vp, _i, _i, _i] + [_vp] * 8 + [_i, _d, _d, _d, _d, _dp, _vp]),
     "sus2_tune_set": (_i, [_vp, _i, _i, _i]),
+    "sus2_tune_messages": (_i, [_vp, _i, _i, _ip]),
     "sus2_tune_get": (_i, [_vp, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "sus2_autotune": (_i, [_vp, _i, _i, _i] + [_vp] * 5 + [_i, _vp, ctypes.POINTER(_i), _dp]),
     "sus2_efs_grad": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),

@@ -522,7 +522,21 @@ int sus2_tune_set(sus2_model* m, int precision, int eb_atoms, int grid_mult) {
     if (!m) throw sus2::Error(SUS2_E_INVALID, "null model");
     if (eb_atoms != 1 && eb_atoms != 2 && eb_atoms != 4) throw sus2::Error(SUS2_E_INVALID, "eb_atoms must be 1, 2 or 4");
     if (grid_mult < 0 || grid_mult > 64) throw sus2::Error(SUS2_E_INVALID, "grid_mult must be in [0, 64]");
-    tune_slot(m, precision) = sus2::EfsTune{eb_atoms, grid_mult};
+    sus2::EfsTune& t = tune_slot(m, precision);
+    t.eb_atoms = eb_atoms;
+    t.grid_mult = grid_mult;
+  });
+}
+
+int sus2_tune_messages(sus2_model* m, int precision, int mode, int* current) {
+  return guard([&] {
+    if (!m) throw sus2::Error(SUS2_E_INVALID, "null model");
+    if (mode < -1 || mode > 3)
+      throw sus2::Error(SUS2_E_INVALID,
+                        "mode must be -1 (query), 0 (combined), 1 (recompute), 2 (split) or 3 (automatic)");
+    sus2::EfsTune& t = tune_slot(m, precision);
+    if (mode >= 0) t.msg_recompute = mode == 3 ? -1 : mode;
+    if (current) *current = t.msg_recompute < 0 ? 3 : t.msg_recompute;
   });
 }
 
@@ -562,7 +576,9 @@ int sus2_autotune(sus2_model* m, int precision, int nframes, int natoms, const i
     sus2::EfsTune chosen = keep;
     try {
       for (int c = 0; c < SUS2_TUNE_CANDIDATES; ++c) {
-        const sus2::EfsTune t{kTuneEb[c / 4], kTuneGrid[c % 4]};
+        sus2::EfsTune t = keep;
+        t.eb_atoms = kTuneEb[c / 4];
+        t.grid_mult = kTuneGrid[c % 4];
         slot = t;
         sus2::efs_dispatch(m->dev, m->ws, precision, b, out, st);  // warm-up (sizes, compiles nothing)
         std::vector<double> ms(reps);

@@ -411,7 +411,7 @@ constexpr int edge_backward_waves() { return (sizeof(typename Cf::T) == 4 && Cf:
 // are contiguous, so two or four atoms' edges fill the lanes better than one
 // atom's ~90).  sA: [NA][KH] adjoints, sV: [NA][9][LR] lane partials, sXR:
 // [F][LR] gate-message columns; tab already loaded.
-template <class Cf, bool GATE, int NA, class T = typename Cf::T>
+template <class Cf, bool GATE, int NA, bool MSG = true, class T = typename Cf::T>
 __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const SpeciesTables<T>& tab, T* sA, T* sV,
                                                     T* sXR, int i0, int natoms, const int32_t* __restrict__ types,
                                                     const int* __restrict__ nbr_off, const Rec<T>* __restrict__ rec,
@@ -420,7 +420,11 @@ __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const Sp
                                                     T* __restrict__ uedge, T* __restrict__ fw) {
   constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, F = Cf::F, KH = Cf::KH;
   const int lane = threadIdx.x;
+  // gated outer pass: the neighbour's gate signal enters the radial weight;
+  // with MSG it also writes the gate-adjoint messages u_e (else the gather
+  // recomputes them)
   const bool hgate = !GATE && m.gated;
+  const bool hmsg = MSG && hgate;
   const int na = min(NA, natoms - i0);
   for (int c = lane; c < na * KH; c += WAVE) sA[c] = adj[(size_t)i0 * KH + c];
 #pragma unroll
@@ -473,10 +477,8 @@ __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const Sp
 #pragma unroll
     for (int mu = 0; mu < F; ++mu) {
       T x1 = x0j[mu];
-      if (hgate) {
-        x1 += alj[mu] * h[(size_t)j * F + mu];
-        sXR[mu * LR + lane] = x0[mu] * R[mu];
-      }
+      if (hgate) x1 += alj[mu] * h[(size_t)j * F + mu];
+      if (hmsg) sXR[mu * LR + lane] = x0[mu] * R[mu];
       const T wt = x0[mu] * x1;
       R[mu] *= wt;
       dR[mu] *= wt;
@@ -496,7 +498,7 @@ __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const Sp
         const T av = sAi[hh * KK + k];
         adR += av * dR[mu];
         aR += av * R[mu];
-        if (hgate) us[mu] += av * Y;
+        if (hmsg) us[mu] += av * Y;
       }
       sY += adR * Y;
       gx += aR * yx; gy += aR * yy; gz += aR * yz;
@@ -508,7 +510,7 @@ __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const Sp
     w[0] += g0; w[1] += g1; w[2] += g2;
     w[3] -= dx * g0; w[4] -= dy * g1; w[5] -= dz * g2;
     w[6] -= dx * g1; w[7] -= dx * g2; w[8] -= dy * g2;
-    if (hgate) {
+    if (hmsg) {
       T* ue = uedge + (size_t)slot * F;
 #pragma unroll
       for (int mu = 0; mu < F; ++mu) ue[mu] = sXR[mu * LR + lane] * us[mu];
@@ -538,7 +540,7 @@ __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const Sp
 
 // Edge backward over groups of NA centres: one group per workgroup, or
 // (PERSIST) a persistent grid striding over the groups.
-template <class Cf, bool GATE, int NA, bool PERSIST, class T = typename Cf::T>
+template <class Cf, bool GATE, int NA, bool PERSIST, bool MSG = true, class T = typename Cf::T>
 __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edge_backward(DModel<T> m, int natoms,
                                                         const int32_t* __restrict__ types,
                                                         const int* __restrict__ nbr_off,
@@ -555,13 +557,82 @@ __global__ __launch_bounds__(WAVE, (edge_backward_waves<Cf, GATE>())) void k_edg
   SpeciesTables<T> tab;
   load_species_tables<Cf>(m, GATE, sA + NA * Cf::KH, tab);
   if constexpr (!PERSIST) {
-    edge_backward_group<Cf, GATE, NA>(m, tab, sA, sV, sXR, blockIdx.x * NA, natoms, types, nbr_off, rec, h, adj,
-                                      in_pos, gedge_in, uedge, fw);
+    edge_backward_group<Cf, GATE, NA, MSG>(m, tab, sA, sV, sXR, blockIdx.x * NA, natoms, types, nbr_off, rec, h,
+                                           adj, in_pos, gedge_in, uedge, fw);
   } else {
     for (int i0 = blockIdx.x * NA; i0 < natoms; i0 += gridDim.x * NA) {
-      edge_backward_group<Cf, GATE, NA>(m, tab, sA, sV, sXR, i0, natoms, types, nbr_off, rec, h, adj, in_pos,
-                                        gedge_in, uedge, fw);
+      edge_backward_group<Cf, GATE, NA, MSG>(m, tab, sA, sV, sXR, i0, natoms, types, nbr_off, rec, h, adj, in_pos,
+                                             gedge_in, uedge, fw);
       __syncthreads();  // the next group overwrites sA and sV
+    }
+  }
+}
+
+// Gate-adjoint messages of the outer pass on their own (split form): for every
+// edge e = i -> j of the NA centres, u_e,mu = tau_{t_i,mu} R_mu(r; t_i, t_j)
+// sum_m abar_i[k,l,m] Y_lm(d_e) at the edge's incoming slot -- the values the
+// combined pass writes, formed by the same operations in the same order, but
+// from radial and harmonic values only (no derivatives, no forces), so the
+// force pass and this one each hold far fewer registers than the combined one.
+template <class Cf, int NA, class T = typename Cf::T>
+__device__ __forceinline__ void edge_messages_group(const DModel<T>& m, const SpeciesTables<T>& tab, T* sA, int i0,
+                                                    int natoms, const int32_t* __restrict__ types,
+                                                    const int* __restrict__ nbr_off, const Rec<T>* __restrict__ rec,
+                                                    const T* __restrict__ adj, const int* __restrict__ in_pos,
+                                                    T* __restrict__ uedge) {
+  constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, F = Cf::F, KH = Cf::KH;
+  const int lane = threadIdx.x;
+  const int na = min(NA, natoms - i0);
+  for (int c = lane; c < na * KH; c += WAVE) sA[c] = adj[(size_t)i0 * KH + c];
+  int eb[NA + 1], ta[NA];
+#pragma unroll
+  for (int a = 0; a <= NA; ++a) eb[a] = nbr_off[i0 + min(a, na)];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) ta[a] = types[i0 + min(a, na - 1)];
+  __syncthreads();
+  for (int e = eb[0] + lane; e < eb[NA]; e += WAVE) {
+    int ai = 0;
+#pragma unroll
+    for (int a = 1; a < NA; ++a) ai += e >= eb[a];
+    int ti = ta[0];
+#pragma unroll
+    for (int a = 1; a < NA; ++a)
+      if (ai == a) ti = ta[a];
+    const T* sAi = sA + ai * KH;
+    const T* x0 = tab.x0 + ti * F;
+    const Rec<T> q = rec[e];
+    const int tj = rec_t(q);
+    const T dx = q.x, dy = q.y, dz = q.z;
+    const T r = dsqrt(dx * dx + dy * dy + dz * dz);
+    const T inv = T(1) / r;
+    T us[F];
+#pragma unroll
+    for (int mu = 0; mu < F; ++mu) us[mu] = T(0);
+    sh_stream_values<T, LL>(dx * inv, dy * inv, dz * inv, [&](int l, int mm, T Y) {
+      const int hh = hidx(l, mm);
+#pragma unroll
+      for (int k = 0; k < KK; ++k) us[k * L1 + l] += sAi[hh * KK + k] * Y;
+    });
+    T* ue = uedge + (size_t)in_pos[e] * F;
+    radial_each<Cf, false>(m, false, tab, ti * m.C + tj, r, [&](int mu, T R, T) { ue[mu] = (x0[mu] * R) * us[mu]; });
+  }
+}
+
+template <class Cf, int NA, bool PERSIST, class T = typename Cf::T>
+__global__ __launch_bounds__(WAVE) void k_edge_messages(DModel<T> m, int natoms, const int32_t* __restrict__ types,
+                                                        const int* __restrict__ nbr_off,
+                                                        const Rec<T>* __restrict__ rec, const T* __restrict__ adj,
+                                                        const int* __restrict__ in_pos, T* __restrict__ uedge) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* sA = reinterpret_cast<T*>(smem_raw);  // [NA][KH], then the outer pass's species tables
+  SpeciesTables<T> tab;
+  load_species_tables<Cf>(m, false, sA + NA * Cf::KH, tab);
+  if constexpr (!PERSIST) {
+    edge_messages_group<Cf, NA>(m, tab, sA, blockIdx.x * NA, natoms, types, nbr_off, rec, adj, in_pos, uedge);
+  } else {
+    for (int i0 = blockIdx.x * NA; i0 < natoms; i0 += gridDim.x * NA) {
+      edge_messages_group<Cf, NA>(m, tab, sA, i0, natoms, types, nbr_off, rec, adj, in_pos, uedge);
+      __syncthreads();  // the next group overwrites sA
     }
   }
 }
@@ -604,12 +675,21 @@ __device__ void gate_reverse_atom(const DModel<T>& m, int i, const T* __restrict
 // either runs the gate graph reverse at once (hbar_out == null) or stores
 // h_bar for a later reverse (the pair-style path, whose ghost adjoints are
 // communicated in between).  Ghost atoms (a >= nowned) start from a zeroed row.
-template <class Cf, bool GATE, class T = typename Cf::T>
+// RECOMP (batch path of an Effect model): the messages are not read from
+// uedge but formed here from the atom's own outgoing records -- each is the
+// reverse of one incoming edge j -> i, so
+//   u_{j->i,mu} = tau_{t_j,mu} R_mu(r; t_j, t_i) sum_m abar_j[k,l,m] Y_lm(x_i - x_j)
+// with the same operations, in the same order, as the edge backward of j
+// would have used; `tab` holds the outer pass's species tables.
+template <class Cf, bool GATE, bool RECOMP = false, class T = typename Cf::T>
 __device__ __forceinline__ void gather_atom(const DModel<T>& m, int i, const int32_t* __restrict__ types,
                                             const int* __restrict__ in_off, const T* __restrict__ gedge_in,
                                             const T* __restrict__ uedge, const T* __restrict__ vg,
-                                            T* __restrict__ gbar, T* __restrict__ hbar_out, T* __restrict__ fw) {
-  constexpr int F = Cf::F;
+                                            T* __restrict__ gbar, T* __restrict__ hbar_out, T* __restrict__ fw,
+                                            const Rec<T>* __restrict__ rec = nullptr,
+                                            const T* __restrict__ abar = nullptr,
+                                            const SpeciesTables<T>* tab = nullptr) {
+  constexpr int F = Cf::F, LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, KH = Cf::KH;
   const int lane = threadIdx.x;
   const bool hgate = !GATE && m.gated;
   T fx = T(0), fy = T(0), fz = T(0);
@@ -618,8 +698,9 @@ __device__ __forceinline__ void gather_atom(const DModel<T>& m, int i, const int
   for (int mu = 0; mu < F; ++mu) hb[mu] = T(0);
   // incoming slots (contiguous); narrow shapes take two batches per step so
   // that their loads are in flight together
-  constexpr int IN_STEP = F <= 16 ? 2 : 1;
+  constexpr int IN_STEP = (F <= 16 && !RECOMP) ? 2 : 1;
   const int k1 = in_off[i + 1];
+  const int ti = RECOMP ? types[i] : 0;
   for (int k0 = in_off[i] + lane; k0 < k1; k0 += IN_STEP * WAVE) {
 #pragma unroll
     for (int u = 0; u < IN_STEP; ++u) {
@@ -627,7 +708,27 @@ __device__ __forceinline__ void gather_atom(const DModel<T>& m, int i, const int
       if (k >= k1) break;
       const T* gi = gedge_in + 3 * (size_t)k;
       fx -= gi[0]; fy -= gi[1]; fz -= gi[2];
-      if (hgate) {
+      if constexpr (RECOMP) {
+        if (hgate) {
+          const Rec<T> q = rec[k];
+          const int j = rec_j(q), tj = rec_t(q);
+          const T dx = -q.x, dy = -q.y, dz = -q.z;  // j -> i
+          const T r = dsqrt(dx * dx + dy * dy + dz * dz);
+          const T inv = T(1) / r;
+          const T* aj = abar + (size_t)j * KH;
+          T S[F];
+#pragma unroll
+          for (int mu = 0; mu < F; ++mu) S[mu] = T(0);
+          sh_stream_values<T, LL>(dx * inv, dy * inv, dz * inv, [&](int l, int mm, T Y) {
+            const int hh = hidx(l, mm);
+#pragma unroll
+            for (int c = 0; c < KK; ++c) S[c * L1 + l] += aj[hh * KK + c] * Y;
+          });
+          const T* x0 = tab->x0 + tj * F;
+          radial_each<Cf, false>(m, false, *tab, tj * m.C + ti, r,
+                                 [&](int mu, T R, T) { hb[mu] += (x0[mu] * R) * S[mu]; });
+        }
+      } else if (hgate) {
         const T* ue = uedge + (size_t)k * F;
 #pragma unroll
         for (int mu = 0; mu < F; ++mu) hb[mu] += ue[mu];
@@ -671,10 +772,10 @@ __device__ __forceinline__ void gather_atom(const DModel<T>& m, int i, const int
         T* o = fw + (size_t)i * 9;
         o[0] += fx; o[1] += fy; o[2] += fz;
       }
-      const int ti = types[i];
+      const int ta = types[i];
       if (lane >= 3 && lane < 3 + F) {
         const int mu = lane - 3;
-        const T s = tot * m.alpha(ti, mu);
+        const T s = tot * m.alpha(ta, mu);
         if (hbar_out) hbar_out[(size_t)i * F + mu] = s;
         else hbs[mu] = s;
       }
@@ -702,10 +803,10 @@ __device__ __forceinline__ void gather_atom(const DModel<T>& m, int i, const int
     o[0] += fx; o[1] += fy; o[2] += fz;
   }
   if (!hgate) return;
-  const int ti = types[i];
+  const int ta = types[i];
 #pragma unroll
   for (int mu = 0; mu < F; ++mu) {
-    const T s = hb[mu] * m.alpha(ti, mu);
+    const T s = hb[mu] * m.alpha(ta, mu);
     if (hbar_out) {
       if (lane == 0) hbar_out[(size_t)i * F + mu] = s;
     } else if (lane == 0) {
@@ -718,17 +819,25 @@ __device__ __forceinline__ void gather_atom(const DModel<T>& m, int i, const int
 
 // Gather over atoms [0, natoms): one atom per workgroup, or (PERSIST) a
 // persistent grid striding over the atoms.
-template <class Cf, bool GATE, bool PERSIST, class T = typename Cf::T>
+template <class Cf, bool GATE, bool PERSIST, bool RECOMP = false, class T = typename Cf::T>
 __global__ __launch_bounds__(WAVE) void k_gather(DModel<T> m, int natoms, const int32_t* __restrict__ types,
                                                  const int* __restrict__ in_off, const T* __restrict__ gedge_in,
                                                  const T* __restrict__ uedge, const T* __restrict__ vg,
                                                  T* __restrict__ gbar, T* __restrict__ hbar_out,
-                                                 T* __restrict__ fw) {
+                                                 T* __restrict__ fw, const Rec<T>* __restrict__ rec,
+                                                 const T* __restrict__ abar) {
+  SpeciesTables<T> tab{};
+  if constexpr (RECOMP) {  // the outer pass's tables after the walk / transpose region
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    load_species_tables<Cf>(m, false, reinterpret_cast<T*>(smem_raw) + gather_region<Cf>(m), tab);
+    __syncthreads();
+  }
   if constexpr (!PERSIST) {
-    gather_atom<Cf, GATE>(m, blockIdx.x, types, in_off, gedge_in, uedge, vg, gbar, hbar_out, fw);
+    gather_atom<Cf, GATE, RECOMP>(m, blockIdx.x, types, in_off, gedge_in, uedge, vg, gbar, hbar_out, fw, rec, abar,
+                                  &tab);
   } else {
     for (int i = blockIdx.x; i < natoms; i += gridDim.x) {
-      gather_atom<Cf, GATE>(m, i, types, in_off, gedge_in, uedge, vg, gbar, hbar_out, fw);
+      gather_atom<Cf, GATE, RECOMP>(m, i, types, in_off, gedge_in, uedge, vg, gbar, hbar_out, fw, rec, abar, &tab);
       __syncthreads();  // the next atom reuses the LDS
     }
   }
@@ -749,12 +858,20 @@ __global__ __launch_bounds__(WAVE) void k_gate_reverse(DModel<T> m, const T* __r
   gate_reverse_atom<Cf>(m, i, vg, gbar, hbs, v, a, scratch);
 }
 
+// Elements of the gather's walk / transpose region: the gate graph walk, or
+// for F >= 16 the 3 + F lane sums transposed through the same region.
 template <class Cf>
-size_t gather_lds(const DModel<typename Cf::T>& m) {
-  const size_t walk = (size_t)2 * m.M + Cf::F + std::max(graph_scratch(m.gp), 1);
-  // F >= 16 also transposes the 3 + F lane sums through the same region
-  const size_t sums = Cf::F >= 16 ? (size_t)(3 + Cf::F) * LR : 0;
-  return sizeof(typename Cf::T) * std::max(walk, sums);
+__host__ __device__ inline int gather_region(const DModel<typename Cf::T>& m) {
+  const int sc = graph_scratch(m.gp);
+  const int walk = 2 * m.M + Cf::F + (sc > 1 ? sc : 1);
+  const int sums = Cf::F >= 16 ? (3 + Cf::F) * LR : 0;
+  return walk > sums ? walk : sums;
+}
+
+template <class Cf>
+size_t gather_lds(const DModel<typename Cf::T>& m, bool recomp = false) {
+  return sizeof(typename Cf::T) *
+         ((size_t)gather_region<Cf>(m) + (recomp ? (size_t)species_tables_size<Cf>(m.C) : 0));
 }
 
 template <class Cf>
@@ -829,7 +946,7 @@ void launch_pass_forward(const DevModelSet& dm, Workspace& ws, int n, const int3
               rec, h, vg, site, abar);
 }
 
-template <class Cf, bool GATE, int NA>
+template <class Cf, bool GATE, int NA, bool MSG>
 void launch_edge_backward_na(const DevModelSet& dm, Workspace& ws, int n, const int32_t* types, const int* off,
                              const Rec<typename Cf::T>* rec, const typename Cf::T* h, const typename Cf::T* adj,
                              const int* in_pos, typename Cf::T* gedge_in, typename Cf::T* uedge,
@@ -838,52 +955,89 @@ void launch_edge_backward_na(const DevModelSet& dm, Workspace& ws, int n, const 
   const DModel<T>& m = dm.get<T>();
   const size_t lb = backward_lds<Cf>(m, NA);
   if (n <= 0) return;
-  const char* name = GATE ? "edge_backward_gate" : "edge_backward_outer";
+  const char* name = GATE ? "edge_backward_gate" : MSG ? "edge_backward_outer" : "edge_backward_outer_nomsg";
   const int groups = (n + NA - 1) / NA;
   if (dm.tune_of<T>().grid_mult <= 0) {
-    allow_lds(k_edge_backward<Cf, GATE, NA, false>, lb);
-    SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_edge_backward<Cf, GATE, NA, false>), dim3(groups), dim3(WAVE), lb, m, n,
-                types, off, rec, h, adj, in_pos, gedge_in, uedge, fw);
+    allow_lds(k_edge_backward<Cf, GATE, NA, false, MSG>, lb);
+    SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_edge_backward<Cf, GATE, NA, false, MSG>), dim3(groups), dim3(WAVE), lb,
+                m, n, types, off, rec, h, adj, in_pos, gedge_in, uedge, fw);
     return;
   }
-  const unsigned g = tuned_grid(dm.tune_of<T>(), k_edge_backward<Cf, GATE, NA, true>, lb, groups);
-  SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_edge_backward<Cf, GATE, NA, true>), dim3(g), dim3(WAVE), lb, m, n, types,
-              off, rec, h, adj, in_pos, gedge_in, uedge, fw);
+  const unsigned g = tuned_grid(dm.tune_of<T>(), k_edge_backward<Cf, GATE, NA, true, MSG>, lb, groups);
+  SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_edge_backward<Cf, GATE, NA, true, MSG>), dim3(g), dim3(WAVE), lb, m, n,
+              types, off, rec, h, adj, in_pos, gedge_in, uedge, fw);
 }
 
-// Edge backward with the tuned centres per group (1, 2 or 4).
-template <class Cf, bool GATE>
+// Edge backward with the tuned centres per group (1, 2 or 4); MSG: the outer
+// pass writes the gate-adjoint messages (else the gather recomputes them).
+template <class Cf, bool GATE, bool MSG = true>
 void launch_edge_backward(const DevModelSet& dm, Workspace& ws, int n, const int32_t* types, const int* off,
                           const Rec<typename Cf::T>* rec, const typename Cf::T* h, const typename Cf::T* adj,
                           const int* in_pos, typename Cf::T* gedge_in, typename Cf::T* uedge, typename Cf::T* fw,
                           hipStream_t st) {
   switch (dm.tune_of<typename Cf::T>().eb_atoms) {
-    case 1: return launch_edge_backward_na<Cf, GATE, 1>(dm, ws, n, types, off, rec, h, adj, in_pos, gedge_in, uedge, fw, st);
-    case 4: return launch_edge_backward_na<Cf, GATE, 4>(dm, ws, n, types, off, rec, h, adj, in_pos, gedge_in, uedge, fw, st);
-    default: return launch_edge_backward_na<Cf, GATE, 2>(dm, ws, n, types, off, rec, h, adj, in_pos, gedge_in, uedge, fw, st);
+    case 1: return launch_edge_backward_na<Cf, GATE, 1, MSG>(dm, ws, n, types, off, rec, h, adj, in_pos, gedge_in, uedge, fw, st);
+    case 4: return launch_edge_backward_na<Cf, GATE, 4, MSG>(dm, ws, n, types, off, rec, h, adj, in_pos, gedge_in, uedge, fw, st);
+    default: return launch_edge_backward_na<Cf, GATE, 2, MSG>(dm, ws, n, types, off, rec, h, adj, in_pos, gedge_in, uedge, fw, st);
   }
 }
 
-template <class Cf, bool GATE>
+template <class Cf, int NA>
+void launch_edge_messages_na(const DevModelSet& dm, Workspace& ws, int n, const int32_t* types, const int* off,
+                             const Rec<typename Cf::T>* rec, const typename Cf::T* adj, const int* in_pos,
+                             typename Cf::T* uedge, hipStream_t st) {
+  using T = typename Cf::T;
+  const DModel<T>& m = dm.get<T>();
+  const size_t lm = sizeof(T) * ((size_t)NA * Cf::KH + species_tables_size<Cf>(m.C));
+  if (n <= 0) return;
+  const int groups = (n + NA - 1) / NA;
+  if (dm.tune_of<T>().grid_mult <= 0) {
+    allow_lds(k_edge_messages<Cf, NA, false>, lm);
+    SUS2_LAUNCH(ws, st, kname<Cf>("edge_messages"), (k_edge_messages<Cf, NA, false>), dim3(groups), dim3(WAVE), lm,
+                m, n, types, off, rec, adj, in_pos, uedge);
+    return;
+  }
+  const unsigned g = tuned_grid(dm.tune_of<T>(), k_edge_messages<Cf, NA, true>, lm, groups);
+  SUS2_LAUNCH(ws, st, kname<Cf>("edge_messages"), (k_edge_messages<Cf, NA, true>), dim3(g), dim3(WAVE), lm, m, n,
+              types, off, rec, adj, in_pos, uedge);
+}
+
+// Messages-only pass with the tuned centres per group.
+template <class Cf>
+void launch_edge_messages(const DevModelSet& dm, Workspace& ws, int n, const int32_t* types, const int* off,
+                          const Rec<typename Cf::T>* rec, const typename Cf::T* adj, const int* in_pos,
+                          typename Cf::T* uedge, hipStream_t st) {
+  switch (dm.tune_of<typename Cf::T>().eb_atoms) {
+    case 1: return launch_edge_messages_na<Cf, 1>(dm, ws, n, types, off, rec, adj, in_pos, uedge, st);
+    case 4: return launch_edge_messages_na<Cf, 4>(dm, ws, n, types, off, rec, adj, in_pos, uedge, st);
+    default: return launch_edge_messages_na<Cf, 2>(dm, ws, n, types, off, rec, adj, in_pos, uedge, st);
+  }
+}
+
+// RECOMP (batch path, outer pass of an Effect model): the messages are formed
+// from rec / abar instead of read from uedge; in_off is then the outgoing CSR.
+template <class Cf, bool GATE, bool RECOMP = false>
 void launch_gather(const DevModelSet& dm, Workspace& ws, int nall, const int32_t* types, const int* in_off,
                    const typename Cf::T* gedge_in, const typename Cf::T* uedge, const typename Cf::T* vg,
-                   typename Cf::T* gbar, typename Cf::T* hbar_out, typename Cf::T* fw, hipStream_t st) {
+                   typename Cf::T* gbar, typename Cf::T* hbar_out, typename Cf::T* fw, hipStream_t st,
+                   const Rec<typename Cf::T>* rec = nullptr, const typename Cf::T* abar = nullptr) {
   using T = typename Cf::T;
   const DModel<T>& m = dm.get<T>();
   // gated outer gathers with F >= 16 sum their messages through LDS; the gate
   // graph reverse of the outer gather needs its walk LDS
-  const size_t lg = (!GATE && m.gated) ? gather_lds<Cf>(m) : 0;
+  const bool outer_gated = !GATE && m.gated;
+  const size_t lg = outer_gated ? gather_lds<Cf>(m, RECOMP) : 0;
   if (nall <= 0) return;
-  const char* name = GATE ? "gather_gate" : "gather_outer";
+  const char* name = GATE ? "gather_gate" : RECOMP ? "gather_outer_recomp" : "gather_outer";
   if (dm.tune_of<T>().grid_mult <= 0) {
-    allow_lds(k_gather<Cf, GATE, false>, lg);
-    SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_gather<Cf, GATE, false>), dim3(nall), dim3(WAVE), lg, m, nall, types,
-                in_off, gedge_in, uedge, vg, gbar, hbar_out, fw);
+    allow_lds(k_gather<Cf, GATE, false, RECOMP>, lg);
+    SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_gather<Cf, GATE, false, RECOMP>), dim3(nall), dim3(WAVE), lg, m, nall,
+                types, in_off, gedge_in, uedge, vg, gbar, hbar_out, fw, rec, abar);
     return;
   }
-  const unsigned g = tuned_grid(dm.tune_of<T>(), k_gather<Cf, GATE, true>, lg, nall);
-  SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_gather<Cf, GATE, true>), dim3(g), dim3(WAVE), lg, m, nall, types, in_off,
-              gedge_in, uedge, vg, gbar, hbar_out, fw);
+  const unsigned g = tuned_grid(dm.tune_of<T>(), k_gather<Cf, GATE, true, RECOMP>, lg, nall);
+  SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_gather<Cf, GATE, true, RECOMP>), dim3(g), dim3(WAVE), lg, m, nall, types,
+              in_off, gedge_in, uedge, vg, gbar, hbar_out, fw, rec, abar);
 }
 
 template <class Cf>
@@ -900,7 +1054,13 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
   ws.site.ensure(sizeof(T) * n1);
   ws.fw.ensure(sizeof(T) * n1 * 9);
   ws.gedge_in.ensure(sizeof(T) * ne * 3);
-  if (m.gated) ws.uedge.ensure(sizeof(T) * ne * Cf::F);
+  // gate-adjoint messages: written by the outer edge backward (0), by a
+  // messages-only pass after a forces-only one (2), or recomputed by the
+  // gather (1, no per-edge message buffer)
+  const int mset = dm.tune_of<T>().msg_recompute;
+  const int mmode = !m.gated ? 0 : mset >= 0 ? mset : (Cf::LL >= 4 ? 2 : 0);
+  const bool recomp = mmode == 1;
+  if (m.gated && !recomp) ws.uedge.ensure(sizeof(T) * ne * Cf::F);
   const int* off = ws.nbr_off.get<int>();
   const Rec<T>* rec = ws.rec.get<Rec<T>>();
   T* h = ws.h.get<T>();
@@ -914,8 +1074,17 @@ void run_efs_shape(const DevModelSet& dm, Workspace& ws, const Batch& b, const E
     const int* ip = ws.in_pos.get<int>();
     if (m.gated) launch_pass_forward<Cf, true>(dm, ws, N, b.types, off, rec, h, vg, ws.site.get<T>(), abar, st);
     launch_pass_forward<Cf, false>(dm, ws, N, b.types, off, rec, h, vg, ws.site.get<T>(), abar, st);
-    launch_edge_backward<Cf, false>(dm, ws, N, b.types, off, rec, h, abar, ip, gi, ue, fw, st);
-    launch_gather<Cf, false>(dm, ws, N, b.types, off, gi, ue, vg, gbar, (T*)nullptr, fw, st);
+    if (recomp) {
+      launch_edge_backward<Cf, false, false>(dm, ws, N, b.types, off, rec, h, abar, ip, gi, ue, fw, st);
+      launch_gather<Cf, false, true>(dm, ws, N, b.types, off, gi, ue, vg, gbar, (T*)nullptr, fw, st, rec, abar);
+    } else if (mmode == 2) {
+      launch_edge_backward<Cf, false, false>(dm, ws, N, b.types, off, rec, h, abar, ip, gi, ue, fw, st);
+      launch_edge_messages<Cf>(dm, ws, N, b.types, off, rec, abar, ip, ue, st);
+      launch_gather<Cf, false>(dm, ws, N, b.types, off, gi, ue, vg, gbar, (T*)nullptr, fw, st);
+    } else {
+      launch_edge_backward<Cf, false>(dm, ws, N, b.types, off, rec, h, abar, ip, gi, ue, fw, st);
+      launch_gather<Cf, false>(dm, ws, N, b.types, off, gi, ue, vg, gbar, (T*)nullptr, fw, st);
+    }
     if (m.gated) {
       launch_edge_backward<Cf, true>(dm, ws, N, b.types, off, rec, h, gbar, ip, gi, ue, fw, st);
       launch_gather<Cf, true>(dm, ws, N, b.types, off, gi, ue, vg, gbar, (T*)nullptr, fw, st);

@@ -188,6 +188,13 @@ struct DevModelT {
 struct EfsTune {
   int eb_atoms = 2;
   int grid_mult = 0;
+  // gate-adjoint messages of the batch path of Effect models: 0 written per
+  // edge by the outer edge backward, 1 recomputed by the outer gather from the
+  // records and the neighbours' moment adjoints (no per-edge message buffer),
+  // 2 written per edge by a messages-only pass after a forces-only one; -1
+  // automatic: 2 from L = 4 on (the combined pass is register-bound there),
+  // else 0
+  int msg_recompute = -1;
 };
 
 struct DevModelSet {

@@ -283,6 +283,16 @@ class Sus2Model:
         """Launch shape of the per-atom EFS kernels (see sus2_tune_set)."""
         _lib.check(_lib.lib().sus2_tune_set(self._h, PRECISION[precision], int(eb_atoms), int(grid_mult)))
 
+    def messages(self, precision, mode=None):
+        """Gate-adjoint message mode of the batch EFS ("store", "recompute",
+        "split" or "auto", see sus2_tune_messages); sets it when given, returns
+        the mode set."""
+        names = ["store", "recompute", "split", "auto"]
+        cur = ctypes.c_int()
+        code = -1 if mode is None else names.index(mode)
+        _lib.check(_lib.lib().sus2_tune_messages(self._h, PRECISION[precision], code, ctypes.byref(cur)))
+        return names[cur.value]
+
     def get_tune(self, precision):
         a, b = ctypes.c_int(), ctypes.c_int()
         _lib.check(_lib.lib().sus2_tune_get(self._h, PRECISION[precision], ctypes.byref(a), ctypes.byref(b)))

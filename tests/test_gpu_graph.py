@@ -165,3 +165,34 @@ def test_pair_contract_with_persistent_shape(lib, oracle_lib):
     assert _maxrel(r["force"], ref["forces"]) <= 1e-10
     assert abs(r["energy"] - ref["energy"][0]) <= 1e-10 * abs(ref["energy"][0])
     assert _maxrel(r["virial"], ref["virial"][0]) <= 1e-10
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_message_modes_agree(lib, oracle_lib, prec):
+    """The three ways of forming the outer pass's gate-adjoint messages (one
+    combined edge pass, a forces-only pass plus a messages-only pass, or
+    recomputation in the gather) give the oracle's EFS; the split form equals
+    the combined one bit for bit."""
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    print(f"[case] message modes {prec}", flush=True)
+    path = W.model_path("effect_l4k6r2o3")
+    frames = _mixed_frames(3, 71)
+    ref = oracle_lib.OracleModel(path).efs(frames, nthreads=16)
+    m = Sus2Model(path)
+    assert m.messages(prec) == "auto"
+    batch = DeviceBatch(frames)
+    out = {}
+    for mode in ("store", "split", "recompute", "auto"):
+        assert m.messages(prec, mode) == mode
+        out[mode] = {k: v.cpu().numpy() for k, v in m.efs_device(batch, prec).items()}
+        torch.cuda.synchronize()
+    for k in ("energy", "forces", "virial", "site"):
+        assert np.array_equal(out["store"][k], out["split"][k]), k
+        assert np.array_equal(out["auto"][k], out["split"][k]), k  # automatic = split at L4
+        assert _maxrel(out["recompute"][k], out["store"][k]) <= (1e-13 if prec == "fp64" else 1e-6), k
+    tol = 1e-10 if prec == "fp64" else 1e-5
+    for mode in ("store", "recompute"):
+        assert _maxrel(out[mode]["forces"], ref["forces"]) <= tol
+        assert _maxrel(out[mode]["energy"], ref["energy"]) <= tol
