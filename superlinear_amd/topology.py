@@ -10,7 +10,8 @@ reference generator (dev_src/sh_model_init.cpp):
 * real-basis Clebsch-Gordan couplings (:439-536), products keyed and merged
   by (left, right, target) with left <= right (:599-657, :970-985);
 * scalar enumeration per body order 2..6 with ``q-total`` (cross) or
-  ``same-k`` (diagonal) factor pruning (:702-838);
+  ``same-k`` (diagonal) factor pruning (:702-838); a same-k rank graph is a
+  single-channel plan replicated per channel (:936-990, :1500-1540);
 * segment layout raw basics | gate rank graph | outer rank graph, and the
   ``*_begin``/``*_product_*`` fields that mark them (:1490-1800).
 
@@ -279,6 +280,32 @@ class GraphBuilder:
                 self.scalar_info.append((body, *q, il))
         self.products = [p for p in self.products if abs(p[3]) >= EPS]
 
+    def bind_single_channel(self, angular):
+        """Same-k scalars of a ranked graph: one single-channel (kmax = 1)
+        angular plan replicated over the channels, highest channel first
+        (dev_src/sh_model_init.cpp:936-990 BindSingleChannelPlan, used for
+        same-k outer and gate rank graphs at :1506-1540).  Channel c's copy
+        maps the plan's basic nodes to this graph's (k = c, l, m) basics and
+        appends fresh nodes for the rest in the plan's order; scalars are
+        interleaved by channel for each plan scalar, with factor indices
+        q -> q kmax + (kmax - 1 - c)."""
+        if angular.kmax != 1 or angular.lmax != self.lmax or self.products or self.scalars \
+                or self.nodes != len(self.basic):
+            raise ValueError("a single-channel plan binds onto a graph holding only its basics")
+        nb = len(angular.basic)
+        where = {b: i for i, b in enumerate(self.basic)}
+        nodes = {}
+        for c in range(self.kmax - 1, -1, -1):
+            m = [where[(c, l, mm)] for _, l, mm in angular.basic]
+            m += [self._node() for _ in range(nb, angular.nodes)]
+            nodes[c] = m
+            self.products += [[m[a], m[b], m[t], v] for a, b, t, v in angular.products]
+        for s, info in zip(angular.scalars, angular.scalar_info):
+            for c in range(self.kmax - 1, -1, -1):
+                self.scalars.append(nodes[c][s])
+                q = [x * self.kmax + (self.kmax - 1 - c) if x >= 0 else x for x in info[1:6]]
+                self.scalar_info.append((info[0], *q, info[6]))
+
 
 def _fmt(x):
     return f"{x:.15e}"
@@ -322,17 +349,30 @@ def build_model(lmax=3, kmax=4, species=1, body_order=4, outer_rank=2, outer_pru
     if not outer_rank:
         raw.add_scalars(body_order)
     g = None
+
+    def ranked(rank, pruning, body):
+        # a same-k rank graph is one single-channel plan bound per channel
+        # (dev_src/sh_model_init.cpp:1500-1540)
+        r = GraphBuilder(lmax, rank, pruning, cache)
+        r.add_all_basics()
+        if pruning == "same-k":
+            a = GraphBuilder(lmax, 1, "same-k", cache)
+            a.add_all_basics()
+            a.add_scalars(body)
+            r.bind_single_channel(a)
+        else:
+            r.add_scalars(body)
+        return r
+
     if gate:
         if not outer_rank or not gate_rank:
             raise ValueError("the Effect generator builds ranked gate and outer graphs")
-        g = GraphBuilder(lmax, gate_rank, gate_pruning, cache)
-        g.add_all_basics()
-        g.add_scalars(gate_body_order)
+        if not 2 <= gate_body_order <= body_order:
+            raise ValueError("the gate body order must be in [2, body_order]")
+        g = ranked(gate_rank, gate_pruning, gate_body_order)
     o = None
     if outer_rank:
-        o = GraphBuilder(lmax, outer_rank, outer_pruning, cache)
-        o.add_all_basics()
-        o.add_scalars(body_order)
+        o = ranked(outer_rank, outer_pruning, body_order)
     goff = raw.nodes
     ooff = raw.nodes + (g.nodes if g else 0)
     M = ooff + (o.nodes if o else 0)
@@ -391,7 +431,9 @@ def build_model(lmax=3, kmax=4, species=1, body_order=4, outer_rank=2, outer_pru
     if outer_rank:
         out.append("outer_k_rank1_enabled = true")
         out.append(f"outer_k_rank = {outer_rank}")
-        out.append("outer_channel_mode = cross-channel")
+        # a same-k outer graph holds no cross-channel product (the reader's
+        # check, dev_src/mtpr.cpp:12225; the writer's flag, sh_model_init.cpp:1698)
+        out.append("outer_channel_mode = " + ("same-channel" if outer_pruning == "same-k" else "cross-channel"))
         out.append("outer_radial_mode = legacy-plus-mixer")
         out.append(f"readout_mode = {readout}")
         out.append(f"outer_k_rank1_basic_begin = {ooff}")

@@ -25,7 +25,7 @@ import os
 
 import numpy as np
 
-from .topology import write_model
+from .topology import build_model
 
 A_DIAMOND = 3.567
 CUTOFF = 5.0
@@ -48,15 +48,21 @@ CONFIG_MODEL = {"static64": "static_l3k4o2", "effect4096": "effect_l3k4r2o2", "t
 
 
 def model_path(name, directory=None):
-    """Generate (once) and return the path of a named synthetic model."""
+    """Return the path of a named synthetic model, (re)written whenever the
+    cached file differs from what the generator gives now."""
     directory = directory or os.environ.get("SUS2_MODEL_DIR") or os.path.join(
         os.path.dirname(os.path.abspath(__file__)), "_models")
     os.makedirs(directory, exist_ok=True)
     path = os.path.join(directory, name + ".sus2")
-    if not os.path.exists(path):
-        tmp = path + f".tmp{os.getpid()}"
-        write_model(tmp, cutoff=CUTOFF, name=name, **MODELS[name])
-        os.replace(tmp, path)
+    text = build_model(cutoff=CUTOFF, name=name, **MODELS[name])
+    if os.path.exists(path):
+        with open(path) as fh:
+            if fh.read() == text:
+                return path
+    tmp = path + f".tmp{os.getpid()}"
+    with open(tmp, "w") as fh:
+        fh.write(text)
+    os.replace(tmp, path)
     return path
 
 

@@ -120,6 +120,68 @@ def test_topology_reproduces_reference_graphs(fixture, kw, keys):
         assert np.abs(x - y).max(initial=0) <= 1e-12, k
 
 
+REF_TOPOLOGY = ROOT / "oracle" / "_ref" / "ref-topology"
+
+# (lmax, kmax, body order, outer pruning, gate rank, outer rank, gate body order, gate pruning); gate rank 0 = Static
+TOPOLOGY_CASES = [
+    (1, 2, 4, "q-total", 0, 0, 0, ""), (3, 4, 4, "q-total", 0, 0, 0, ""), (3, 4, 4, "same-k", 0, 0, 0, ""),
+    (2, 3, 5, "q-total", 0, 0, 0, ""), (2, 3, 5, "same-k", 0, 0, 0, ""), (2, 2, 6, "q-total", 0, 0, 0, ""),
+    (2, 2, 6, "same-k", 0, 0, 0, ""), (6, 1, 4, "q-total", 0, 0, 0, ""), (3, 4, 4, "q-total", 0, 2, 0, ""),
+    (3, 4, 4, "same-k", 2, 2, 4, "q-total"), (3, 4, 4, "q-total", 2, 2, 4, "q-total"),
+    (2, 3, 5, "same-k", 2, 3, 4, "q-total"), (2, 3, 5, "same-k", 3, 2, 4, "same-k"),
+    (3, 3, 5, "q-total", 2, 2, 5, "same-k"), (2, 2, 6, "same-k", 2, 2, 6, "q-total"),
+    (4, 6, 4, "same-k", 2, 3, 4, "q-total"), (5, 6, 4, "same-k", 2, 3, 4, "q-total"),
+]
+
+
+@pytest.mark.skipif(not REF_TOPOLOGY.exists(), reason="reference generator not built (oracle/Makefile ref)")
+@pytest.mark.parametrize("case", TOPOLOGY_CASES, ids=lambda c: "L%dK%d_b%d_%s_r%do%d_g%d%s" % c)
+def test_topology_matches_reference_generator(case, tmp_path):
+    """Graphs at body orders 4..6, both factor prunings, Static and Effect
+    (ranked gate and outer graphs, same-k rank graphs bound per channel)
+    equal the reference generator's (dev_src/sh_model_init.cpp, built from
+    its sources into oracle/_ref/ref-topology by oracle/Makefile): nodes,
+    products with their coefficients, scalar nodes and body metadata."""
+    import subprocess
+    from superlinear_amd.topology import build_model
+    L, K, body, pr, R, O, gbody, gpr = case
+    opts = {"l-max": L, "k-max": K, "body-order": body, "species-count": 2, "radial-basis-size": 3,
+            "sh-factor-pruning": pr}
+    kw = dict(lmax=L, kmax=K, species=2, body_order=body, outer_rank=O, outer_pruning=pr, rb_size=3)
+    keys = GRAPH_KEYS[:-1]
+    if O:
+        opts["outer-k-rank"] = O
+        keys = GRAPH_KEYS + ["outer_k_rank1_basic_begin", "outer_k_rank1_product_begin", "outer_channel_mode"]
+    if R:
+        opts.update({"two-layer-gate": None, "two-layer-gate-mode": "mu-scalar-full",
+                     "two-layer-gate-site-mode": "neighbor", "two-layer-gate-radial-mode": "shared-radial",
+                     "two-layer-gate-k-rank": R, "two-layer-gate-body-order": gbody,
+                     "two-layer-gate-factor-pruning": gpr})
+        kw.update(gate=True, gate_rank=R, gate_body_order=gbody, gate_pruning=gpr)
+        keys = GATE_KEYS + ["outer_channel_mode", "two_layer_gate_factor_pruning"]
+    out = tmp_path / "ref.sus2"
+    args = [str(REF_TOPOLOGY), str(out)] + [k if v is None else f"{k}={v}" for k, v in opts.items()]
+    subprocess.run(args, check=True, capture_output=True, timeout=120)
+    a = _fields(out.read_text())
+    b = _fields(build_model(**kw))
+    for k in keys:
+        if k in ("outer_channel_mode", "two_layer_gate_factor_pruning"):
+            # the reference writes the channel mode only where it is not the default
+            assert a.get(k, "cross-channel") == b[k], k
+            continue
+        x, y = _nums(a[k]), _nums(b[k])
+        assert x.shape == y.shape, k
+        assert np.abs(x - y).max(initial=0) <= 1e-12, k
+
+
+def test_topology_rejects_gate_body_order_above_outer():
+    """The reference generator refuses a gate body order above the outer one
+    (dev_src/sh_model_init.cpp:1374-1380)."""
+    from superlinear_amd.topology import build_model
+    with pytest.raises(ValueError):
+        build_model(lmax=2, kmax=3, species=2, body_order=4, outer_rank=2, gate=True, gate_rank=2, gate_body_order=5)
+
+
 def test_real_cg_orthogonality():
     """Real coupling coefficients form an orthogonal transform of the
     (l1 x l2) product space (sum over L, M of C C = delta), for l1, l2 <= 6."""
