@@ -129,12 +129,52 @@ int sus2_efs_grad(sus2_model* m, int precision, int nframes, int natoms, const i
                   const double* ene_weight, const double* frc_weight, const double* str_weight, double* grad,
                   void* stream);
 
+/* --- objective options (the NonLinearRegression members the trainer sets) ----
+ * rel_forces      wgt_rel_forces (train --scale-by-force): when > 0, atom i's
+ *                 force weight is wF rel / (|F0_i|^2 + rel)
+ * norm_by_forces  n >= 0 (int, the reference's norm_by_forces): a frame's
+ *                 energy and force terms are scaled by 0.1 / (0.1 + n mean_i |F0_i|^2)
+ * force_loss      SUS2_FORCE_L2 (residual^2) or SUS2_FORCE_LOG_COSH
+ *                 (s^2 log cosh(residual / s), s = log_cosh_scale > 0)
+ * std_weight, stdd_weight
+ *                 per-species site-energy spread regularisers (--std-weight,
+ *                 --stdd-weight): std_weight sum_i (E_i - <E>_t)^2
+ *                 200 / (200 + |F0_i|^2) / N and stdd_weight
+ *                 sum_t (<E>_t - E0 / N)^2 per frame, <E>_t the frame's mean
+ *                 site energy of species t
+ * constr_weight   legacy radial norm / orthogonality penalty
+ *                 constr sum_mu (|c_mu|^2 - 1)^2 + constr sum_{mu<nu} (c_mu . c_nu)^2
+ *                 over the outer radial rows (MLMTPR::AddPenaltyGrad), added
+ *                 once to the objective; train uses 1e-9 for these models
+ * (dev_src/non_linear_regression.cpp:936-1052, dev_src/force_loss.h,
+ * dev_src/mtpr.cpp:14050, dev_src/mlp/mtpr_train.cpp:1827-1849,2077-2090,2832).
+ * A new handle holds sus2_objective_default: the objective of
+ * `mlp-sus2 train` for Static and Effect models (rel_forces 0,
+ * norm_by_forces 0, L2, log_cosh_scale 2, no std terms, constr_weight 1e-9).
+ * The options apply to sus2_loss_grad, the trainers and graphs created after
+ * they are set; sus2_train_linear takes norm_by_forces and constr_weight
+ * (the reference's AddToSLAE ignores the others). */
+enum { SUS2_FORCE_L2 = 0, SUS2_FORCE_LOG_COSH = 1 };
+typedef struct sus2_objective {
+  double rel_forces;
+  int norm_by_forces;
+  int force_loss;
+  double log_cosh_scale;
+  double std_weight;
+  double stdd_weight;
+  double constr_weight;
+} sus2_objective;
+void sus2_objective_default(sus2_objective* out);
+int sus2_objective_get(const sus2_model* m, sus2_objective* out);
+int sus2_objective_set(sus2_model* m, const sus2_objective* in);
+
 /* --- training objective + gradient (CalcObjectiveFunctionGrad) -------------
  * Labels: e0 [nframes], f0 [natoms*3], s0 [nframes*6] (xx yy zz xy xz yz).
  * weighting: 0 vibrations, 1 structures, 2 molecules; weights wE wF wS.
- * loss_out (host double) = (1/nframes) sum_f L_f; grad [ncoeff] (dev FP64)
- * is overwritten with the gradient of loss_out.  Returns once loss_out is
- * known; the gradient kernels finish on `stream` after the return. */
+ * loss_out (host double) = (1/nframes) sum_f L_f + penalty (the options
+ * above); grad [ncoeff] (dev FP64) is overwritten with the gradient of
+ * loss_out.  Returns once loss_out is known; the gradient kernels finish on
+ * `stream` after the return. */
 int sus2_loss_grad(sus2_model* m, int precision, int nframes, int natoms, const int64_t* frame_off,
                    const int32_t* types, const double* pos, const double* cell, const int32_t* periodic,
                    const double* e0, const double* f0, const double* s0, int weighting, double wE, double wF,

@@ -47,16 +47,19 @@ def lib():
         L.orc_efs_edges.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_long, _ip, _dp,
                                     _dp, _dp, _dp, _dp]
         L.orc_loss_grad.argtypes = [ctypes.c_void_p, ctypes.c_int, _lp, _ip, _dp, _dp, _ip, _dp, _dp, _dp,
-                                    ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                    ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp,
                                     _dp, _dp, _dp, ctypes.c_int]
         L.orc_efs_grad.argtypes = [ctypes.c_void_p, ctypes.c_int, _lp, _ip, _dp, _dp, _ip, _dp, _dp, _dp, _dp,
                                    ctypes.c_int]
         L.orc_loss.argtypes = [ctypes.c_void_p, ctypes.c_int, _lp, _ip, _dp, _dp, _ip, _dp, _dp, _dp,
-                               ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+                               ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp]
         L.orc_set_grad_subset.argtypes = [ctypes.c_void_p, ctypes.c_int, _ip]
         L.orc_frame_loss.restype = ctypes.c_double
         L.orc_frame_loss.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                     ctypes.c_double, _dp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp, _dp]
+                                     ctypes.c_double, _dp, _dp, ctypes.c_double, _dp, _dp, _dp, _dp, _ip, _dp, _dp,
+                                     _dp]
+        L.orc_radial_penalty.restype = ctypes.c_double
+        L.orc_radial_penalty.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, _dp]
         _LIB = L
     return _LIB
 
@@ -129,7 +132,14 @@ class OracleModel:
         a = np.ascontiguousarray(np.zeros(0) if idx is None else idx, np.int32)
         _check(lib().orc_set_grad_subset(self.h, a.size, _ptr(a, _ip)))
 
-    def loss_grad(self, frames, labels, mode=1, wE=1.0, wF=0.01, wS=0.0, nthreads=1):
+    def radial_penalty(self, constr, theta=None):
+        """The legacy radial penalty of theta (default: the model's) and its gradient."""
+        th = np.ascontiguousarray(self.coeffs() if theta is None else theta, np.float64)
+        g = np.zeros(th.size)
+        v = lib().orc_radial_penalty(self.h, _ptr(th, _dp), float(constr), _ptr(g, _dp))
+        return v, g
+
+    def loss_grad(self, frames, labels, mode=1, wE=1.0, wF=0.01, wS=0.0, nthreads=1, objective=None):
         b = pack(frames)
         nf = len(frames)
         n = int(b["off"][-1])
@@ -143,8 +153,9 @@ class OracleModel:
         sb = np.zeros((nf, 9))
         _check(lib().orc_loss_grad(self.h, nf, _ptr(b["off"], _lp), _ptr(b["types"], _ip), _ptr(b["pos"], _dp),
                                    _ptr(b["cell"], _dp), _ptr(b["periodic"], _ip), _ptr(E0, _dp), _ptr(F0, _dp),
-                                   _ptr(S0, _dp), mode, wE, wF, wS, _ptr(loss, _dp), _ptr(grad, _dp),
-                                   _ptr(eb, _dp), _ptr(fb, _dp), _ptr(sb, _dp), nthreads))
+                                   _ptr(S0, _dp), mode, wE, wF, wS, _ptr(objective_array(objective), _dp),
+                                   _ptr(loss, _dp), _ptr(grad, _dp), _ptr(eb, _dp), _ptr(fb, _dp), _ptr(sb, _dp),
+                                   nthreads))
         return dict(loss=float(loss[0]), grad=grad, ebar=eb, fbar=fb, sbar=sb)
 
     def efs_grad(self, frames, ene_w, frc_w, str_w, nthreads=1):
@@ -177,7 +188,7 @@ class OracleModel:
                                    _ptr(E, _dp), _ptr(F, _dp), _ptr(W, _dp), _ptr(S, _dp)))
         return float(E[0]), F
 
-    def loss(self, frames, labels, mode=1, wE=1.0, wF=0.01, wS=0.0):
+    def loss(self, frames, labels, mode=1, wE=1.0, wF=0.01, wS=0.0, objective=None):
         b = pack(frames)
         nf = len(frames)
         n = int(b["off"][-1])
@@ -187,13 +198,35 @@ class OracleModel:
         loss = np.zeros(1)
         _check(lib().orc_loss(self.h, nf, _ptr(b["off"], _lp), _ptr(b["types"], _ip), _ptr(b["pos"], _dp),
                               _ptr(b["cell"], _dp), _ptr(b["periodic"], _ip), _ptr(E0, _dp), _ptr(F0, _dp),
-                              _ptr(S0, _dp), mode, wE, wF, wS, _ptr(loss, _dp)))
+                              _ptr(S0, _dp), mode, wE, wF, wS, _ptr(objective_array(objective), _dp),
+                              _ptr(loss, _dp)))
         return float(loss[0])
 
 
-def frame_loss(mode, wE, wF, wS, E, F, S, E0, F0, S0):
+# objective options in the order of the C functions' opts[7]; defaults are
+# those of `mlp-sus2 train` for Static and Effect models
+OBJECTIVE_KEYS = ("rel_forces", "norm_by_forces", "force_loss", "log_cosh_scale", "std_weight", "stdd_weight",
+                  "constr_weight")
+OBJECTIVE_DEFAULT = dict(rel_forces=0.0, norm_by_forces=0, force_loss="l2", log_cosh_scale=2.0, std_weight=0.0,
+                         stdd_weight=0.0, constr_weight=1e-9)
+
+
+def objective_array(objective=None):
+    o = dict(OBJECTIVE_DEFAULT)
+    if objective:
+        unknown = set(objective) - set(o)
+        if unknown:
+            raise ValueError(f"unknown objective options {sorted(unknown)}")
+        o.update(objective)
+    o["force_loss"] = {"l2": 0, "log-cosh": 1}.get(o["force_loss"], o["force_loss"])
+    return np.array([float(o[k]) for k in OBJECTIVE_KEYS], np.float64)
+
+
+def frame_loss(mode, wE, wF, wS, E, F, S, E0, F0, S0, objective=None, site=None, types=None):
     """One frame's loss and adjoints from given predictions (E, F [n,3], S
-    Voigt [6]) and labels: returns (loss, dL/dE_i [n], dL/dF [n,3], dL/dS [3,3])."""
+    Voigt [6]) and labels, under the objective options (site energies and
+    species only for the std terms): returns (loss, dL/dE_i [n], dL/dF [n,3],
+    dL/dS [3,3])."""
     F = np.ascontiguousarray(F, np.float64).reshape(-1, 3)
     F0 = np.ascontiguousarray(F0, np.float64).reshape(-1, 3)
     S = np.ascontiguousarray(S, np.float64)
@@ -202,8 +235,14 @@ def frame_loss(mode, wE, wF, wS, E, F, S, E0, F0, S0):
     dE = np.zeros(n)
     dF = np.zeros((n, 3))
     dS = np.zeros(9)
+    o = objective_array(objective)
+    se = None if site is None else np.ascontiguousarray(site, np.float64)
+    ty = None if types is None else np.ascontiguousarray(types, np.int32)
     loss = lib().orc_frame_loss(n, mode, wE, wF, wS, E, _ptr(F, _dp), _ptr(S, _dp), E0, _ptr(F0, _dp),
-                                _ptr(S0, _dp), _ptr(dE, _dp), _ptr(dF, _dp), _ptr(dS, _dp))
+                                _ptr(S0, _dp), _ptr(o, _dp), None if se is None else _ptr(se, _dp),
+                                None if ty is None else _ptr(ty, _ip), _ptr(dE, _dp), _ptr(dF, _dp), _ptr(dS, _dp))
+    if np.isnan(loss):
+        raise RuntimeError(lib().orc_last_error().decode())
     return loss, dE, dF, dS.reshape(3, 3)
 
 

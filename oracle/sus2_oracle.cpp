@@ -685,24 +685,61 @@ EFS efs(const Model& m, const double* theta, int n, const int* type, const std::
 }
 
 // ---------------------------------------------------------------- loss
-// Per-frame loss of dev_src/non_linear_regression.cpp:936-1052 with
-// norm_by_forces = 0, L2 force residual, no std terms; `mode` 0 = vibrations,
-// 1 = structures, 2 = molecules.  Returns the frame loss and writes the
-// adjoints handed to the model (dL/dE_i, dL/dF_i, str_weights = dLdS of the
-// reference, i.e. -2 w_S (S - S0) / N).
+// Objective options (NonLinearRegression members, dev_src/non_linear_regression.h:110-140):
+// opts[7] = rel_forces, norm_by_forces, force_loss (0 l2, 1 log-cosh),
+// log_cosh_scale, std_weight, stdd_weight, constr_weight.  Null = the
+// objective of `mlp-sus2 train` for these models (dev_src/mlp/mtpr_train.cpp:
+// 1827-1849, 2077-2090, 2832: constr 1e-9, log-cosh scale 2, the rest 0).
+struct Objective {
+  double rel = 0; int nbf = 0; int kind = 0; double lcs = 2.0; double stdw = 0, stddw = 0, constr = 1e-9;
+};
+Objective objective_of(const double* o) {
+  Objective ob;
+  if (!o) return ob;
+  ob.rel = o[0]; ob.nbf = (int)o[1]; ob.kind = (int)o[2]; ob.lcs = o[3]; ob.stdw = o[4]; ob.stddw = o[5];
+  ob.constr = o[6];
+  if (ob.kind != 0 && ob.kind != 1) fail("force loss kind must be 0 (l2) or 1 (log-cosh)");
+  if (!(ob.lcs > 0)) fail("log-cosh scale must be positive");
+  return ob;
+}
+
+// Force residual loss and derivative (dev_src/force_loss.h:47-70): L2 and the
+// log-cosh residual s^2 log cosh(d / s), here by its series-free identity
+// log cosh x = log((1 + e^{-2|x|}) / 2) + |x|.
+double rho(double d, const Objective& o) {
+  if (o.kind == 0) return d * d;
+  const double x = std::fabs(d) / o.lcs;
+  return o.lcs * o.lcs * (x + std::log((1.0 + std::exp(-2.0 * x)) / 2.0));
+}
+double drho(double d, const Objective& o) { return o.kind == 0 ? 2.0 * d : o.lcs * std::tanh(d / o.lcs); }
+
+// Per-frame loss of dev_src/non_linear_regression.cpp:936-1052; `mode` 0 =
+// vibrations, 1 = structures, 2 = molecules.  Returns the frame loss and
+// writes the adjoints handed to the model (dL/dE_i, dL/dF_i, str_weights =
+// dLdS of the reference, i.e. -2 w_S (S - S0) / N).  The std terms need the
+// site energies `se` and species `type` of the frame's atoms.
 double frame_loss(int n, int mode, double wE, double wF, double wS, double E, const double* F, const double* S,
                   double E0, const double* F0, const double* S0, bool hasE, bool hasF, bool hasS,
-                  double* dEi, double* dF, double* dS) {
+                  double* dEi, double* dF, double* dS, const Objective& ob = Objective{},
+                  const double* se = nullptr, const int* type = nullptr) {
   double loss = 0;
   for (int i = 0; i < 3 * n; ++i) dF[i] = 0;
   for (int q = 0; q < 9; ++q) dS[q] = 0;
+  auto f0sq = [&](int i) { return F0[3 * i] * F0[3 * i] + F0[3 * i + 1] * F0[3 * i + 1] + F0[3 * i + 2] * F0[3 * i + 2]; };
+  double avef = 0;
+  if (hasF)
+    for (int i = 0; i < n; ++i) avef += f0sq(i) / n;
+  const double nb = 0.1 / (0.1 + ob.nbf * avef);
   if (wF != 0 && hasF) {
-    double w = wF;
-    if (mode == 1) w /= n;
     for (int i = 0; i < n; ++i) {
-      double fl = 0;
-      for (int a = 0; a < 3; ++a) { const double d = F[3 * i + a] - F0[3 * i + a]; fl += d * d; dF[3 * i + a] = w * 2 * d; }
-      loss += w * fl;
+      double w = ob.rel <= 0 ? wF : wF * ob.rel / (f0sq(i) + ob.rel);
+      if (mode == 1) w /= n;
+      w *= nb;
+      for (int a = 0; a < 3; ++a) {
+        const double d = F[3 * i + a] - F0[3 * i + a];
+        loss += w * rho(d, ob);
+        dF[3 * i + a] = w * drho(d, ob);
+      }
     }
   }
   double we = wE, ws = wS;
@@ -714,9 +751,59 @@ double frame_loss(int n, int mode, double wE, double wF, double wS, double E, co
     loss += ws * fro / n;
   }
   double dE = 0;
-  if (we != 0 && hasE) { loss += we * (E - E0) * (E - E0) / n; dE = 2.0 * we * (E - E0) / n; }
+  if (we != 0 && hasE) { loss += we * (E - E0) * (E - E0) * nb / n; dE = 2.0 * we * (E - E0) * nb / n; }
   for (int i = 0; i < n; ++i) dEi[i] = dE;
+  if ((ob.stdw != 0 || ob.stddw != 0) && n > 0) {
+    if (!se || !type) fail("the std terms need the frame's site energies and species");
+    // per-species count, mean site energy, and the joint term of the adjoint
+    std::map<int, double> cnt, mean, joint;
+    for (int i = 0; i < n; ++i) cnt[type[i]] += 1;
+    for (int i = 0; i < n; ++i) mean[type[i]] += se[i] / cnt[type[i]];
+    const double mean0 = E0 / n;
+    double sd = 0, sdd = 0;
+    for (int i = 0; i < n; ++i) {
+      const double g = 200.0 / (200.0 + f0sq(i)), d = se[i] - mean[type[i]];
+      joint[type[i]] += d * g / cnt[type[i]] / n;
+      sd += d * d * g / n;
+    }
+    for (const auto& kv : mean) sdd += (kv.second - mean0) * (kv.second - mean0);
+    loss += ob.stdw * sd + ob.stddw * sdd;
+    for (int i = 0; i < n; ++i) {
+      const int t = type[i];
+      dEi[i] += ob.stdw * 2.0 * ((se[i] - mean[t]) * 200.0 / (200.0 + f0sq(i)) / n - joint[t]) +
+                ob.stddw * 2.0 * (mean[t] - mean0) / cnt[t];
+    }
+  }
   return loss;
+}
+
+// Legacy radial penalty (MLMTPR::AddPenaltyGrad, dev_src/mtpr.cpp:14050):
+// rows c_mu = theta[o_rad + mu (B + C) + b], b < B, mu < F;
+// constr (sum_mu (|c_mu|^2 - 1)^2 + sum_{mu<nu} (c_mu . c_nu)^2), gradient
+// added to grad (may be null).
+double radial_penalty(const Model& m, const double* theta, double constr, double* grad) {
+  if (constr == 0) return 0.0;
+  const int F = m.F, B = m.B, st = m.B + m.C, o = m.o_rad;
+  auto c = [&](int k, int l) { return theta[o + k * st + l]; };
+  double v = 0;
+  for (int k = 0; k < F; ++k) {
+    double nn = 0;
+    for (int l = 0; l < B; ++l) nn += c(k, l) * c(k, l);
+    v += constr * (nn - 1) * (nn - 1);
+    if (grad) for (int l = 0; l < B; ++l) grad[o + k * st + l] += 4 * constr * (nn - 1) * c(k, l);
+  }
+  for (int k1 = 0; k1 < F; ++k1)
+    for (int k2 = k1 + 1; k2 < F; ++k2) {
+      double sc = 0;
+      for (int l = 0; l < B; ++l) sc += c(k1, l) * c(k2, l);
+      v += constr * sc * sc;
+      if (grad)
+        for (int l = 0; l < B; ++l) {
+          grad[o + k1 * st + l] += 2 * constr * sc * c(k2, l);
+          grad[o + k2 * st + l] += 2 * constr * sc * c(k1, l);
+        }
+    }
+  return v;
 }
 
 // Parameter gradient of one frame:  sum_i dEi_i dE_i/dtheta + d/dtheta of
@@ -880,10 +967,11 @@ int orc_efs_edges(void* h, int n, int nall, const int* type, long nedge, const i
 // coefficient at a time.  Slow by design: it is the checker.
 int orc_loss_grad(void* h, int nframes, const long* off, const int* type, const double* x, const double* cell,
                   const int* periodic, const double* E0, const double* F0, const double* S0, int mode, double wE,
-                  double wF, double wS, double* loss_out, double* grad, double* ebar_out, double* fbar_out,
-                  double* sbar_out, int nthreads) {
+                  double wF, double wS, const double* opts, double* loss_out, double* grad, double* ebar_out,
+                  double* fbar_out, double* sbar_out, int nthreads) {
   auto* m = static_cast<Model*>(h);
   return guard([&] {
+    const Objective ob = objective_of(opts);
     const int P = m->ncoeff;
     std::vector<double> G((size_t)nframes * P, 0.0);
     std::vector<double> Lf(nframes, 0.0);
@@ -902,7 +990,7 @@ int orc_loss_grad(void* h, int nframes, const long* off, const int* type, const 
         std::vector<double> dEi(n), dF(3 * n);
         double dS[9];
         Lf[f] = frame_loss(n, mode, wE, wF, wS, r.energy, r.force.data(), S, E0[f], F0 + 3 * a, S0f, true, true, true,
-                           dEi.data(), dF.data(), dS);
+                           dEi.data(), dF.data(), dS, ob, r.site.data(), type + a);
         if (ebar_out) for (int i = 0; i < n; ++i) ebar_out[a + i] = dEi[i];
         if (fbar_out) for (int i = 0; i < 3 * n; ++i) fbar_out[3 * a + i] = dF[i];
         if (sbar_out) for (int q = 0; q < 9; ++q) sbar_out[9 * f + q] = dS[q];
@@ -921,6 +1009,9 @@ int orc_loss_grad(void* h, int nframes, const long* off, const int* type, const 
       for (int f = 0; f < nframes; ++f) s += G[(size_t)f * P + p];
       grad[p] = s / nframes;
     }
+    // the penalty is added once per frame and divided by the frame count
+    // with the rest (dev_src/non_linear_regression.cpp:1052, mtpr_trainer.cpp:4737)
+    if (nframes > 0) *loss_out += radial_penalty(*m, m->theta.data(), ob.constr, grad);
   });
 }
 
@@ -961,9 +1052,10 @@ int orc_efs_grad(void* h, int nframes, const long* off, const int* type, const d
 // Plain loss value (for finite differences of the gradient).
 int orc_loss(void* h, int nframes, const long* off, const int* type, const double* x, const double* cell,
              const int* periodic, const double* E0, const double* F0, const double* S0, int mode, double wE,
-             double wF, double wS, double* loss_out) {
+             double wF, double wS, const double* opts, double* loss_out) {
   auto* m = static_cast<Model*>(h);
   return guard([&] {
+    const Objective ob = objective_of(opts);
     double L = 0;
     for (int f = 0; f < nframes; ++f) {
       const long a = off[f];
@@ -975,9 +1067,9 @@ int orc_loss(void* h, int nframes, const long* off, const int* type, const doubl
       voigt_to_full(S0 + 6 * f, S0f);
       std::vector<double> dEi(n), dF(3 * n);
       L += frame_loss(n, mode, wE, wF, wS, r.energy, r.force.data(), S, E0[f], F0 + 3 * a, S0f, true, true, true,
-                      dEi.data(), dF.data(), dS);
+                      dEi.data(), dF.data(), dS, ob, r.site.data(), type + a);
     }
-    *loss_out = L / nframes;
+    *loss_out = L / nframes + (nframes > 0 ? radial_penalty(*m, m->theta.data(), ob.constr, nullptr) : 0.0);
   });
 }
 
@@ -994,15 +1086,32 @@ int orc_set_grad_subset(void* h, int n, const int* idx) {
 }
 
 // One frame's loss and adjoints from given predictions and labels (no model):
-// E, E0 scalars, F, F0 [n*3], S, S0 Voigt xx yy zz xy xz yz.  Outputs dEi [n],
-// dF [n*3], dS [9] row-major; returns the frame loss
-// (dev_src/non_linear_regression.cpp:986-1037 with norm_by_forces = 0).
+// E, E0 scalars, F, F0 [n*3], S, S0 Voigt xx yy zz xy xz yz, objective options
+// opts[7] (null = defaults), site energies se [n] and species type [n] (used
+// by the std terms only; may be null otherwise).  Outputs dEi [n], dF [n*3],
+// dS [9] row-major; returns the frame loss (NaN on an invalid option)
+// (dev_src/non_linear_regression.cpp:936-1052).
 double orc_frame_loss(int n, int mode, double wE, double wF, double wS, double E, const double* F, const double* S,
-                      double E0, const double* F0, const double* S0, double* dEi, double* dF, double* dS) {
+                      double E0, const double* F0, const double* S0, const double* opts, const double* se,
+                      const int* type, double* dEi, double* dF, double* dS) {
   double Sf[9], S0f[9];
   voigt_to_full(S, Sf);
   voigt_to_full(S0, S0f);
-  return frame_loss(n, mode, wE, wF, wS, E, F, Sf, E0, F0, S0f, true, true, true, dEi, dF, dS);
+  double L = 0;
+  if (guard([&] {
+        L = frame_loss(n, mode, wE, wF, wS, E, F, Sf, E0, F0, S0f, true, true, true, dEi, dF, dS, objective_of(opts),
+                       se, type);
+      }) != 0)
+    return std::nan("");
+  return L;
+}
+
+// The legacy radial penalty of the model's coefficients theta (ncoeff) and
+// its gradient (grad [ncoeff] overwritten; may be null).
+double orc_radial_penalty(void* h, const double* theta, double constr, double* grad) {
+  auto* m = static_cast<Model*>(h);
+  if (grad) std::fill(grad, grad + m->ncoeff, 0.0);
+  return radial_penalty(*m, theta, constr, grad);
 }
 
 }  // extern "C"

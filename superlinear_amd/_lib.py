@@ -25,8 +25,20 @@ _d = ctypes.c_double
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
 
+
+
+class Sus2Objective(ctypes.Structure):
+    """sus2_objective: the objective options of a model handle."""
+    _fields_ = [("rel_forces", ctypes.c_double), ("norm_by_forces", ctypes.c_int), ("force_loss", ctypes.c_int),
+                ("log_cosh_scale", ctypes.c_double), ("std_weight", ctypes.c_double),
+                ("stdd_weight", ctypes.c_double), ("constr_weight", ctypes.c_double)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
+    "sus2_objective_default": (None, [ctypes.POINTER(Sus2Objective)]),
+    "sus2_objective_get": (_i, [_vp, ctypes.POINTER(Sus2Objective)]),
+    "sus2_objective_set": (_i, [_vp, ctypes.POINTER(Sus2Objective)]),
     "sus2_abi_version": (_i, []),
     "sus2_last_error": (ctypes.c_char_p, []),
     "sus2_model_load": (_i, [ctypes.c_char_p, ctypes.POINTER(_vp)]),

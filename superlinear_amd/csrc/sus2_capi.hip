@@ -21,6 +21,7 @@ struct sus2_model {
   sus2::Model model;
   sus2::DevModelSet dev;
   sus2::Workspace ws;
+  sus2::ObjectiveOpts obj;  // objective options of the loss, trainers and graphs
   bool uploaded = false;
   int device = -1;
 };
@@ -337,7 +338,7 @@ int sus2_loss_grad(sus2_model* m, int precision, int nframes, int natoms, const 
     ensure_device(m);
     sus2::loss_grad_dispatch(m->dev, m->ws, precision,
                              make_batch(nframes, natoms, frame_off, types, pos, cell, periodic), e0, f0, s0,
-                             weighting, wE, wF, wS, loss_out, grad, (hipStream_t)stream);
+                             weighting, wE, wF, wS, m->obj, loss_out, grad, (hipStream_t)stream);
   });
 }
 
@@ -368,7 +369,7 @@ int sus2_train_lbfgs(sus2_model* m, int precision, int nframes, int natoms, cons
       std::memcpy(m->model.theta.data(), th.data(), sizeof(double) * P);
       m->dev.upload_theta(m->model);
       double f = 0;
-      sus2::loss_grad_dispatch(m->dev, m->ws, precision, b, e0, f0, s0, weighting, wE, wF, wS, &f, gdev.get<double>(),
+      sus2::loss_grad_dispatch(m->dev, m->ws, precision, b, e0, f0, s0, weighting, wE, wF, wS, m->obj, &f, gdev.get<double>(),
                                st);
       // the gradient kernels finish on the stream after the objective is known
       if (hipStreamSynchronize(st) != hipSuccess) throw sus2::Error(SUS2_E_DEVICE, "stream synchronize failed");
@@ -482,7 +483,7 @@ int sus2_graph_loss_grad_create(sus2_model* m, int precision, int nframes, int n
     const sus2::Batch b = make_batch(nframes, natoms, frame_off, types, pos, cell, periodic);
     const sus2::EfsOut o{energy, force, virial, nullptr};
     build_graph(g.get(), nframes, frame_off, (hipStream_t)stream, [&](hipStream_t st) {
-      sus2::loss_grad_dispatch(m->dev, g->ws, precision, b, e0, f0, s0, weighting, wE, wF, wS, nullptr, grad, st,
+      sus2::loss_grad_dispatch(m->dev, g->ws, precision, b, e0, f0, s0, weighting, wE, wF, wS, m->obj, nullptr, grad, st,
                                loss, &o);
     });
     *out = g.release();
@@ -635,7 +636,7 @@ int sus2_train_bfgs(sus2_model* m, int precision, int nframes, int natoms, const
       std::memcpy(m->model.theta.data(), th.data(), sizeof(double) * P);
       m->dev.upload_theta(m->model);
       double f = 0;
-      sus2::loss_grad_dispatch(m->dev, m->ws, precision, b, e0, f0, s0, weighting, wE, wF, wS, &f, gdev.get<double>(),
+      sus2::loss_grad_dispatch(m->dev, m->ws, precision, b, e0, f0, s0, weighting, wE, wF, wS, m->obj, &f, gdev.get<double>(),
                                st);
       if (hipStreamSynchronize(st) != hipSuccess) throw sus2::Error(SUS2_E_DEVICE, "stream synchronize failed");
       d2h(grad.data(), gdev.get<double>(), sizeof(double) * P);
@@ -665,8 +666,43 @@ int sus2_train_linear(sus2_model* m, int precision, int nframes, int natoms, con
     ensure_device(m);
     const double v = sus2::train_linear(m->model, m->dev, m->ws, precision,
                                         make_batch(nframes, natoms, frame_off, types, pos, cell, periodic), e0, f0,
-                                        s0, weighting, wE, wF, wS, ridge, (hipStream_t)stream);
+                                        s0, weighting, wE, wF, wS, ridge, m->obj, (hipStream_t)stream);
     if (objective_out) *objective_out = v;
+  });
+}
+
+// ------------------------------------------------------------------ objective options
+void sus2_objective_default(sus2_objective* out) {
+  if (!out) return;
+  const sus2::ObjectiveOpts d;
+  *out = sus2_objective{d.rel_forces, d.norm_by_forces, d.force_loss, d.log_cosh_scale,
+                        d.std_weight, d.stdd_weight, d.constr_weight};
+}
+
+int sus2_objective_get(const sus2_model* m, sus2_objective* out) {
+  return guard([&] {
+    if (!m || !out) throw sus2::Error(SUS2_E_INVALID, "null model or options");
+    const sus2::ObjectiveOpts& o = m->obj;
+    *out = sus2_objective{o.rel_forces, o.norm_by_forces, o.force_loss, o.log_cosh_scale,
+                          o.std_weight, o.stdd_weight, o.constr_weight};
+  });
+}
+
+int sus2_objective_set(sus2_model* m, const sus2_objective* in) {
+  return guard([&] {
+    if (!m || !in) throw sus2::Error(SUS2_E_INVALID, "null model or options");
+    sus2::ObjectiveOpts o;
+    o.rel_forces = in->rel_forces;
+    o.norm_by_forces = in->norm_by_forces;
+    o.force_loss = in->force_loss;
+    o.log_cosh_scale = in->log_cosh_scale;
+    o.std_weight = in->std_weight;
+    o.stdd_weight = in->stdd_weight;
+    o.constr_weight = in->constr_weight;
+    o.validate();
+    if (o.std_terms() && m->model.C > 64)
+      throw sus2::Error(SUS2_E_UNSUPPORTED, "the std terms support at most 64 species");
+    m->obj = o;
   });
 }
 

@@ -10,11 +10,12 @@
 // x = 0 (a readout column's forces and virial come from that EFS; a species
 // column has n_t(f) in the energy rows only).  The rows of frame f, in order:
 // its energy, 3 N forces, 9 virial entries, scaled by the square roots of the
-// objective's weights (AddToSLAE, dev_src/mtpr_trainer.cpp:1186-1448 with
-// norm_by_forces = 0):
+// objective's weights (AddToSLAE, dev_src/mtpr_trainer.cpp:1186-1448):
 //   energy  wE / N   (structures: / N again; molecules: * N)
 //   forces  wF       (structures: / N)
 //   virial  wS / N   (structures: / N again; molecules: * N)
+// with the energy and force rows times 0.1 / (0.1 + norm_by_forces mean |F0|^2);
+// the objective's other options are not part of these equations.
 // A = J^T J is formed on the device by FP64 MFMA tiles (16 x 16 x 4) over
 // row chunks and reduced in a fixed order; b = J^T r likewise in FP64 VALU;
 // the n x n system (n = C + C Q) is solved on the host by Cholesky.
@@ -212,7 +213,8 @@ std::vector<double> spd_solve(std::vector<double> A, int n, int ld, std::vector<
 
 double train_linear(Model& host, DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* e0,
                     const double* f0, const double* s0, int mode, double wE, double wF, double wS, double ridge,
-                    hipStream_t st) {
+                    const ObjectiveOpts& obj, hipStream_t st) {
+  obj.validate();
   if (!host.species_readout)
     throw Error(SUS2_E_UNSUPPORTED, "linear readout solve needs a per-element readout (readout_mode = species)");
   if (mode < 0 || mode > 2) throw Error(SUS2_E_INVALID, "weighting must be 0, 1 or 2");
@@ -223,6 +225,13 @@ double train_linear(Model& host, DevModelSet& dm, Workspace& ws, int precision, 
   std::vector<int64_t> off(nf + 1);
   check(hipMemcpy(off.data(), b.frame_off, sizeof(int64_t) * (nf + 1), hipMemcpyDeviceToHost), "frame offsets");
   const int stress = wS != 0 ? 1 : 0;
+  // norm_by_forces: the frame's energy and force rows scaled by
+  // 0.1 / (0.1 + nbf mean |F0|^2) (dev_src/mtpr_trainer.cpp:1203-1212)
+  std::vector<double> lab;
+  if (obj.norm_by_forces != 0 && N > 0) {
+    lab.resize(3 * (size_t)N);
+    check(hipMemcpy(lab.data(), f0, sizeof(double) * lab.size(), hipMemcpyDeviceToHost), "force labels");
+  }
   std::vector<int64_t> roff(nf + 1, 0);
   std::vector<double> wrow(3 * (size_t)std::max(nf, 1));
   for (int f = 0; f < nf; ++f) {
@@ -231,6 +240,14 @@ double train_linear(Model& host, DevModelSet& dm, Workspace& ws, int precision, 
     double we = na > 0 ? wE / na : 0, wf = wF, wsv = na > 0 ? wS / na : 0;
     if (mode == 1 && na > 0) { we /= na; wf /= na; wsv /= na; }
     else if (mode == 2) { we *= na; wsv *= na; }
+    if (!lab.empty() && na > 0) {
+      double avef = 0;
+      for (int64_t i = off[f]; i < off[f + 1]; ++i)
+        avef += (lab[3 * i] * lab[3 * i] + lab[3 * i + 1] * lab[3 * i + 1] + lab[3 * i + 2] * lab[3 * i + 2]) / na;
+      const double nb = 0.1 / (0.1 + obj.norm_by_forces * avef);
+      we *= nb;
+      wf *= nb;
+    }
     wrow[3 * f] = std::sqrt(std::max(we, 0.0));
     wrow[3 * f + 1] = std::sqrt(std::max(wf, 0.0));
     wrow[3 * f + 2] = std::sqrt(std::max(wsv, 0.0));
@@ -325,7 +342,8 @@ double train_linear(Model& host, DevModelSet& dm, Workspace& ws, int precision, 
     for (int j = 0; j < n; ++j) s += A[(size_t)i * ld + j] * x[j];
     xAx += x[i] * (s - (i >= C ? ridge * x[i] : 0.0));
   }
-  return nf > 0 ? (rr - 2 * bx + xAx) / nf : 0.0;
+  // the radial penalty does not depend on the solved coefficients
+  return (nf > 0 ? (rr - 2 * bx + xAx) / nf : 0.0) + radial_penalty(host, host.theta.data(), obj.constr_weight, nullptr);
 }
 
 }  // namespace sus2

@@ -118,7 +118,7 @@ struct Workspace {
   DevBuf hdot, vgd, gmom, gmomd, msg, hbar, gpart, wpart, gfold;
   DevBuf rec_alt;  // FP32 copy of FP64 edge records (mixed-precision objective)
   // training objective: EFS outputs, loss adjoints and per-frame losses (FP64)
-  DevBuf lE, lF, lV, lew, lfw, lsw, lloss;
+  DevBuf lE, lF, lV, lew, lfw, lsw, lloss, lsite, lpen;
   long nedges = 0;
   // edge capacity (edges, not bytes) fixed by the last host-sized build: the
   // per-edge buffers of either precision are sized from it, so that a build
@@ -219,21 +219,40 @@ void efs_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Bat
 void grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* ene_w,
                    const double* frc_w, const double* str_w, double* grad, double scale, bool add,
                    bool reuse_neighbors, hipStream_t st);
-// Training objective (1/nframes) sum_f L_f of the batch with its parameter
-// gradient (grad overwritten); labels e0/f0/s0 on the device, loss to host.
-// loss_dev (optional): the objective is also written to this device word;
-// efs_out (optional): the EFS outputs of the batch go to these buffers.  With
-// loss_host null or in capture-safe mode nothing is read back.
+// Objective options of a handle (the NonLinearRegression members the
+// trainer sets; sus2_objective in the C ABI).  The defaults are those of
+// `mlp-sus2 train` for Static and Effect models.
+struct ObjectiveOpts {
+  double rel_forces = 0.0;     // wgt_rel_forces
+  int norm_by_forces = 0;      // norm_by_forces
+  int force_loss = 0;          // 0 L2, 1 log-cosh
+  double log_cosh_scale = 2.0;
+  double std_weight = 0.0, stdd_weight = 0.0;
+  double constr_weight = 1e-9;  // legacy radial norm / orthogonality penalty
+  bool std_terms() const { return std_weight != 0.0 || stdd_weight != 0.0; }
+  void validate() const;
+};
+// The penalty constr (sum_mu (|c_mu|^2 - 1)^2 + sum_{mu<nu} (c_mu . c_nu)^2)
+// over the outer radial rows of theta, and (grad != nullptr) its gradient
+// added to grad (host FP64; MLMTPR::AddPenaltyGrad, dev_src/mtpr.cpp:14050).
+double radial_penalty(const Model& m, const double* theta, double constr, double* grad);
+// Training objective (1/nframes) sum_f L_f + penalty of the batch with its
+// parameter gradient (grad overwritten); labels e0/f0/s0 on the device, loss
+// to host.  loss_dev (optional): the objective is also written to this device
+// word; efs_out (optional): the EFS outputs of the batch go to these buffers.
+// With loss_host null or in capture-safe mode nothing is read back.
 void loss_grad_dispatch(const DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* e0,
                         const double* f0, const double* s0, int mode, double wE, double wF, double wS,
-                        double* loss_host, double* grad, hipStream_t st, double* loss_dev = nullptr,
-                        const EfsOut* efs_out = nullptr);
+                        const ObjectiveOpts& obj, double* loss_host, double* grad, hipStream_t st,
+                        double* loss_dev = nullptr, const EfsOut* efs_out = nullptr);
 
 // Linear readout solve (species constants and per-element readout weights)
-// of the training objective; writes the solution into host.theta and the
-// device copies, returns the objective at the solution.
+// of the training objective (its norm_by_forces scaling and penalty; the
+// other options are not part of the reference's normal equations); writes the
+// solution into host.theta and the device copies, returns the objective at
+// the solution.
 double train_linear(Model& host, DevModelSet& dm, Workspace& ws, int precision, const Batch& b, const double* e0,
                     const double* f0, const double* s0, int mode, double wE, double wF, double wS, double ridge,
-                    hipStream_t st);
+                    const ObjectiveOpts& obj, hipStream_t st);
 
 }  // namespace sus2

@@ -15,6 +15,8 @@ import numpy as np
 from . import _lib
 
 PRECISION = {"fp64": _lib.SUS2_FP64, "fp32": _lib.SUS2_FP32, "mixed": _lib.SUS2_MIXED, 0: 0, 1: 1, 2: 2}
+FORCE_LOSS = {"l2": 0, "log-cosh": 1}
+FORCE_LOSS_NAME = {0: "l2", 1: "log-cosh"}
 DIM_NAMES = ("C", "L", "K", "B", "F", "M", "Q", "R", "O", "Qg", "Kw", "nbasic", "nprod", "gated",
              "species_readout", "H")
 OFF_NAMES = ("shift", "scal", "rad", "grad", "gscal", "alpha", "gtype", "W", "gw", "ow", "species", "read", "ncoeff")
@@ -112,6 +114,30 @@ class Sus2Model:
     def save(self, path):
         _lib.check(_lib.lib().sus2_model_save(self._h, str(path).encode()))
 
+    # ------------------------------------------------------------ objective options
+    def objective(self):
+        """The handle's objective options (sus2_objective) as a dict; a new
+        handle holds the options of `mlp-sus2 train` for these models."""
+        o = _lib.Sus2Objective()
+        _lib.check(_lib.lib().sus2_objective_get(self._h, ctypes.byref(o)))
+        d = {name: getattr(o, name) for name, _ in o._fields_}
+        d["force_loss"] = FORCE_LOSS_NAME[d["force_loss"]]
+        return d
+
+    def set_objective(self, **opts):
+        """Change objective options (rel_forces, norm_by_forces, force_loss
+        'l2' / 'log-cosh', log_cosh_scale, std_weight, stdd_weight,
+        constr_weight); unnamed options keep their values.  Used by the
+        objective calls, trainers and graphs made afterwards."""
+        cur = self.objective()
+        unknown = set(opts) - set(cur)
+        if unknown:
+            raise ValueError(f"unknown objective options {sorted(unknown)}")
+        cur.update(opts)
+        cur["force_loss"] = FORCE_LOSS.get(cur["force_loss"], cur["force_loss"])
+        o = _lib.Sus2Objective(**cur)
+        _lib.check(_lib.lib().sus2_objective_set(self._h, ctypes.byref(o)))
+
     # ------------------------------------------------------------ diagnostics
     def last_edge_count(self):
         n = ctypes.c_int64()
@@ -202,7 +228,8 @@ class Sus2Model:
 
     def loss_grad_device(self, batch, e0, f0, s0=None, weighting="structures", wE=1.0, wF=0.01, wS=0.0,
                          precision="fp32", grad=None, stream=None):
-        """Training objective (1/nframes) sum_f L_f and its gradient
+        """Training objective (1/nframes) sum_f L_f + penalty (the handle's
+        objective options) and its gradient
         (NonLinearRegression::CalcObjectiveFunctionGrad,
         dev_src/non_linear_regression.cpp:1112).  Labels are device tensors:
         e0 [nf], f0 [N,3], s0 [nf,6] (xx yy zz xy xz yz).  Returns

@@ -115,6 +115,66 @@ def test_loss_grad_matches_oracle(lib, oracle_lib, mode):
         assert _rel(g, ref["grad"]) <= tol
 
 
+GPU_OBJECTIVES = [
+    dict(rel_forces=0.5),
+    dict(norm_by_forces=1),
+    dict(force_loss="log-cosh", log_cosh_scale=0.05),
+    dict(std_weight=0.2, stdd_weight=1e-3),
+    dict(constr_weight=1e-4),
+    dict(rel_forces=2.0, norm_by_forces=2, force_loss="log-cosh", log_cosh_scale=0.5, std_weight=0.05,
+         stdd_weight=1e-4, constr_weight=1e-6),
+]
+
+
+@pytest.mark.parametrize("objective", GPU_OBJECTIVES, ids=lambda o: "-".join(o))
+def test_loss_grad_options_match_oracle(lib, oracle_lib, objective):
+    """Each objective option (set on the handle through sus2_objective_set)
+    against the oracle's objective with the same options: FP64 to 1e-12 in
+    the objective and 1e-10 in the gradient, FP32 and mixed as above."""
+    import torch
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import DeviceBatch, Sus2Model
+    print(f"[case] loss_grad options {objective}", flush=True)
+    path = W.model_path("effect_l3k4r2o2")
+    frames = _frames("mixed", 2, 81)
+    om = oracle_lib.OracleModel(path)
+    theta = om.coeffs()
+    rng = np.random.default_rng(82)
+    om.set_coeffs(theta * (1 + 0.05 * rng.normal(size=theta.size)))
+    teach = om.efs(frames)
+    om.set_coeffs(theta)
+    labels = dict(energy=teach["energy"], forces=teach["forces"], virial=teach["virial"])
+    ref = om.loss_grad(frames, labels, nthreads=16, mode=1, wE=1.0, wF=0.01, wS=0.001, objective=objective)
+    base = om.loss_grad(frames, labels, nthreads=16, mode=1, wE=1.0, wF=0.01, wS=0.001)
+    assert _rel(ref["grad"], base["grad"]) > 1e-8  # the option matters here
+    m = Sus2Model(path)
+    m.set_objective(**objective)
+    batch = DeviceBatch(frames)
+    e0, f0, s0 = (torch.from_numpy(teach[k]).cuda() for k in ("energy", "forces", "virial"))
+    for prec, tol in (("fp64", TOL64), ("fp32", TOL32), ("mixed", TOLMIX)):
+        loss, g = m.loss_grad_device(batch, e0, f0, s0, weighting=1, wE=1.0, wF=0.01, wS=0.001, precision=prec)
+        g = g.cpu().numpy()
+        print(f"[result] {prec}: loss rel {abs(loss - ref['loss']) / abs(ref['loss']):.3e} "
+              f"grad rel {_rel(g, ref['grad']):.3e}", flush=True)
+        assert abs(loss - ref["loss"]) <= (1e-6 if prec == "fp32" else 1e-12) * abs(ref["loss"])
+        assert _rel(g, ref["grad"]) <= tol
+
+
+def test_objective_options_rejected(lib):
+    """Invalid options fail with SUS2_E_INVALID and leave the handle's options."""
+    from superlinear_amd import _lib as L
+    from superlinear_amd import workloads as W
+    from superlinear_amd.model import Sus2Model
+    m = Sus2Model(W.model_path("effect_l3k4r2o2"))
+    before = m.objective()
+    for bad in (dict(log_cosh_scale=0.0), dict(rel_forces=-1.0), dict(std_weight=-0.1), dict(norm_by_forces=-1),
+                dict(force_loss=7), dict(constr_weight=float("inf"))):
+        with pytest.raises(L.Sus2Error) as e:
+            m.set_objective(**bad)
+        assert e.value.code == 1
+    assert m.objective() == before
+
+
 def test_autograd_function_backward(lib, oracle_lib):
     """EFSFunction: forward equals sus2_efs; backward of a linear functional of
     (E, F, V) equals the oracle's EFS-combination gradient with the matching
@@ -339,8 +399,8 @@ def test_back_to_back_objectives_on_growing_batches(lib):
         assert queued[0][0] != queued[1][0]
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2])
-def test_linear_readout_solve_is_stationary(lib, mode):
+@pytest.mark.parametrize("mode,objective", [(0, None), (1, None), (2, None), (1, dict(norm_by_forces=1, constr_weight=1e-4))])
+def test_linear_readout_solve_is_stationary(lib, mode, objective):
     """sus2_train_linear: the species constants and per-element readout
     weights it returns make the objective stationary in those coefficients
     (their gradient entries, by the independent gradient kernels, vanish to
@@ -349,7 +409,7 @@ def test_linear_readout_solve_is_stationary(lib, mode):
     import torch
     from superlinear_amd import workloads as W
     from superlinear_amd.model import DeviceBatch, Sus2Model
-    print(f"[case] linear readout mode {mode}", flush=True)
+    print(f"[case] linear readout mode {mode} objective {objective}", flush=True)
     path = W.model_path("effect_l3k4r2o2")
     frames = W.periodic_frames(8, 8, 40, 2, 0.15, seed=131) + W.molecule_frames(2, 8, 20, 2, seed=132)
     batch = DeviceBatch(frames)
@@ -358,6 +418,8 @@ def test_linear_readout_solve_is_stationary(lib, mode):
     teacher.set_coeffs(th * (1 + 0.05 * np.random.default_rng(133).standard_normal(th.size)))
     lab = teacher.efs_device(batch, "fp64")
     m = Sus2Model(path)
+    if objective:
+        m.set_objective(**objective)
     off = m.offsets()
     lin = np.r_[off["species"]:off["species"] + 2, off["read"]:off["ncoeff"]]
     kw = dict(weighting=mode, wE=1.0, wF=0.01, wS=0.001, precision="fp64")

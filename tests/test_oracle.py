@@ -278,7 +278,8 @@ def test_efs_grad_weights_by_finite_differences(oracle_lib, mname, species):
 
 def test_efs_grad_agrees_with_loss_grad(oracle_lib):
     """The loss gradient is the EFS-combination gradient with the loss
-    adjoints as weights, divided by the frame count."""
+    adjoints as weights, divided by the frame count, plus the radial
+    penalty's gradient (the default objective's constr_weight 1e-9)."""
     from superlinear_amd import workloads as W
     m = oracle_lib.OracleModel(W.model_path("effect_l3k4r2o2"))
     frames = W.periodic_frames(2, 5, 7, 2, 0.15, seed=43)
@@ -287,7 +288,9 @@ def test_efs_grad_agrees_with_loss_grad(oracle_lib):
     labels = dict(energy=rng.normal(size=2), forces=rng.normal(size=(n, 3)) * 0.1, virial=rng.normal(size=(2, 6)))
     out = m.loss_grad(frames, labels, mode=2, wE=1.0, wF=0.01, wS=0.001, nthreads=8)
     g = m.efs_grad(frames, out["ebar"], out["fbar"], out["sbar"], nthreads=8) / 2
-    close(g, out["grad"], 1e-12)
+    pv, pg = m.radial_penalty(1e-9)
+    assert pv > 0 and np.abs(pg).max() > 0
+    close(g + pg, out["grad"], 1e-12)
 
 
 # Hand-derived loss values, dev_src/non_linear_regression.cpp:986-1037
@@ -358,7 +361,8 @@ def test_frame_loss_invariants(oracle_lib, mode):
 
 def test_objective_is_mean_of_frame_losses(oracle_lib):
     """orc_loss = (1/K) sum_f L_f over the frames (the trainer's /K,
-    dev_src/mtpr_trainer.cpp:4737-4744), L_f from each frame's EFS."""
+    dev_src/mtpr_trainer.cpp:4737-4744), L_f from each frame's EFS, plus the
+    radial penalty once (added per frame, then divided by K)."""
     from superlinear_amd import workloads as W
     m = oracle_lib.OracleModel(W.model_path("effect_l3k4r2o2"))
     frames = W.periodic_frames(3, 5, 8, 2, 0.15, seed=17)
@@ -372,7 +376,9 @@ def test_objective_is_mean_of_frame_losses(oracle_lib):
         tot += oracle_lib.frame_loss(1, 1.0, 0.01, 0.001, r["energy"][f], r["forces"][a:a + k], r["virial"][f],
                                      labels["energy"][f], labels["forces"][a:a + k], labels["virial"][f])[0]
         a += k
-    assert abs(m.loss(frames, labels, 1, 1.0, 0.01, 0.001) - tot / 3) <= 1e-13 * tot
+    pen = m.radial_penalty(1e-9)[0]
+    assert abs(m.loss(frames, labels, 1, 1.0, 0.01, 0.001) - (tot / 3 + pen)) <= 1e-13 * tot
+    assert abs(m.loss(frames, labels, 1, 1.0, 0.01, 0.001, objective=dict(constr_weight=0.0)) - tot / 3) <= 1e-13 * tot
 
 
 HI_L = [("effect_l4k6r2o3", 3), ("effect_l5k6r2o3", 2)]
@@ -421,3 +427,188 @@ def test_forces_and_virial_by_finite_differences_high_l(oracle_lib, mname, speci
             return _energy(m, f)
         fd = -(strained(1) - strained(-1)) / (2 * h)
         assert abs(fd - W6[a, b]) <= 1e-7 * (1 + abs(fd))
+
+
+# ---------------------------------------------------------------- objective options
+# Hand-derived values (dev_src/non_linear_regression.cpp:936-1052, force_loss.h,
+# mtpr.cpp:14050).  Two atoms with F0 = [(1, 0, 0), (0, 0, 0)] (|F0|^2 = 1, 0),
+# force residuals dF = [(0.1, 0, 0), (0, 0.2, 0)], energy residual 0.5.
+OPT_F0 = np.array([[1.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+OPT_DF = np.array([[0.1, 0.0, 0.0], [0.0, 0.2, 0.0]])
+
+
+@pytest.mark.parametrize("mode,scale", [(0, 1.0), (1, 0.5)])
+def test_frame_loss_rel_forces_hand_derived(oracle_lib, mode, scale):
+    """rel_forces = 1: atom weights wF 1 / (|F0|^2 + 1) = 0.005 and 0.01, so
+    L = 0.005 * 0.01 + 0.01 * 0.04 = 4.5e-4 and dL/dF = (0.001, 0, 0),
+    (0, 0.004, 0) in vibrations; structures divides both by N = 2."""
+    L, dEi, dF, dS = oracle_lib.frame_loss(mode, 0.0, 0.01, 0.0, 0.0, OPT_F0 + OPT_DF, np.zeros(6), 0.0, OPT_F0,
+                                           np.zeros(6), objective=dict(rel_forces=1.0))
+    assert abs(L - 4.5e-4 * scale) <= 1e-18
+    assert np.allclose(dF, np.array([[0.001, 0, 0], [0, 0.004, 0]]) * scale, rtol=0, atol=1e-18)
+    assert not dEi.any()
+
+
+def test_frame_loss_norm_by_forces_hand_derived(oracle_lib):
+    """norm_by_forces = 1: mean |F0|^2 = 0.5, energy and force terms times
+    0.1 / (0.1 + 0.5) = 1/6; vibrations, wE = 1, wF = 0.01, wS = 0.001 with a
+    stress residual (0.3, 0, 0, 0, 0, 0): L = 0.25 / 6 / 2 + 0.01 * 0.05 / 6
+    + 0.001 * 0.09 / 2, dL/dE_i = 1/12, dL/dF = 0.02 dF / 6, and the stress
+    adjoint -0.001 dS is not scaled."""
+    dSv = np.array([0.3, 0, 0, 0, 0, 0])
+    L, dEi, dF, dS = oracle_lib.frame_loss(0, 1.0, 0.01, 0.001, 1.5, OPT_F0 + OPT_DF, dSv, 1.0, OPT_F0, np.zeros(6),
+                                           objective=dict(norm_by_forces=1))
+    want = 0.25 / 12 + 0.0005 / 6 + 0.000045
+    assert abs(L - want) <= 1e-16
+    assert np.allclose(dEi, 1.0 / 12.0, rtol=0, atol=1e-16)
+    assert np.allclose(dF, OPT_DF * (0.02 / 6), rtol=0, atol=1e-17)
+    assert abs(dS[0, 0] + 0.0003) <= 1e-18 and abs(dS).sum() - 0.0003 <= 1e-18
+
+
+def test_frame_loss_log_cosh(oracle_lib):
+    """Log-cosh force residual s^2 log cosh(d / s), s = 2: quadratic d^2 / 2
+    for small d (half the L2 term), linear s |d| - s^2 log 2 for large d,
+    even in d, and its adjoint is the derivative of the loss (central
+    differences over every force component)."""
+    one = np.zeros((1, 3))
+    obj = dict(force_loss="log-cosh", log_cosh_scale=2.0)
+    # (the reference's form |x| + log1p(e^-2|x|) - log 2 cancels to ~1e-16 s^2
+    # absolute at small |x|, hence the relative 1e-6 there)
+    for d, want, tol in ((1e-4, 0.5e-8, 1e-6), (50.0, 2.0 * 50.0 - 4.0 * np.log(2.0), 1e-14)):
+        for sgn in (1, -1):
+            F = one + np.array([[sgn * d, 0, 0]])
+            L = oracle_lib.frame_loss(0, 0.0, 1.0, 0.0, 0.0, F, np.zeros(6), 0.0, one, np.zeros(6), objective=obj)[0]
+            assert abs(L - want) <= tol * want, (d, sgn, L, want)
+    rng = np.random.default_rng(4)
+    F0 = rng.normal(size=(3, 3))
+    F = F0 + rng.normal(size=(3, 3)) * 1.5
+    for mode in (0, 1):
+        L, dEi, dF, dS = oracle_lib.frame_loss(mode, 0.0, 0.01, 0.0, 0.0, F, np.zeros(6), 0.0, F0, np.zeros(6),
+                                               objective=obj)
+        h = 1e-6
+        for i in range(3):
+            for a in range(3):
+                Fp, Fm = F.copy(), F.copy()
+                Fp[i, a] += h
+                Fm[i, a] -= h
+                lp = oracle_lib.frame_loss(mode, 0.0, 0.01, 0.0, 0.0, Fp, np.zeros(6), 0.0, F0, np.zeros(6),
+                                           objective=obj)[0]
+                lm = oracle_lib.frame_loss(mode, 0.0, 0.01, 0.0, 0.0, Fm, np.zeros(6), 0.0, F0, np.zeros(6),
+                                           objective=obj)[0]
+                assert abs((lp - lm) / (2 * h) - dF[i, a]) <= 1e-8 * (1 + abs(dF[i, a]))
+        assert abs(dF).max() <= 0.01 * 2.0 + 1e-15  # |s tanh| < s
+
+
+# Three atoms, species (0, 0, 1), site energies (1, 3, 5), label energy 6
+# (mean 2): species means 2 and 5.  With F0 = 0 the force factor 200 / (200 +
+# |F0|^2) is 1: std = ((1-2)^2 + (3-2)^2 + 0) / 3 = 2/3, stdd = 0 + 3^2 = 9,
+# dL/dE_i = w (2 ((E_i - <E>_t) / 3 - joint_t)) + v (2 (<E>_t - 2) / n_t) with
+# joint 0: (-w 2/3, w 2/3, v 6).  With F0_1 = (10, 0, 0) the factor of atom 1
+# is 2/3: std = (2/3 + 1) / 3 = 5/9, joint_0 = (-2/3 + 1) / 2 / 3 = 1/18,
+# dL/dE = (-w 5/9, w 5/9, v 6).
+@pytest.mark.parametrize("f0x,std,d01", [(0.0, 2.0 / 3.0, 2.0 / 3.0), (10.0, 5.0 / 9.0, 5.0 / 9.0)])
+def test_frame_loss_std_terms_hand_derived(oracle_lib, f0x, std, d01):
+    w, v = 0.2, 1e-3
+    types = np.array([0, 0, 1])
+    se = np.array([1.0, 3.0, 5.0])
+    F0 = np.zeros((3, 3))
+    F0[0, 0] = f0x
+    L, dEi, dF, dS = oracle_lib.frame_loss(0, 0.0, 0.0, 0.0, 9.0, F0, np.zeros(6), 6.0, F0, np.zeros(6),
+                                           objective=dict(std_weight=w, stdd_weight=v), site=se, types=types)
+    assert abs(L - (w * std + v * 9.0)) <= 1e-15
+    assert np.allclose(dEi, [-w * d01, w * d01, v * 6.0], rtol=0, atol=1e-15)
+    # the adjoint is the derivative of the loss in the site energies
+    h = 1e-6
+    for i in range(3):
+        sp, sm = se.copy(), se.copy()
+        sp[i] += h
+        sm[i] -= h
+        lp = oracle_lib.frame_loss(0, 0.0, 0.0, 0.0, 9.0, F0, np.zeros(6), 6.0, F0, np.zeros(6),
+                                   objective=dict(std_weight=w, stdd_weight=v), site=sp, types=types)[0]
+        lm = oracle_lib.frame_loss(0, 0.0, 0.0, 0.0, 9.0, F0, np.zeros(6), 6.0, F0, np.zeros(6),
+                                   objective=dict(std_weight=w, stdd_weight=v), site=sm, types=types)[0]
+        assert abs((lp - lm) / (2 * h) - dEi[i]) <= 1e-9
+
+
+def test_radial_penalty_hand_derived(oracle_lib):
+    """Outer radial rows (F = 16 of B = 10 coefficients) set to e_(mu mod 10):
+    every norm is 1 and rows mu, mu + 10 (mu < 6) coincide, so the penalty is
+    6 constr and the gradient of row mu (and mu + 10) is 2 constr e_mu; with
+    every row zero the penalty is 16 constr and the gradient vanishes; for
+    random rows the gradient is the derivative of the penalty."""
+    from superlinear_amd import workloads as W
+    m = oracle_lib.OracleModel(W.model_path("effect_l3k4r2o2"))
+    off = m.offsets()
+    o, C = off["rad"], m.dims()["C"]
+    theta = m.coeffs()
+    B, F, st = 10, 16, 10 + C
+    t = theta.copy()
+    for mu in range(F):
+        t[o + mu * st:o + mu * st + B] = 0.0
+        t[o + mu * st + mu % B] = 1.0
+    v, g = m.radial_penalty(0.5, t)
+    assert abs(v - 3.0) <= 1e-15
+    want = np.zeros_like(t)
+    for mu in range(F):
+        if mu % B < 6:
+            want[o + mu * st + mu % B] = 1.0
+    assert np.array_equal(g, want)
+    z = theta.copy()
+    for mu in range(F):
+        z[o + mu * st:o + mu * st + B] = 0.0
+    v, g = m.radial_penalty(0.5, z)
+    assert v == 8.0 and not g.any()
+    v, g = m.radial_penalty(0.7)
+    h = 1e-6
+    for p in [o, o + 3, o + st + 2, o + 5 * st + 9, o + 15 * st]:
+        tp, tm = theta.copy(), theta.copy()
+        tp[p] += h
+        tm[p] -= h
+        fd = (m.radial_penalty(0.7, tp)[0] - m.radial_penalty(0.7, tm)[0]) / (2 * h)
+        assert abs(fd - g[p]) <= 1e-8 * (1 + abs(g[p]))
+    # only the outer radial coefficients (not the species constants tau) enter
+    mask = np.zeros(len(theta), bool)
+    for mu in range(F):
+        mask[o + mu * st:o + mu * st + B] = True
+    assert not g[~mask].any()
+
+
+OBJECTIVE_CASES = [
+    dict(rel_forces=0.5),
+    dict(norm_by_forces=1),
+    dict(force_loss="log-cosh", log_cosh_scale=0.05),
+    dict(std_weight=0.2, stdd_weight=1e-3),
+    dict(constr_weight=1e-4),
+]
+
+
+@pytest.mark.parametrize("objective", OBJECTIVE_CASES, ids=lambda o: "-".join(o))
+def test_loss_gradient_with_options_by_finite_differences(oracle_lib, objective):
+    """With each objective option on, the hyper-dual gradient of the
+    objective matches central finite differences of the objective."""
+    from superlinear_amd import workloads as W
+    m = oracle_lib.OracleModel(W.model_path("effect_l3k4r2o2"))
+    frames = W.periodic_frames(2, 5, 7, 2, 0.15, seed=23)
+    n = sum(len(f["types"]) for f in frames)
+    rng = np.random.default_rng(12)
+    r = m.efs(frames)
+    labels = dict(energy=r["energy"] + rng.normal(size=2), forces=r["forces"] + rng.normal(size=(n, 3)) * 0.1,
+                  virial=rng.normal(size=(2, 6)))
+    kw = dict(mode=1, wE=1.0, wF=0.01, wS=0.001, objective=objective)
+    out = m.loss_grad(frames, labels, nthreads=4, **kw)
+    theta = m.coeffs()
+    g = out["grad"]
+    idx = np.unique(np.concatenate([np.argsort(-np.abs(g))[:12], rng.choice(len(theta), 12, replace=False)]))
+    h = 1e-6
+    for p in idx:
+        t = theta.copy()
+        t[p] += h
+        m.set_coeffs(t)
+        lp = m.loss(frames, labels, **kw)
+        t[p] -= 2 * h
+        m.set_coeffs(t)
+        lm = m.loss(frames, labels, **kw)
+        m.set_coeffs(theta)
+        assert abs((lp - lm) / (2 * h) - g[p]) <= 1e-5 * abs(g[p]) + 1e-9 * (1 + abs(out["loss"])), p
+    base = m.loss(frames, labels, mode=1, wE=1.0, wF=0.01, wS=0.001)
+    assert out["loss"] != base  # the option changes the objective
