@@ -74,13 +74,16 @@ __device__ __forceinline__ double lds_sum_where(const T* __restrict__ v, const T
 
 // ------------------------------------------------------------------ wave graph walks
 // Basic nodes take the moments; rank nodes the K -> rank contraction
-// (sus2_sl_kernel.cpp:104-105, 217-221).  v: LDS [M].
+// (sus2_sl_kernel.cpp:104-105, 217-221).  v: LDS window of nc node values
+// starting at node lo (node n at v[n - lo]; lo = 0, nc = M for all nodes);
+// basic nodes outside the window are not used by the pass and are skipped.
 template <class T, int KK>
-__device__ void wave_contract(const DModel<T>& m, const DPass<T>& p, bool gate, const T* mom, T* v) {
+__device__ void wave_contract(const DModel<T>& m, const DPass<T>& p, bool gate, const T* mom, T* v, int lo = 0,
+                              int nc = 1 << 30) {
   const int lane = threadIdx.x, H = m.H, KH = KK * H;
   for (int c = lane; c < KH; c += WAVE) {
-    const int node = m.basic_node[c];
-    if (node >= 0) v[node] = mom[c];
+    const int node = m.basic_node[c] - lo;
+    if (node >= 0 && node < nc) v[node] = mom[c];
   }
   const int nr = p.rank * H;
   for (int idx = lane; idx < nr; idx += WAVE) {
@@ -90,7 +93,7 @@ __device__ void wave_contract(const DModel<T>& m, const DPass<T>& p, bool gate, 
 #pragma unroll
     for (int k = 0; k < KK; ++k)
       if (m.basic_node[hh * KK + k] >= 0) s += m.rankw(gate, l, a, k) * mom[hh * KK + k];
-    v[rank_node(p.begin, p.rank, H, l, mm, a)] = s;
+    v[rank_node(p.begin, p.rank, H, l, mm, a) - lo] = s;
   }
   __syncthreads();
 }
@@ -113,7 +116,7 @@ __device__ __forceinline__ void wave_rows(int r0, int r1, Fn&& fn) {
 // Product layers with rows spread over the lanes, then per-output sums of the
 // staged row products in serialised row order (sus2_sl_kernel.cpp:308-313).
 template <class T>
-__device__ void wave_graph_forward(const DPass<T>& p, T* v, T* scratch) {
+__device__ void wave_graph_forward(const DPass<T>& p, T* v, T* scratch, int lo = 0) {
   const int lane = threadIdx.x;
   for (int l = 0; l < p.nfl; ++l) {
     const int4 bd = p.fl_bnd[l];
@@ -124,9 +127,9 @@ __device__ void wave_graph_forward(const DPass<T>& p, T* v, T* scratch) {
     if (oa < o1) {
       ob = p.f_roff[oa];
       oe = p.f_roff[oa + 1];
-      on = p.f_out[oa];
+      on = p.f_out[oa] - lo;
     }
-    wave_rows(r0, r1, [&](int r) { scratch[r - r0] = p.f_c[r] * v[p.f_a[r]] * v[p.f_b[r]]; });
+    wave_rows(r0, r1, [&](int r) { scratch[r - r0] = p.f_c[r] * v[p.f_a[r] - lo] * v[p.f_b[r] - lo]; });
     __syncthreads();
     if (oa < o1) {
       T s = T(0);
@@ -137,7 +140,7 @@ __device__ void wave_graph_forward(const DPass<T>& p, T* v, T* scratch) {
       T s = T(0);
       const int e = p.f_roff[o + 1] - r0;
       for (int r = p.f_roff[o] - r0; r < e; ++r) s += scratch[r];
-      v[p.f_out[o]] = s;
+      v[p.f_out[o] - lo] = s;
     }
     __syncthreads();
   }
@@ -146,7 +149,7 @@ __device__ void wave_graph_forward(const DPass<T>& p, T* v, T* scratch) {
 // Reverse: a[n] += sum over consumer rows c a[out] v[partner], nodes in
 // reverse-layer order; a holds the seeds on entry.
 template <class T>
-__device__ void wave_graph_reverse(const DPass<T>& p, const T* v, T* a, T* scratch) {
+__device__ void wave_graph_reverse(const DPass<T>& p, const T* v, T* a, T* scratch, int lo = 0) {
   const int lane = threadIdx.x;
   for (int l = 0; l < p.nrl; ++l) {
     const int4 bd = p.rl_bnd[l];
@@ -157,9 +160,9 @@ __device__ void wave_graph_reverse(const DPass<T>& p, const T* v, T* a, T* scrat
     if (ka < n1) {
       kb = p.r_roff[ka];
       ke = p.r_roff[ka + 1];
-      kn = p.r_node[ka];
+      kn = p.r_node[ka] - lo;
     }
-    wave_rows(q0, q1, [&](int q) { scratch[q - q0] = p.r_c[q] * a[p.r_out[q]] * v[p.r_partner[q]]; });
+    wave_rows(q0, q1, [&](int q) { scratch[q - q0] = p.r_c[q] * a[p.r_out[q] - lo] * v[p.r_partner[q] - lo]; });
     __syncthreads();
     if (ka < n1) {
       T s = a[kn];
@@ -167,7 +170,7 @@ __device__ void wave_graph_reverse(const DPass<T>& p, const T* v, T* a, T* scrat
       a[kn] = s;
     }
     for (int k = ka + WAVE; k < n1; k += WAVE) {
-      const int n = p.r_node[k];
+      const int n = p.r_node[k] - lo;
       T s = a[n];
       const int e = p.r_roff[k + 1] - q0;
       for (int q = p.r_roff[k] - q0; q < e; ++q) s += scratch[q];
@@ -179,7 +182,8 @@ __device__ void wave_graph_reverse(const DPass<T>& p, const T* v, T* a, T* scrat
 
 // Transposed contraction to canonical basic adjoints out[(h,k)].
 template <class T, int KK>
-__device__ void wave_adjoint_to_basic(const DModel<T>& m, const DPass<T>& p, bool gate, const T* a, T* out) {
+__device__ void wave_adjoint_to_basic(const DModel<T>& m, const DPass<T>& p, bool gate, const T* a, T* out,
+                                      int lo = 0, int nc = 1 << 30) {
   const int lane = threadIdx.x, H = m.H, KH = KK * H;
   for (int c = lane; c < KH; c += WAVE) {
     const int hh = c / KK, k = c % KK;
@@ -187,8 +191,10 @@ __device__ void wave_adjoint_to_basic(const DModel<T>& m, const DPass<T>& p, boo
     const int node = m.basic_node[c];
     T s = T(0);
     if (node >= 0) {
-      s = a[node];
-      for (int aa = 0; aa < p.rank; ++aa) s += m.rankw(gate, l, aa, k) * a[rank_node(p.begin, p.rank, H, l, mm, aa)];
+      // (a basic node outside the window has no adjoint in this pass)
+      if (node - lo >= 0 && node - lo < nc) s = a[node - lo];
+      for (int aa = 0; aa < p.rank; ++aa)
+        s += m.rankw(gate, l, aa, k) * a[rank_node(p.begin, p.rank, H, l, mm, aa) - lo];
     }
     out[c] = s;
   }
@@ -201,22 +207,59 @@ __host__ __device__ inline int graph_scratch(const DPass<T>& p) {
 
 // ------------------------------------------------------------------ pass forward
 // LDS of k_pass_forward: the species tables, then one region used first for
-// the edge staging [F + H][LR] and afterwards (the staging is dead once the
-// moments are in registers) for moments [KH], node values and adjoints [2M]
-// and the graph scratch -- the overlay keeps the footprint small enough for
-// every workgroup of a 4096-atom frame to be resident at once.
+// the edge staging [F + H][row] and afterwards (the staging is dead once the
+// moments are in registers) for moments [KH], the pass's node values and
+// adjoints [2 NC] (NC = the larger pass's contiguous node range) and the graph
+// scratch -- the overlay keeps the footprint small enough for every workgroup
+// of a 4096-atom frame to be resident at once.  Wide shapes (F > 16) stage a
+// batch of 64 edges in two halves of 32 rows (the second half's radial values
+// wait in registers, its harmonics are formed again), halving the staging.
 template <class Cf>
-__host__ __device__ inline size_t forward_region(int M, int sc) {
-  const size_t stage = (size_t)(Cf::F + Cf::H) * LR;
-  const size_t graph = (size_t)Cf::KH + 2 * (size_t)M + (size_t)(sc > 1 ? sc : 1);
+struct FwdStage {
+  static constexpr bool HALF = Cf::F > 16;
+  static constexpr int ROW = HALF ? WAVE / 2 + 1 : LR;  // padded staging row
+  static constexpr int COLS = HALF ? WAVE / 2 : WAVE;   // edges per staged batch
+};
+
+template <class Cf>
+__host__ __device__ inline size_t forward_region(int nc, int sc) {
+  const size_t stage = (size_t)(Cf::F + Cf::H) * FwdStage<Cf>::ROW;
+  const size_t graph = (size_t)Cf::KH + 2 * (size_t)nc + (size_t)(sc > 1 ? sc : 1);
   return stage > graph ? stage : graph;
+}
+
+// LDS node windows: the gate pass's range, and the larger of the two passes'.
+template <class T>
+__host__ __device__ inline int gate_nodes(const DModel<T>& m) {
+  return m.gp.node_cnt > 1 ? m.gp.node_cnt : 1;
+}
+
+template <class T>
+__host__ __device__ inline int pass_nodes(const DModel<T>& m) {
+  const int nc = m.gp.node_cnt > m.op.node_cnt ? m.gp.node_cnt : m.op.node_cnt;
+  return nc > 1 ? nc : 1;
 }
 
 template <class Cf>
 size_t forward_lds(const DModel<typename Cf::T>& m) {
   // the scratch also holds the gate scalars S [Qg] of the gate pass
   const int sc = std::max({graph_scratch(m.gp), graph_scratch(m.op), m.Qg});
-  return sizeof(typename Cf::T) * (species_tables_size<Cf>(m.C) + forward_region<Cf>(m.M, sc));
+  return sizeof(typename Cf::T) * (species_tables_size<Cf>(m.C) + forward_region<Cf>(pass_nodes(m), sc));
+}
+
+// Moments of one staged batch of `cnt` edges: lane = basic moment (k, h).
+template <class Cf, class T = typename Cf::T>
+__device__ __forceinline__ void stage_moments(const T* sR, const T* sY, int cnt, T* acc) {
+  constexpr int KK = Cf::KK, L1 = Cf::L1, KH = Cf::KH, NM = Cf::NM, ROW = FwdStage<Cf>::ROW;
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int t = 0; t < NM; ++t) {
+    const int c = lane + WAVE * t;
+    if (c < KH) {
+      const int hh = c / KK, k = c % KK;
+      acc[t] += lds_dot(sR + (k * L1 + l_of(hh)) * ROW, sY + hh * ROW, cnt);
+    }
+  }
 }
 
 // One centre atom i of a forward pass (the workgroup's LDS: species tables
@@ -227,13 +270,17 @@ __device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const Spec
                                                   const int* __restrict__ nbr_off, const Rec<T>* __restrict__ rec,
                                                   T* __restrict__ h, T* __restrict__ vg, T* __restrict__ site,
                                                   T* __restrict__ abar) {
-  constexpr int LL = Cf::LL, KK = Cf::KK, L1 = Cf::L1, H = Cf::H, F = Cf::F, KH = Cf::KH, NM = Cf::NM;
-  T* sR = region;                          // [F][LR]  edge phase
-  T* sY = sR + F * LR;                     // [H][LR]
-  T* mom = region;                         // [KH]     after the edge phase
-  T* v = mom + KH;                         // [M]
-  T* a = v + m.M;                          // [M]
-  T* scratch = a + m.M;
+  constexpr int LL = Cf::LL, KK = Cf::KK, H = Cf::H, F = Cf::F, KH = Cf::KH, NM = Cf::NM;
+  constexpr bool HALF = FwdStage<Cf>::HALF;
+  constexpr int ROW = FwdStage<Cf>::ROW, HW = WAVE / 2;
+  const DPass<T>& p = GATE ? m.gp : m.op;
+  const int lo = p.node_lo, nc = p.node_cnt;  // the pass's node window
+  T* sR = region;                          // [F][ROW]  edge phase
+  T* sY = sR + F * ROW;                    // [H][ROW]
+  T* mom = region;                         // [KH]      after the edge phase
+  T* v = mom + KH;                         // [nc]      node n at v[n - lo]
+  T* a = v + pass_nodes(m);                // [nc]
+  T* scratch = a + pass_nodes(m);
   const int lane = threadIdx.x;
   const int ti = types[i];
   const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
@@ -246,25 +293,58 @@ __device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const Spec
     const int e = base + lane;
     const Rec<T> q = qn;
     if (e + WAVE < e1) qn = rec[e + WAVE];
-    if (e < e1) {
-      const int j = rec_j(q);
-      const T r = dsqrt(q.x * q.x + q.y * q.y + q.z * q.z);
-      const T inv = T(1) / r;
-      const int tj = rec_t(q);
-      const T* x0i = tab.x0 + ti * F;
-      const T* x0j = tab.x0 + tj * F;
-      const T* alj = tab.alpha + tj * F;
-      if constexpr (F > 16) {  // wide shapes: each channel and harmonic staged as soon as formed
+    if constexpr (HALF) {
+      // wide shapes: lanes [0, 32) stage their edges at once, lanes [32, 64)
+      // keep their weighted radial values and stage them (and their
+      // harmonics, formed again) after the first half's moments
+      const bool first = lane < HW;
+      const int col = lane & (HW - 1);
+      T keep[F];
+      T ux = T(0), uy = T(0), uz = T(0);
+      if (e < e1) {
+        const int j = rec_j(q);
+        const T r = dsqrt(q.x * q.x + q.y * q.y + q.z * q.z);
+        const T inv = T(1) / r;
+        const int tj = rec_t(q);
+        const T* x0i = tab.x0 + ti * F;
+        const T* x0j = tab.x0 + tj * F;
+        const T* alj = tab.alpha + tj * F;
         T hj[F];
         if (!GATE && m.gated) load_row<T, F>(h + (size_t)j * F, hj);  // issued before the basis work
         radial_each<Cf, false>(m, GATE, tab, ti * m.C + tj, r, [&](int mu, T R, T) {
           T x1 = x0j[mu];
           if (!GATE && m.gated) x1 += alj[mu] * hj[mu];
-          sR[mu * LR + lane] = x0i[mu] * x1 * R;
+          keep[mu] = x0i[mu] * x1 * R;
+          if (first) sR[mu * ROW + col] = keep[mu];
         });
-        sh_stream_values<T, LL>(q.x * inv, q.y * inv, q.z * inv,
-                                [&](int l, int mm, T Y) { sY[hidx(l, mm) * LR + lane] = Y; });
-      } else {
+        ux = q.x * inv;
+        uy = q.y * inv;
+        uz = q.z * inv;
+        if (first) sh_stream_values<T, LL>(ux, uy, uz, [&](int l, int mm, T Y) { sY[hidx(l, mm) * ROW + col] = Y; });
+      }
+      __syncthreads();
+      stage_moments<Cf>(sR, sY, min(HW, e1 - base), acc);
+      __syncthreads();
+      const int cnt2 = min(HW, e1 - base - HW);
+      if (cnt2 > 0) {
+        if (!first && e < e1) {
+#pragma unroll
+          for (int mu = 0; mu < F; ++mu) sR[mu * ROW + col] = keep[mu];
+          sh_stream_values<T, LL>(ux, uy, uz, [&](int l, int mm, T Y) { sY[hidx(l, mm) * ROW + col] = Y; });
+        }
+        __syncthreads();
+        stage_moments<Cf>(sR, sY, cnt2, acc);
+        __syncthreads();
+      }
+    } else {
+      if (e < e1) {
+        const int j = rec_j(q);
+        const T r = dsqrt(q.x * q.x + q.y * q.y + q.z * q.z);
+        const T inv = T(1) / r;
+        const int tj = rec_t(q);
+        const T* x0i = tab.x0 + ti * F;
+        const T* x0j = tab.x0 + tj * F;
+        const T* alj = tab.alpha + tj * F;
         T Rv[F], Y[H], hj[F];
         if (!GATE && m.gated) load_row<T, F>(h + (size_t)j * F, hj);  // issued before the basis work
         radial<Cf, false>(m, GATE, tab, ti * m.C + tj, r, Rv, nullptr);
@@ -273,43 +353,32 @@ __device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const Spec
         for (int mu = 0; mu < F; ++mu) {
           T x1 = x0j[mu];
           if (!GATE && m.gated) x1 += alj[mu] * hj[mu];
-          sR[mu * LR + lane] = x0i[mu] * x1 * Rv[mu];
+          sR[mu * ROW + lane] = x0i[mu] * x1 * Rv[mu];
         }
 #pragma unroll
-        for (int hh = 0; hh < H; ++hh) sY[hh * LR + lane] = Y[hh];
+        for (int hh = 0; hh < H; ++hh) sY[hh * ROW + lane] = Y[hh];
       }
+      __syncthreads();
+      stage_moments<Cf>(sR, sY, min(WAVE, e1 - base), acc);
+      __syncthreads();
     }
-    __syncthreads();
-    const int cnt = min(WAVE, e1 - base);
-#pragma unroll
-    for (int t = 0; t < NM; ++t) {
-      const int c = lane + WAVE * t;
-      if (c < KH) {
-        const int hh = c / KK, k = c % KK;
-        const T* pr = sR + (k * L1 + l_of(hh)) * LR;
-        const T* py = sY + hh * LR;
-        acc[t] += lds_dot(pr, py, cnt);
-      }
-    }
-    __syncthreads();
   }
   // (the edge loop ends with a barrier, so the staging region is free here)
-  for (int n = lane; n < m.M; n += WAVE) v[n] = T(0);
+  for (int n = lane; n < nc; n += WAVE) v[n] = T(0);
 #pragma unroll
   for (int t = 0; t < NM; ++t) {
     const int c = lane + WAVE * t;
     if (c < KH) mom[c] = acc[t];
   }
   __syncthreads();
-  const DPass<T>& p = GATE ? m.gp : m.op;
-  wave_contract<T, KK>(m, p, GATE, mom, v);
-  wave_graph_forward(p, v, scratch);
+  wave_contract<T, KK>(m, p, GATE, mom, v, lo, nc);
+  wave_graph_forward(p, v, scratch, lo);
   if (GATE) {
     // h_i = W S_i (S gathered into the free scratch first; narrow shapes
     // spread each channel's sum over several lanes), node values of the gate
     // range for the reverse
     T* sS = scratch;
-    for (int q = lane; q < m.Qg; q += WAVE) sS[q] = v[m.gate_nodes[q]];
+    for (int q = lane; q < m.Qg; q += WAVE) sS[q] = v[m.gate_nodes[q] - lo];
     __syncthreads();
     if constexpr (F <= WAVE / 2) {
       // HP lanes per channel, each over a contiguous part of the scalars; the
@@ -346,25 +415,25 @@ __device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const Spec
       }
     }
     T* out = vg + (size_t)i * p.node_cnt;
-    for (int n = lane; n < p.node_cnt; n += WAVE) out[n] = v[p.node_lo + n];
+    for (int n = lane; n < p.node_cnt; n += WAVE) out[n] = v[n];
     return;
   }
   T part = T(0);
-  for (int q = lane; q < m.Q; q += WAVE) part += m.readout(ti, q) * v[m.scalar_map[q]];
+  for (int q = lane; q < m.Q; q += WAVE) part += m.readout(ti, q) * v[m.scalar_map[q] - lo];
   // the per-species constant (species + shift coefficients) is added in FP64
   // when the site energies are written out
   const T E = wave_sum(part);
   if (lane == 0) site[i] = E;
-  for (int n = lane; n < m.M; n += WAVE) a[n] = T(0);
+  for (int n = lane; n < nc; n += WAVE) a[n] = T(0);
   __syncthreads();
   if (m.scalar_unique) {  // distinct nodes: one lane per scalar
-    for (int q = lane; q < m.Q; q += WAVE) a[m.scalar_map[q]] += m.readout(ti, q);
+    for (int q = lane; q < m.Q; q += WAVE) a[m.scalar_map[q] - lo] += m.readout(ti, q);
   } else if (lane == 0) {  // repeated nodes: in scalar order
-    for (int q = 0; q < m.Q; ++q) a[m.scalar_map[q]] += m.readout(ti, q);
+    for (int q = 0; q < m.Q; ++q) a[m.scalar_map[q] - lo] += m.readout(ti, q);
   }
   __syncthreads();
-  wave_graph_reverse(p, v, a, scratch);
-  wave_adjoint_to_basic<T, KK>(m, p, false, a, abar + (size_t)i * KH);
+  wave_graph_reverse(p, v, a, scratch, lo);
+  wave_adjoint_to_basic<T, KK>(m, p, false, a, abar + (size_t)i * KH, lo, nc);
 }
 
 // Forward pass over atoms [0, natoms): one atom per workgroup (grid = natoms),
@@ -641,15 +710,17 @@ __global__ __launch_bounds__(WAVE) void k_edge_messages(DModel<T> m, int natoms,
 // Gate graph reverse of one atom from its gate-signal adjoint h_bar (LDS hbs,
 // alpha already applied): seeds S_bar_q = sum_mu h_bar_mu W_mu,q, reverse over
 // the stored gate node values, transposed contraction -> gate basic adjoints.
+// v, a: LDS windows over the gate pass's node range (node n at [n - node_lo]).
 template <class Cf, class T = typename Cf::T>
 __device__ void gate_reverse_atom(const DModel<T>& m, int i, const T* __restrict__ vg, T* __restrict__ gbar,
                                   const T* hbs, T* v, T* a, T* scratch) {
   constexpr int KK = Cf::KK, F = Cf::F;
   const int lane = threadIdx.x;
   const DPass<T>& p = m.gp;
-  const T* vi = vg + (size_t)i * p.node_cnt;
-  wave_rows(0, m.M, [&](int n) {  // four node loads in flight per lane
-    v[n] = (n >= p.node_lo && n < p.node_lo + p.node_cnt) ? vi[n - p.node_lo] : T(0);
+  const int lo = p.node_lo, nc = p.node_cnt;
+  const T* vi = vg + (size_t)i * nc;
+  wave_rows(0, nc, [&](int n) {  // four node loads in flight per lane
+    v[n] = vi[n];
     a[n] = T(0);
   });
   __syncthreads();
@@ -661,11 +732,11 @@ __device__ void gate_reverse_atom(const DModel<T>& m, int i, const T* __restrict
       s1 += hbs[mu + 1] * m.W(mu + 1, q);
     }
     if (F % 2) s0 += hbs[F - 1] * m.W(F - 1, q);
-    a[m.gate_nodes[q]] = s0 + s1;
+    a[m.gate_nodes[q] - lo] = s0 + s1;
   }
   __syncthreads();
-  wave_graph_reverse(p, v, a, scratch);
-  wave_adjoint_to_basic<T, KK>(m, p, true, a, gbar + (size_t)i * Cf::KH);
+  wave_graph_reverse(p, v, a, scratch, lo);
+  wave_adjoint_to_basic<T, KK>(m, p, true, a, gbar + (size_t)i * Cf::KH, lo, nc);
 }
 
 // F_a -= sum_{e' in in(a)} g_e' over the atom's incoming-list slots (written
@@ -736,9 +807,10 @@ __device__ __forceinline__ void gather_atom(const DModel<T>& m, int i, const int
     }
   }
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* v = reinterpret_cast<T*>(smem_raw);  // [M]
-  T* a = v + m.M;                          // [M]
-  T* hbs = a + m.M;                        // [F]
+  const int gn = gate_nodes(m);
+  T* v = reinterpret_cast<T*>(smem_raw);  // [gn] gate-pass node window
+  T* a = v + gn;                           // [gn]
+  T* hbs = a + gn;                         // [F]
   T* scratch = hbs + F;
   if constexpr (F >= 16) {
     if (hgate) {
@@ -849,9 +921,10 @@ __global__ __launch_bounds__(WAVE) void k_gate_reverse(DModel<T> m, const T* __r
                                                        const T* __restrict__ vg, T* __restrict__ gbar) {
   constexpr int F = Cf::F;
   extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int gn = gate_nodes(m);
   T* v = reinterpret_cast<T*>(smem_raw);
-  T* a = v + m.M;
-  T* hbs = a + m.M;
+  T* a = v + gn;
+  T* hbs = a + gn;
   T* scratch = hbs + F;
   const int i = blockIdx.x;
   for (int mu = threadIdx.x; mu < F; mu += WAVE) hbs[mu] = hbar[(size_t)i * F + mu];
@@ -863,7 +936,7 @@ __global__ __launch_bounds__(WAVE) void k_gate_reverse(DModel<T> m, const T* __r
 template <class Cf>
 __host__ __device__ inline int gather_region(const DModel<typename Cf::T>& m) {
   const int sc = graph_scratch(m.gp);
-  const int walk = 2 * m.M + Cf::F + (sc > 1 ? sc : 1);
+  const int walk = 2 * gate_nodes(m) + Cf::F + (sc > 1 ? sc : 1);
   const int sums = Cf::F >= 16 ? (3 + Cf::F) * LR : 0;
   return walk > sums ? walk : sums;
 }
