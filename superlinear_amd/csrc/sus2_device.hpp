@@ -211,29 +211,56 @@ __device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
 
 // Per-workgroup copies (LDS) of the small per-species coefficient tables of
 // one pass, so per-edge lookups with the neighbour's species are LDS reads.
+// The pass's radial coefficients sit there too, one row of BB values per
+// radial function padded to whole 16-byte vectors: every lane of a wave reads
+// the same row, so a row is a few broadcast LDS reads straight into the FMA
+// operands.  (Read from the coefficient vector instead, these uniform values
+// are hoisted into scalar registers -- F x BB of them, far more than a wave
+// holds -- and every use then pays a lane read of the spilled copy.)
 template <class T>
 struct SpeciesTables {
   T* slope;  // [NBLK][C*C]
   T* shift;  // [NBLK][C*C]
   T* x0;     // [C][F] centre factor: tau (outer) or gate type g (gate)
   T* alpha;  // [C][F] (outer pass of an Effect model)
+  T* coef;   // [F][coef_row] radial coefficients, 16-byte aligned rows
 };
 
+// Radial coefficient row length: BB rounded up to whole 16-byte vectors.
 template <class Cf>
-__host__ __device__ constexpr int species_tables_size(int C) {
-  return 2 * Cf::NBLK * C * C + 2 * C * Cf::F;
+__host__ __device__ constexpr int coef_row() {
+  constexpr int v = 16 / (int)sizeof(typename Cf::T);
+  return (Cf::BB + v - 1) / v * v;
 }
 
-// Fill the tables from the coefficient vector (all lanes of the workgroup).
-template <class Cf>
+// LDS elements of the tables; CL: with the radial coefficient rows (the last
+// term is room to align them to 16 bytes).
+template <class Cf, bool CL = true>
+__host__ __device__ constexpr int species_tables_size(int C) {
+  return 2 * Cf::NBLK * C * C + 2 * C * Cf::F + (CL ? Cf::F * coef_row<Cf>() + 16 / (int)sizeof(typename Cf::T) : 0);
+}
+
+// First 16-byte aligned element at or after p (p aligned to sizeof(T); an
+// LDS pointer stays an LDS pointer).
+template <class T>
+__device__ __forceinline__ T* align16(T* p) {
+  const int mis = (int)(reinterpret_cast<uintptr_t>(p) & 15);
+  return mis ? p + (16 - mis) / (int)sizeof(T) : p;
+}
+
+// Fill the tables from the coefficient vector (all lanes of the workgroup);
+// CL: with the radial coefficient rows.
+template <class Cf, bool CL = true>
 __device__ void load_species_tables(const DModel<typename Cf::T>& m, bool gate, typename Cf::T* base,
                                     SpeciesTables<typename Cf::T>& t) {
   using T = typename Cf::T;
+  constexpr int BB = Cf::BB, CR = coef_row<Cf>();
   const int C = m.C, CC = C * C, F = Cf::F;
   t.slope = base;
   t.shift = base + Cf::NBLK * CC;
   t.x0 = t.shift + Cf::NBLK * CC;
   t.alpha = t.x0 + C * F;
+  t.coef = CL ? align16(t.alpha + C * F) : nullptr;
   const int so = (gate && m.o_gscal >= 0) ? m.o_gscal : m.o_scal;
   for (int k = threadIdx.x; k < Cf::NBLK * CC; k += blockDim.x) {
     const int b = k / CC, pq = k % CC;
@@ -245,6 +272,16 @@ __device__ void load_species_tables(const DModel<typename Cf::T>& m, bool gate, 
     t.x0[k] = gate ? m.gtype(s, mu) : m.tau(s, mu);
     t.alpha[k] = (!gate && m.gated) ? m.alpha(s, mu) : T(0);
   }
+  if constexpr (!CL) return;
+  // radial coefficients: the gate pass's own block when the model has one,
+  // else the outer block (rows of BB basis weights, then C type factors)
+  const bool own_rad = gate && m.o_grad >= 0;
+  const T* __restrict__ coef = m.theta + (own_rad ? m.o_grad : m.o_rad);
+  const int stride = own_rad ? BB : BB + C;
+  for (int k = threadIdx.x; k < F * CR; k += blockDim.x) {
+    const int mu = k / CR, q = k % CR;
+    t.coef[k] = q < BB ? coef[mu * stride + q] : T(0);
+  }
 }
 
 // Radial functions R_mu(r) (and dR/dr when DER) of one directed edge with
@@ -255,62 +292,108 @@ __device__ void load_species_tables(const DModel<typename Cf::T>& m, bool gate, 
 // recurrence is evaluated once and only for the channels that share it.
 // Each channel goes to fn(mu, R_mu, dR_mu/dr) as soon as it is formed; only
 // the warp blocks [B0, B1) are evaluated.
-template <class Cf, bool DER, int B0 = 0, int B1 = Cf::NBLK, class Fn>
+// Radial functions of warp blocks [B0, B1) in evaluation order (block by
+// block, channels (k, l) of a block in order): the i-th one's index mu and
+// block, and how many there are.  Constant-folded once the callers' loops are
+// unrolled.
+template <class Cf, int B0, int B1>
+__host__ __device__ constexpr int radial_count() {
+  int n = 0;
+  for (int b = B0; b < B1; ++b)
+    for (int k = 0; k < Cf::KK; ++k)
+      for (int l = 0; l < Cf::L1; ++l) n += Cf::blk(k, l) == b;
+  return n;
+}
+template <class Cf, int B0, int B1>
+__host__ __device__ constexpr int radial_at(int i, bool want_block) {
+  for (int b = B0; b < B1; ++b)
+    for (int k = 0; k < Cf::KK; ++k)
+      for (int l = 0; l < Cf::L1; ++l)
+        if (Cf::blk(k, l) == b && i-- == 0) return want_block ? b : k * Cf::L1 + l;
+  return -1;
+}
+
+template <class Cf, bool DER, int B0 = 0, int B1 = Cf::NBLK, bool CL = true, class Fn>
 __device__ __forceinline__ void radial_each(const DModel<typename Cf::T>& m, bool gate,
                                             const SpeciesTables<typename Cf::T>& t, int pair, typename Cf::T r,
                                             Fn&& fn) {
   using T = typename Cf::T;
-  constexpr int L1 = Cf::L1, BB = Cf::BB;
-  const bool own_rad = gate && m.o_grad >= 0;
-  const T* __restrict__ coef = m.theta + (own_rad ? m.o_grad : m.o_rad);
-  const int stride = own_rad ? BB : BB + m.C;
+  constexpr int L1 = Cf::L1, BB = Cf::BB, CR = coef_row<Cf>(), N = radial_count<Cf, B0, B1>();
   const int CC = m.C * m.C;
   const T d0 = r - m.rc, f0 = d0 * d0, g0 = T(2) * d0;
   const bool act = r < m.rca;
   const T d1 = r - m.rca, f1 = act ? d1 * d1 : T(0), g1 = act ? T(2) * d1 : T(0);
+  // Coefficient rows come from LDS two at a time: row i + 1 is read while row
+  // i is used.  Each read's address takes a zero from an empty asm whose
+  // inputs are the edge's distance (row 0 and 1) or the sums of row i - 1: the
+  // reads stay inside the caller's edge loop and wait for the row two back to
+  // be formed, so the F x BB coefficients are never all held in registers
+  // (read once out of the loop, or all at the start of an edge, they would
+  // be).  The asm touches no memory, so loads and stores around it move
+  // freely.
+  // Without the LDS rows (CL false), rows are read from the coefficient
+  // vector, and the compiler keeps them in scalar registers.
+  const bool own_rad = gate && m.o_grad >= 0;
+  const T* __restrict__ gcoef = m.theta + (own_rad ? m.o_grad : m.o_rad);
+  const int gstride = own_rad ? BB : BB + m.C;
+  T cbuf[2][CR];
+  if constexpr (CL) {
+    int z;
+    asm("" : "=v"(z) : "0"(0), "v"(r));
+    load_row<T, CR>(t.coef + (z + radial_at<Cf, B0, B1>(0, false) * CR), cbuf[0]);
+  }
+  T dep0 = r, dep1 = r;  // the last row's sums
+  T Tb[BB], Db[BB];
 #pragma unroll
-  for (int b = B0; b < B1; ++b) {
-    const T slope = t.slope[b * CC + pair], shift = t.shift[b * CC + pair];
-    const T xi = dtanh(slope * (r - shift) * T(0.5));
-    const T dxi = (T(1) - xi * xi) * slope * T(0.5);
-    T Tb[BB], Db[BB];
-    Tb[0] = m.scaling;
-    Db[0] = T(0);
-    if (BB > 1) {
-      Tb[1] = m.scaling * xi;
-      Db[1] = m.scaling * dxi;
+  for (int i = 0; i < N; ++i) {
+    const int mu = radial_at<Cf, B0, B1>(i, false), b = radial_at<Cf, B0, B1>(i, true);
+    const int l = mu % L1;
+    if (CL && i + 1 < N) {
+      int z;
+      asm("" : "=v"(z) : "0"(0), "v"(dep0), "v"(dep1));
+      load_row<T, CR>(t.coef + (z + radial_at<Cf, B0, B1>(i + 1, false) * CR), cbuf[(i + 1) & 1]);
     }
+    if (i == 0 || radial_at<Cf, B0, B1>(i - 1, true) != b) {
+      // the block's tanh and Chebyshev recurrence, once for its channels
+      const T slope = t.slope[b * CC + pair], shift = t.shift[b * CC + pair];
+      const T xi = dtanh(slope * (r - shift) * T(0.5));
+      const T dxi = (T(1) - xi * xi) * slope * T(0.5);
+      Tb[0] = m.scaling;
+      Db[0] = T(0);
+      if (BB > 1) {
+        Tb[1] = m.scaling * xi;
+        Db[1] = m.scaling * dxi;
+      }
 #pragma unroll
-    for (int q = 2; q < BB; ++q) {
-      Tb[q] = T(2) * xi * Tb[q - 1] - Tb[q - 2];
-      if (DER) Db[q] = T(2) * (dxi * Tb[q - 1] + xi * Db[q - 1]) - Db[q - 2];
-    }
-#pragma unroll
-    for (int k = 0; k < Cf::KK; ++k) {
-#pragma unroll
-      for (int l = 0; l < L1; ++l) {
-        if (Cf::blk(k, l) != b) continue;
-        const int mu = k * L1 + l;
-        const T* c = coef + mu * stride;
-        T s = T(0), sd = T(0);
-#pragma unroll
-        for (int q = 0; q < BB; ++q) {
-          s += c[q] * Tb[q];
-          if (DER) sd += c[q] * Db[q];
-        }
-        const T f = l == 0 ? f0 : f1, g = l == 0 ? g0 : g1;
-        fn(mu, f * s, DER ? g * s + f * sd : T(0));
+      for (int q = 2; q < BB; ++q) {
+        Tb[q] = T(2) * xi * Tb[q - 1] - Tb[q - 2];
+        if (DER) Db[q] = T(2) * (dxi * Tb[q - 1] + xi * Db[q - 1]) - Db[q - 2];
       }
     }
+    if constexpr (!CL) {
+#pragma unroll
+      for (int q = 0; q < BB; ++q) cbuf[i & 1][q] = gcoef[mu * gstride + q];
+    }
+    const T* c = cbuf[i & 1];
+    T s = T(0), sd = T(0);
+#pragma unroll
+    for (int q = 0; q < BB; ++q) {
+      s += c[q] * Tb[q];
+      if (DER) sd += c[q] * Db[q];
+    }
+    dep0 = s;
+    dep1 = DER ? sd : s;
+    const T f = l == 0 ? f0 : f1, g = l == 0 ? g0 : g1;
+    fn(mu, f * s, DER ? g * s + f * sd : T(0));
   }
 }
 
 // All channels into R[F] (and dR[F] when DER).
-template <class Cf, bool DER>
+template <class Cf, bool DER, bool CL = true>
 __device__ __forceinline__ void radial(const DModel<typename Cf::T>& m, bool gate, const SpeciesTables<typename Cf::T>& t,
                                        int pair, typename Cf::T r, typename Cf::T* R, typename Cf::T* dR) {
   using T = typename Cf::T;
-  radial_each<Cf, DER>(m, gate, t, pair, r, [&](int mu, T v, T dv) {
+  radial_each<Cf, DER, 0, Cf::NBLK, CL>(m, gate, t, pair, r, [&](int mu, T v, T dv) {
     R[mu] = v;
     if (DER) dR[mu] = dv;
   });

@@ -217,6 +217,10 @@ __host__ __device__ inline int graph_scratch(const DPass<T>& p) {
 template <class Cf>
 struct FwdStage {
   static constexpr bool HALF = Cf::F > 16;
+  // radial coefficient rows staged in LDS (narrow shapes); wide shapes read
+  // them from the coefficient vector: the LDS rows and the registers their
+  // reads hold would cost these passes a workgroup per SIMD
+  static constexpr bool CL = !HALF;
   static constexpr int ROW = HALF ? WAVE / 2 + 1 : LR;  // padded staging row
   static constexpr int COLS = HALF ? WAVE / 2 : WAVE;   // edges per staged batch
 };
@@ -244,7 +248,8 @@ template <class Cf>
 size_t forward_lds(const DModel<typename Cf::T>& m) {
   // the scratch also holds the gate scalars S [Qg] of the gate pass
   const int sc = std::max({graph_scratch(m.gp), graph_scratch(m.op), m.Qg});
-  return sizeof(typename Cf::T) * (species_tables_size<Cf>(m.C) + forward_region<Cf>(pass_nodes(m), sc));
+  return sizeof(typename Cf::T) *
+         (species_tables_size<Cf, FwdStage<Cf>::CL>(m.C) + forward_region<Cf>(pass_nodes(m), sc));
 }
 
 // Moments of one staged batch of `cnt` edges: lane = basic moment (k, h).
@@ -311,7 +316,7 @@ __device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const Spec
         const T* alj = tab.alpha + tj * F;
         T hj[F];
         if (!GATE && m.gated) load_row<T, F>(h + (size_t)j * F, hj);  // issued before the basis work
-        radial_each<Cf, false>(m, GATE, tab, ti * m.C + tj, r, [&](int mu, T R, T) {
+        radial_each<Cf, false, 0, Cf::NBLK, FwdStage<Cf>::CL>(m, GATE, tab, ti * m.C + tj, r, [&](int mu, T R, T) {
           T x1 = x0j[mu];
           if (!GATE && m.gated) x1 += alj[mu] * hj[mu];
           keep[mu] = x0i[mu] * x1 * R;
@@ -347,7 +352,7 @@ __device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const Spec
         const T* alj = tab.alpha + tj * F;
         T Rv[F], Y[H], hj[F];
         if (!GATE && m.gated) load_row<T, F>(h + (size_t)j * F, hj);  // issued before the basis work
-        radial<Cf, false>(m, GATE, tab, ti * m.C + tj, r, Rv, nullptr);
+        radial<Cf, false, FwdStage<Cf>::CL>(m, GATE, tab, ti * m.C + tj, r, Rv, nullptr);
         sh_values<T, LL>(q.x * inv, q.y * inv, q.z * inv, Y);
 #pragma unroll
         for (int mu = 0; mu < F; ++mu) {
@@ -448,9 +453,9 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, int natoms, 
                                                        T* __restrict__ abar) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* tabs_base = reinterpret_cast<T*>(smem_raw);
-  T* region = tabs_base + species_tables_size<Cf>(m.C);
+  T* region = tabs_base + species_tables_size<Cf, FwdStage<Cf>::CL>(m.C);
   SpeciesTables<T> tab;
-  load_species_tables<Cf>(m, GATE, tabs_base, tab);
+  load_species_tables<Cf, FwdStage<Cf>::CL>(m, GATE, tabs_base, tab);
   __syncthreads();
   if constexpr (!PERSIST) {
     pass_forward_atom<Cf, GATE>(m, tab, region, blockIdx.x, types, nbr_off, rec, h, vg, site, abar);
@@ -537,6 +542,10 @@ __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const Sp
     const T inv = T(1) / r;
     const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
     const int tj = rec_t(q);
+    // the neighbour's gate-signal row: one batch of vector loads, issued
+    // before the basis work (not one dependent load per channel)
+    T hj[F];
+    if (hgate) load_row<T, F>(h + (size_t)j * F, hj);
     // weighted radial values x0 x1 R and x0 x1 R' in registers; x0 R (the gate
     // message factor) parked in this lane's own LDS column
     T R[F], dR[F], us[F];
@@ -546,7 +555,7 @@ __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const Sp
 #pragma unroll
     for (int mu = 0; mu < F; ++mu) {
       T x1 = x0j[mu];
-      if (hgate) x1 += alj[mu] * h[(size_t)j * F + mu];
+      if (hgate) x1 += alj[mu] * hj[mu];
       if (hmsg) sXR[mu * LR + lane] = x0[mu] * R[mu];
       const T wt = x0[mu] * x1;
       R[mu] *= wt;
