@@ -73,19 +73,27 @@ constexpr double dfact(int m) {  // (2m-1)!!
 }
 __host__ __device__ constexpr int hidx(int l, int m) { return l * l + l + m; }
 
-// Normalisations and (2m-1)!! for l <= 6, evaluated at compile time.
-struct YTable {
-  double n[49];
-  double df[7];
+// Normalised associated-Legendre recurrences for l <= 6, evaluated at compile
+// time.  With P_l = n_lm p_l (p_l the Legendre factor in u_z and rho = |u|^2,
+// n_lm the harmonic's normalisation), the three-term recurrence
+//   p_l = ((2l - 1) u_z p_{l-1} - (l + m - 1) rho p_{l-2}) / (l - m)
+// becomes P_l = a_lm u_z P_{l-1} - b_lm rho P_{l-2}, so no normalisation is
+// multiplied in per harmonic; P_m = n_mm (2m-1)!!.
+struct YRec {
+  double a[49], b[49], p0[7];
 };
-constexpr YTable make_ytable() {
-  YTable t{};
-  for (int l = 0; l <= 6; ++l)
-    for (int m = 0; m <= l; ++m) t.n[l * l + l + m] = ynorm(l, m);
-  for (int m = 0; m <= 6; ++m) t.df[m] = dfact(m);
+constexpr YRec make_yrec() {
+  YRec t{};
+  for (int m = 0; m <= 6; ++m) {
+    t.p0[m] = ynorm(m, m) * dfact(m);
+    for (int l = m + 1; l <= 6; ++l) {
+      t.a[l * l + l + m] = (2 * l - 1) * ynorm(l, m) / ((l - m) * ynorm(l - 1, m));
+      t.b[l * l + l + m] = l >= m + 2 ? (l + m - 1) * ynorm(l, m) / ((l - m) * ynorm(l - 2, m)) : 0.0;
+    }
+  }
   return t;
 }
-inline constexpr YTable kYT = make_ytable();
+inline constexpr YRec kYR = make_yrec();
 
 // Compile-time shape of a model: max l, channels K, radial basis size B and
 // warp-sharing mode SM (0 pair, 1 L, 2 K, 3 LK).
@@ -99,11 +107,19 @@ struct Cfg {
 };
 
 // ------------------------------------------------------------------ harmonics
+// One step of the normalised Legendre recurrence, P_l from P_{l-1}, P_{l-2}
+// (rho = 1): both harmonic streams use it, so their values agree bit for bit.
+template <class T>
+__device__ __forceinline__ T legendre_step(int l, int m, T uz, T p1, T p2) {
+  const T a = T(kYR.a[hidx(l, m)]), b = T(kYR.b[hidx(l, m)]);
+  return l == m + 1 ? a * (uz * p1) : a * (uz * p1) - b * p2;
+}
+
 // Real Y_lm(u) for a unit vector u, l <= LL, flat index l^2+l+m; Condon-
-// Shortley phase cancelled, Y_1{-1,0,1} ~ (y, z, x).  Associated Legendre
-// part by the three-term recurrence in l; azimuthal part (ux + i uy)^m.
-// Values streamed to fn(l, m, Y) in (m, l) order, so a caller can consume
-// each one at once instead of holding all H.
+// Shortley phase cancelled, Y_1{-1,0,1} ~ (y, z, x).  Legendre part by the
+// normalised recurrence in l; azimuthal part (ux + i uy)^m.  Values streamed
+// to fn(l, m, Y) in (m, l) order, so a caller can consume each one at once
+// instead of holding all H.
 template <class T, int LL, class Fn>
 __device__ __forceinline__ void sh_stream_values(T ux, T uy, T uz, Fn&& fn) {
   T cm = T(1), sm = T(0);
@@ -117,12 +133,13 @@ __device__ __forceinline__ void sh_stream_values(T ux, T uy, T uz, Fn&& fn) {
     T p2 = T(0), p1 = T(0);
 #pragma unroll
     for (int l = m; l <= LL; ++l) {
-      T p;
-      if (l == m) p = T(kYT.df[m]);
-      else p = (T(2 * l - 1) * uz * p1 - T(l + m - 1) * p2) * T(1.0 / (l - m));
-      const T n = T(kYT.n[hidx(l, m)]);
-      fn(l, m, n * p * cm);
-      if (m > 0) fn(l, -m, n * p * sm);
+      const T p = l == m ? T(kYR.p0[m]) : legendre_step(l, m, uz, p1, p2);
+      if (m == 0) {
+        fn(l, 0, p);
+      } else {
+        fn(l, m, p * cm);
+        fn(l, -m, p * sm);
+      }
       p2 = p1;
       p1 = p;
     }
@@ -134,14 +151,19 @@ __device__ __forceinline__ void sh_values(T ux, T uy, T uz, T* Y) {
   sh_stream_values<T, LL>(ux, uy, uz, [&](int l, int m, T v) { Y[hidx(l, m)] = v; });
 }
 
-// Y_lm(d/|d|) and grad_d Y_lm streamed to fn(l, m, Y, gx, gy, gz) in (m, l)
-// order.  The Legendre factor is carried as a polynomial in (u_z, rho = |u|^2)
-// with both partial derivatives so that the solid harmonic S = r^l Y and its
-// gradient are exact; grad_d Y = (grad_u S - l S u) / r.
+// Y_lm(u) and the gradient G = grad_u S_lm of the solid harmonic
+// S_lm(x) = |x|^l Y_lm(x/|x|) at x = u (|u| = 1), streamed to
+// fn(l, m, Y, Gx, Gy, Gz) in (m, l) order.  The Legendre factor is carried as
+// a polynomial in (u_z, rho = |u|^2) with both partial derivatives, so S and
+// G are exact polynomials.  For an edge d = r u the Cartesian gradient of the
+// harmonic is grad_d Y_lm = (G - l Y u) / r: a caller summing
+// sum_h c_h grad_d Y_h forms (sum c G - u sum c l Y) / r once per edge
+// instead of per harmonic.
 template <class T, int LL, class Fn>
-__device__ __forceinline__ void sh_stream_grad(T ux, T uy, T uz, T inv_r, Fn&& fn) {
+__device__ __forceinline__ void sh_stream_solid(T ux, T uy, T uz, Fn&& fn) {
   T cm = T(1), sm = T(0);
   T cx = T(0), cy = T(0), sx = T(0), sy = T(0);  // d(cm)/dux, d(cm)/duy, d(sm)/dux, d(sm)/duy
+  const T ux2 = T(2) * ux, uy2 = T(2) * uy, uz2 = T(2) * uz;
 #pragma unroll
   for (int m = 0; m <= LL; ++m) {
     if (m > 0) {
@@ -151,32 +173,34 @@ __device__ __forceinline__ void sh_stream_grad(T ux, T uy, T uz, T inv_r, Fn&& f
       sm = ux * sm + uy * cm;
       cm = c2;
     }
-    T p2 = T(0), p1 = T(0), z2 = T(0), z1 = T(0), r2 = T(0), r1 = T(0);
+    // P = n p, Z = n dp/du_z, Q = n dp/drho at rho = 1
+    T p2 = T(0), p1 = T(0), z2 = T(0), z1 = T(0), q2 = T(0), q1 = T(0);
 #pragma unroll
     for (int l = m; l <= LL; ++l) {
-      T p, pz, pr;
-      if (l == m) { p = T(kYT.df[m]); pz = T(0); pr = T(0); }
-      else {
-        const T A = T(2 * l - 1), Bc = T(l + m - 1), inv = T(1.0 / (l - m));
-        p = (A * uz * p1 - Bc * p2) * inv;
-        pz = (A * (p1 + uz * z1) - Bc * z2) * inv;
-        pr = (A * uz * r1 - Bc * (p2 + r2)) * inv;
+      T p, z, q;
+      if (l == m) {
+        p = T(kYR.p0[m]); z = T(0); q = T(0);
+      } else {
+        const T a = T(kYR.a[hidx(l, m)]), b = T(kYR.b[hidx(l, m)]);
+        p = legendre_step(l, m, uz, p1, p2);
+        if (l == m + 1) {
+          z = a * p1;
+          q = T(0);
+        } else {
+          z = a * (p1 + uz * z1) - b * z2;
+          q = a * (uz * q1) - b * (p2 + q2);
+        }
       }
-      const T n = T(kYT.n[hidx(l, m)]);
-      const T gpx = T(2) * ux * pr, gpy = T(2) * uy * pr, gpz = pz + T(2) * uz * pr;
-      {
-        const T S = n * p * cm;
-        const T gx = n * (gpx * cm + p * cx), gy = n * (gpy * cm + p * cy), gz = n * gpz * cm;
-        const T lS = T(l) * S;
-        fn(l, m, S, (gx - lS * ux) * inv_r, (gy - lS * uy) * inv_r, (gz - lS * uz) * inv_r);
+      if (m == 0) {
+        // cm = 1, sm = 0: no azimuthal factor
+        if (l < 2) fn(l, 0, p, T(0), T(0), z);
+        else fn(l, 0, p, ux2 * q, uy2 * q, z + uz2 * q);
+      } else {
+        const T gx = ux2 * q, gy = uy2 * q, gz = z + uz2 * q;
+        fn(l, m, p * cm, gx * cm + p * cx, gy * cm + p * cy, gz * cm);
+        fn(l, -m, p * sm, gx * sm + p * sx, gy * sm + p * sy, gz * sm);
       }
-      if (m > 0) {
-        const T S = n * p * sm;
-        const T gx = n * (gpx * sm + p * sx), gy = n * (gpy * sm + p * sy), gz = n * gpz * sm;
-        const T lS = T(l) * S;
-        fn(l, -m, S, (gx - lS * ux) * inv_r, (gy - lS * uy) * inv_r, (gz - lS * uz) * inv_r);
-      }
-      p2 = p1; p1 = p; z2 = z1; z1 = pz; r2 = r1; r1 = pr;
+      p2 = p1; p1 = p; z2 = z1; z1 = z; q2 = q1; q1 = q;
     }
   }
 }

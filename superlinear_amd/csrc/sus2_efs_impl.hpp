@@ -562,8 +562,11 @@ __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const Sp
       dR[mu] *= wt;
       us[mu] = T(0);
     }
-    T sY = T(0), gx = T(0), gy = T(0), gz = T(0);
-    sh_stream_grad<T, LL>(ux, uy, uz, inv, [&](int l, int mm, T Y, T yx, T yy, T yz) {
+    // sY = sum_h (sum_k abar R') Y_h, g = sum_h (sum_k abar R) G_h and, per
+    // degree l, ls[l] = sum (sum_k abar R) Y_h: the harmonics' Cartesian
+    // gradients (G - l Y u) / r are assembled once after the stream
+    T sY = T(0), gx = T(0), gy = T(0), gz = T(0), lsum = T(0);
+    sh_stream_solid<T, LL>(ux, uy, uz, [&](int l, int mm, T Y, T Gx, T Gy, T Gz) {
       const int hh = hidx(l, mm);
       // gate pass: keep the moment-adjoint LDS reads of this (l, m) here;
       // hoisted over the whole unrolled stream they would hold K H registers
@@ -579,9 +582,11 @@ __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const Sp
         if (hmsg) us[mu] += av * Y;
       }
       sY += adR * Y;
-      gx += aR * yx; gy += aR * yy; gz += aR * yz;
+      gx += aR * Gx; gy += aR * Gy; gz += aR * Gz;
+      if (l > 0) lsum += (T(l) * aR) * Y;
     });
-    const T g0 = ux * sY + gx, g1 = uy * sY + gy, g2 = uz * sY + gz;
+    const T sr = sY - lsum * inv;
+    const T g0 = ux * sr + gx * inv, g1 = uy * sr + gy * inv, g2 = uz * sr + gz * inv;
     const int slot = in_pos[e];  // the edge's position in its target's incoming list
     T* gi = gedge_in + 3 * (size_t)slot;
     gi[0] = g0; gi[1] = g1; gi[2] = g2;

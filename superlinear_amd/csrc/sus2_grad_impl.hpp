@@ -315,10 +315,11 @@ __device__ void grad_sweep_moments(const DModel<typename Cf::T>& m, const Specie
         sS1[c * LR + lane] = x1 * R[c];
         sS2[c * LR + lane] = x1 * dR[c] * wr + xd * R[c];
       }
-      sh_stream_grad<T, LL>(ux, uy, uz, inv, [&](int l, int mm, T Y, T yx, T yy, T yz) {
+      // w . grad_d Y = (w . G - l Y (u . w)) / r
+      sh_stream_solid<T, LL>(ux, uy, uz, [&](int l, int mm, T Y, T Gx, T Gy, T Gz) {
         const int hh = hidx(l, mm);
         sY[hh * LR + lane] = Y;
-        sYw[hh * LR + lane] = yx * wx + yy * wy + yz * wz;
+        sYw[hh * LR + lane] = (Gx * wx + Gy * wy + Gz * wz - T(l) * wr * Y) * inv;
       });
     }
     __syncthreads();
@@ -389,9 +390,7 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
   T* sCd = sCv + FP * LR;       // [FP][LR]
   T* sX1 = sCd + FP * LR;       // [FP][LR]  x0 Z  (d Phi / d x1)
   T* sAl = sX1 + FP * LR;       // [FP][LR]  x0 (Z h_j + R q h_dot_j)  (outer, gated)
-  const bool own_rad = GATE && m.o_grad >= 0;
-  const T* __restrict__ coef = m.theta + (own_rad ? m.o_grad : m.o_rad);
-  const int stride = own_rad ? BB : BB + C;
+  constexpr int CR = coef_row<Cf>();
   const int so = (GATE && m.o_gscal >= 0) ? m.o_gscal : m.o_scal;
   const T* x0 = tab.x0 + ti * F;
   // c-gradient lanes: (channel of the half, subset)
@@ -422,10 +421,16 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
       const int pair = ti * C + tj;
       const T* x0j = tab.x0 + tj * F;
       const T* alj = tab.alpha + tj * F;
+      // the radial coefficient rows come from LDS; a zero the compiler cannot
+      // see through (an empty asm of the edge's distance) keeps their reads in
+      // the edge loop instead of holding all of them in registers
+      int zc;
+      asm("" : "=v"(zc) : "0"(0), "v"(r));
       // per-channel work of one radial function mu = (k, l): p, q, q' from the
-      // harmonics, the radial values and their coefficient / warp gradients
+      // harmonics, the radial values and their coefficient / warp gradients;
+      // hj, hdj: the neighbour's gate signal and its tangent for mu (gated)
       auto channel = [&](int mu, int k, int l, const T* Yv, const T* Ywv, const T* Tb, const T* T1, const T* T2,
-                         T xr, T xs, T xt, T xrs, T xrt, T& ds, T& dt) {
+                         T xr, T xs, T xt, T xrs, T xrt, T hj, T hdj, T& ds, T& dt) {
         T pp = T(0), qv = T(0), qw = T(0);
 #pragma unroll
         for (int mm = -l; mm <= l; ++mm) {
@@ -435,7 +440,8 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
           qv += qc * Yv[hh];
           qw += qc * Ywv[hh];
         }
-        const T* c = coef + mu * stride;
+        T c[CR];
+        load_row<T, CR>(tab.coef + (zc + mu * CR), c);
         T C0 = T(0), C1 = T(0), C2 = T(0);
 #pragma unroll
         for (int qq = 0; qq < BB; ++qq) {
@@ -445,10 +451,8 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
         }
         const T f = l == 0 ? f0 : f1, g = l == 0 ? g0 : g1;
         const T R = f * C0, Rp = g * C0 + f * C1 * xr;
-        T x1 = x0j[mu], xd = T(0), hj = T(0), hdj = T(0);
+        T x1 = x0j[mu], xd = T(0);
         if (hg) {
-          hj = h[(size_t)j * F + mu];
-          hdj = hd[(size_t)j * F + mu];
           x1 += alj[mu] * hj;
           xd = alj[mu] * hdj;
         }
@@ -479,21 +483,30 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
         cheb3<BB>(m.scaling, xi, Tb, T1, T2);
       };
       T Yv[Cf::H], Ywv[Cf::H];
-      sh_stream_grad<T, LL>(ux, uy, uz, inv, [&](int l, int mm, T Y, T yx, T yy, T yz) {
+      // w . grad_d Y = (w . G - l Y (u . w)) / r
+      sh_stream_solid<T, LL>(ux, uy, uz, [&](int l, int mm, T Y, T Gx, T Gy, T Gz) {
         const int hh = hidx(l, mm);
         Yv[hh] = Y;
-        Ywv[hh] = yx * wx + yy * wy + yz * wz;
+        Ywv[hh] = (Gx * wx + Gy * wy + Gz * wz - T(l) * wr * Y) * inv;
       });
       if constexpr (Cf::SM == 2) {
         // blocks are the channels k of the half: one block's basis live at a time
 #pragma unroll 1
         for (int k = K0; k < K0 + KP; ++k) {
+          // the block's channels of the neighbour's gate row, loaded together
+          // before the basis work
+          T hk[L1], hdk[L1];
+#pragma unroll
+          for (int l = 0; l < L1; ++l) {
+            hk[l] = hg ? h[(size_t)j * F + k * L1 + l] : T(0);
+            hdk[l] = hg ? hd[(size_t)j * F + k * L1 + l] : T(0);
+          }
           T Tb[BB], T1[BB], T2[BB], xi, xr, xs, xt, xrs, xrt;
           block_basis(k, Tb, T1, T2, xi, xr, xs, xt, xrs, xrt);
           T ds = T(0), dt = T(0);
 #pragma unroll
           for (int l = 0; l < L1; ++l)
-            channel(k * L1 + l, k, l, Yv, Ywv, Tb, T1, T2, xr, xs, xt, xrs, xrt, ds, dt);
+            channel(k * L1 + l, k, l, Yv, Ywv, Tb, T1, T2, xr, xs, xt, xrs, xrt, hk[l], hdk[l], ds, dt);
           sXi[(k - B0) * LR + lane] = xi;
           sXr[(k - B0) * LR + lane] = xr;
           sDs[(k - B0) * LR + lane] = ds;
@@ -509,7 +522,10 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
           for (int k = 0; k < KK; ++k)
 #pragma unroll
             for (int l = 0; l < L1; ++l)
-              if (Cf::blk(k, l) == b) channel(k * L1 + l, k, l, Yv, Ywv, Tb, T1, T2, xr, xs, xt, xrs, xrt, ds, dt);
+              if (Cf::blk(k, l) == b)
+                channel(k * L1 + l, k, l, Yv, Ywv, Tb, T1, T2, xr, xs, xt, xrs, xrt,
+                        hg ? h[(size_t)j * F + k * L1 + l] : T(0), hg ? hd[(size_t)j * F + k * L1 + l] : T(0), ds,
+                        dt);
           sXi[b * LR + lane] = xi;
           sXr[b * LR + lane] = xr;
           sDs[b * LR + lane] = ds;
