@@ -230,8 +230,24 @@ __device__ __forceinline__ void load_row(const T* __restrict__ p, T* out) {
 
 __device__ __forceinline__ float dtanh(float x) { return tanhf(x); }
 __device__ __forceinline__ double dtanh(double x) { return tanh(x); }
-__device__ __forceinline__ float dsqrt(float x) { return sqrtf(x); }
-__device__ __forceinline__ double dsqrt(double x) { return sqrt(x); }
+// Length r of an edge vector with squared length d2 (> 0) and its reciprocal.
+// FP32: one hardware reciprocal square root instead of a square root and a
+// division, corrected once with the residual e = 1 - d2 y^2 formed by a fused
+// multiply-add, y <- y + y e / 2 (close to the correctly rounded reciprocal);
+// r is a product the compiler may not fuse into the differences that use it
+// (r - rc, r - shift), so every kernel sees the same r and passes that
+// recompute an edge agree bit for bit.  FP64: correctly rounded square root
+// and division.
+__device__ __forceinline__ void edge_length(float d2, float& r, float& inv) {
+  const float y = __builtin_amdgcn_rsqf(d2);
+  const float e = fmaf(-(d2 * y), y, 1.0f);
+  inv = fmaf(0.5f * y, e, y);
+  r = __fmul_rn(d2, inv);
+}
+__device__ __forceinline__ void edge_length(double d2, double& r, double& inv) {
+  r = sqrt(d2);
+  inv = 1.0 / r;
+}
 
 // Per-workgroup copies (LDS) of the small per-species coefficient tables of
 // one pass, so per-edge lookups with the neighbour's species are LDS reads.

@@ -870,16 +870,15 @@ __global__ void k_incoming(long nedges, const Rec<T>* __restrict__ rec, const in
 }
 
 // ------------------------------------------------------------------ finalize
-// One 256-thread block per frame: FP64 sums of site energies and virial
-// partials in a fixed order (strided per thread, then a fixed tree).
 // Frame reduction in chunks of FIN_CHUNK atoms: block (f, s) covers atoms
 // [frame_off[f] + s FIN_CHUNK, ...) of frame f, writes their FP64 forces and
 // site energies, and sums energy and virial in a fixed order (per-thread
 // strided sums, a fixed wave-shuffle tree, then the waves in order).  With
 // one chunk per frame (nsub == 1) the block writes the frame's energy and
-// virial; otherwise its partial, and k_finalize_frames adds the chunks in
-// order.
-constexpr int FIN_THREADS = 1024, FIN_CHUNK = 4096;
+// virial; otherwise its partial, and k_finalize_frames adds the chunks in a
+// fixed order.  Chunks of 512 atoms spread a large frame over many compute
+// units (one block per 4096-atom frame was a single CU's load bandwidth).
+constexpr int FIN_THREADS = 256, FIN_CHUNK = 512;
 
 template <class T>
 __global__ __launch_bounds__(FIN_THREADS) void k_finalize(int nsub, const int64_t* __restrict__ frame_off,
@@ -944,15 +943,29 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(int nsub, const int64_
   }
 }
 
-__global__ void k_finalize_frames(int nframes, int nsub, const double* __restrict__ part,
-                                  double* __restrict__ energy, double* __restrict__ virial) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nframes * 7) return;
-  const int f = idx / 7, q = idx % 7;
-  double v = 0;
-  for (int s = 0; s < nsub; ++s) v += part[((size_t)f * nsub + s) * 7 + q];
-  if (q == 0) energy[f] = v;
-  else virial[6 * f + q - 1] = v;
+// One block per frame: the frame's chunk partials summed in a fixed order
+// (per-thread strided sums, a fixed wave-shuffle tree, the waves in order).
+constexpr int FINF_THREADS = 256;
+__global__ __launch_bounds__(FINF_THREADS) void k_finalize_frames(int nsub, const double* __restrict__ part,
+                                                                  double* __restrict__ energy,
+                                                                  double* __restrict__ virial) {
+  __shared__ double red[7][FINF_THREADS / WAVE];
+  const int f = blockIdx.x, t = threadIdx.x;
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int s = t; s < nsub; s += FINF_THREADS)
+#pragma unroll
+    for (int q = 0; q < 7; ++q) acc[q] += part[((size_t)f * nsub + s) * 7 + q];
+  wave_sum_n(acc);
+  if (t % WAVE == 0)
+#pragma unroll
+    for (int q = 0; q < 7; ++q) red[q][t / WAVE] = acc[q];
+  __syncthreads();
+  if (t < 7) {
+    double v = 0;
+    for (int k = 0; k < FINF_THREADS / WAVE; ++k) v += red[t][k];
+    if (t == 0) energy[f] = v;
+    else virial[6 * f + t - 1] = v;
+  }
 }
 
 // Forces of atoms [first, natoms) outside every frame (the ghosts of a pair-style call).
@@ -1204,8 +1217,8 @@ void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, const SiteCons
                 b.types, sc, ws.site.get<T>(), ws.fw.get<T>(), out.energy, out.virial, out.force, out.site,
                 ws.fin_part.get<double>());
     if (nsub > 1)
-      SUS2_LAUNCH(ws, st, "finalize_frames", k_finalize_frames, dim3((7 * b.nframes + 255) / 256), dim3(256), 0,
-                  b.nframes, nsub, ws.fin_part.get<double>(), out.energy, out.virial);
+      SUS2_LAUNCH(ws, st, "finalize_frames", k_finalize_frames, dim3(b.nframes), dim3(FINF_THREADS), 0, nsub,
+                  ws.fin_part.get<double>(), out.energy, out.virial);
   }
   if (first < N)
     SUS2_LAUNCH(ws, st, "atom_out", k_atom_out<T>, dim3((N - first + 255) / 256), dim3(256), 0, first, N,
