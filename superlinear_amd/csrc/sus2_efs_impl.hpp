@@ -308,8 +308,8 @@ __device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const Spec
       T ux = T(0), uy = T(0), uz = T(0);
       if (e < e1) {
         const int j = rec_j(q);
-        const T r = dsqrt(q.x * q.x + q.y * q.y + q.z * q.z);
-        const T inv = T(1) / r;
+        T r, inv;
+        edge_length(q.x * q.x + q.y * q.y + q.z * q.z, r, inv);
         const int tj = rec_t(q);
         const T* x0i = tab.x0 + ti * F;
         const T* x0j = tab.x0 + tj * F;
@@ -344,8 +344,8 @@ __device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const Spec
     } else {
       if (e < e1) {
         const int j = rec_j(q);
-        const T r = dsqrt(q.x * q.x + q.y * q.y + q.z * q.z);
-        const T inv = T(1) / r;
+        T r, inv;
+        edge_length(q.x * q.x + q.y * q.y + q.z * q.z, r, inv);
         const int tj = rec_t(q);
         const T* x0i = tab.x0 + ti * F;
         const T* x0j = tab.x0 + tj * F;
@@ -538,8 +538,8 @@ __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const Sp
     const Rec<T> q = rec[e];
     const int j = rec_j(q);
     const T dx = q.x, dy = q.y, dz = q.z;
-    const T r = dsqrt(dx * dx + dy * dy + dz * dz);
-    const T inv = T(1) / r;
+    T r, inv;
+    edge_length(dx * dx + dy * dy + dz * dz, r, inv);
     const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
     const int tj = rec_t(q);
     // the neighbour's gate-signal row: one batch of vector loads, issued
@@ -686,8 +686,8 @@ __device__ __forceinline__ void edge_messages_group(const DModel<T>& m, const Sp
     const Rec<T> q = rec[e];
     const int tj = rec_t(q);
     const T dx = q.x, dy = q.y, dz = q.z;
-    const T r = dsqrt(dx * dx + dy * dy + dz * dz);
-    const T inv = T(1) / r;
+    T r, inv;
+    edge_length(dx * dx + dy * dy + dz * dz, r, inv);
     T us[F];
 #pragma unroll
     for (int mu = 0; mu < F; ++mu) us[mu] = T(0);
@@ -798,8 +798,8 @@ __device__ __forceinline__ void gather_atom(const DModel<T>& m, int i, const int
           const Rec<T> q = rec[k];
           const int j = rec_j(q), tj = rec_t(q);
           const T dx = -q.x, dy = -q.y, dz = -q.z;  // j -> i
-          const T r = dsqrt(dx * dx + dy * dy + dz * dz);
-          const T inv = T(1) / r;
+          T r, inv;
+          edge_length(dx * dx + dy * dy + dz * dz, r, inv);
           const T* aj = abar + (size_t)j * KH;
           T S[F];
 #pragma unroll

@@ -284,8 +284,8 @@ __device__ void grad_sweep_moments(const DModel<typename Cf::T>& m, const Specie
       const Rec<T> q = rec[e];
       const int j = rec_j(q);
       const T dx = q.x, dy = q.y, dz = q.z;
-      const T r = dsqrt(dx * dx + dy * dy + dz * dz);
-      const T inv = T(1) / r;
+      T r, inv;
+      edge_length(dx * dx + dy * dy + dz * dz, r, inv);
       const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
       const int tj = rec_t(q);
       const double* fj = frc_w + 3 * (size_t)j;
@@ -405,8 +405,8 @@ __device__ void grad_sweep_params(const DModel<typename Cf::T>& m, const Species
       const Rec<T> q = rec[e];
       const int j = rec_j(q);
       const T dx = q.x, dy = q.y, dz = q.z;
-      const T r = dsqrt(dx * dx + dy * dy + dz * dz);
-      const T inv = T(1) / r;
+      T r, inv;
+      edge_length(dx * dx + dy * dy + dz * dz, r, inv);
       const T ux = dx * inv, uy = dy * inv, uz = dz * inv;
       const int tj = rec_t(q);
       lane_t = tj;
