@@ -252,9 +252,9 @@ size_t forward_lds(const DModel<typename Cf::T>& m) {
          (species_tables_size<Cf, FwdStage<Cf>::CL>(m.C) + forward_region<Cf>(pass_nodes(m), sc));
 }
 
-// Moments of one staged batch of `cnt` edges: lane = basic moment (k, h).
+// Moments of the staged columns [c0, c1) of a batch: lane = basic moment (k, h).
 template <class Cf, class T = typename Cf::T>
-__device__ __forceinline__ void stage_moments(const T* sR, const T* sY, int cnt, T* acc) {
+__device__ __forceinline__ void stage_moments(const T* sR, const T* sY, int c0, int c1, T* acc) {
   constexpr int KK = Cf::KK, L1 = Cf::L1, KH = Cf::KH, NM = Cf::NM, ROW = FwdStage<Cf>::ROW;
   const int lane = threadIdx.x;
 #pragma unroll
@@ -262,42 +262,60 @@ __device__ __forceinline__ void stage_moments(const T* sR, const T* sY, int cnt,
     const int c = lane + WAVE * t;
     if (c < KH) {
       const int hh = c / KK, k = c % KK;
-      acc[t] += lds_dot(sR + (k * L1 + l_of(hh)) * ROW, sY + hh * ROW, cnt);
+      acc[t] += lds_dot(sR + (k * L1 + l_of(hh)) * ROW + c0, sY + hh * ROW + c0, c1 - c0);
     }
   }
 }
 
-// One centre atom i of a forward pass (the workgroup's LDS: species tables
-// already loaded into tab, then `region`).
-template <class Cf, bool GATE, class T = typename Cf::T>
-__device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const SpeciesTables<T>& tab, T* region, int i,
-                                                  const int32_t* __restrict__ types,
-                                                  const int* __restrict__ nbr_off, const Rec<T>* __restrict__ rec,
-                                                  T* __restrict__ h, T* __restrict__ vg, T* __restrict__ site,
-                                                  T* __restrict__ abar) {
-  constexpr int LL = Cf::LL, KK = Cf::KK, H = Cf::H, F = Cf::F, KH = Cf::KH, NM = Cf::NM;
+// Moments of each of the NA centres from the staged columns [0, cnt) that
+// hold edges [first, first + cnt): centre a owns edges [eb[a], eb[a + 1]).
+template <class Cf, int NA, class T = typename Cf::T>
+__device__ __forceinline__ void stage_moments_split(const T* sR, const T* sY, int first, int cnt, const int* eb,
+                                                    T (&acc)[NA][Cf::NM]) {
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    const int c0 = max(0, min(cnt, eb[a] - first)), c1 = max(0, min(cnt, eb[a + 1] - first));
+    if (c1 > c0) stage_moments<Cf>(sR, sY, c0, c1, acc[a]);
+  }
+}
+
+// Edge phase of a forward pass for NA consecutive centres i0 .. i0 + na - 1
+// (their edge lists are contiguous, so the staged batches of two centres'
+// ~90 edges each fill the lanes better than one centre's): the moments of
+// centre a into acc[a] (the workgroup's LDS: species tables already loaded
+// into tab, then `region`).
+template <class Cf, bool GATE, int NA, class T = typename Cf::T>
+__device__ __forceinline__ void forward_edges(const DModel<T>& m, const SpeciesTables<T>& tab, T* region, int i0,
+                                              int na, const int32_t* __restrict__ types,
+                                              const int* __restrict__ nbr_off, const Rec<T>* __restrict__ rec,
+                                              const T* __restrict__ h, T (&acc)[NA][Cf::NM]) {
+  constexpr int LL = Cf::LL, H = Cf::H, F = Cf::F, NM = Cf::NM;
   constexpr bool HALF = FwdStage<Cf>::HALF;
   constexpr int ROW = FwdStage<Cf>::ROW, HW = WAVE / 2;
-  const DPass<T>& p = GATE ? m.gp : m.op;
-  const int lo = p.node_lo, nc = p.node_cnt;  // the pass's node window
-  T* sR = region;                          // [F][ROW]  edge phase
-  T* sY = sR + F * ROW;                    // [H][ROW]
-  T* mom = region;                         // [KH]      after the edge phase
-  T* v = mom + KH;                         // [nc]      node n at v[n - lo]
-  T* a = v + pass_nodes(m);                // [nc]
-  T* scratch = a + pass_nodes(m);
+  T* sR = region;        // [F][ROW]
+  T* sY = sR + F * ROW;  // [H][ROW]
   const int lane = threadIdx.x;
-  const int ti = types[i];
-  const int e0 = nbr_off[i], e1 = nbr_off[i + 1];
-  T acc[NM];
+  int eb[NA + 1], ta[NA];
 #pragma unroll
-  for (int t = 0; t < NM; ++t) acc[t] = T(0);
+  for (int a = 0; a <= NA; ++a) eb[a] = nbr_off[i0 + min(a, na)];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) ta[a] = types[i0 + min(a, na - 1)];
+#pragma unroll
+  for (int a = 0; a < NA; ++a)
+#pragma unroll
+    for (int t = 0; t < NM; ++t) acc[a][t] = T(0);
+  const int e0 = eb[0], e1 = eb[NA];
   // the next batch's record loaded while this one is worked on
   Rec<T> qn = e0 + lane < e1 ? rec[e0 + lane] : Rec<T>{T(1), T(0), T(0), T(0)};
   for (int base = e0; base < e1; base += WAVE) {
     const int e = base + lane;
     const Rec<T> q = qn;
     if (e + WAVE < e1) qn = rec[e + WAVE];
+    // the centre of the lane's edge
+    int ti = ta[0];
+#pragma unroll
+    for (int a = 1; a < NA; ++a)
+      if (e >= eb[a]) ti = ta[a];
     if constexpr (HALF) {
       // wide shapes: lanes [0, 32) stage their edges at once, lanes [32, 64)
       // keep their weighted radial values and stage them (and their
@@ -328,7 +346,7 @@ __device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const Spec
         if (first) sh_stream_values<T, LL>(ux, uy, uz, [&](int l, int mm, T Y) { sY[hidx(l, mm) * ROW + col] = Y; });
       }
       __syncthreads();
-      stage_moments<Cf>(sR, sY, min(HW, e1 - base), acc);
+      stage_moments_split<Cf, NA>(sR, sY, base, min(HW, e1 - base), eb, acc);
       __syncthreads();
       const int cnt2 = min(HW, e1 - base - HW);
       if (cnt2 > 0) {
@@ -338,7 +356,7 @@ __device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const Spec
           sh_stream_values<T, LL>(ux, uy, uz, [&](int l, int mm, T Y) { sY[hidx(l, mm) * ROW + col] = Y; });
         }
         __syncthreads();
-        stage_moments<Cf>(sR, sY, cnt2, acc);
+        stage_moments_split<Cf, NA>(sR, sY, base + HW, cnt2, eb, acc);
         __syncthreads();
       }
     } else {
@@ -364,11 +382,29 @@ __device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const Spec
         for (int hh = 0; hh < H; ++hh) sY[hh * ROW + lane] = Y[hh];
       }
       __syncthreads();
-      stage_moments<Cf>(sR, sY, min(WAVE, e1 - base), acc);
+      stage_moments_split<Cf, NA>(sR, sY, base, min(WAVE, e1 - base), eb, acc);
       __syncthreads();
     }
   }
-  // (the edge loop ends with a barrier, so the staging region is free here)
+  // (the edge loop ends with a barrier, so the staging region is free after it)
+}
+
+// Graph phase of a forward pass for centre i of species ti from its moments
+// acc: rank contraction, product graph, then the gate signals h_i and node
+// values (gate pass) or the site energy and the moment adjoints abar_i
+// (outer pass).  The staging region is free on entry.
+template <class Cf, bool GATE, class T = typename Cf::T>
+__device__ __forceinline__ void forward_graph(const DModel<T>& m, T* region, int i, int ti, const T* acc,
+                                              T* __restrict__ h, T* __restrict__ vg, T* __restrict__ site,
+                                              T* __restrict__ abar) {
+  constexpr int KK = Cf::KK, F = Cf::F, KH = Cf::KH, NM = Cf::NM;
+  const DPass<T>& p = GATE ? m.gp : m.op;
+  const int lo = p.node_lo, nc = p.node_cnt;  // the pass's node window
+  T* mom = region;                            // [KH]
+  T* v = mom + KH;                            // [nc]      node n at v[n - lo]
+  T* a = v + pass_nodes(m);                   // [nc]
+  T* scratch = a + pass_nodes(m);
+  const int lane = threadIdx.x;
   for (int n = lane; n < nc; n += WAVE) v[n] = T(0);
 #pragma unroll
   for (int t = 0; t < NM; ++t) {
@@ -441,12 +477,35 @@ __device__ __forceinline__ void pass_forward_atom(const DModel<T>& m, const Spec
   wave_adjoint_to_basic<T, KK>(m, p, false, a, abar + (size_t)i * KH, lo, nc);
 }
 
-// Forward pass over atoms [0, natoms): one atom per workgroup (grid = natoms),
-// or (PERSIST) a persistent grid striding over the atoms with the species
-// tables loaded once per workgroup.  The two are separate instances so that
-// the one-atom form keeps its straight-line register allocation.
-template <class Cf, bool GATE, bool PERSIST, class T = typename Cf::T>
-__global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, int natoms, const int32_t* __restrict__ types,
+// NA consecutive centres from i0 (fewer past natoms): their edge phase
+// together, then each centre's graph phase in turn.
+template <class Cf, bool GATE, int NA, class T = typename Cf::T>
+__device__ __forceinline__ void pass_forward_group(const DModel<T>& m, const SpeciesTables<T>& tab, T* region, int i0,
+                                                   int natoms, const int32_t* __restrict__ types,
+                                                   const int* __restrict__ nbr_off, const Rec<T>* __restrict__ rec,
+                                                   T* __restrict__ h, T* __restrict__ vg, T* __restrict__ site,
+                                                   T* __restrict__ abar) {
+  const int na = min(NA, natoms - i0);
+  T acc[NA][Cf::NM];
+  forward_edges<Cf, GATE, NA>(m, tab, region, i0, na, types, nbr_off, rec, h, acc);
+#pragma unroll
+  for (int a = 0; a < NA; ++a) {
+    if (a >= na) break;
+    if (a > 0) __syncthreads();  // the previous centre's graph phase used the region
+    forward_graph<Cf, GATE>(m, region, i0 + a, types[i0 + a], acc[a], h, vg, site, abar);
+  }
+}
+
+// Forward pass over atoms [0, natoms) in groups of NA consecutive centres: one
+// group per workgroup (grid = natoms / NA), or (PERSIST) a persistent grid
+// striding over the groups with the species tables loaded once per
+// workgroup.  The two are separate instances so that the one-group form keeps
+// its straight-line register allocation.
+template <class Cf, int NA>
+constexpr int forward_waves() { return (sizeof(typename Cf::T) == 4 && NA > 1) ? 4 : 1; }
+
+template <class Cf, bool GATE, bool PERSIST, int NA = 1, class T = typename Cf::T>
+__global__ __launch_bounds__(WAVE, (forward_waves<Cf, NA>())) void k_pass_forward(DModel<T> m, int natoms, const int32_t* __restrict__ types,
                                                        const int* __restrict__ nbr_off,
                                                        const Rec<T>* __restrict__ rec, T* __restrict__ h,
                                                        T* __restrict__ vg, T* __restrict__ site,
@@ -458,11 +517,11 @@ __global__ __launch_bounds__(WAVE) void k_pass_forward(DModel<T> m, int natoms, 
   load_species_tables<Cf, FwdStage<Cf>::CL>(m, GATE, tabs_base, tab);
   __syncthreads();
   if constexpr (!PERSIST) {
-    pass_forward_atom<Cf, GATE>(m, tab, region, blockIdx.x, types, nbr_off, rec, h, vg, site, abar);
+    pass_forward_group<Cf, GATE, NA>(m, tab, region, blockIdx.x * NA, natoms, types, nbr_off, rec, h, vg, site, abar);
   } else {
-    for (int i = blockIdx.x; i < natoms; i += gridDim.x) {
-      pass_forward_atom<Cf, GATE>(m, tab, region, i, types, nbr_off, rec, h, vg, site, abar);
-      __syncthreads();  // the next atom overwrites the region
+    for (int i0 = blockIdx.x * NA; i0 < natoms; i0 += gridDim.x * NA) {
+      pass_forward_group<Cf, GATE, NA>(m, tab, region, i0, natoms, types, nbr_off, rec, h, vg, site, abar);
+      __syncthreads();  // the next group overwrites the region
     }
   }
 }
@@ -1013,24 +1072,41 @@ unsigned tuned_grid(const EfsTune& t, K kern, size_t lds, long units) {
   return (unsigned)std::min<long>(units, (long)t.grid_mult * resident_groups(kern, lds, false));
 }
 
+// Batches of at least this many atoms run the forward passes with two centres
+// per workgroup (two centres' ~90 edges each fill the lanes of the 64-edge
+// batches better than one centre's); smaller ones keep one centre per
+// workgroup, so that every SIMD still has several workgroups in flight.
+constexpr int kFwdPairAtoms = 32768;
+
+template <class Cf, bool GATE, int NA>
+void launch_pass_forward_na(const DevModelSet& dm, Workspace& ws, int n, const int32_t* types, const int* off,
+                            const Rec<typename Cf::T>* rec, typename Cf::T* h, typename Cf::T* vg,
+                            typename Cf::T* site, typename Cf::T* abar, hipStream_t st) {
+  using T = typename Cf::T;
+  const DModel<T>& m = dm.get<T>();
+  const size_t lf = forward_lds<Cf>(m);
+  const char* name = GATE ? "pass_forward_gate" : "pass_forward_outer";
+  const long groups = (n + NA - 1) / NA;
+  if (dm.tune_of<T>().grid_mult <= 0) {
+    allow_lds(k_pass_forward<Cf, GATE, false, NA>, lf);
+    SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_pass_forward<Cf, GATE, false, NA>), dim3((unsigned)groups), dim3(WAVE),
+                lf, m, n, types, off, rec, h, vg, site, abar);
+    return;
+  }
+  const unsigned g = tuned_grid(dm.tune_of<T>(), k_pass_forward<Cf, GATE, true, NA>, lf, groups);
+  SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_pass_forward<Cf, GATE, true, NA>), dim3(g), dim3(WAVE), lf, m, n, types,
+              off, rec, h, vg, site, abar);
+}
+
 template <class Cf, bool GATE>
 void launch_pass_forward(const DevModelSet& dm, Workspace& ws, int n, const int32_t* types, const int* off,
                          const Rec<typename Cf::T>* rec, typename Cf::T* h, typename Cf::T* vg, typename Cf::T* site,
                          typename Cf::T* abar, hipStream_t st) {
-  using T = typename Cf::T;
-  const DModel<T>& m = dm.get<T>();
-  const size_t lf = forward_lds<Cf>(m);
   if (n <= 0) return;
-  const char* name = GATE ? "pass_forward_gate" : "pass_forward_outer";
-  if (dm.tune_of<T>().grid_mult <= 0) {
-    allow_lds(k_pass_forward<Cf, GATE, false>, lf);
-    SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_pass_forward<Cf, GATE, false>), dim3(n), dim3(WAVE), lf, m, n, types, off,
-                rec, h, vg, site, abar);
-    return;
-  }
-  const unsigned g = tuned_grid(dm.tune_of<T>(), k_pass_forward<Cf, GATE, true>, lf, n);
-  SUS2_LAUNCH(ws, st, kname<Cf>(name), (k_pass_forward<Cf, GATE, true>), dim3(g), dim3(WAVE), lf, m, n, types, off,
-              rec, h, vg, site, abar);
+  if (n >= kFwdPairAtoms)
+    launch_pass_forward_na<Cf, GATE, 2>(dm, ws, n, types, off, rec, h, vg, site, abar, st);
+  else
+    launch_pass_forward_na<Cf, GATE, 1>(dm, ws, n, types, off, rec, h, vg, site, abar, st);
 }
 
 template <class Cf, bool GATE, int NA, bool MSG>
