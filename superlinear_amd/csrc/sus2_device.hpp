@@ -12,15 +12,25 @@ constexpr int WAVE = 64;
 constexpr int BMAX = 16;  // largest radial basis size the kernels are built for
 
 // ------------------------------------------------------------------ model view
+// One product-graph row as a 16-byte record, so that a row is one vector
+// load: forward (x, y) = (a, b), the row adds c v[a] v[b] to its output;
+// reverse (x, y) = (out, partner), the row adds c a[out] v[partner] to its
+// node.
+template <class T>
+struct __align__(16) GRow {
+  int x, y;
+  T c;
+};
+
 template <class T>
 struct DPass {  // layered product-graph schedule of one pass (see PassPlan)
   int nfl = 0, nrl = 0, rank = 0, begin = 0, node_lo = 0, node_cnt = 0;
   int nfout = 0, nrnode = 0;  // total forward outs / reverse nodes
   int max_fw_rows = 0, max_rv_rows = 0;
-  const int *fl_off = nullptr, *f_out = nullptr, *f_roff = nullptr, *f_a = nullptr, *f_b = nullptr;
-  const T* f_c = nullptr;
-  const int *rl_off = nullptr, *r_node = nullptr, *r_roff = nullptr, *r_out = nullptr, *r_partner = nullptr;
-  const T* r_c = nullptr;
+  const int *fl_off = nullptr, *f_out = nullptr, *f_roff = nullptr;
+  const GRow<T>* f_row = nullptr;
+  const int *rl_off = nullptr, *r_node = nullptr, *r_roff = nullptr;
+  const GRow<T>* r_row = nullptr;
   // per layer {first out / node, end, first row, end row}: one 16-byte load
   const int4 *fl_bnd = nullptr, *rl_bnd = nullptr;
 };

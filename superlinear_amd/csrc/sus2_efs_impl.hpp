@@ -129,7 +129,10 @@ __device__ void wave_graph_forward(const DPass<T>& p, T* v, T* scratch, int lo =
       oe = p.f_roff[oa + 1];
       on = p.f_out[oa] - lo;
     }
-    wave_rows(r0, r1, [&](int r) { scratch[r - r0] = p.f_c[r] * v[p.f_a[r] - lo] * v[p.f_b[r] - lo]; });
+    wave_rows(r0, r1, [&](int r) {
+      const GRow<T> g = p.f_row[r];
+      scratch[r - r0] = g.c * v[g.x - lo] * v[g.y - lo];
+    });
     __syncthreads();
     if (oa < o1) {
       T s = T(0);
@@ -162,7 +165,10 @@ __device__ void wave_graph_reverse(const DPass<T>& p, const T* v, T* a, T* scrat
       ke = p.r_roff[ka + 1];
       kn = p.r_node[ka] - lo;
     }
-    wave_rows(q0, q1, [&](int q) { scratch[q - q0] = p.r_c[q] * a[p.r_out[q] - lo] * v[p.r_partner[q] - lo]; });
+    wave_rows(q0, q1, [&](int q) {
+      const GRow<T> g = p.r_row[q];
+      scratch[q - q0] = g.c * a[g.x - lo] * v[g.y - lo];
+    });
     __syncthreads();
     if (ka < n1) {
       T s = a[kn];

@@ -181,8 +181,9 @@ __device__ void wave_graph_forward_tan(const DPass<T>& p, T* v, T* vd, T* s1, T*
       on = p.f_out[oa];
     }
     wave_rows(r0, r1, [&](int r) {
-      const T c = p.f_c[r];
-      const int a = p.f_a[r], b = p.f_b[r];
+      const GRow<T> g = p.f_row[r];
+      const T c = g.c;
+      const int a = g.x, b = g.y;
       s1[r - r0] = c * v[a] * v[b];
       s2[r - r0] = c * (vd[a] * v[b] + v[a] * vd[b]);
     });
@@ -222,8 +223,9 @@ __device__ void wave_graph_reverse_tan(const DPass<T>& p, const T* v, const T* v
       kn = p.r_node[ka];
     }
     wave_rows(q0, q1, [&](int q) {
-      const T c = p.r_c[q];
-      const int o = p.r_out[q], pt = p.r_partner[q];
+      const GRow<T> g = p.r_row[q];
+      const T c = g.c;
+      const int o = g.x, pt = g.y;
       s1[q - q0] = c * (P[o] * v[pt] + Q[o] * vd[pt]);
       s2[q - q0] = c * Q[o] * v[pt];
     });

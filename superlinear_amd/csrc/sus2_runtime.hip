@@ -320,21 +320,22 @@ void fill_view(DevModelT<T>& dv, const Model& m, const int* ibase, const std::ve
   std::vector<T> th(m.theta.begin(), m.theta.end());
   dv.theta.ensure(sizeof(T) * th.size());
   RT_CHECK(hipMemcpy(dv.theta.p, th.data(), sizeof(T) * th.size(), hipMemcpyHostToDevice));
-  std::vector<T> fc, rc;
-  const size_t gfc = 0;
-  fc.insert(fc.end(), m.gate_plan.fw_c.begin(), m.gate_plan.fw_c.end());
-  const size_t ofc = fc.size();
-  fc.insert(fc.end(), m.outer_plan.fw_c.begin(), m.outer_plan.fw_c.end());
-  const size_t grc = 0;
-  rc.insert(rc.end(), m.gate_plan.rv_c.begin(), m.gate_plan.rv_c.end());
-  const size_t orc = rc.size();
-  rc.insert(rc.end(), m.outer_plan.rv_c.begin(), m.outer_plan.rv_c.end());
-  fc.push_back(T(0));
-  rc.push_back(T(0));
-  dv.fc.ensure(sizeof(T) * fc.size());
-  dv.rc.ensure(sizeof(T) * rc.size());
-  RT_CHECK(hipMemcpy(dv.fc.p, fc.data(), sizeof(T) * fc.size(), hipMemcpyHostToDevice));
-  RT_CHECK(hipMemcpy(dv.rc.p, rc.data(), sizeof(T) * rc.size(), hipMemcpyHostToDevice));
+  // product-graph rows as 16-byte records: gate forward, outer forward, gate
+  // reverse, outer reverse
+  std::vector<GRow<T>> rows;
+  size_t roff[4];
+  for (int part = 0; part < 4; ++part) {
+    const PassPlan& pp = part % 2 == 0 ? m.gate_plan : m.outer_plan;
+    roff[part] = rows.size();
+    if (part < 2)
+      for (size_t r = 0; r < pp.fw_a.size(); ++r) rows.push_back(GRow<T>{pp.fw_a[r], pp.fw_b[r], T(pp.fw_c[r])});
+    else
+      for (size_t r = 0; r < pp.rv_out.size(); ++r)
+        rows.push_back(GRow<T>{pp.rv_out[r], pp.rv_partner[r], T(pp.rv_c[r])});
+  }
+  rows.push_back(GRow<T>{0, 0, T(0)});
+  dv.rows.ensure(sizeof(GRow<T>) * rows.size());
+  RT_CHECK(hipMemcpy(dv.rows.p, rows.data(), sizeof(GRow<T>) * rows.size(), hipMemcpyHostToDevice));
   DModel<T>& v = dv.view;
   v.C = m.C; v.L = m.L; v.H = m.H; v.K = m.K; v.F = m.F; v.B = m.B; v.M = m.M; v.Q = m.Q; v.Kw = m.Kw;
   v.smode = m.smode; v.R = m.R; v.O = m.O; v.Qg = m.Qg;
@@ -367,13 +368,11 @@ void fill_view(DevModelT<T>& dv, const Model& m, const int* ibase, const std::ve
     dp.nfout = pp.fw_out.empty() ? 0 : (int)pp.fw_out.size();
     dp.nrnode = pp.rv_node.empty() ? 0 : (int)pp.rv_node.size();
     dp.fl_off = ibase + io[k++]; dp.f_out = ibase + io[k++]; dp.f_roff = ibase + io[k++];
-    dp.f_a = ibase + io[k++]; dp.f_b = ibase + io[k++];
     dp.rl_off = ibase + io[k++]; dp.r_node = ibase + io[k++]; dp.r_roff = ibase + io[k++];
-    dp.r_out = ibase + io[k++]; dp.r_partner = ibase + io[k++];
     dp.fl_bnd = reinterpret_cast<const int4*>(ibase + io[k++]);
     dp.rl_bnd = reinterpret_cast<const int4*>(ibase + io[k++]);
-    dp.f_c = dv.fc.template get<T>() + (pass == 0 ? gfc : ofc);
-    dp.r_c = dv.rc.template get<T>() + (pass == 0 ? grc : orc);
+    dp.f_row = dv.rows.template get<GRow<T>>() + roff[pass];
+    dp.r_row = dv.rows.template get<GRow<T>>() + roff[2 + pass];
   }
 }
 }  // namespace
@@ -390,8 +389,8 @@ void DevModelSet::upload(const Model& m) {
   io.push_back(pk.add(m.gate_nodes));
   for (int pass = 0; pass < 2; ++pass) {
     const PassPlan& pp = pass == 0 ? m.gate_plan : m.outer_plan;
-    for (const auto* vec : {&pp.fw_layer_off, &pp.fw_out, &pp.fw_row_off, &pp.fw_a, &pp.fw_b, &pp.rv_layer_off,
-                            &pp.rv_node, &pp.rv_row_off, &pp.rv_out, &pp.rv_partner})
+    for (const auto* vec : {&pp.fw_layer_off, &pp.fw_out, &pp.fw_row_off, &pp.rv_layer_off, &pp.rv_node,
+                            &pp.rv_row_off})
       io.push_back(pk.add(*vec));
     // layer bounds, 16-byte aligned in the packed array
     auto bounds = [](const std::vector<int>& lay, const std::vector<int>& roff) {

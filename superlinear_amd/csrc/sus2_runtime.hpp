@@ -176,7 +176,7 @@ void poll_report(Workspace& ws, int k, int n, int64_t seq, hipStream_t st, const
 // One precision's device view of a model.
 template <class T>
 struct DevModelT {
-  DevBuf theta, fc, rc;  // coefficient vector, forward/reverse product coefficients (gate then outer)
+  DevBuf theta, rows;  // coefficient vector; product-graph rows (gate forward, outer forward, gate reverse, outer reverse)
   DModel<T> view{};
 };
 
