@@ -118,11 +118,14 @@ struct Cfg {
 
 // ------------------------------------------------------------------ harmonics
 // One step of the normalised Legendre recurrence, P_l from P_{l-1}, P_{l-2}
-// (rho = 1): both harmonic streams use it, so their values agree bit for bit.
+// (rho = 1).  Every harmonic stream uses it, and the steps shared by the
+// streams (this one and the azimuthal recurrence) are written as explicit
+// fused multiply-adds, so the streams' values agree bit for bit whatever
+// contractions the compiler would choose in each kernel.
 template <class T>
 __device__ __forceinline__ T legendre_step(int l, int m, T uz, T p1, T p2) {
   const T a = T(kYR.a[hidx(l, m)]), b = T(kYR.b[hidx(l, m)]);
-  return l == m + 1 ? a * (uz * p1) : a * (uz * p1) - b * p2;
+  return l == m + 1 ? a * (uz * p1) : fma(a, uz * p1, -(b * p2));
 }
 
 // Real Y_lm(u) for a unit vector u, l <= LL, flat index l^2+l+m; Condon-
@@ -136,8 +139,8 @@ __device__ __forceinline__ void sh_stream_values(T ux, T uy, T uz, Fn&& fn) {
 #pragma unroll
   for (int m = 0; m <= LL; ++m) {
     if (m > 0) {
-      const T c2 = ux * cm - uy * sm;
-      sm = ux * sm + uy * cm;
+      const T c2 = fma(ux, cm, -(uy * sm));
+      sm = fma(ux, sm, uy * cm);
       cm = c2;
     }
     T p2 = T(0), p1 = T(0);
@@ -153,6 +156,40 @@ __device__ __forceinline__ void sh_stream_values(T ux, T uy, T uz, Fn&& fn) {
       p2 = p1;
       p1 = p;
     }
+  }
+}
+
+// The same Y_lm (the same operations, so bit for bit) in (l, m) order: fn(l, Yl)
+// once per degree l with Yl[l + m] = Y_lm, m = -l..l.  The recurrence state
+// of every m (2 (LL + 1) values) and the azimuthal factors are carried
+// instead of one m's, so a caller can finish everything of degree l before
+// the next degree is formed.
+template <class T, int LL, class Fn>
+__device__ __forceinline__ void sh_values_by_l(T ux, T uy, T uz, Fn&& fn) {
+  T cm[LL + 1], sm[LL + 1], p1[LL + 1], p2[LL + 1];
+  cm[0] = T(1);
+  sm[0] = T(0);
+#pragma unroll
+  for (int m = 1; m <= LL; ++m) {
+    cm[m] = fma(ux, cm[m - 1], -(uy * sm[m - 1]));
+    sm[m] = fma(ux, sm[m - 1], uy * cm[m - 1]);
+  }
+#pragma unroll
+  for (int l = 0; l <= LL; ++l) {
+    T Yl[2 * LL + 1];
+#pragma unroll
+    for (int m = 0; m <= l; ++m) {
+      const T p = l == m ? T(kYR.p0[m]) : legendre_step(l, m, uz, p1[m], l == m + 1 ? T(0) : p2[m]);
+      p2[m] = l == m ? T(0) : p1[m];
+      p1[m] = p;
+      if (m == 0) {
+        Yl[l] = p;
+      } else {
+        Yl[l + m] = p * cm[m];
+        Yl[l - m] = p * sm[m];
+      }
+    }
+    fn(l, (const T*)Yl);
   }
 }
 
@@ -179,8 +216,8 @@ __device__ __forceinline__ void sh_stream_solid(T ux, T uy, T uz, Fn&& fn) {
     if (m > 0) {
       cx = T(m) * cm; cy = -T(m) * sm;
       sx = T(m) * sm; sy = T(m) * cm;
-      const T c2 = ux * cm - uy * sm;
-      sm = ux * sm + uy * cm;
+      const T c2 = fma(ux, cm, -(uy * sm));
+      sm = fma(ux, sm, uy * cm);
       cm = c2;
     }
     // P = n p, Z = n dp/du_z, Q = n dp/drho at rho = 1

@@ -633,18 +633,21 @@ __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const Sp
     T sY = T(0), gx = T(0), gy = T(0), gz = T(0), lsum = T(0);
     sh_stream_solid<T, LL>(ux, uy, uz, [&](int l, int mm, T Y, T Gx, T Gy, T Gz) {
       const int hh = hidx(l, mm);
-      // gate pass: keep the moment-adjoint LDS reads of this (l, m) here;
-      // hoisted over the whole unrolled stream they would hold K H registers
-      if constexpr (GATE) asm volatile("" ::: "memory");
+      // the moment adjoints of this (l, m) are read here: their address takes
+      // a zero from an empty asm of Y (hoisted over the whole unrolled stream
+      // they would hold K H registers; the asm touches no memory)
+      int z;
+      asm("" : "=v"(z) : "0"(0), "v"(Y));
+      const T* sAh = sAi + z;
       // channel sums first (sum_k abar R', sum_k abar R), then Y and grad Y once
       T adR = T(0), aR = T(0);
 #pragma unroll
       for (int k = 0; k < KK; ++k) {
         const int mu = k * L1 + l;
-        const T av = sAi[hh * KK + k];
+        const T av = sAh[hh * KK + k];
         adR += av * dR[mu];
         aR += av * R[mu];
-        if (hmsg) us[mu] += av * Y;
+        if (hmsg) us[mu] = fma(av, Y, us[mu]);
       }
       sY += adR * Y;
       gx += aR * Gx; gy += aR * Gy; gz += aR * Gz;
@@ -753,16 +756,33 @@ __device__ __forceinline__ void edge_messages_group(const DModel<T>& m, const Sp
     const T dx = q.x, dy = q.y, dz = q.z;
     T r, inv;
     edge_length(dx * dx + dy * dy + dz * dz, r, inv);
-    T us[F];
-#pragma unroll
-    for (int mu = 0; mu < F; ++mu) us[mu] = T(0);
-    sh_stream_values<T, LL>(dx * inv, dy * inv, dz * inv, [&](int l, int mm, T Y) {
-      const int hh = hidx(l, mm);
-#pragma unroll
-      for (int k = 0; k < KK; ++k) us[k * L1 + l] += sAi[hh * KK + k] * Y;
-    });
+    // x0 R per channel first, then the harmonics degree by degree: the K
+    // messages of degree l are finished and stored before degree l + 1 is
+    // formed (the m terms summed in the order of the combined pass, so the
+    // values are the same bit for bit)
+    T w[F];
+    radial_each<Cf, false>(m, false, tab, ti * m.C + tj, r, [&](int mu, T R, T) { w[mu] = x0[mu] * R; });
     T* ue = uedge + (size_t)in_pos[e] * F;
-    radial_each<Cf, false>(m, false, tab, ti * m.C + tj, r, [&](int mu, T R, T) { ue[mu] = (x0[mu] * R) * us[mu]; });
+    T dep = r;
+    sh_values_by_l<T, LL>(dx * inv, dy * inv, dz * inv, [&](int l, const T* Yl) {
+      // the adjoint reads of degree l wait for the previous degree's sums (an
+      // empty asm's zero): read all at once they would hold K H registers
+      int z;
+      asm("" : "=v"(z) : "0"(0), "v"(dep), "v"(Yl[l]));
+      const T* sAh = sAi + z;
+#pragma unroll
+      for (int k = 0; k < KK; ++k) {
+        T u = T(0);
+        u = fma(sAh[hidx(l, 0) * KK + k], Yl[l], u);
+#pragma unroll
+        for (int mm = 1; mm <= l; ++mm) {
+          u = fma(sAh[hidx(l, mm) * KK + k], Yl[l + mm], u);
+          u = fma(sAh[hidx(l, -mm) * KK + k], Yl[l - mm], u);
+        }
+        ue[k * L1 + l] = w[k * L1 + l] * u;
+        dep = u;
+      }
+    });
   }
 }
 
@@ -871,8 +891,11 @@ __device__ __forceinline__ void gather_atom(const DModel<T>& m, int i, const int
           for (int mu = 0; mu < F; ++mu) S[mu] = T(0);
           sh_stream_values<T, LL>(dx * inv, dy * inv, dz * inv, [&](int l, int mm, T Y) {
             const int hh = hidx(l, mm);
+            int z;  // (the neighbour's adjoints of this (l, m) are read here)
+            asm("" : "=v"(z) : "0"(0), "v"(Y));
+            const T* ajh = aj + z;
 #pragma unroll
-            for (int c = 0; c < KK; ++c) S[c * L1 + l] += aj[hh * KK + c] * Y;
+            for (int c = 0; c < KK; ++c) S[c * L1 + l] += ajh[hh * KK + c] * Y;
           });
           const T* x0 = tab->x0 + tj * F;
           radial_each<Cf, false>(m, false, *tab, tj * m.C + ti, r,
