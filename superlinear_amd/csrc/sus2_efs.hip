@@ -591,7 +591,16 @@ __global__ void k_zero_int(int n, int* __restrict__ a) {
 // atom's fixed-capacity slots out[i*cap + k] for k < cap and the count to
 // cnt[i].  Fill mode (cap == 0): hits go to out[off[i] + k]; only atoms whose
 // count exceeds `over` run (the ones whose hits did not fit their slots).
-__device__ __forceinline__ int floor_div(int a, int n) { return a >= 0 ? a / n : -((-a + n - 1) / n); }
+// floor(a / n) for |a| < 2^23 and 1 <= n <= 2^16, from the float reciprocal
+// rn = 1 / n: the float estimate is within one of the quotient, and one step
+// each way makes it exact -- a few instructions instead of an integer
+// division by a value known only at run time.
+__device__ __forceinline__ int floor_div_r(int a, int n, float rn) {
+  int q = (int)floorf(((float)a + 0.5f) * rn);
+  q -= q * n > a;
+  q += (q + 1) * n <= a;
+  return q;
+}
 
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __restrict__ atom_frame,
@@ -634,6 +643,11 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
   int count = 0;
   const long w0 = cap ? (long)i * cap : (long)off[i];
   const int n0 = 2 * g.S[0] + 1, n1 = 2 * g.S[1] + 1, n2 = 2 * g.S[2] + 1, nof = n0 * n1 * n2;
+  // reciprocals for the offset decode and the bin centres (the bin test has
+  // 1e-6 A of slack, far above their rounding)
+  const float rq0 = 1.0f / n0, rq1 = 1.0f / n1;
+  const float rb0 = 1.0f / g.n[0], rb1 = 1.0f / g.n[1], rb2 = 1.0f / g.n[2];
+  const double rd0 = 1.0 / g.n[0], rd1 = 1.0 / g.n[1], rd2 = 1.0 / g.n[2];
   // offsets OPL at a time, two adjacent ones per lane (o = ob + 2 lane, +1),
   // so that a 5 x 5 x 5 search resolves all its bins in one chunk
   constexpr int OPL = 2 * WAVE;
@@ -644,16 +658,17 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
       const int o = ob + 2 * lane + u;
       int nb = 0, bs = 0, s0 = 0, s1 = 0, s2 = 0;
       if (o < nof) {
-        int x = c0 + o % n0 - g.S[0], y = c1 + (o / n0) % n1 - g.S[1], z = c2 + o / (n0 * n1) - g.S[2];
+        const int oq = floor_div_r(o, n0, rq0), oz = floor_div_r(oq, n1, rq1);
+        int x = c0 + (o - oq * n0) - g.S[0], y = c1 + (oq - oz * n1) - g.S[1], z = c2 + oz - g.S[2];
         bool ok = true;
-        if (g.per & 1) { s0 = floor_div(x, g.n[0]); x -= s0 * g.n[0]; }
+        if (g.per & 1) { s0 = floor_div_r(x, g.n[0], rb0); x -= s0 * g.n[0]; }
         else ok = ok && x >= 0 && x < g.n[0];
-        if (g.per & 2) { s1 = floor_div(y, g.n[1]); y -= s1 * g.n[1]; }
+        if (g.per & 2) { s1 = floor_div_r(y, g.n[1], rb1); y -= s1 * g.n[1]; }
         else ok = ok && y >= 0 && y < g.n[1];
-        if (g.per & 4) { s2 = floor_div(z, g.n[2]); z -= s2 * g.n[2]; }
+        if (g.per & 4) { s2 = floor_div_r(z, g.n[2], rb2); z -= s2 * g.n[2]; }
         else ok = ok && z >= 0 && z < g.n[2];
         if (ok) {
-          const double u0 = (x + 0.5) / g.n[0] + s0, u1 = (y + 0.5) / g.n[1] + s1, u2 = (z + 0.5) / g.n[2] + s2;
+          const double u0 = (x + 0.5) * rd0 + s0, u1 = (y + 0.5) * rd1 + s1, u2 = (z + 0.5) * rd2 + s2;
           const double ex = g.org[0] + u0 * g.A[0] + u1 * g.A[3] + u2 * g.A[6] - gx;
           const double ey = g.org[1] + u0 * g.A[1] + u1 * g.A[4] + u2 * g.A[7] - gy;
           const double ez = g.org[2] + u0 * g.A[2] + u1 * g.A[5] + u2 * g.A[8] - gz;
@@ -800,6 +815,8 @@ __global__ __launch_bounds__(WAVE) void k_compact_rev(int natoms, int cap, const
   const double* a = cell + 9 * f;
   const int cbase = (int)frame_off[f];
   const int n0 = 2 * g.S[0] + 1, n1 = 2 * g.S[1] + 1, n2 = 2 * g.S[2] + 1, nof = n0 * n1 * n2;
+  const float rq0 = 1.0f / n0, rq1 = 1.0f / n1;
+  const float rb0 = 1.0f / g.n[0], rb1 = 1.0f / g.n[1], rb2 = 1.0f / g.n[2];
   const bool tabled = nof <= NOF_TAB;
   const int bin_i = atom_home[i].j;
   const int bin_start = cell_start[bin_i], bin_end = cell_start[bin_i + 1];
@@ -837,11 +854,12 @@ __global__ __launch_bounds__(WAVE) void k_compact_rev(int natoms, int cap, const
     const int c0 = ct % g.n[0];
     ct /= g.n[0];
     const int c1 = ct % g.n[1], c2 = ct / g.n[1];
-    int x = c0 + os % n0 - g.S[0], y = c1 + (os / n0) % n1 - g.S[1], z = c2 + os / (n0 * n1) - g.S[2];
+    const int oq = floor_div_r(os, n0, rq0), oz = floor_div_r(oq, n1, rq1);
+    const int x = c0 + (os - oq * n0) - g.S[0], y = c1 + (oq - oz * n1) - g.S[1], z = c2 + oz - g.S[2];
     int sv[3] = {0, 0, 0};
-    if (g.per & 1) sv[0] = floor_div(x, g.n[0]);
-    if (g.per & 2) sv[1] = floor_div(y, g.n[1]);
-    if (g.per & 4) sv[2] = floor_div(z, g.n[2]);
+    if (g.per & 1) sv[0] = floor_div_r(x, g.n[0], rb0);
+    if (g.per & 2) sv[1] = floor_div_r(y, g.n[1], rb1);
+    if (g.per & 4) sv[2] = floor_div_r(z, g.n[2], rb2);
     const double sx = lattice_comb(sv, a, 0), sy = lattice_comb(sv, a, 1), sz = lattice_comb(sv, a, 2);
     const bool zero = (sv[0] | sv[1] | sv[2]) == 0;
     int before = 0;
