@@ -291,6 +291,14 @@ __device__ __forceinline__ double pair_dist2(double d0, double d1, double d2) {
   return __dadd_rn(__dadd_rn(__dmul_rn(d0, d0), __dmul_rn(d1, d1)), __dmul_rn(d2, d2));
 }
 
+// Workgroup barrier over LDS only: global stores still in flight are not
+// waited for (k_frame_cells never reads back what it writes to global memory).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
     int nframes, const int64_t* __restrict__ frame_off, const double* __restrict__ pos,
     const int32_t* __restrict__ types, const double* __restrict__ cell, const int32_t* __restrict__ periodic,
@@ -305,29 +313,31 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
   const int f = blockIdx.x, t = threadIdx.x, nt = blockDim.x;
   const long b0 = frame_off[f];
   const int na = (int)(frame_off[f + 1] - b0);
-  if (t == 0) {
-    Grid g;
-    g.per = grid_mask(cell + 9 * f, periodic[f]);
-    g_sh = g;
-  }
-  // the thread's first FC_PER positions, loaded while the grid is formed
+  // one pass of FC_PER atoms per thread covers the frame: the positions and
+  // species loaded here serve the binning and the packing
+  const bool one = na <= FC_PER * nt;
+  const int per = grid_mask(cell + 9 * f, periodic[f]);  // the same in every thread
   double xs[FC_PER][3];
+  int ts[FC_PER], bws[FC_PER];
 #pragma unroll
   for (int r = 0; r < FC_PER; ++r) {
     const int k = t + r * nt;
-    if (k < na)
+    if (k < na) {
 #pragma unroll
       for (int q = 0; q < 3; ++q) xs[r][q] = pos[3 * (b0 + k) + q];
+      ts[r] = types[b0 + k];
+    }
   }
-  __syncthreads();
+  // bin counts start at zero (a frame has at most max(na, 1) bins)
+  for (int k = t; k <= max(na, 1); k += nt) cnt[k] = 0;
   // basis rows: lattice rows for periodic directions, unit normals for open ones
   double a[9];
-  grid_basis(cell + 9 * f, g_sh.per, a);
+  grid_basis(cell + 9 * f, per, a);
   // open directions: extent of the atoms along each normal (block min / max)
   double org[3] = {0, 0, 0};
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
-    if (g_sh.per & (1 << q)) continue;
+    if (per & (1 << q)) continue;
     double lo = 1e300, hi = -1e300;
     for (int k = t; k < na; k += nt) {
       const double* x = pos + 3 * (b0 + k);
@@ -337,17 +347,17 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
     }
     red_lo[t] = lo;
     red_hi[t] = hi;
-    __syncthreads();
+    lds_barrier();
     for (int w = nt / 2; w > 0; w >>= 1) {
       if (t < w) {
         red_lo[t] = fmin(red_lo[t], red_lo[t + w]);
         red_hi[t] = fmax(red_hi[t], red_hi[t + w]);
       }
-      __syncthreads();
+      lds_barrier();
     }
     lo = na > 0 ? red_lo[0] : 0.0;
     hi = na > 0 ? red_hi[0] : 0.0;
-    __syncthreads();
+    lds_barrier();
     const double ext = fmax(hi - lo, 1e-6) * (1 + 1e-9) + 1e-9;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -356,20 +366,19 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
     }
   }
   if (t == 0) {
-    Grid g = g_sh;
+    Grid g;
+    g.per = per;
 #pragma unroll
     for (int q = 0; q < 3; ++q) g.org[q] = org[q];
     grid_finish(g, a, rc, na);
     g_sh = g;
     grid[f] = g;
   }
-  __syncthreads();
+  lds_barrier();
   const Grid g = g_sh;
   const int nbins = g.n[0] * g.n[1] * g.n[2];
   int* cur = cnt + FRAME_CELLS_MAX + 1;  // [nbins] placement cursors
   int* tmp = cur + FRAME_CELLS_MAX;      // [na] atoms placed per bin (unordered)
-  for (int k = t; k <= nbins; k += nt) cnt[k] = 0;
-  __syncthreads();
   // the thread's atoms FC_PER at a time: every position loaded before any is
   // binned (the first FC_PER are already in registers)
   for (int k0 = t; k0 < na; k0 += FC_PER * nt) {
@@ -395,14 +404,14 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
         c[q] = min(g.n[q] - 1, max(0, (int)((fr[q] - w) * g.n[q])));
       }
       const int id = c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
-      bin[k] = bin_word(id, wi);
+      bws[r] = bin[k] = bin_word(id, wi);
       cell_of[b0 + k] = (int)b0 + id;
       in_cur[b0 + k] = 0;
       atom_frame[b0 + k] = f;
       atomicAdd(cnt + id, 1);
     }
   }
-  __syncthreads();
+  lds_barrier();
   {  // exclusive scan of the bin counts: chunk sums per thread, wave prefix
      // sums, then a scan of the wave totals
     __shared__ int wtot[FRAME_CELLS_THREADS / WAVE];
@@ -417,7 +426,7 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
       if (lane >= d) incl += y;
     }
     if (lane == WAVE - 1) wtot[wv] = incl;
-    __syncthreads();
+    lds_barrier();
     if (wv == 0) {
       constexpr int NW = FRAME_CELLS_THREADS / WAVE;
       int x = lane < NW ? wtot[lane] : 0;
@@ -428,7 +437,7 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
       }
       if (lane < NW) wtot[lane] = x;
     }
-    __syncthreads();
+    lds_barrier();
     int run = (wv ? wtot[wv - 1] : 0) + incl - s;
     for (int k = c0; k < c1; ++k) {
       const int v = cnt[k];
@@ -438,30 +447,29 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
     }
     if (t == nt - 1) cnt[nbins] = wtot[FRAME_CELLS_THREADS / WAVE - 1];
   }
-  __syncthreads();
+  lds_barrier();
   for (int k = t; k <= na; k += nt) cell_start[b0 + k] = (int)b0 + (k < nbins ? cnt[k] : na);
   for (int k = t; k < na; k += nt) tmp[atomicAdd(cur + (bin[k] & BIN_ID_MASK), 1)] = k;
-  __syncthreads();
-  // ascending atom order inside each bin: rank among the bin's members
+  lds_barrier();
+  // ascending atom order inside each bin: each atom's rank among its bin's
+  // members places it, in the binning pass's atom order
   const double* a_rows = cell + 9 * f;
-  for (int p0 = t; p0 < na; p0 += FC_PER * nt) {
-    double xs[FC_PER][3];
-    int ks[FC_PER], ts[FC_PER];
+  for (int k0 = t; k0 < na; k0 += FC_PER * nt) {
+    if (!one)
 #pragma unroll
-    for (int r = 0; r < FC_PER; ++r) {
-      const int p = p0 + r * nt;
-      if (p < na) {
-        const int k = tmp[p];
-        ks[r] = k;
+      for (int r = 0; r < FC_PER; ++r) {
+        const int k = k0 + r * nt;
+        if (k < na) {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) xs[r][q] = pos[3 * (b0 + k) + q];
-        ts[r] = types[b0 + k];
+          for (int q = 0; q < 3; ++q) xs[r][q] = pos[3 * (b0 + k) + q];
+          ts[r] = types[b0 + k];
+        }
       }
-    }
 #pragma unroll
     for (int r = 0; r < FC_PER; ++r) {
-      if (p0 + r * nt >= na) break;
-      const int k = ks[r], bw = bin[k], id = bw & BIN_ID_MASK;
+      const int k = k0 + r * nt;
+      if (k >= na) break;
+      const int bw = one ? bws[r] : bin[k], id = bw & BIN_ID_MASK;
       const int s0 = cnt[id], s1 = cnt[id + 1];
       int r0 = 0, r1 = 0, r2 = 0, r3 = 0, q = s0;
       for (; q + 3 < s1; q += 4) {  // four independent LDS reads in flight
