@@ -80,12 +80,17 @@ __device__ inline void grid_box(Grid& g, const double* a) {
 }
 
 // Periodic mask of frame f: lattice rows that are non-zero (periodic frames).
+// The nine entries are read together and tested without short-circuits, so
+// that the loads are one round trip rather than a chain of them.
 __device__ inline int grid_mask(const double* c, int periodic) {
+  double v[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) v[k] = c[k];
   int per = 0;
 #pragma unroll
   for (int q = 0; q < 3; ++q)
-    if (periodic && (c[3 * q] != 0.0 || c[3 * q + 1] != 0.0 || c[3 * q + 2] != 0.0)) per |= 1 << q;
-  return per;
+    per |= (int)((v[3 * q] != 0.0) | (v[3 * q + 1] != 0.0) | (v[3 * q + 2] != 0.0)) << q;
+  return periodic ? per : 0;
 }
 
 __device__ inline void cross3(const double* u, const double* v, double* w) {
@@ -317,6 +322,8 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
   // species loaded here serve the binning and the packing
   const bool one = na <= FC_PER * nt;
   const int per = grid_mask(cell + 9 * f, periodic[f]);  // the same in every thread
+  const double* fpos = pos + 3 * b0;  // the frame's atoms (32-bit offsets below)
+  const int32_t* ftypes = types + b0;
   double xs[FC_PER][3];
   int ts[FC_PER], bws[FC_PER];
 #pragma unroll
@@ -324,8 +331,8 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
     const int k = t + r * nt;
     if (k < na) {
 #pragma unroll
-      for (int q = 0; q < 3; ++q) xs[r][q] = pos[3 * (b0 + k) + q];
-      ts[r] = types[b0 + k];
+      for (int q = 0; q < 3; ++q) xs[r][q] = fpos[3 * k + q];
+      ts[r] = ftypes[k];
     }
   }
   // bin counts start at zero (a frame has at most max(na, 1) bins)
@@ -388,7 +395,7 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
         const int k = k0 + r * nt;
         if (k < na)
 #pragma unroll
-          for (int q = 0; q < 3; ++q) xs[r][q] = pos[3 * (b0 + k) + q];
+          for (int q = 0; q < 3; ++q) xs[r][q] = fpos[3 * k + q];
       }
 #pragma unroll
     for (int r = 0; r < FC_PER; ++r) {
@@ -461,8 +468,8 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
         const int k = k0 + r * nt;
         if (k < na) {
 #pragma unroll
-          for (int q = 0; q < 3; ++q) xs[r][q] = pos[3 * (b0 + k) + q];
-          ts[r] = types[b0 + k];
+          for (int q = 0; q < 3; ++q) xs[r][q] = fpos[3 * k + q];
+          ts[r] = ftypes[k];
         }
       }
 #pragma unroll
