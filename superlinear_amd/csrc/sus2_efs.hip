@@ -256,9 +256,8 @@ __device__ __forceinline__ int bin_word(int id, const int* w) {
 
 // One atom of a bin-ordered cell list: its position moved into the home cell
 // (x - wrap . lattice rows), index and species, so that a neighbour candidate
-// is one 32-byte load.  The per-atom copy (atom_home, in atom order, written by
-// the neighbour search for its centre) holds the global bin id in place of
-// the index.
+// is one 32-byte load.  The per-atom copy (atom_home, in atom order, written
+// with the cell lists) holds the global bin id in place of the index.
 // Fractional coordinate q of x in the grid's basis, with every product and
 // sum rounded on its own (no contraction), so that two evaluations in one
 // kernel bin an atom identically.
@@ -307,9 +306,8 @@ __device__ __forceinline__ void lds_barrier() {
 __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
     int nframes, const int64_t* __restrict__ frame_off, const double* __restrict__ pos,
     const int32_t* __restrict__ types, const double* __restrict__ cell, const int32_t* __restrict__ periodic,
-    double rc, Grid* __restrict__ grid, int* __restrict__ cell_of, int* __restrict__ wrap,
-    int* __restrict__ cell_start, CellAtom* __restrict__ cell_pack, int* __restrict__ in_cur,
-    int* __restrict__ atom_frame) {
+    double rc, Grid* __restrict__ grid, int* __restrict__ cell_start, CellAtom* __restrict__ cell_pack,
+    CellAtom* __restrict__ atom_home, int* __restrict__ in_cur, int* __restrict__ atom_frame) {
   extern __shared__ int lds_cells[];
   int* bin = lds_cells;                      // [na] local bin of each atom
   int* cnt = bin + FRAME_CELLS_MAX;          // [nbins + 1] counts, then exclusive starts
@@ -407,12 +405,10 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
       for (int q = 0; q < 3; ++q) {
         const double w = (g.per & (1 << q)) ? floor(fr[q]) : 0.0;
         wi[q] = (int)w;
-        wrap[3 * (b0 + k) + q] = wi[q];
         c[q] = min(g.n[q] - 1, max(0, (int)((fr[q] - w) * g.n[q])));
       }
       const int id = c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
       bws[r] = bin[k] = bin_word(id, wi);
-      cell_of[b0 + k] = (int)b0 + id;
       in_cur[b0 + k] = 0;
       atom_frame[b0 + k] = f;
       atomicAdd(cnt + id, 1);
@@ -493,7 +489,9 @@ __global__ __launch_bounds__(FRAME_CELLS_THREADS) void k_frame_cells(
         for (int d = 0; d < 3; ++d) w[d] = ((bw >> (BIN_ID_BITS + 6 * d)) & 63) - 32;
       else
         for (int d = 0; d < 3; ++d) w[d] = (g.per & (1 << d)) ? (int)floor(frac_coord(g, xs[r], d)) : 0;
-      cell_pack[b0 + s0 + (r0 + r1) + (r2 + r3)] = cell_atom(xs[r], w, a_rows, (int)b0 + k, ts[r]);
+      const CellAtom ca = cell_atom(xs[r], w, a_rows, (int)b0 + k, ts[r]);
+      cell_pack[b0 + s0 + (r0 + r1) + (r2 + r3)] = ca;
+      atom_home[b0 + k] = CellAtom{ca.x, ca.y, ca.z, (int)b0 + id, 0};
     }
   }
 }
@@ -537,11 +535,14 @@ __global__ void k_fill_bins(int natoms, const int* __restrict__ cell_of, const i
 __global__ void k_pack_cells(int natoms, const int* __restrict__ cell_atoms, const int* __restrict__ atom_frame,
                              const double* __restrict__ pos, const int* __restrict__ wrap,
                              const int32_t* __restrict__ types, const double* __restrict__ cell,
-                             CellAtom* __restrict__ cell_pack) {
+                             const int* __restrict__ cell_of, CellAtom* __restrict__ cell_pack,
+                             CellAtom* __restrict__ atom_home) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= natoms) return;
   const int j = cell_atoms[s];
-  cell_pack[s] = cell_atom(pos + 3 * j, wrap + 3 * j, cell + 9 * atom_frame[j], j, types[j]);
+  const CellAtom ca = cell_atom(pos + 3 * j, wrap + 3 * j, cell + 9 * atom_frame[j], j, types[j]);
+  cell_pack[s] = ca;
+  atom_home[j] = CellAtom{ca.x, ca.y, ca.z, cell_of[j], 0};
 }
 
 // Sort the ids of one list [b, e) ascending (one workgroup of LIST_THREADS
@@ -620,39 +621,33 @@ __device__ __forceinline__ int floor_div_r(int a, int n, float rn) {
 template <class T>
 __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __restrict__ atom_frame,
                                                     const int64_t* __restrict__ frame_off,
-                                                    const double* __restrict__ pos, const double* __restrict__ cell,
-                                                    const Grid* __restrict__ grid, const int* __restrict__ cell_of,
-                                                    const int* __restrict__ wrap, const int* __restrict__ cell_start,
+                                                    const double* __restrict__ cell, const Grid* __restrict__ grid,
+                                                    const CellAtom* __restrict__ atom_home,
+                                                    const int* __restrict__ cell_start,
                                                     const CellAtom* __restrict__ cell_pack, double rc,
                                                     const int* __restrict__ off, int* __restrict__ cnt,
                                                     Rec<T>* __restrict__ rec, int* __restrict__ oslot,
-                                                    uint16_t* __restrict__ otab, CellAtom* __restrict__ atom_home,
-                                                    int cap, int over) {
+                                                    uint16_t* __restrict__ otab, int cap, int over) {
   __shared__ int o_b[2 * WAVE], o_cum[2 * WAVE], o_z[2 * WAVE], o_hits[2 * WAVE];
   __shared__ double o_sh[3][2 * WAVE];  // image shift of each bin offset (lattice rows); o_z: shift zero
   const int i = blockIdx.x, lane = threadIdx.x;
   if (i >= natoms) return;
   if (cap == 0 && cnt[i] <= over) return;
+  // the atom in its home cell and its bin (written by the cell-list build, the
+  // same record as the cell lists hold), then its frame's grid
+  const CellAtom home = atom_home[i];
   const int f = atom_frame[i];
   const Grid g = grid[f];
   const double* a = cell + 9 * f;
   const int cbase = (int)frame_off[f];
-  int ci = cell_of[i] - cbase;
+  int ci = home.j - cbase;
   const int c0 = ci % g.n[0];
   ci /= g.n[0];
   const int c1 = ci % g.n[1], c2 = ci / g.n[1];
-  const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
-  const int wi0 = wrap[3 * i], wi1 = wrap[3 * i + 1], wi2 = wrap[3 * i + 2];
   const double rc2 = rc * rc;
-  // the atom in its home cell, as the cell lists hold every atom (the same
-  // expression as the cell lists' records); a bin box whose centre is farther
-  // than rc + (bin half-diagonal) holds no neighbour (1e-6 A of slack)
-  const CellAtom home = cell_atom(pos + 3 * i, wrap + 3 * i, a, i, 0);
   const double hx = home.x, hy = home.y, hz = home.z;
-  if (atom_home && lane == 0) atom_home[i] = CellAtom{hx, hy, hz, cell_of[i], 0};
-  const double gx = xi - wi0 * g.A[0] - wi1 * g.A[3] - wi2 * g.A[6];
-  const double gy = yi - wi0 * g.A[1] - wi1 * g.A[4] - wi2 * g.A[7];
-  const double gz = zi - wi0 * g.A[2] - wi1 * g.A[5] - wi2 * g.A[8];
+  // a bin box whose centre is farther than rc + (bin half-diagonal) from the
+  // atom holds no neighbour (1e-6 A of slack)
   const double far2 = (rc + g.rbox + 1e-6) * (rc + g.rbox + 1e-6);
   const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
   int count = 0;
@@ -684,9 +679,9 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
         else ok = ok && z >= 0 && z < g.n[2];
         if (ok) {
           const double u0 = (x + 0.5) * rd0 + s0, u1 = (y + 0.5) * rd1 + s1, u2 = (z + 0.5) * rd2 + s2;
-          const double ex = g.org[0] + u0 * g.A[0] + u1 * g.A[3] + u2 * g.A[6] - gx;
-          const double ey = g.org[1] + u0 * g.A[1] + u1 * g.A[4] + u2 * g.A[7] - gy;
-          const double ez = g.org[2] + u0 * g.A[2] + u1 * g.A[5] + u2 * g.A[8] - gz;
+          const double ex = g.org[0] + u0 * g.A[0] + u1 * g.A[3] + u2 * g.A[6] - hx;
+          const double ey = g.org[1] + u0 * g.A[1] + u1 * g.A[4] + u2 * g.A[7] - hy;
+          const double ez = g.org[2] + u0 * g.A[2] + u1 * g.A[5] + u2 * g.A[8] - hz;
           ok = ex * ex + ey * ey + ez * ez <= far2;
         }
         if (ok) {
@@ -908,10 +903,14 @@ __global__ void k_incoming(long nedges, const Rec<T>* __restrict__ rec, const in
 // site energies, and sums energy and virial in a fixed order (per-thread
 // strided sums, a fixed wave-shuffle tree, then the waves in order).  With
 // one chunk per frame (nsub == 1) the block writes the frame's energy and
-// virial; otherwise its partial, and k_finalize_frames adds the chunks in a
-// fixed order.  Chunks of 512 atoms spread a large frame over many compute
-// units (one block per 4096-atom frame was a single CU's load bandwidth).
-constexpr int FIN_THREADS = 256, FIN_CHUNK = 512;
+// virial; otherwise its partial.  Up to FIN_FOLD chunks per frame (done given),
+// the last of the frame's chunks to finish (a per-frame count, left at zero
+// again by that chunk) adds the partials in a fixed order -- which chunk is
+// last does not change the sum; more chunks than that would queue on the one
+// counter, and k_finalize_frames adds their partials in the same order.
+// Chunks of 512 atoms spread a large frame over many compute units (one block
+// per 4096-atom frame was a single CU's load bandwidth).
+constexpr int FIN_THREADS = 256, FIN_CHUNK = 512, FIN_FOLD = 16;
 
 template <class T>
 __global__ __launch_bounds__(FIN_THREADS) void k_finalize(int nsub, const int64_t* __restrict__ frame_off,
@@ -919,7 +918,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(int nsub, const int64_
                                                           const T* __restrict__ site, const T* __restrict__ fw,
                                                           double* __restrict__ energy, double* __restrict__ virial,
                                                           double* __restrict__ force, double* __restrict__ site_out,
-                                                          double* __restrict__ part) {
+                                                          double* __restrict__ part, int* __restrict__ done) {
   constexpr int CMAX = 64;  // per-species site constants staged in LDS
   __shared__ double red[7][FIN_THREADS / WAVE];
   __shared__ double spc[CMAX];
@@ -974,28 +973,55 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(int nsub, const int64_
       part[((size_t)f * nsub + s) * 7 + t] = v;
     }
   }
-}
-
-// One block per frame: the frame's chunk partials summed in a fixed order
-// (per-thread strided sums, a fixed wave-shuffle tree, the waves in order).
-constexpr int FINF_THREADS = 256;
-__global__ __launch_bounds__(FINF_THREADS) void k_finalize_frames(int nsub, const double* __restrict__ part,
-                                                                  double* __restrict__ energy,
-                                                                  double* __restrict__ virial) {
-  __shared__ double red[7][FINF_THREADS / WAVE];
-  const int f = blockIdx.x, t = threadIdx.x;
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (int s = t; s < nsub; s += FINF_THREADS)
+  if (nsub == 1 || !done) return;
+  // the partial (written by wave 0) is visible device-wide before it is counted
+  __shared__ int last;
+  if (t == 0) {
+    __threadfence();
+    last = atomicAdd(done + f, 1) == nsub - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();  // the other chunks' partials, counted before this one
+  // the frame's partials: per-thread strided sums, a fixed wave-shuffle
+  // tree, the waves in order
+  double sum[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int c = t; c < nsub; c += FIN_THREADS)
 #pragma unroll
-    for (int q = 0; q < 7; ++q) acc[q] += part[((size_t)f * nsub + s) * 7 + q];
-  wave_sum_n(acc);
+    for (int q = 0; q < 7; ++q) sum[q] += part[((size_t)f * nsub + c) * 7 + q];
+  wave_sum_n(sum);
   if (t % WAVE == 0)
 #pragma unroll
-    for (int q = 0; q < 7; ++q) red[q][t / WAVE] = acc[q];
+    for (int q = 0; q < 7; ++q) red[q][t / WAVE] = sum[q];
   __syncthreads();
   if (t < 7) {
     double v = 0;
-    for (int k = 0; k < FINF_THREADS / WAVE; ++k) v += red[t][k];
+    for (int k = 0; k < FIN_THREADS / WAVE; ++k) v += red[t][k];
+    if (t == 0) energy[f] = v;
+    else virial[6 * f + t - 1] = v;
+  }
+  if (t == 0) done[f] = 0;  // zero again for the next call
+}
+
+// One block per frame: the frame's chunk partials summed in the order of the
+// folded form above.
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize_frames(int nsub, const double* __restrict__ part,
+                                                                 double* __restrict__ energy,
+                                                                 double* __restrict__ virial) {
+  __shared__ double red[7][FIN_THREADS / WAVE];
+  const int f = blockIdx.x, t = threadIdx.x;
+  double sum[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int c = t; c < nsub; c += FIN_THREADS)
+#pragma unroll
+    for (int q = 0; q < 7; ++q) sum[q] += part[((size_t)f * nsub + c) * 7 + q];
+  wave_sum_n(sum);
+  if (t % WAVE == 0)
+#pragma unroll
+    for (int q = 0; q < 7; ++q) red[q][t / WAVE] = sum[q];
+  __syncthreads();
+  if (t < 7) {
+    double v = 0;
+    for (int k = 0; k < FIN_THREADS / WAVE; ++k) v += red[t][k];
     if (t == 0) energy[f] = v;
     else virial[6 * f + t - 1] = v;
   }
@@ -1125,8 +1151,8 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
       }
       SUS2_LAUNCH(ws, st, "frame_cells", k_frame_cells, dim3(b.nframes), dim3(FRAME_CELLS_THREADS),
                   frame_cells_lds(), b.nframes, b.frame_off, b.pos, b.types, b.cell, b.periodic, rc,
-                  ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(),
-                  ws.cell_pack.get<CellAtom>(), ws.in_cur.get<int>(), ws.atom_frame.get<int>());
+                  ws.grid.get<Grid>(), ws.cell_start.get<int>(), ws.cell_pack.get<CellAtom>(),
+                  ws.atom_home.get<CellAtom>(), ws.in_cur.get<int>(), ws.atom_frame.get<int>());
     }
   } else {
     SUS2_LAUNCH(ws, st, "grid_setup", k_grid_setup, dim3((std::max(b.nframes, N) + 63) / 64), dim3(64), 0,
@@ -1141,14 +1167,14 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     SUS2_LAUNCH(ws, st, "sort_bins", k_sort_lists, dim3(N), dim3(LIST_THREADS), 0, N, ws.cell_start.get<int>(),
                 ws.cell_atoms.get<int>(), (int*)nullptr);
     SUS2_LAUNCH(ws, st, "pack_cells", k_pack_cells, dim3(nb), dim3(256), 0, N, ws.cell_atoms.get<int>(),
-                ws.atom_frame.get<int>(), b.pos, ws.wrap.get<int>(), b.types, b.cell, ws.cell_pack.get<CellAtom>());
+                ws.atom_frame.get<int>(), b.pos, ws.wrap.get<int>(), b.types, b.cell, ws.cell_of.get<int>(),
+                ws.cell_pack.get<CellAtom>(), ws.atom_home.get<CellAtom>());
   }
   if (N > 0) {
-    SUS2_LAUNCH(ws, st, "neighbors" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(), b.frame_off, b.pos,
-                b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(), ws.cell_start.get<int>(),
+    SUS2_LAUNCH(ws, st, "neighbors" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(),
+                b.frame_off, b.cell, ws.grid.get<Grid>(), ws.atom_home.get<CellAtom>(), ws.cell_start.get<int>(),
                 ws.cell_pack.get<CellAtom>(), rc, (const int*)nullptr, ws.nbr_cnt.get<int>(),
-                ws.nbr_slots.get<Rec<T>>(), ws.nbr_oslot.get<int>(), ws.nbr_otab.get<uint16_t>(),
-                ws.atom_home.get<CellAtom>(), cap, 0);
+                ws.nbr_slots.get<Rec<T>>(), ws.nbr_oslot.get<int>(), ws.nbr_otab.get<uint16_t>(), cap, 0);
   }
   // edge capacity left by earlier builds: the host-sized one, in records of T
   // (outside capture the edge buffers are widened to it first, as after a
@@ -1200,10 +1226,9 @@ void build_neighbors(Workspace& ws, const Batch& b, double rc, hipStream_t st) {
     SUS2_LAUNCH(ws, st, "compact" + tn, k_compact<T>, dim3(N), dim3(WAVE), 0, N, cap, ws.nbr_cnt.get<int>(),
                 ws.nbr_off.get<int>(), ws.nbr_slots.get<Rec<T>>(), ws.rec.get<Rec<T>>());
     SUS2_LAUNCH(ws, st, "neighbors_fill" + tn, k_neighbors<T>, dim3(N), dim3(WAVE), 0, N, ws.atom_frame.get<int>(),
-                b.frame_off, b.pos, b.cell, ws.grid.get<Grid>(), ws.cell_of.get<int>(), ws.wrap.get<int>(),
-                ws.cell_start.get<int>(), ws.cell_pack.get<CellAtom>(), rc, ws.nbr_off.get<int>(),
-                ws.nbr_cnt.get<int>(), ws.rec.get<Rec<T>>(), (int*)nullptr, (uint16_t*)nullptr, (CellAtom*)nullptr,
-                0, cap);
+                b.frame_off, b.cell, ws.grid.get<Grid>(), ws.atom_home.get<CellAtom>(), ws.cell_start.get<int>(),
+                ws.cell_pack.get<CellAtom>(), rc, ws.nbr_off.get<int>(), ws.nbr_cnt.get<int>(), ws.rec.get<Rec<T>>(),
+                (int*)nullptr, (uint16_t*)nullptr, 0, cap);
     SUS2_LAUNCH(ws, st, "incoming" + tn, k_incoming<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, total,
                 ws.rec.get<Rec<T>>(), ws.nbr_off.get<int>(), ws.in_cur.get<int>(), ws.in_edges.get<int>());
     SUS2_LAUNCH(ws, st, "sort_incoming", k_sort_lists, dim3(N), dim3(LIST_THREADS), 0, N, ws.nbr_off.get<int>(),
@@ -1245,12 +1270,20 @@ void finish_efs(Workspace& ws, const Batch& b, const EfsOut& out, const SiteCons
   if (b.nframes > 0) {
     const long mx = std::max<long>(1, std::min<long>(ws.max_frame_atoms, N));
     const int nsub = (int)((mx + FIN_CHUNK - 1) / FIN_CHUNK);
+    const bool fold = nsub > 1 && nsub <= FIN_FOLD;
     if (nsub > 1) ws.fin_part.ensure(sizeof(double) * 7 * (size_t)b.nframes * nsub);
+    if (fold) {
+      // per-frame chunk counts: zero when the buffer is new (each call leaves
+      // them at zero)
+      const void* p0 = ws.fin_done.p;
+      ws.fin_done.ensure(sizeof(int) * (size_t)b.nframes);
+      if (ws.fin_done.p != p0) fill_zero(ws, ws.fin_done.get<int>(), ws.fin_done.bytes / sizeof(int), st);
+    }
     SUS2_LAUNCH(ws, st, "finalize", k_finalize<T>, dim3(b.nframes, nsub), dim3(FIN_THREADS), 0, nsub, b.frame_off,
                 b.types, sc, ws.site.get<T>(), ws.fw.get<T>(), out.energy, out.virial, out.force, out.site,
-                ws.fin_part.get<double>());
-    if (nsub > 1)
-      SUS2_LAUNCH(ws, st, "finalize_frames", k_finalize_frames, dim3(b.nframes), dim3(FINF_THREADS), 0, nsub,
+                ws.fin_part.get<double>(), fold ? ws.fin_done.get<int>() : (int*)nullptr);
+    if (nsub > FIN_FOLD)
+      SUS2_LAUNCH(ws, st, "finalize_frames", k_finalize_frames, dim3(b.nframes), dim3(FIN_THREADS), 0, nsub,
                   ws.fin_part.get<double>(), out.energy, out.virial);
   }
   if (first < N)

@@ -127,6 +127,7 @@ struct Workspace {
   long edge_cap = 0;
   long max_frame_atoms = 0;  // upper bound on the atoms of one frame of the current batch
   DevBuf fin_part;           // per-chunk partial frame sums of the frame reduction
+  DevBuf fin_done;           // per-frame count of finished chunks (zero between calls)
   int64_t* host_total = nullptr;  // pinned [4]
   // Sizing of the neighbour build without a host round trip: a build may take
   // its edge capacity from earlier calls (nbr_known) when the caller verifies
