@@ -349,6 +349,23 @@ __device__ void load_species_tables(const DModel<typename Cf::T>& m, bool gate, 
   t.alpha = t.x0 + C * F;
   t.coef = CL ? align16(t.alpha + C * F) : nullptr;
   const int so = (gate && m.o_gscal >= 0) ? m.o_gscal : m.o_scal;
+  // radial coefficients: the gate pass's own block when the model has one,
+  // else the outer block (rows of BB basis weights, then C type factors).
+  // Their count is known at compile time, so every thread's loads are issued
+  // first and land while the two small tables below make their round trips
+  // (a load-then-store loop would pay one round trip per blockDim values).
+  constexpr int NCO = CL ? Cf::F * CR : 0, ITC = (NCO + WAVE - 1) / WAVE;  // blockDim >= WAVE
+  T cv[ITC > 0 ? ITC : 1];
+  if constexpr (CL) {
+    const bool own_rad = gate && m.o_grad >= 0;
+    const T* __restrict__ coef = m.theta + (own_rad ? m.o_grad : m.o_rad);
+    const int stride = own_rad ? BB : BB + C;
+#pragma unroll
+    for (int it = 0; it < ITC; ++it) {
+      const int k = threadIdx.x + it * (int)blockDim.x, mu = k / CR, q = k % CR;
+      cv[it] = (k < NCO && q < BB) ? coef[mu * stride + q] : T(0);
+    }
+  }
   for (int k = threadIdx.x; k < Cf::NBLK * CC; k += blockDim.x) {
     const int b = k / CC, pq = k % CC;
     t.slope[k] = m.theta[so + 2 * b * CC + pq];
@@ -359,15 +376,12 @@ __device__ void load_species_tables(const DModel<typename Cf::T>& m, bool gate, 
     t.x0[k] = gate ? m.gtype(s, mu) : m.tau(s, mu);
     t.alpha[k] = (!gate && m.gated) ? m.alpha(s, mu) : T(0);
   }
-  if constexpr (!CL) return;
-  // radial coefficients: the gate pass's own block when the model has one,
-  // else the outer block (rows of BB basis weights, then C type factors)
-  const bool own_rad = gate && m.o_grad >= 0;
-  const T* __restrict__ coef = m.theta + (own_rad ? m.o_grad : m.o_rad);
-  const int stride = own_rad ? BB : BB + C;
-  for (int k = threadIdx.x; k < F * CR; k += blockDim.x) {
-    const int mu = k / CR, q = k % CR;
-    t.coef[k] = q < BB ? coef[mu * stride + q] : T(0);
+  if constexpr (CL) {
+#pragma unroll
+    for (int it = 0; it < ITC; ++it) {
+      const int k = threadIdx.x + it * (int)blockDim.x;
+      if (k < NCO) t.coef[k] = cv[it];
+    }
   }
 }
 
