@@ -366,15 +366,27 @@ __device__ void load_species_tables(const DModel<typename Cf::T>& m, bool gate, 
       cv[it] = (k < NCO && q < BB) ? coef[mu * stride + q] : T(0);
     }
   }
-  for (int k = threadIdx.x; k < Cf::NBLK * CC; k += blockDim.x) {
-    const int b = k / CC, pq = k % CC;
-    t.slope[k] = m.theta[so + 2 * b * CC + pq];
-    t.shift[k] = m.theta[so + (2 * b + 1) * CC + pq];
-  }
-  for (int k = threadIdx.x; k < C * F; k += blockDim.x) {
-    const int s = k / F, mu = k % F;
-    t.x0[k] = gate ? m.gtype(s, mu) : m.tau(s, mu);
-    t.alpha[k] = (!gate && m.gated) ? m.alpha(s, mu) : T(0);
+  // the two small tables in one loop: each step's four loads before its stores
+  const int n1 = Cf::NBLK * CC, n2 = C * F;
+  for (int k = threadIdx.x; k < max(n1, n2); k += blockDim.x) {
+    const int b = k / CC, pq = k % CC, s = k / F, mu = k % F;
+    T sl = T(0), sh = T(0), x0 = T(0), al = T(0);
+    if (k < n1) {
+      sl = m.theta[so + 2 * b * CC + pq];
+      sh = m.theta[so + (2 * b + 1) * CC + pq];
+    }
+    if (k < n2) {
+      x0 = gate ? m.gtype(s, mu) : m.tau(s, mu);
+      if (!gate && m.gated) al = m.alpha(s, mu);
+    }
+    if (k < n1) {
+      t.slope[k] = sl;
+      t.shift[k] = sh;
+    }
+    if (k < n2) {
+      t.x0[k] = x0;
+      t.alpha[k] = al;
+    }
   }
   if constexpr (CL) {
 #pragma unroll

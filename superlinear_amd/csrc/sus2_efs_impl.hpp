@@ -550,6 +550,33 @@ constexpr int edge_backward_waves() { return (sizeof(typename Cf::T) == 4 && Cf:
 // are contiguous, so two or four atoms' edges fill the lanes better than one
 // atom's ~90).  sA: [NA][KH] adjoints, sV: [NA][9][LR] lane partials, sXR:
 // [F][LR] gate-message columns; tab already loaded.
+// The NA centres' moment adjoints [na][KH] into LDS: NA KH is known at
+// compile time, so each lane issues its loads U at a time before storing them
+// (a load-then-store loop would pay one memory round trip per 64 values,
+// seven at L5 with two centres); U is kept small so that the staging adds
+// few registers to the kernels' peak.
+template <class Cf, int NA, class T = typename Cf::T>
+__device__ __forceinline__ void stage_adjoints(T* sA, const T* __restrict__ adj, int i0, int na) {
+  constexpr int NAK = NA * Cf::KH, IT = (NAK + WAVE - 1) / WAVE;
+  constexpr int U = IT < 16 / (int)sizeof(T) ? IT : 16 / (int)sizeof(T);
+  const int n = na * Cf::KH;
+  const T* src = adj + (size_t)i0 * Cf::KH;
+#pragma unroll
+  for (int it0 = 0; it0 < IT; it0 += U) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c = (int)threadIdx.x + (it0 + u) * WAVE;
+      v[u] = c < n ? src[c] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c = (int)threadIdx.x + (it0 + u) * WAVE;
+      if (c < n) sA[c] = v[u];
+    }
+  }
+}
+
 template <class Cf, bool GATE, int NA, bool MSG = true, class T = typename Cf::T>
 __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const SpeciesTables<T>& tab, T* sA, T* sV,
                                                     T* sXR, int i0, int natoms, const int32_t* __restrict__ types,
@@ -565,7 +592,7 @@ __device__ __forceinline__ void edge_backward_group(const DModel<T>& m, const Sp
   const bool hgate = !GATE && m.gated;
   const bool hmsg = MSG && hgate;
   const int na = min(NA, natoms - i0);
-  for (int c = lane; c < na * KH; c += WAVE) sA[c] = adj[(size_t)i0 * KH + c];
+  stage_adjoints<Cf, NA>(sA, adj, i0, na);
 #pragma unroll
   for (int c = 0; c < NA * 9; ++c) sV[c * LR + lane] = T(0);
   // the centres' edge ranges: [eb[a], eb[a + 1]) for atom i0 + a
