@@ -876,19 +876,24 @@ __global__ __launch_bounds__(WAVE) void k_grad_gate_reverse(DModel<T> m, GradLds
       for (int col = lane; col < NC; col += WAVE) s.hbs[col] = column_sum(col, e0, 1);
     }
     __syncthreads();  // the group sums in the staging region are consumed
-    const T* vi = vg + (size_t)i * p.node_cnt;
-    const T* vdi = vgd + (size_t)i * p.node_cnt;
-    for (int n = lane; n < m.M; n += WAVE) {
+    // the atom's gate node values, tangents and moments: rows four at a time,
+    // so that their loads are in flight together (a load-then-store loop
+    // would wait for each row's loads before issuing the next)
+    const T* __restrict__ vi = vg + (size_t)i * p.node_cnt;
+    const T* __restrict__ vdi = vgd + (size_t)i * p.node_cnt;
+    wave_rows(0, m.M, [&](int n) {
       const bool in = n >= p.node_lo && n < p.node_lo + p.node_cnt;
       s.v[n] = in ? vi[n - p.node_lo] : T(0);
       s.vd[n] = in ? vdi[n - p.node_lo] : T(0);
       s.P[n] = T(0);
       s.Q[n] = T(0);
-    }
-    for (int c = lane; c < KH; c += WAVE) {
-      s.At[c] = gmom[(size_t)i * KH + c];
-      s.Atd[c] = gmomd[(size_t)i * KH + c];
-    }
+    });
+    const T* __restrict__ ai = gmom + (size_t)i * KH;
+    const T* __restrict__ adi = gmomd + (size_t)i * KH;
+    wave_rows(0, KH, [&](int c) {
+      s.At[c] = ai[c];
+      s.Atd[c] = adi[c];
+    });
     __syncthreads();
     for (int mu = lane; mu < 2 * F; mu += WAVE) hbar[(size_t)i * 2 * F + mu] = s.hbs[mu];
     for (int q = lane; q < m.Qg; q += WAVE) {
