@@ -714,7 +714,7 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
       o_z[q] = (sv2[u][0] | sv2[u][1] | sv2[u][2]) == 0;
       o_hits[q] = 0;
     }
-    __syncthreads();
+    lds_barrier();  // LDS tables only (the records in flight are not read here)
     const int nvalid = min(OPL, nof - ob), chunk_start = count;
     // candidate k of the flattened range: its bin offset (binary search), the
     // packed atom, and whether it is a neighbour (d = (x_j - x_i) + shift, both
@@ -762,7 +762,7 @@ __global__ __launch_bounds__(WAVE) void k_neighbors(int natoms, const int* __res
       if (hb && otab) atomicAdd(&o_hits[obb - ob], 1);
       count += __popcll(mb);
     }
-    __syncthreads();
+    lds_barrier();
     if (otab && nof <= NOF_TAB) {  // hits before each offset of the chunk
       const int h0 = o_hits[2 * lane], h1 = o_hits[2 * lane + 1], hp = h0 + h1;
       int inc = hp;
